@@ -1,0 +1,223 @@
+"""ctypes binding of the FP64 host oracle -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, bench.py's cpu_baseline leg and __graft_entry__.smoke() as
+the checker; the product never imports it.  The state lives in numpy arrays in
+the reference layout (cases.PARTICLE_FIELDS); the C structs point into them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from sph_fem_2d_cutting_gpu_amd import cases
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "libsphfem_oracle.so")
+
+_dbl = C.c_double
+_P = C.c_void_p
+
+
+def _struct(name, names, ctype=_dbl):
+    return type(name, (C.Structure,), {"_fields_": [(n, ctype) for n in names]})
+
+
+OPhys = _struct("OPhys", ["E", "nu", "rho0", "K", "G", "mass", "mass_tool"])
+OTrml = _struct("OTrml", ["cp", "tq", "k", "alpha", "T_init", "eta"])
+OCorr = _struct("OCorr", ["wdeltap", "stresseps", "xspheps", "alpha", "beta", "eta"])
+OJoco = _struct("OJoco", ["A", "B", "C", "n", "m", "Tmelt", "Tref", "eps_dot_ref",
+                          "tanh_a", "tanh_b", "tanh_c", "tanh_d"])
+
+
+class OGeom(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("bbmin_x", _dbl), ("bbmin_y", _dbl), ("dx", _dbl)]
+
+
+OToolc = _struct("OToolc", ["cp", "tq", "k", "eta", "rho", "T0", "h", "beta", "mu", "contact_alpha",
+                            "slave_mass", "hc"])
+OWearc = _struct("OWearc", ["A", "B", "C", "D"])
+
+
+class OConsts(C.Structure):
+    _fields_ = [("phys", OPhys), ("trml_wp", OTrml), ("trml_tool", OTrml), ("corr", OCorr),
+                ("joco", OJoco), ("geom", OGeom), ("tc", OToolc), ("wear", OWearc), ("dt", _dbl),
+                ("rr2_min", _dbl), ("wear_pro", C.c_int), ("thermal_bc", C.c_int), ("plastic", C.c_int)]
+
+
+_PF = [name for name, _, _ in cases.PARTICLE_FIELDS]
+
+
+class OParticles(C.Structure):
+    _fields_ = [("N", C.c_int)] + [(n, _P) for n in _PF]
+
+
+class OStepper(C.Structure):
+    _fields_ = [(n, _P) for n in ("pos_init", "vel_init", "S_init", "rho_init", "T_init",
+                                  "cell_start", "cell_end")] + [("num_cell", C.c_int)]
+
+
+_TOOL_ARRAYS = ("node_pos", "elem", "layer", "row_ptr", "col_ind", "K", "K_temp", "M", "Minv", "f",
+                "f_temp", "T", "T_old", "k", "left", "right", "n", "heat_exchange", "fric_heat",
+                "wear_rate", "wear_nodes", "physical_para", "sliding_force", "marks")
+_TOOL_SCALARS = ("vel_x", "vel_y", "tl_x", "tl_y", "br_x", "br_y", "ref_dist", "lowest_point",
+                 "right_point", "flank_retreat", "flank_contact")
+
+
+class OTool(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in ("n_nodes", "n_elem", "nnz", "n_seg", "n_colors", "n_cut_colors")]
+                + [(n, _P) for n in _TOOL_ARRAYS] + [(n, _dbl) for n in _TOOL_SCALARS])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(LIB_PATH)
+        pc = C.POINTER(OConsts)
+        pp = C.POINTER(OParticles)
+        ps = C.POINTER(OStepper)
+        pt = C.POINTER(OTool)
+        sig = {
+            "o_compute_hashes": (pc, pp), "o_sort_and_cells": (pc, pp, ps), "o_init": (pp, ps),
+            "o_predict": (pp, ps, _dbl), "o_correct": (pp, ps, _dbl), "o_material_eos": (pc, pp),
+            "o_artificial_stress": (pc, pp), "o_interactions_monaghan": (pc, pp, ps),
+            "o_stress_rate_jaumann": (pc, pp), "o_continuity": (pp,), "o_momentum": (pc, pp),
+            "o_advection": (pp,), "o_fric_heat_v21": (pc, pp, pt), "o_plasticity": (pc, pp),
+            "o_contact_s1": (pc, pp, pt), "o_contact_s2": (pc, pp, pt), "o_boundary_conditions": (pp,),
+            "o_boundary_conditions_thermal": (pc, pp), "o_invalidate": (pp,),
+            "o_fem_reset_server": (pt,), "o_fem_calculate_temp": (pt, _dbl), "o_fem_apply_robin": (pc, pt),
+            "o_fem_calculate_temp_rate": (pt,), "o_segments_wear_rate_reset": (pt,),
+            "o_segments_update": (pt, _dbl), "o_tool_update": (pt, _dbl), "o_wear_adjustment": (pc, pt),
+            "o_step": (pc, pp, ps, pt), "o_run": (pc, pp, ps, pt, C.c_int),
+            "o_report_force": (pp, C.POINTER(_dbl), C.POINTER(_dbl)),
+            "o_cubic_spline": (_dbl, _dbl, _dbl, _dbl, _dbl, _dbl, C.POINTER(_dbl)),
+            "o_sigma_yield": (C.POINTER(OJoco), _dbl, _dbl, _dbl),
+            "o_solve_secant": (C.POINTER(OJoco), _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, C.POINTER(C.c_int)),
+            "o_set_num_threads": (C.c_int,), "o_num_threads": (),
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = list(args)
+            f.restype = None
+        L.o_sigma_yield.restype = _dbl
+        L.o_solve_secant.restype = _dbl
+        L.o_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _fill(struct, d):
+    for k, v in d.items():
+        setattr(struct, k, v)
+    return struct
+
+
+def make_consts(case: cases.Case, rr2_min=None, wear_pro=None) -> OConsts:
+    k = case.const
+    g = cases.grid_geometry(case.grid_bbmin, case.grid_bbmax, case.h)
+    c = OConsts()
+    _fill(c.phys, k.phys)
+    _fill(c.trml_wp, k.trml_wp)
+    _fill(c.trml_tool, k.trml_tool)
+    _fill(c.corr, k.corr)
+    _fill(c.joco, k.joco)
+    _fill(c.tc, k.tool)
+    _fill(c.wear, k.wear)
+    c.geom.nx, c.geom.ny = g["nx"], g["ny"]
+    c.geom.bbmin_x, c.geom.bbmin_y, c.geom.dx = g["bbmin_x"], g["bbmin_y"], g["dx"]
+    c.dt = case.dt
+    c.rr2_min = cases.default_rr2_min(case.dx) if rr2_min is None else rr2_min
+    c.wear_pro = int(case.wear if wear_pro is None else wear_pro)
+    c.thermal_bc = int(k.trml_tool["alpha"] != 0.0)
+    c.plastic = 1
+    return c
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """The reference step on the host, over numpy arrays in the reference layout."""
+
+    def __init__(self, case: cases.Case, rr2_min=None, wear_pro=None, threads=None):
+        self.case = case
+        self.c = make_consts(case, rr2_min, wear_pro)
+        if threads:
+            lib().o_set_num_threads(int(threads))
+        self.P = cases.initial_particles(case)
+        N = case.N
+        nc = self.c.geom.nx * self.c.geom.ny
+        self.st = dict(pos_init=np.zeros((N, 2)), vel_init=np.zeros((N, 2)), S_init=np.zeros((N, 4)),
+                       rho_init=np.zeros(N), T_init=np.zeros(N),
+                       cell_start=np.full(nc, -1, dtype=np.int32), cell_end=np.zeros(nc, dtype=np.int32))
+        self.t = cases.initial_tool(case)
+        self._bind()
+
+    def _bind(self):
+        self.Pc = OParticles()
+        self.Pc.N = self.case.N
+        for name in _PF:
+            setattr(self.Pc, name, _ptr(self.P[name]))
+        self.Sc = OStepper()
+        for name, a in self.st.items():
+            setattr(self.Sc, name, _ptr(a))
+        self.Sc.num_cell = len(self.st["cell_start"])
+        if self.t is None:
+            self.Tc = None
+            return
+        t = self.t
+        self.Tc = OTool()
+        self.Tc.n_nodes, self.Tc.n_elem = len(t["node_pos"]), len(t["elem"])
+        self.Tc.nnz, self.Tc.n_seg = len(t["col_ind"]), len(t["left"])
+        self.Tc.n_colors, self.Tc.n_cut_colors = t["n_colors"], t["n_cut_colors"]
+        for name in _TOOL_ARRAYS:
+            setattr(self.Tc, name, _ptr(t[name]))
+        _fill(self.Tc, t["scalars"])
+
+    @property
+    def tool_scalars(self):
+        return {k: getattr(self.Tc, k) for k in _TOOL_SCALARS}
+
+    def _args(self, sig):
+        m = {"c": C.byref(self.c), "p": C.byref(self.Pc), "s": C.byref(self.Sc),
+             "t": C.byref(self.Tc) if self.Tc is not None else None}
+        return [m[ch] for ch in sig]
+
+    def call(self, fname, sig, *extra):
+        getattr(lib(), fname)(*self._args(sig), *extra)
+
+    def step(self, n=1):
+        tc = C.byref(self.Tc) if self.Tc is not None else None
+        lib().o_run(C.byref(self.c), C.byref(self.Pc), C.byref(self.Sc), tc, int(n))
+
+    def report_force(self):
+        fx, fy = _dbl(), _dbl()
+        lib().o_report_force(C.byref(self.Pc), C.byref(fx), C.byref(fy))
+        return fx.value, fy.value
+
+
+def cubic_spline(pi, pj, h, rr2_min=1e-8):
+    out = (_dbl * 3)()
+    lib().o_cubic_spline(pi[0], pi[1], pj[0], pj[1], h, rr2_min, out)
+    return np.array(out[:])
+
+
+def joco_struct(joco: dict) -> OJoco:
+    return _fill(OJoco(), joco)
+
+
+def sigma_yield(joco: dict, eps, eps_dot, T):
+    return lib().o_sigma_yield(C.byref(joco_struct(joco)), eps, eps_dot, T)
+
+
+def solve_secant(joco: dict, init, tol, norm_S, eps, T, dt, mu):
+    it = C.c_int()
+    v = lib().o_solve_secant(C.byref(joco_struct(joco)), init, tol, norm_S, eps, T, dt, mu, C.byref(it))
+    return v, it.value
