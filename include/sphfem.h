@@ -1,0 +1,211 @@
+/*
+ * sphfem.h -- C ABI of the MI355X (gfx950) SPH-FEM cutting hot path.
+ *
+ * Drop-in boundary for the per-timestep path of nyzhang1227/SPH_FEM_2D_cutting_gpu:
+ * every entry point replaces one reference function (cited per declaration) and
+ * operates on device arrays laid out as the reference's structs.  All calls are
+ * asynchronous on the context's HIP stream; only sphfem_synchronize and the
+ * download helpers block.  Errors are returned as status codes (0 = OK) and
+ * described by sphfem_last_error(); nothing throws across this boundary.
+ *
+ * Units are the reference's (cm, g, us, K; stress in units of 100 GPa).
+ */
+#ifndef SPHFEM_H
+#define SPHFEM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPHFEM_OK 0
+#define SPHFEM_ERR_HIP 1      /* a HIP runtime call failed */
+#define SPHFEM_ERR_ARG 2      /* invalid argument */
+#define SPHFEM_ERR_STATE 3    /* call out of order (e.g. constants missing) */
+#define SPHFEM_ERR_RANGE 4    /* a particle left the cell grid */
+
+#define SPHFEM_FP64 0         /* plain FP64 form (reference arithmetic) */
+#define SPHFEM_FP32C 1        /* FP32 pair sums with compensated accumulation */
+
+typedef struct { double x, y; } sphfem_double2;
+typedef struct { double x, y, z, w; } sphfem_double4;
+typedef struct { int x, y, z, w; } sphfem_int4;
+
+/* constants_structs.h:18-123 (the friction-model fields under SHEAR_LIMIT_FRI etc. are off) */
+typedef struct { double E, nu, rho0, K, G, mass, mass_tool; } sphfem_phys_constants;
+typedef struct { double cp, tq, k, alpha, T_init, eta; } sphfem_trml_constants;
+typedef struct { double wdeltap, stresseps, xspheps, alpha, beta, eta; } sphfem_corr_constants;
+typedef struct { double A, B, C, n, m, Tmelt, Tref, eps_dot_ref, tanh_a, tanh_b, tanh_c, tanh_d; } sphfem_joco_constants;
+typedef struct { int nx, ny; double bbmin_x, bbmin_y, dx; } sphfem_geom_constants;
+typedef struct { double cp, tq, k, eta, rho, T0, h, beta, mu, contact_alpha, slave_mass, hc; } sphfem_tool_constants;
+typedef struct { double A, B, C, D; } sphfem_wear_constants;
+
+/* particle_gpu.h:23-83: device pointers in the reference's field order.  Like the
+ * reference, sorting and stepping swap these pointers with internal buffers, so
+ * callers re-read them after every call that takes the struct by pointer. */
+typedef struct sphfem_particles {
+    sphfem_double2 *pos, *vel;
+    double *h, *rho, *p;
+    sphfem_double4 *S, *R;
+    sphfem_double2 *fc, *ft, *n;
+    double *fixed, *blanked, *tool_particle;
+    double *eps_pl, *eps_pl_dot;
+    double *T;
+    sphfem_double2 *pos_t, *vel_t;
+    double *rho_t;
+    sphfem_double4 *S_t;
+    double *T_t;
+    sphfem_double4 *S_der, *v_der;
+    int *idx, *hash, *on_seg;
+    unsigned int N;
+} sphfem_particles;
+
+/* Host-side description of a rigid FEM tool, the data set_up_segments_from_mesh and
+ * set_up_FEM_tool_gpu leave on the device (tool_FEM.cu:771-1039, 1539-1544;
+ * FEM_thermal_gpu.cu:584-659; segment_FEM_gpu.h:31-56; mesh_structure_GPU.h:80-142).
+ * All pointers are host pointers; sphfem_tool_create copies them. */
+typedef struct sphfem_tool_desc {
+    int node_num, ele_num, nz_num, seg_num, color_total, color_cutting_edge;
+    const sphfem_double2 *position;      /* [node_num] */
+    const sphfem_int4 *node_num_el;      /* [ele_num] 3 node ids + colour */
+    const int *layer;                    /* [ele_num] 1 if in cutting boundary layer 1 or 2 */
+    const int *csr_row_ptr;              /* [node_num + 1] */
+    const int *csr_col_ind;              /* [nz_num] */
+    const double *K, *M, *f, *T;         /* stiffness [nz_num], lumped mass, rhs, temperature [node_num] */
+    const sphfem_double2 *left, *right, *n;   /* [seg_num] */
+    const sphfem_int4 *marks;            /* [seg_num] element, colour, left node, right node */
+    sphfem_double2 vel;                  /* tool velocity (tool.h:48-58) */
+    sphfem_double2 tl_point_ref, br_point;
+    double ref_dist, lowest_point, right_point;
+} sphfem_tool_desc;
+
+/* tool kinematic state kept on the device (tool_FEM.h:34-60 host fields + tool_FEM.cu:19-20) */
+typedef struct {
+    double vel_x, vel_y, tl_x, tl_y, br_x, br_y, ref_dist, lowest_point, right_point;
+    double flank_retreat, flank_contact;
+} sphfem_tool_kinematics;
+
+typedef struct sphfem_ctx sphfem_ctx;
+typedef struct sphfem_grid sphfem_grid;
+typedef struct sphfem_tool sphfem_tool;
+typedef struct sphfem_leap_frog sphfem_leap_frog;
+
+/* ---- context (replaces the per-translation-unit __constant__ state and the legacy stream) */
+int sphfem_create(int device, sphfem_ctx **out);
+int sphfem_destroy(sphfem_ctx *ctx);
+const char *sphfem_last_error(sphfem_ctx *ctx);
+int sphfem_synchronize(sphfem_ctx *ctx);
+/* check_cuda_error (types.cpp:11-37): synchronise and report device-side errors */
+int sphfem_check_error(sphfem_ctx *ctx);
+
+/* ---- constants (actions_gpu.cu:913-954, interactions_gpu.cu:476-507, FEM_thermal_gpu.cu:597) */
+int sphfem_setup_physical_constants(sphfem_ctx *ctx, const sphfem_phys_constants *c);
+int sphfem_setup_corrector_constants(sphfem_ctx *ctx, const sphfem_corr_constants *c);
+int sphfem_setup_johnson_cook_constants(sphfem_ctx *ctx, const sphfem_joco_constants *c);
+int sphfem_setup_thermal_constants_wp(sphfem_ctx *ctx, const sphfem_trml_constants *c);
+int sphfem_setup_thermal_constants_tool(sphfem_ctx *ctx, const sphfem_trml_constants *c);
+int sphfem_setup_tool_constants(sphfem_ctx *ctx, const sphfem_tool_constants *c);
+int sphfem_setup_wear_constants(sphfem_ctx *ctx, const sphfem_wear_constants *c);
+/* global_dt (benchmarks.cpp:12) and wear_pro (mfree_gpu.cpp:47) */
+int sphfem_set_dt(sphfem_ctx *ctx, double dt);
+int sphfem_set_wear(sphfem_ctx *ctx, int wear_pro);
+/* lower pair cut-off of cubic_spline (kernels.cuh:38, 1e-8 cm^2 in the reference) */
+int sphfem_set_kernel_cutoff(sphfem_ctx *ctx, double rr2_min);
+
+/* ---- device memory helpers */
+int sphfem_malloc(sphfem_ctx *ctx, void **ptr, size_t bytes);
+int sphfem_free(sphfem_ctx *ctx, void *ptr);
+int sphfem_memcpy_htod(sphfem_ctx *ctx, void *dst, const void *src, size_t bytes);
+int sphfem_memcpy_dtoh(sphfem_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* ---- particles (particle_gpu.cu:11-104: zeroed arrays, idx = 0..N-1, on_seg = 0) */
+int sphfem_particles_alloc(sphfem_ctx *ctx, unsigned int N, sphfem_particles *out);
+int sphfem_particles_free(sphfem_ctx *ctx, sphfem_particles *p);
+
+/* ---- cell list, grid_gpu_green (grid.h:23-64, grid_gpu_green.h:84-85) */
+int sphfem_grid_green_create(sphfem_ctx *ctx, int num_part, sphfem_double2 bbmin, sphfem_double2 bbmax,
+                             double h, sphfem_grid **out);
+int sphfem_grid_destroy(sphfem_ctx *ctx, sphfem_grid *g);
+int sphfem_grid_geometry(const sphfem_grid *g, sphfem_geom_constants *geom, int *num_cell);
+/* grid.cu:80-86 */
+int sphfem_grid_assign_hashes(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles *p);
+/* grid_gpu_green.cu:184-213: stable radix sort, cell boundaries, gather of 20 arrays, pointer swap */
+int sphfem_grid_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles *p);
+/* grid_gpu_green.cu:215-218: device-to-device copy into caller arrays [num_cell] */
+int sphfem_grid_get_cells(sphfem_ctx *ctx, sphfem_grid *g, int *cell_start, int *cell_end);
+
+/* ---- pair interactions (interactions_gpu.cu:381-432, 466-474) */
+int sphfem_interactions_setup_geometry_constants(sphfem_ctx *ctx, const sphfem_grid *g);
+int sphfem_interactions_monaghan(sphfem_ctx *ctx, sphfem_particles *p, const int *cell_start,
+                                 const int *cell_end, int num_cell);
+
+/* ---- per-particle actions (actions_gpu.h:33-72) */
+int sphfem_material_eos(sphfem_ctx *ctx, sphfem_particles *p);
+int sphfem_corrector_artificial_stress(sphfem_ctx *ctx, sphfem_particles *p);
+int sphfem_material_stress_rate_jaumann(sphfem_ctx *ctx, sphfem_particles *p);
+int sphfem_contmech_continuity(sphfem_ctx *ctx, sphfem_particles *p);
+int sphfem_contmech_momentum(sphfem_ctx *ctx, sphfem_particles *p);
+int sphfem_contmech_advection(sphfem_ctx *ctx, sphfem_particles *p);
+int sphfem_plasticity_johnson_cook(sphfem_ctx *ctx, sphfem_particles *p);
+int sphfem_perform_boundary_conditions(sphfem_ctx *ctx, sphfem_particles *p);
+int sphfem_perform_boundary_conditions_thermal(sphfem_ctx *ctx, sphfem_particles *p);
+int sphfem_debug_invalidate(sphfem_ctx *ctx, sphfem_particles *p);
+int sphfem_material_fric_heat_gen_FEM_v2(sphfem_ctx *ctx, sphfem_particles *p, sphfem_tool *t);
+int sphfem_compute_contact_forces_FEM_tool_v3(sphfem_ctx *ctx, sphfem_particles *p, sphfem_tool *t);
+int sphfem_wear_adjustment(sphfem_ctx *ctx, sphfem_tool *t);
+
+/* ---- FEM tool (tool_FEM.h:57-83, FEM_thermal_gpu.h:67-91) */
+int sphfem_tool_create(sphfem_ctx *ctx, const sphfem_tool_desc *desc, sphfem_tool **out);
+int sphfem_tool_destroy(sphfem_ctx *ctx, sphfem_tool *t);
+int sphfem_tool_set_vel(sphfem_ctx *ctx, sphfem_tool *t, sphfem_double2 vel);
+int sphfem_tool_get_kinematics(sphfem_ctx *ctx, sphfem_tool *t, sphfem_tool_kinematics *out);
+int sphfem_tool_segments_update(sphfem_ctx *ctx, sphfem_tool *t, double dt);
+int sphfem_tool_update_FEM_gpu(sphfem_ctx *ctx, sphfem_tool *t, double dt);
+int sphfem_tool_segments_wear_rate_reset(sphfem_ctx *ctx, sphfem_tool *t);
+int sphfem_tool_apply_heat_boundary_condition(sphfem_ctx *ctx, sphfem_tool *t);
+int sphfem_FEM_reset_server(sphfem_ctx *ctx, sphfem_tool *t);
+int sphfem_FEM_calculate_temp(sphfem_ctx *ctx, sphfem_tool *t, double dt);
+int sphfem_FEM_calculate_temp_rate(sphfem_ctx *ctx, sphfem_tool *t);
+/* copy the per-step tool state to host arrays (NULL pointers are skipped) */
+int sphfem_tool_download(sphfem_ctx *ctx, sphfem_tool *t, sphfem_double2 *position, double *T, double *K,
+                         double *M, double *k, sphfem_double2 *left, sphfem_double2 *right, sphfem_double2 *n,
+                         sphfem_double2 *heat_exchange, double *fric_heat, sphfem_double4 *physical_para,
+                         double *sliding_force, double *wear_nodes);
+
+/* restore a per-step tool state from host arrays (same layout as sphfem_tool_download; NULL skips) */
+int sphfem_tool_upload(sphfem_ctx *ctx, sphfem_tool *t, const sphfem_double2 *position, const double *T,
+                       const double *K, const double *M, const double *k, const sphfem_double2 *left,
+                       const sphfem_double2 *right, const sphfem_double2 *n, const sphfem_double2 *heat_exchange,
+                       const double *fric_heat, const sphfem_double4 *physical_para, const double *sliding_force,
+                       const double *wear_nodes);
+int sphfem_tool_set_kinematics(sphfem_ctx *ctx, sphfem_tool *t, const sphfem_tool_kinematics *kin);
+
+/* ---- time integration (leap_frog.h:46-49, leap_frog.cu:25-96, 162-248) */
+int sphfem_leap_frog_create(sphfem_ctx *ctx, unsigned int num_part, unsigned int num_cell, int node_num,
+                            sphfem_leap_frog **out);
+int sphfem_leap_frog_destroy(sphfem_ctx *ctx, sphfem_leap_frog *lf);
+int sphfem_leap_frog_set_precision(sphfem_ctx *ctx, sphfem_leap_frog *lf, int precision);
+int sphfem_leap_frog_init(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p);
+int sphfem_leap_frog_predict(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, double dt);
+int sphfem_leap_frog_correct(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, double dt);
+/* one full step (fused kernels); tool may be NULL for a tool-free workload */
+int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,
+                          sphfem_tool *t);
+/* the cell arrays the last step used (leap_frog.h:35-36), device pointers [num_cell] */
+int sphfem_leap_frog_cells(sphfem_leap_frog *lf, const int **cell_start, const int **cell_end);
+
+/* ---- observables (output.cu:22-30) */
+int sphfem_report_force(sphfem_ctx *ctx, sphfem_particles *p, sphfem_double2 *out);
+
+/* ---- measurement: per-kernel HIP-event timing on the context stream */
+int sphfem_timing_enable(sphfem_ctx *ctx, int enable);
+int sphfem_timing_reset(sphfem_ctx *ctx);
+/* accumulated milliseconds and launch count of one kernel (names listed by sphfem_timing_names) */
+int sphfem_timing_get(sphfem_ctx *ctx, const char *kernel, double *total_ms, long *launches);
+const char *sphfem_timing_names(sphfem_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -181,6 +181,23 @@ class Oracle:
             setattr(self.Tc, name, _ptr(t[name]))
         _fill(self.Tc, t["scalars"])
 
+    def clone(self) -> "Oracle":
+        """Independent copy of the current state (arrays, tool kinematics and constants)."""
+        import copy
+        o = Oracle.__new__(Oracle)
+        o.case = self.case
+        o.c = OConsts()
+        C.pointer(o.c)[0] = self.c
+        o.P = {k: v.copy() for k, v in self.P.items()}
+        o.st = {k: v.copy() for k, v in self.st.items()}
+        if self.t is None:
+            o.t = None
+        else:
+            o.t = {k: (v.copy() if isinstance(v, np.ndarray) else copy.deepcopy(v)) for k, v in self.t.items()}
+            o.t["scalars"] = self.tool_scalars
+        o._bind()
+        return o
+
     @property
     def tool_scalars(self):
         return {k: getattr(self.Tc, k) for k in _TOOL_SCALARS}

@@ -1,0 +1,169 @@
+"""ctypes binding of libsphfem_hip.so (the C ABI declared in include/sphfem.h).
+
+The library is loaded from the package directory only; a missing library raises
+immediately -- there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsphfem_hip.so")
+
+OK = 0
+FP64 = 0
+FP32C = 1
+
+_dbl = C.c_double
+_vp = C.c_void_p
+
+
+class Double2(C.Structure):
+    _fields_ = [("x", _dbl), ("y", _dbl)]
+
+
+def _s(name, fields, t=_dbl):
+    return type(name, (C.Structure,), {"_fields_": [(f, t) for f in fields]})
+
+
+PhysConstants = _s("PhysConstants", ["E", "nu", "rho0", "K", "G", "mass", "mass_tool"])
+TrmlConstants = _s("TrmlConstants", ["cp", "tq", "k", "alpha", "T_init", "eta"])
+CorrConstants = _s("CorrConstants", ["wdeltap", "stresseps", "xspheps", "alpha", "beta", "eta"])
+JocoConstants = _s("JocoConstants", ["A", "B", "C", "n", "m", "Tmelt", "Tref", "eps_dot_ref",
+                                     "tanh_a", "tanh_b", "tanh_c", "tanh_d"])
+ToolConstants = _s("ToolConstants", ["cp", "tq", "k", "eta", "rho", "T0", "h", "beta", "mu",
+                                     "contact_alpha", "slave_mass", "hc"])
+WearConstants = _s("WearConstants", ["A", "B", "C", "D"])
+ToolKinematics = _s("ToolKinematics", ["vel_x", "vel_y", "tl_x", "tl_y", "br_x", "br_y", "ref_dist",
+                                       "lowest_point", "right_point", "flank_retreat", "flank_contact"])
+
+
+class GeomConstants(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("bbmin_x", _dbl), ("bbmin_y", _dbl), ("dx", _dbl)]
+
+
+# particle_gpu.h:23-83 field order
+PARTICLE_FIELDS = ("pos", "vel", "h", "rho", "p", "S", "R", "fc", "ft", "n", "fixed", "blanked", "tool_particle",
+                   "eps_pl", "eps_pl_dot", "T", "pos_t", "vel_t", "rho_t", "S_t", "T_t", "S_der", "v_der", "idx",
+                   "hash", "on_seg")
+
+
+class Particles(C.Structure):
+    _fields_ = [(f, _vp) for f in PARTICLE_FIELDS] + [("N", C.c_uint)]
+
+
+class ToolDesc(C.Structure):
+    _fields_ = ([(f, C.c_int) for f in ("node_num", "ele_num", "nz_num", "seg_num", "color_total",
+                                        "color_cutting_edge")]
+                + [(f, _vp) for f in ("position", "node_num_el", "layer", "csr_row_ptr", "csr_col_ind", "K", "M",
+                                      "f", "T", "left", "right", "n", "marks")]
+                + [("vel", Double2), ("tl_point_ref", Double2), ("br_point", Double2),
+                   ("ref_dist", _dbl), ("lowest_point", _dbl), ("right_point", _dbl)])
+
+
+_P = C.POINTER
+_ctx = _vp
+_SIGS = {
+    "sphfem_create": (C.c_int, _P(_vp)),
+    "sphfem_destroy": (_ctx,),
+    "sphfem_synchronize": (_ctx,),
+    "sphfem_check_error": (_ctx,),
+    "sphfem_setup_physical_constants": (_ctx, _P(PhysConstants)),
+    "sphfem_setup_corrector_constants": (_ctx, _P(CorrConstants)),
+    "sphfem_setup_johnson_cook_constants": (_ctx, _P(JocoConstants)),
+    "sphfem_setup_thermal_constants_wp": (_ctx, _P(TrmlConstants)),
+    "sphfem_setup_thermal_constants_tool": (_ctx, _P(TrmlConstants)),
+    "sphfem_setup_tool_constants": (_ctx, _P(ToolConstants)),
+    "sphfem_setup_wear_constants": (_ctx, _P(WearConstants)),
+    "sphfem_set_dt": (_ctx, _dbl),
+    "sphfem_set_wear": (_ctx, C.c_int),
+    "sphfem_set_kernel_cutoff": (_ctx, _dbl),
+    "sphfem_malloc": (_ctx, _P(_vp), C.c_size_t),
+    "sphfem_free": (_ctx, _vp),
+    "sphfem_memcpy_htod": (_ctx, _vp, _vp, C.c_size_t),
+    "sphfem_memcpy_dtoh": (_ctx, _vp, _vp, C.c_size_t),
+    "sphfem_particles_alloc": (_ctx, C.c_uint, _P(Particles)),
+    "sphfem_particles_free": (_ctx, _P(Particles)),
+    "sphfem_grid_green_create": (_ctx, C.c_int, Double2, Double2, _dbl, _P(_vp)),
+    "sphfem_grid_destroy": (_ctx, _vp),
+    "sphfem_grid_geometry": (_vp, _P(GeomConstants), _P(C.c_int)),
+    "sphfem_grid_assign_hashes": (_ctx, _vp, _P(Particles)),
+    "sphfem_grid_sort": (_ctx, _vp, _P(Particles)),
+    "sphfem_grid_get_cells": (_ctx, _vp, _vp, _vp),
+    "sphfem_interactions_setup_geometry_constants": (_ctx, _vp),
+    "sphfem_interactions_monaghan": (_ctx, _P(Particles), _vp, _vp, C.c_int),
+    "sphfem_tool_create": (_ctx, _P(ToolDesc), _P(_vp)),
+    "sphfem_tool_destroy": (_ctx, _vp),
+    "sphfem_tool_set_vel": (_ctx, _vp, Double2),
+    "sphfem_tool_get_kinematics": (_ctx, _vp, _P(ToolKinematics)),
+    "sphfem_tool_segments_update": (_ctx, _vp, _dbl),
+    "sphfem_tool_update_FEM_gpu": (_ctx, _vp, _dbl),
+    "sphfem_tool_segments_wear_rate_reset": (_ctx, _vp),
+    "sphfem_tool_apply_heat_boundary_condition": (_ctx, _vp),
+    "sphfem_FEM_reset_server": (_ctx, _vp),
+    "sphfem_FEM_calculate_temp": (_ctx, _vp, _dbl),
+    "sphfem_FEM_calculate_temp_rate": (_ctx, _vp),
+    "sphfem_tool_download": (_ctx, _vp) + (_vp,) * 13,
+    "sphfem_tool_upload": (_ctx, _vp) + (_vp,) * 13,
+    "sphfem_tool_set_kinematics": (_ctx, _vp, _P(ToolKinematics)),
+    "sphfem_leap_frog_create": (_ctx, C.c_uint, C.c_uint, C.c_int, _P(_vp)),
+    "sphfem_leap_frog_destroy": (_ctx, _vp),
+    "sphfem_leap_frog_set_precision": (_ctx, _vp, C.c_int),
+    "sphfem_leap_frog_init": (_ctx, _vp, _P(Particles)),
+    "sphfem_leap_frog_predict": (_ctx, _vp, _P(Particles), _dbl),
+    "sphfem_leap_frog_correct": (_ctx, _vp, _P(Particles), _dbl),
+    "sphfem_leap_frog_step": (_ctx, _vp, _P(Particles), _vp, _vp),
+    "sphfem_report_force": (_ctx, _P(Particles), _P(Double2)),
+    "sphfem_timing_enable": (_ctx, C.c_int),
+    "sphfem_timing_reset": (_ctx,),
+    "sphfem_timing_get": (_ctx, C.c_char_p, _P(_dbl), _P(C.c_long)),
+}
+for _name in ("material_eos", "corrector_artificial_stress", "material_stress_rate_jaumann", "contmech_continuity",
+              "contmech_momentum", "contmech_advection", "plasticity_johnson_cook", "perform_boundary_conditions",
+              "perform_boundary_conditions_thermal", "debug_invalidate"):
+    _SIGS["sphfem_" + _name] = (_ctx, _P(Particles))
+for _name in ("material_fric_heat_gen_FEM_v2", "compute_contact_forces_FEM_tool_v3"):
+    _SIGS["sphfem_" + _name] = (_ctx, _P(Particles), _vp)
+_SIGS["sphfem_wear_adjustment"] = (_ctx, _vp)
+
+# symbols that do not return int
+_RESTYPES = {"sphfem_last_error": C.c_char_p, "sphfem_timing_names": C.c_char_p}
+
+_lib = None
+
+
+class SphfemError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libsphfem_hip.so from the package directory (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SphfemError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = list(args)
+            f.restype = C.c_int
+        for name, rt in _RESTYPES.items():
+            f = getattr(L, name)
+            f.argtypes = [_ctx]
+            f.restype = rt
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return sorted(set(_SIGS) | set(_RESTYPES))
+
+
+def check(rc, ctx=None, what=""):
+    if rc != OK:
+        msg = ""
+        if ctx is not None:
+            m = lib().sphfem_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise SphfemError(f"{what} failed with status {rc}: {msg}")

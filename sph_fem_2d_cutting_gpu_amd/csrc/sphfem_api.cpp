@@ -1,0 +1,936 @@
+// C ABI of libsphfem_hip: context, constants, memory, and the per-step orchestration.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <utility>
+
+#include "sphfem_internal.h"
+
+using namespace sphfem;
+
+namespace {
+
+enum : unsigned { SET_PHYS = 1, SET_CORR = 2, SET_JOCO = 4, SET_TRML_WP = 8, SET_TOOLC = 16, SET_DT = 32 };
+
+template <class F>
+int guarded(sphfem_ctx *ctx, F &&f) {
+    if (!ctx) return SPHFEM_ERR_ARG;
+    try {
+        SPHFEM_HIP_CHECK(hipSetDevice(ctx->device));
+        int rc = f();
+        if (rc == SPHFEM_OK) {
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) {
+                ctx->err = std::string("kernel launch: ") + hipGetErrorString(e);
+                return SPHFEM_ERR_HIP;
+            }
+        }
+        return rc;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return SPHFEM_ERR_HIP;
+    }
+}
+
+int fail(sphfem_ctx *ctx, int code, const std::string &msg) {
+    ctx->err = msg;
+    return code;
+}
+
+template <class T>
+T *dalloc(size_t n) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    SPHFEM_HIP_CHECK(hipMalloc(&p, n * sizeof(T)));
+    return static_cast<T *>(p);
+}
+
+template <class T>
+void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+template <class T>
+T *dupload(hipStream_t s, const T *host, size_t n) {
+    T *d = dalloc<T>(n);
+    if (host && n) SPHFEM_HIP_CHECK(hipMemcpyAsync(d, host, n * sizeof(T), hipMemcpyHostToDevice, s));
+    return d;
+}
+
+// allocate the 26 arrays of particle_gpu(N) (particle_gpu.cu:11-104)
+void alloc_particles(hipStream_t s, unsigned N, sphfem_particles &p) {
+    std::memset(&p, 0, sizeof(p));
+    p.N = N;
+    p.pos = dalloc<sphfem_double2>(N);
+    p.vel = dalloc<sphfem_double2>(N);
+    p.h = dalloc<double>(N);
+    p.rho = dalloc<double>(N);
+    p.p = dalloc<double>(N);
+    p.S = dalloc<sphfem_double4>(N);
+    p.R = dalloc<sphfem_double4>(N);
+    p.fc = dalloc<sphfem_double2>(N);
+    p.ft = dalloc<sphfem_double2>(N);
+    p.n = dalloc<sphfem_double2>(N);
+    p.fixed = dalloc<double>(N);
+    p.blanked = dalloc<double>(N);
+    p.tool_particle = dalloc<double>(N);
+    p.eps_pl = dalloc<double>(N);
+    p.eps_pl_dot = dalloc<double>(N);
+    p.T = dalloc<double>(N);
+    p.pos_t = dalloc<sphfem_double2>(N);
+    p.vel_t = dalloc<sphfem_double2>(N);
+    p.rho_t = dalloc<double>(N);
+    p.S_t = dalloc<sphfem_double4>(N);
+    p.T_t = dalloc<double>(N);
+    p.S_der = dalloc<sphfem_double4>(N);
+    p.v_der = dalloc<sphfem_double4>(N);
+    p.idx = dalloc<int>(N);
+    p.hash = dalloc<int>(N);
+    p.on_seg = dalloc<int>(N);
+    // zero everything (doubles and ints alike)
+    auto z = [&](void *ptr, size_t bytes) { SPHFEM_HIP_CHECK(hipMemsetAsync(ptr, 0, bytes, s)); };
+    z(p.pos, 16ul * N); z(p.vel, 16ul * N); z(p.h, 8ul * N); z(p.rho, 8ul * N); z(p.p, 8ul * N);
+    z(p.S, 32ul * N); z(p.R, 32ul * N); z(p.fc, 16ul * N); z(p.ft, 16ul * N); z(p.n, 16ul * N);
+    z(p.fixed, 8ul * N); z(p.blanked, 8ul * N); z(p.tool_particle, 8ul * N); z(p.eps_pl, 8ul * N);
+    z(p.eps_pl_dot, 8ul * N); z(p.T, 8ul * N); z(p.pos_t, 16ul * N); z(p.vel_t, 16ul * N);
+    z(p.rho_t, 8ul * N); z(p.S_t, 32ul * N); z(p.T_t, 8ul * N); z(p.S_der, 32ul * N); z(p.v_der, 32ul * N);
+    z(p.hash, 4ul * N); z(p.on_seg, 4ul * N);
+    std::vector<int> seq(N);
+    for (unsigned i = 0; i < N; i++) seq[i] = (int)i;
+    SPHFEM_HIP_CHECK(hipMemcpyAsync(p.idx, seq.data(), 4ul * N, hipMemcpyHostToDevice, s));
+    SPHFEM_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void free_particles(sphfem_particles &p) {
+    void **f = reinterpret_cast<void **>(&p);
+    const int nptr = (int)((offsetof(sphfem_particles, N)) / sizeof(void *));
+    for (int k = 0; k < nptr; k++) {
+        if (f[k]) (void)hipFree(f[k]);
+        f[k] = nullptr;
+    }
+}
+
+// grid_gpu_green.cu:152-182: swap the 20 gathered arrays with the buffer
+void swap_gathered(sphfem_particles &p, sphfem_particles &b) {
+    std::swap(p.pos, b.pos); std::swap(p.vel, b.vel); std::swap(p.fc, b.fc); std::swap(p.ft, b.ft);
+    std::swap(p.h, b.h); std::swap(p.n, b.n); std::swap(p.rho, b.rho); std::swap(p.S, b.S);
+    std::swap(p.eps_pl, b.eps_pl); std::swap(p.eps_pl_dot, b.eps_pl_dot); std::swap(p.T, b.T);
+    std::swap(p.fixed, b.fixed); std::swap(p.blanked, b.blanked); std::swap(p.tool_particle, b.tool_particle);
+    std::swap(p.pos_t, b.pos_t); std::swap(p.vel_t, b.vel_t); std::swap(p.rho_t, b.rho_t); std::swap(p.S_t, b.S_t);
+    std::swap(p.T_t, b.T_t); std::swap(p.on_seg, b.on_seg);
+}
+
+// after the fused step: the arrays written in the sorted buffer become the particle arrays
+void swap_fused(sphfem_particles &p, sphfem_particles &b) {
+    std::swap(p.pos_t, b.pos_t); std::swap(p.vel_t, b.vel_t); std::swap(p.rho_t, b.rho_t); std::swap(p.S_t, b.S_t);
+    std::swap(p.T_t, b.T_t); std::swap(p.fc, b.fc); std::swap(p.ft, b.ft); std::swap(p.n, b.n);
+    std::swap(p.on_seg, b.on_seg); std::swap(p.h, b.h); std::swap(p.fixed, b.fixed);
+    std::swap(p.tool_particle, b.tool_particle); std::swap(p.p, b.p); std::swap(p.R, b.R);
+}
+
+}  // namespace
+
+namespace sphfem {
+
+void timing_begin(sphfem_ctx *ctx, const char *name, hipEvent_t *ev) {
+    if (!ctx->timing.enabled) return;
+    (void)hipEventCreate(ev);
+    (void)hipEventRecord(*ev, ctx->stream);
+}
+
+void timing_end(sphfem_ctx *ctx, const char *name, hipEvent_t ev) {
+    if (!ctx->timing.enabled || !ev) return;
+    hipEvent_t b;
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(b, ctx->stream);
+    ctx->timing.pending[name].push_back({ev, b});
+}
+
+}  // namespace sphfem
+
+static void timing_collect(sphfem_ctx *ctx) {
+    for (auto &kv : ctx->timing.pending) {
+        for (auto &r : kv.second) {
+            (void)hipEventSynchronize(r.b);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, r.a, r.b);
+            ctx->timing.total_ms[kv.first] += ms;
+            ctx->timing.count[kv.first] += 1;
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        kv.second.clear();
+    }
+}
+
+extern "C" {
+
+int sphfem_create(int device, sphfem_ctx **out) {
+    if (!out) return SPHFEM_ERR_ARG;
+    *out = nullptr;
+    auto *ctx = new sphfem_ctx();
+    ctx->device = device;
+    int rc = guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        ctx->d_flags = dalloc<int>(8);
+        SPHFEM_HIP_CHECK(hipMemsetAsync(ctx->d_flags, 0, 8 * sizeof(int), ctx->stream));
+        ctx->c.rr2_min = 1e-8;
+        return SPHFEM_OK;
+    });
+    if (rc != SPHFEM_OK) {
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return SPHFEM_OK;
+}
+
+int sphfem_destroy(sphfem_ctx *ctx) {
+    if (!ctx) return SPHFEM_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    timing_collect(ctx);
+    dfree(ctx->d_flags);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SPHFEM_OK;
+}
+
+const char *sphfem_last_error(sphfem_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int sphfem_synchronize(sphfem_ctx *ctx) {
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_check_error(sphfem_ctx *ctx) {
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        int f = 0;
+        SPHFEM_HIP_CHECK(hipMemcpy(&f, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost));
+        if (f & 1) return fail(ctx, SPHFEM_ERR_RANGE, "a particle left the cell grid (hash clamped)");
+        if (f & 6) return fail(ctx, SPHFEM_ERR_STATE, "inconsistent sort or cell table detected");
+        return SPHFEM_OK;
+    });
+}
+
+#define SETC(fn, field, bit, extra)                                           \
+    int fn(sphfem_ctx *ctx, const decltype(sphfem::DevConsts::field) *c) {    \
+        if (!ctx || !c) return SPHFEM_ERR_ARG;                                \
+        ctx->c.field = *c;                                                    \
+        ctx->set_mask |= bit;                                                 \
+        extra;                                                                \
+        return SPHFEM_OK;                                                     \
+    }
+
+SETC(sphfem_setup_physical_constants, phys, SET_PHYS, )
+SETC(sphfem_setup_corrector_constants, corr, SET_CORR, )
+SETC(sphfem_setup_johnson_cook_constants, joco, SET_JOCO, ctx->c.plastic = 1)
+SETC(sphfem_setup_thermal_constants_wp, trml_wp, SET_TRML_WP, )
+SETC(sphfem_setup_thermal_constants_tool, trml_tool, 0, ctx->c.thermal_bc = (c->alpha != 0.) ? 1 : 0)
+SETC(sphfem_setup_tool_constants, tc, SET_TOOLC, )
+SETC(sphfem_setup_wear_constants, wear, 0, )
+#undef SETC
+
+int sphfem_set_dt(sphfem_ctx *ctx, double dt) {
+    if (!ctx || !(dt > 0.)) return SPHFEM_ERR_ARG;
+    ctx->c.dt = dt;
+    ctx->set_mask |= SET_DT;
+    return SPHFEM_OK;
+}
+
+int sphfem_set_wear(sphfem_ctx *ctx, int wear_pro) {
+    if (!ctx) return SPHFEM_ERR_ARG;
+    ctx->c.wear_pro = wear_pro ? 1 : 0;
+    return SPHFEM_OK;
+}
+
+int sphfem_set_kernel_cutoff(sphfem_ctx *ctx, double rr2_min) {
+    if (!ctx || !(rr2_min >= 0.)) return SPHFEM_ERR_ARG;
+    ctx->c.rr2_min = rr2_min;
+    return SPHFEM_OK;
+}
+
+int sphfem_malloc(sphfem_ctx *ctx, void **ptr, size_t bytes) {
+    if (!ptr) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipMalloc(ptr, bytes ? bytes : 1));
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_free(sphfem_ctx *ctx, void *ptr) {
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (ptr) SPHFEM_HIP_CHECK(hipFree(ptr));
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_memcpy_htod(sphfem_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if ((!dst || !src) && bytes) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_memcpy_dtoh(sphfem_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if ((!dst || !src) && bytes) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_particles_alloc(sphfem_ctx *ctx, unsigned int N, sphfem_particles *out) {
+    if (!out || N == 0) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        alloc_particles(ctx->stream, N, *out);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_particles_free(sphfem_ctx *ctx, sphfem_particles *p) {
+    if (!p) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        free_particles(*p);
+        return SPHFEM_OK;
+    });
+}
+
+// ---- grid (grid.cu:130-149, grid_gpu_green.cu:220-268)
+int sphfem_grid_green_create(sphfem_ctx *ctx, int num_part, sphfem_double2 bbmin, sphfem_double2 bbmax, double h,
+                             sphfem_grid **out) {
+    if (!out || num_part <= 0 || !(h > 0.)) return SPHFEM_ERR_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        auto *g = new sphfem_grid();
+        double bx0 = bbmin.x - 1e-6, by0 = bbmin.y - 1e-6;
+        double bx1 = bbmax.x + 1e-6, by1 = bbmax.y + 1e-6;
+        double kernel_width = 2.;
+        double dx = h * kernel_width / 2.;
+        double nxd = std::ceil((bx1 - bx0) / dx), nyd = std::ceil((by1 - by0) / dx);
+        if (!(nxd * nyd < 2.0e9)) {
+            delete g;
+            return fail(ctx, SPHFEM_ERR_ARG, "grid too large");
+        }
+        g->geom.nx = (int)nxd;
+        g->geom.ny = (int)nyd;
+        g->geom.bbmin_x = bx0;
+        g->geom.bbmin_y = by0;
+        g->geom.dx = dx;
+        g->bbmax_x = bx1;
+        g->bbmax_y = by1;
+        g->num_cell = g->geom.nx * g->geom.ny;
+        g->num_part = num_part;
+        int bits = 1;
+        while (bits < 31 && (1ll << bits) < (long long)g->num_cell) bits++;
+        g->key_bits = bits;
+        g->cell_start = dalloc<int>(g->num_cell);
+        g->cell_end = dalloc<int>(g->num_cell);
+        SPHFEM_HIP_CHECK(hipMemsetAsync(g->cell_end, 0, sizeof(int) * (size_t)g->num_cell, ctx->stream));
+        launch_fill_int(ctx, g->cell_start, g->num_cell, -1);
+        g->keys_alt = dalloc<int>(num_part);
+        g->vals_alt = dalloc<int>(num_part);
+        g->nblocks = nblk(num_part, 4096);
+        g->hist = dalloc<int>(256ul * g->nblocks);
+        g->digit_tot = dalloc<int>(256);
+        alloc_particles(ctx->stream, (unsigned)num_part, g->buf);
+        g->first_sort = true;
+        ctx->c.geom = g->geom;
+        *out = g;
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_grid_destroy(sphfem_ctx *ctx, sphfem_grid *g) {
+    if (!g) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        dfree(g->cell_start); dfree(g->cell_end); dfree(g->keys_alt); dfree(g->vals_alt);
+        dfree(g->hist); dfree(g->digit_tot);
+        free_particles(g->buf);
+        delete g;
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_grid_geometry(const sphfem_grid *g, sphfem_geom_constants *geom, int *num_cell) {
+    if (!g) return SPHFEM_ERR_ARG;
+    if (geom) *geom = g->geom;
+    if (num_cell) *num_cell = g->num_cell;
+    return SPHFEM_OK;
+}
+
+int sphfem_grid_assign_hashes(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles *p) {
+    if (!g || !p || (int)p->N != g->num_part) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        ctx->c.geom = g->geom;
+        launch_hash(ctx, g, *p, false, nullptr);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_grid_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles *p) {
+    if (!g || !p || (int)p->N != g->num_part) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        launch_radix_sort(ctx, g, *p, false);
+        launch_fill_int(ctx, g->cell_start, g->num_cell, -1);
+        launch_reorder_full(ctx, g, *p);
+        swap_gathered(*p, g->buf);
+        g->first_sort = false;
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_grid_get_cells(sphfem_ctx *ctx, sphfem_grid *g, int *cell_start, int *cell_end) {
+    if (!g || !cell_start || !cell_end) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(cell_start, g->cell_start, sizeof(int) * (size_t)g->num_cell,
+                                        hipMemcpyDeviceToDevice, ctx->stream));
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(cell_end, g->cell_end, sizeof(int) * (size_t)g->num_cell,
+                                        hipMemcpyDeviceToDevice, ctx->stream));
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_interactions_setup_geometry_constants(sphfem_ctx *ctx, const sphfem_grid *g) {
+    if (!ctx || !g) return SPHFEM_ERR_ARG;
+    ctx->c.geom = g->geom;
+    return SPHFEM_OK;
+}
+
+static int need(sphfem_ctx *ctx, unsigned mask) {
+    if ((ctx->set_mask & mask) != mask) return fail(ctx, SPHFEM_ERR_STATE, "constants not set up");
+    return SPHFEM_OK;
+}
+
+int sphfem_interactions_monaghan(sphfem_ctx *ctx, sphfem_particles *p, const int *cell_start, const int *cell_end,
+                                 int num_cell) {
+    if (!p || !cell_start || !cell_end || num_cell != ctx->c.geom.nx * ctx->c.geom.ny) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (int rc = need(ctx, SET_PHYS | SET_CORR | SET_TRML_WP)) return rc;
+        launch_interactions_ref(ctx, *p, cell_start, cell_end);
+        return SPHFEM_OK;
+    });
+}
+
+#define ACTION(fn, which, mask)                                   \
+    int fn(sphfem_ctx *ctx, sphfem_particles *p) {                \
+        if (!p) return SPHFEM_ERR_ARG;                            \
+        return guarded(ctx, [&] {                                 \
+            if (int rc = need(ctx, mask)) return rc;              \
+            launch_actions(ctx, which, *p);                       \
+            return SPHFEM_OK;                                     \
+        });                                                       \
+    }
+
+ACTION(sphfem_material_eos, ACT_EOS, SET_PHYS)
+ACTION(sphfem_corrector_artificial_stress, ACT_ARTSTRESS, SET_CORR)
+ACTION(sphfem_material_stress_rate_jaumann, ACT_JAUMANN, SET_PHYS)
+ACTION(sphfem_contmech_continuity, ACT_CONTINUITY, 0)
+ACTION(sphfem_contmech_momentum, ACT_MOMENTUM, SET_PHYS)
+ACTION(sphfem_contmech_advection, ACT_ADVECTION, 0)
+ACTION(sphfem_perform_boundary_conditions, ACT_BC, 0)
+ACTION(sphfem_debug_invalidate, ACT_INVALIDATE, 0)
+#undef ACTION
+
+int sphfem_plasticity_johnson_cook(sphfem_ctx *ctx, sphfem_particles *p) {
+    if (!p) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->c.plastic) return SPHFEM_OK;   // actions_gpu.cu:875
+        if (int rc = need(ctx, SET_PHYS | SET_TRML_WP | SET_DT)) return rc;
+        launch_actions(ctx, ACT_PLASTICITY, *p);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_perform_boundary_conditions_thermal(sphfem_ctx *ctx, sphfem_particles *p) {
+    if (!p) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->c.thermal_bc) return SPHFEM_OK;   // actions_gpu.cu:882
+        launch_actions(ctx, ACT_BC_THERMAL, *p);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_material_fric_heat_gen_FEM_v2(sphfem_ctx *ctx, sphfem_particles *p, sphfem_tool *t) {
+    if (!p || !t) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (int rc = need(ctx, SET_PHYS | SET_TRML_WP | SET_TOOLC | SET_DT)) return rc;
+        launch_fric_heat(ctx, *p, t);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_compute_contact_forces_FEM_tool_v3(sphfem_ctx *ctx, sphfem_particles *p, sphfem_tool *t) {
+    if (!p || !t) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (int rc = need(ctx, SET_TRML_WP | SET_TOOLC | SET_DT)) return rc;
+        launch_contact(ctx, *p, t);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_wear_adjustment(sphfem_ctx *ctx, sphfem_tool *t) {
+    if (!t) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (int rc = need(ctx, SET_TOOLC | SET_DT)) return rc;
+        launch_wear(ctx, t);
+        return SPHFEM_OK;
+    });
+}
+
+// ---- tool
+int sphfem_tool_create(sphfem_ctx *ctx, const sphfem_tool_desc *d, sphfem_tool **out) {
+    if (!d || !out || d->node_num <= 0 || d->ele_num <= 0 || d->nz_num <= 0 || d->seg_num <= 0) return SPHFEM_ERR_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        auto *t = new sphfem_tool();
+        hipStream_t s = ctx->stream;
+        const int nn = d->node_num, ne = d->ele_num, nz = d->nz_num, ns = d->seg_num;
+        t->n_nodes = nn; t->n_elem = ne; t->nnz = nz; t->n_seg = ns;
+        t->n_colors = d->color_total; t->n_cut_colors = d->color_cutting_edge;
+        t->pos = dupload(s, d->position, nn);
+        t->elem = dupload(s, d->node_num_el, ne);
+        t->row_ptr = dupload(s, d->csr_row_ptr, nn + 1);
+        t->col_ind = dupload(s, d->csr_col_ind, nz);
+        t->K = dupload(s, d->K, nz);
+        t->K_temp = dupload(s, d->K, nz);
+        t->M = dupload(s, d->M, nn);
+        std::vector<double> minv(nn);
+        for (int i = 0; i < nn; i++) minv[i] = 1. / d->M[i];
+        t->Minv = dupload(s, minv.data(), nn);
+        t->f = dupload(s, d->f, nn);
+        t->f_temp = dupload(s, d->f, nn);
+        t->T = dupload(s, d->T, nn);
+        t->T_old = dupload(s, d->T, nn);
+        std::vector<double> zeros((size_t)std::max(nn, 4 * ns), 0.);
+        t->k = dupload(s, zeros.data(), nn);
+        t->left = dupload(s, d->left, ns);
+        t->right = dupload(s, d->right, ns);
+        t->n = dupload(s, d->n, ns);
+        t->marks = dupload(s, d->marks, ns);
+        t->heat_exchange = dupload(s, reinterpret_cast<const sphfem_double2 *>(zeros.data()), ns);
+        t->fric_heat = dupload(s, zeros.data(), ns);
+        t->wear_rate = dupload(s, reinterpret_cast<const sphfem_double2 *>(zeros.data()), ns);
+        t->wear_nodes = dupload(s, zeros.data(), ns);
+        t->physical_para = dupload(s, reinterpret_cast<const sphfem_double4 *>(zeros.data()), ns);
+        t->sliding_force = dupload(s, zeros.data(), ns);
+        // diagonal index per row
+        std::vector<int> diag(nn, -1);
+        for (int i = 0; i < nn; i++)
+            for (int k = d->csr_row_ptr[i]; k < d->csr_row_ptr[i + 1]; k++)
+                if (d->csr_col_ind[k] == i) diag[i] = k;
+        for (int i = 0; i < nn; i++)
+            if (diag[i] < 0) throw std::runtime_error("CSR row without diagonal entry");
+        t->diag = dupload(s, diag.data(), nn);
+        // Robin contributions per node in (colour, segment) order
+        std::vector<std::vector<std::pair<long, int>>> rob(nn);
+        for (int q = 0; q < ns; q++) {
+            const sphfem_int4 m = d->marks[q];
+            long key = (long)m.y * (1l << 31) + q;
+            rob[m.z].push_back({key, q});
+            if (m.w != m.z) rob[m.w].push_back({key, q});
+        }
+        std::vector<int2> nrob(nn, make_int2(-1, -1));
+        for (int i = 0; i < nn; i++) {
+            auto &v = rob[i];
+            std::sort(v.begin(), v.end());
+            if (v.size() > 2) throw std::runtime_error("a tool node belongs to more than two cutting segments");
+            if (v.size() > 0) nrob[i].x = v[0].second;
+            if (v.size() > 1) nrob[i].y = v[1].second;
+        }
+        t->node_robin = dupload(s, nrob.data(), nn);
+        // nodal-shift contributions per node in segment order
+        std::vector<std::vector<int2>> sh(nn);
+        for (int q = 0; q < ns; q++) {
+            sh[d->marks[q].z].push_back(make_int2(q, 0));
+            sh[d->marks[q].w].push_back(make_int2(q, 1));
+        }
+        std::vector<int> shp(nn + 1, 0);
+        std::vector<int2> shl;
+        for (int i = 0; i < nn; i++) {
+            std::sort(sh[i].begin(), sh[i].end(), [](int2 a, int2 b) { return a.x < b.x || (a.x == b.x && a.y < b.y); });
+            for (auto e : sh[i]) shl.push_back(e);
+            shp[i + 1] = (int)shl.size();
+        }
+        t->node_shift_ptr = dupload(s, shp.data(), nn + 1);
+        t->node_shift = dupload(s, shl.data(), shl.size());
+        // boundary-layer elements in (colour, element) order and their scatter lists
+        std::vector<int> lay;
+        for (int e = 0; e < ne; e++)
+            if (d->layer && d->layer[e]) lay.push_back(e);
+        std::stable_sort(lay.begin(), lay.end(),
+                         [&](int a, int b) { return d->node_num_el[a].w < d->node_num_el[b].w; });
+        t->n_layer = (int)lay.size();
+        std::vector<std::vector<int>> nzl(nz), ndl(nn);
+        auto find = [&](int row, int col) {
+            for (int k = d->csr_row_ptr[row]; k < d->csr_row_ptr[row + 1]; k++)
+                if (d->csr_col_ind[k] == col) return k;
+            throw std::runtime_error("element entry missing from the CSR pattern");
+        };
+        for (int j = 0; j < t->n_layer; j++) {
+            const sphfem_int4 el = d->node_num_el[lay[j]];
+            const int nd[3] = {el.x, el.y, el.z};
+            for (int a = 0; a < 3; a++) {
+                ndl[nd[a]].push_back((j << 2) | a);
+                for (int b = 0; b < 3; b++) nzl[find(nd[a], nd[b])].push_back((j << 4) | (a * 3 + b));
+            }
+        }
+        std::vector<int> nzp(nz + 1, 0), nzv, ndp(nn + 1, 0), ndv;
+        for (int k = 0; k < nz; k++) {
+            for (int v : nzl[k]) nzv.push_back(v);
+            nzp[k + 1] = (int)nzv.size();
+        }
+        for (int i = 0; i < nn; i++) {
+            for (int v : ndl[i]) ndv.push_back(v);
+            ndp[i + 1] = (int)ndv.size();
+        }
+        t->layer_elems = dupload(s, lay.data(), lay.size());
+        t->layer_old = dalloc<double>(10ul * std::max(t->n_layer, 1));
+        t->layer_new = dalloc<double>(10ul * std::max(t->n_layer, 1));
+        t->nnz_layer_ptr = dupload(s, nzp.data(), nz + 1);
+        t->nnz_layer = dupload(s, nzv.data(), nzv.size());
+        t->node_layer_ptr = dupload(s, ndp.data(), nn + 1);
+        t->node_layer = dupload(s, ndv.data(), ndv.size());
+        sphfem_tool_kinematics k{};
+        k.vel_x = d->vel.x; k.vel_y = d->vel.y;
+        k.tl_x = d->tl_point_ref.x; k.tl_y = d->tl_point_ref.y;
+        k.br_x = d->br_point.x; k.br_y = d->br_point.y;
+        k.ref_dist = d->ref_dist; k.lowest_point = d->lowest_point; k.right_point = d->right_point;
+        k.flank_retreat = 0.; k.flank_contact = d->lowest_point;
+        t->kin = dupload(s, &k, 1);
+        t->host_kin = k;
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(s));
+        *out = t;
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_tool_destroy(sphfem_ctx *ctx, sphfem_tool *t) {
+    if (!t) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        dfree(t->pos); dfree(t->elem); dfree(t->row_ptr); dfree(t->col_ind); dfree(t->diag);
+        dfree(t->K); dfree(t->K_temp); dfree(t->M); dfree(t->Minv); dfree(t->f); dfree(t->f_temp);
+        dfree(t->T); dfree(t->T_old); dfree(t->k); dfree(t->left); dfree(t->right); dfree(t->n);
+        dfree(t->heat_exchange); dfree(t->fric_heat); dfree(t->wear_rate); dfree(t->wear_nodes);
+        dfree(t->physical_para); dfree(t->sliding_force); dfree(t->marks); dfree(t->node_robin);
+        dfree(t->node_shift_ptr); dfree(t->node_shift); dfree(t->layer_elems); dfree(t->layer_old);
+        dfree(t->layer_new); dfree(t->nnz_layer_ptr); dfree(t->nnz_layer); dfree(t->node_layer_ptr);
+        dfree(t->node_layer); dfree(t->succ); dfree(t->kin);
+        delete t;
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_tool_get_kinematics(sphfem_ctx *ctx, sphfem_tool *t, sphfem_tool_kinematics *out) {
+    if (!t || !out) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(out, t->kin, sizeof(*out), hipMemcpyDeviceToHost, ctx->stream));
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_tool_set_vel(sphfem_ctx *ctx, sphfem_tool *t, sphfem_double2 vel) {
+    if (!t) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        sphfem_tool_kinematics k;
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(&k, t->kin, sizeof(k), hipMemcpyDeviceToHost, ctx->stream));
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        k.vel_x = vel.x;
+        k.vel_y = vel.y;
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(t->kin, &k, sizeof(k), hipMemcpyHostToDevice, ctx->stream));
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_tool_segments_update(sphfem_ctx *ctx, sphfem_tool *t, double dt) {
+    if (!t) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        launch_tool_move(ctx, t, dt, true, false);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_tool_update_FEM_gpu(sphfem_ctx *ctx, sphfem_tool *t, double dt) {
+    if (!t) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        launch_tool_move(ctx, t, dt, false, true);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_tool_segments_wear_rate_reset(sphfem_ctx *ctx, sphfem_tool *t) {
+    if (!t) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        launch_wear_reset(ctx, t);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_tool_apply_heat_boundary_condition(sphfem_ctx *ctx, sphfem_tool *t) {
+    if (!t) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (int rc = need(ctx, SET_TOOLC)) return rc;
+        launch_fem_robin(ctx, t);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_FEM_reset_server(sphfem_ctx *ctx, sphfem_tool *t) {
+    if (!t) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        launch_fem_reset(ctx, t);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_FEM_calculate_temp(sphfem_ctx *ctx, sphfem_tool *t, double dt) {
+    if (!t) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        launch_fem_temp(ctx, t, dt);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_FEM_calculate_temp_rate(sphfem_ctx *ctx, sphfem_tool *t) {
+    if (!t) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        launch_fem_rate(ctx, t);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_tool_download(sphfem_ctx *ctx, sphfem_tool *t, sphfem_double2 *position, double *T, double *K, double *M,
+                         double *k, sphfem_double2 *left, sphfem_double2 *right, sphfem_double2 *n,
+                         sphfem_double2 *heat_exchange, double *fric_heat, sphfem_double4 *physical_para,
+                         double *sliding_force, double *wear_nodes) {
+    if (!t) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        auto cp = [&](void *dst, const void *src, size_t bytes) {
+            if (dst) SPHFEM_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        };
+        cp(position, t->pos, 16ul * t->n_nodes);
+        cp(T, t->T, 8ul * t->n_nodes);
+        cp(K, t->K, 8ul * t->nnz);
+        cp(M, t->M, 8ul * t->n_nodes);
+        cp(k, t->k, 8ul * t->n_nodes);
+        cp(left, t->left, 16ul * t->n_seg);
+        cp(right, t->right, 16ul * t->n_seg);
+        cp(n, t->n, 16ul * t->n_seg);
+        cp(heat_exchange, t->heat_exchange, 16ul * t->n_seg);
+        cp(fric_heat, t->fric_heat, 8ul * t->n_seg);
+        cp(physical_para, t->physical_para, 32ul * t->n_seg);
+        cp(sliding_force, t->sliding_force, 8ul * t->n_seg);
+        cp(wear_nodes, t->wear_nodes, 8ul * t->n_seg);
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_tool_upload(sphfem_ctx *ctx, sphfem_tool *t, const sphfem_double2 *position, const double *T,
+                       const double *K, const double *M, const double *k, const sphfem_double2 *left,
+                       const sphfem_double2 *right, const sphfem_double2 *n, const sphfem_double2 *heat_exchange,
+                       const double *fric_heat, const sphfem_double4 *physical_para, const double *sliding_force,
+                       const double *wear_nodes) {
+    if (!t) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        auto cp = [&](void *dst, const void *src, size_t bytes) {
+            if (src) SPHFEM_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        };
+        cp(t->pos, position, 16ul * t->n_nodes);
+        cp(t->T, T, 8ul * t->n_nodes);
+        cp(t->K, K, 8ul * t->nnz);
+        cp(t->M, M, 8ul * t->n_nodes);
+        cp(t->k, k, 8ul * t->n_nodes);
+        cp(t->left, left, 16ul * t->n_seg);
+        cp(t->right, right, 16ul * t->n_seg);
+        cp(t->n, n, 16ul * t->n_seg);
+        cp(t->heat_exchange, heat_exchange, 16ul * t->n_seg);
+        cp(t->fric_heat, fric_heat, 8ul * t->n_seg);
+        cp(t->physical_para, physical_para, 32ul * t->n_seg);
+        cp(t->sliding_force, sliding_force, 8ul * t->n_seg);
+        cp(t->wear_nodes, wear_nodes, 8ul * t->n_seg);
+        if (M) {
+            std::vector<double> minv(t->n_nodes);
+            for (int i = 0; i < t->n_nodes; i++) minv[i] = 1. / M[i];
+            SPHFEM_HIP_CHECK(hipMemcpyAsync(t->Minv, minv.data(), 8ul * t->n_nodes, hipMemcpyHostToDevice,
+                                            ctx->stream));
+            SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        }
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_tool_set_kinematics(sphfem_ctx *ctx, sphfem_tool *t, const sphfem_tool_kinematics *kin) {
+    if (!t || !kin) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(t->kin, kin, sizeof(*kin), hipMemcpyHostToDevice, ctx->stream));
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return SPHFEM_OK;
+    });
+}
+
+// ---- leap frog
+int sphfem_leap_frog_create(sphfem_ctx *ctx, unsigned int num_part, unsigned int num_cell, int node_num,
+                            sphfem_leap_frog **out) {
+    (void)node_num;
+    if (!out || num_part == 0) return SPHFEM_ERR_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        auto *lf = new sphfem_leap_frog();
+        lf->num_part = num_part;
+        lf->num_cell = num_cell;
+        lf->pos_init = dalloc<sphfem_double2>(num_part);
+        lf->vel_init = dalloc<sphfem_double2>(num_part);
+        lf->S_init = dalloc<sphfem_double4>(num_part);
+        lf->rho_init = dalloc<double>(num_part);
+        lf->T_init = dalloc<double>(num_part);
+        lf->cell_start = dalloc<int>(std::max(num_cell, 1u));
+        lf->cell_end = dalloc<int>(std::max(num_cell, 1u));
+        lf->near_list = dalloc<int>(num_part);
+        lf->contact_list = dalloc<int>(num_part);
+        *out = lf;
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_leap_frog_destroy(sphfem_ctx *ctx, sphfem_leap_frog *lf) {
+    if (!lf) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        dfree(lf->pos_init); dfree(lf->vel_init); dfree(lf->S_init); dfree(lf->rho_init); dfree(lf->T_init);
+        dfree(lf->cell_start); dfree(lf->cell_end); dfree(lf->near_list); dfree(lf->contact_list);
+        delete lf;
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_leap_frog_set_precision(sphfem_ctx *ctx, sphfem_leap_frog *lf, int precision) {
+    if (!ctx || !lf) return SPHFEM_ERR_ARG;
+    if (precision != SPHFEM_FP64) return fail(ctx, SPHFEM_ERR_ARG, "unsupported precision");
+    lf->precision = precision;
+    return SPHFEM_OK;
+}
+
+int sphfem_leap_frog_init(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p) {
+    if (!lf || !p) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        launch_leapfrog(ctx, LF_INIT, *p, lf, 0.);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_leap_frog_predict(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, double dt) {
+    if (!lf || !p) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        launch_leapfrog(ctx, LF_PREDICT, *p, lf, dt);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_leap_frog_correct(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, double dt) {
+    if (!lf || !p) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        launch_leapfrog(ctx, LF_CORRECT, *p, lf, dt);
+        return SPHFEM_OK;
+    });
+}
+
+// leap_frog.cu:162-248, as fused kernels:
+//  tool_begin | hash+digit-0 histogram | radix passes | reorder+init+predict+EOS+art. stress |
+//  pair sums+Jaumann+continuity+momentum+advection+correct+friction heat+plasticity+far contact+
+//  BC+invalidate | FEM rate+temperature | wear | tool move | near contact s1 (+BC) | contact s2
+int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,
+                          sphfem_tool *t) {
+    if (!lf || !p || !g || (int)p->N != g->num_part || p->N != lf->num_part) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (int rc = need(ctx, SET_PHYS | SET_CORR | SET_TRML_WP | SET_DT | (t ? SET_TOOLC : 0u))) return rc;
+        ctx->c.geom = g->geom;
+        if (t) launch_tool_begin(ctx, t);
+        else launch_fill_int(ctx, ctx->d_flags + 1, 2, 0);
+        launch_radix_sort(ctx, g, *p, true);
+        g->first_sort = false;
+        launch_reorder_predict(ctx, g, *p, lf);
+        sphfem_particles B = g->buf;
+        B.hash = p->hash;
+        B.idx = p->idx;
+        B.N = p->N;
+        launch_fused_update(ctx, B, *p, lf, g->cell_start, g->cell_end, t);
+        swap_fused(*p, g->buf);
+        if (t) {
+            launch_tool_fem_end(ctx, t);
+            if (ctx->c.wear_pro) launch_wear(ctx, t);
+            launch_tool_move(ctx, t, ctx->c.dt, true, true);
+            launch_near_contact(ctx, *p, t, lf);
+        }
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_leap_frog_cells(sphfem_leap_frog *lf, const int **cell_start, const int **cell_end) {
+    (void)lf; (void)cell_start; (void)cell_end;
+    return SPHFEM_ERR_STATE;
+}
+
+int sphfem_report_force(sphfem_ctx *ctx, sphfem_particles *p, sphfem_double2 *out) {
+    if (!p || !out) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        double *d = dalloc<double>(2 + 2 * 256);
+        launch_report_force(ctx, *p, d);
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(out, d, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        dfree(d);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_timing_enable(sphfem_ctx *ctx, int enable) {
+    if (!ctx) return SPHFEM_ERR_ARG;
+    ctx->timing.enabled = enable != 0;
+    return SPHFEM_OK;
+}
+
+int sphfem_timing_reset(sphfem_ctx *ctx) {
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        timing_collect(ctx);
+        ctx->timing.total_ms.clear();
+        ctx->timing.count.clear();
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_timing_get(sphfem_ctx *ctx, const char *kernel, double *total_ms, long *launches) {
+    if (!kernel) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        timing_collect(ctx);
+        auto it = ctx->timing.total_ms.find(kernel);
+        if (total_ms) *total_ms = it == ctx->timing.total_ms.end() ? 0. : it->second;
+        if (launches) *launches = ctx->timing.count.count(kernel) ? ctx->timing.count[kernel] : 0;
+        return SPHFEM_OK;
+    });
+}
+
+const char *sphfem_timing_names(sphfem_ctx *ctx) {
+    if (!ctx) return "";
+    timing_collect(ctx);
+    ctx->timing.names.clear();
+    for (auto &kv : ctx->timing.total_ms) ctx->timing.names += kv.first + ",";
+    return ctx->timing.names.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,181 @@
+// Internal declarations shared by the HIP translation units of libsphfem_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/sphfem.h"
+
+namespace sphfem {
+
+// All constant blocks, passed to kernels by value (replaces the __constant__ copies that
+// the reference keeps per translation unit: interactions_gpu.cu:15-20, actions_gpu.cu:23-29,
+// FEM_thermal_gpu.cu:22, tool_FEM.cu:19-21).
+struct DevConsts {
+    sphfem_phys_constants phys;
+    sphfem_trml_constants trml_wp;
+    sphfem_trml_constants trml_tool;
+    sphfem_corr_constants corr;
+    sphfem_joco_constants joco;
+    sphfem_geom_constants geom;
+    sphfem_tool_constants tc;
+    sphfem_wear_constants wear;
+    double dt;
+    double rr2_min;
+    int wear_pro;
+    int thermal_bc;   // actions_gpu.cu:938-943
+    int plastic;      // actions_gpu.cu:921-924
+    int pad;
+};
+
+// Device-resident tool kinematics (the host bookkeeping of tool_FEM.cu:1552-1570 plus the
+// __device__ globals flank_retreat / flank_contact of tool_FEM.cu:19-20).
+typedef sphfem_tool_kinematics ToolKin;
+
+// Views of the particle arrays passed to kernels.
+typedef sphfem_particles Particles;
+
+struct Timing {
+    bool enabled = false;
+    struct Rec { hipEvent_t a, b; };
+    std::map<std::string, std::vector<Rec>> pending;
+    std::map<std::string, double> total_ms;
+    std::map<std::string, long> count;
+    std::string names;
+};
+
+}  // namespace sphfem
+
+struct sphfem_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    sphfem::DevConsts c{};
+    unsigned set_mask = 0;
+    std::string err;
+    int *d_flags = nullptr;       // [0]: out-of-grid, [1]: near-list count, [2]: contact count
+    sphfem::Timing timing;
+};
+
+struct sphfem_grid {
+    sphfem_geom_constants geom{};
+    int num_cell = 0;
+    int num_part = 0;
+    double bbmax_x = 0, bbmax_y = 0;
+    int key_bits = 1;
+    int *cell_start = nullptr, *cell_end = nullptr;
+    // radix sort scratch
+    int *keys_alt = nullptr, *vals_alt = nullptr;
+    int *hist = nullptr;           // [256 * nblocks]
+    int *digit_tot = nullptr;      // [256]
+    int nblocks = 0;
+    // double buffer of the 20 gathered arrays (grid_gpu_green.h:21-50)
+    sphfem_particles buf{};
+    bool first_sort = true;
+};
+
+struct sphfem_tool {
+    int n_nodes = 0, n_elem = 0, nnz = 0, n_seg = 0, n_colors = 0, n_cut_colors = 0;
+    // mesh and FEM system
+    sphfem_double2 *pos = nullptr;
+    sphfem_int4 *elem = nullptr;
+    int *row_ptr = nullptr, *col_ind = nullptr, *diag = nullptr;
+    double *K = nullptr, *K_temp = nullptr, *M = nullptr, *Minv = nullptr, *f = nullptr, *f_temp = nullptr;
+    double *T = nullptr, *T_old = nullptr, *k = nullptr;
+    // segments (segment_FEM_gpu.h:31-56)
+    sphfem_double2 *left = nullptr, *right = nullptr, *n = nullptr, *heat_exchange = nullptr;
+    double *fric_heat = nullptr;
+    sphfem_double2 *wear_rate = nullptr;
+    double *wear_nodes = nullptr;
+    sphfem_double4 *physical_para = nullptr;
+    double *sliding_force = nullptr;
+    sphfem_int4 *marks = nullptr;
+    // deterministic gather lists
+    int2 *node_robin = nullptr;        // [n_nodes] up to two segments, (colour, segment) order, -1 = none
+    int *node_shift_ptr = nullptr;     // [n_nodes + 1] CSR of (segment, side) wear contributions
+    int2 *node_shift = nullptr;
+    int n_layer = 0;
+    int *layer_elems = nullptr;        // [n_layer] element ids in (colour, element) order
+    double *layer_old = nullptr, *layer_new = nullptr;   // [n_layer * 10] mass + 3x3 stiffness
+    int *nnz_layer_ptr = nullptr;      // [nnz + 1] CSR over layer contributions (layer slot * 16 + a*3+b)
+    int *nnz_layer = nullptr;
+    int *node_layer_ptr = nullptr;     // [n_nodes + 1]
+    int *node_layer = nullptr;         // layer slot * 4 + a
+    int *succ = nullptr;               // [n_seg] first segment j with left[j] == right[s], else -1
+    sphfem::ToolKin *kin = nullptr;    // device
+    sphfem_tool_kinematics host_kin{};
+};
+
+struct sphfem_leap_frog {
+    unsigned int num_part = 0, num_cell = 0;
+    int precision = SPHFEM_FP64;
+    sphfem_double2 *pos_init = nullptr, *vel_init = nullptr;
+    sphfem_double4 *S_init = nullptr;
+    double *rho_init = nullptr, *T_init = nullptr;
+    int *cell_start = nullptr, *cell_end = nullptr;   // only used by the unfused path
+    int *near_list = nullptr;      // [num_part] particles near the tool
+    int *contact_list = nullptr;   // [num_part] particles in contact (+ flag bit 30 = invalid)
+    bool intermediates = false;    // also write S_der / v_der in the fused step
+};
+
+namespace sphfem {
+
+// launch helpers ------------------------------------------------------------
+void timing_begin(sphfem_ctx *ctx, const char *name, hipEvent_t *ev);
+void timing_end(sphfem_ctx *ctx, const char *name, hipEvent_t ev);
+
+struct KTimer {
+    sphfem_ctx *ctx;
+    const char *name;
+    hipEvent_t ev = nullptr;
+    KTimer(sphfem_ctx *c, const char *n) : ctx(c), name(n) { timing_begin(ctx, name, &ev); }
+    ~KTimer() { timing_end(ctx, name, ev); }
+};
+
+inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// sort / cells (sphfem_sort.hip)
+void launch_hash(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, bool clear_old, int *hist0);
+void launch_radix_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool hash_done_with_hist0);
+void launch_fill_int(sphfem_ctx *ctx, int *ptr, long n, int value);
+void launch_reorder_full(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p);
+void launch_reorder_predict(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, sphfem_leap_frog *lf);
+
+// particles (sphfem_particles.hip)
+void launch_actions(sphfem_ctx *ctx, int which, const sphfem_particles &p);
+void launch_leapfrog(sphfem_ctx *ctx, int which, const sphfem_particles &p, sphfem_leap_frog *lf, double dt);
+void launch_interactions_ref(sphfem_ctx *ctx, const sphfem_particles &p, const int *cs, const int *ce);
+void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfem_particles &A,
+                         sphfem_leap_frog *lf, const int *cs, const int *ce, sphfem_tool *t);
+void launch_report_force(sphfem_ctx *ctx, const sphfem_particles &p, double *d_out);
+
+// tool (sphfem_tool.hip)
+void launch_fric_heat(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t);
+void launch_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t);
+void launch_tool_begin(sphfem_ctx *ctx, sphfem_tool *t);
+void launch_tool_fem_end(sphfem_ctx *ctx, sphfem_tool *t);
+void launch_tool_move(sphfem_ctx *ctx, sphfem_tool *t, double dt, bool segments, bool nodes);
+void launch_wear(sphfem_ctx *ctx, sphfem_tool *t);
+void launch_fem_reset(sphfem_ctx *ctx, sphfem_tool *t);
+void launch_fem_temp(sphfem_ctx *ctx, sphfem_tool *t, double dt);
+void launch_fem_robin(sphfem_ctx *ctx, sphfem_tool *t);
+void launch_fem_rate(sphfem_ctx *ctx, sphfem_tool *t);
+void launch_wear_reset(sphfem_ctx *ctx, sphfem_tool *t);
+void launch_near_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t, sphfem_leap_frog *lf);
+
+enum { ACT_EOS, ACT_ARTSTRESS, ACT_JAUMANN, ACT_CONTINUITY, ACT_MOMENTUM, ACT_ADVECTION, ACT_PLASTICITY,
+       ACT_BC, ACT_BC_THERMAL, ACT_INVALIDATE };
+enum { LF_INIT, LF_PREDICT, LF_CORRECT };
+
+}  // namespace sphfem
+
+#define SPHFEM_HIP_CHECK(expr)                                         \
+    do {                                                               \
+        hipError_t _e = (expr);                                        \
+        if (_e != hipSuccess) {                                        \
+            throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+        }                                                              \
+    } while (0)

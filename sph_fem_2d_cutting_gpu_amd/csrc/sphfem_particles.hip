@@ -1,0 +1,529 @@
+// Particle kernels: one kernel per reference function (actions_gpu.cu, leap_frog.cu,
+// interactions_gpu.cu) for the drop-in entry points, and the fused FP64 step kernel that runs
+// pair sums, Jaumann rate, continuity, momentum, advection, corrector, friction heat,
+// plasticity, the far-from-tool contact reset, boundary conditions and NaN invalidation in one
+// pass over the sorted particles.
+#include "sphfem_physics.h"
+
+#pragma clang fp contract(off)
+
+namespace sphfem {
+
+// ---------------------------------------------------------------------------------------------
+// pair sums of interactions_gpu.cu:143-379 for particle i, neighbours visited in the reference
+// order: column ii, then row jj, then sorted index.  Within a column the occupied cells of
+// rows [low_j, high_j) are one contiguous run of sorted indices, so each column is one loop.
+// Pairs outside the kernel support contribute exact zeros in the reference and are skipped.
+// ---------------------------------------------------------------------------------------------
+struct PairSums {
+    sphfem_double4 v_der, S_der;
+    sphfem_double2 vel_t, pos_t;
+    double T_lapl;
+};
+
+__device__ __forceinline__ void column_range(const int *__restrict__ cs, const int *__restrict__ ce, int col0,
+                                             int low_j, int high_j, int &first, int &last) {
+    first = -1;
+    last = -1;
+    for (int jj = low_j; jj < high_j; jj++) {
+        int s = cs[col0 + jj];
+        if (s != -1) { first = s; break; }
+    }
+    if (first < 0) return;
+    for (int jj = high_j - 1; jj >= low_j; jj--) {
+        if (cs[col0 + jj] != -1) { last = ce[col0 + jj]; break; }
+    }
+}
+
+__device__ __forceinline__ PairSums pair_sums_fp64(const DevConsts &c, const sphfem_particles &P, int pidx,
+                                                   double thermal_alpha, const int *__restrict__ cs,
+                                                   const int *__restrict__ ce) {
+    const double mass = c.phys.mass, K = c.phys.K;
+    const double wdeltap = c.corr.wdeltap, alpha = c.corr.alpha, beta = c.corr.beta;
+    const double eta = c.corr.eta, eps = c.corr.xspheps;
+    const int nx = c.geom.nx, ny = c.geom.ny;
+    const double rr2_min = c.rr2_min;
+    const double is_tool_i = P.tool_particle[pidx];
+    const sphfem_double2 pi = P.pos[pidx], vi = P.vel[pidx];
+    const sphfem_double4 Si = P.S[pidx], Ri = P.R[pidx];
+    const double hi = P.h[pidx], rhoi = P.rho[pidx], prsi = P.p[pidx];
+    double Ti = 0.;
+    if (thermal_alpha != 0.) Ti = P.T[pidx];
+    const double rhoi21 = 1. / (rhoi * rhoi);
+    const double fourh2 = 4 * hi * hi;
+    const int hashi = P.hash[pidx];
+    const int gi = hashi / ny;
+    const int gj = hashi - gi * ny;
+    const int low_i = gi - 2 < 0 ? 0 : gi - 2;
+    const int low_j = gj - 2 < 0 ? 0 : gj - 2;
+    const int high_i = gi + 3 > nx ? nx : gi + 3;
+    const int high_j = gj + 3 > ny ? ny : gj + 3;
+    PairSums r;
+    r.v_der = {0., 0., 0., 0.};
+    r.S_der = {0., 0., 0., 0.};
+    r.vel_t = {0., 0.};
+    r.pos_t = {0., 0.};
+    r.T_lapl = 0.;
+    for (int ii = low_i; ii < high_i; ii++) {
+        int first, last;
+        column_range(cs, ce, ii * ny, low_j, high_j, first, last);
+        if (first < 0 || last > (int)P.N) continue;   // empty column (or an inconsistent cell table)
+        for (int iter = first; iter < last; iter++) {
+            if (P.blanked[iter] == 1.) continue;
+            const sphfem_double2 pj = P.pos[iter];
+            double xij = pi.x - pj.x;
+            double yij = pi.y - pj.y;
+            double rr2 = xij * xij + yij * yij;
+            if (rr2 >= fourh2 || rr2 < rr2_min) continue;
+            double w, w_x, w_y;
+            cubic_spline(pi.x, pi.y, pj.x, pj.y, hi, rr2_min, w, w_x, w_y);
+            const double rhoj = P.rho[iter];
+            const double is_tool_j = P.tool_particle[iter];
+            if (!(is_tool_i == 1.) && !(is_tool_j == 1.)) {
+                const sphfem_double2 vj = P.vel[iter];
+                const sphfem_double4 Sj = P.S[iter], Rj = P.R[iter];
+                const double hj = P.h[iter], prsj = P.p[iter];
+                double volj = mass / rhoj;
+                double rhoj21 = 1. / (rhoj * rhoj);
+                r.v_der.x += (vj.x - vi.x) * w_x * volj;
+                r.v_der.y += (vj.x - vi.x) * w_y * volj;
+                r.v_der.z += (vj.y - vi.y) * w_x * volj;
+                r.v_der.w += (vj.y - vi.y) * w_y * volj;
+                double fab = w / wdeltap;
+                fab *= fab;
+                fab *= fab;
+                double Rx = fab * (Ri.x + Rj.x);
+                double Ry = fab * (Ri.y + Rj.y);
+                double Rz = fab * (Ri.z + Rj.z);
+                r.S_der.x += mass * ((Si.x - prsi) * rhoi21 + (Sj.x - prsj) * rhoj21 + Rx) * w_x;
+                r.S_der.y += mass * (Si.y * rhoi21 + Sj.y * rhoj21 + Ry) * w_y;
+                r.S_der.z += mass * (Si.y * rhoi21 + Sj.y * rhoj21 + Ry) * w_x;
+                r.S_der.w += mass * ((Si.z - prsi) * rhoi21 + (Sj.z - prsj) * rhoj21 + Rz) * w_y;
+                double vijx = vi.x - vj.x;
+                double vijy = vi.y - vj.y;
+                double vijposij = vijx * xij + vijy * yij;
+                double rhoij = 0.5 * (rhoi + rhoj);
+                if (vijposij < 0.) {
+                    double ci = sqrtf_d(K / rhoi);
+                    double cj = sqrtf_d(K / rhoj);
+                    double cij = 0.5 * (ci + cj);
+                    double hij = 0.5 * (hi + hj);
+                    double r2ij = xij * xij + yij * yij;
+                    double muij = (hij * vijposij) / (r2ij + eta * eta * hij * hij);
+                    double piij = (-alpha * cij * muij + beta * muij * muij) / rhoij;
+                    r.vel_t.x += -mass * piij * w_x;
+                    r.vel_t.y += -mass * piij * w_y;
+                }
+                r.pos_t.x += -eps * w * mass / rhoij * vijx;
+                r.pos_t.y += -eps * w * mass / rhoij * vijy;
+            }
+            if (thermal_alpha != 0.) {
+                double rij = sqrt(xij * xij + yij * yij);
+                if (rij > 1e-8) {
+                    const double Tj = P.T[iter];
+                    double eijx = xij / rij;
+                    double eijy = yij / rij;
+                    double rij1 = 1. / rij;
+                    r.T_lapl += 2.0 * (mass / rhoj) * (Ti - Tj) * rij1 * (eijx * w_x + eijy * w_y);
+                }
+            }
+        }
+    }
+    return r;
+}
+
+__device__ __forceinline__ double interaction_alpha(const DevConsts &c, double is_tool) {
+    // the tool block of interactions_gpu.cu:19 is never set by the FEM setups (benchmarks_NZ.cpp:354)
+    return (is_tool == 0.) ? c.trml_wp.alpha : 0.;
+}
+
+// interactions_gpu.cu:143-379 as one kernel (drop-in entry point)
+__global__ void __launch_bounds__(256) k_interactions_ref(DevConsts c, sphfem_particles P, const int *__restrict__ cs,
+                                                          const int *__restrict__ ce) {
+    int pidx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pidx >= (int)P.N) return;
+    if (P.blanked[pidx] == 1.) return;
+    double thermal_alpha = interaction_alpha(c, P.tool_particle[pidx]);
+    PairSums r = pair_sums_fp64(c, P, pidx, thermal_alpha, cs, ce);
+    P.S_der[pidx] = r.S_der;
+    P.v_der[pidx] = r.v_der;
+    P.pos_t[pidx] = r.pos_t;
+    P.vel_t[pidx] = r.vel_t;
+    if (thermal_alpha != 0.) P.T_t[pidx] = thermal_alpha * r.T_lapl;
+}
+
+void launch_interactions_ref(sphfem_ctx *ctx, const sphfem_particles &p, const int *cs, const int *ce) {
+    KTimer t(ctx, "interactions_ref");
+    k_interactions_ref<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, cs, ce);
+}
+
+// ---------------------------------------------------------------------------------------------
+// per-reference-function kernels (actions_gpu.cu)
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_action(DevConsts c, sphfem_particles P, int which) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int)P.N) return;
+    switch (which) {
+    case ACT_EOS:   // :222-232
+        if (P.tool_particle[i] != 0.) return;
+        P.p[i] = eos_pressure(c, P.rho[i]);
+        break;
+    case ACT_ARTSTRESS:   // :234-278
+        if (P.tool_particle[i] != 0.) return;
+        P.R[i] = artificial_stress(c, P.rho[i], P.p[i], P.S[i]);
+        break;
+    case ACT_JAUMANN:   // :280-314
+        if (P.tool_particle[i] == 1.) return;
+        P.S_t[i] = jaumann_rate(c, P.v_der[i], P.S[i]);
+        break;
+    case ACT_CONTINUITY:   // :626-639
+        if (P.tool_particle[i] != 0.) return;
+        P.rho_t[i] = -P.rho[i] * (P.v_der[i].x + P.v_der[i].w);
+        break;
+    case ACT_MOMENTUM: {   // :641-671
+        if (P.tool_particle[i] != 0.) return;
+        const double mass = c.phys.mass;
+        sphfem_double4 sd = P.S_der[i];
+        sphfem_double2 fc = P.fc[i], ft = P.ft[i], vt = P.vel_t[i];
+        vt.x += sd.x + sd.y + fc.x / mass + ft.x / mass;
+        vt.y += sd.z + sd.w + fc.y / mass + ft.y / mass;
+        P.vel_t[i] = vt;
+        break;
+    }
+    case ACT_ADVECTION: {   // :673-687
+        if (P.tool_particle[i] != 0.) return;
+        sphfem_double2 pt = P.pos_t[i], v = P.vel[i];
+        pt.x = pt.x + v.x;
+        pt.y = pt.y + v.y;
+        P.pos_t[i] = pt;
+        break;
+    }
+    case ACT_PLASTICITY: {   // :689-754
+        if (P.tool_particle[i] != 0.) return;
+        if (P.blanked[i] == 1.) return;
+        sphfem_double4 S = P.S[i];
+        double eps = P.eps_pl[i], epsd = P.eps_pl_dot[i], T = P.T[i];
+        double S_in[4] = {S.x, S.y, S.z, S.w};
+        bool melt = plasticity_jc(c, S, P.p[i], eps, epsd, T, P.rho[i]);
+        P.eps_pl_dot[i] = epsd;
+        if (S.x != S_in[0] || S.y != S_in[1] || S.z != S_in[2] || S.w != S_in[3] || eps != P.eps_pl[i]) {
+            P.eps_pl[i] = eps;
+            P.S[i] = S;
+        }
+        P.T[i] = T;
+        if (melt) P.blanked[i] = 1.;
+        break;
+    }
+    case ACT_BC:   // :765-781
+        if (P.tool_particle[i] != 0.) return;
+        if (P.fixed[i] != 0.) {
+            sphfem_double2 z = {0., 0.};
+            P.vel[i] = z;
+            P.fc[i] = z;
+            P.pos_t[i] = z;
+            P.vel_t[i] = z;
+        }
+        break;
+    case ACT_BC_THERMAL:   // :756-763
+        if (P.fixed[i] == 1.) P.T[i] = c.trml_wp.T_init;
+        break;
+    case ACT_INVALIDATE: {   // :787-824
+        if (P.blanked[i] == 1.) return;
+        sphfem_double2 pt = P.pos_t[i], vt = P.vel_t[i];
+        sphfem_double4 st = P.S_t[i];
+        bool bad = isnaninf(pt.x) || isnaninf(pt.y) || isnaninf(vt.x) || isnaninf(vt.y) || isnaninf(st.x) ||
+                   isnaninf(st.y) || isnaninf(st.z) || isnaninf(st.w) || isnaninf(P.rho_t[i]) || isnaninf(P.T_t[i]);
+        if (bad) P.blanked[i] = 1.;
+        break;
+    }
+    }
+}
+
+void launch_actions(sphfem_ctx *ctx, int which, const sphfem_particles &p) {
+    static const char *names[] = {"eos", "artificial_stress", "jaumann", "continuity", "momentum", "advection",
+                                  "plasticity", "bc", "bc_thermal", "invalidate"};
+    KTimer t(ctx, names[which]);
+    k_action<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, which);
+}
+
+// leap_frog.cu:25-96
+__global__ void __launch_bounds__(256) k_leapfrog(sphfem_particles P, int which, double dt,
+                                                  sphfem_double2 *pos_init, sphfem_double2 *vel_init,
+                                                  sphfem_double4 *S_init, double *rho_init, double *T_init) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int)P.N) return;
+    if (P.blanked[i] == 1.) return;
+    if (which == LF_INIT) {
+        pos_init[i] = P.pos[i];
+        vel_init[i] = P.vel[i];
+        S_init[i] = P.S[i];
+        rho_init[i] = P.rho[i];
+        T_init[i] = P.T[i];
+        return;
+    }
+    double f = which == LF_PREDICT ? 0.5 * dt : dt;
+    sphfem_double2 pt = P.pos_t[i], vt = P.vel_t[i], p0 = pos_init[i], v0 = vel_init[i];
+    sphfem_double4 St = P.S_t[i], S0 = S_init[i];
+    // predict: x0 + 0.5*dt*x_t, correct: x0 + dt*x_t (same products as the reference)
+    P.pos[i] = {p0.x + f * pt.x, p0.y + f * pt.y};
+    P.vel[i] = {v0.x + f * vt.x, v0.y + f * vt.y};
+    P.S[i] = {S0.x + f * St.x, S0.y + f * St.y, S0.z + f * St.z, S0.w + f * St.w};
+    P.rho[i] = rho_init[i] + f * P.rho_t[i];
+    P.T[i] = T_init[i] + f * P.T_t[i];
+}
+
+void launch_leapfrog(sphfem_ctx *ctx, int which, const sphfem_particles &p, sphfem_leap_frog *lf, double dt) {
+    static const char *names[] = {"lf_init", "lf_predict", "lf_correct"};
+    KTimer t(ctx, names[which]);
+    k_leapfrog<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(p, which, dt, lf->pos_init, lf->vel_init, lf->S_init,
+                                                        lf->rho_init, lf->T_init);
+}
+
+// ---------------------------------------------------------------------------------------------
+// fused step kernel (FP64 form).  B: sorted predicted state (read), A: corrected state (write).
+// Fields: A.pos/vel/S/rho/T/eps_pl/eps_pl_dot/blanked are outputs; B.fc/ft/n/on_seg are updated
+// in place (own particle only); derivatives go to A.pos_t/vel_t/rho_t/S_t/T_t.
+// ---------------------------------------------------------------------------------------------
+struct ToolView {
+    const double *T_nodes;
+    const sphfem_int4 *marks;
+    double *fric_heat;
+    sphfem_double4 *physical_para;
+    double *sliding_force;
+    const ToolKin *kin;
+    int *near_list;
+    int *near_count;
+    int active;
+};
+
+// actions_gpu.cu:345-420 for one particle; returns false if the particle was blanked
+__device__ __forceinline__ bool fric_heat_v21(const DevConsts &c, const ToolView &tv, const ToolKin &kin,
+                                              double eps_pl, sphfem_double2 ft, int seg, sphfem_double2 normal,
+                                              sphfem_double2 vp, sphfem_double2 fcv, double rho, double &T) {
+    if (eps_pl > 20.) return true;
+    double f_fric_mag = sqrtf_d(ft.x * ft.x + ft.y * ft.y);
+    if (seg == -1) return true;
+    const double partition_wp = c.tc.beta;
+    const double dt = c.dt;
+    double vdx = vp.x - kin.vel_x, vdy = vp.y - kin.vel_y;
+    double vdn = vdx * normal.x + vdy * normal.y;
+    double vrx = vdx - vdn * normal.x, vry = vdy - vdn * normal.y;
+    double v_rel_mag = sqrtf_d(vrx * vrx + vry * vry);
+    double value = c.tc.eta * f_fric_mag * v_rel_mag;
+    if (isnan(v_rel_mag)) return false;
+    double f_normal = sqrtf_d(fcv.x * fcv.x + fcv.y * fcv.y);
+    double length = sqrtf_d(c.tc.slave_mass / rho);
+    double pressure_trial = f_normal / length;
+    if (v_rel_mag > 0.03) return true;
+    double T_buffer = T + value * (1 - partition_wp) * dt / c.trml_wp.cp / c.phys.mass;
+    if (T_buffer > 1550.) {
+        T = 1550.;
+        return true;
+    }
+    T = T_buffer;
+    unsafeAtomicAdd(&tv.fric_heat[seg], value * partition_wp);
+    sphfem_int4 mk = tv.marks[seg];
+    double T_seg = 0.5 * (tv.T_nodes[mk.z] + tv.T_nodes[mk.w]);
+    double vel_tool_norm = sqrt(kin.vel_x * kin.vel_x + kin.vel_y * kin.vel_y);
+    if (v_rel_mag > 0.008) return true;
+    else if (v_rel_mag > vel_tool_norm * 8.0 && fabs(normal.y) > 0.5) return true;
+    unsafeAtomicAdd(&tv.physical_para[seg].x, 1.);
+    unsafeAtomicAdd(&tv.physical_para[seg].y, pressure_trial);
+    unsafeAtomicAdd(&tv.physical_para[seg].z, v_rel_mag);
+    unsafeAtomicAdd(&tv.physical_para[seg].w, T_seg);
+    unsafeAtomicAdd(&tv.sliding_force[seg], f_fric_mag / length);
+    return true;
+}
+
+// B = sorted predicted state (the grid buffer); A = the caller's arrays, whose unsorted content
+// is dead after the reorder except the previous derivatives.  Outputs: corrected state into
+// A.pos/vel/S/rho/T/eps_pl/eps_pl_dot/blanked, derivatives into B.pos_t/vel_t/rho_t/S_t/T_t,
+// contact fields in place in B.fc/ft/n/on_seg; S_der/v_der into A when requested.
+__global__ void __launch_bounds__(256) k_fused_update_fp64(DevConsts c, sphfem_particles B, sphfem_particles A,
+                                                           const sphfem_double2 *__restrict__ pos_init,
+                                                           const sphfem_double2 *__restrict__ vel_init,
+                                                           const sphfem_double4 *__restrict__ S_init,
+                                                           const double *__restrict__ rho_init,
+                                                           const double *__restrict__ T_init,
+                                                           const int *__restrict__ cs, const int *__restrict__ ce,
+                                                           ToolView tv, int write_intermediates) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int)B.N) return;
+    const double blanked0 = B.blanked[i];
+    const double tool = B.tool_particle[i];
+    const double fixed = B.fixed[i];
+    sphfem_double2 pos = B.pos[i], vel = B.vel[i];
+    sphfem_double4 S = B.S[i];
+    double rho = B.rho[i], T = B.T[i];
+    double eps_pl = B.eps_pl[i], eps_pl_dot = B.eps_pl_dot[i];
+    double blanked = blanked0;
+    sphfem_double2 fc = B.fc[i], ft = B.ft[i], nrm = B.n[i];
+    int on_seg = B.on_seg[i];
+    bool contact_written = false;
+    bool near = false;
+    sphfem_double2 pos_t = {0., 0.}, vel_t = {0., 0.};
+    sphfem_double4 S_t = {0., 0., 0., 0.};
+    double rho_t = 0., T_t = 0.;
+    const bool live = blanked0 != 1.;
+    if (live) {
+        const double thermal_alpha = interaction_alpha(c, tool);
+        PairSums r = pair_sums_fp64(c, B, i, thermal_alpha, cs, ce);
+        pos_t = r.pos_t;
+        vel_t = r.vel_t;
+        T_t = thermal_alpha != 0. ? thermal_alpha * r.T_lapl : A.T_t[A.idx[i]];
+        if (tool == 0.) {
+            S_t = jaumann_rate(c, r.v_der, S);        // actions_gpu.cu:280-314
+            rho_t = -rho * (r.v_der.x + r.v_der.w);     // :626-639
+            const double mass = c.phys.mass;            // :641-671
+            vel_t.x += r.S_der.x + r.S_der.y + fc.x / mass + ft.x / mass;
+            vel_t.y += r.S_der.z + r.S_der.w + fc.y / mass + ft.y / mass;
+            pos_t.x = pos_t.x + vel.x;                  // :673-687
+            pos_t.y = pos_t.y + vel.y;
+        }
+        if (write_intermediates) {
+            A.S_der[i] = r.S_der;
+            A.v_der[i] = r.v_der;
+        }
+        // leap_frog.cu:71-96
+        const double dt = c.dt;
+        sphfem_double2 p0 = pos_init[i], v0 = vel_init[i];
+        sphfem_double4 S0 = S_init[i];
+        pos.x = p0.x + dt * pos_t.x;
+        pos.y = p0.y + dt * pos_t.y;
+        vel.x = v0.x + dt * vel_t.x;
+        vel.y = v0.y + dt * vel_t.y;
+        S.x = S0.x + dt * S_t.x;
+        S.y = S0.y + dt * S_t.y;
+        S.z = S0.z + dt * S_t.z;
+        S.w = S0.w + dt * S_t.w;
+        rho = rho_init[i] + dt * rho_t;
+        T = T_init[i] + dt * T_t;
+        if (tv.active) {
+            if (!fric_heat_v21(c, tv, *tv.kin, eps_pl, ft, on_seg, nrm, vel, fc, rho, T)) blanked = 1.;
+        }
+        if (c.plastic && tool == 0. && blanked != 1.) {
+            if (plasticity_jc(c, S, B.p[i], eps_pl, eps_pl_dot, T, rho)) blanked = 1.;
+        }
+        // contact s1 for particles outside the 1.05 ref_dist disc of the moved tool
+        // (actions_gpu.cu:438-453); the others are finished by the near-tool pass
+        if (tv.active && blanked != 1.) {
+            ToolKin k = *tv.kin;
+            k.tl_x += k.vel_x * dt;
+            k.tl_y += k.vel_y * dt;
+            if (near_tool(k, pos)) {
+                near = true;
+                int slot = atomicAdd(tv.near_count, 1);
+                tv.near_list[slot] = i;
+            } else {
+                sphfem_double2 z = {0., 0.};
+                fc = z; ft = z; nrm = z;
+                on_seg = -1;
+                contact_written = true;
+            }
+        }
+    }
+    if (!near) {
+        // actions_gpu.cu:765-781, 756-763, 787-824
+        if (tool == 0. && fixed != 0.) {
+            sphfem_double2 z = {0., 0.};
+            vel = z;
+            fc = z;
+            pos_t = z;
+            vel_t = z;
+            contact_written = true;
+        }
+        if (c.thermal_bc && fixed == 1.) T = c.trml_wp.T_init;
+        if (blanked != 1. && live) {
+            bool bad = isnaninf(pos_t.x) || isnaninf(pos_t.y) || isnaninf(vel_t.x) || isnaninf(vel_t.y) ||
+                       isnaninf(S_t.x) || isnaninf(S_t.y) || isnaninf(S_t.z) || isnaninf(S_t.w) ||
+                       isnaninf(rho_t) || isnaninf(T_t);
+            if (bad) blanked = 1.;
+        }
+    }
+    B.pos_t[i] = pos_t;
+    B.vel_t[i] = vel_t;
+    B.S_t[i] = S_t;
+    B.rho_t[i] = rho_t;
+    B.T_t[i] = T_t;
+    A.pos[i] = pos;
+    A.vel[i] = vel;
+    A.S[i] = S;
+    A.rho[i] = rho;
+    A.T[i] = T;
+    A.eps_pl[i] = eps_pl;
+    A.eps_pl_dot[i] = eps_pl_dot;
+    A.blanked[i] = blanked;
+    if (contact_written) {
+        B.fc[i] = fc;
+        B.ft[i] = ft;
+        B.n[i] = nrm;
+        B.on_seg[i] = on_seg;
+    }
+}
+
+void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfem_particles &A, sphfem_leap_frog *lf,
+                         const int *cs, const int *ce, sphfem_tool *t) {
+    ToolView tv{};
+    if (t) {
+        tv.T_nodes = t->T;
+        tv.marks = t->marks;
+        tv.fric_heat = t->fric_heat;
+        tv.physical_para = t->physical_para;
+        tv.sliding_force = t->sliding_force;
+        tv.kin = t->kin;
+        tv.near_list = lf->near_list;
+        tv.near_count = ctx->d_flags + 1;
+        tv.active = 1;
+    }
+    KTimer tm(ctx, "fused_update");
+    k_fused_update_fp64<<<nblk(B.N, 256), 256, 0, ctx->stream>>>(ctx->c, B, A, lf->pos_init, lf->vel_init,
+                                                                  lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv,
+                                                                  lf->intermediates ? 1 : 0);
+}
+
+// ---------------------------------------------------------------------------------------------
+// output.cu:22-30: sum of fc and ft over all particles (deterministic two-level reduction)
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_force_partial(sphfem_particles P, double *part) {
+    __shared__ double sx[256], sy[256];
+    double x = 0., y = 0.;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < (int)P.N; i += gridDim.x * 256) {
+        sphfem_double2 a = P.fc[i], b = P.ft[i];
+        x += a.x + b.x;
+        y += a.y + b.y;
+    }
+    sx[threadIdx.x] = x;
+    sy[threadIdx.x] = y;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sx[threadIdx.x] += sx[threadIdx.x + s];
+            sy[threadIdx.x] += sy[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sx[0];
+        part[2 * blockIdx.x + 1] = sy[0];
+    }
+}
+
+__global__ void k_force_final(const double *part, int n, double *out) {
+    if (threadIdx.x != 0) return;
+    double x = 0., y = 0.;
+    for (int b = 0; b < n; b++) {
+        x += part[2 * b];
+        y += part[2 * b + 1];
+    }
+    out[0] = x;
+    out[1] = y;
+}
+
+void launch_report_force(sphfem_ctx *ctx, const sphfem_particles &p, double *d_out) {
+    const int blocks = 256;
+    KTimer t(ctx, "report_force");
+    k_force_partial<<<blocks, 256, 0, ctx->stream>>>(p, d_out + 2);
+    k_force_final<<<1, 64, 0, ctx->stream>>>(d_out + 2, blocks, d_out);
+}
+
+}  // namespace sphfem

@@ -1,0 +1,379 @@
+// Cell list of the locked Green grid (grid.cu:25-42, grid_gpu_green.cu:13-213) for gfx950:
+//  * FP64 cell hash with C truncation; blanked particles keep their previous hash;
+//  * stable LSD radix sort of (hash, index), 8-bit digits, 4096-key tiles, ranking inside a
+//    tile by 64-lane ballot multisplit (stable: wave order, round order, lane order);
+//  * cell boundaries from neighbouring sorted keys via a lane shuffle, and the gather of the
+//    particle arrays (all 20 for the reference-form sort, or fused with init/predict/EOS/
+//    artificial stress for the step).
+#include "sphfem_physics.h"
+
+#pragma clang fp contract(off)
+
+namespace sphfem {
+
+constexpr int SORT_BS = 256;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_TILE = SORT_BS * SORT_ITEMS;   // 4096 keys per block
+constexpr int RADIX = 256;
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ int hash_of(const DevConsts &c, sphfem_double2 pos) {
+    int ix = (int)((pos.x - c.geom.bbmin_x) / c.geom.dx);
+    int iy = (int)((pos.y - c.geom.bbmin_y) / c.geom.dx);
+    return ix * c.geom.ny + iy;
+}
+
+// grid.cu:25-42; also clears the cells the previous step occupied (their starts were the only
+// non -1 entries) and, optionally, counts digit 0 of the new keys per 4096-key tile.
+__global__ void __launch_bounds__(SORT_BS) k_hash(DevConsts c, const double *__restrict__ blanked,
+                                                  const sphfem_double2 *__restrict__ pos,
+                                                  const int *__restrict__ old_hash, int *__restrict__ hash_out,
+                                                  int *__restrict__ cell_start, int clear_old, int N,
+                                                  int num_cell, int *__restrict__ hist0, int nblocks,
+                                                  int *__restrict__ flags) {
+    __shared__ int lh[RADIX];
+    for (int d = threadIdx.x; d < RADIX; d += SORT_BS) lh[d] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * SORT_TILE;
+    for (int it = 0; it < SORT_ITEMS; it++) {
+        int i = base + it * SORT_BS + threadIdx.x;
+        if (i >= N) break;
+        int oh = old_hash[i];
+        if (clear_old && oh >= 0 && oh < num_cell) cell_start[oh] = -1;
+        int h = oh;
+        if (blanked[i] != 1.) h = hash_of(c, pos[i]);
+        if (h < 0 || h >= num_cell) {
+            atomicOr(&flags[0], 1);
+            h = h < 0 ? 0 : num_cell - 1;
+        }
+        hash_out[i] = h;
+        if (hist0) atomicAdd(&lh[h & (RADIX - 1)], 1);
+    }
+    if (hist0) {
+        __syncthreads();
+        for (int d = threadIdx.x; d < RADIX; d += SORT_BS) hist0[d * nblocks + blockIdx.x] = lh[d];
+    }
+}
+
+__global__ void __launch_bounds__(SORT_BS) k_upsweep(const int *__restrict__ keys, int N, int shift,
+                                                     int *__restrict__ hist, int nblocks) {
+    __shared__ int lh[RADIX];
+    for (int d = threadIdx.x; d < RADIX; d += SORT_BS) lh[d] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * SORT_TILE;
+    for (int it = 0; it < SORT_ITEMS; it++) {
+        int i = base + it * SORT_BS + threadIdx.x;
+        if (i >= N) break;
+        atomicAdd(&lh[(keys[i] >> shift) & (RADIX - 1)], 1);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < RADIX; d += SORT_BS) hist[d * nblocks + blockIdx.x] = lh[d];
+}
+
+// one block per digit: exclusive scan of that digit's per-tile counts, total into digit_tot
+__global__ void __launch_bounds__(SORT_BS) k_scan_digits(int *__restrict__ hist, int nblocks,
+                                                         int *__restrict__ digit_tot) {
+    __shared__ int part[SORT_BS];
+    int *row = hist + (long)blockIdx.x * nblocks;
+    int per = (nblocks + SORT_BS - 1) / SORT_BS;
+    int lo = threadIdx.x * per;
+    int hi = min(lo + per, nblocks);
+    int s = 0;
+    for (int j = lo; j < hi; j++) s += row[j];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 256 partials
+    for (int off = 1; off < SORT_BS; off <<= 1) {
+        int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+    for (int j = lo; j < hi; j++) {
+        int v = row[j];
+        row[j] = run;
+        run += v;
+    }
+    if (threadIdx.x == SORT_BS - 1) digit_tot[blockIdx.x] = part[SORT_BS - 1];
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const unsigned lane = __lane_id();
+    return lane ? ((~0ull) >> (64 - lane)) : 0ull;
+}
+
+// stable scatter of one pass; vals_in == nullptr means the values are the input positions
+__global__ void __launch_bounds__(SORT_BS) k_downsweep(const int *__restrict__ keys_in,
+                                                       const int *__restrict__ vals_in,
+                                                       int *__restrict__ keys_out, int *__restrict__ vals_out,
+                                                       int N, int shift, const int *__restrict__ hist,
+                                                       int nblocks, const int *__restrict__ digit_tot,
+                                                       int *__restrict__ flags) {
+    __shared__ int block_off[RADIX];
+    __shared__ int wcount[SORT_BS / WAVE][RADIX];
+    __shared__ int scan[RADIX];
+    const int tid = threadIdx.x;
+    const int w = tid / WAVE;
+    const int lane = tid % WAVE;
+    // digit bases: exclusive scan over the 256 digit totals
+    scan[tid] = digit_tot[tid];
+    for (int k = 0; k < SORT_BS / WAVE; k++) wcount[k][tid] = 0;
+    __syncthreads();
+    for (int off = 1; off < RADIX; off <<= 1) {
+        int v = tid >= off ? scan[tid - off] : 0;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+    }
+    block_off[tid] = (tid ? scan[tid - 1] : 0) + hist[(long)tid * nblocks + blockIdx.x];
+    __syncthreads();
+
+    int key[SORT_ITEMS], val[SORT_ITEMS], rank[SORT_ITEMS];
+    const int base = blockIdx.x * SORT_TILE + w * (SORT_ITEMS * WAVE);
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; r++) {
+        int i = base + r * WAVE + lane;
+        bool valid = i < N;
+        int k = valid ? keys_in[i] : 0;
+        key[r] = k;
+        val[r] = valid ? (vals_in ? vals_in[i] : i) : 0;
+        int d = (k >> shift) & (RADIX - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            uint64_t m = __ballot((d >> b) & 1);
+            peers &= ((d >> b) & 1) ? m : ~m;
+        }
+        int before = __popcll(peers & lt);
+        int cnt = __popcll(peers);
+        int old = 0;
+        if (valid) old = wcount[w][d];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && before == 0) wcount[w][d] = old + cnt;
+        __builtin_amdgcn_wave_barrier();
+        rank[r] = valid ? old + before : -1;
+    }
+    __syncthreads();
+    // prefix of each digit over the waves of this block
+    {
+        int d = tid;
+        int run = 0;
+        for (int k = 0; k < SORT_BS / WAVE; k++) {
+            int v = wcount[k][d];
+            wcount[k][d] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; r++) {
+        if (rank[r] < 0) continue;
+        int d = (key[r] >> shift) & (RADIX - 1);
+        int dst = block_off[d] + wcount[w][d] + rank[r];
+        if ((unsigned)dst >= (unsigned)N) {   // cannot happen for a consistent histogram
+            atomicOr(&flags[0], 2);
+            continue;
+        }
+        keys_out[dst] = key[r];
+        vals_out[dst] = val[r];
+    }
+}
+
+__global__ void k_fill_int(int *p, long n, int v) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+void launch_fill_int(sphfem_ctx *ctx, int *ptr, long n, int value) {
+    if (n <= 0) return;
+    int blocks = (int)std::min<long>(nblk(n, 256), 8192);
+    KTimer t(ctx, "fill_int");
+    k_fill_int<<<blocks, 256, 0, ctx->stream>>>(ptr, n, value);
+}
+
+void launch_hash(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, bool clear_old, int *hist0) {
+    // writes into p.hash (the reference's in-place layout); the fused step passes hist0
+    (void)hist0;
+    int N = (int)p.N;
+    int nb = nblk(N, SORT_TILE);
+    KTimer t(ctx, "hash");
+    k_hash<<<nb, SORT_BS, 0, ctx->stream>>>(ctx->c, p.blanked, p.pos, p.hash, p.hash, g->cell_start,
+                                            clear_old ? 1 : 0, N, g->num_cell, nullptr, nb, ctx->d_flags);
+}
+
+// Radix sort of (p.hash, index) leaving sorted keys in p.hash and the permutation in p.idx.
+// If hash_with_hist0, the hash pass (which computes the keys) is run here with the digit-0
+// histogram fused and writes the keys into the buffer the pass parity requires.
+void launch_radix_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool hash_with_hist0) {
+    const int N = (int)p.N;
+    const int nb = nblk(N, SORT_TILE);
+    const int passes = (g->key_bits + 7) / 8;
+    int *kin, *vin = nullptr;
+    // choose buffers so that the last pass writes p.hash / p.idx
+    int *kbuf[2] = {p.hash, g->keys_alt};
+    int *vbuf[2] = {p.idx, g->vals_alt};
+    int src = (passes % 2 == 1) ? 1 : 0;   // buffer holding the unsorted keys
+    if (hash_with_hist0) {
+        KTimer t(ctx, "hash");
+        k_hash<<<nb, SORT_BS, 0, ctx->stream>>>(ctx->c, p.blanked, p.pos, p.hash, kbuf[src], g->cell_start,
+                                                g->first_sort ? 0 : 1, N, g->num_cell, g->hist, nb,
+                                                ctx->d_flags);
+    } else if (src == 1) {
+        (void)hipMemcpyAsync(g->keys_alt, p.hash, sizeof(int) * (size_t)N, hipMemcpyDeviceToDevice, ctx->stream);
+    }
+    kin = kbuf[src];
+    for (int ps = 0; ps < passes; ps++) {
+        int shift = 8 * ps;
+        int dst = 1 - src;
+        if (!(ps == 0 && hash_with_hist0)) {
+            KTimer t(ctx, "radix_upsweep");
+            k_upsweep<<<nb, SORT_BS, 0, ctx->stream>>>(kin, N, shift, g->hist, nb);
+        }
+        {
+            KTimer t(ctx, "radix_scan");
+            k_scan_digits<<<RADIX, SORT_BS, 0, ctx->stream>>>(g->hist, nb, g->digit_tot);
+        }
+        {
+            KTimer t(ctx, "radix_downsweep");
+            k_downsweep<<<nb, SORT_BS, 0, ctx->stream>>>(kin, ps == 0 ? nullptr : vin, kbuf[dst], vbuf[dst], N,
+                                                         shift, g->hist, nb, g->digit_tot, ctx->d_flags);
+        }
+        kin = kbuf[dst];
+        vin = vbuf[dst];
+        src = dst;
+    }
+}
+
+// grid_gpu_green.cu:13-115: boundaries + gather of the 20 arrays into the buffer
+__global__ void __launch_bounds__(256) k_reorder_full(const sphfem_particles p, sphfem_particles b,
+                                                      int *__restrict__ cell_start, int *__restrict__ cell_end,
+                                                      int num_cell, int *__restrict__ flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int N = (int)p.N;
+    if (i >= N) return;
+    int h = p.hash[i];
+    const int s = p.idx[i];
+    if ((unsigned)h >= (unsigned)num_cell || (unsigned)s >= (unsigned)N) {
+        atomicOr(&flags[0], 4);
+        return;
+    }
+    int hp = __shfl_up(h, 1);
+    if (__lane_id() == 0 && i > 0) hp = p.hash[i - 1];
+    if (i == 0 || h != hp) {
+        cell_start[h] = i;
+        if (i > 0) cell_end[hp] = i;
+    }
+    if (i == N - 1) cell_end[h] = i + 1;
+    b.pos[i] = p.pos[s];
+    b.vel[i] = p.vel[s];
+    b.fc[i] = p.fc[s];
+    b.ft[i] = p.ft[s];
+    b.h[i] = p.h[s];
+    b.n[i] = p.n[s];
+    b.rho[i] = p.rho[s];
+    b.S[i] = p.S[s];
+    b.eps_pl[i] = p.eps_pl[s];
+    b.eps_pl_dot[i] = p.eps_pl_dot[s];
+    b.T[i] = p.T[s];
+    b.fixed[i] = p.fixed[s];
+    b.blanked[i] = p.blanked[s];
+    b.tool_particle[i] = p.tool_particle[s];
+    b.pos_t[i] = p.pos_t[s];
+    b.vel_t[i] = p.vel_t[s];
+    b.rho_t[i] = p.rho_t[s];
+    b.S_t[i] = p.S_t[s];
+    b.T_t[i] = p.T_t[s];
+    b.on_seg[i] = p.on_seg[s];
+}
+
+void launch_reorder_full(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p) {
+    KTimer t(ctx, "reorder");
+    k_reorder_full<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(p, g->buf, g->cell_start, g->cell_end,
+                                                            g->num_cell, ctx->d_flags);
+}
+
+// Fused: boundaries, gather (state and old derivatives), leap_frog init (leap_frog.cu:25-37),
+// predict (:45-69), EOS (actions_gpu.cu:222-232) and artificial stress (:234-278).
+// Reads the unsorted arrays p via idx, writes the sorted predicted state to b.
+__global__ void __launch_bounds__(256) k_reorder_predict(DevConsts c, const sphfem_particles p, sphfem_particles b,
+                                                         int *__restrict__ cell_start, int *__restrict__ cell_end,
+                                                         sphfem_double2 *__restrict__ pos_init,
+                                                         sphfem_double2 *__restrict__ vel_init,
+                                                         sphfem_double4 *__restrict__ S_init,
+                                                         double *__restrict__ rho_init, double *__restrict__ T_init,
+                                                         int num_cell, int *__restrict__ flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int N = (int)p.N;
+    if (i >= N) return;
+    int h = p.hash[i];
+    const int s = p.idx[i];
+    if ((unsigned)h >= (unsigned)num_cell || (unsigned)s >= (unsigned)N) {
+        atomicOr(&flags[0], 4);
+        return;
+    }
+    int hp = __shfl_up(h, 1);
+    if (__lane_id() == 0 && i > 0) hp = p.hash[i - 1];
+    if (i == 0 || h != hp) {
+        cell_start[h] = i;
+        if (i > 0) cell_end[hp] = i;
+    }
+    if (i == N - 1) cell_end[h] = i + 1;
+    sphfem_double2 pos = p.pos[s], vel = p.vel[s];
+    sphfem_double4 S = p.S[s];
+    double rho = p.rho[s], T = p.T[s];
+    double blanked = p.blanked[s], tool = p.tool_particle[s];
+    if (blanked != 1.) {
+        const double dt = c.dt;
+        pos_init[i] = pos;
+        vel_init[i] = vel;
+        S_init[i] = S;
+        rho_init[i] = rho;
+        T_init[i] = T;
+        sphfem_double2 pt = p.pos_t[s], vt = p.vel_t[s];
+        sphfem_double4 St = p.S_t[s];
+        double rt = p.rho_t[s], Tt = p.T_t[s];
+        pos.x = pos.x + 0.5 * dt * pt.x;
+        pos.y = pos.y + 0.5 * dt * pt.y;
+        vel.x = vel.x + 0.5 * dt * vt.x;
+        vel.y = vel.y + 0.5 * dt * vt.y;
+        S.x = S.x + 0.5 * dt * St.x;
+        S.y = S.y + 0.5 * dt * St.y;
+        S.z = S.z + 0.5 * dt * St.z;
+        S.w = S.w + 0.5 * dt * St.w;
+        rho = rho + 0.5 * dt * rt;
+        T = T + 0.5 * dt * Tt;
+    }
+    b.pos[i] = pos;
+    b.vel[i] = vel;
+    b.S[i] = S;
+    b.rho[i] = rho;
+    b.T[i] = T;
+    b.blanked[i] = blanked;
+    b.tool_particle[i] = tool;
+    b.fc[i] = p.fc[s];
+    b.ft[i] = p.ft[s];
+    b.h[i] = p.h[s];
+    b.n[i] = p.n[s];
+    b.eps_pl[i] = p.eps_pl[s];
+    b.eps_pl_dot[i] = p.eps_pl_dot[s];
+    b.fixed[i] = p.fixed[s];
+    b.on_seg[i] = p.on_seg[s];
+    if (tool == 0.) {
+        double pr = eos_pressure(c, rho);
+        b.p[i] = pr;
+        b.R[i] = artificial_stress(c, rho, pr, S);
+    }
+}
+
+void launch_reorder_predict(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, sphfem_leap_frog *lf) {
+    KTimer t(ctx, "reorder_predict");
+    k_reorder_predict<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, g->buf, g->cell_start, g->cell_end,
+                                                               lf->pos_init, lf->vel_init, lf->S_init,
+                                                               lf->rho_init, lf->T_init, g->num_cell,
+                                                               ctx->d_flags);
+}
+
+}  // namespace sphfem

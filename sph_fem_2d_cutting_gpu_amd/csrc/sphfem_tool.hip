@@ -1,0 +1,523 @@
+// Rigid FEM tool per step: transient heat conduction (FEM_thermal_gpu.cu:342-468, 865-921),
+// tool / segment motion (tool_FEM.cu:109-170, 1546-1609), Usui nodal-shift wear
+// (FEM_thermal_gpu.cu:476-577, tool_FEM.cu:299-432, 520-587) and the particle-tool contact
+// (actions_gpu.cu:345-617).  Where the reference scatters with atomics in colour-ordered
+// launches, these kernels gather per destination from lists built once at tool creation, in
+// the same (colour, element/segment) order, so results are deterministic.
+#include "sphfem_physics.h"
+
+#pragma clang fp contract(off)
+
+namespace sphfem {
+
+// ---------------------------------------------------------------------------------------------
+// FEM heat conduction
+// ---------------------------------------------------------------------------------------------
+
+// Robin contribution of segment s to the rhs and to the stiffness diagonal (FEM_thermal_gpu.cu:342-384)
+__device__ __forceinline__ bool robin_term(const DevConsts &c, const sphfem_tool &t, int s, double &df, double &dk) {
+    sphfem_double2 he = t.heat_exchange[s];
+    if (he.y == 0) return false;
+    double T_cont_particle = he.x / he.y;
+    double fric_eng = t.fric_heat[s];
+    sphfem_double2 l = t.left[s], r = t.right[s];
+    double ddx = l.x - r.x, ddy = l.y - r.y;
+    double dist = sqrtf_d(ddx * ddx + ddy * ddy);
+    df = 0.5 * c.tc.h * dist * T_cont_particle + 0.5 * fric_eng;
+    dk = c.tc.h * (0.5 * dist);
+    return true;
+}
+
+// reset_FEM_server's T_old = T and calculate_temp(0.5 dt) for the nodes; zero the wear
+// accumulators (segments_wear_rate_reset) and the contact-list counters.
+__global__ void __launch_bounds__(256) k_tool_begin(sphfem_tool t, double half_dt, int *counters) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        counters[1] = 0;
+        counters[2] = 0;
+    }
+    if (i < t.n_nodes) {
+        double T = t.T[i];
+        t.T_old[i] = T;
+        t.T[i] = half_dt * t.k[i] + T;
+    }
+    if (i < t.n_seg) {
+        t.wear_rate[i] = {0., 0.};
+        t.wear_nodes[i] = 0.;
+        t.physical_para[i] = {0., 0., 0., 0.};
+        t.sliding_force[i] = 0.;
+    }
+}
+
+void launch_tool_begin(sphfem_ctx *ctx, sphfem_tool *t) {
+    int n = std::max(std::max(t->n_nodes, t->n_seg), 1);
+    KTimer tm(ctx, "tool_begin");
+    k_tool_begin<<<nblk(n, 256), 256, 0, ctx->stream>>>(*t, 0.5 * ctx->c.dt, ctx->d_flags);
+}
+
+// k = M^-1 (f + robin - (K + robin_diag) T), one row per thread (FEM_thermal_gpu.cu:443-458 with
+// the Robin terms of :342-384 applied to the copies f_temp / K_temp of reset_FEM_server)
+__global__ void __launch_bounds__(256) k_fem_rate_fused(DevConsts c, sphfem_tool t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= t.n_nodes) return;
+    double f = t.f[i];
+    double kd = t.K[t.diag[i]];
+    int2 rs = t.node_robin[i];
+    double df, dk;
+    if (rs.x >= 0 && robin_term(c, t, rs.x, df, dk)) { f += df; kd += dk; }
+    if (rs.y >= 0 && robin_term(c, t, rs.y, df, dk)) { f += df; kd += dk; }
+    double buffer = 0.;
+    const int dg = t.diag[i];
+    for (int k = t.row_ptr[i]; k < t.row_ptr[i + 1]; k++) {
+        double kv = (k == dg) ? kd : t.K[k];
+        buffer += kv * t.T[t.col_ind[k]];
+    }
+    t.k[i] = t.Minv[i] * (f - buffer);
+}
+
+// calculate_temp (FEM_thermal_gpu.cu:460-468)
+__global__ void __launch_bounds__(256) k_fem_temp(sphfem_tool t, double dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= t.n_nodes) return;
+    t.T[i] = dt * t.k[i] + t.T_old[i];
+}
+
+void launch_tool_fem_end(sphfem_ctx *ctx, sphfem_tool *t) {
+    {
+        KTimer tm(ctx, "fem_rate");
+        k_fem_rate_fused<<<nblk(t->n_nodes, 256), 256, 0, ctx->stream>>>(ctx->c, *t);
+    }
+    KTimer tm(ctx, "fem_temp");
+    k_fem_temp<<<nblk(t->n_nodes, 256), 256, 0, ctx->stream>>>(*t, ctx->c.dt);
+}
+
+// drop-in pieces: reset_FEM_server, apply_robin_bdc, calculate_temp_rate, calculate_temp
+__global__ void k_fem_reset(sphfem_tool t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < t.n_nodes) {
+        t.T_old[i] = t.T[i];
+        t.f_temp[i] = t.f[i];
+    }
+    if (i < t.nnz) t.K_temp[i] = t.K[i];
+}
+
+__global__ void k_fem_robin(DevConsts c, sphfem_tool t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= t.n_nodes) return;
+    int2 rs = t.node_robin[i];
+    double df, dk;
+    if (rs.x >= 0 && robin_term(c, t, rs.x, df, dk)) { t.f_temp[i] += df; t.K_temp[t.diag[i]] += dk; }
+    if (rs.y >= 0 && robin_term(c, t, rs.y, df, dk)) { t.f_temp[i] += df; t.K_temp[t.diag[i]] += dk; }
+}
+
+__global__ void k_fem_rate(sphfem_tool t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= t.n_nodes) return;
+    double buffer = 0.;
+    for (int k = t.row_ptr[i]; k < t.row_ptr[i + 1]; k++) buffer += t.K_temp[k] * t.T[t.col_ind[k]];
+    t.k[i] = t.Minv[i] * (t.f_temp[i] - buffer);
+}
+
+void launch_fem_reset(sphfem_ctx *ctx, sphfem_tool *t) {
+    KTimer tm(ctx, "fem_reset");
+    k_fem_reset<<<nblk(std::max(t->n_nodes, t->nnz), 256), 256, 0, ctx->stream>>>(*t);
+}
+void launch_fem_robin(sphfem_ctx *ctx, sphfem_tool *t) {
+    KTimer tm(ctx, "fem_robin");
+    k_fem_robin<<<nblk(t->n_nodes, 256), 256, 0, ctx->stream>>>(ctx->c, *t);
+}
+void launch_fem_rate(sphfem_ctx *ctx, sphfem_tool *t) {
+    KTimer tm(ctx, "fem_rate_ref");
+    k_fem_rate<<<nblk(t->n_nodes, 256), 256, 0, ctx->stream>>>(*t);
+}
+void launch_fem_temp(sphfem_ctx *ctx, sphfem_tool *t, double dt) {
+    KTimer tm(ctx, "fem_temp");
+    k_fem_temp<<<nblk(t->n_nodes, 256), 256, 0, ctx->stream>>>(*t, dt);
+}
+
+__global__ void k_wear_reset(sphfem_tool t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= t.n_seg) return;
+    t.wear_rate[i] = {0., 0.};
+    t.wear_nodes[i] = 0.;
+    t.physical_para[i] = {0., 0., 0., 0.};
+    t.sliding_force[i] = 0.;
+}
+void launch_wear_reset(sphfem_ctx *ctx, sphfem_tool *t) {
+    KTimer tm(ctx, "wear_reset");
+    k_wear_reset<<<nblk(std::max(t->n_seg, 1), 256), 256, 0, ctx->stream>>>(*t);
+}
+
+// ---------------------------------------------------------------------------------------------
+// tool motion: segments_update (tool_FEM.cu:119-139) and tool_update_FEM_gpu (:109-117, :1552-1570)
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_tool_move(sphfem_tool t, double dt, int do_segments, int do_nodes) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const ToolKin k = *t.kin;
+    if (do_segments && i < t.n_seg) {
+        sphfem_double2 l = t.left[i], r = t.right[i];
+        t.left[i] = {l.x + k.vel_x * dt, l.y + k.vel_y * dt + k.flank_retreat};
+        t.right[i] = {r.x + k.vel_x * dt, r.y + k.vel_y * dt + k.flank_retreat};
+        t.heat_exchange[i] = {0., 0.};
+        t.fric_heat[i] = 0.;
+    }
+    if (do_nodes && i < t.n_nodes) {
+        sphfem_double2 p = t.pos[i];
+        p.x += k.vel_x * dt;
+        p.y += k.vel_y * dt + k.flank_retreat;
+        t.pos[i] = p;
+    }
+    if (do_nodes && i == 0) {
+        ToolKin n = k;
+        n.tl_x += k.vel_x * dt;
+        n.tl_y += k.vel_y * dt;
+        n.br_x += k.vel_x * dt;
+        n.br_y += k.vel_y * dt;
+        n.lowest_point += k.vel_y * dt;
+        n.right_point += k.vel_x * dt;
+        n.flank_contact = n.lowest_point;
+        *t.kin = n;
+    }
+}
+
+void launch_tool_move(sphfem_ctx *ctx, sphfem_tool *t, double dt, bool segments, bool nodes) {
+    int n = std::max(std::max(t->n_nodes, t->n_seg), 1);
+    KTimer tm(ctx, "tool_move");
+    k_tool_move<<<nblk(n, 256), 256, 0, ctx->stream>>>(*t, dt, segments ? 1 : 0, nodes ? 1 : 0);
+}
+
+// ---------------------------------------------------------------------------------------------
+// wear (actions_gpu.cu:1009-1020)
+// ---------------------------------------------------------------------------------------------
+
+// element matrices of FEM_thermal_gpu.cu:71-109, 126-152 (Ke[a][b] = row a, column b)
+__device__ __forceinline__ void element_matrices(const DevConsts &c, const sphfem_tool &t, int e, double *out) {
+    sphfem_int4 el = t.elem[e];
+    sphfem_double2 p0 = t.pos[el.x], p1 = t.pos[el.y], p2 = t.pos[el.z];
+    double j00 = p1.x - p0.x, j01 = p1.y - p0.y;
+    double j10 = p2.x - p0.x, j11 = p2.y - p0.y;
+    double det = fabs(j00 * j11 - j10 * j01);
+    double w = 1. / 3. * 0.5;
+    out[0] = w * c.tc.cp * c.tc.rho * det;
+    double dd = j00 * j11 - j01 * j10;
+    double ood = 1. / dd;
+    double i00 = j11 * ood, i01 = -j10 * ood, i10 = -j01 * ood, i11 = j00 * ood;
+    const double gx[3] = {-1., 1., 0.}, gy[3] = {-1., 0., 1.};
+    double G0[3], G1[3];
+    for (int a = 0; a < 3; a++) {
+        G0[a] = i00 * gx[a] + i10 * gy[a];
+        G1[a] = i01 * gx[a] + i11 * gy[a];
+    }
+    double s = (w * 3.) * (c.tc.k * det);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) out[1 + 3 * a + b] = (s * G0[a]) * G0[b] + (s * G1[a]) * G1[b];
+}
+
+__global__ void k_layer_mats(DevConsts c, sphfem_tool t, double *dst) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= t.n_layer) return;
+    double m[10];
+    element_matrices(c, t, t.layer_elems[j], m);
+    for (int q = 0; q < 10; q++) dst[(long)j * 10 + q] = m[q];
+}
+
+// Usui rake/flank wear depth of segment s (tool_FEM.cu:299-432 with USUI_RAKE_FLANK,
+// RESTRICT_WEAR_TEMP); false if the segment is skipped
+__device__ __forceinline__ bool segment_wear(const DevConsts &c, const sphfem_tool &t, int s, double &wl, double &wr) {
+    sphfem_double4 wear = t.physical_para[s];
+    if (wear.x == 0.) return false;
+    sphfem_int4 mk = t.marks[s];
+    double pressure = wear.y / wear.x;
+    double vel = wear.z / wear.x;
+    double tl = t.T[mk.z], tr = t.T[mk.w];
+    if (vel > 0.008) return false;
+    tl = fmin(tl, 1350.);
+    tr = fmin(tr, 1350.);
+    vel = fmin(vel, 0.008);
+    pressure = fmin(pressure, 0.04);
+    const double dt = c.dt;
+    if (fabs(t.n[s].x) > 0.85) {
+        wl = dt * c.wear.C * pressure * vel * exp(-c.wear.D / tl);
+        wr = dt * c.wear.C * pressure * vel * exp(-c.wear.D / tr);
+    } else {
+        wl = dt * c.wear.A * pressure * vel * exp(-c.wear.B / tl);
+        wr = dt * c.wear.A * pressure * vel * exp(-c.wear.B / tr);
+    }
+    return true;
+}
+
+// nodal shift: each node gathers the shifts of its segments in segment order
+__global__ void k_nodal_shift(DevConsts c, sphfem_tool t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < t.n_seg) {
+        double wl, wr;
+        if (segment_wear(c, t, i, wl, wr)) t.wear_nodes[i] = (wl + wr) / c.dt;
+    }
+    if (i >= t.n_nodes) return;
+    int a = t.node_shift_ptr[i], b = t.node_shift_ptr[i + 1];
+    if (a == b) return;
+    sphfem_double2 p = t.pos[i];
+    for (int q = a; q < b; q++) {
+        int2 e = t.node_shift[q];
+        double wl, wr;
+        if (!segment_wear(c, t, e.x, wl, wr)) continue;
+        double w = e.y ? wr : wl;
+        sphfem_double2 nn = t.n[e.x];
+        p.x += nn.x * w;
+        p.y += nn.y * w;
+    }
+    t.pos[i] = p;
+}
+
+// K, M: subtract the old and add the new boundary-layer element matrices in (colour, element)
+// order (mesh_retreat_wear / mesh_resume_wear), then M^-1 (do_inverse_mass_matrix_lumped)
+__global__ void k_wear_assemble(sphfem_tool t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < t.nnz) {
+        int a = t.nnz_layer_ptr[i], b = t.nnz_layer_ptr[i + 1];
+        if (a != b) {
+            double v = t.K[i];
+            for (int q = a; q < b; q++) {
+                int code = t.nnz_layer[q];
+                v += -1.0 * t.layer_old[(long)(code >> 4) * 10 + 1 + (code & 15)];
+            }
+            for (int q = a; q < b; q++) {
+                int code = t.nnz_layer[q];
+                v += 1.0 * t.layer_new[(long)(code >> 4) * 10 + 1 + (code & 15)];
+            }
+            t.K[i] = v;
+        }
+    }
+    if (i < t.n_nodes) {
+        int a = t.node_layer_ptr[i], b = t.node_layer_ptr[i + 1];
+        double m = t.M[i];
+        if (a != b) {
+            for (int q = a; q < b; q++) m += -1.0 * t.layer_old[(long)(t.node_layer[q] >> 2) * 10];
+            for (int q = a; q < b; q++) m += 1.0 * t.layer_new[(long)(t.node_layer[q] >> 2) * 10];
+            t.M[i] = m;
+        }
+        t.Minv[i] = 1. / m;
+    }
+}
+
+// segment_wear_update (tool_FEM.cu:520-548) and correct_flank_contact (:559-587) for any number of
+// segments: one block, min over left.y
+__global__ void __launch_bounds__(1024) k_segment_wear_update(sphfem_tool t) {
+    __shared__ double red[1024];
+    double mn = SPHFEM_FLT_MAX;
+    for (int s = threadIdx.x; s < t.n_seg; s += blockDim.x) {
+        sphfem_int4 mk = t.marks[s];
+        sphfem_double2 lp = t.pos[mk.z], rp = t.pos[mk.w];
+        t.left[s] = lp;
+        t.right[s] = rp;
+        double vx = rp.y - lp.y, vy = -(rp.x - lp.x);
+        double inv = 1. / sqrt(vx * vx + vy * vy);
+        t.n[s] = {vx * inv, vy * inv};
+        mn = fmin(mn, lp.y);
+    }
+    red[threadIdx.x] = mn;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) t.kin->flank_retreat = t.kin->flank_contact - red[0];
+}
+
+void launch_wear(sphfem_ctx *ctx, sphfem_tool *t) {
+    KTimer tm(ctx, "wear");
+    const int nl = std::max(t->n_layer, 1);
+    k_layer_mats<<<nblk(nl, 128), 128, 0, ctx->stream>>>(ctx->c, *t, t->layer_old);
+    k_nodal_shift<<<nblk(std::max(t->n_nodes, t->n_seg), 256), 256, 0, ctx->stream>>>(ctx->c, *t);
+    k_layer_mats<<<nblk(nl, 128), 128, 0, ctx->stream>>>(ctx->c, *t, t->layer_new);
+    k_wear_assemble<<<nblk(std::max(t->nnz, t->n_nodes), 256), 256, 0, ctx->stream>>>(*t);
+    k_segment_wear_update<<<1, 1024, 0, ctx->stream>>>(*t);
+}
+
+// ---------------------------------------------------------------------------------------------
+// contact
+// ---------------------------------------------------------------------------------------------
+
+// material_fric_heat_gen_FEM_v2 (actions_gpu.cu:345-420, 967-981) over all particles
+__global__ void __launch_bounds__(256) k_fric_heat(DevConsts c, sphfem_particles P, sphfem_tool t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int)P.N) return;
+    if (P.blanked[i] == 1.) return;
+    if (P.eps_pl[i] > 20.) return;
+    sphfem_double2 ft = P.ft[i];
+    double f_fric_mag = sqrtf_d(ft.x * ft.x + ft.y * ft.y);
+    int seg = P.on_seg[i];
+    if (seg == -1) return;
+    const ToolKin k = *t.kin;
+    const double partition_wp = c.tc.beta, dt = c.dt;
+    sphfem_double2 normal = P.n[i], vp = P.vel[i];
+    double vdx = vp.x - k.vel_x, vdy = vp.y - k.vel_y;
+    double vdn = vdx * normal.x + vdy * normal.y;
+    double vrx = vdx - vdn * normal.x, vry = vdy - vdn * normal.y;
+    double v_rel_mag = sqrtf_d(vrx * vrx + vry * vry);
+    double value = c.tc.eta * f_fric_mag * v_rel_mag;
+    if (isnan(v_rel_mag)) {
+        P.blanked[i] = 1.;
+        return;
+    }
+    sphfem_double2 fcv = P.fc[i];
+    double f_normal = sqrtf_d(fcv.x * fcv.x + fcv.y * fcv.y);
+    double length = sqrtf_d(c.tc.slave_mass / P.rho[i]);
+    double pressure_trial = f_normal / length;
+    if (v_rel_mag > 0.03) return;
+    double T_buffer = P.T[i] + value * (1 - partition_wp) * dt / c.trml_wp.cp / c.phys.mass;
+    if (T_buffer > 1550.) {
+        P.T[i] = 1550.;
+        return;
+    }
+    P.T[i] = T_buffer;
+    unsafeAtomicAdd(&t.fric_heat[seg], value * partition_wp);
+    sphfem_int4 mk = t.marks[seg];
+    double T_seg = 0.5 * (t.T[mk.z] + t.T[mk.w]);
+    double vel_tool_norm = sqrt(k.vel_x * k.vel_x + k.vel_y * k.vel_y);
+    if (v_rel_mag > 0.008) return;
+    else if (v_rel_mag > vel_tool_norm * 8.0 && fabs(normal.y) > 0.5) return;
+    unsafeAtomicAdd(&t.physical_para[seg].x, 1.);
+    unsafeAtomicAdd(&t.physical_para[seg].y, pressure_trial);
+    unsafeAtomicAdd(&t.physical_para[seg].z, v_rel_mag);
+    unsafeAtomicAdd(&t.physical_para[seg].w, T_seg);
+    unsafeAtomicAdd(&t.sliding_force[seg], f_fric_mag / length);
+}
+
+void launch_fric_heat(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t) {
+    KTimer tm(ctx, "fric_heat");
+    k_fric_heat<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, *t);
+}
+
+// compute_contact_forces_FEM_tool_v3 step 1 (actions_gpu.cu:423-583) over all particles
+__global__ void __launch_bounds__(256) k_contact_s1(DevConsts c, sphfem_particles P, sphfem_tool t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int)P.N) return;
+    if (P.blanked[i] == 1.) return;
+    const ToolKin k = *t.kin;
+    sphfem_double2 pos = P.pos[i];
+    sphfem_double2 fc, ft = P.ft[i], nrm;
+    int seg = -1;
+    bool blank = false;
+    if (near_tool(k, pos)) {
+        seg = contact_s1_near(c, k, pos, P.vel[i], fc, ft, nrm, blank, t.left, t.right, t.n, t.n_seg);
+    } else {
+        fc = {0., 0.};
+        ft = {0., 0.};
+        nrm = {0., 0.};
+    }
+    if (blank) P.blanked[i] = 1.;
+    P.fc[i] = fc;
+    P.ft[i] = ft;
+    P.n[i] = nrm;
+    P.on_seg[i] = seg;
+}
+
+// step 2 (actions_gpu.cu:585-617) for one particle
+__device__ __forceinline__ void contact_s2_one(const DevConsts &c, const sphfem_tool &t, int seg_no, double rho,
+                                               double &T) {
+    sphfem_int4 mk = t.marks[seg_no];
+    double T_ = T;
+    double T_seg = 0.5 * (t.T[mk.z] + t.T[mk.w]);
+    double delta_T = T_seg - T_;
+    double length = sqrtf_d(c.tc.slave_mass / rho);
+    unsafeAtomicAdd(&t.heat_exchange[seg_no].x, T_ * length);
+    unsafeAtomicAdd(&t.heat_exchange[seg_no].y, length);
+    double dTp = delta_T * c.tc.h * length / c.trml_wp.cp / c.tc.slave_mass * c.dt;
+    if (dTp < 0) T += fmax(dTp, delta_T);
+    else T += fmin(dTp, delta_T);
+}
+
+__global__ void __launch_bounds__(256) k_contact_s2(DevConsts c, sphfem_particles P, sphfem_tool t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int)P.N) return;
+    if (P.blanked[i] == 1.) return;
+    int seg_no = P.on_seg[i];
+    if (seg_no == -1) return;
+    double T = P.T[i];
+    contact_s2_one(c, t, seg_no, P.rho[i], T);
+    P.T[i] = T;
+}
+
+void launch_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t) {
+    {
+        KTimer tm(ctx, "contact_s1");
+        k_contact_s1<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, *t);
+    }
+    KTimer tm(ctx, "contact_s2");
+    k_contact_s2<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, *t);
+}
+
+// Near-tool particles of the fused step: contact s1, mechanical boundary condition, NaN check;
+// particles left in contact go to the contact list (bit 30 = invalidate after s2), the others
+// also get the thermal boundary condition and their invalidation here.
+__global__ void __launch_bounds__(256) k_near_contact(DevConsts c, sphfem_particles P, sphfem_tool t,
+                                                      const int *__restrict__ near_list, int *counters,
+                                                      int *contact_list) {
+    const int count = counters[1];
+    const ToolKin k = *t.kin;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+        const int i = near_list[q];
+        sphfem_double2 fc, ft = P.ft[i], nrm;
+        bool blank = false;
+        int seg = contact_s1_near(c, k, P.pos[i], P.vel[i], fc, ft, nrm, blank, t.left, t.right, t.n, t.n_seg);
+        double blanked = blank ? 1. : P.blanked[i];
+        sphfem_double2 pos_t = P.pos_t[i], vel_t = P.vel_t[i];
+        if (P.tool_particle[i] == 0. && P.fixed[i] != 0.) {
+            sphfem_double2 z = {0., 0.};
+            P.vel[i] = z;
+            fc = z;
+            pos_t = z;
+            vel_t = z;
+            P.pos_t[i] = z;
+            P.vel_t[i] = z;
+        }
+        P.fc[i] = fc;
+        P.ft[i] = ft;
+        P.n[i] = nrm;
+        P.on_seg[i] = seg;
+        bool bad = false;
+        if (blanked != 1.) {
+            sphfem_double4 st = P.S_t[i];
+            bad = isnaninf(pos_t.x) || isnaninf(pos_t.y) || isnaninf(vel_t.x) || isnaninf(vel_t.y) ||
+                  isnaninf(st.x) || isnaninf(st.y) || isnaninf(st.z) || isnaninf(st.w) || isnaninf(P.rho_t[i]) ||
+                  isnaninf(P.T_t[i]);
+        }
+        if (seg != -1) {
+            int slot = atomicAdd(&counters[2], 1);
+            contact_list[slot] = i | (bad ? (1 << 30) : 0);
+        } else {
+            if (c.thermal_bc && P.fixed[i] == 1.) P.T[i] = c.trml_wp.T_init;
+            if (bad) blanked = 1.;
+        }
+        if (blanked == 1.) P.blanked[i] = 1.;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_contact_list_s2(DevConsts c, sphfem_particles P, sphfem_tool t,
+                                                         const int *counters, const int *contact_list) {
+    const int count = counters[2];
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+        const int code = contact_list[q];
+        const int i = code & ((1 << 30) - 1);
+        double T = P.T[i];
+        contact_s2_one(c, t, P.on_seg[i], P.rho[i], T);
+        if (c.thermal_bc && P.fixed[i] == 1.) T = c.trml_wp.T_init;
+        P.T[i] = T;
+        if (code & (1 << 30)) P.blanked[i] = 1.;
+    }
+}
+
+void launch_near_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t, sphfem_leap_frog *lf) {
+    int blocks = std::max(1, std::min(nblk(p.N, 256), 1024));
+    {
+        KTimer tm(ctx, "near_contact");
+        k_near_contact<<<blocks, 256, 0, ctx->stream>>>(ctx->c, p, *t, lf->near_list, ctx->d_flags,
+                                                        lf->contact_list);
+    }
+    KTimer tm(ctx, "contact_list_s2");
+    k_contact_list_s2<<<std::max(1, std::min(blocks, 64)), 256, 0, ctx->stream>>>(ctx->c, p, *t, ctx->d_flags,
+                                                                                 lf->contact_list);
+}
+
+}  // namespace sphfem

@@ -1,0 +1,364 @@
+"""Host-side mirror of the reference's interface for the per-step path, over the C ABI.
+
+Class and function names follow the reference (particle_gpu, grid_gpu_green, tool_FEM,
+leap_frog, material_eos, interactions_monaghan, ..., report_force) so a caller of the
+reference finds the same entry points; each call goes straight to libsphfem_hip.
+`Simulation` wires a generated workload (cases.Case) the way the reference's setup
+(benchmarks_NZ.cpp:51-367) and driver (mfree_gpu.cpp:229-366) do.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from . import cases
+
+# trailing shapes and dtypes of the particle arrays (particle_gpu.h:23-83)
+_FIELD_SPEC = {name: (shp, dt) for name, shp, dt in cases.PARTICLE_FIELDS}
+
+
+class Context:
+    """A device context: HIP stream plus the constant blocks (replaces the per-file __constant__ state)."""
+
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        check(lib().sphfem_create(device, C.byref(self.h)), None, "sphfem_create")
+
+    def close(self):
+        if self.h:
+            lib().sphfem_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def call(self, name, *args):
+        check(getattr(lib(), name)(self.h, *args), self.h, name)
+
+    def synchronize(self):
+        self.call("sphfem_synchronize")
+
+    def check_error(self):
+        self.call("sphfem_check_error")
+
+    def setup_constants(self, const: cases.Constants, dt: float, rr2_min: float, wear: bool):
+        """actions_setup_* / interactions_setup_* of benchmarks_NZ.cpp:341-356."""
+        L = _lib
+        self.call("sphfem_setup_physical_constants", C.byref(L.PhysConstants(**const.phys)))
+        self.call("sphfem_setup_corrector_constants", C.byref(L.CorrConstants(**const.corr)))
+        self.call("sphfem_setup_johnson_cook_constants", C.byref(L.JocoConstants(**const.joco)))
+        self.call("sphfem_setup_thermal_constants_wp", C.byref(L.TrmlConstants(**const.trml_wp)))
+        self.call("sphfem_setup_thermal_constants_tool", C.byref(L.TrmlConstants(**const.trml_tool)))
+        self.call("sphfem_setup_tool_constants", C.byref(L.ToolConstants(**const.tool)))
+        self.call("sphfem_setup_wear_constants", C.byref(L.WearConstants(**const.wear)))
+        self.call("sphfem_set_dt", float(dt))
+        self.call("sphfem_set_kernel_cutoff", float(rr2_min))
+        self.call("sphfem_set_wear", int(bool(wear)))
+
+    def htod(self, dptr, arr: np.ndarray):
+        a = np.ascontiguousarray(arr)
+        self.call("sphfem_memcpy_htod", C.c_void_p(dptr), a.ctypes.data_as(C.c_void_p), a.nbytes)
+
+    def dtoh(self, dptr, shape, dtype) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        self.call("sphfem_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), C.c_void_p(dptr), out.nbytes)
+        return out
+
+    # measurement
+    def timing(self, enable: bool):
+        self.call("sphfem_timing_enable", int(enable))
+
+    def timing_reset(self):
+        self.call("sphfem_timing_reset")
+
+    def timing_get(self, name: str):
+        ms, n = C.c_double(), C.c_long()
+        self.call("sphfem_timing_get", name.encode(), C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+
+class particle_gpu:
+    """particle_gpu.h:23-83; constructor as particle_gpu(pos, vel, rho, T, h, fixed, N)."""
+
+    def __init__(self, ctx: Context, N: int, **fields):
+        self.ctx = ctx
+        self.s = _lib.Particles()
+        ctx.call("sphfem_particles_alloc", int(N), C.byref(self.s))
+        self.N = int(N)
+        for k, v in fields.items():
+            if v is not None:
+                self.upload(k, v)
+
+    def ptr(self, field):
+        return getattr(self.s, field)
+
+    def upload(self, field, arr):
+        shp, dt = _FIELD_SPEC[field]
+        a = np.asarray(arr, dtype=dt).reshape((self.N,) + shp)
+        self.ctx.htod(self.ptr(field), a)
+
+    def download(self, field) -> np.ndarray:
+        shp, dt = _FIELD_SPEC[field]
+        return self.ctx.dtoh(self.ptr(field), (self.N,) + shp, dt)
+
+    def state(self, fields=None) -> dict:
+        return {f: self.download(f) for f in (fields or _FIELD_SPEC)}
+
+    def free(self):
+        if self.s.pos:
+            self.ctx.call("sphfem_particles_free", C.byref(self.s))
+
+
+class grid_gpu_green:
+    """grid_gpu_green(num_part, bbmin, bbmax, h): locked Green grid (grid.cu:130-149)."""
+
+    def __init__(self, ctx: Context, num_part: int, bbmin, bbmax, h: float):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx.call("sphfem_grid_green_create", int(num_part), _lib.Double2(*map(float, bbmin)),
+                 _lib.Double2(*map(float, bbmax)), float(h), C.byref(self.h))
+        g = _lib.GeomConstants()
+        nc = C.c_int()
+        check(lib().sphfem_grid_geometry(self.h, C.byref(g), C.byref(nc)), None, "grid_geometry")
+        self.geom = g
+        self.num_cell = nc.value
+
+    def nx(self):
+        return self.geom.nx
+
+    def ny(self):
+        return self.geom.ny
+
+    def assign_hashes(self, p: particle_gpu):
+        self.ctx.call("sphfem_grid_assign_hashes", self.h, C.byref(p.s))
+
+    def sort(self, p: particle_gpu):
+        self.ctx.call("sphfem_grid_sort", self.h, C.byref(p.s))
+
+    def get_cells(self, cell_start_dptr, cell_end_dptr):
+        self.ctx.call("sphfem_grid_get_cells", self.h, C.c_void_p(cell_start_dptr), C.c_void_p(cell_end_dptr))
+
+    def download_cells(self):
+        a, b = C.c_void_p(), C.c_void_p()
+        self.ctx.call("sphfem_malloc", C.byref(a), 4 * self.num_cell)
+        self.ctx.call("sphfem_malloc", C.byref(b), 4 * self.num_cell)
+        try:
+            self.get_cells(a.value, b.value)
+            cs = self.ctx.dtoh(a.value, (self.num_cell,), np.int32)
+            ce = self.ctx.dtoh(b.value, (self.num_cell,), np.int32)
+        finally:
+            self.ctx.call("sphfem_free", a)
+            self.ctx.call("sphfem_free", b)
+        return cs, ce
+
+
+def _p(a, dt):
+    a = np.ascontiguousarray(a, dtype=dt)
+    return a, a.ctypes.data_as(C.c_void_p)
+
+
+class tool_FEM:
+    """Rigid FEM tool with its thermal solver and cutting-edge segments (tool_FEM.h:34-83)."""
+
+    def __init__(self, ctx: Context, case: cases.Case):
+        self.ctx = ctx
+        tm = case.tool
+        init = cases.initial_tool(case)
+        keep = []
+
+        def arr(a, dt):
+            a2, ptr = _p(a, dt)
+            keep.append(a2)
+            return ptr
+
+        d = _lib.ToolDesc()
+        d.node_num, d.ele_num, d.nz_num = len(tm.nodes), len(tm.elems), len(tm.col_ind)
+        d.seg_num, d.color_total, d.color_cutting_edge = len(tm.seg_left), tm.n_colors, tm.n_cut_colors
+        d.position = arr(init["node_pos"], np.float64)
+        d.node_num_el = arr(init["elem"], np.int32)
+        d.layer = arr(init["layer"], np.int32)
+        d.csr_row_ptr = arr(init["row_ptr"], np.int32)
+        d.csr_col_ind = arr(init["col_ind"], np.int32)
+        d.K = arr(init["K"], np.float64)
+        d.M = arr(init["M"], np.float64)
+        d.f = arr(init["f"], np.float64)
+        d.T = arr(init["T"], np.float64)
+        d.left = arr(init["left"], np.float64)
+        d.right = arr(init["right"], np.float64)
+        d.n = arr(init["n"], np.float64)
+        d.marks = arr(init["marks"], np.int32)
+        sc = init["scalars"]
+        d.vel = _lib.Double2(sc["vel_x"], sc["vel_y"])
+        d.tl_point_ref = _lib.Double2(sc["tl_x"], sc["tl_y"])
+        d.br_point = _lib.Double2(sc["br_x"], sc["br_y"])
+        d.ref_dist, d.lowest_point, d.right_point = sc["ref_dist"], sc["lowest_point"], sc["right_point"]
+        self.h = C.c_void_p()
+        ctx.call("sphfem_tool_create", C.byref(d), C.byref(self.h))
+        self.n_nodes, self.n_seg, self.nnz = d.node_num, d.seg_num, d.nz_num
+
+    def get_vel(self):
+        k = self.kinematics()
+        return np.array([k["vel_x"], k["vel_y"]])
+
+    def set_vel(self, v):
+        self.ctx.call("sphfem_tool_set_vel", self.h, _lib.Double2(float(v[0]), float(v[1])))
+
+    def kinematics(self) -> dict:
+        k = _lib.ToolKinematics()
+        self.ctx.call("sphfem_tool_get_kinematics", self.h, C.byref(k))
+        return {f: getattr(k, f) for f, _ in _lib.ToolKinematics._fields_}
+
+    def segments_update(self, dt):
+        self.ctx.call("sphfem_tool_segments_update", self.h, float(dt))
+
+    def tool_update_FEM_gpu(self, dt):
+        self.ctx.call("sphfem_tool_update_FEM_gpu", self.h, float(dt))
+
+    def segments_wear_rate_reset(self):
+        self.ctx.call("sphfem_tool_segments_wear_rate_reset", self.h)
+
+    def apply_heat_boundary_condition(self):
+        self.ctx.call("sphfem_tool_apply_heat_boundary_condition", self.h)
+
+    def reset_FEM_server(self):
+        self.ctx.call("sphfem_FEM_reset_server", self.h)
+
+    def calculate_temp(self, dt):
+        self.ctx.call("sphfem_FEM_calculate_temp", self.h, float(dt))
+
+    def calculate_temp_rate(self):
+        self.ctx.call("sphfem_FEM_calculate_temp_rate", self.h)
+
+    _STATE = ("node_pos", "T", "K", "M", "k", "left", "right", "n", "heat_exchange", "fric_heat",
+              "physical_para", "sliding_force", "wear_nodes")
+
+    def upload(self, state: dict):
+        """Restore tool arrays (keys as download(); missing keys are left unchanged) and kinematics."""
+        keep, ptrs = [], []
+        for k in self._STATE:
+            if k in state and state[k] is not None:
+                a = np.ascontiguousarray(state[k], dtype=np.float64)
+                keep.append(a)
+                ptrs.append(a.ctypes.data_as(C.c_void_p))
+            else:
+                ptrs.append(None)
+        self.ctx.call("sphfem_tool_upload", self.h, *ptrs)
+        if "scalars" in state:
+            kin = _lib.ToolKinematics(**{f: float(state["scalars"][f]) for f, _ in _lib.ToolKinematics._fields_})
+            self.ctx.call("sphfem_tool_set_kinematics", self.h, C.byref(kin))
+
+    def download(self) -> dict:
+        nn, ns, nz = self.n_nodes, self.n_seg, self.nnz
+        out = dict(node_pos=np.empty((nn, 2)), T=np.empty(nn), K=np.empty(nz), M=np.empty(nn), k=np.empty(nn),
+                   left=np.empty((ns, 2)), right=np.empty((ns, 2)), n=np.empty((ns, 2)),
+                   heat_exchange=np.empty((ns, 2)), fric_heat=np.empty(ns), physical_para=np.empty((ns, 4)),
+                   sliding_force=np.empty(ns), wear_nodes=np.empty(ns))
+        order = ("node_pos", "T", "K", "M", "k", "left", "right", "n", "heat_exchange", "fric_heat",
+                 "physical_para", "sliding_force", "wear_nodes")
+        self.ctx.call("sphfem_tool_download", self.h, *[out[k].ctypes.data_as(C.c_void_p) for k in order])
+        out["scalars"] = self.kinematics()
+        return out
+
+
+class leap_frog:
+    """leap_frog(num_part, num_cell, node_num) and step(particles, grid, tool) (leap_frog.h:46-49)."""
+
+    def __init__(self, ctx: Context, num_part: int, num_cell: int, node_num: int = 0, precision=_lib.FP64):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx.call("sphfem_leap_frog_create", int(num_part), int(num_cell), int(node_num), C.byref(self.h))
+        ctx.call("sphfem_leap_frog_set_precision", self.h, int(precision))
+
+    def step(self, p: particle_gpu, g: grid_gpu_green, tool: tool_FEM | None = None):
+        self.ctx.call("sphfem_leap_frog_step", self.h, C.byref(p.s), g.h, tool.h if tool else None)
+
+    def init(self, p):
+        self.ctx.call("sphfem_leap_frog_init", self.h, C.byref(p.s))
+
+    def predict(self, p, dt):
+        self.ctx.call("sphfem_leap_frog_predict", self.h, C.byref(p.s), float(dt))
+
+    def correct(self, p, dt):
+        self.ctx.call("sphfem_leap_frog_correct", self.h, C.byref(p.s), float(dt))
+
+
+# ---- actions_gpu.h:33-72 / interactions_gpu.h:27-43 / output.h
+def _act(name):
+    def f(p: particle_gpu):
+        p.ctx.call("sphfem_" + name, C.byref(p.s))
+    f.__name__ = name
+    return f
+
+
+material_eos = _act("material_eos")
+corrector_artificial_stress = _act("corrector_artificial_stress")
+material_stress_rate_jaumann = _act("material_stress_rate_jaumann")
+contmech_continuity = _act("contmech_continuity")
+contmech_momentum = _act("contmech_momentum")
+contmech_advection = _act("contmech_advection")
+plasticity_johnson_cook = _act("plasticity_johnson_cook")
+perform_boundary_conditions = _act("perform_boundary_conditions")
+perform_boundary_conditions_thermal = _act("perform_boundary_conditions_thermal")
+debug_invalidate = _act("debug_invalidate")
+
+
+def material_fric_heat_gen_FEM_v2(p: particle_gpu, tool: tool_FEM):
+    p.ctx.call("sphfem_material_fric_heat_gen_FEM_v2", C.byref(p.s), tool.h)
+
+
+def compute_contact_forces_FEM_tool_v3(p: particle_gpu, tool: tool_FEM):
+    p.ctx.call("sphfem_compute_contact_forces_FEM_tool_v3", C.byref(p.s), tool.h)
+
+
+def wear_adjustment(tool: tool_FEM):
+    tool.ctx.call("sphfem_wear_adjustment", tool.h)
+
+
+def interactions_setup_geometry_constants(g: grid_gpu_green):
+    g.ctx.call("sphfem_interactions_setup_geometry_constants", g.h)
+
+
+def interactions_monaghan(p: particle_gpu, cell_start_dptr, cell_end_dptr, num_cell):
+    p.ctx.call("sphfem_interactions_monaghan", C.byref(p.s), C.c_void_p(cell_start_dptr),
+               C.c_void_p(cell_end_dptr), int(num_cell))
+
+
+def report_force(p: particle_gpu):
+    out = _lib.Double2()
+    p.ctx.call("sphfem_report_force", C.byref(p.s), C.byref(out))
+    return np.array([out.x, out.y])
+
+
+class Simulation:
+    """A workload on the device: constants, particles, locked grid, FEM tool and stepper."""
+
+    def __init__(self, case: cases.Case, precision=_lib.FP64, device=0, rr2_min=None, wear=None, ctx=None):
+        self.case = case
+        self.ctx = ctx or Context(device)
+        rr2 = cases.default_rr2_min(case.dx) if rr2_min is None else rr2_min
+        self.ctx.setup_constants(case.const, case.dt, rr2, case.wear if wear is None else wear)
+        init = cases.initial_particles(case)
+        self.p = particle_gpu(self.ctx, case.N, **{k: init[k] for k in ("pos", "vel", "rho", "T", "h", "fixed", "S")})
+        self.grid = grid_gpu_green(self.ctx, case.N, case.grid_bbmin, case.grid_bbmax, case.h)
+        self.tool = tool_FEM(self.ctx, case) if case.tool is not None else None
+        self.stepper = leap_frog(self.ctx, case.N, self.grid.num_cell,
+                                 self.tool.n_nodes if self.tool else 0, precision)
+
+    def step(self, n: int = 1):
+        for _ in range(n):
+            self.stepper.step(self.p, self.grid, self.tool)
+
+    def synchronize(self):
+        self.ctx.synchronize()
+
+    def state(self, fields=None):
+        return self.p.state(fields)
+
+    def tool_state(self):
+        return self.tool.download() if self.tool else None
