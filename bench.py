@@ -1,0 +1,135 @@
+"""Benchmark: particle updates per second of the cutting step on one MI355X.
+
+Workload (default): the 4M-particle Ti6Al4V case -- 4x1 mm workpiece on a 4000x1000 jittered
+lattice (dx = 1 um), a ~50k-triangle rigid FEM tool with transient heat conduction and Usui
+nodal-shift wear, 70 m/min.  The step is the full leap-frog step (cell list, pair sums,
+corrector, friction heat, plasticity, FEM tool heat, wear, tool motion, contact, boundary
+conditions), inputs resident in HBM.
+
+Prints one JSON line with the metric, a roofline object for the interaction kernel (HIP
+events on the library's stream inside the timed region) and a CPU baseline (the FP64 host
+oracle on a bounded sample of the same workload, on this host's cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+INTERACTION_KERNEL = {"fp64": "fused_update", "fp32c": "fused_update_fp32c"}
+
+
+def interaction_bytes(N, n_occ_cells, w):
+    """Algorithmic bytes of the fused pair-sum stage: 14 input and 10 output scalars of w bytes,
+    2 flag bytes per particle, 8 bytes per occupied cell."""
+    return N * (24 * w + 2) + 8 * n_occ_cells
+
+
+def make_case(name):
+    from sph_fem_2d_cutting_gpu_amd import cases
+    if name == "ti64_4m":
+        return cases.config3(), "ti64_4M_cutting: 4000x1000 lattice dx=1um, ~50k-triangle FEM tool, FEM heat + Usui wear"
+    if name == "aisi1045_160k":
+        return cases.config2(), "aisi1045_160k_cutting: 800x200 lattice dx=1.25um, ~8k-triangle FEM tool"
+    if name == "ti64_6400":
+        return cases.config1(), "ti64_6400_cutting: 160x40 lattice dx=2.5um, ~2k-triangle FEM tool"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(case, budget_s):
+    from oracle.oracle import Oracle, lib
+    o = Oracle(case)
+    threads = lib().o_num_threads()
+    t0 = time.perf_counter()
+    o.step(1)
+    t1 = time.perf_counter() - t0
+    n = int(max(1, min(20, budget_s / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    o.step(n)
+    dt = time.perf_counter() - t0
+    return {"value": case.N * n / dt, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{n} steps (after 1 warm-up step) of the same workload on the FP64 host oracle, "
+                      f"{threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="ti64_4m")
+    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (the solver has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from sph_fem_2d_cutting_gpu_amd import _lib
+    from sph_fem_2d_cutting_gpu_amd import solver as S
+
+    case, workload = make_case(args.workload)
+    prec = {"fp64": _lib.FP64}[args.precision]
+    sim = S.Simulation(case, precision=prec)
+    ctx = sim.ctx
+    for _ in range(args.warmup):
+        sim.step(1)
+    ctx.synchronize()
+    ctx.check_error()
+
+    kname = INTERACTION_KERNEL[args.precision]
+    ctx.timing_select(kname)
+    ctx.timing(True)
+    ctx.timing_reset()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.timing(False)
+    ctx.check_error()
+    k_ms, k_n = ctx.timing_get(kname)
+
+    cs, _ = sim.grid.download_cells()
+    n_occ = int(np.count_nonzero(cs != -1))
+    w = 8 if args.precision == "fp64" else 4
+    b_int = interaction_bytes(case.N, n_occ, w)
+    k_avg_s = (k_ms / 1e3) / max(k_n, 1)
+    achieved = b_int / k_avg_s / 1e9
+
+    ups = case.N * args.steps / elapsed
+    out = {
+        "metric": "particle-updates/s at 4M particles, FP32-comp vs FP64; % HBM roofline",
+        "value": ups,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded jittered lattice and generated tool mesh)",
+        "config": {"workload": workload, "particles": case.N, "tool_triangles": int(len(case.tool.elems)),
+                   "cutting_segments": int(len(case.tool.seg_left)), "wear": bool(case.wear),
+                   "precision": args.precision, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": kname, "kernel_ms": k_avg_s * 1e3, "algorithmic_bytes": b_int,
+                     "occupied_cells": n_occ},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_budget)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

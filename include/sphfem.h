@@ -192,8 +192,6 @@ int sphfem_leap_frog_correct(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_parti
 /* one full step (fused kernels); tool may be NULL for a tool-free workload */
 int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,
                           sphfem_tool *t);
-/* the cell arrays the last step used (leap_frog.h:35-36), device pointers [num_cell] */
-int sphfem_leap_frog_cells(sphfem_leap_frog *lf, const int **cell_start, const int **cell_end);
 
 /* ---- observables (output.cu:22-30) */
 int sphfem_report_force(sphfem_ctx *ctx, sphfem_particles *p, sphfem_double2 *out);
@@ -201,6 +199,8 @@ int sphfem_report_force(sphfem_ctx *ctx, sphfem_particles *p, sphfem_double2 *ou
 /* ---- measurement: per-kernel HIP-event timing on the context stream */
 int sphfem_timing_enable(sphfem_ctx *ctx, int enable);
 int sphfem_timing_reset(sphfem_ctx *ctx);
+/* restrict timing to a comma-separated list of kernel names (NULL or "" = all kernels) */
+int sphfem_timing_select(sphfem_ctx *ctx, const char *names);
 /* accumulated milliseconds and launch count of one kernel (names listed by sphfem_timing_names) */
 int sphfem_timing_get(sphfem_ctx *ctx, const char *kernel, double *total_ms, long *launches);
 const char *sphfem_timing_names(sphfem_ctx *ctx);

@@ -117,6 +117,7 @@ _SIGS = {
     "sphfem_report_force": (_ctx, _P(Particles), _P(Double2)),
     "sphfem_timing_enable": (_ctx, C.c_int),
     "sphfem_timing_reset": (_ctx,),
+    "sphfem_timing_select": (_ctx, C.c_char_p),
     "sphfem_timing_get": (_ctx, C.c_char_p, _P(_dbl), _P(C.c_long)),
 }
 for _name in ("material_eos", "corrector_artificial_stress", "material_stress_rate_jaumann", "contmech_continuity",

@@ -260,15 +260,23 @@ def _normalize_glm(v):
 
 
 def mesh_tool(length, height, rake_deg, clear_deg, r, target_tris, tc) -> ToolMesh:
-    """Graded triangle mesh of the tool with the FEM set-up of FEM_thermal_gpu.cu:584-603."""
-    from scipy.spatial import Delaunay
+    """Graded triangle mesh of the tool with about target_tris triangles (FEM set-up of
+    FEM_thermal_gpu.cu:584-603).  The base spacing is adjusted until the count is within 10 %."""
+    area = length * height
+    s_max = math.sqrt(2.0 * area / (target_tris * 0.62))
+    # the quadtree quantises cell sizes to root / 2^k: search the root scale as well
+    best = None
+    for u in (0.0, 0.125, 0.25, 0.375, 0.5, 0.625, 0.75, 0.875):
+        ntri = 2 * len(_tool_points(length, height, rake_deg, clear_deg, r, s_max, 2.0 ** u)[0])
+        if best is None or abs(ntri - target_tris) < abs(best[1] - target_tris):
+            best = (u, ntri)
+    return _mesh_tool(length, height, rake_deg, clear_deg, r, s_max, tc, 2.0 ** best[0])
 
+
+def _tool_points(length, height, rake_deg, clear_deg, r, s_max, root_scale=1.0):
     g = tool_outline(length, height, rake_deg, clear_deg, r)
     tl, tr, bl, trc, blc, centre = g["tl"], g["tr"], g["bl"], g["trc"], g["blc"], g["centre"]
     edge = g["br"] if centre is None else centre
-    area = length * height
-    # base spacing from the target count (2 triangles per point, grading factor ~1.6)
-    s_max = math.sqrt(2.0 * area / (target_tris * 0.62))
     s_min = min(s_max, (r if r > 0 else s_max) / 5.0)
     grad = 0.22
 
@@ -288,7 +296,7 @@ def mesh_tool(length, height, rake_deg, clear_deg, r, target_tris, tc) -> ToolMe
 
     # interior points from a graded quadtree over the bounding square
     lo = np.array([min(tl[0], bl[0]), min(bl[1], g["br"][1])])
-    size = max(length + abs(g["br"][0] - tr[0]), height) * 1.0001
+    size = max(length + abs(g["br"][0] - tr[0]), height) * 1.0001 * root_scale
     interior = []
     stack = [(lo[0], lo[1], size)]
     while stack:
@@ -319,8 +327,30 @@ def mesh_tool(length, height, rake_deg, clear_deg, r, target_tris, tc) -> ToolMe
     pts = pts[dmin > 0.55 * loc[keep]]
     order = np.lexsort((pts[:, 1], pts[:, 0]))
     pts = pts[order]
-
     nodes = np.concatenate([boundary, pts])
+    return nodes, g, cut, fixed, boundary, nb_cut, s_min
+
+
+def _mesh_tool(length, height, rake_deg, clear_deg, r, s_max, tc, root_scale=1.0) -> ToolMesh:
+    from scipy.spatial import Delaunay
+
+    nodes, g, cut, fixed, boundary, nb_cut, s_min = _tool_points(length, height, rake_deg, clear_deg, r, s_max,
+                                                                 root_scale)
+    tl = g["tl"]
+    # Laplacian smoothing of the interior nodes (boundary nodes stay put); the tool is convex, so a
+    # neighbour average stays inside it.  Removes the slivers of the quadtree/boundary transition.
+    nbd0 = len(boundary)
+    for _ in range(4):
+        tri = Delaunay(nodes).simplices
+        acc = np.zeros_like(nodes)
+        cnt = np.zeros(len(nodes))
+        for a, b in ((0, 1), (1, 2), (2, 0)):
+            np.add.at(acc, tri[:, a], nodes[tri[:, b]])
+            np.add.at(acc, tri[:, b], nodes[tri[:, a]])
+            np.add.at(cnt, tri[:, a], 1.0)
+            np.add.at(cnt, tri[:, b], 1.0)
+        nodes = nodes.copy()
+        nodes[nbd0:] = acc[nbd0:] / cnt[nbd0:, None]
     tri = Delaunay(nodes).simplices.astype(np.int64)
     # orient counter-clockwise
     p0, p1, p2 = nodes[tri[:, 0]], nodes[tri[:, 1]], nodes[tri[:, 2]]

@@ -134,8 +134,15 @@ void swap_fused(sphfem_particles &p, sphfem_particles &b) {
 
 namespace sphfem {
 
+static bool timed(const sphfem_ctx *ctx, const char *name) {
+    if (!ctx->timing.enabled) return false;
+    if (ctx->timing.only.empty()) return true;
+    return ("," + ctx->timing.only + ",").find("," + std::string(name) + ",") != std::string::npos;
+}
+
 void timing_begin(sphfem_ctx *ctx, const char *name, hipEvent_t *ev) {
-    if (!ctx->timing.enabled) return;
+    *ev = nullptr;
+    if (!timed(ctx, name)) return;
     (void)hipEventCreate(ev);
     (void)hipEventRecord(*ev, ctx->stream);
 }
@@ -881,11 +888,6 @@ int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particle
     });
 }
 
-int sphfem_leap_frog_cells(sphfem_leap_frog *lf, const int **cell_start, const int **cell_end) {
-    (void)lf; (void)cell_start; (void)cell_end;
-    return SPHFEM_ERR_STATE;
-}
-
 int sphfem_report_force(sphfem_ctx *ctx, sphfem_particles *p, sphfem_double2 *out) {
     if (!p || !out) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {
@@ -901,6 +903,12 @@ int sphfem_report_force(sphfem_ctx *ctx, sphfem_particles *p, sphfem_double2 *ou
 int sphfem_timing_enable(sphfem_ctx *ctx, int enable) {
     if (!ctx) return SPHFEM_ERR_ARG;
     ctx->timing.enabled = enable != 0;
+    return SPHFEM_OK;
+}
+
+int sphfem_timing_select(sphfem_ctx *ctx, const char *names) {
+    if (!ctx) return SPHFEM_ERR_ARG;
+    ctx->timing.only = names ? names : "";
     return SPHFEM_OK;
 }
 

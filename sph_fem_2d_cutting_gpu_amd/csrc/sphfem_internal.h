@@ -41,6 +41,7 @@ typedef sphfem_particles Particles;
 
 struct Timing {
     bool enabled = false;
+    std::string only;     // comma-separated kernel names to time (empty: all)
     struct Rec { hipEvent_t a, b; };
     std::map<std::string, std::vector<Rec>> pending;
     std::map<std::string, double> total_ms;

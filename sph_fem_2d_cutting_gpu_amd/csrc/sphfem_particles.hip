@@ -284,56 +284,6 @@ void launch_leapfrog(sphfem_ctx *ctx, int which, const sphfem_particles &p, sphf
 // Fields: A.pos/vel/S/rho/T/eps_pl/eps_pl_dot/blanked are outputs; B.fc/ft/n/on_seg are updated
 // in place (own particle only); derivatives go to A.pos_t/vel_t/rho_t/S_t/T_t.
 // ---------------------------------------------------------------------------------------------
-struct ToolView {
-    const double *T_nodes;
-    const sphfem_int4 *marks;
-    double *fric_heat;
-    sphfem_double4 *physical_para;
-    double *sliding_force;
-    const ToolKin *kin;
-    int *near_list;
-    int *near_count;
-    int active;
-};
-
-// actions_gpu.cu:345-420 for one particle; returns false if the particle was blanked
-__device__ __forceinline__ bool fric_heat_v21(const DevConsts &c, const ToolView &tv, const ToolKin &kin,
-                                              double eps_pl, sphfem_double2 ft, int seg, sphfem_double2 normal,
-                                              sphfem_double2 vp, sphfem_double2 fcv, double rho, double &T) {
-    if (eps_pl > 20.) return true;
-    double f_fric_mag = sqrtf_d(ft.x * ft.x + ft.y * ft.y);
-    if (seg == -1) return true;
-    const double partition_wp = c.tc.beta;
-    const double dt = c.dt;
-    double vdx = vp.x - kin.vel_x, vdy = vp.y - kin.vel_y;
-    double vdn = vdx * normal.x + vdy * normal.y;
-    double vrx = vdx - vdn * normal.x, vry = vdy - vdn * normal.y;
-    double v_rel_mag = sqrtf_d(vrx * vrx + vry * vry);
-    double value = c.tc.eta * f_fric_mag * v_rel_mag;
-    if (isnan(v_rel_mag)) return false;
-    double f_normal = sqrtf_d(fcv.x * fcv.x + fcv.y * fcv.y);
-    double length = sqrtf_d(c.tc.slave_mass / rho);
-    double pressure_trial = f_normal / length;
-    if (v_rel_mag > 0.03) return true;
-    double T_buffer = T + value * (1 - partition_wp) * dt / c.trml_wp.cp / c.phys.mass;
-    if (T_buffer > 1550.) {
-        T = 1550.;
-        return true;
-    }
-    T = T_buffer;
-    unsafeAtomicAdd(&tv.fric_heat[seg], value * partition_wp);
-    sphfem_int4 mk = tv.marks[seg];
-    double T_seg = 0.5 * (tv.T_nodes[mk.z] + tv.T_nodes[mk.w]);
-    double vel_tool_norm = sqrt(kin.vel_x * kin.vel_x + kin.vel_y * kin.vel_y);
-    if (v_rel_mag > 0.008) return true;
-    else if (v_rel_mag > vel_tool_norm * 8.0 && fabs(normal.y) > 0.5) return true;
-    unsafeAtomicAdd(&tv.physical_para[seg].x, 1.);
-    unsafeAtomicAdd(&tv.physical_para[seg].y, pressure_trial);
-    unsafeAtomicAdd(&tv.physical_para[seg].z, v_rel_mag);
-    unsafeAtomicAdd(&tv.physical_para[seg].w, T_seg);
-    unsafeAtomicAdd(&tv.sliding_force[seg], f_fric_mag / length);
-    return true;
-}
 
 // B = sorted predicted state (the grid buffer); A = the caller's arrays, whose unsorted content
 // is dead after the reorder except the previous derivatives.  Outputs: corrected state into

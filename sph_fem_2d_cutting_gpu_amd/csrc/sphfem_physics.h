@@ -332,4 +332,99 @@ __device__ __forceinline__ bool near_tool(const ToolKin &kin, sphfem_double2 pos
 
 __device__ __forceinline__ bool isnaninf(double v) { return isnan(v) || isinf(v); }
 
+// Atomic add of v[q] into dst[q][key * stride] for the executing lanes with `active`, aggregated
+// per distinct key within the wavefront: the values of lanes with equal keys are summed in lane
+// order (reading only those lanes) and the lowest of them issues one atomic per value.  Safe in
+// divergent code: only executing lanes take part.  Keeps the first step cheap, when every
+// particle still carries segment 0 (particle_gpu.cu:98) and would hit one address.
+template <int NV>
+__device__ __forceinline__ void wave_atomic_add(bool active, int key, const double (&v)[NV], double *const (&dst)[NV],
+                                                int stride) {
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int lkey = __shfl(key, leader);
+        const bool mine = active && key == lkey && ((pending >> __lane_id()) & 1ull);
+        const uint64_t group = __ballot(mine);
+        double s[NV];
+        for (int q = 0; q < NV; q++) s[q] = 0.0;
+        uint64_t g = group;
+        while (g) {
+            const int l = __ffsll((unsigned long long)g) - 1;
+            for (int q = 0; q < NV; q++) s[q] += __shfl(v[q], l);
+            g &= g - 1;
+        }
+        if (__lane_id() == leader)
+            for (int q = 0; q < NV; q++) unsafeAtomicAdd(dst[q] + (long)lkey * stride, s[q]);
+        pending &= ~group;
+    }
+}
+
+// tool-side arrays the particle kernels accumulate into
+struct ToolView {
+    const double *T_nodes;
+    const sphfem_int4 *marks;
+    double *fric_heat;
+    sphfem_double4 *physical_para;
+    double *sliding_force;
+    const ToolKin *kin;
+    int *near_list;
+    int *near_count;
+    int active;
+};
+
+// actions_gpu.cu:345-420 for one particle; returns false if the particle was blanked
+__device__ __forceinline__ bool fric_heat_v21(const DevConsts &c, const ToolView &tv, const ToolKin &kin,
+                                              double eps_pl, sphfem_double2 ft, int seg, sphfem_double2 normal,
+                                              sphfem_double2 vp, sphfem_double2 fcv, double rho, double &T) {
+    bool keep = true, add_heat = false, add_wear = false;
+    double heat = 0., pp[4] = {0., 0., 0., 0.}, slide = 0.;
+    do {
+        if (eps_pl > 20.) break;
+        double f_fric_mag = sqrtf_d(ft.x * ft.x + ft.y * ft.y);
+        if (seg == -1) break;
+        const double partition_wp = c.tc.beta;
+        const double dt = c.dt;
+        double vdx = vp.x - kin.vel_x, vdy = vp.y - kin.vel_y;
+        double vdn = vdx * normal.x + vdy * normal.y;
+        double vrx = vdx - vdn * normal.x, vry = vdy - vdn * normal.y;
+        double v_rel_mag = sqrtf_d(vrx * vrx + vry * vry);
+        double value = c.tc.eta * f_fric_mag * v_rel_mag;
+        if (isnan(v_rel_mag)) { keep = false; break; }
+        double f_normal = sqrtf_d(fcv.x * fcv.x + fcv.y * fcv.y);
+        double length = sqrtf_d(c.tc.slave_mass / rho);
+        double pressure_trial = f_normal / length;
+        if (v_rel_mag > 0.03) break;
+        double T_buffer = T + value * (1 - partition_wp) * dt / c.trml_wp.cp / c.phys.mass;
+        if (T_buffer > 1550.) {
+            T = 1550.;
+            break;
+        }
+        T = T_buffer;
+        add_heat = true;
+        heat = value * partition_wp;
+        sphfem_int4 mk = tv.marks[seg];
+        double T_seg = 0.5 * (tv.T_nodes[mk.z] + tv.T_nodes[mk.w]);
+        double vel_tool_norm = sqrt(kin.vel_x * kin.vel_x + kin.vel_y * kin.vel_y);
+        if (v_rel_mag > 0.008) break;
+        else if (v_rel_mag > vel_tool_norm * 8.0 && fabs(normal.y) > 0.5) break;
+        add_wear = true;
+        pp[0] = 1.;
+        pp[1] = pressure_trial;
+        pp[2] = v_rel_mag;
+        pp[3] = T_seg;
+        slide = f_fric_mag / length;
+    } while (false);
+    const double hv[1] = {heat};
+    double *const hd[1] = {tv.fric_heat};
+    wave_atomic_add<1>(add_heat, seg, hv, hd, 1);
+    double *const pd[4] = {&tv.physical_para[0].x, &tv.physical_para[0].y, &tv.physical_para[0].z,
+                           &tv.physical_para[0].w};
+    wave_atomic_add<4>(add_wear, seg, pp, pd, 4);
+    const double sv[1] = {slide};
+    double *const sd[1] = {tv.sliding_force};
+    wave_atomic_add<1>(add_wear, seg, sv, sd, 1);
+    return keep;
+}
+
 }  // namespace sphfem

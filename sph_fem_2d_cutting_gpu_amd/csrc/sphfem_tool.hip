@@ -343,45 +343,16 @@ __global__ void __launch_bounds__(256) k_fric_heat(DevConsts c, sphfem_particles
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int)P.N) return;
     if (P.blanked[i] == 1.) return;
-    if (P.eps_pl[i] > 20.) return;
-    sphfem_double2 ft = P.ft[i];
-    double f_fric_mag = sqrtf_d(ft.x * ft.x + ft.y * ft.y);
-    int seg = P.on_seg[i];
-    if (seg == -1) return;
-    const ToolKin k = *t.kin;
-    const double partition_wp = c.tc.beta, dt = c.dt;
-    sphfem_double2 normal = P.n[i], vp = P.vel[i];
-    double vdx = vp.x - k.vel_x, vdy = vp.y - k.vel_y;
-    double vdn = vdx * normal.x + vdy * normal.y;
-    double vrx = vdx - vdn * normal.x, vry = vdy - vdn * normal.y;
-    double v_rel_mag = sqrtf_d(vrx * vrx + vry * vry);
-    double value = c.tc.eta * f_fric_mag * v_rel_mag;
-    if (isnan(v_rel_mag)) {
+    ToolView tv{};
+    tv.T_nodes = t.T;
+    tv.marks = t.marks;
+    tv.fric_heat = t.fric_heat;
+    tv.physical_para = t.physical_para;
+    tv.sliding_force = t.sliding_force;
+    double T = P.T[i];
+    if (!fric_heat_v21(c, tv, *t.kin, P.eps_pl[i], P.ft[i], P.on_seg[i], P.n[i], P.vel[i], P.fc[i], P.rho[i], T))
         P.blanked[i] = 1.;
-        return;
-    }
-    sphfem_double2 fcv = P.fc[i];
-    double f_normal = sqrtf_d(fcv.x * fcv.x + fcv.y * fcv.y);
-    double length = sqrtf_d(c.tc.slave_mass / P.rho[i]);
-    double pressure_trial = f_normal / length;
-    if (v_rel_mag > 0.03) return;
-    double T_buffer = P.T[i] + value * (1 - partition_wp) * dt / c.trml_wp.cp / c.phys.mass;
-    if (T_buffer > 1550.) {
-        P.T[i] = 1550.;
-        return;
-    }
-    P.T[i] = T_buffer;
-    unsafeAtomicAdd(&t.fric_heat[seg], value * partition_wp);
-    sphfem_int4 mk = t.marks[seg];
-    double T_seg = 0.5 * (t.T[mk.z] + t.T[mk.w]);
-    double vel_tool_norm = sqrt(k.vel_x * k.vel_x + k.vel_y * k.vel_y);
-    if (v_rel_mag > 0.008) return;
-    else if (v_rel_mag > vel_tool_norm * 8.0 && fabs(normal.y) > 0.5) return;
-    unsafeAtomicAdd(&t.physical_para[seg].x, 1.);
-    unsafeAtomicAdd(&t.physical_para[seg].y, pressure_trial);
-    unsafeAtomicAdd(&t.physical_para[seg].z, v_rel_mag);
-    unsafeAtomicAdd(&t.physical_para[seg].w, T_seg);
-    unsafeAtomicAdd(&t.sliding_force[seg], f_fric_mag / length);
+    P.T[i] = T;
 }
 
 void launch_fric_heat(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t) {
@@ -421,8 +392,9 @@ __device__ __forceinline__ void contact_s2_one(const DevConsts &c, const sphfem_
     double T_seg = 0.5 * (t.T[mk.z] + t.T[mk.w]);
     double delta_T = T_seg - T_;
     double length = sqrtf_d(c.tc.slave_mass / rho);
-    unsafeAtomicAdd(&t.heat_exchange[seg_no].x, T_ * length);
-    unsafeAtomicAdd(&t.heat_exchange[seg_no].y, length);
+    const double hv[2] = {T_ * length, length};
+    double *const hd[2] = {&t.heat_exchange[0].x, &t.heat_exchange[0].y};
+    wave_atomic_add<2>(true, seg_no, hv, hd, 2);
     double dTp = delta_T * c.tc.h * length / c.trml_wp.cp / c.tc.slave_mass * c.dt;
     if (dTp < 0) T += fmax(dTp, delta_T);
     else T += fmin(dTp, delta_T);

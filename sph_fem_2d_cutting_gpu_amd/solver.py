@@ -74,6 +74,9 @@ class Context:
     def timing(self, enable: bool):
         self.call("sphfem_timing_enable", int(enable))
 
+    def timing_select(self, names: str):
+        self.call("sphfem_timing_select", names.encode())
+
     def timing_reset(self):
         self.call("sphfem_timing_reset")
 
