@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-INTERACTION_KERNEL = {"fp64": "fused_update", "fp32c": "fused_update_fp32c"}
+INTERACTION_KERNEL = {"fp64": "fused_update", "fp32c": "pairs_fp32c"}
 
 
 def interaction_bytes(N, n_occ_cells, w):
@@ -56,13 +56,58 @@ def cpu_baseline(case, budget_s):
                       f"{threads} OpenMP threads"}
 
 
+def run_form(case, precision, steps, warmup, breakdown=False):
+    """Time `steps` full steps of one arithmetic form; returns (seconds, kernel stats, sim)."""
+    from sph_fem_2d_cutting_gpu_amd import _lib
+    from sph_fem_2d_cutting_gpu_amd import solver as S
+    prec = {"fp64": _lib.FP64, "fp32c": _lib.FP32C}[precision]
+    sim = S.Simulation(case, precision=prec)
+    ctx = sim.ctx
+    for _ in range(warmup):
+        sim.step(1)
+    ctx.synchronize()
+    ctx.check_error()
+    kname = INTERACTION_KERNEL[precision]
+    ctx.timing_select(kname)
+    ctx.timing(True)
+    ctx.timing_reset()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.timing(False)
+    ctx.check_error()
+    k_ms, k_n = ctx.timing_get(kname)
+    if breakdown:
+        # separate, untimed pass with an event pair around every kernel
+        ctx.timing_select("")
+        ctx.timing(True)
+        ctx.timing_reset()
+        sim.step(steps)
+        ctx.synchronize()
+        ctx.timing(False)
+        rows = []
+        for name in ctx.timing_names():
+            ms, n = ctx.timing_get(name)
+            if n:
+                rows.append((ms / steps, n / steps, name))
+        rows.sort(reverse=True)
+        print(f"[{precision}] per-step kernel time (ms), {steps} steps:", file=sys.stderr)
+        for ms, n, name in rows:
+            print(f"  {name:28s} {ms:9.4f}  x{n:.0f}", file=sys.stderr)
+    return elapsed, (k_ms / 1e3) / max(k_n, 1), sim
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="ti64_4m")
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--precision", default="fp32c", choices=["fp32c", "fp64"])
+    ap.add_argument("--no-compare", action="store_true", help="skip timing the FP64 form beside FP32C")
+    ap.add_argument("--breakdown", action="store_true", help="per-kernel times on stderr")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -72,39 +117,18 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    from sph_fem_2d_cutting_gpu_amd import _lib
-    from sph_fem_2d_cutting_gpu_amd import solver as S
 
     case, workload = make_case(args.workload)
-    prec = {"fp64": _lib.FP64}[args.precision]
-    sim = S.Simulation(case, precision=prec)
-    ctx = sim.ctx
-    for _ in range(args.warmup):
-        sim.step(1)
-    ctx.synchronize()
-    ctx.check_error()
-
-    kname = INTERACTION_KERNEL[args.precision]
-    ctx.timing_select(kname)
-    ctx.timing(True)
-    ctx.timing_reset()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    sim.step(args.steps)
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    ctx.timing(False)
-    ctx.check_error()
-    k_ms, k_n = ctx.timing_get(kname)
-
+    elapsed, k_avg_s, sim = run_form(case, args.precision, args.steps, args.warmup, args.breakdown)
     cs, _ = sim.grid.download_cells()
     n_occ = int(np.count_nonzero(cs != -1))
     w = 8 if args.precision == "fp64" else 4
     b_int = interaction_bytes(case.N, n_occ, w)
-    k_avg_s = (k_ms / 1e3) / max(k_n, 1)
     achieved = b_int / k_avg_s / 1e9
-
     ups = case.N * args.steps / elapsed
+    sim.close()
+    del sim
+
     out = {
         "metric": "particle-updates/s at 4M particles, FP32-comp vs FP64; % HBM roofline",
         "value": ups,
@@ -116,16 +140,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32 pair sums (FP64 block accumulation) + f64 state",
         "data": "synthetic (seeded jittered lattice and generated tool mesh)",
         "config": {"workload": workload, "particles": case.N, "tool_triangles": int(len(case.tool.elems)),
                    "cutting_segments": int(len(case.tool.seg_left)), "wear": bool(case.wear),
                    "precision": args.precision, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": kname, "kernel_ms": k_avg_s * 1e3, "algorithmic_bytes": b_int,
-                     "occupied_cells": n_occ},
+                     "kernel": INTERACTION_KERNEL[args.precision], "kernel_ms": k_avg_s * 1e3,
+                     "algorithmic_bytes": b_int, "occupied_cells": n_occ},
     }
+    if args.precision == "fp32c" and not args.no_compare:
+        e64, k64, sim64 = run_form(case, "fp64", args.steps, args.warmup, args.breakdown)
+        sim64.close()
+        ups64 = case.N * args.steps / e64
+        out["fp64_form"] = {"value": ups64, "ms_per_step": e64 / args.steps * 1e3,
+                            "kernel": INTERACTION_KERNEL["fp64"], "kernel_ms": k64 * 1e3,
+                            "roofline_frac": interaction_bytes(case.N, n_occ, 8) / k64 / 1e9 / HBM_PEAK_GBS}
+        out["speedup_vs_fp64"] = ups / ups64
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_budget)
     print(json.dumps(out))

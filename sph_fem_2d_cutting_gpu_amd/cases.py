@@ -744,6 +744,27 @@ PARTICLE_FIELDS = (
 )
 
 
+def perturbed(case: Case, seed: int = DEFAULT_SEED + 1, vel_sd: float = 5e-4, S_sd: float = 3e-3,
+              rho_rel_sd: float = 1e-3, T_sd: float = 50.0) -> Case:
+    """The same case with seeded random fields on the non-fixed particles: velocity N(0, vel_sd)
+    per axis (cm/us), deviatoric stress N(0, S_sd) per component with S_zz = -(S_xx + S_yy),
+    density rho0 * (1 + N(0, rho_rel_sd)) and temperature T + |N(0, T_sd)|.  Every term of the
+    pair sums is then non-zero on the first step (from rest they all vanish)."""
+    import dataclasses
+    n = case.N
+    free = (case.fixed == 0.0)[:, None]
+    vel = case.vel + free * normal01(seed, 2 * n, stream=10).reshape(n, 2) * vel_sd
+    sd = normal01(seed, 3 * n, stream=12).reshape(n, 3) * S_sd
+    S = np.zeros((n, 4))
+    S[:, 0], S[:, 1], S[:, 2] = sd[:, 0], sd[:, 1], sd[:, 2]
+    S[:, 3] = -(S[:, 0] + S[:, 2])
+    S0 = case.S if case.S is not None else np.zeros((n, 4))
+    S = S0 + free * S
+    rho = case.rho * (1.0 + free[:, 0] * normal01(seed, n, stream=14) * rho_rel_sd)
+    T = case.T + free[:, 0] * np.abs(normal01(seed, n, stream=15)) * T_sd
+    return dataclasses.replace(case, name=case.name + "_perturbed", vel=vel, S=S, rho=rho, T=T)
+
+
 def initial_particles(case: Case) -> dict:
     """State after particle_gpu(pos, vel, rho, T, h, fixed, N) (particle_gpu.cu:11-104, 203-206):
     every array zero, idx = 0..N-1, on_seg = 0, then the given fields copied in."""

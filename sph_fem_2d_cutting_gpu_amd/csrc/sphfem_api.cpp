@@ -821,6 +821,7 @@ int sphfem_leap_frog_destroy(sphfem_ctx *ctx, sphfem_leap_frog *lf) {
         SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         dfree(lf->pos_init); dfree(lf->vel_init); dfree(lf->S_init); dfree(lf->rho_init); dfree(lf->T_init);
         dfree(lf->cell_start); dfree(lf->cell_end); dfree(lf->near_list); dfree(lf->contact_list);
+        dfree(lf->rec); dfree(lf->psum); dfree(lf->psum_T);
         delete lf;
         return SPHFEM_OK;
     });
@@ -828,9 +829,22 @@ int sphfem_leap_frog_destroy(sphfem_ctx *ctx, sphfem_leap_frog *lf) {
 
 int sphfem_leap_frog_set_precision(sphfem_ctx *ctx, sphfem_leap_frog *lf, int precision) {
     if (!ctx || !lf) return SPHFEM_ERR_ARG;
-    if (precision != SPHFEM_FP64) return fail(ctx, SPHFEM_ERR_ARG, "unsupported precision");
-    lf->precision = precision;
-    return SPHFEM_OK;
+    if (precision != SPHFEM_FP64 && precision != SPHFEM_FP32C) return fail(ctx, SPHFEM_ERR_ARG, "unsupported precision");
+    return guarded(ctx, [&] {
+        if (precision == SPHFEM_FP32C) {
+            // neighbour records are addressed by 32-bit byte offsets (16 B each)
+            if (lf->num_part >= (1u << 28)) return fail(ctx, SPHFEM_ERR_RANGE, "FP32C supports < 2^28 particles");
+            if (!lf->rec) {
+                // record blocks r0 [N + 4], r1, r2, r3 [N]; the padding of r0 is zeroed once
+                lf->rec = dalloc<float4>(4ul * lf->num_part + 4);
+                SPHFEM_HIP_CHECK(hipMemsetAsync(lf->rec + lf->num_part, 0, 4 * sizeof(float4), ctx->stream));
+            }
+            if (!lf->psum) lf->psum = dalloc<float4>(2ul * lf->num_part);
+            if (!lf->psum_T) lf->psum_T = dalloc<float>(lf->num_part);
+        }
+        lf->precision = precision;
+        return SPHFEM_OK;
+    });
 }
 
 int sphfem_leap_frog_init(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p) {

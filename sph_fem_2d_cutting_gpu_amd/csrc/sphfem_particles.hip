@@ -132,6 +132,250 @@ __device__ __forceinline__ PairSums pair_sums_fp64(const DevConsts &c, const sph
     return r;
 }
 
+// ---------------------------------------------------------------------------------------------
+// FP32-compensated pair sums (same physics as pair_sums_fp64, different arithmetic).
+//  * positions: cell-relative FP32 (x_ij = dx_rel + (cell_i - cell_j) * h), so the pair distance
+//    carries ~1e-7 of the cell size whatever the absolute coordinates;
+//  * transcendental work is one v_rsq (the distance), one v_rcp (2 / (rho_i + rho_j)) and one
+//    v_rcp for the viscosity denominator of approaching pairs;
+//  * the grad-T term uses e_ij . grad W / r_ij = g (the radial kernel derivative factor), which
+//    is the reference's algebra without the extra sqrt and divisions;
+//  * each lane first compacts its in-support candidates (column order, sorted index within a
+//    column: the reference order) into an LDS list of NB_CHUNK entries, then runs the pair body
+//    over the dense list, so the wavefront does not execute the body for rejected candidates;
+//  * summation: blocks of 8 pair terms in FP32, block sums accumulated in FP64, which bounds the
+//    rounding of the sums by 8 units of FP32 roundoff of the sum of |terms|.
+// Constant factors (mass, -eps*mass, 2) are applied once in FP64 after the sums.
+// ---------------------------------------------------------------------------------------------
+constexpr int NB_CHUNK = 32;
+constexpr int FUSED_BS = 256;
+
+// FP32C neighbour records (written by the reorder kernel), four SoA blocks of float4:
+//   r0[j] = {x - cell corner x, y - cell corner y, cell row, h}   (block padded by 4 records)
+//   r1[j] = {vx, vy, (Sxx - p)/rho^2, Sxy/rho^2}     r2[j] = {(Syy - p)/rho^2, Rxx, Rxy, Ryy}
+//   r3[j] = {+-mass/rho, rho, sqrt(K/rho), T}
+// The volume is stored negative for tool particles; blanked particles have x = 1e30.  SoA keeps a
+// wavefront's gather of one field within few cache lines.
+struct RecView {
+    const float4 *__restrict__ r0;
+    const float4 *__restrict__ r1;
+    const float4 *__restrict__ r2;
+    const float4 *__restrict__ r3;
+};
+
+// record load at a 32-bit byte offset from a uniform base (one VGPR offset, no 64-bit address math)
+__device__ __forceinline__ float4 ldrec(const float4 *__restrict__ base, unsigned byte_off) {
+    return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
+struct PairI {            // particle i and the per-i kernel constants
+    float fxi, fyi, rowi, hi, vxi, vyi, sxi, syi, szi, Rxi, Ryi, Rzi, rhoi, ci, Ti;
+    float h1, fac, c_out_w, c_out_g, c_in_g;
+    bool thermal;
+};
+
+struct PairAcc {          // FP32 block sums
+    float v0, v1, v2, v3, ax, ay, p0, p1, T;
+};
+
+__device__ __forceinline__ void pair_term(const PairI &I, float hf, float inv_wdp, float alpha, float beta,
+                                          float eta2, const float4 q0, const float4 q1, const float4 q2,
+                                          const float4 q3, int e, PairAcc &a) {
+#pragma clang fp contract(fast)
+    const float dcol = (float)((int)((unsigned)e >> 29) - 2);     // column of j minus column of i
+    const float xij = (I.fxi - q0.x) - dcol * hf;
+    const float yij = (I.fyi - q0.y) + (I.rowi - q0.z) * hf;
+    const float rr2 = xij * xij + yij * yij;
+    const float rinv = __builtin_amdgcn_rsqf(rr2);
+    const float q = rr2 * rinv * I.h1;
+    float w, g;
+    if (q > 1.f) {
+        const float tq = 2.f - q;
+        const float tq2 = tq * tq;
+        w = I.c_out_w * (tq2 * tq2 * tq2);
+        g = I.c_out_g * tq2 * rinv;
+    } else {
+        w = I.fac * (1.f - 1.5f * q * q * (1.f - 0.5f * q));
+        g = I.c_in_g * (1.f - 0.75f * q);
+    }
+    const float wx = g * xij, wy = g * yij;
+    const float volj = fabsf(q3.x);
+    if (I.thermal && rr2 > 1e-16f) a.T += volj * (I.Ti - q3.w) * g;
+    if (q3.x > 0.f) {                                    // j is not a tool particle
+        const float dvx = q1.x - I.vxi, dvy = q1.y - I.vyi;
+        const float gx = dvx * volj, gy = dvy * volj;
+        a.v0 += gx * wx;
+        a.v1 += gx * wy;
+        a.v2 += gy * wx;
+        a.v3 += gy * wy;
+        float f = w * inv_wdp;
+        f *= f;
+        f *= f;
+        float Ax = (I.sxi + q1.z) + f * (I.Rxi + q2.y);
+        const float Ay = (I.syi + q1.w) + f * (I.Ryi + q2.z);
+        float Az = (I.szi + q2.x) + f * (I.Rzi + q2.w);
+        const float inv_rhoij = 2.f * __builtin_amdgcn_rcpf(I.rhoi + q3.y);
+        const float vp = -(dvx * xij + dvy * yij);          // (v_i - v_j) . x_ij
+        if (vp < 0.f) {
+            const float cij = 0.5f * (I.ci + q3.z);
+            const float hij = 0.5f * (I.hi + q0.w);
+            const float mu = hij * vp * __builtin_amdgcn_rcpf(rr2 + eta2 * hij * hij);
+            const float piij = mu * (beta * mu - alpha * cij) * inv_rhoij;
+            Ax -= piij;
+            Az -= piij;
+        }
+        a.ax += Ax * wx + Ay * wy;
+        a.ay += Ay * wx + Az * wy;
+        const float wr = w * inv_rhoij;
+        a.p0 -= wr * dvx;                                   // sum w / rho_ij * (v_i - v_j)
+        a.p1 -= wr * dvy;
+    }
+}
+
+// FP32-compensated pair sums (same physics as pair_sums_fp64, different arithmetic), one lane per
+// sorted particle:
+//  * positions are cell-relative FP32 (x_ij = dx_rel + (cell_i - cell_j) * h): the pair distance
+//    carries ~1e-7 of the cell size whatever the absolute coordinates;
+//  * one v_rsq per pair (the distance), one v_rcp (2 / (rho_i + rho_j)) and one v_rcp for the
+//    viscosity denominator of approaching pairs;
+//  * the grad-T term uses e_ij . grad W / r_ij = g (the radial kernel-derivative factor): the
+//    reference's algebra without its extra sqrt and divisions;
+//  * each lane compacts its in-support candidates (columns in order, sorted index within a
+//    column: the reference order) into an LDS list of NB_CHUNK entries, loading candidates four
+//    at a time, then evaluates the dense list with the records of the next entry in flight;
+//  * summation: blocks of 8 pair terms in FP32, block sums accumulated in FP64, which bounds the
+//    rounding of each sum by ~8 units of FP32 roundoff of the sum of |terms|; one final FP32
+//    rounding on output.
+// Outputs (FP32): psum[i] = velocity gradient (xx, xy, yx, yy); psum[N + i] = (mass * stress
+// divergence - mass * viscous term) per axis, XSPH velocity per axis; psum_T[i] = Laplacian of T.
+__global__ void __launch_bounds__(FUSED_BS) k_pairs_fp32c(DevConsts c, const double *__restrict__ blanked,
+                                                         const int *__restrict__ hash, int N, RecView rv,
+                                                         const int *__restrict__ cs, const int *__restrict__ ce,
+                                                         float4 *__restrict__ psum, float *__restrict__ psum_T) {
+#pragma clang fp contract(fast)
+    __shared__ int nb_list[NB_CHUNK * FUSED_BS];
+    const int i = blockIdx.x * FUSED_BS + threadIdx.x;
+    if (i >= N) return;
+    int *__restrict__ list = nb_list + threadIdx.x;
+    double d_v0 = 0., d_v1 = 0., d_v2 = 0., d_v3 = 0.;
+    double d_ax = 0., d_ay = 0., d_p0 = 0., d_p1 = 0., d_T = 0.;
+    const float4 a3 = rv.r3[i];
+    // tool particles (negative volume) have no sums (interactions_gpu.cu:216; their thermal alpha
+    // is zero); blanked particles are skipped
+    if (blanked[i] != 1. && a3.x > 0.f) {
+        const int nx = c.geom.nx, ny = c.geom.ny;
+        const int hashi = hash[i];
+        const int gi = hashi / ny;
+        const int gj = hashi - gi * ny;
+        const int low_j = gj - 2 < 0 ? 0 : gj - 2;
+        const int high_j = gj + 3 > ny ? ny : gj + 3;
+        // cell-table entries of the five columns, issued together: the lowest and highest row of
+        // each column's window decide its sorted range unless those cells are empty
+        int cs_lo[5], cs_hi[5], ce_hi[5];
+#pragma unroll
+        for (int d = 0; d < 5; d++) {
+            const int col = gi - 2 + d;
+            const bool ok = col >= 0 && col < nx;
+            const int c0 = (ok ? col : gi) * ny;
+            cs_lo[d] = ok ? cs[c0 + low_j] : -1;
+            cs_hi[d] = ok ? cs[c0 + high_j - 1] : -1;
+            ce_hi[d] = ok ? ce[c0 + high_j - 1] : -1;
+        }
+        const float4 a0 = rv.r0[i], a1 = rv.r1[i], a2 = rv.r2[i];
+        PairI I;
+        I.fxi = a0.x; I.fyi = a0.y; I.rowi = a0.z; I.hi = a0.w;
+        I.vxi = a1.x; I.vyi = a1.y; I.sxi = a1.z; I.syi = a1.w;
+        I.szi = a2.x; I.Rxi = a2.y; I.Ryi = a2.z; I.Rzi = a2.w;
+        I.rhoi = a3.y; I.ci = a3.z; I.Ti = a3.w;
+        I.h1 = 1.f / I.hi;
+        I.fac = (float)SPHFEM_PI_FAC * I.h1 * I.h1;
+        I.c_out_w = 0.25f * I.fac;
+        I.c_out_g = -0.75f * I.h1 * I.fac;
+        I.c_in_g = -3.f * I.h1 * I.h1 * I.fac;
+        I.thermal = c.trml_wp.alpha != 0.;     // the workpiece's alpha (interactions_gpu.cu:19)
+        const float fourh2 = 4.f * I.hi * I.hi;
+        const float rr2min = (float)c.rr2_min;
+        const float hf = (float)c.geom.dx;
+        const float inv_wdp = (float)(1. / c.corr.wdeltap);
+        const float alpha = (float)c.corr.alpha, beta = (float)c.corr.beta;
+        const float eta2 = (float)(c.corr.eta * c.corr.eta);
+
+        auto flush = [&](PairAcc &a) {
+            d_v0 += a.v0; d_v1 += a.v1; d_v2 += a.v2; d_v3 += a.v3;
+            d_ax += a.ax; d_ay += a.ay; d_p0 += a.p0; d_p1 += a.p1; d_T += a.T;
+            a = PairAcc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        };
+        // dense list: two entries per iteration, the records of the next entry loaded before the
+        // current one is evaluated (ping-pong registers, no copies)
+        auto run_list = [&](int n) {
+            if (n <= 0) return;
+            PairAcc acc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            int eA = list[0];
+            unsigned oA = (unsigned)(eA & 0x0fffffff) << 4;
+            float4 A0 = ldrec(rv.r0, oA), A1 = ldrec(rv.r1, oA), A2 = ldrec(rv.r2, oA), A3 = ldrec(rv.r3, oA);
+            for (int k = 0; k < n; k += 2) {
+                const int eB = list[(k + 1 < n ? k + 1 : n - 1) * FUSED_BS];
+                const unsigned oB = (unsigned)(eB & 0x0fffffff) << 4;
+                const float4 B0 = ldrec(rv.r0, oB), B1 = ldrec(rv.r1, oB), B2 = ldrec(rv.r2, oB),
+                             B3 = ldrec(rv.r3, oB);
+                pair_term(I, hf, inv_wdp, alpha, beta, eta2, A0, A1, A2, A3, eA, acc);
+                if (k + 1 < n) {
+                    eA = list[(k + 2 < n ? k + 2 : n - 1) * FUSED_BS];
+                    oA = (unsigned)(eA & 0x0fffffff) << 4;
+                    A0 = ldrec(rv.r0, oA); A1 = ldrec(rv.r1, oA); A2 = ldrec(rv.r2, oA); A3 = ldrec(rv.r3, oA);
+                    pair_term(I, hf, inv_wdp, alpha, beta, eta2, B0, B1, B2, B3, eB, acc);
+                }
+                if (((k + 2) & 7) == 0) flush(acc);
+            }
+            flush(acc);
+        };
+
+        int n = 0;
+#pragma unroll
+        for (int d = 0; d < 5; d++) {
+            const int col = gi - 2 + d;
+            if (col < 0 || col >= nx) continue;
+            const int c0 = col * ny;
+            int first = cs_lo[d];
+            for (int jr = low_j + 1; first == -1 && jr < high_j; jr++) first = cs[c0 + jr];
+            if (first == -1) continue;                                   // empty window
+            int last = cs_hi[d] != -1 ? ce_hi[d] : -1;
+            for (int jr = high_j - 2; last == -1 && jr >= low_j; jr--)
+                if (cs[c0 + jr] != -1) last = ce[c0 + jr];
+            if (last > N || first < 0) continue;                        // inconsistent cell table
+            const float cofs = (float)(gi - col) * hf;
+            const int tag = d << 29;
+            for (int j = first; j < last; j += 4) {
+                // four scan records; up to three past `last` are read (the scan block is padded)
+                const unsigned off = (unsigned)j << 4;
+                const float4 b0 = ldrec(rv.r0, off), b1 = ldrec(rv.r0, off + 16u), b2 = ldrec(rv.r0, off + 32u),
+                             b3 = ldrec(rv.r0, off + 48u);
+                const float4 bb[4] = {b0, b1, b2, b3};
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const float xij = (I.fxi - bb[u].x) + cofs;
+                    const float yij = (I.fyi - bb[u].y) + (I.rowi - bb[u].z) * hf;
+                    const float rr2 = xij * xij + yij * yij;
+                    if (j + u < last && rr2 < fourh2 && rr2 >= rr2min) {
+                        list[n * FUSED_BS] = (j + u) | tag;
+                        n++;
+                    }
+                }
+                if (n > NB_CHUNK - 4) {                                  // dense region: empty the list
+                    run_list(n);
+                    n = 0;
+                }
+            }
+        }
+        run_list(n);
+    }
+    const double mass = c.phys.mass;
+    psum[i] = {(float)d_v0, (float)d_v1, (float)d_v2, (float)d_v3};
+    psum[(size_t)N + i] = {(float)(mass * d_ax), (float)(mass * d_ay), (float)(-c.corr.xspheps * mass * d_p0),
+                           (float)(-c.corr.xspheps * mass * d_p1)};
+    psum_T[i] = (float)(2. * d_T);
+}
+
 __device__ __forceinline__ double interaction_alpha(const DevConsts &c, double is_tool) {
     // the tool block of interactions_gpu.cu:19 is never set by the FEM setups (benchmarks_NZ.cpp:354)
     return (is_tool == 0.) ? c.trml_wp.alpha : 0.;
@@ -289,18 +533,38 @@ void launch_leapfrog(sphfem_ctx *ctx, int which, const sphfem_particles &p, sphf
 // is dead after the reorder except the previous derivatives.  Outputs: corrected state into
 // A.pos/vel/S/rho/T/eps_pl/eps_pl_dot/blanked, derivatives into B.pos_t/vel_t/rho_t/S_t/T_t,
 // contact fields in place in B.fc/ft/n/on_seg; S_der/v_der into A when requested.
-__global__ void __launch_bounds__(256) k_fused_update_fp64(DevConsts c, sphfem_particles B, sphfem_particles A,
-                                                           const sphfem_double2 *__restrict__ pos_init,
-                                                           const sphfem_double2 *__restrict__ vel_init,
-                                                           const sphfem_double4 *__restrict__ S_init,
-                                                           const double *__restrict__ rho_init,
-                                                           const double *__restrict__ T_init,
-                                                           const int *__restrict__ cs, const int *__restrict__ ce,
-                                                           ToolView tv, int write_intermediates) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+template <bool F32C>
+__global__ void __launch_bounds__(FUSED_BS) k_fused_update(DevConsts c, sphfem_particles B, sphfem_particles A,
+                                                          const sphfem_double2 *__restrict__ pos_init,
+                                                          const sphfem_double2 *__restrict__ vel_init,
+                                                          const sphfem_double4 *__restrict__ S_init,
+                                                          const double *__restrict__ rho_init,
+                                                          const double *__restrict__ T_init,
+                                                          const int *__restrict__ cs, const int *__restrict__ ce,
+                                                          ToolView tv, int write_intermediates,
+                                                          const float4 *__restrict__ psum,
+                                                          const float *__restrict__ psum_T) {
+    const int i = blockIdx.x * FUSED_BS + threadIdx.x;
     if (i >= (int)B.N) return;
     const double blanked0 = B.blanked[i];
     const double tool = B.tool_particle[i];
+    const bool live = blanked0 != 1.;
+    const double thermal_alpha = interaction_alpha(c, tool);
+    // pair sums first: the per-particle state below is not live across the neighbour loop
+    PairSums r;
+    if (live) {
+        if constexpr (F32C) {
+            // sums of k_pairs_fp32c: stress divergence and viscous acceleration come summed per axis
+            const float4 g = psum[i], a = psum[(size_t)B.N + i];
+            r.v_der = {g.x, g.y, g.z, g.w};
+            r.S_der = {a.x, 0., a.y, 0.};
+            r.vel_t = {0., 0.};
+            r.pos_t = {a.z, a.w};
+            r.T_lapl = psum_T[i];
+        } else {
+            r = pair_sums_fp64(c, B, i, thermal_alpha, cs, ce);
+        }
+    }
     const double fixed = B.fixed[i];
     sphfem_double2 pos = B.pos[i], vel = B.vel[i];
     sphfem_double4 S = B.S[i];
@@ -314,10 +578,7 @@ __global__ void __launch_bounds__(256) k_fused_update_fp64(DevConsts c, sphfem_p
     sphfem_double2 pos_t = {0., 0.}, vel_t = {0., 0.};
     sphfem_double4 S_t = {0., 0., 0., 0.};
     double rho_t = 0., T_t = 0.;
-    const bool live = blanked0 != 1.;
     if (live) {
-        const double thermal_alpha = interaction_alpha(c, tool);
-        PairSums r = pair_sums_fp64(c, B, i, thermal_alpha, cs, ce);
         pos_t = r.pos_t;
         vel_t = r.vel_t;
         T_t = thermal_alpha != 0. ? thermal_alpha * r.T_lapl : A.T_t[A.idx[i]];
@@ -425,10 +686,25 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
         tv.near_count = ctx->d_flags + 1;
         tv.active = 1;
     }
-    KTimer tm(ctx, "fused_update");
-    k_fused_update_fp64<<<nblk(B.N, 256), 256, 0, ctx->stream>>>(ctx->c, B, A, lf->pos_init, lf->vel_init,
-                                                                  lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv,
-                                                                  lf->intermediates ? 1 : 0);
+    const int wi = lf->intermediates ? 1 : 0;
+    if (lf->precision == SPHFEM_FP32C) {
+        const size_t n = B.N;
+        RecView rv{lf->rec, lf->rec + n + 4, lf->rec + 2 * n + 4, lf->rec + 3 * n + 4};
+        {
+            KTimer tm(ctx, "pairs_fp32c");
+            k_pairs_fp32c<<<nblk(B.N, FUSED_BS), FUSED_BS, 0, ctx->stream>>>(ctx->c, B.blanked, B.hash, (int)B.N,
+                                                                             rv, cs, ce, lf->psum, lf->psum_T);
+        }
+        KTimer tm(ctx, "update_fp32c");
+        k_fused_update<true><<<nblk(B.N, FUSED_BS), FUSED_BS, 0, ctx->stream>>>(
+            ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
+            lf->psum, lf->psum_T);
+    } else {
+        KTimer tm(ctx, "fused_update");
+        k_fused_update<false><<<nblk(B.N, FUSED_BS), FUSED_BS, 0, ctx->stream>>>(
+            ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
+            nullptr, nullptr);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -304,7 +304,8 @@ __global__ void __launch_bounds__(256) k_reorder_predict(DevConsts c, const sphf
                                                          sphfem_double2 *__restrict__ vel_init,
                                                          sphfem_double4 *__restrict__ S_init,
                                                          double *__restrict__ rho_init, double *__restrict__ T_init,
-                                                         int num_cell, int *__restrict__ flags) {
+                                                         int num_cell, int *__restrict__ flags,
+                                                         float4 *__restrict__ rec) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int N = (int)p.N;
     if (i >= N) return;
@@ -355,16 +356,53 @@ __global__ void __launch_bounds__(256) k_reorder_predict(DevConsts c, const sphf
     b.tool_particle[i] = tool;
     b.fc[i] = p.fc[s];
     b.ft[i] = p.ft[s];
-    b.h[i] = p.h[s];
+    const double hh = p.h[s];
+    b.h[i] = hh;
     b.n[i] = p.n[s];
     b.eps_pl[i] = p.eps_pl[s];
     b.eps_pl_dot[i] = p.eps_pl_dot[s];
     b.fixed[i] = p.fixed[s];
     b.on_seg[i] = p.on_seg[s];
+    sphfem_double4 R = {0., 0., 0., 0.};
+    double pr = 0.;
     if (tool == 0.) {
-        double pr = eos_pressure(c, rho);
+        pr = eos_pressure(c, rho);
+        R = artificial_stress(c, rho, pr, S);
         b.p[i] = pr;
-        b.R[i] = artificial_stress(c, rho, pr, S);
+        b.R[i] = R;
+    }
+    if (rec) {
+        // FP32C neighbour records: positions relative to the cell corner keep ~1e-7 of the cell size;
+        // the stress terms are the per-particle factors of the pair sum, (S - p)/rho^2 and S_xy/rho^2;
+        // four SoA blocks; the first has 4 padding records that absorb four-wide scan loads
+        const int ny = c.geom.ny;
+        const int ix = h / ny, iy = h - ix * ny;
+        const double ox = c.geom.bbmin_x + ix * c.geom.dx;
+        const double oy = c.geom.bbmin_y + iy * c.geom.dx;
+        const double rho21 = 1. / (rho * rho);
+        const float vol = (float)(c.phys.mass / rho);
+        float4 r0, r1, r2, r3;
+        r0.x = blanked == 1. ? 1e30f : (float)(pos.x - ox);     // blanked: never inside a support
+        r0.y = (float)(pos.y - oy);
+        r0.z = (float)iy;
+        r0.w = (float)hh;
+        r1.x = (float)vel.x;
+        r1.y = (float)vel.y;
+        r1.z = tool == 0. ? (float)((S.x - pr) * rho21) : 0.f;
+        r1.w = tool == 0. ? (float)(S.y * rho21) : 0.f;
+        r2.x = tool == 0. ? (float)((S.z - pr) * rho21) : 0.f;
+        r2.y = (float)R.x;
+        r2.z = (float)R.y;
+        r2.w = (float)R.z;
+        r3.x = tool == 1. ? -vol : vol;
+        r3.y = (float)rho;
+        r3.z = (float)sqrtf_d(c.phys.K / rho);
+        r3.w = (float)T;
+        const size_t n4 = (size_t)N + 4;
+        rec[i] = r0;
+        rec[n4 + i] = r1;
+        rec[n4 + (size_t)N + i] = r2;
+        rec[n4 + 2 * (size_t)N + i] = r3;
     }
 }
 
@@ -373,7 +411,8 @@ void launch_reorder_predict(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_partic
     k_reorder_predict<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, g->buf, g->cell_start, g->cell_end,
                                                                lf->pos_init, lf->vel_init, lf->S_init,
                                                                lf->rho_init, lf->T_init, g->num_cell,
-                                                               ctx->d_flags);
+                                                               ctx->d_flags,
+                                                               lf->precision == SPHFEM_FP32C ? lf->rec : nullptr);
 }
 
 }  // namespace sphfem

@@ -85,6 +85,10 @@ class Context:
         self.call("sphfem_timing_get", name.encode(), C.byref(ms), C.byref(n))
         return ms.value, n.value
 
+    def timing_names(self):
+        raw = lib().sphfem_timing_names(self.h)
+        return [n for n in (raw.decode() if raw else "").split(",") if n]
+
 
 class particle_gpu:
     """particle_gpu.h:23-83; constructor as particle_gpu(pos, vel, rho, T, h, fixed, N)."""
@@ -365,3 +369,13 @@ class Simulation:
 
     def tool_state(self):
         return self.tool.download() if self.tool else None
+
+    def close(self):
+        """Release the device objects (stepper, tool, grid, particles) and the context."""
+        self.ctx.synchronize()
+        self.ctx.call("sphfem_leap_frog_destroy", self.stepper.h)
+        if self.tool is not None:
+            self.ctx.call("sphfem_tool_destroy", self.tool.h)
+        self.ctx.call("sphfem_grid_destroy", self.grid.h)
+        self.p.free()
+        self.ctx.close()
