@@ -29,6 +29,20 @@ def interaction_bytes(N, n_occ_cells, w):
     return N * (24 * w + 2) + 8 * n_occ_cells
 
 
+def measured_traffic(kernel, workload_key):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
+    this workload (profiles/traffic_<workload>.json), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload_key}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    prof_name = {"pairs_fp32c": "sphfem::k_pairs_fp32c", "fused_update": "sphfem::k_fused_update<false>",
+                 "update_fp32c": "sphfem::k_fused_update<true>"}.get(kernel)
+    v = t.get("per_launch", {}).get(prof_name)
+    return None if v is None else v["read_bytes"] + v["write_bytes"]
+
+
 def make_case(name):
     from sph_fem_2d_cutting_gpu_amd import cases
     if name == "ti64_4m":
@@ -110,7 +124,19 @@ def main():
     ap.add_argument("--breakdown", action="store_true", help="per-kernel times on stderr")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 256 MiB elementwise copy (y = x * 1, fp32) 5 times: the reference "
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(64 * 1024 * 1024, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": x.numel() * 4, "bytes_written": y.numel() * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the solver has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
@@ -146,7 +172,9 @@ def main():
                    "cutting_segments": int(len(case.tool.seg_left)), "wear": bool(case.wear),
                    "precision": args.precision, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(INTERACTION_KERNEL[args.precision],
+                                                 {"ti64_4m": "ti64_4m"}.get(args.workload, args.workload)),
                      "kernel": INTERACTION_KERNEL[args.precision], "kernel_ms": k_avg_s * 1e3,
                      "algorithmic_bytes": b_int, "occupied_cells": n_occ},
     }

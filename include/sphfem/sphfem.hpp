@@ -1,0 +1,387 @@
+/*
+ * sphfem.hpp -- C++ host mirror of the reference's per-step interface over the C ABI.
+ *
+ * Names and call shapes follow the reference so that a driver written against
+ * particle_gpu.h, grid.h / grid_gpu_green.h, leap_frog.h, tool_FEM.h, actions_gpu.h,
+ * interactions_gpu.h and output.h compiles against this header instead:
+ *
+ *   particle_gpu        particle_gpu.h:23-83 (same field names; the six constructors)
+ *   grid_base / grid_gpu_green   grid.h:23-64, grid_gpu_green.h:84-85
+ *   leap_frog           leap_frog.h:31-50 (step(particles, grid[, tool_FEM]))
+ *   tool / tool_FEM     tool.h:48-58, tool_FEM.h:34-83 (per-step methods)
+ *   free functions      actions_gpu.h:33-72, interactions_gpu.h:27-43, output.h:17
+ *   global_dt, wear_pro benchmarks.cpp:12, mfree_gpu.cpp:47
+ *
+ * Differences a caller sees: errors throw std::runtime_error (the reference prints and
+ * continues); objects release their device memory in destructors; tool_FEM is built from a
+ * host mesh description (sphfem_tool_desc) instead of a Gmsh file; all work is queued on one
+ * process-wide context (device 0 unless sphfem_shim::use_device is called first).
+ * Header-only, C++17; link with libsphfem_hip.so.
+ */
+#ifndef SPHFEM_HPP
+#define SPHFEM_HPP
+
+#include <stdexcept>
+#include <string>
+
+#include "../sphfem.h"
+
+typedef sphfem_double2 float2_t;
+typedef sphfem_double4 float4_t;
+
+struct vec2_t {
+    double x = 0., y = 0.;
+    vec2_t() = default;
+    vec2_t(double a, double b) : x(a), y(b) {}
+};
+
+typedef sphfem_phys_constants phys_constants;
+typedef sphfem_trml_constants trml_constants;
+typedef sphfem_corr_constants corr_constants;
+typedef sphfem_joco_constants joco_constants;
+typedef sphfem_geom_constants geom_constants;
+typedef sphfem_tool_constants tool_constants;
+typedef sphfem_wear_constants wear_constants;
+
+namespace sphfem_shim {
+
+inline int &device_slot() {
+    static int d = 0;
+    return d;
+}
+
+inline sphfem_ctx *&ctx_slot() {
+    static sphfem_ctx *c = nullptr;
+    return c;
+}
+
+inline void check(int rc, const char *what) {
+    if (rc != SPHFEM_OK) {
+        const char *msg = ctx_slot() ? sphfem_last_error(ctx_slot()) : "no context";
+        throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) + "): " + msg);
+    }
+}
+
+// the process-wide context, created on first use
+inline sphfem_ctx *ctx() {
+    if (!ctx_slot()) check(sphfem_create(device_slot(), &ctx_slot()), "sphfem_create");
+    return ctx_slot();
+}
+
+inline void use_device(int device) {
+    if (ctx_slot()) throw std::runtime_error("sphfem_shim::use_device after the context was created");
+    device_slot() = device;
+}
+
+template <class T>
+inline void upload(T *dst, const T *src, unsigned int n) {
+    if (src && n) check(sphfem_memcpy_htod(ctx(), dst, src, sizeof(T) * (size_t)n), "sphfem_memcpy_htod");
+}
+
+}  // namespace sphfem_shim
+
+// benchmarks.cpp:12 and mfree_gpu.cpp:47: read by leap_frog::step on every call
+inline double global_dt = 0.;
+inline bool wear_pro = false;
+
+// ---------------------------------------------------------------------------------------------
+// particle_gpu.h:23-83 -- the device arrays are the fields of sphfem_particles
+// ---------------------------------------------------------------------------------------------
+struct particle_gpu : sphfem_particles {
+    explicit particle_gpu(unsigned int n) {
+        sphfem_shim::check(sphfem_particles_alloc(sphfem_shim::ctx(), n, this), "particle_gpu(N)");
+    }
+    particle_gpu(const float2_t *pos_h, const float2_t *vel_init, const double *rho_h, const double *h_h,
+                 unsigned int n)
+        : particle_gpu(n) {
+        init(pos_h, vel_init, rho_h, nullptr, h_h, nullptr, nullptr, nullptr);
+    }
+    particle_gpu(const float2_t *pos_h, const float2_t *vel_init, const double *rho_h, const double *h_h,
+                 const double *fixed_h, unsigned int n)
+        : particle_gpu(n) {
+        init(pos_h, vel_init, rho_h, nullptr, h_h, fixed_h, nullptr, nullptr);
+    }
+    particle_gpu(const float2_t *pos_h, const float2_t *vel_init, const double *rho_h, const double *T_init,
+                 const double *h_h, const double *fixed_h, unsigned int n)
+        : particle_gpu(n) {
+        init(pos_h, vel_init, rho_h, T_init, h_h, fixed_h, nullptr, nullptr);
+    }
+    particle_gpu(const float2_t *pos_h, const float2_t *vel_init, const double *rho_h, const double *T_init,
+                 const double *h_h, const double *fixed_h, const double *tool_p, unsigned int n)
+        : particle_gpu(n) {
+        init(pos_h, vel_init, rho_h, T_init, h_h, fixed_h, tool_p, nullptr);
+    }
+    particle_gpu(const float2_t *pos_h, const float2_t *vel_init, const double *rho_h, const double *h_h,
+                 const float4_t *S_init, unsigned int n)
+        : particle_gpu(n) {
+        init(pos_h, vel_init, rho_h, nullptr, h_h, nullptr, nullptr, S_init);
+    }
+    ~particle_gpu() {
+        if (sphfem_shim::ctx_slot()) (void)sphfem_particles_free(sphfem_shim::ctx_slot(), this);
+    }
+    particle_gpu(const particle_gpu &) = delete;
+    particle_gpu &operator=(const particle_gpu &) = delete;
+
+  private:
+    void init(const float2_t *pos_h, const float2_t *vel_h, const double *rho_h, const double *T_h,
+              const double *h_h, const double *fixed_h, const double *tool_h, const float4_t *S_h) {
+        using sphfem_shim::upload;
+        upload(pos, pos_h, N);
+        upload(vel, vel_h, N);
+        upload(rho, rho_h, N);
+        upload(T, T_h, N);
+        upload(h, h_h, N);
+        upload(fixed, fixed_h, N);
+        upload(tool_particle, tool_h, N);
+        upload(S, S_h, N);
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// tool.h:31-58 / tool_FEM.h:30-83
+// ---------------------------------------------------------------------------------------------
+class tool {
+  public:
+    virtual ~tool() = default;
+    virtual void set_vel(vec2_t vel) { m_velocity = vel; }
+    vec2_t get_vel() const { return m_velocity; }
+
+  protected:
+    vec2_t m_velocity;
+};
+
+// the segments live inside the tool object; the type exists for signature compatibility
+struct segment_FEM_gpu;
+
+class tool_FEM : public tool {
+  public:
+    // the device data of set_up_segments_from_mesh + set_up_FEM_tool_gpu, from a host mesh
+    explicit tool_FEM(const sphfem_tool_desc &desc) {
+        sphfem_shim::check(sphfem_tool_create(sphfem_shim::ctx(), &desc, &m_handle), "tool_FEM");
+        node_number = desc.node_num;
+        element_number = desc.ele_num;
+        cutting_segment_size = desc.seg_num;
+        m_velocity = vec2_t(desc.vel.x, desc.vel.y);
+    }
+    ~tool_FEM() override {
+        if (m_handle && sphfem_shim::ctx_slot()) (void)sphfem_tool_destroy(sphfem_shim::ctx_slot(), m_handle);
+    }
+    tool_FEM(const tool_FEM &) = delete;
+    tool_FEM &operator=(const tool_FEM &) = delete;
+
+    int node_number = 0;
+    int element_number = 0;
+    int cutting_segment_size = 0;
+    segment_FEM_gpu *segments_FEM = nullptr;
+
+    void set_vel(vec2_t vel) override {
+        m_velocity = vel;
+        sphfem_shim::check(sphfem_tool_set_vel(sphfem_shim::ctx(), m_handle, sphfem_double2{vel.x, vel.y}),
+                           "tool_FEM::set_vel");
+    }
+    void apply_heat_boundary_condition() {
+        sphfem_shim::check(sphfem_tool_apply_heat_boundary_condition(sphfem_shim::ctx(), m_handle),
+                           "apply_heat_boundary_condition");
+    }
+    void tool_update_FEM_gpu(double dt) {
+        sphfem_shim::check(sphfem_tool_update_FEM_gpu(sphfem_shim::ctx(), m_handle, dt), "tool_update_FEM_gpu");
+    }
+    void segments_update(double dt) {
+        sphfem_shim::check(sphfem_tool_segments_update(sphfem_shim::ctx(), m_handle, dt), "segments_update");
+    }
+    void segments_wear_rate_reset() {
+        sphfem_shim::check(sphfem_tool_segments_wear_rate_reset(sphfem_shim::ctx(), m_handle),
+                           "segments_wear_rate_reset");
+    }
+    sphfem_tool_kinematics kinematics() const {
+        sphfem_tool_kinematics k{};
+        sphfem_shim::check(sphfem_tool_get_kinematics(sphfem_shim::ctx(), m_handle, &k), "tool kinematics");
+        return k;
+    }
+    sphfem_tool *handle() const { return m_handle; }
+
+  private:
+    sphfem_tool *m_handle = nullptr;
+};
+
+// ---------------------------------------------------------------------------------------------
+// grid.h:23-64, grid_gpu_green.h:84-85 (locked-geometry constructor)
+// ---------------------------------------------------------------------------------------------
+class grid_base {
+  public:
+    virtual ~grid_base() {
+        if (m_handle && sphfem_shim::ctx_slot()) (void)sphfem_grid_destroy(sphfem_shim::ctx_slot(), m_handle);
+    }
+    int nx() const { return m_geom.nx; }
+    int ny() const { return m_geom.ny; }
+    double bbmin_x() const { return m_geom.bbmin_x; }
+    double bbmin_y() const { return m_geom.bbmin_y; }
+    double dx() const { return m_geom.dx; }
+    bool is_locked() const { return true; }
+    int num_cell() const { return m_num_cell; }
+
+    void assign_hashes(particle_gpu *particles, tool *) const {
+        sphfem_shim::check(sphfem_grid_assign_hashes(sphfem_shim::ctx(), m_handle, particles), "assign_hashes");
+    }
+    virtual void sort(particle_gpu *particles, tool *t) const = 0;
+    virtual void get_cells(particle_gpu *particles, int *cell_start, int *cell_end) = 0;
+    sphfem_grid *handle() const { return m_handle; }
+
+  protected:
+    grid_base() = default;
+    void adopt(sphfem_grid *g) {
+        m_handle = g;
+        sphfem_shim::check(sphfem_grid_geometry(g, &m_geom, &m_num_cell), "grid geometry");
+    }
+    sphfem_grid *m_handle = nullptr;
+    geom_constants m_geom{};
+    int m_num_cell = 0;
+};
+
+class grid_gpu_green : public grid_base {
+  public:
+    grid_gpu_green(int num_part, float2_t bbmin, float2_t bbmax, double h) {
+        sphfem_grid *g = nullptr;
+        sphfem_shim::check(sphfem_grid_green_create(sphfem_shim::ctx(), num_part, bbmin, bbmax, h, &g),
+                           "grid_gpu_green");
+        adopt(g);
+    }
+    void sort(particle_gpu *particles, tool *) const override {
+        sphfem_shim::check(sphfem_grid_sort(sphfem_shim::ctx(), m_handle, particles), "grid_gpu_green::sort");
+    }
+    void get_cells(particle_gpu *, int *cell_start, int *cell_end) override {
+        sphfem_shim::check(sphfem_grid_get_cells(sphfem_shim::ctx(), m_handle, cell_start, cell_end),
+                           "grid_gpu_green::get_cells");
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// leap_frog.h:31-50
+// ---------------------------------------------------------------------------------------------
+class leap_frog {
+  public:
+    leap_frog(unsigned int num_part, unsigned int num_cell) : leap_frog(num_part, num_cell, 0) {}
+    leap_frog(unsigned int num_part, unsigned int num_cell, int node_num_in_FEM_tool) {
+        sphfem_shim::check(sphfem_leap_frog_create(sphfem_shim::ctx(), num_part, num_cell, node_num_in_FEM_tool,
+                                                   &m_handle),
+                           "leap_frog");
+    }
+    ~leap_frog() {
+        if (m_handle && sphfem_shim::ctx_slot()) (void)sphfem_leap_frog_destroy(sphfem_shim::ctx_slot(), m_handle);
+    }
+    leap_frog(const leap_frog &) = delete;
+    leap_frog &operator=(const leap_frog &) = delete;
+
+    // SPHFEM_FP64 (default) or SPHFEM_FP32C
+    void set_precision(int precision) {
+        sphfem_shim::check(sphfem_leap_frog_set_precision(sphfem_shim::ctx(), m_handle, precision),
+                           "leap_frog::set_precision");
+    }
+    // leap_frog.cu:162-248, with global_dt and wear_pro read as the reference reads them
+    void step(particle_gpu *particles, grid_base *g, tool_FEM *m_tool) {
+        sphfem_ctx *c = sphfem_shim::ctx();
+        sphfem_shim::check(sphfem_set_dt(c, global_dt), "set_dt");
+        sphfem_shim::check(sphfem_set_wear(c, wear_pro ? 1 : 0), "set_wear");
+        sphfem_shim::check(sphfem_leap_frog_step(c, m_handle, particles, g->handle(),
+                                                 m_tool ? m_tool->handle() : nullptr),
+                           "leap_frog::step");
+    }
+    void step(particle_gpu *particles, grid_base *g) { step(particles, g, nullptr); }
+
+  private:
+    sphfem_leap_frog *m_handle = nullptr;
+};
+
+// ---------------------------------------------------------------------------------------------
+// actions_gpu.h:33-72, interactions_gpu.h:27-43: the constant blocks are shared by all kernels
+// ---------------------------------------------------------------------------------------------
+#define SPHFEM_SHIM_CALL(expr, name) sphfem_shim::check((expr), name)
+
+inline void actions_setup_physical_constants(phys_constants c) {
+    SPHFEM_SHIM_CALL(sphfem_setup_physical_constants(sphfem_shim::ctx(), &c), "setup_physical_constants");
+}
+inline void actions_setup_corrector_constants(corr_constants c) {
+    SPHFEM_SHIM_CALL(sphfem_setup_corrector_constants(sphfem_shim::ctx(), &c), "setup_corrector_constants");
+}
+inline void actions_setup_johnson_cook_constants(joco_constants c) {
+    SPHFEM_SHIM_CALL(sphfem_setup_johnson_cook_constants(sphfem_shim::ctx(), &c), "setup_johnson_cook_constants");
+}
+inline void actions_setup_thermal_constants_wp(trml_constants c) {
+    SPHFEM_SHIM_CALL(sphfem_setup_thermal_constants_wp(sphfem_shim::ctx(), &c), "setup_thermal_constants_wp");
+}
+inline void actions_setup_thermal_constants_tool(trml_constants c) {
+    SPHFEM_SHIM_CALL(sphfem_setup_thermal_constants_tool(sphfem_shim::ctx(), &c), "setup_thermal_constants_tool");
+}
+inline void actions_setup_tool_constants(tool_constants c) {
+    SPHFEM_SHIM_CALL(sphfem_setup_tool_constants(sphfem_shim::ctx(), &c), "setup_tool_constants");
+}
+inline void actions_setup_wear_constants(wear_constants c) {
+    SPHFEM_SHIM_CALL(sphfem_setup_wear_constants(sphfem_shim::ctx(), &c), "setup_wear_constants");
+}
+inline void interactions_setup_physical_constants(phys_constants c) { actions_setup_physical_constants(c); }
+inline void interactions_setup_corrector_constants(corr_constants c) { actions_setup_corrector_constants(c); }
+inline void interactions_setup_thermal_constants_workpiece(trml_constants c) { actions_setup_thermal_constants_wp(c); }
+inline void interactions_setup_tool_constants(tool_constants c) { actions_setup_tool_constants(c); }
+inline void interactions_setup_geometry_constants(grid_base *g) {
+    SPHFEM_SHIM_CALL(sphfem_interactions_setup_geometry_constants(sphfem_shim::ctx(), g->handle()),
+                     "interactions_setup_geometry_constants");
+}
+inline void interactions_monaghan(particle_gpu *p, const int *cell_start, const int *cell_end, int num_cell) {
+    SPHFEM_SHIM_CALL(sphfem_interactions_monaghan(sphfem_shim::ctx(), p, cell_start, cell_end, num_cell),
+                     "interactions_monaghan");
+}
+
+inline void material_eos(particle_gpu *p) { SPHFEM_SHIM_CALL(sphfem_material_eos(sphfem_shim::ctx(), p), "material_eos"); }
+inline void corrector_artificial_stress(particle_gpu *p) {
+    SPHFEM_SHIM_CALL(sphfem_corrector_artificial_stress(sphfem_shim::ctx(), p), "corrector_artificial_stress");
+}
+inline void material_stress_rate_jaumann(particle_gpu *p) {
+    SPHFEM_SHIM_CALL(sphfem_material_stress_rate_jaumann(sphfem_shim::ctx(), p), "material_stress_rate_jaumann");
+}
+inline void contmech_continuity(particle_gpu *p) {
+    SPHFEM_SHIM_CALL(sphfem_contmech_continuity(sphfem_shim::ctx(), p), "contmech_continuity");
+}
+inline void contmech_momentum(particle_gpu *p) {
+    SPHFEM_SHIM_CALL(sphfem_contmech_momentum(sphfem_shim::ctx(), p), "contmech_momentum");
+}
+inline void contmech_advection(particle_gpu *p) {
+    SPHFEM_SHIM_CALL(sphfem_contmech_advection(sphfem_shim::ctx(), p), "contmech_advection");
+}
+inline void plasticity_johnson_cook(particle_gpu *p) {
+    SPHFEM_SHIM_CALL(sphfem_plasticity_johnson_cook(sphfem_shim::ctx(), p), "plasticity_johnson_cook");
+}
+inline void perform_boundary_conditions(particle_gpu *p) {
+    SPHFEM_SHIM_CALL(sphfem_perform_boundary_conditions(sphfem_shim::ctx(), p), "perform_boundary_conditions");
+}
+inline void perform_boundary_conditions_thermal(particle_gpu *p) {
+    SPHFEM_SHIM_CALL(sphfem_perform_boundary_conditions_thermal(sphfem_shim::ctx(), p),
+                     "perform_boundary_conditions_thermal");
+}
+inline void debug_invalidate(particle_gpu *p) {
+    SPHFEM_SHIM_CALL(sphfem_debug_invalidate(sphfem_shim::ctx(), p), "debug_invalidate");
+}
+inline void material_fric_heat_gen_FEM_v2(particle_gpu *p, segment_FEM_gpu *, vec2_t, tool_FEM *t) {
+    SPHFEM_SHIM_CALL(sphfem_material_fric_heat_gen_FEM_v2(sphfem_shim::ctx(), p, t->handle()),
+                     "material_fric_heat_gen_FEM_v2");
+}
+inline void compute_contact_forces_FEM_tool_v3(particle_gpu *p, segment_FEM_gpu *, tool_FEM *t) {
+    SPHFEM_SHIM_CALL(sphfem_compute_contact_forces_FEM_tool_v3(sphfem_shim::ctx(), p, t->handle()),
+                     "compute_contact_forces_FEM_tool_v3");
+}
+inline void wear_adjustment(tool_FEM *t) {
+    SPHFEM_SHIM_CALL(sphfem_wear_adjustment(sphfem_shim::ctx(), t->handle()), "wear_adjustment");
+}
+
+// output.h:17
+inline float2_t report_force(particle_gpu *p) {
+    float2_t f{0., 0.};
+    SPHFEM_SHIM_CALL(sphfem_report_force(sphfem_shim::ctx(), p, &f), "report_force");
+    return f;
+}
+
+// types.cpp:11-37
+inline bool check_cuda_error() { return sphfem_check_error(sphfem_shim::ctx()) == SPHFEM_OK; }
+
+#undef SPHFEM_SHIM_CALL
+
+#endif /* SPHFEM_HPP */

@@ -1,4 +1,6 @@
 """Workload generator: determinism, SplitMix64, lattice, tool mesh and FEM set-up invariants."""
+import os
+
 import numpy as np
 import pytest
 
@@ -106,3 +108,21 @@ def test_chip_workload_density_ratio():
     d, _ = cKDTree(c.pos).query(c.pos, k=2)
     nn = d[:, 1]
     assert nn.max() / nn.min() > 2.5
+
+
+def test_case_dir_for_the_cpp_driver(tmp_path):
+    """case_io writes plain `key float` lines and raw arrays of the right sizes."""
+    from sph_fem_2d_cutting_gpu_amd import case_io
+    c = cases.small_cutting_case(nx=16, ny=8)
+    d = case_io.write_case_dir(c, str(tmp_path))
+    sc = {}
+    for line in open(tmp_path / "scalars.txt"):
+        k, v = line.split()
+        sc[k] = float(v)                      # what the driver's `>> double` accepts
+    for k in ("N", "dt", "h", "bbmin_x", "bbmax_y", "rr2_min", "phys.mass", "joco.tanh_d", "tool.ref_dist",
+              "node_num", "seg_num"):
+        assert k in sc, k
+    assert sc["N"] == c.N
+    assert os.path.getsize(os.path.join(d, "p_pos.bin")) == 16 * c.N
+    assert os.path.getsize(os.path.join(d, "t_row_ptr.bin")) == 4 * (int(sc["node_num"]) + 1)
+    assert os.path.getsize(os.path.join(d, "t_marks.bin")) == 16 * int(sc["seg_num"])
