@@ -146,6 +146,17 @@ struct KTimer {
 
 inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of the chip, each with its own L2.
+// Map the hardware block id to a logical one so that each XCD works on one contiguous run of
+// logical blocks (neighbouring particles, which share neighbour records, stay in one L2).
+// Bijective for any grid size.
+__device__ __forceinline__ int xcd_logical_block() {
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, local = b >> 3;
+    return x < r ? x * (q + 1) + local : r * (q + 1) + (x - r) * q + local;
+}
+
 // sort / cells (sphfem_sort.hip)
 void launch_hash(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, bool clear_old, int *hist0);
 void launch_radix_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool hash_done_with_hist0);

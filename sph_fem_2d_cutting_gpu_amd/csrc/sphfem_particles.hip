@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(FUSED_BS) k_pairs_fp32c(DevConsts c, const dou
                                                          float4 *__restrict__ psum, float *__restrict__ psum_T) {
 #pragma clang fp contract(fast)
     __shared__ int nb_list[NB_CHUNK * FUSED_BS];
-    const int i = blockIdx.x * FUSED_BS + threadIdx.x;
+    const int i = xcd_logical_block() * FUSED_BS + threadIdx.x;
     if (i >= N) return;
     int *__restrict__ list = nb_list + threadIdx.x;
     double d_v0 = 0., d_v1 = 0., d_v2 = 0., d_v3 = 0.;
@@ -544,7 +544,7 @@ __global__ void __launch_bounds__(FUSED_BS) k_fused_update(DevConsts c, sphfem_p
                                                           ToolView tv, int write_intermediates,
                                                           const float4 *__restrict__ psum,
                                                           const float *__restrict__ psum_T) {
-    const int i = blockIdx.x * FUSED_BS + threadIdx.x;
+    const int i = xcd_logical_block() * FUSED_BS + threadIdx.x;
     if (i >= (int)B.N) return;
     const double blanked0 = B.blanked[i];
     const double tool = B.tool_particle[i];

@@ -325,6 +325,151 @@ __device__ __forceinline__ int contact_s1_near(const DevConsts &c, const ToolKin
     return seg_no;
 }
 
+// ---- wavefront-cooperative form of contact s1: one particle per wavefront, segments over lanes.
+// Every lane must execute these (uniform control flow over the wavefront).
+__device__ __forceinline__ int wave_sum_int(int v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// lexicographic minimum of (d, i) over the wavefront; i = -1 sorts last among equal d
+__device__ __forceinline__ void wave_argmin(double &d, int &i) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const double od = __shfl_xor(d, off);
+        const int oi = __shfl_xor(i, off);
+        if (od < d || (od == d && (unsigned)oi < (unsigned)i)) {
+            d = od;
+            i = oi;
+        }
+    }
+}
+
+__device__ __forceinline__ int wave_min_int(int v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(v, off);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// find_shortest_distance_v1 (actions_gpu.cu:139-220) over the lanes.  Each loop of the reference
+// keeps the first segment that attains the minimum (strict <), which is the lexicographic
+// (distance, index) minimum computed here; the normal-averaging loop takes the lowest index j with
+// left[j] == right[seg].  Same arithmetic per segment, so the result is identical.
+__device__ __forceinline__ int find_shortest_distance_wave(sphfem_double2 qp, double &dN, double &nre_x,
+                                                           double &nre_y, const sphfem_double2 *left,
+                                                           const sphfem_double2 *right, const sphfem_double2 *n,
+                                                           int num_seg, int lane) {
+    const double limit_value = 2.5e-4;
+    double depth = SPHFEM_FLT_MAX;
+    int seg = -1;
+    for (int i = lane; i < num_seg; i += 64) {
+        sphfem_double2 l = left[i], r = right[i];
+        double ymax = find_max(l.y, r.y);
+        double ymin = find_min(l.y, r.y);
+        double xmax = find_max(l.x, r.x);
+        double xmin = find_min(l.x, r.x);
+        if (qp.x > xmin - 0.0008 && qp.x < xmax + 0.0008 && qp.y > ymin - 0.0008 && qp.y < ymax + 0.0008) {
+            sphfem_double2 ni = n[i];
+            double d = (qp.x - l.x) * ni.x + (qp.y - l.y) * ni.y;
+            if (d >= 0. && d < depth) {
+                double ix = qp.x - d * ni.x;
+                double iy = qp.y - d * ni.y;
+                if (iy <= ymax && iy >= ymin && ix <= xmax && ix >= xmin) {
+                    depth = d;
+                    seg = i;
+                }
+            }
+        }
+    }
+    wave_argmin(depth, seg);
+    if (seg != -1 && depth <= limit_value) {
+        nre_x = -n[seg].x;
+        nre_y = -n[seg].y;
+        dN = -depth;
+        return seg;
+    }
+    depth = SPHFEM_FLT_MAX;
+    seg = -1;
+    for (int i = lane; i < num_seg; i += 64) {
+        sphfem_double2 r = right[i];
+        double d_trial = sqrtf_d((qp.x - r.x) * (qp.x - r.x) + (qp.y - r.y) * (qp.y - r.y));
+        if (d_trial < depth) {
+            depth = d_trial;
+            seg = i;
+        }
+    }
+    wave_argmin(depth, seg);
+    if (seg == -1) return seg;
+    const sphfem_double2 rs = right[seg];
+    int jmatch = 0x7fffffff;
+    for (int j = lane; j < num_seg; j += 64) {
+        if (left[j].y == rs.y && left[j].x == rs.x) {
+            jmatch = j;
+            break;
+        }
+    }
+    jmatch = wave_min_int(jmatch);
+    double nsx = n[seg].x, nsy = n[seg].y;
+    if (jmatch != 0x7fffffff) {
+        nsx = (n[jmatch].x + n[seg].x) / 2.;
+        nsy = (n[jmatch].y + n[seg].y) / 2.;
+    }
+    double n_length = sqrtf_d(nsx * nsx + nsy * nsy);
+    nre_x = -nsx / n_length;
+    nre_y = -nsy / n_length;
+    dN = -1.0 * depth;
+    return seg;
+}
+
+// contact_s1_near with the segment loops spread over the wavefront's lanes; every lane returns the
+// same result
+__device__ __forceinline__ int contact_s1_near_wave(const DevConsts &c, const ToolKin &kin, sphfem_double2 pos,
+                                                    sphfem_double2 vel, sphfem_double2 &fc, sphfem_double2 &ft,
+                                                    sphfem_double2 &nrm, bool &blank, const sphfem_double2 *left,
+                                                    const sphfem_double2 *right, const sphfem_double2 *n, int n_seg,
+                                                    int lane) {
+    blank = false;
+    int cnt = 0;
+    for (int i = lane; i < n_seg; i += 64) cnt += on_segment_mesh(left[i], right[i], pos);
+    cnt = wave_sum_int(cnt);
+    sphfem_double2 z = {0., 0.};
+    if (cnt % 2 == 0) {
+        fc = z; ft = z; nrm = z;
+        return -1;
+    }
+    double gN = 0., nx_ = 0., ny_ = 0.;
+    int seg_no = find_shortest_distance_wave(pos, gN, nx_, ny_, left, right, n, n_seg, lane);
+    if (seg_no == -1 || gN < -0.00025) {
+        if (seg_no != -1) blank = true;
+        fc = z; ft = z; nrm = z;
+        return -1;
+    }
+    const double dt = c.dt, m = c.tc.slave_mass, ca = c.tc.contact_alpha, cof = c.tc.mu;
+    double dt2 = dt * dt;
+    double fNx = -m * gN * nx_ / dt2 * ca;
+    double fNy = -m * gN * ny_ / dt2 * ca;
+    double vx = vel.x - kin.vel_x, vy = vel.y - kin.vel_y;
+    double vdn = vx * nx_ + vy * ny_;
+    double vrx = vx - nx_ * vdn, vry = vy - ny_ * vdn;
+    double kx = ca * m * vrx / dt, ky = ca * m * vry / dt;
+    double fy = cof * sqrt(fNx * fNx + fNy * fNy);
+    double fsx = ft.x - kx, fsy = ft.y - ky;
+    double lfs = sqrt(fsx * fsx + fsy * fsy);
+    double fTx, fTy;
+    if (lfs > fy) {
+        fTx = fy * fsx / lfs;
+        fTy = fy * fsy / lfs;
+    } else {
+        fTx = fsx;
+        fTy = fsy;
+    }
+    fc.x = fNx; fc.y = fNy;
+    ft.x = fTx; ft.y = fTy;
+    nrm.x = nx_; nrm.y = ny_;
+    return seg_no;
+}
+
 __device__ __forceinline__ bool near_tool(const ToolKin &kin, sphfem_double2 pos) {
     double ddx = pos.x - kin.tl_x, ddy = pos.y - kin.tl_y;
     return sqrt(ddx * ddx + ddy * ddy) < 1.05 * kin.ref_dist;

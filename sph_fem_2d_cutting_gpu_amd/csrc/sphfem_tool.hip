@@ -426,13 +426,19 @@ void launch_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t) 
 __global__ void __launch_bounds__(256) k_near_contact(DevConsts c, sphfem_particles P, sphfem_tool t,
                                                       const int *__restrict__ near_list, int *counters,
                                                       int *contact_list) {
+    // one particle of the near list per wavefront (wave-uniform loop); the segment loops of
+    // contact s1 run over the lanes
     const int count = counters[1];
     const ToolKin k = *t.kin;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < count; q += nwaves) {
         const int i = near_list[q];
         sphfem_double2 fc, ft = P.ft[i], nrm;
         bool blank = false;
-        int seg = contact_s1_near(c, k, P.pos[i], P.vel[i], fc, ft, nrm, blank, t.left, t.right, t.n, t.n_seg);
+        const int seg = contact_s1_near_wave(c, k, P.pos[i], P.vel[i], fc, ft, nrm, blank, t.left, t.right, t.n,
+                                             t.n_seg, lane);
+        if (lane != 0) continue;
         double blanked = blank ? 1. : P.blanked[i];
         sphfem_double2 pos_t = P.pos_t[i], vel_t = P.vel_t[i];
         if (P.tool_particle[i] == 0. && P.fixed[i] != 0.) {
