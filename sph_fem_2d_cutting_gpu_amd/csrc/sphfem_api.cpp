@@ -891,7 +891,13 @@ int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particle
         B.idx = p->idx;
         B.N = p->N;
         launch_fused_update(ctx, B, *p, lf, g->cell_start, g->cell_end, t);
-        swap_fused(*p, g->buf);
+        if (lf->precision == SPHFEM_FP32C) {
+            // the whole new state is in the buffer; R is not maintained by the FP32C form
+            swap_gathered(*p, g->buf);
+            std::swap(p->p, g->buf.p);
+        } else {
+            swap_fused(*p, g->buf);
+        }
         if (t) {
             launch_tool_fem_end(ctx, t);
             if (ctx->c.wear_pro) launch_wear(ctx, t);

@@ -566,13 +566,19 @@ __global__ void __launch_bounds__(FUSED_BS) k_fused_update(DevConsts c, sphfem_p
         }
     }
     const double fixed = B.fixed[i];
-    sphfem_double2 pos = B.pos[i], vel = B.vel[i];
+    // FP32C: the reorder did not write the predicted position and temperature (the pair sums read
+    // the FP32 records); the pre-step state is read through idx from the unsorted arrays A, and the
+    // corrected state is written in place into B (then every array is swapped)
+    const int src = F32C ? B.idx[i] : i;
+    sphfem_double2 pos = F32C ? A.pos[src] : B.pos[i], vel = B.vel[i];
     sphfem_double4 S = B.S[i];
-    double rho = B.rho[i], T = B.T[i];
+    double rho = B.rho[i], T = F32C ? A.T[src] : B.T[i];
     double eps_pl = B.eps_pl[i], eps_pl_dot = B.eps_pl_dot[i];
     double blanked = blanked0;
-    sphfem_double2 fc = B.fc[i], ft = B.ft[i], nrm = B.n[i];
-    int on_seg = B.on_seg[i];
+    const sphfem_double2 fc0 = B.fc[i], ft0 = B.ft[i], nrm0 = B.n[i];
+    const int on_seg0 = B.on_seg[i];
+    sphfem_double2 fc = fc0, ft = ft0, nrm = nrm0;
+    int on_seg = on_seg0;
     bool contact_written = false;
     bool near = false;
     sphfem_double2 pos_t = {0., 0.}, vel_t = {0., 0.};
@@ -597,8 +603,9 @@ __global__ void __launch_bounds__(FUSED_BS) k_fused_update(DevConsts c, sphfem_p
         }
         // leap_frog.cu:71-96
         const double dt = c.dt;
-        sphfem_double2 p0 = pos_init[i], v0 = vel_init[i];
-        sphfem_double4 S0 = S_init[i];
+        const sphfem_double2 p0 = F32C ? pos : pos_init[i], v0 = F32C ? A.vel[src] : vel_init[i];
+        const sphfem_double4 S0 = F32C ? A.S[src] : S_init[i];
+        const double rho0 = F32C ? A.rho[src] : rho_init[i], T0 = F32C ? T : T_init[i];
         pos.x = p0.x + dt * pos_t.x;
         pos.y = p0.y + dt * pos_t.y;
         vel.x = v0.x + dt * vel_t.x;
@@ -607,8 +614,8 @@ __global__ void __launch_bounds__(FUSED_BS) k_fused_update(DevConsts c, sphfem_p
         S.y = S0.y + dt * S_t.y;
         S.z = S0.z + dt * S_t.z;
         S.w = S0.w + dt * S_t.w;
-        rho = rho_init[i] + dt * rho_t;
-        T = T_init[i] + dt * T_t;
+        rho = rho0 + dt * rho_t;
+        T = T0 + dt * T_t;
         if (tv.active) {
             if (!fric_heat_v21(c, tv, *tv.kin, eps_pl, ft, on_seg, nrm, vel, fc, rho, T)) blanked = 1.;
         }
@@ -656,15 +663,18 @@ __global__ void __launch_bounds__(FUSED_BS) k_fused_update(DevConsts c, sphfem_p
     B.S_t[i] = S_t;
     B.rho_t[i] = rho_t;
     B.T_t[i] = T_t;
-    A.pos[i] = pos;
-    A.vel[i] = vel;
-    A.S[i] = S;
-    A.rho[i] = rho;
-    A.T[i] = T;
-    A.eps_pl[i] = eps_pl;
-    A.eps_pl_dot[i] = eps_pl_dot;
-    A.blanked[i] = blanked;
-    if (contact_written) {
+    sphfem_particles &O = F32C ? B : A;
+    O.pos[i] = pos;
+    O.vel[i] = vel;
+    O.S[i] = S;
+    O.rho[i] = rho;
+    O.T[i] = T;
+    O.eps_pl[i] = eps_pl;
+    O.eps_pl_dot[i] = eps_pl_dot;
+    O.blanked[i] = blanked;
+    // contact fields in place, written only when they change (far from the tool they stay zero)
+    if (contact_written && (fc.x != fc0.x || fc.y != fc0.y || ft.x != ft0.x || ft.y != ft0.y || nrm.x != nrm0.x ||
+                            nrm.y != nrm0.y || on_seg != on_seg0)) {
         B.fc[i] = fc;
         B.ft[i] = ft;
         B.n[i] = nrm;

@@ -326,13 +326,19 @@ __global__ void __launch_bounds__(256) k_reorder_predict(DevConsts c, const sphf
     sphfem_double4 S = p.S[s];
     double rho = p.rho[s], T = p.T[s];
     double blanked = p.blanked[s], tool = p.tool_particle[s];
+    // FP32C (rec != null): the update kernel reads the pre-step state through idx, and the pair
+    // sums read position and temperature from the records, so the init copies and the FP64
+    // predicted position, temperature and artificial stress are not written
+    const bool f32c = rec != nullptr;
     if (blanked != 1.) {
         const double dt = c.dt;
-        pos_init[i] = pos;
-        vel_init[i] = vel;
-        S_init[i] = S;
-        rho_init[i] = rho;
-        T_init[i] = T;
+        if (!f32c) {
+            pos_init[i] = pos;
+            vel_init[i] = vel;
+            S_init[i] = S;
+            rho_init[i] = rho;
+            T_init[i] = T;
+        }
         sphfem_double2 pt = p.pos_t[s], vt = p.vel_t[s];
         sphfem_double4 St = p.S_t[s];
         double rt = p.rho_t[s], Tt = p.T_t[s];
@@ -347,11 +353,11 @@ __global__ void __launch_bounds__(256) k_reorder_predict(DevConsts c, const sphf
         rho = rho + 0.5 * dt * rt;
         T = T + 0.5 * dt * Tt;
     }
-    b.pos[i] = pos;
+    if (!f32c) b.pos[i] = pos;
     b.vel[i] = vel;
     b.S[i] = S;
     b.rho[i] = rho;
-    b.T[i] = T;
+    if (!f32c) b.T[i] = T;
     b.blanked[i] = blanked;
     b.tool_particle[i] = tool;
     b.fc[i] = p.fc[s];
@@ -369,7 +375,7 @@ __global__ void __launch_bounds__(256) k_reorder_predict(DevConsts c, const sphf
         pr = eos_pressure(c, rho);
         R = artificial_stress(c, rho, pr, S);
         b.p[i] = pr;
-        b.R[i] = R;
+        if (!f32c) b.R[i] = R;
     }
     if (rec) {
         // FP32C neighbour records: positions relative to the cell corner keep ~1e-7 of the cell size;
