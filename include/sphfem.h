@@ -132,6 +132,14 @@ int sphfem_grid_geometry(const sphfem_grid *g, sphfem_geom_constants *geom, int 
 int sphfem_grid_assign_hashes(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles *p);
 /* grid_gpu_green.cu:184-213: stable radix sort, cell boundaries, gather of 20 arrays, pointer swap */
 int sphfem_grid_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles *p);
+/* The step re-sorts incrementally when at most max_changed particles changed cells since the
+ * previous step (the result equals the full stable sort); otherwise, and on the first step, it
+ * runs the full radix sort.  0 always runs the full sort.  Values above 4096 are clamped.
+ * No counterpart in the reference (a tuning knob of this implementation). */
+int sphfem_grid_set_resort_limit(sphfem_ctx *ctx, sphfem_grid *g, int max_changed);
+/* the last step's re-sort: mode 1 = full radix sort, 2 = incremental; changed = particles that
+ * changed cells (blocks until the stream is idle) */
+int sphfem_grid_resort_stats(sphfem_ctx *ctx, int *mode, int *changed);
 /* grid_gpu_green.cu:215-218: device-to-device copy into caller arrays [num_cell] */
 int sphfem_grid_get_cells(sphfem_ctx *ctx, sphfem_grid *g, int *cell_start, int *cell_end);
 

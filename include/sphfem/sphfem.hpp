@@ -249,6 +249,11 @@ class grid_gpu_green : public grid_base {
     void sort(particle_gpu *particles, tool *) const override {
         sphfem_shim::check(sphfem_grid_sort(sphfem_shim::ctx(), m_handle, particles), "grid_gpu_green::sort");
     }
+    // incremental re-sort limit of the step (see sphfem_grid_set_resort_limit)
+    void set_resort_limit(int max_changed) {
+        sphfem_shim::check(sphfem_grid_set_resort_limit(sphfem_shim::ctx(), m_handle, max_changed),
+                           "grid_gpu_green::set_resort_limit");
+    }
     void get_cells(particle_gpu *, int *cell_start, int *cell_end) override {
         sphfem_shim::check(sphfem_grid_get_cells(sphfem_shim::ctx(), m_handle, cell_start, cell_end),
                            "grid_gpu_green::get_cells");

@@ -87,6 +87,8 @@ _SIGS = {
     "sphfem_particles_free": (_ctx, _P(Particles)),
     "sphfem_grid_green_create": (_ctx, C.c_int, Double2, Double2, _dbl, _P(_vp)),
     "sphfem_grid_destroy": (_ctx, _vp),
+    "sphfem_grid_set_resort_limit": (_ctx, _vp, C.c_int),
+    "sphfem_grid_resort_stats": (_ctx, _P(C.c_int), _P(C.c_int)),
     "sphfem_grid_geometry": (_vp, _P(GeomConstants), _P(C.c_int)),
     "sphfem_grid_assign_hashes": (_ctx, _vp, _P(Particles)),
     "sphfem_grid_sort": (_ctx, _vp, _P(Particles)),

@@ -350,6 +350,13 @@ int sphfem_grid_green_create(sphfem_ctx *ctx, int num_part, sphfem_double2 bbmin
         g->nblocks = nblk(num_part, 4096);
         g->hist = dalloc<int>(256ul * g->nblocks);
         g->digit_tot = dalloc<int>(256);
+        const int nw = nblk(num_part, 64);
+        g->cmask = dalloc<unsigned long long>(nw);
+        g->wprefix = dalloc<int>(nw + 1);
+        g->bprefix = dalloc<int>(g->nblocks + 1);
+        g->clist = dalloc<int2>(SPHFEM_RESORT_CAP);
+        g->csorted = dalloc<unsigned long long>(SPHFEM_RESORT_CAP);
+        g->cdest = dalloc<int4>(SPHFEM_RESORT_CAP);
         alloc_particles(ctx->stream, (unsigned)num_part, g->buf);
         g->first_sort = true;
         ctx->c.geom = g->geom;
@@ -364,6 +371,7 @@ int sphfem_grid_destroy(sphfem_ctx *ctx, sphfem_grid *g) {
         SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         dfree(g->cell_start); dfree(g->cell_end); dfree(g->keys_alt); dfree(g->vals_alt);
         dfree(g->hist); dfree(g->digit_tot);
+        dfree(g->cmask); dfree(g->wprefix); dfree(g->bprefix); dfree(g->clist); dfree(g->csorted); dfree(g->cdest);
         free_particles(g->buf);
         delete g;
         return SPHFEM_OK;
@@ -382,6 +390,7 @@ int sphfem_grid_assign_hashes(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles 
     return guarded(ctx, [&] {
         ctx->c.geom = g->geom;
         launch_hash(ctx, g, *p, false, nullptr);
+        g->first_sort = true;   // p.hash no longer holds the keys of the cell table
         return SPHFEM_OK;
     });
 }
@@ -394,6 +403,24 @@ int sphfem_grid_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles *p) {
         launch_reorder_full(ctx, g, *p);
         swap_gathered(*p, g->buf);
         g->first_sort = false;
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_grid_set_resort_limit(sphfem_ctx *ctx, sphfem_grid *g, int max_changed) {
+    if (!ctx || !g || max_changed < 0) return SPHFEM_ERR_ARG;
+    g->resort_limit = std::min(max_changed, SPHFEM_RESORT_CAP);
+    return SPHFEM_OK;
+}
+
+int sphfem_grid_resort_stats(sphfem_ctx *ctx, int *mode, int *changed) {
+    if (!ctx) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        int h[8];
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(h, ctx->d_flags, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (mode) *mode = h[3];
+        if (changed) *changed = h[4];
         return SPHFEM_OK;
     });
 }
@@ -883,7 +910,10 @@ int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particle
         ctx->c.geom = g->geom;
         if (t) launch_tool_begin(ctx, t);
         else launch_fill_int(ctx, ctx->d_flags + 1, 2, 0);
-        launch_radix_sort(ctx, g, *p, true);
+        // the cell table is cleared by the hash pass (cells of the previous keys) except when the
+        // previous keys are unknown
+        if (g->first_sort) launch_fill_int(ctx, g->cell_start, g->num_cell, -1);
+        launch_resort(ctx, g, *p);
         g->first_sort = false;
         launch_reorder_predict(ctx, g, *p, lf);
         sphfem_particles B = g->buf;

@@ -10,6 +10,9 @@
 
 #include "../../include/sphfem.h"
 
+// largest number of particles that may change cells in one step for the incremental re-sort
+#define SPHFEM_RESORT_CAP 4096
+
 namespace sphfem {
 
 // All constant blocks, passed to kernels by value (replaces the __constant__ copies that
@@ -57,7 +60,8 @@ struct sphfem_ctx {
     sphfem::DevConsts c{};
     unsigned set_mask = 0;
     std::string err;
-    int *d_flags = nullptr;       // [0]: out-of-grid, [1]: near-list count, [2]: contact count
+    int *d_flags = nullptr;       // [0]: error bits, [1]: near-list count, [2]: contact count,
+                                  // [3]: re-sort mode (1 full, 2 incremental), [4]: changed count
     sphfem::Timing timing;
 };
 
@@ -76,6 +80,14 @@ struct sphfem_grid {
     // double buffer of the 20 gathered arrays (grid_gpu_green.h:21-50)
     sphfem_particles buf{};
     bool first_sort = true;
+    // incremental re-sort of the step (particles stored in the previous sorted order)
+    unsigned long long *cmask = nullptr;   // [nw] bit per position: cell changed this step
+    int *wprefix = nullptr;                // [nw] changed positions before each 64-block, within its tile
+    int *bprefix = nullptr;                // [nblocks + 1] changed positions before each 4096-tile
+    int2 *clist = nullptr;                 // [SPHFEM_RESORT_CAP] changed (key, position), position order
+    unsigned long long *csorted = nullptr; // [SPHFEM_RESORT_CAP] changed (key << 32 | position), sorted
+    int4 *cdest = nullptr;                 // [SPHFEM_RESORT_CAP] (destination, key, position)
+    int resort_limit = SPHFEM_RESORT_CAP;  // max changed particles for the incremental path (0: off)
 };
 
 struct sphfem_tool {
@@ -160,6 +172,7 @@ __device__ __forceinline__ int xcd_logical_block() {
 // sort / cells (sphfem_sort.hip)
 void launch_hash(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, bool clear_old, int *hist0);
 void launch_radix_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool hash_done_with_hist0);
+void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p);
 void launch_fill_int(sphfem_ctx *ctx, int *ptr, long n, int value);
 void launch_reorder_full(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p);
 void launch_reorder_predict(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, sphfem_leap_frog *lf);

@@ -55,8 +55,10 @@ __global__ void __launch_bounds__(SORT_BS) k_hash(DevConsts c, const double *__r
     }
 }
 
+// `mode`, when given, points at the device re-sort mode: the full-sort kernels run only in mode 1
 __global__ void __launch_bounds__(SORT_BS) k_upsweep(const int *__restrict__ keys, int N, int shift,
-                                                     int *__restrict__ hist, int nblocks) {
+                                                     int *__restrict__ hist, int nblocks, const int *mode) {
+    if (mode && *mode != 1) return;
     __shared__ int lh[RADIX];
     for (int d = threadIdx.x; d < RADIX; d += SORT_BS) lh[d] = 0;
     __syncthreads();
@@ -72,7 +74,8 @@ __global__ void __launch_bounds__(SORT_BS) k_upsweep(const int *__restrict__ key
 
 // one block per digit: exclusive scan of that digit's per-tile counts, total into digit_tot
 __global__ void __launch_bounds__(SORT_BS) k_scan_digits(int *__restrict__ hist, int nblocks,
-                                                         int *__restrict__ digit_tot) {
+                                                         int *__restrict__ digit_tot, const int *mode) {
+    if (mode && *mode != 1) return;
     __shared__ int part[SORT_BS];
     int *row = hist + (long)blockIdx.x * nblocks;
     int per = (nblocks + SORT_BS - 1) / SORT_BS;
@@ -109,7 +112,8 @@ __global__ void __launch_bounds__(SORT_BS) k_downsweep(const int *__restrict__ k
                                                        int *__restrict__ keys_out, int *__restrict__ vals_out,
                                                        int N, int shift, const int *__restrict__ hist,
                                                        int nblocks, const int *__restrict__ digit_tot,
-                                                       int *__restrict__ flags) {
+                                                       int *__restrict__ flags, const int *mode) {
+    if (mode && *mode != 1) return;
     __shared__ int block_off[RADIX];
     __shared__ int wcount[SORT_BS / WAVE][RADIX];
     __shared__ int scan[RADIX];
@@ -230,20 +234,284 @@ void launch_radix_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, boo
         int dst = 1 - src;
         if (!(ps == 0 && hash_with_hist0)) {
             KTimer t(ctx, "radix_upsweep");
-            k_upsweep<<<nb, SORT_BS, 0, ctx->stream>>>(kin, N, shift, g->hist, nb);
+            k_upsweep<<<nb, SORT_BS, 0, ctx->stream>>>(kin, N, shift, g->hist, nb, nullptr);
         }
         {
             KTimer t(ctx, "radix_scan");
-            k_scan_digits<<<RADIX, SORT_BS, 0, ctx->stream>>>(g->hist, nb, g->digit_tot);
+            k_scan_digits<<<RADIX, SORT_BS, 0, ctx->stream>>>(g->hist, nb, g->digit_tot, nullptr);
         }
         {
             KTimer t(ctx, "radix_downsweep");
             k_downsweep<<<nb, SORT_BS, 0, ctx->stream>>>(kin, ps == 0 ? nullptr : vin, kbuf[dst], vbuf[dst], N,
-                                                         shift, g->hist, nb, g->digit_tot, ctx->d_flags);
+                                                         shift, g->hist, nb, g->digit_tot, ctx->d_flags, nullptr);
         }
         kin = kbuf[dst];
         vin = vbuf[dst];
         src = dst;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Incremental re-sort for the step.  After a step the particles are stored in the previous sorted
+// order, so `old` (the previous sorted keys) is ascending.  A particle keeps its cell unless it
+// crossed a cell boundary (rare at dt ~ 1e-4 of the cell crossing time).  The stable sort of the
+// new keys with the previous position as tie-break is then the merge of
+//   U: unchanged positions, already in (key, position) order, and
+//   C: changed positions sorted by (new key, position):
+//   dest(u) = u - changed_before(u) + |{c in C : (key_c, c) < (key_u, u)}|
+//   dest(C[r]) = r + P - changed_before(P),  P = first p with (old[p], p) >= (key_c, c)
+// which is exactly the result of the full stable radix sort (same keys, same order).
+// The path is chosen on the device: the full radix sort runs on the first step and whenever more
+// than `cap` particles changed cells.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int changed_before(const unsigned long long *__restrict__ cmask,
+                                              const int *__restrict__ wlocal, const int *__restrict__ bprefix,
+                                              int p, int N, int m) {
+    if (p >= N) return m;
+    const unsigned long long bits = cmask[p >> 6];
+    const int b = p & 63;
+    return bprefix[p / SORT_TILE] + wlocal[p >> 6] + __popcll(b ? (bits & ((~0ull) >> (64 - b))) : 0ull);
+}
+
+// grid.cu:25-42 hash (as k_hash, with the digit-0 histogram for the full-sort fallback), plus the
+// changed bit of every position, the changed count before each 64 positions within the tile
+// (wlocal) and the tile's total (bcount)
+__global__ void __launch_bounds__(SORT_BS) k_hash_mark(DevConsts c, const double *__restrict__ blanked,
+                                                       const sphfem_double2 *__restrict__ pos,
+                                                       const int *__restrict__ old_hash, int *__restrict__ hash_out,
+                                                       int *__restrict__ cell_start, int clear_old, int N,
+                                                       int num_cell, int *__restrict__ hist0, int nblocks,
+                                                       int *__restrict__ flags, unsigned long long *__restrict__ cmask,
+                                                       int *__restrict__ wlocal, int *__restrict__ bcount) {
+    __shared__ int lh[RADIX];
+    __shared__ int wc[SORT_TILE / 64];
+    for (int d = threadIdx.x; d < RADIX; d += SORT_BS) lh[d] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * SORT_TILE;
+    for (int it = 0; it < SORT_ITEMS; it++) {
+        const int i = base + it * SORT_BS + threadIdx.x;
+        bool changed = false;
+        if (i < N) {
+            const int oh = old_hash[i];
+            if (clear_old && oh >= 0 && oh < num_cell) cell_start[oh] = -1;
+            int h = oh;
+            if (blanked[i] != 1.) h = hash_of(c, pos[i]);
+            if (h < 0 || h >= num_cell) {
+                atomicOr(&flags[0], 1);
+                h = h < 0 ? 0 : num_cell - 1;
+            }
+            hash_out[i] = h;
+            atomicAdd(&lh[h & (RADIX - 1)], 1);
+            changed = h != oh;
+        }
+        const unsigned long long bits = __ballot(changed);
+        if (__lane_id() == 0) {
+            wc[(i - base) >> 6] = __popcll(bits);
+            if (i < N) cmask[i >> 6] = bits;
+        }
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < RADIX; d += SORT_BS) hist0[d * nblocks + blockIdx.x] = lh[d];
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int k = 0; k < SORT_TILE / 64; k++) {
+            const int g = (base >> 6) + k;
+            if ((long)g * 64 < N) wlocal[g] = run;
+            run += wc[k];
+        }
+        bcount[blockIdx.x] = run;
+    }
+}
+
+// one block: exclusive prefix of the tile totals (in place, coalesced chunks), the total and the
+// path decision
+constexpr int RESORT_SCAN_BS = 1024;
+__global__ void __launch_bounds__(RESORT_SCAN_BS) k_resort_scan(int *__restrict__ bprefix, int nb,
+                                                                int *__restrict__ flags, int force_full, int cap) {
+    __shared__ int part[RESORT_SCAN_BS];
+    int carry = 0;
+    for (int c0 = 0; c0 < nb; c0 += RESORT_SCAN_BS) {
+        const int j = c0 + threadIdx.x;
+        const int v = j < nb ? bprefix[j] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < RESORT_SCAN_BS; off <<= 1) {
+            const int u = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += u;
+            __syncthreads();
+        }
+        if (j < nb) bprefix[j] = carry + part[threadIdx.x] - v;
+        carry += part[RESORT_SCAN_BS - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        bprefix[nb] = carry;
+        flags[4] = carry;
+        flags[3] = (force_full || carry > cap) ? 1 : 2;
+    }
+}
+
+// changed positions, in position order
+__global__ void __launch_bounds__(256) k_resort_compact(const int *__restrict__ keys, int N,
+                                                        const unsigned long long *__restrict__ cmask,
+                                                        const int *__restrict__ wlocal,
+                                                        const int *__restrict__ bprefix, int2 *__restrict__ clist,
+                                                        const int *flags) {
+    if (flags[3] != 2) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const unsigned long long bits = cmask[i >> 6];
+    if (!((bits >> (i & 63)) & 1ull)) return;
+    const int k = changed_before(cmask, wlocal, bprefix, i, N, 0);
+    clist[k] = make_int2(keys[i], i);
+}
+
+// one block: rank of each changed position among the changed ones (counting in LDS; the composite
+// keys are distinct) and its destination
+__global__ void __launch_bounds__(RESORT_SCAN_BS) k_resort_rank(const int2 *__restrict__ clist,
+                                                                const int *__restrict__ old_keys, int N,
+                                                                const unsigned long long *__restrict__ cmask,
+                                                                const int *__restrict__ wlocal,
+                                                                const int *__restrict__ bprefix,
+                                                                unsigned long long *__restrict__ csorted,
+                                                                int4 *__restrict__ cdest, int *__restrict__ flags) {
+    __shared__ unsigned long long comp[SPHFEM_RESORT_CAP];
+    if (flags[3] != 2) return;
+    const int m = flags[4];
+    for (int k = threadIdx.x; k < m; k += RESORT_SCAN_BS) {
+        const int2 e = clist[k];
+        comp[k] = ((unsigned long long)(unsigned)e.x << 32) | (unsigned)e.y;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < m; k += RESORT_SCAN_BS) {
+        const unsigned long long ck = comp[k];
+        int r = 0;
+        for (int q = 0; q < m; q++) r += comp[q] < ck;
+        // P: first position p with (old[p], p) >= (key, position); (old[p], p) increases with p
+        int lo = 0, hi = N;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const unsigned long long cm = ((unsigned long long)(unsigned)old_keys[mid] << 32) | (unsigned)mid;
+            if (cm < ck) lo = mid + 1;
+            else hi = mid;
+        }
+        const int dest = r + lo - changed_before(cmask, wlocal, bprefix, lo, N, m);
+        if ((unsigned)dest >= (unsigned)N || r >= m) {
+            atomicOr(&flags[0], 2);
+            continue;
+        }
+        csorted[r] = ck;
+        cdest[r] = make_int4(dest, (int)(ck >> 32), (int)(ck & 0xffffffffu), 0);
+    }
+}
+
+// scatter: unchanged positions by the merge formula, changed ones from cdest
+__global__ void __launch_bounds__(256) k_resort_scatter(const int *__restrict__ keys, int N,
+                                                        const unsigned long long *__restrict__ cmask,
+                                                        const int *__restrict__ wlocal,
+                                                        const int *__restrict__ bprefix,
+                                                        const unsigned long long *__restrict__ csorted,
+                                                        const int4 *__restrict__ cdest, int *__restrict__ keys_out,
+                                                        int *__restrict__ idx_out, int *__restrict__ flags) {
+    if (flags[3] != 2) return;
+    const int m = flags[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < m) {
+        const int4 e = cdest[i];
+        if ((unsigned)e.x < (unsigned)N) {
+            keys_out[e.x] = e.y;
+            idx_out[e.x] = e.z;
+        }
+    }
+    if (i >= N) return;
+    if ((cmask[i >> 6] >> (i & 63)) & 1ull) return;
+    const int key = keys[i];
+    const unsigned long long ck = ((unsigned long long)(unsigned)key << 32) | (unsigned)i;
+    int lo = 0, hi = m;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (csorted[mid] < ck) lo = mid + 1;
+        else hi = mid;
+    }
+    const int dest = i - changed_before(cmask, wlocal, bprefix, i, N, m) + lo;
+    if ((unsigned)dest >= (unsigned)N) {
+        atomicOr(&flags[0], 2);
+        return;
+    }
+    keys_out[dest] = key;
+    idx_out[dest] = i;
+}
+
+__global__ void k_copy_keys_if_full(const int *__restrict__ src, int *__restrict__ dst, int N, const int *mode) {
+    if (*mode != 1) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+// Re-sort of the step: new keys and the permutation end in p.hash / p.idx (as launch_radix_sort)
+void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p) {
+    const int N = (int)p.N;
+    const int nb = nblk(N, SORT_TILE);
+    int *flags = ctx->d_flags;
+    const int *mode = flags + 3;
+    {
+        KTimer t(ctx, "hash");
+        k_hash_mark<<<nb, SORT_BS, 0, ctx->stream>>>(ctx->c, p.blanked, p.pos, p.hash, g->keys_alt, g->cell_start,
+                                                     g->first_sort ? 0 : 1, N, g->num_cell, g->hist, nb, flags,
+                                                     g->cmask, g->wprefix, g->bprefix);
+    }
+    {
+        KTimer t(ctx, "resort_scan");
+        const int cap = std::min(g->resort_limit, SPHFEM_RESORT_CAP);
+        k_resort_scan<<<1, RESORT_SCAN_BS, 0, ctx->stream>>>(g->bprefix, nb, flags,
+                                                              (g->first_sort || cap <= 0) ? 1 : 0, cap);
+    }
+    // full stable radix sort (mode 1): unsorted keys are in keys_alt
+    const int passes = (g->key_bits + 7) / 8;
+    int *kbuf[2] = {p.hash, g->keys_alt};
+    int *vbuf[2] = {p.idx, g->vals_alt};
+    int src = 1;
+    if (passes % 2 == 0) {
+        KTimer t(ctx, "radix_copy");
+        k_copy_keys_if_full<<<std::min(nblk(N, 256), 2048), 256, 0, ctx->stream>>>(g->keys_alt, p.hash, N, mode);
+        src = 0;
+    }
+    int *kin = kbuf[src], *vin = nullptr;
+    for (int ps = 0; ps < passes; ps++) {
+        const int shift = 8 * ps;
+        const int dst = 1 - src;
+        if (ps > 0) {
+            KTimer t(ctx, "radix_upsweep");
+            k_upsweep<<<nb, SORT_BS, 0, ctx->stream>>>(kin, N, shift, g->hist, nb, mode);
+        }
+        {
+            KTimer t(ctx, "radix_scan");
+            k_scan_digits<<<RADIX, SORT_BS, 0, ctx->stream>>>(g->hist, nb, g->digit_tot, mode);
+        }
+        {
+            KTimer t(ctx, "radix_downsweep");
+            k_downsweep<<<nb, SORT_BS, 0, ctx->stream>>>(kin, ps == 0 ? nullptr : vin, kbuf[dst], vbuf[dst], N,
+                                                         shift, g->hist, nb, g->digit_tot, flags, mode);
+        }
+        kin = kbuf[dst];
+        vin = vbuf[dst];
+        src = dst;
+    }
+    // incremental path (mode 2)
+    {
+        KTimer t(ctx, "resort_compact");
+        k_resort_compact<<<nblk(N, 256), 256, 0, ctx->stream>>>(g->keys_alt, N, g->cmask, g->wprefix, g->bprefix,
+                                                                g->clist, flags);
+    }
+    {
+        KTimer t(ctx, "resort_rank");
+        k_resort_rank<<<1, RESORT_SCAN_BS, 0, ctx->stream>>>(g->clist, p.hash, N, g->cmask, g->wprefix, g->bprefix,
+                                                              g->csorted, g->cdest, flags);
+    }
+    {
+        KTimer t(ctx, "resort_scatter");
+        k_resort_scatter<<<nblk(N, 256), 256, 0, ctx->stream>>>(g->keys_alt, N, g->cmask, g->wprefix, g->bprefix,
+                                                                g->csorted, g->cdest, p.hash, p.idx, flags);
     }
 }
 

@@ -148,6 +148,16 @@ class grid_gpu_green:
     def sort(self, p: particle_gpu):
         self.ctx.call("sphfem_grid_sort", self.h, C.byref(p.s))
 
+    def resort_stats(self):
+        """(mode, changed) of the last step's re-sort: mode 1 = full radix sort, 2 = incremental."""
+        m, n = C.c_int(), C.c_int()
+        self.ctx.call("sphfem_grid_resort_stats", C.byref(m), C.byref(n))
+        return m.value, n.value
+
+    def set_resort_limit(self, max_changed: int):
+        """Largest number of cell changes per step re-sorted incrementally (0: always the full sort)."""
+        self.ctx.call("sphfem_grid_set_resort_limit", self.h, int(max_changed))
+
     def get_cells(self, cell_start_dptr, cell_end_dptr):
         self.ctx.call("sphfem_grid_get_cells", self.h, C.c_void_p(cell_start_dptr), C.c_void_p(cell_end_dptr))
 
