@@ -70,31 +70,47 @@ def cpu_baseline(case, budget_s):
                       f"{threads} OpenMP threads"}
 
 
-def run_form(case, precision, steps, warmup, breakdown=False):
-    """Time `steps` full steps of one arithmetic form; returns (seconds, kernel stats, sim)."""
+def run_form(case, precision, steps, warmup, breakdown=False, graph=True):
+    """Time `steps` full steps of one arithmetic form; returns (seconds, interaction-kernel seconds, sim).
+
+    The timed loop replays a captured two-step HIP graph (an odd step count ends with one eager
+    step).  The interaction kernel's time comes from a separate eager pass with HIP events around
+    that kernel only, on the library's stream."""
     from sph_fem_2d_cutting_gpu_amd import _lib
     from sph_fem_2d_cutting_gpu_amd import solver as S
     prec = {"fp64": _lib.FP64, "fp32c": _lib.FP32C}[precision]
     sim = S.Simulation(case, precision=prec)
     ctx = sim.ctx
-    for _ in range(warmup):
+    for _ in range(max(warmup, 1)):
         sim.step(1)
     ctx.synchronize()
     ctx.check_error()
+    # interaction-kernel time (eager steps, events around that kernel only)
     kname = INTERACTION_KERNEL[precision]
     ctx.timing_select(kname)
     ctx.timing(True)
     ctx.timing_reset()
+    sim.step(min(steps, 20))
+    ctx.synchronize()
+    ctx.timing(False)
+    k_ms, k_n = ctx.timing_get(kname)
+    g = sim.capture(2) if graph else None
+    if g is not None:
+        g.launch(1)                                    # graph warm-up
     ctx.synchronize()
     t0 = time.perf_counter()
-    sim.step(steps)
+    if g is not None:
+        g.launch(steps // 2)
+        sim.step(steps % 2)
+    else:
+        sim.step(steps)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
-    ctx.timing(False)
     ctx.check_error()
-    k_ms, k_n = ctx.timing_get(kname)
+    if g is not None:
+        g.destroy()
     if breakdown:
-        # separate, untimed pass with an event pair around every kernel
+        # separate, untimed eager pass with an event pair around every kernel
         ctx.timing_select("")
         ctx.timing(True)
         ctx.timing_reset()
@@ -107,7 +123,7 @@ def run_form(case, precision, steps, warmup, breakdown=False):
             if n:
                 rows.append((ms / steps, n / steps, name))
         rows.sort(reverse=True)
-        print(f"[{precision}] per-step kernel time (ms), {steps} steps:", file=sys.stderr)
+        print(f"[{precision}] per-step kernel time (ms), {steps} eager steps:", file=sys.stderr)
         for ms, n, name in rows:
             print(f"  {name:28s} {ms:9.4f}  x{n:.0f}", file=sys.stderr)
     return elapsed, (k_ms / 1e3) / max(k_n, 1), sim
@@ -122,6 +138,7 @@ def main():
     ap.add_argument("--precision", default="fp32c", choices=["fp32c", "fp64"])
     ap.add_argument("--no-compare", action="store_true", help="skip timing the FP64 form beside FP32C")
     ap.add_argument("--breakdown", action="store_true", help="per-kernel times on stderr")
+    ap.add_argument("--eager", action="store_true", help="time eager steps instead of the captured step graph")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -145,7 +162,7 @@ def main():
     import numpy as np
 
     case, workload = make_case(args.workload)
-    elapsed, k_avg_s, sim = run_form(case, args.precision, args.steps, args.warmup, args.breakdown)
+    elapsed, k_avg_s, sim = run_form(case, args.precision, args.steps, args.warmup, args.breakdown, not args.eager)
     cs, _ = sim.grid.download_cells()
     n_occ = int(np.count_nonzero(cs != -1))
     w = 8 if args.precision == "fp64" else 4
@@ -170,7 +187,7 @@ def main():
         "data": "synthetic (seeded jittered lattice and generated tool mesh)",
         "config": {"workload": workload, "particles": case.N, "tool_triangles": int(len(case.tool.elems)),
                    "cutting_segments": int(len(case.tool.seg_left)), "wear": bool(case.wear),
-                   "precision": args.precision, "parallelism": "single GPU"},
+                   "precision": args.precision, "parallelism": "single GPU", "step_graph": not args.eager},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(INTERACTION_KERNEL[args.precision],
@@ -179,7 +196,7 @@ def main():
                      "algorithmic_bytes": b_int, "occupied_cells": n_occ},
     }
     if args.precision == "fp32c" and not args.no_compare:
-        e64, k64, sim64 = run_form(case, "fp64", args.steps, args.warmup, args.breakdown)
+        e64, k64, sim64 = run_form(case, "fp64", args.steps, args.warmup, args.breakdown, not args.eager)
         sim64.close()
         ups64 = case.N * args.steps / e64
         out["fp64_form"] = {"value": ups64, "ms_per_step": e64 / args.steps * 1e3,

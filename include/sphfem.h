@@ -201,6 +201,17 @@ int sphfem_leap_frog_correct(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_parti
 int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,
                           sphfem_tool *t);
 
+/* Step graphs: `steps` (even) steps of sphfem_leap_frog_step recorded into one HIP graph and
+ * replayed with sphfem_leap_frog_graph_launch (each launch advances `steps` steps).  Capture
+ * records only (the state does not advance) and needs one eager step first; the graph bakes in
+ * the constants, dt, wear switch and precision at capture.  No counterpart in the reference
+ * (its step is a sequence of launches with host synchronisations). */
+typedef struct sphfem_graph sphfem_graph;
+int sphfem_leap_frog_graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,
+                                  sphfem_tool *t, int steps, sphfem_graph **out);
+int sphfem_leap_frog_graph_launch(sphfem_ctx *ctx, sphfem_graph *graph, int repeats);
+int sphfem_leap_frog_graph_destroy(sphfem_ctx *ctx, sphfem_graph *graph);
+
 /* ---- observables (output.cu:22-30) */
 int sphfem_report_force(sphfem_ctx *ctx, sphfem_particles *p, sphfem_double2 *out);
 

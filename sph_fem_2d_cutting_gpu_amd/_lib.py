@@ -88,6 +88,9 @@ _SIGS = {
     "sphfem_grid_green_create": (_ctx, C.c_int, Double2, Double2, _dbl, _P(_vp)),
     "sphfem_grid_destroy": (_ctx, _vp),
     "sphfem_grid_set_resort_limit": (_ctx, _vp, C.c_int),
+    "sphfem_leap_frog_graph_create": (_ctx, _vp, _P(Particles), _vp, _vp, C.c_int, _P(_vp)),
+    "sphfem_leap_frog_graph_launch": (_ctx, _vp, C.c_int),
+    "sphfem_leap_frog_graph_destroy": (_ctx, _vp),
     "sphfem_grid_resort_stats": (_ctx, _P(C.c_int), _P(C.c_int)),
     "sphfem_grid_geometry": (_vp, _P(GeomConstants), _P(C.c_int)),
     "sphfem_grid_assign_hashes": (_ctx, _vp, _P(Particles)),

@@ -902,38 +902,105 @@ int sphfem_leap_frog_correct(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_parti
 //  tool_begin | hash+digit-0 histogram | radix passes | reorder+init+predict+EOS+art. stress |
 //  pair sums+Jaumann+continuity+momentum+advection+correct+friction heat+plasticity+far contact+
 //  BC+invalidate | FEM rate+temperature | wear | tool move | near contact s1 (+BC) | contact s2
+static void step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g, sphfem_tool *t) {
+    ctx->c.geom = g->geom;
+    if (t) launch_tool_begin(ctx, t);
+    else launch_fill_int(ctx, ctx->d_flags + 1, 2, 0);
+    // the cell table is cleared by the hash pass (cells of the previous keys) except when the
+    // previous keys are unknown
+    if (g->first_sort) launch_fill_int(ctx, g->cell_start, g->num_cell, -1);
+    launch_resort(ctx, g, *p);
+    g->first_sort = false;
+    launch_reorder_predict(ctx, g, *p, lf);
+    sphfem_particles B = g->buf;
+    B.hash = p->hash;
+    B.idx = p->idx;
+    B.N = p->N;
+    launch_fused_update(ctx, B, *p, lf, g->cell_start, g->cell_end, t);
+    if (lf->precision == SPHFEM_FP32C) {
+        // the whole new state is in the buffer; R is not maintained by the FP32C form
+        swap_gathered(*p, g->buf);
+        std::swap(p->p, g->buf.p);
+    } else {
+        swap_fused(*p, g->buf);
+    }
+    if (t) {
+        launch_tool_fem_end(ctx, t);
+        if (ctx->c.wear_pro) launch_wear(ctx, t);
+        launch_tool_move(ctx, t, ctx->c.dt, true, true);
+        launch_near_contact(ctx, *p, t, lf);
+    }
+}
+
 int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,
                           sphfem_tool *t) {
     if (!lf || !p || !g || (int)p->N != g->num_part || p->N != lf->num_part) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {
         if (int rc = need(ctx, SET_PHYS | SET_CORR | SET_TRML_WP | SET_DT | (t ? SET_TOOLC : 0u))) return rc;
-        ctx->c.geom = g->geom;
-        if (t) launch_tool_begin(ctx, t);
-        else launch_fill_int(ctx, ctx->d_flags + 1, 2, 0);
-        // the cell table is cleared by the hash pass (cells of the previous keys) except when the
-        // previous keys are unknown
-        if (g->first_sort) launch_fill_int(ctx, g->cell_start, g->num_cell, -1);
-        launch_resort(ctx, g, *p);
-        g->first_sort = false;
-        launch_reorder_predict(ctx, g, *p, lf);
-        sphfem_particles B = g->buf;
-        B.hash = p->hash;
-        B.idx = p->idx;
-        B.N = p->N;
-        launch_fused_update(ctx, B, *p, lf, g->cell_start, g->cell_end, t);
-        if (lf->precision == SPHFEM_FP32C) {
-            // the whole new state is in the buffer; R is not maintained by the FP32C form
-            swap_gathered(*p, g->buf);
-            std::swap(p->p, g->buf.p);
-        } else {
-            swap_fused(*p, g->buf);
+        step_body(ctx, lf, p, g, t);
+        return SPHFEM_OK;
+    });
+}
+
+}  // extern "C"
+
+struct sphfem_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int steps = 0;
+};
+
+extern "C" {
+
+int sphfem_leap_frog_graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,
+                                  sphfem_tool *t, int steps, sphfem_graph **out) {
+    if (!lf || !p || !g || !out || (int)p->N != g->num_part || p->N != lf->num_part) return SPHFEM_ERR_ARG;
+    if (steps < 2 || steps % 2) return fail(ctx, SPHFEM_ERR_ARG, "graph steps must be even (pointer swaps)");
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        if (int rc = need(ctx, SET_PHYS | SET_CORR | SET_TRML_WP | SET_DT | (t ? SET_TOOLC : 0u))) return rc;
+        if (g->first_sort) return fail(ctx, SPHFEM_ERR_STATE, "run one step before capturing (first sort)");
+        const bool timing = ctx->timing.enabled;
+        ctx->timing.enabled = false;
+        auto *gr = new sphfem_graph();
+        gr->steps = steps;
+        try {
+            SPHFEM_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            // recording only: the device state does not advance; after an even number of steps the
+            // host-side pointer assignment is the one the graph starts from
+            for (int k = 0; k < steps; k++) step_body(ctx, lf, p, g, t);
+            SPHFEM_HIP_CHECK(hipStreamEndCapture(ctx->stream, &gr->graph));
+            SPHFEM_HIP_CHECK(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0));
+        } catch (...) {
+            hipGraph_t dead = nullptr;
+            (void)hipStreamEndCapture(ctx->stream, &dead);
+            if (dead) (void)hipGraphDestroy(dead);
+            if (gr->graph) (void)hipGraphDestroy(gr->graph);
+            delete gr;
+            ctx->timing.enabled = timing;
+            throw;
         }
-        if (t) {
-            launch_tool_fem_end(ctx, t);
-            if (ctx->c.wear_pro) launch_wear(ctx, t);
-            launch_tool_move(ctx, t, ctx->c.dt, true, true);
-            launch_near_contact(ctx, *p, t, lf);
-        }
+        ctx->timing.enabled = timing;
+        *out = gr;
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_leap_frog_graph_launch(sphfem_ctx *ctx, sphfem_graph *gr, int repeats) {
+    if (!gr || repeats < 0) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        for (int r = 0; r < repeats; r++) SPHFEM_HIP_CHECK(hipGraphLaunch(gr->exec, ctx->stream));
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_leap_frog_graph_destroy(sphfem_ctx *ctx, sphfem_graph *gr) {
+    if (!gr) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+        if (gr->graph) (void)hipGraphDestroy(gr->graph);
+        delete gr;
         return SPHFEM_OK;
     });
 }

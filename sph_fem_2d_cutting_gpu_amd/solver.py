@@ -295,6 +295,13 @@ class leap_frog:
     def step(self, p: particle_gpu, g: grid_gpu_green, tool: tool_FEM | None = None):
         self.ctx.call("sphfem_leap_frog_step", self.h, C.byref(p.s), g.h, tool.h if tool else None)
 
+    def capture(self, p: particle_gpu, g: grid_gpu_green, tool: tool_FEM | None = None, steps: int = 2):
+        """Record `steps` (even) steps into a HIP graph; returns a StepGraph.  Needs one eager step first."""
+        gh = C.c_void_p()
+        self.ctx.call("sphfem_leap_frog_graph_create", self.h, C.byref(p.s), g.h, tool.h if tool else None,
+                      int(steps), C.byref(gh))
+        return StepGraph(self.ctx, gh, steps)
+
     def init(self, p):
         self.ctx.call("sphfem_leap_frog_init", self.h, C.byref(p.s))
 
@@ -303,6 +310,21 @@ class leap_frog:
 
     def correct(self, p, dt):
         self.ctx.call("sphfem_leap_frog_correct", self.h, C.byref(p.s), float(dt))
+
+
+class StepGraph:
+    """A captured run of an even number of steps; launch(n) advances n * steps steps."""
+
+    def __init__(self, ctx: Context, h, steps: int):
+        self.ctx, self.h, self.steps = ctx, h, steps
+
+    def launch(self, repeats: int = 1):
+        self.ctx.call("sphfem_leap_frog_graph_launch", self.h, int(repeats))
+
+    def destroy(self):
+        if self.h:
+            self.ctx.call("sphfem_leap_frog_graph_destroy", self.h)
+            self.h = C.c_void_p()
 
 
 # ---- actions_gpu.h:33-72 / interactions_gpu.h:27-43 / output.h
@@ -370,6 +392,10 @@ class Simulation:
     def step(self, n: int = 1):
         for _ in range(n):
             self.stepper.step(self.p, self.grid, self.tool)
+
+    def capture(self, steps: int = 2) -> StepGraph:
+        """Graph of `steps` steps (one eager step must have run)."""
+        return self.stepper.capture(self.p, self.grid, self.tool, steps)
 
     def synchronize(self):
         self.ctx.synchronize()
