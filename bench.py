@@ -129,6 +129,50 @@ def run_form(case, precision, steps, warmup, breakdown=False, graph=True):
     return elapsed, (k_ms / 1e3) / max(k_n, 1), sim
 
 
+def run_batch(args):
+    """Batched sweep: cases advance together (one stream and captured graph per case) vs one after
+    another; aggregate particle-updates/s of both."""
+    from sph_fem_2d_cutting_gpu_amd import _lib, cases
+    from sph_fem_2d_cutting_gpu_amd import solver as S
+    cs = cases.batch_cases(n_cases=args.batch_cases)
+    prec = {"fp64": _lib.FP64, "fp32c": _lib.FP32C}[args.precision]
+    batch = S.BatchedSimulation(cs, precision=prec)
+    batch.step(max(args.warmup, 1))
+    batch.synchronize()
+    batch.check_error()
+    batch.capture()
+    batch.launch(1)
+    batch.synchronize()
+    reps = max(1, args.steps // 2)
+    t0 = time.perf_counter()
+    batch.launch(reps)
+    batch.synchronize()
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batch.launch_sequential(reps)
+    t_seq = time.perf_counter() - t0
+    batch.check_error()
+    n_tot = batch.total_particles
+    steps = 2 * reps
+    ns = [c.N for c in cs]
+    out = {
+        "metric": f"aggregate particle-updates/s, {len(cs)}-case batched cutting sweep",
+        "value": n_tot * steps / t_batch, "unit": "particle-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": t_batch / steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32 pair sums (FP64 block accumulation) + f64 state",
+        "data": "synthetic (seeded parameter sweep, generated tool meshes)",
+        "config": {"workload": f"batch{len(cs)}_sweep: dx=1.25um, length U(0.5,3) mm, speed U(30,300) m/min, "
+                               "rake U(-10,15) deg, doc U(20,80) um, Ti6Al4V/AISI1045",
+                   "cases": len(cs), "particles_total": n_tot, "particles_min": min(ns), "particles_max": max(ns),
+                   "precision": args.precision, "parallelism": "single GPU, one stream + step graph per case"},
+        "sequential": {"value": n_tot * steps / t_seq, "ms_per_step": t_seq / steps * 1e3},
+        "speedup_vs_sequential": t_seq / t_batch,
+    }
+    batch.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +183,7 @@ def main():
     ap.add_argument("--no-compare", action="store_true", help="skip timing the FP64 form beside FP32C")
     ap.add_argument("--breakdown", action="store_true", help="per-kernel times on stderr")
     ap.add_argument("--eager", action="store_true", help="time eager steps instead of the captured step graph")
+    ap.add_argument("--batch-cases", type=int, default=64, help="cases of the batch64 workload")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -161,6 +206,9 @@ def main():
 
     import numpy as np
 
+    if args.workload == "batch64":
+        run_batch(args)
+        return
     case, workload = make_case(args.workload)
     elapsed, k_avg_s, sim = run_form(case, args.precision, args.steps, args.warmup, args.breakdown, not args.eager)
     cs, _ = sim.grid.download_cells()

@@ -668,6 +668,26 @@ def config3(seed=DEFAULT_SEED, nx=4000, ny=1000, tool_tris=50000, wear=True):
                         5e-4, tool_tris, wear=wear, seed=seed)
 
 
+def batch_cases(n_cases=64, seed=DEFAULT_SEED, dx=1.25e-4, tool_tris=1000, rows=(50, 125)):
+    """Parameter sweep of independent cutting cases (dx = 1.25 um): workpiece length U(0.5, 3) mm,
+    height U(rows) lattice rows (about 20k-300k particles per case), cutting speed U(30, 300) m/min,
+    rake angle U(-10, 15) deg, depth of cut U(20, 80) um, Ti6Al4V or AISI 1045 at random."""
+    u = uniform01(seed, 8 * n_cases, stream=20).reshape(n_cases, 8)
+    out = []
+    for k in range(n_cases):
+        length = 0.05 + 0.25 * u[k, 0]
+        nx = int(round(length / dx))
+        ny = int(round(rows[0] + (rows[1] - rows[0]) * u[k, 1]))
+        speed = 30.0 + 270.0 * u[k, 2]
+        rake = -10.0 + 25.0 * u[k, 3]
+        doc = 2e-3 + 6e-3 * u[k, 4]
+        material = "Ti6Al4V" if u[k, 5] < 0.5 else "AISI1045"
+        doc = min(doc, 0.8 * ny * dx)
+        out.append(cutting_case(f"batch{k:02d}_{material}_{nx}x{ny}", nx, ny, dx, material, speed, doc, 0.35,
+                                rake, 7.0, 5e-4, tool_tris, wear=False, seed=seed + 1000 + k))
+    return out
+
+
 def chip_case(n_target=16_000_000, seed=DEFAULT_SEED, dx=1e-4, material="Ti6Al4V"):
     """Kernel-only workload: slab plus inclined chip band with a 10:1 density ramp.
 

@@ -415,3 +415,54 @@ class Simulation:
         self.ctx.call("sphfem_grid_destroy", self.grid.h)
         self.p.free()
         self.ctx.close()
+
+
+class BatchedSimulation:
+    """Independent cases advanced together: one context (HIP stream + constants) and one captured
+    two-step graph per case.  `launch` queues one graph replay per case without synchronising, so
+    the cases run concurrently on the GPU; `launch_sequential` runs them one after another (the
+    baseline a per-case loop would give)."""
+
+    def __init__(self, case_list, precision=_lib.FP64, device=0, graph_steps=2):
+        self.cases = list(case_list)
+        self.sims = [Simulation(c, precision=precision, device=device) for c in self.cases]
+        self.graph_steps = graph_steps
+        self.graphs = None
+        self.total_particles = sum(c.N for c in self.cases)
+
+    def step(self, n: int = 1):
+        """Eager steps, queued case by case (concurrent across the cases' streams)."""
+        for _ in range(n):
+            for s in self.sims:
+                s.step(1)
+
+    def capture(self):
+        """One graph per case (each case must have run an eager step)."""
+        self.graphs = [s.capture(self.graph_steps) for s in self.sims]
+
+    def launch(self, repeats: int = 1):
+        """Advance every case by repeats * graph_steps steps, all cases in flight together."""
+        for _ in range(repeats):
+            for g in self.graphs:
+                g.launch(1)
+
+    def launch_sequential(self, repeats: int = 1):
+        for s, g in zip(self.sims, self.graphs):
+            g.launch(repeats)
+            s.ctx.synchronize()
+
+    def synchronize(self):
+        for s in self.sims:
+            s.ctx.synchronize()
+
+    def check_error(self):
+        for s in self.sims:
+            s.ctx.check_error()
+
+    def close(self):
+        if self.graphs:
+            for g in self.graphs:
+                g.destroy()
+            self.graphs = None
+        for s in self.sims:
+            s.close()
