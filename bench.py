@@ -70,7 +70,7 @@ def cpu_baseline(case, budget_s):
                       f"{threads} OpenMP threads"}
 
 
-def run_form(case, precision, steps, warmup, breakdown=False, graph=True):
+def run_form(case, precision, steps, warmup, breakdown=False, graph=True, tune=True):
     """Time `steps` full steps of one arithmetic form; returns (seconds, interaction-kernel seconds, sim).
 
     The timed loop replays a captured two-step HIP graph (an odd step count ends with one eager
@@ -85,6 +85,10 @@ def run_form(case, precision, steps, warmup, breakdown=False, graph=True):
         sim.step(1)
     ctx.synchronize()
     ctx.check_error()
+    tuning = None
+    if tune:                                           # untimed: advances the warm-up
+        from sph_fem_2d_cutting_gpu_amd import autotune
+        tuning = autotune.tune_simulation(sim, prec, steps=4, repeats=2)["best"]
     # interaction-kernel time (eager steps, events around that kernel only)
     kname = INTERACTION_KERNEL[precision]
     ctx.timing_select(kname)
@@ -126,6 +130,7 @@ def run_form(case, precision, steps, warmup, breakdown=False, graph=True):
         print(f"[{precision}] per-step kernel time (ms), {steps} eager steps:", file=sys.stderr)
         for ms, n, name in rows:
             print(f"  {name:28s} {ms:9.4f}  x{n:.0f}", file=sys.stderr)
+    sim.bench_tuning = tuning
     return elapsed, (k_ms / 1e3) / max(k_n, 1), sim
 
 
@@ -140,10 +145,15 @@ def run_batch(args):
     batch.step(max(args.warmup, 1))
     batch.synchronize()
     batch.check_error()
-    batch.capture()
+    tuned = None
+    if not args.no_tune:
+        from sph_fem_2d_cutting_gpu_amd import autotune
+        tuned = autotune.tune_batch(batch, prec)
+    else:
+        batch.capture()
     batch.launch(1)
     batch.synchronize()
-    reps = max(1, args.steps // 2)
+    reps = max(1, args.steps // batch.graph_steps)
     t0 = time.perf_counter()
     batch.launch(reps)
     batch.synchronize()
@@ -153,7 +163,7 @@ def run_batch(args):
     t_seq = time.perf_counter() - t0
     batch.check_error()
     n_tot = batch.total_particles
-    steps = 2 * reps
+    steps = batch.graph_steps * reps
     ns = [c.N for c in cs]
     out = {
         "metric": f"aggregate particle-updates/s, {len(cs)}-case batched cutting sweep",
@@ -165,7 +175,9 @@ def run_batch(args):
         "config": {"workload": f"batch{len(cs)}_sweep: dx=1.25um, length U(0.5,3) mm, speed U(30,300) m/min, "
                                "rake U(-10,15) deg, doc U(20,80) um, Ti6Al4V/AISI1045",
                    "cases": len(cs), "particles_total": n_tot, "particles_min": min(ns), "particles_max": max(ns),
-                   "precision": args.precision, "parallelism": "single GPU, one stream + step graph per case"},
+                   "precision": args.precision, "parallelism": "single GPU, one stream + step graph per case",
+                   "graph_steps": batch.graph_steps,
+                   "tuning_ms_per_graph_step": tuned["ms_per_step"] if tuned else None},
         "sequential": {"value": n_tot * steps / t_seq, "ms_per_step": t_seq / steps * 1e3},
         "speedup_vs_sequential": t_seq / t_batch,
     }
@@ -184,6 +196,7 @@ def main():
     ap.add_argument("--breakdown", action="store_true", help="per-kernel times on stderr")
     ap.add_argument("--eager", action="store_true", help="time eager steps instead of the captured step graph")
     ap.add_argument("--batch-cases", type=int, default=64, help="cases of the batch64 workload")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch parameters)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -210,7 +223,9 @@ def main():
         run_batch(args)
         return
     case, workload = make_case(args.workload)
-    elapsed, k_avg_s, sim = run_form(case, args.precision, args.steps, args.warmup, args.breakdown, not args.eager)
+    elapsed, k_avg_s, sim = run_form(case, args.precision, args.steps, args.warmup, args.breakdown, not args.eager,
+                                     not args.no_tune)
+    tuning = sim.bench_tuning
     cs, _ = sim.grid.download_cells()
     n_occ = int(np.count_nonzero(cs != -1))
     w = 8 if args.precision == "fp64" else 4
@@ -235,7 +250,8 @@ def main():
         "data": "synthetic (seeded jittered lattice and generated tool mesh)",
         "config": {"workload": workload, "particles": case.N, "tool_triangles": int(len(case.tool.elems)),
                    "cutting_segments": int(len(case.tool.seg_left)), "wear": bool(case.wear),
-                   "precision": args.precision, "parallelism": "single GPU", "step_graph": not args.eager},
+                   "precision": args.precision, "parallelism": "single GPU", "step_graph": not args.eager,
+                   "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(INTERACTION_KERNEL[args.precision],
@@ -244,7 +260,8 @@ def main():
                      "algorithmic_bytes": b_int, "occupied_cells": n_occ},
     }
     if args.precision == "fp32c" and not args.no_compare:
-        e64, k64, sim64 = run_form(case, "fp64", args.steps, args.warmup, args.breakdown, not args.eager)
+        e64, k64, sim64 = run_form(case, "fp64", args.steps, args.warmup, args.breakdown, not args.eager,
+                                   not args.no_tune)
         sim64.close()
         ups64 = case.N * args.steps / e64
         out["fp64_form"] = {"value": ups64, "ms_per_step": e64 / args.steps * 1e3,

@@ -39,6 +39,9 @@ ToolKinematics = _s("ToolKinematics", ["vel_x", "vel_y", "tl_x", "tl_y", "br_x",
                                        "lowest_point", "right_point", "flank_retreat", "flank_contact"])
 
 
+Tuning = _s("Tuning", ["pair_block", "pair_chunk", "update_block"], C.c_int)
+
+
 class GeomConstants(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("bbmin_x", _dbl), ("bbmin_y", _dbl), ("dx", _dbl)]
 
@@ -90,6 +93,8 @@ _SIGS = {
     "sphfem_grid_set_resort_limit": (_ctx, _vp, C.c_int),
     "sphfem_leap_frog_graph_create": (_ctx, _vp, _P(Particles), _vp, _vp, C.c_int, _P(_vp)),
     "sphfem_leap_frog_graph_launch": (_ctx, _vp, C.c_int),
+    "sphfem_leap_frog_set_tuning": (_ctx, _vp, _P(Tuning)),
+    "sphfem_leap_frog_get_tuning": (_ctx, _vp, _P(Tuning)),
     "sphfem_leap_frog_graph_destroy": (_ctx, _vp),
     "sphfem_grid_resort_stats": (_ctx, _P(C.c_int), _P(C.c_int)),
     "sphfem_grid_geometry": (_vp, _P(GeomConstants), _P(C.c_int)),

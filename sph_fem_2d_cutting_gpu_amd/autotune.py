@@ -1,0 +1,100 @@
+"""Autotuner for the step's launch parameters and the batch's graph depth.
+
+Tunables (all result-neutral except the FP32 summation blocks of the FP32C form):
+  * pair_block   -- FP32C pair kernel workgroup size (128, 256)
+  * pair_chunk   -- FP32C neighbour-list entries per lane in LDS (16, 32, 48): larger lists run
+                    fewer list flushes in dense regions, smaller ones leave LDS for more waves
+  * update_block -- update kernel workgroup size (128, 256)
+  * graph_steps  -- steps per captured graph in a batch (2, 4, 8): fewer host launches per step
+The tuner times each candidate on the live workload (it advances the simulation: run it in the
+warm-up), keeps the fastest and can store the choice in a JSON cache keyed by workload size,
+precision and device.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+import time
+
+from . import _lib
+
+PAIR_BLOCKS = (128, 256)
+PAIR_CHUNKS = (16, 32, 48)
+UPDATE_BLOCKS = (128, 256)
+GRAPH_STEPS = (2, 4, 8)
+
+
+def candidates(precision):
+    if precision == _lib.FP32C:
+        return [dict(pair_block=b, pair_chunk=c, update_block=u)
+                for b, c, u in itertools.product(PAIR_BLOCKS, PAIR_CHUNKS, UPDATE_BLOCKS)]
+    return [dict(pair_block=256, pair_chunk=32, update_block=u) for u in UPDATE_BLOCKS]
+
+
+def _time_steps(sim, steps):
+    sim.ctx.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.ctx.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def tune_simulation(sim, precision, steps=6, repeats=2) -> dict:
+    """Time every candidate for `steps` steps (best of `repeats`), apply the fastest; returns the
+    choice and all timings (ms per step).  Advances the simulation by
+    len(candidates) * steps * repeats steps."""
+    rows = []
+    for cand in candidates(precision):
+        sim.set_tuning(**cand)
+        t = min(_time_steps(sim, steps) for _ in range(repeats))
+        rows.append((t, cand))
+    sim.ctx.check_error()
+    t_best, best = min(rows, key=lambda r: r[0])
+    sim.set_tuning(**best)
+    return {"best": best, "ms_per_step": t_best * 1e3,
+            "timings": [{**c, "ms_per_step": t * 1e3} for t, c in rows]}
+
+
+def tune_batch(batch, precision, steps=4, launches=2) -> dict:
+    """Per-case kernel tuning, then the graph depth of the batch (recaptures the graphs)."""
+    per_case = [tune_simulation(s, precision, steps=steps, repeats=1)["best"] for s in batch.sims]
+    timings = {}
+    for gs in GRAPH_STEPS:
+        if batch.graphs:
+            for g in batch.graphs:
+                g.destroy()
+        batch.graph_steps = gs
+        batch.capture()
+        batch.launch(1)
+        batch.synchronize()
+        reps = max(1, (launches * max(GRAPH_STEPS)) // gs)
+        t0 = time.perf_counter()
+        batch.launch(reps)
+        batch.synchronize()
+        timings[gs] = (time.perf_counter() - t0) / (reps * gs)
+    best = min(timings, key=timings.get)
+    for g in batch.graphs:
+        g.destroy()
+    batch.graph_steps = best
+    batch.capture()
+    batch.check_error()
+    return {"graph_steps": best, "ms_per_step": {k: v * 1e3 for k, v in timings.items()}, "per_case": per_case}
+
+
+def cache_key(n_particles, precision, device_name=""):
+    bucket = max(0, int(n_particles).bit_length() - 1)
+    return f"{device_name}|{'fp32c' if precision == _lib.FP32C else 'fp64'}|2^{bucket}"
+
+
+def load_cache(path) -> dict:
+    if path and os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def save_cache(path, cache: dict):
+    if path:
+        with open(path, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)

@@ -874,6 +874,23 @@ int sphfem_leap_frog_set_precision(sphfem_ctx *ctx, sphfem_leap_frog *lf, int pr
     });
 }
 
+int sphfem_leap_frog_set_tuning(sphfem_ctx *ctx, sphfem_leap_frog *lf, const sphfem_tuning *tuning) {
+    if (!ctx || !lf || !tuning) return SPHFEM_ERR_ARG;
+    const sphfem_tuning &t = *tuning;
+    const bool ok = (t.pair_block == 128 || t.pair_block == 256) &&
+                    (t.pair_chunk == 16 || t.pair_chunk == 32 || t.pair_chunk == 48) &&
+                    (t.update_block == 128 || t.update_block == 256);
+    if (!ok) return fail(ctx, SPHFEM_ERR_ARG, "unsupported tuning");
+    lf->tuning = t;
+    return SPHFEM_OK;
+}
+
+int sphfem_leap_frog_get_tuning(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_tuning *tuning) {
+    if (!ctx || !lf || !tuning) return SPHFEM_ERR_ARG;
+    *tuning = lf->tuning;
+    return SPHFEM_OK;
+}
+
 int sphfem_leap_frog_init(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p) {
     if (!lf || !p) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {

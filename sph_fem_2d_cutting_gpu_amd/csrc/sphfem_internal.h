@@ -125,6 +125,7 @@ struct sphfem_tool {
 struct sphfem_leap_frog {
     unsigned int num_part = 0, num_cell = 0;
     int precision = SPHFEM_FP64;
+    sphfem_tuning tuning = {256, 32, 256};
     sphfem_double2 *pos_init = nullptr, *vel_init = nullptr;
     sphfem_double4 *S_init = nullptr;
     double *rho_init = nullptr, *T_init = nullptr;

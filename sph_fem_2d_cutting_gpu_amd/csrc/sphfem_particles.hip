@@ -133,22 +133,8 @@ __device__ __forceinline__ PairSums pair_sums_fp64(const DevConsts &c, const sph
 }
 
 // ---------------------------------------------------------------------------------------------
-// FP32-compensated pair sums (same physics as pair_sums_fp64, different arithmetic).
-//  * positions: cell-relative FP32 (x_ij = dx_rel + (cell_i - cell_j) * h), so the pair distance
-//    carries ~1e-7 of the cell size whatever the absolute coordinates;
-//  * transcendental work is one v_rsq (the distance), one v_rcp (2 / (rho_i + rho_j)) and one
-//    v_rcp for the viscosity denominator of approaching pairs;
-//  * the grad-T term uses e_ij . grad W / r_ij = g (the radial kernel derivative factor), which
-//    is the reference's algebra without the extra sqrt and divisions;
-//  * each lane first compacts its in-support candidates (column order, sorted index within a
-//    column: the reference order) into an LDS list of NB_CHUNK entries, then runs the pair body
-//    over the dense list, so the wavefront does not execute the body for rejected candidates;
-//  * summation: blocks of 8 pair terms in FP32, block sums accumulated in FP64, which bounds the
-//    rounding of the sums by 8 units of FP32 roundoff of the sum of |terms|.
-// Constant factors (mass, -eps*mass, 2) are applied once in FP64 after the sums.
+// FP32-compensated form
 // ---------------------------------------------------------------------------------------------
-constexpr int NB_CHUNK = 32;
-constexpr int FUSED_BS = 256;
 
 // FP32C neighbour records (written by the reorder kernel), four SoA blocks of float4:
 //   r0[j] = {x - cell corner x, y - cell corner y, cell row, h}   (block padded by 4 records)
@@ -241,20 +227,21 @@ __device__ __forceinline__ void pair_term(const PairI &I, float hf, float inv_wd
 //  * the grad-T term uses e_ij . grad W / r_ij = g (the radial kernel-derivative factor): the
 //    reference's algebra without its extra sqrt and divisions;
 //  * each lane compacts its in-support candidates (columns in order, sorted index within a
-//    column: the reference order) into an LDS list of NB_CHUNK entries, loading candidates four
+//    column: the reference order) into an LDS list of CHUNK entries, loading candidates four
 //    at a time, then evaluates the dense list with the records of the next entry in flight;
 //  * summation: blocks of 8 pair terms in FP32, block sums accumulated in FP64, which bounds the
 //    rounding of each sum by ~8 units of FP32 roundoff of the sum of |terms|; one final FP32
 //    rounding on output.
 // Outputs (FP32): psum[i] = velocity gradient (xx, xy, yx, yy); psum[N + i] = (mass * stress
 // divergence - mass * viscous term) per axis, XSPH velocity per axis; psum_T[i] = Laplacian of T.
-__global__ void __launch_bounds__(FUSED_BS) k_pairs_fp32c(DevConsts c, const double *__restrict__ blanked,
+template <int BS, int CHUNK>
+__global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *__restrict__ blanked,
                                                          const int *__restrict__ hash, int N, RecView rv,
                                                          const int *__restrict__ cs, const int *__restrict__ ce,
                                                          float4 *__restrict__ psum, float *__restrict__ psum_T) {
 #pragma clang fp contract(fast)
-    __shared__ int nb_list[NB_CHUNK * FUSED_BS];
-    const int i = xcd_logical_block() * FUSED_BS + threadIdx.x;
+    __shared__ int nb_list[CHUNK * BS];
+    const int i = xcd_logical_block() * BS + threadIdx.x;
     if (i >= N) return;
     int *__restrict__ list = nb_list + threadIdx.x;
     double d_v0 = 0., d_v1 = 0., d_v2 = 0., d_v3 = 0.;
@@ -314,13 +301,13 @@ __global__ void __launch_bounds__(FUSED_BS) k_pairs_fp32c(DevConsts c, const dou
             unsigned oA = (unsigned)(eA & 0x0fffffff) << 4;
             float4 A0 = ldrec(rv.r0, oA), A1 = ldrec(rv.r1, oA), A2 = ldrec(rv.r2, oA), A3 = ldrec(rv.r3, oA);
             for (int k = 0; k < n; k += 2) {
-                const int eB = list[(k + 1 < n ? k + 1 : n - 1) * FUSED_BS];
+                const int eB = list[(k + 1 < n ? k + 1 : n - 1) * BS];
                 const unsigned oB = (unsigned)(eB & 0x0fffffff) << 4;
                 const float4 B0 = ldrec(rv.r0, oB), B1 = ldrec(rv.r1, oB), B2 = ldrec(rv.r2, oB),
                              B3 = ldrec(rv.r3, oB);
                 pair_term(I, hf, inv_wdp, alpha, beta, eta2, A0, A1, A2, A3, eA, acc);
                 if (k + 1 < n) {
-                    eA = list[(k + 2 < n ? k + 2 : n - 1) * FUSED_BS];
+                    eA = list[(k + 2 < n ? k + 2 : n - 1) * BS];
                     oA = (unsigned)(eA & 0x0fffffff) << 4;
                     A0 = ldrec(rv.r0, oA); A1 = ldrec(rv.r1, oA); A2 = ldrec(rv.r2, oA); A3 = ldrec(rv.r3, oA);
                     pair_term(I, hf, inv_wdp, alpha, beta, eta2, B0, B1, B2, B3, eB, acc);
@@ -357,11 +344,11 @@ __global__ void __launch_bounds__(FUSED_BS) k_pairs_fp32c(DevConsts c, const dou
                     const float yij = (I.fyi - bb[u].y) + (I.rowi - bb[u].z) * hf;
                     const float rr2 = xij * xij + yij * yij;
                     if (j + u < last && rr2 < fourh2 && rr2 >= rr2min) {
-                        list[n * FUSED_BS] = (j + u) | tag;
+                        list[n * BS] = (j + u) | tag;
                         n++;
                     }
                 }
-                if (n > NB_CHUNK - 4) {                                  // dense region: empty the list
+                if (n > CHUNK - 4) {                                     // dense region: empty the list
                     run_list(n);
                     n = 0;
                 }
@@ -533,8 +520,8 @@ void launch_leapfrog(sphfem_ctx *ctx, int which, const sphfem_particles &p, sphf
 // is dead after the reorder except the previous derivatives.  Outputs: corrected state into
 // A.pos/vel/S/rho/T/eps_pl/eps_pl_dot/blanked, derivatives into B.pos_t/vel_t/rho_t/S_t/T_t,
 // contact fields in place in B.fc/ft/n/on_seg; S_der/v_der into A when requested.
-template <bool F32C>
-__global__ void __launch_bounds__(FUSED_BS) k_fused_update(DevConsts c, sphfem_particles B, sphfem_particles A,
+template <bool F32C, int BS>
+__global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particles B, sphfem_particles A,
                                                           const sphfem_double2 *__restrict__ pos_init,
                                                           const sphfem_double2 *__restrict__ vel_init,
                                                           const sphfem_double4 *__restrict__ S_init,
@@ -544,7 +531,7 @@ __global__ void __launch_bounds__(FUSED_BS) k_fused_update(DevConsts c, sphfem_p
                                                           ToolView tv, int write_intermediates,
                                                           const float4 *__restrict__ psum,
                                                           const float *__restrict__ psum_T) {
-    const int i = xcd_logical_block() * FUSED_BS + threadIdx.x;
+    const int i = xcd_logical_block() * BS + threadIdx.x;
     if (i >= (int)B.N) return;
     const double blanked0 = B.blanked[i];
     const double tool = B.tool_particle[i];
@@ -702,18 +689,37 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
         RecView rv{lf->rec, lf->rec + n + 4, lf->rec + 2 * n + 4, lf->rec + 3 * n + 4};
         {
             KTimer tm(ctx, "pairs_fp32c");
-            k_pairs_fp32c<<<nblk(B.N, FUSED_BS), FUSED_BS, 0, ctx->stream>>>(ctx->c, B.blanked, B.hash, (int)B.N,
-                                                                             rv, cs, ce, lf->psum, lf->psum_T);
+            const int pb = lf->tuning.pair_block, pc = lf->tuning.pair_chunk;
+#define SPHFEM_PAIRS(BSZ, CH)                                                                              \
+    k_pairs_fp32c<BSZ, CH><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(ctx->c, B.blanked, B.hash, (int)B.N, rv, cs, \
+                                                                    ce, lf->psum, lf->psum_T)
+            if (pb == 128 && pc == 16) SPHFEM_PAIRS(128, 16);
+            else if (pb == 128 && pc == 32) SPHFEM_PAIRS(128, 32);
+            else if (pb == 128 && pc == 48) SPHFEM_PAIRS(128, 48);
+            else if (pb == 256 && pc == 16) SPHFEM_PAIRS(256, 16);
+            else if (pb == 256 && pc == 48) SPHFEM_PAIRS(256, 48);
+            else SPHFEM_PAIRS(256, 32);
+#undef SPHFEM_PAIRS
         }
         KTimer tm(ctx, "update_fp32c");
-        k_fused_update<true><<<nblk(B.N, FUSED_BS), FUSED_BS, 0, ctx->stream>>>(
-            ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
-            lf->psum, lf->psum_T);
+        if (lf->tuning.update_block == 128)
+            k_fused_update<true, 128><<<nblk(B.N, 128), 128, 0, ctx->stream>>>(
+                ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
+                lf->psum, lf->psum_T);
+        else
+            k_fused_update<true, 256><<<nblk(B.N, 256), 256, 0, ctx->stream>>>(
+                ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
+                lf->psum, lf->psum_T);
     } else {
         KTimer tm(ctx, "fused_update");
-        k_fused_update<false><<<nblk(B.N, FUSED_BS), FUSED_BS, 0, ctx->stream>>>(
-            ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
-            nullptr, nullptr);
+        if (lf->tuning.update_block == 128)
+            k_fused_update<false, 128><<<nblk(B.N, 128), 128, 0, ctx->stream>>>(
+                ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
+                nullptr, nullptr);
+        else
+            k_fused_update<false, 256><<<nblk(B.N, 256), 256, 0, ctx->stream>>>(
+                ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
+                nullptr, nullptr);
     }
 }
 

@@ -295,6 +295,15 @@ class leap_frog:
     def step(self, p: particle_gpu, g: grid_gpu_green, tool: tool_FEM | None = None):
         self.ctx.call("sphfem_leap_frog_step", self.h, C.byref(p.s), g.h, tool.h if tool else None)
 
+    def set_tuning(self, pair_block: int = 256, pair_chunk: int = 32, update_block: int = 256):
+        t = _lib.Tuning(int(pair_block), int(pair_chunk), int(update_block))
+        self.ctx.call("sphfem_leap_frog_set_tuning", self.h, C.byref(t))
+
+    def get_tuning(self) -> dict:
+        t = _lib.Tuning()
+        self.ctx.call("sphfem_leap_frog_get_tuning", self.h, C.byref(t))
+        return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}
+
     def capture(self, p: particle_gpu, g: grid_gpu_green, tool: tool_FEM | None = None, steps: int = 2):
         """Record `steps` (even) steps into a HIP graph; returns a StepGraph.  Needs one eager step first."""
         gh = C.c_void_p()
@@ -396,6 +405,12 @@ class Simulation:
     def capture(self, steps: int = 2) -> StepGraph:
         """Graph of `steps` steps (one eager step must have run)."""
         return self.stepper.capture(self.p, self.grid, self.tool, steps)
+
+    def set_tuning(self, **kw):
+        self.stepper.set_tuning(**{**self.stepper.get_tuning(), **kw})
+
+    def get_tuning(self) -> dict:
+        return self.stepper.get_tuning()
 
     def synchronize(self):
         self.ctx.synchronize()

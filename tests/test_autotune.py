@@ -1,0 +1,18 @@
+"""Autotuner bookkeeping (CPU): candidate sets and the JSON cache."""
+from sph_fem_2d_cutting_gpu_amd import _lib, autotune
+
+
+def test_candidate_sets():
+    c32 = autotune.candidates(_lib.FP32C)
+    assert len(c32) == 12 and {"pair_block": 256, "pair_chunk": 32, "update_block": 256} in c32
+    c64 = autotune.candidates(_lib.FP64)
+    assert all(c["pair_block"] == 256 and c["pair_chunk"] == 32 for c in c64)
+
+
+def test_cache_roundtrip(tmp_path):
+    path = str(tmp_path / "tuning.json")
+    key = autotune.cache_key(4_000_000, _lib.FP32C, "gfx950")
+    assert key == "gfx950|fp32c|2^21"
+    autotune.save_cache(path, {key: {"pair_block": 128, "pair_chunk": 48, "update_block": 256}})
+    assert autotune.load_cache(path)[key]["pair_chunk"] == 48
+    assert autotune.load_cache(str(tmp_path / "absent.json")) == {}

@@ -1,0 +1,60 @@
+"""Every launch-parameter candidate of the autotuner gives the default's results: bit-identical
+sorted order; FP64 state identical (to the segment-atomics rounding); FP32C derivatives within
+1e-6 norm-wise (the FP32 summation blocks follow the list chunks), on the cutting case and on the
+10:1-density chip domain where the neighbour lists overflow their chunks."""
+import numpy as np
+import pytest
+
+from helpers import assert_equal
+from sph_fem_2d_cutting_gpu_amd import _lib, autotune, cases
+from sph_fem_2d_cutting_gpu_amd import solver as S
+
+pytestmark = pytest.mark.gpu
+
+DERIVS = ("pos_t", "vel_t", "rho_t", "S_t", "T_t")
+
+
+def _run(case, prec, tuning, steps):
+    sim = S.Simulation(case, precision=prec)
+    sim.set_tuning(**tuning)
+    sim.step(steps)
+    sim.ctx.check_error()
+    out = sim.state(("hash", "idx") + DERIVS + ("pos", "vel", "S", "T"))
+    sim.close()
+    return out
+
+
+@pytest.mark.parametrize("prec", [_lib.FP64, _lib.FP32C])
+@pytest.mark.parametrize("which", ["cutting", "chip"])
+def test_all_candidates_agree(prec, which):
+    if which == "cutting":
+        case, steps = cases.perturbed(cases.small_cutting_case(nx=48, ny=16, wear=True)), 3
+    else:
+        case, steps = cases.chip_case(n_target=100_000), 1
+    ref = _run(case, prec, {"pair_block": 256, "pair_chunk": 32, "update_block": 256}, steps)
+    tol = 1e-11 if prec == _lib.FP64 else 1e-6
+    for cand in autotune.candidates(prec):
+        got = _run(case, prec, cand, steps)
+        assert_equal(f"{cand} hash", got["hash"], ref["hash"])
+        assert_equal(f"{cand} idx", got["idx"], ref["idx"])
+        for f in DERIVS + ("pos", "vel", "S", "T"):
+            scale = max(float(np.max(np.abs(ref[f]))), 1e-300)
+            err = float(np.max(np.abs(got[f] - ref[f]))) / scale
+            assert err <= tol, f"{cand} {f}: {err:.3e}"
+
+
+def test_tune_simulation_applies_a_candidate():
+    sim = S.Simulation(cases.small_cutting_case(nx=32, ny=12), precision=_lib.FP32C)
+    sim.step(1)
+    res = autotune.tune_simulation(sim, _lib.FP32C, steps=2, repeats=1)
+    assert res["best"] in autotune.candidates(_lib.FP32C)
+    assert sim.get_tuning() == res["best"]
+    assert len(res["timings"]) == 12
+    sim.close()
+
+
+def test_invalid_tuning_is_rejected():
+    sim = S.Simulation(cases.small_cutting_case(nx=16, ny=8))
+    with pytest.raises(_lib.SphfemError):
+        sim.set_tuning(pair_block=96)
+    sim.close()
