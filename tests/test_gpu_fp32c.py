@@ -110,3 +110,33 @@ def test_fp32c_observables_2000_steps():
     print(f"fp32c force error {ferr}, dT {dT_g:.6g} vs {dT_o:.6g}")
     assert np.all(ferr <= 5e-3), ferr
     assert abs(dT_g - dT_o) <= 5e-3 * dT_o
+
+
+@pytest.mark.parametrize("prec", [_lib.FP64, _lib.FP32C])
+def test_plastic_observables_2000_steps(prec):
+    """6,400-particle case started from a random deviatoric stress near yield (600 MPa per component):
+    plastic flow and plastic-work heating act over the whole 2000-step horizon (peak dT ~80 K, max
+    eps_pl ~0.04).  Cutting force within 0.5 % of the oracle's running maximum; peak temperature rise,
+    max and mean plastic strain within 0.5 %."""
+    case = cases.perturbed(cases.config1(), vel_sd=1e-5, S_sd=6e-3, rho_rel_sd=1e-4, T_sd=20.0)
+    o = Oracle(case)
+    sim = S.Simulation(case, precision=prec)
+    F_o, F_g = [], []
+    for k in range(100):
+        o.step(20)
+        sim.step(20)
+        F_o.append(o.report_force())
+        F_g.append(S.report_force(sim.p))
+    sim.ctx.check_error()
+    F_o, F_g = np.array(F_o), np.array(F_g)
+    ferr = np.max(np.abs(F_g - F_o), axis=0) / np.max(np.abs(F_o), axis=0)
+    a = sim.state(("T", "eps_pl", "blanked"))
+    T0 = case.const.joco["Tref"]
+    dT_o, dT_g = o.P["T"].max() - T0, a["T"].max() - T0
+    print(f"force error {ferr}, dT {dT_g:.8g} vs {dT_o:.8g}, eps max {a['eps_pl'].max():.8g} vs "
+          f"{o.P['eps_pl'].max():.8g}")
+    assert dT_o > 10.0 and o.P["eps_pl"].max() > 0.01          # the horizon is plastic
+    assert np.all(ferr <= 5e-3), ferr
+    assert abs(dT_g - dT_o) <= 5e-3 * dT_o
+    assert abs(a["eps_pl"].max() - o.P["eps_pl"].max()) <= 5e-3 * o.P["eps_pl"].max()
+    assert abs(a["eps_pl"].mean() - o.P["eps_pl"].mean()) <= 5e-3 * o.P["eps_pl"].mean()

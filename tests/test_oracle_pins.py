@@ -415,3 +415,181 @@ def test_small_run_is_finite_and_conserves_particles():
     assert sorted(o.P["idx"].tolist()) == list(range(case.N))
     assert (o.P["on_seg"] >= 0).sum() > 0        # contact established
     assert o.t["T"].max() > 300.0                 # the tool heats up
+
+
+# ---------------------------------------------------------------- friction heat, contact heat, wear
+def _f32sqrt(x):
+    """(double) sqrtf((float) x): the reference's single-precision root of a double."""
+    return float(np.sqrt(np.float32(x)))
+
+
+def _one_contact_particle(o, seg):
+    o.P["on_seg"][:] = -1
+    o.P["on_seg"][0] = seg
+    for k in ("fric_heat", "sliding_force"):
+        o.t[k][:] = 0.0
+    o.t["physical_para"][:] = 0.0
+    o.t["heat_exchange"][:] = 0.0
+
+
+@pytest.mark.parametrize("vy,T0,expect", [(2e-3, 400.0, "full"), (1e-2, 400.0, "no_usui"), (4e-2, 400.0, "none"),
+                                          (2e-3, 1549.99995, "capped")])
+def test_fric_heat_closed_form(vy, T0, expect):
+    """actions_gpu.cu:345-420 for one particle in contact: power eta |f_T| |v_rel,t| split (1-beta) into the
+    particle's temperature and beta into the segment; the Usui inputs (count, pressure, v_rel, T_seg)
+    and the sliding force only below 0.008 cm/us; nothing above 0.03; T capped at 1550 K."""
+    case = cases.small_cutting_case(nx=16, ny=8)
+    o = O.Oracle(case)
+    seg = 2
+    _one_contact_particle(o, seg)
+    o.Tc.vel_x, o.Tc.vel_y = 1.1667e-4, 0.0
+    o.P["n"][0] = [1.0, 0.0]
+    o.P["vel"][0] = [3e-4, vy]                     # normal component is removed: v_rel,t = vy
+    o.P["ft"][0] = [0.0, -2.0e-9]
+    o.P["fc"][0] = [5.0e-9, 0.0]
+    o.P["rho"][0] = 4.5
+    o.P["T"][0] = T0
+    o.P["eps_pl"][0] = 0.3
+    o.t["T"][:] = 350.0
+    o.call("o_fric_heat_v21", "cpt")
+    tc, c = case.const.tool, case.const
+    f_fric = _f32sqrt(4.0e-18)
+    v_rel = _f32sqrt(vy * vy)
+    value = tc["eta"] * f_fric * v_rel
+    length = _f32sqrt(tc["slave_mass"] / 4.5)
+    T_new = T0 + value * (1 - tc["beta"]) * case.dt / c.trml_wp["cp"] / c.phys["mass"]
+    if expect == "none":
+        assert o.P["T"][0] == T0 and o.t["fric_heat"][seg] == 0.0
+        return
+    if expect == "capped":
+        assert T_new > 1550.0
+        assert o.P["T"][0] == 1550.0 and o.t["fric_heat"][seg] == 0.0
+        return
+    assert o.P["T"][0] == pytest.approx(T_new, rel=1e-15)
+    assert o.t["fric_heat"][seg] == pytest.approx(value * tc["beta"], rel=1e-15)
+    if expect == "no_usui":
+        assert np.all(o.t["physical_para"][seg] == 0.0) and o.t["sliding_force"][seg] == 0.0
+        return
+    pp = o.t["physical_para"][seg]
+    assert pp[0] == 1.0
+    assert pp[1] == pytest.approx(_f32sqrt(25e-18) / length, rel=1e-15)
+    assert pp[2] == pytest.approx(v_rel, rel=1e-15) and pp[3] == 350.0
+    assert o.t["sliding_force"][seg] == pytest.approx(f_fric / length, rel=1e-15)
+    others = np.ones(len(o.t["fric_heat"]), bool)
+    others[seg] = False
+    assert np.all(o.t["fric_heat"][others] == 0.0)
+
+
+@pytest.mark.parametrize("T_tool,h_scale", [(500.0, 1.0), (300.0, 1.0), (900.0, 1e12)])
+def test_contact_heat_exchange_closed_form(T_tool, h_scale):
+    """actions_gpu.cu:585-617: the segment collects T_p * l and l (l = sqrt(m_slave / rho)); the particle moves
+    towards the segment temperature by h (T_seg - T_p) l dt / (c_p m_slave), never past T_seg."""
+    case = cases.small_cutting_case(nx=16, ny=8)
+    o = O.Oracle(case)
+    seg = 4
+    _one_contact_particle(o, seg)
+    o.c.tc.h = case.const.tool["h"] * h_scale
+    Tp, rho = 410.0, 4.4
+    o.P["T"][0], o.P["rho"][0] = Tp, rho
+    l_no, r_no = o.t["marks"][seg][2], o.t["marks"][seg][3]
+    o.t["T"][:] = 300.0
+    o.t["T"][l_no] = T_tool
+    o.t["T"][r_no] = T_tool + 20.0
+    o.call("o_contact_s2", "cpt")
+    T_seg = T_tool + 10.0
+    l = _f32sqrt(case.const.tool["slave_mass"] / rho)
+    assert o.t["heat_exchange"][seg][0] == pytest.approx(Tp * l, rel=1e-15)
+    assert o.t["heat_exchange"][seg][1] == pytest.approx(l, rel=1e-15)
+    d = (T_seg - Tp) * o.c.tc.h * l / case.const.trml_wp["cp"] / case.const.tool["slave_mass"] * case.dt
+    want = Tp + (max(d, T_seg - Tp) if d < 0 else min(d, T_seg - Tp))
+    assert o.P["T"][0] == pytest.approx(want, rel=1e-15)
+    if h_scale > 1.0:
+        assert o.P["T"][0] == pytest.approx(T_seg, rel=1e-15)      # clamped at the segment temperature
+
+
+def test_usui_wear_closed_form():
+    """tool_FEM.cu:299-432 (USUI_RAKE_FLANK, RESTRICT_WEAR_TEMP): for a segment with Usui inputs
+    (count, sum p, sum v, sum T_seg) both end nodes move along the segment normal by
+    dt C p v exp(-D / T_node) on the rake (|n.x| > 0.85), A, B on the flank; T <= 1350, v <= 0.008,
+    p <= 0.04; the segment end points, normal and flank retreat follow the moved nodes."""
+    case = cases.small_cutting_case(nx=16, ny=8, wear=True)
+    o = O.Oracle(case, wear_pro=True)
+    o.c.wear.A, o.c.wear.B, o.c.wear.C, o.c.wear.D = 86.0, 5000.0, 40.0, 9000.0     # distinct branches
+    t = o.t
+    t["physical_para"][:] = 0.0
+    rake = int(np.argmax(np.abs(t["n"][:, 0])))
+    flank = int(np.argmin(np.abs(t["n"][:, 0])))
+    assert abs(t["n"][rake][0]) > 0.85 and abs(t["n"][flank][0]) <= 0.85
+    t["physical_para"][rake] = [2.0, 2 * 0.03, 2 * 0.005, 0.0]
+    t["physical_para"][flank] = [1.0, 0.09, 0.002, 0.0]           # pressure clamped to 0.04
+    t["T"][:] = 300.0
+    for s, T in ((rake, 1500.0), (flank, 900.0)):                  # 1500 K clamped to 1350 K
+        t["T"][t["marks"][s][2]] = T
+        t["T"][t["marks"][s][3]] = T - 100.0
+    nodes0 = t["node_pos"].copy()
+    n0 = t["n"].copy()
+    o.call("o_wear_adjustment", "ct")
+    dt = case.dt
+    for s, (C_, D_), p, v in ((rake, (40.0, 9000.0), 0.03, 0.005), (flank, (86.0, 5000.0), 0.04, 0.002)):
+        l_no, r_no = t["marks"][s][2], t["marks"][s][3]
+        Tl = min(t["T"][l_no], 1350.0)
+        Tr = min(t["T"][r_no], 1350.0)
+        wl = dt * C_ * p * v * math.exp(-D_ / Tl)
+        wr = dt * C_ * p * v * math.exp(-D_ / Tr)
+        assert wl > 1e-13 and wr > 1e-13                            # resolvable against coordinate rounding
+        ulp = 2e-17                                                 # rounding of x + shift at |x| ~ 0.06
+        assert np.allclose(t["node_pos"][l_no] - nodes0[l_no], n0[s] * wl, rtol=1e-9, atol=ulp)
+        assert np.allclose(t["node_pos"][r_no] - nodes0[r_no], n0[s] * wr, rtol=1e-9, atol=ulp)
+        assert t["wear_nodes"][s] == pytest.approx((wl + wr) / dt, rel=1e-12)
+    moved = np.any(t["node_pos"] != nodes0, axis=1)
+    assert moved.sum() == 4
+    lp, rp = t["left"], t["right"]
+    assert np.array_equal(lp, t["node_pos"][t["marks"][:, 2]]) and np.array_equal(rp, t["node_pos"][t["marks"][:, 3]])
+    nn = np.stack([rp[:, 1] - lp[:, 1], -(rp[:, 0] - lp[:, 0])], axis=1)
+    assert np.allclose(t["n"], nn / np.linalg.norm(nn, axis=1)[:, None], rtol=0, atol=1e-15)
+    assert o.Tc.flank_retreat == pytest.approx(o.Tc.flank_contact - lp[:, 1].min(), abs=1e-18)
+
+
+def _sigma_y(jc, eps, eps_dot, T):
+    theta = min((T - jc["Tref"]) / (jc["Tmelt"] - jc["Tref"]), 1.0)
+    rate = eps_dot / jc["eps_dot_ref"]
+    tb = 1.0 + jc["C"] * math.log(rate) if rate > 1.0 else (1.0 + rate) ** jc["C"]
+    return (jc["A"] + jc["B"] * eps ** jc["n"]) * tb * (1.0 - theta ** jc["m"])
+
+
+def test_plasticity_return_mapping():
+    """actions_gpu.cu:689-754: with dlambda from the secant (pinned above) started at
+    max(eps_dot dt sqrt(2/3), 1e-8) with tolerance 1e-6, the return scales the deviatoric stress by
+    (1 - 2 mu dlambda / |S|) (|S| the single-precision trial norm), eps += sqrt(2/3) dlambda,
+    eps_dot = sqrt(2/3) dlambda / dt, and plastic work heats by tq / (c_p rho) d(eps) sigma_Y(new);
+    an elastic trial (von Mises below sigma_Y) only resets eps_dot."""
+    case = cases.small_cutting_case(nx=16, ny=8)
+    o = O.Oracle(case)
+    jc, G, dt = case.const.joco, case.const.phys["G"], case.dt
+    sd = np.array([0.012, 0.004, -0.009])                      # deviatoric trial above yield
+    o.P["S"][:] = 0.0
+    o.P["S"][0] = [sd[0], sd[1], sd[2], -(sd[0] + sd[2])]
+    o.P["S"][1] = [0.002, 0.0, -0.001, -0.001]                 # elastic
+    o.P["p"][:] = 0.0
+    o.P["eps_pl"][:2] = [0.1, 0.05]
+    o.P["eps_pl_dot"][:2] = [1e-6 * 50, 3e-6]
+    o.P["T"][:2] = [500.0, 400.0]
+    o.P["rho"][:2] = 4.43
+    S0 = o.P["S"][0].copy()
+    o.call("o_plasticity", "cp")
+    Sx, Sxy, Sy, Sz = S0
+    normS = _f32sqrt(Sx * Sx + Sy * Sy + Sz * Sz + 2 * Sxy * Sxy)   # the reference's (float) trial norm
+    init = max(1e-6 * 50 * dt * math.sqrt(2.0 / 3.0), 1e-8)
+    dl, _ = O.solve_secant(jc, init, 1e-6, normS, 0.1, 500.0, dt, G)
+    assert dl > 0
+    S1 = o.P["S"][0]
+    assert np.allclose(S1, S0 * (1 - 2 * G * dl / normS), rtol=1e-13, atol=0)
+    s23 = float(np.sqrt(np.float32(2.0 / 3.0)))
+    dlf = float(np.float32(dl))
+    eps_new, epsd_new = o.P["eps_pl"][0], o.P["eps_pl_dot"][0]
+    assert eps_new == pytest.approx(0.1 + s23 * dlf, rel=1e-15)
+    assert epsd_new == pytest.approx(s23 * dlf / dt, rel=1e-15)
+    sy = _sigma_y(jc, eps_new, epsd_new, 500.0)
+    heat = case.const.trml_wp["tq"] / (case.const.trml_wp["cp"] * 4.43) * (eps_new - 0.1) * sy
+    assert o.P["T"][0] == pytest.approx(500.0 + heat, rel=1e-12)
+    assert np.array_equal(o.P["S"][1], [0.002, 0.0, -0.001, -0.001]) and o.P["eps_pl_dot"][1] == 0.0
