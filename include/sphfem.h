@@ -147,6 +147,10 @@ int sphfem_grid_get_cells(sphfem_ctx *ctx, sphfem_grid *g, int *cell_start, int 
 int sphfem_interactions_setup_geometry_constants(sphfem_ctx *ctx, const sphfem_grid *g);
 int sphfem_interactions_monaghan(sphfem_ctx *ctx, sphfem_particles *p, const int *cell_start,
                                  const int *cell_end, int num_cell);
+/* interactions_gpu.cu:84-140, 434-464 (PSE heat conduction, compiled out of the reference's
+ * flag set): T_t = alpha sum (T_j - T_i) 4/(pi h^4) exp(-r^2/h^2) m/rho_j over the 5x5 cells */
+int sphfem_interactions_heat_pse(sphfem_ctx *ctx, sphfem_particles *p, const int *cell_start,
+                                 const int *cell_end, int num_cell);
 
 /* ---- per-particle actions (actions_gpu.h:33-72) */
 int sphfem_material_eos(sphfem_ctx *ctx, sphfem_particles *p);

@@ -336,6 +336,11 @@ inline void interactions_monaghan(particle_gpu *p, const int *cell_start, const 
                      "interactions_monaghan");
 }
 
+inline void interactions_heat_pse(particle_gpu *p, const int *cell_start, const int *cell_end, int num_cell) {
+    SPHFEM_SHIM_CALL(sphfem_interactions_heat_pse(sphfem_shim::ctx(), p, cell_start, cell_end, num_cell),
+                     "interactions_heat_pse");
+}
+
 inline void material_eos(particle_gpu *p) { SPHFEM_SHIM_CALL(sphfem_material_eos(sphfem_shim::ctx(), p), "material_eos"); }
 inline void corrector_artificial_stress(particle_gpu *p) {
     SPHFEM_SHIM_CALL(sphfem_corrector_artificial_stress(sphfem_shim::ctx(), p), "corrector_artificial_stress");

@@ -87,6 +87,7 @@ def lib():
             "o_compute_hashes": (pc, pp), "o_sort_and_cells": (pc, pp, ps), "o_init": (pp, ps),
             "o_predict": (pp, ps, _dbl), "o_correct": (pp, ps, _dbl), "o_material_eos": (pc, pp),
             "o_artificial_stress": (pc, pp), "o_interactions_monaghan": (pc, pp, ps),
+            "o_interactions_heat_pse": (pc, pp, ps),
             "o_stress_rate_jaumann": (pc, pp), "o_continuity": (pp,), "o_momentum": (pc, pp),
             "o_advection": (pp,), "o_fric_heat_v21": (pc, pp, pt), "o_plasticity": (pc, pp),
             "o_contact_s1": (pc, pp, pt), "o_contact_s2": (pc, pp, pt), "o_boundary_conditions": (pp,),

@@ -16,6 +16,8 @@
 
 #include <float.h>
 #include <math.h>
+
+#define O_PI 3.14159265358979323846   /* M_PI */
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -272,6 +274,51 @@ void o_artificial_stress(const o_consts *c, o_particles *P) {
         Ri.z = sin_theta2 * rot_rxx + cos_theta2 * rot_ryy;
         Ri.w = 0.;
         P->R[i] = Ri;
+    }
+}
+
+/* interactions_gpu.cu:84-140, 434-464 with lapl_pse of kernels.cuh:67-84: particle strength
+ * exchange heat conduction (compiled out of the reference's flag set, kept as an entry point).
+ * Every particle of the 5x5 cells (no support cut-off, blanked neighbours included),
+ * T_t = alpha * sum (T_j - T_i) 4/(pi h^4) exp(-|x_ij|^2/h^2) m/rho_j, alpha of the tool or the
+ * workpiece block.  Off when the workpiece is not thermal. */
+void o_interactions_heat_pse(const o_consts *c, o_particles *P, const o_stepper *st) {
+    if (c->trml_wp.alpha == 0.) return;
+    const double mass = c->phys.mass;
+    const int nx = c->geom.nx, ny = c->geom.ny;
+#pragma omp parallel for schedule(dynamic, 128)
+    for (int pidx = 0; pidx < P->N; pidx++) {
+        if (P->blanked[pidx] == 1.) continue;
+        o_d2 pi = P->pos[pidx];
+        double hi = P->h[pidx], Ti = P->T[pidx];
+        double alpha = (P->tool_particle[pidx] == 1.) ? c->trml_tool.alpha : c->trml_wp.alpha;
+        int hashi = P->hash[pidx];
+        int gi = hashi / ny;
+        int gj = hashi - gi * ny;
+        int low_i = gi - 2 < 0 ? 0 : gi - 2;
+        int low_j = gj - 2 < 0 ? 0 : gj - 2;
+        int high_i = gi + 3 > nx ? nx : gi + 3;
+        int high_j = gj + 3 > ny ? ny : gj + 3;
+        double T_ti = 0.;
+        for (int ii = low_i; ii < high_i; ii++) {
+            for (int jj = low_j; jj < high_j; jj++) {
+                int idx = ii * ny + jj;
+                int c_start = st->cell_start[idx];
+                if (c_start == -1) continue;
+                int c_end = st->cell_end[idx];
+                for (int iter = c_start; iter < c_end; iter++) {
+                    o_d2 pj = P->pos[iter];
+                    double Tj = P->T[iter], rhoj = P->rho[iter];
+                    double xij = pi.x - pj.x, yij = pi.y - pj.y;
+                    double xx = sqrt(xij * xij + yij * yij);
+                    double h2 = hi * hi;
+                    double h4 = h2 * h2;
+                    double w2 = +4. / (h4 * O_PI) * exp(-xx * xx / (h2));
+                    T_ti += (Tj - Ti) * w2 * mass / rhoj;
+                }
+            }
+        }
+        P->T_t[pidx] = alpha * T_ti;
     }
 }
 

@@ -449,10 +449,21 @@ static int need(sphfem_ctx *ctx, unsigned mask) {
 
 int sphfem_interactions_monaghan(sphfem_ctx *ctx, sphfem_particles *p, const int *cell_start, const int *cell_end,
                                  int num_cell) {
-    if (!p || !cell_start || !cell_end || num_cell != ctx->c.geom.nx * ctx->c.geom.ny) return SPHFEM_ERR_ARG;
+    if (!ctx || !p || !cell_start || !cell_end || num_cell != ctx->c.geom.nx * ctx->c.geom.ny) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {
         if (int rc = need(ctx, SET_PHYS | SET_CORR | SET_TRML_WP)) return rc;
         launch_interactions_ref(ctx, *p, cell_start, cell_end);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_interactions_heat_pse(sphfem_ctx *ctx, sphfem_particles *p, const int *cell_start, const int *cell_end,
+                                 int num_cell) {
+    if (!ctx || !p || !cell_start || !cell_end || num_cell != ctx->c.geom.nx * ctx->c.geom.ny) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (int rc = need(ctx, SET_PHYS | SET_TRML_WP)) return rc;
+        if (ctx->c.trml_wp.alpha == 0.) return SPHFEM_OK;     // the workpiece is not thermal
+        launch_interactions_pse(ctx, *p, cell_start, cell_end);
         return SPHFEM_OK;
     });
 }

@@ -182,6 +182,7 @@ void launch_reorder_predict(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_partic
 void launch_actions(sphfem_ctx *ctx, int which, const sphfem_particles &p);
 void launch_leapfrog(sphfem_ctx *ctx, int which, const sphfem_particles &p, sphfem_leap_frog *lf, double dt);
 void launch_interactions_ref(sphfem_ctx *ctx, const sphfem_particles &p, const int *cs, const int *ce);
+void launch_interactions_pse(sphfem_ctx *ctx, const sphfem_particles &p, const int *cs, const int *ce);
 void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfem_particles &A,
                          sphfem_leap_frog *lf, const int *cs, const int *ce, sphfem_tool *t);
 void launch_report_force(sphfem_ctx *ctx, const sphfem_particles &p, double *d_out);

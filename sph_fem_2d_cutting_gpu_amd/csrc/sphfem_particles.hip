@@ -383,6 +383,50 @@ __global__ void __launch_bounds__(256) k_interactions_ref(DevConsts c, sphfem_pa
     if (thermal_alpha != 0.) P.T_t[pidx] = thermal_alpha * r.T_lapl;
 }
 
+// interactions_gpu.cu:84-140 with lapl_pse (kernels.cuh:67-84): particle strength exchange heat
+// conduction over every particle of the 5x5 cells (no support cut-off; blanked neighbours count),
+// the column-run order being the reference's cell order
+__global__ void __launch_bounds__(256) k_interactions_pse(DevConsts c, sphfem_particles P, const int *__restrict__ cs,
+                                                          const int *__restrict__ ce) {
+    const int pidx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pidx >= (int)P.N) return;
+    if (P.blanked[pidx] == 1.) return;
+    const double mass = c.phys.mass;
+    const int nx = c.geom.nx, ny = c.geom.ny;
+    const sphfem_double2 pi = P.pos[pidx];
+    const double hi = P.h[pidx], Ti = P.T[pidx];
+    const double alpha = (P.tool_particle[pidx] == 1.) ? c.trml_tool.alpha : c.trml_wp.alpha;
+    const int hashi = P.hash[pidx];
+    const int gi = hashi / ny;
+    const int gj = hashi - gi * ny;
+    const int low_i = gi - 2 < 0 ? 0 : gi - 2;
+    const int low_j = gj - 2 < 0 ? 0 : gj - 2;
+    const int high_i = gi + 3 > nx ? nx : gi + 3;
+    const int high_j = gj + 3 > ny ? ny : gj + 3;
+    double T_ti = 0.;
+    for (int ii = low_i; ii < high_i; ii++) {
+        int first, last;
+        column_range(cs, ce, ii * ny, low_j, high_j, first, last);
+        if (first < 0 || last > (int)P.N) continue;
+        for (int iter = first; iter < last; iter++) {
+            const sphfem_double2 pj = P.pos[iter];
+            const double Tj = P.T[iter], rhoj = P.rho[iter];
+            const double xij = pi.x - pj.x, yij = pi.y - pj.y;
+            const double xx = sqrt(xij * xij + yij * yij);
+            const double h2 = hi * hi;
+            const double h4 = h2 * h2;
+            const double w2 = +4. / (h4 * 3.14159265358979323846) * exp(-xx * xx / (h2));
+            T_ti += (Tj - Ti) * w2 * mass / rhoj;
+        }
+    }
+    P.T_t[pidx] = alpha * T_ti;
+}
+
+void launch_interactions_pse(sphfem_ctx *ctx, const sphfem_particles &p, const int *cs, const int *ce) {
+    KTimer t(ctx, "interactions_pse");
+    k_interactions_pse<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, cs, ce);
+}
+
 void launch_interactions_ref(sphfem_ctx *ctx, const sphfem_particles &p, const int *cs, const int *ce) {
     KTimer t(ctx, "interactions_ref");
     k_interactions_ref<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, cs, ce);

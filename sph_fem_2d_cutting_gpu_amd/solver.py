@@ -377,6 +377,12 @@ def interactions_monaghan(p: particle_gpu, cell_start_dptr, cell_end_dptr, num_c
                C.c_void_p(cell_end_dptr), int(num_cell))
 
 
+def interactions_heat_pse(p: particle_gpu, cell_start_dptr, cell_end_dptr, num_cell):
+    """interactions_gpu.cu:434-464: PSE heat conduction into T_t."""
+    p.ctx.call("sphfem_interactions_heat_pse", C.byref(p.s), C.c_void_p(cell_start_dptr),
+               C.c_void_p(cell_end_dptr), int(num_cell))
+
+
 def report_force(p: particle_gpu):
     out = _lib.Double2()
     p.ctx.call("sphfem_report_force", C.byref(p.s), C.byref(out))

@@ -22,6 +22,7 @@ pytestmark = pytest.mark.gpu
 ATOMIC_RTOL = 1e-13    # sums of the same terms in another order
 LIBM_RTOL = 1e-6       # float trig of the artificial stress (ulp-level differences, relative to max|R|)
 POW_RTOL = 1e-10       # double pow/log/exp of Johnson-Cook / Usui
+PSE_RTOL = 1e-12       # double exp of the PSE kernel, summed in the same order
 
 
 @pytest.fixture(scope="module")
@@ -105,7 +106,7 @@ def _run_unfused_from(sim, o, stage):
     elif stage == "artificial_stress":
         o.call("o_artificial_stress", "cp")
         S.corrector_artificial_stress(p)
-    elif stage == "interactions":
+    elif stage in ("interactions", "heat_pse"):
         # cell arrays of the oracle's last sort
         import ctypes as C
         nc = len(o.st["cell_start"])
@@ -114,8 +115,12 @@ def _run_unfused_from(sim, o, stage):
         ctx.call("sphfem_malloc", C.byref(b), 4 * nc)
         ctx.htod(a.value, o.st["cell_start"])
         ctx.htod(b.value, o.st["cell_end"])
-        o.call("o_interactions_monaghan", "cps")
-        S.interactions_monaghan(p, a.value, b.value, nc)
+        if stage == "heat_pse":
+            o.call("o_interactions_heat_pse", "cps")
+            S.interactions_heat_pse(p, a.value, b.value, nc)
+        else:
+            o.call("o_interactions_monaghan", "cps")
+            S.interactions_monaghan(p, a.value, b.value, nc)
         ctx.synchronize()
         ctx.call("sphfem_free", a)
         ctx.call("sphfem_free", b)
@@ -193,6 +198,19 @@ def test_stage_bitexact(world, stage):
             assert_close(f, tg[f], o.t[f], ATOMIC_RTOL)
 
 
+def test_stage_heat_pse(world):
+    """PSE conduction (interactions_gpu.cu:84-140): same neighbour order as the oracle, the only
+    difference is the device's double exp (ulp-level)."""
+    case, o0, sim = world
+    o = o0.clone()
+    rng = np.random.default_rng(3)
+    o.P["T"][:] += rng.normal(0.0, 20.0, case.N)         # a rough field so that every pair contributes
+    _run_unfused_from(sim, o, "heat_pse")
+    live_i = live(o)
+    assert np.abs(o.P["T_t"][live_i]).max() > 0
+    assert_close("T_t", sim.p.download("T_t"), o.P["T_t"], PSE_RTOL, mask=live_i)
+
+
 def test_stage_artificial_stress(world):
     case, o0, sim = world
     o = o0.clone()
@@ -263,3 +281,24 @@ def test_wear_stage(world):
         assert_close(f, g[f], o.t[f], POW_RTOL)
     assert abs(g["scalars"]["flank_retreat"] - o.tool_scalars["flank_retreat"]) <= 1e-12 * max(
         1e-12, abs(o.tool_scalars["flank_contact"]))
+
+
+def test_heat_pse_tool_particles_and_blanked():
+    """Tool particles take the tool's diffusivity, blanked particles keep their rate; lattice whose
+    grid cells are 2.5 h wide (the window then covers the Gaussian's support)."""
+    from test_oracle_pins import _pse_lattice
+    o, hp, inner = _pse_lattice(n=40)
+    N = o.case.N
+    sim = S.Simulation(o.case)
+    rng = np.random.default_rng(8)
+    x = o.P["pos"] - o.P["pos"].mean(axis=0)
+    o.P["T"][:] = 300.0 + (x[:, 0] ** 2 + x[:, 1] ** 2) / hp ** 2 + rng.normal(0, 1.0, N)
+    o.P["tool_particle"][::3] = 1.0
+    o.P["blanked"][1::7] = 1.0
+    o.P["T_t"][:] = -7.0
+    _run_unfused_from(sim, o, "heat_pse")
+    g = sim.p.download("T_t")
+    blank = o.P["blanked"] == 1.0
+    assert np.all(g[blank] == -7.0)
+    assert_close("T_t", g, o.P["T_t"], PSE_RTOL, mask=~blank)
+    sim.close()

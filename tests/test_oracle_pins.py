@@ -243,6 +243,78 @@ def test_interactions_stress_divergence():
     assert np.allclose(sd[:, 0], a / rho, rtol=0.03)
 
 
+def _pse_lattice(n=44, cell_over_h=2.5):
+    """Lattice with grid cells wider than the particles' h: the 5x5-cell window of the PSE sum then
+    reaches at least 2 * cell_over_h * h, where the Gaussian's second moment is left with
+    (1 + r^2/h^2) exp(-r^2/h^2) < 1e-9 of its mass."""
+    dx = 2.5e-4
+    hp = 1.5 * dx
+    c = cases.make_constants("Ti6Al4V", dx, 0.35)
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    pos = np.stack([ii.ravel() * dx, jj.ravel() * dx], axis=1) + 0.005
+    N = len(pos)
+    case = cases.Case(name="pse_lattice", N=N, dx=dx, h=cell_over_h * hp, dt=1e-5, pos=pos, vel=np.zeros((N, 2)),
+                      rho=np.full(N, c.phys["rho0"]), T=np.full(N, 300.0), hh=np.full(N, hp),
+                      fixed=np.zeros(N), const=c, grid_bbmin=pos.min(axis=0) - 1e-8,
+                      grid_bbmax=pos.max(axis=0) + 1e-8, tool=None, tool_vel=np.zeros(2), wear=False)
+    o = O.Oracle(case)
+    o.call("o_compute_hashes", "cp")
+    o.call("o_sort_and_cells", "cps")
+    inner = _interior(o, margin=int(np.ceil(3 * cell_over_h * 1.5)) + 1)
+    assert inner.sum() >= 36
+    return o, hp, inner
+
+
+def test_pse_laplacian_closed_forms():
+    """interactions_gpu.cu:84-140 with kernels.cuh:67-84: sum_j (T_j - T_i) 4/(pi h^4) exp(-r^2/h^2) m/rho_j.
+    On a lattice with m/rho = dx^2 the sum is the lattice quadrature of the PSE integral, which returns
+    the Laplacian of any quadratic exactly (second moment of the kernel = 4); the Poisson-summation
+    error of a Gaussian of h = 1.5 dx is exp(-pi^2 h^2/dx^2) ~ 1e-10.  A constant field gives 0 exactly."""
+    o, hp, inner = _pse_lattice()
+    alpha = o.case.const.trml_wp["alpha"]
+    x = o.P["pos"] - o.P["pos"].mean(axis=0)
+    a = 1.0 / hp ** 2                                     # T varies by O(1) K over h
+    o.P["T"][:] = 500.0
+    o.call("o_interactions_heat_pse", "cps")
+    assert np.all(o.P["T_t"] == 0.0)
+    for field, lap in ((a * (x[:, 0] ** 2 + x[:, 1] ** 2), 4 * a), (a * x[:, 0] ** 2, 2 * a),
+                       (a * x[:, 1] ** 2, 2 * a), (a * x[:, 0] * x[:, 1], 0.0),
+                       (a * hp * (3 * x[:, 0] - x[:, 1]), 0.0)):
+        o.P["T"][:] = 300.0 + field
+        o.call("o_interactions_heat_pse", "cps")
+        got = o.P["T_t"][inner] / alpha
+        assert np.abs(got - lap).max() <= 1e-6 * 4 * a, (lap / a, np.abs(got - lap).max() / a)
+
+
+def test_pse_tool_alpha_blanked_and_off_switch():
+    """Tool particles use the tool's diffusivity, blanked particles keep their rate, alpha_wp = 0 skips
+    the call (interactions_gpu.cu:436, 486)."""
+    o, hp, inner = _pse_lattice(n=32, cell_over_h=1.0)
+    x = o.P["pos"] - o.P["pos"].mean(axis=0)
+    o.P["T"][:] = 300.0 + (x[:, 0] ** 2 + x[:, 1] ** 2) / hp ** 2
+    o.call("o_interactions_heat_pse", "cps")
+    base = o.P["T_t"].copy()
+    assert np.all(base[inner] > 0)
+    tool = np.zeros(o.case.N, bool)
+    tool[::3] = True
+    blank = np.zeros(o.case.N, bool)
+    blank[1::7] = True
+    o.P["tool_particle"][:] = tool
+    o.P["blanked"][:] = blank
+    o.P["T_t"][:] = -7.0
+    o.call("o_interactions_heat_pse", "cps")
+    aw, at = o.case.const.trml_wp["alpha"], o.case.const.trml_tool["alpha"]
+    assert np.all(o.P["T_t"][blank] == -7.0)
+    live = ~blank
+    want = np.where(tool, base / aw * at, base)
+    # blanked neighbours still contribute (the reference skips only blanked i)
+    assert np.allclose(o.P["T_t"][live], want[live], rtol=1e-14, atol=0.0)
+    o.c.trml_wp.alpha = 0.0
+    o.P["T_t"][:] = -7.0
+    o.call("o_interactions_heat_pse", "cps")
+    assert np.all(o.P["T_t"] == -7.0)
+
+
 def test_eos_and_artificial_stress_principal_frame():
     c = TI
     o = _lattice_oracle(nx=6, ny=6)
