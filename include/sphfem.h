@@ -28,6 +28,9 @@ extern "C" {
 #define SPHFEM_FP64 0         /* plain FP64 form (reference arithmetic) */
 #define SPHFEM_FP32C 1        /* FP32 pair sums with compensated accumulation */
 
+#define SPHFEM_GRID_GREEN 0    /* grid_gpu_green: gather into a double buffer, pointer swap */
+#define SPHFEM_GRID_ROTHLIN 1  /* grid_gpu_rothlin: in-place gathers, dense cell offsets */
+
 typedef struct { double x, y; } sphfem_double2;
 typedef struct { double x, y, z, w; } sphfem_double4;
 typedef struct { int x, y, z, w; } sphfem_int4;
@@ -126,11 +129,23 @@ int sphfem_particles_free(sphfem_ctx *ctx, sphfem_particles *p);
 /* ---- cell list, grid_gpu_green (grid.h:23-64, grid_gpu_green.h:84-85) */
 int sphfem_grid_green_create(sphfem_ctx *ctx, int num_part, sphfem_double2 bbmin, sphfem_double2 bbmax,
                              double h, sphfem_grid **out);
+/* grid_gpu_rothlin.h:79-80 (locked-geometry constructor, grid.cu:130-149 geometry).  Same cell
+ * geometry and hashes as the Green grid; sort and get_cells follow grid_gpu_rothlin.cu (below). */
+int sphfem_grid_rothlin_create(sphfem_ctx *ctx, int num_part, sphfem_double2 bbmin, sphfem_double2 bbmax,
+                               double h, sphfem_grid **out);
 int sphfem_grid_destroy(sphfem_ctx *ctx, sphfem_grid *g);
 int sphfem_grid_geometry(const sphfem_grid *g, sphfem_geom_constants *geom, int *num_cell);
+/* SPHFEM_GRID_GREEN or SPHFEM_GRID_ROTHLIN */
+int sphfem_grid_kind(const sphfem_grid *g, int *kind);
 /* grid.cu:80-86 */
 int sphfem_grid_assign_hashes(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles *p);
-/* grid_gpu_green.cu:184-213: stable radix sort, cell boundaries, gather of 20 arrays, pointer swap */
+/* Green (grid_gpu_green.cu:184-213): stable radix sort, cell boundaries, gather of 20 arrays,
+ * pointer swap (p's array pointers change).
+ * Rothlin (grid_gpu_rothlin.cu:129-249): stable radix sort, in-place gathers (p's pointers are
+ * kept) of idx (which then carries particle labels), pos, vel, fc, h, rho, S, eps_pl, eps_pl_dot,
+ * fixed, blanked, tool_particle, T, pos_t, vel_t, rho_t, S_t, T_t and on_seg; ft and n stay in
+ * place as in the reference.  on_seg is gathered, where the reference copies the gathered idx
+ * into it (:244-245). */
 int sphfem_grid_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles *p);
 /* The step re-sorts incrementally when at most max_changed particles changed cells since the
  * previous step (the result equals the full stable sort); otherwise, and on the first step, it
@@ -140,8 +155,12 @@ int sphfem_grid_set_resort_limit(sphfem_ctx *ctx, sphfem_grid *g, int max_change
 /* the last step's re-sort: mode 1 = full radix sort, 2 = incremental; changed = particles that
  * changed cells (blocks until the stream is idle) */
 int sphfem_grid_resort_stats(sphfem_ctx *ctx, int *mode, int *changed);
-/* grid_gpu_green.cu:215-218: device-to-device copy into caller arrays [num_cell] */
-int sphfem_grid_get_cells(sphfem_ctx *ctx, sphfem_grid *g, int *cell_start, int *cell_end);
+/* grid.h:57 get_cells(particles, cell_start, cell_end) into caller arrays [num_cell].
+ * Green (grid_gpu_green.cu:215-218): copy of the tables of the last sort (-1 = empty cell; p is
+ * not read and may be NULL).  Rothlin (grid_gpu_rothlin.cu:19-126): dense offsets from p's sorted
+ * hashes, cell_start[c] = #{hash < c}, cell_end[c] = #{hash < c + 1} (p required). */
+int sphfem_grid_get_cells(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles *p, int *cell_start,
+                          int *cell_end);
 
 /* ---- pair interactions (interactions_gpu.cu:381-432, 466-474) */
 int sphfem_interactions_setup_geometry_constants(sphfem_ctx *ctx, const sphfem_grid *g);
@@ -209,7 +228,9 @@ int sphfem_leap_frog_correct(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_parti
 typedef struct { int pair_block, pair_chunk, update_block; } sphfem_tuning;
 int sphfem_leap_frog_set_tuning(sphfem_ctx *ctx, sphfem_leap_frog *lf, const sphfem_tuning *tuning);
 int sphfem_leap_frog_get_tuning(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_tuning *tuning);
-/* one full step (fused kernels); tool may be NULL for a tool-free workload */
+/* leap_frog.cu:162-248: one full step (fused kernels); tool may be NULL for a tool-free workload.
+ * With a Rothlin grid the step sorts as grid_gpu_rothlin::sort does (ft and n not reordered, idx
+ * carrying labels) and then runs the same fused kernels over the sorted arrays. */
 int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,
                           sphfem_tool *t);
 

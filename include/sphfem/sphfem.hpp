@@ -6,7 +6,8 @@
  * interactions_gpu.h and output.h compiles against this header instead:
  *
  *   particle_gpu        particle_gpu.h:23-83 (same field names; the six constructors)
- *   grid_base / grid_gpu_green   grid.h:23-64, grid_gpu_green.h:84-85
+ *   grid_base / grid_gpu_green / grid_gpu_rothlin
+ *                       grid.h:23-64, grid_gpu_green.h:84-85, grid_gpu_rothlin.h:79-80
  *   leap_frog           leap_frog.h:31-50 (step(particles, grid[, tool_FEM]))
  *   tool / tool_FEM     tool.h:48-58, tool_FEM.h:34-83 (per-step methods)
  *   free functions      actions_gpu.h:33-72, interactions_gpu.h:27-43, output.h:17
@@ -254,9 +255,28 @@ class grid_gpu_green : public grid_base {
         sphfem_shim::check(sphfem_grid_set_resort_limit(sphfem_shim::ctx(), m_handle, max_changed),
                            "grid_gpu_green::set_resort_limit");
     }
-    void get_cells(particle_gpu *, int *cell_start, int *cell_end) override {
-        sphfem_shim::check(sphfem_grid_get_cells(sphfem_shim::ctx(), m_handle, cell_start, cell_end),
+    void get_cells(particle_gpu *particles, int *cell_start, int *cell_end) override {
+        sphfem_shim::check(sphfem_grid_get_cells(sphfem_shim::ctx(), m_handle, particles, cell_start, cell_end),
                            "grid_gpu_green::get_cells");
+    }
+};
+
+// grid_gpu_rothlin.h:79-80: in-place gathers (idx carries labels; ft and n are not reordered) and
+// dense cell offsets computed from the particles' sorted hashes
+class grid_gpu_rothlin : public grid_base {
+  public:
+    grid_gpu_rothlin(int num_part, float2_t bbmin, float2_t bbmax, double h) {
+        sphfem_grid *g = nullptr;
+        sphfem_shim::check(sphfem_grid_rothlin_create(sphfem_shim::ctx(), num_part, bbmin, bbmax, h, &g),
+                           "grid_gpu_rothlin");
+        adopt(g);
+    }
+    void sort(particle_gpu *particles, tool *) const override {
+        sphfem_shim::check(sphfem_grid_sort(sphfem_shim::ctx(), m_handle, particles), "grid_gpu_rothlin::sort");
+    }
+    void get_cells(particle_gpu *particles, int *cell_start, int *cell_end) override {
+        sphfem_shim::check(sphfem_grid_get_cells(sphfem_shim::ctx(), m_handle, particles, cell_start, cell_end),
+                           "grid_gpu_rothlin::get_cells");
     }
 };
 

@@ -85,6 +85,7 @@ def lib():
         pt = C.POINTER(OTool)
         sig = {
             "o_compute_hashes": (pc, pp), "o_sort_and_cells": (pc, pp, ps), "o_init": (pp, ps),
+            "o_sort_rothlin": (pc, pp, ps), "o_run_rothlin": (pc, pp, ps, pt, C.c_int),
             "o_predict": (pp, ps, _dbl), "o_correct": (pp, ps, _dbl), "o_material_eos": (pc, pp),
             "o_artificial_stress": (pc, pp), "o_interactions_monaghan": (pc, pp, ps),
             "o_interactions_heat_pse": (pc, pp, ps),
@@ -147,7 +148,10 @@ def _ptr(a):
 class Oracle:
     """The reference step on the host, over numpy arrays in the reference layout."""
 
-    def __init__(self, case: cases.Case, rr2_min=None, wear_pro=None, threads=None):
+    def __init__(self, case: cases.Case, rr2_min=None, wear_pro=None, threads=None, grid="green"):
+        if grid not in ("green", "rothlin"):
+            raise ValueError(grid)
+        self.grid = grid
         self.case = case
         self.c = make_consts(case, rr2_min, wear_pro)
         if threads:
@@ -186,6 +190,7 @@ class Oracle:
         """Independent copy of the current state (arrays, tool kinematics and constants)."""
         import copy
         o = Oracle.__new__(Oracle)
+        o.grid = self.grid
         o.case = self.case
         o.c = OConsts()
         C.pointer(o.c)[0] = self.c
@@ -213,7 +218,8 @@ class Oracle:
 
     def step(self, n=1):
         tc = C.byref(self.Tc) if self.Tc is not None else None
-        lib().o_run(C.byref(self.c), C.byref(self.Pc), C.byref(self.Sc), tc, int(n))
+        run = lib().o_run_rothlin if self.grid == "rothlin" else lib().o_run
+        run(C.byref(self.c), C.byref(self.Pc), C.byref(self.Sc), tc, int(n))
 
     def report_force(self):
         fx, fy = _dbl(), _dbl()

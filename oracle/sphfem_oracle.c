@@ -183,6 +183,55 @@ void o_sort_and_cells(const o_consts *c, o_particles *P, o_stepper *st) {
     GATHER(T_t, double); GATHER(on_seg, int);
 }
 
+/* grid_gpu_rothlin.cu:129-249 and :19-126.  The sort: stable sort of the keys with the
+ * sequence as values, then in-place gathers of idx (so idx keeps the particle label rather than
+ * the permutation), pos, vel, fc, h, rho, S, eps_pl, eps_pl_dot, fixed, blanked, tool_particle, T,
+ * pos_t, vel_t, rho_t, S_t, T_t and on_seg; ft and n are not gathered (no gather for them in the
+ * reference).  The reference's on_seg line copies the gathered idx buffer instead of the gathered
+ * on_seg (:244-245); on_seg is gathered here.  The cells: unique_by_key_copy + differences +
+ * scatter + inclusive scan with the last cell fixed up (:19-78) give cell offsets equal to the
+ * number of keys below each cell, so cell_start[c] = #{keys < c}, cell_end[c] = #{keys < c+1}:
+ * dense, with start == end for empty cells. */
+void o_sort_rothlin(const o_consts *c, o_particles *P, o_stepper *st) {
+    const int N = P->N;
+    const int num_cell = c->geom.nx * c->geom.ny;
+    int *count = (int *)calloc((size_t)num_cell + 1, sizeof(int));
+    for (int i = 0; i < N; i++) {
+        int h = P->hash[i];
+        if (h < 0 || h >= num_cell) abort();
+        count[h + 1]++;
+    }
+    for (int k = 0; k < num_cell; k++) count[k + 1] += count[k];
+    /* count[k] = #{keys < k}: the dense offsets */
+    for (int k = 0; k < num_cell; k++) {
+        st->cell_start[k] = count[k];
+        st->cell_end[k] = count[k + 1];
+    }
+    int *keys = (int *)malloc(sizeof(int) * (size_t)N);
+    int *perm = (int *)malloc(sizeof(int) * (size_t)N);
+    int *label = (int *)malloc(sizeof(int) * (size_t)N);
+    for (int i = 0; i < N; i++) {
+        int h = P->hash[i];
+        int dst = count[h]++;
+        keys[dst] = h;
+        perm[dst] = i;
+    }
+    memcpy(P->hash, keys, sizeof(int) * (size_t)N);
+    for (int i = 0; i < N; i++) label[i] = P->idx[perm[i]];
+    /* GATHER reads the permutation from P->idx */
+    memcpy(P->idx, perm, sizeof(int) * (size_t)N);
+    GATHER(pos, o_d2); GATHER(vel, o_d2); GATHER(fc, o_d2); GATHER(h, double);
+    GATHER(rho, double); GATHER(S, o_d4); GATHER(eps_pl, double); GATHER(eps_pl_dot, double);
+    GATHER(fixed, double); GATHER(blanked, double); GATHER(tool_particle, double); GATHER(T, double);
+    GATHER(pos_t, o_d2); GATHER(vel_t, o_d2); GATHER(rho_t, double); GATHER(S_t, o_d4);
+    GATHER(T_t, double); GATHER(on_seg, int);
+    memcpy(P->idx, label, sizeof(int) * (size_t)N);
+    free(label);
+    free(perm);
+    free(keys);
+    free(count);
+}
+
 /* leap_frog.cu:25-37 */
 void o_init(o_particles *P, o_stepper *st) {
 #pragma omp parallel for schedule(static)
@@ -930,9 +979,10 @@ void o_report_force(const o_particles *P, double *fx, double *fy) {
 }
 
 /* leap_frog.cu:162-248 */
-void o_step(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t) {
+static void step_impl(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t, int rothlin) {
     o_compute_hashes(c, P);
-    o_sort_and_cells(c, P, st);
+    if (rothlin) o_sort_rothlin(c, P, st);
+    else o_sort_and_cells(c, P, st);
     o_init(P, st);
     if (t) { o_fem_reset_server(t); o_segments_wear_rate_reset(t); }
     o_predict(P, st, c->dt);
@@ -964,6 +1014,13 @@ void o_step(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t) {
     o_invalidate(P);
 }
 
+void o_step(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t) { step_impl(c, P, st, t, 0); }
+
 void o_run(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t, int nsteps) {
-    for (int s = 0; s < nsteps; s++) o_step(c, P, st, t);
+    for (int s = 0; s < nsteps; s++) step_impl(c, P, st, t, 0);
+}
+
+/* leap_frog::step with a grid_gpu_rothlin (the same step, the Rothlin sort and cells) */
+void o_run_rothlin(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t, int nsteps) {
+    for (int s = 0; s < nsteps; s++) step_impl(c, P, st, t, 1);
 }

@@ -78,6 +78,8 @@ double o_solve_secant(const o_joco *jc, double init, double tol, double norm_Str
 /* grid.cu:25-42 / grid_gpu_green.cu:13-115, 184-209 */
 void o_compute_hashes(const o_consts *c, o_particles *P);
 void o_sort_and_cells(const o_consts *c, o_particles *P, o_stepper *st);
+/* grid_gpu_rothlin.cu:19-126, 129-249 */
+void o_sort_rothlin(const o_consts *c, o_particles *P, o_stepper *st);
 
 /* leap_frog.cu:25-96 */
 void o_init(o_particles *P, o_stepper *st);
@@ -117,6 +119,7 @@ void o_report_force(const o_particles *P, double *fx, double *fy);
 /* leap_frog.cu:162-248: one full step in the reference order */
 void o_step(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t);
 void o_run(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t, int nsteps);
+void o_run_rothlin(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t, int nsteps);
 
 int o_num_threads(void);
 void o_set_num_threads(int n);

@@ -89,6 +89,8 @@ _SIGS = {
     "sphfem_particles_alloc": (_ctx, C.c_uint, _P(Particles)),
     "sphfem_particles_free": (_ctx, _P(Particles)),
     "sphfem_grid_green_create": (_ctx, C.c_int, Double2, Double2, _dbl, _P(_vp)),
+    "sphfem_grid_rothlin_create": (_ctx, C.c_int, Double2, Double2, _dbl, _P(_vp)),
+    "sphfem_grid_kind": (_vp, _P(C.c_int)),
     "sphfem_grid_destroy": (_ctx, _vp),
     "sphfem_grid_set_resort_limit": (_ctx, _vp, C.c_int),
     "sphfem_leap_frog_graph_create": (_ctx, _vp, _P(Particles), _vp, _vp, C.c_int, _P(_vp)),
@@ -100,7 +102,7 @@ _SIGS = {
     "sphfem_grid_geometry": (_vp, _P(GeomConstants), _P(C.c_int)),
     "sphfem_grid_assign_hashes": (_ctx, _vp, _P(Particles)),
     "sphfem_grid_sort": (_ctx, _vp, _P(Particles)),
-    "sphfem_grid_get_cells": (_ctx, _vp, _vp, _vp),
+    "sphfem_grid_get_cells": (_ctx, _vp, _P(Particles), _vp, _vp),
     "sphfem_interactions_setup_geometry_constants": (_ctx, _vp),
     "sphfem_interactions_monaghan": (_ctx, _P(Particles), _vp, _vp, C.c_int),
     "sphfem_interactions_heat_pse": (_ctx, _P(Particles), _vp, _vp, C.c_int),

@@ -313,9 +313,9 @@ int sphfem_particles_free(sphfem_ctx *ctx, sphfem_particles *p) {
     });
 }
 
-// ---- grid (grid.cu:130-149, grid_gpu_green.cu:220-268)
-int sphfem_grid_green_create(sphfem_ctx *ctx, int num_part, sphfem_double2 bbmin, sphfem_double2 bbmax, double h,
-                             sphfem_grid **out) {
+// ---- grid (grid.cu:130-149, grid_gpu_green.cu:220-268, grid_gpu_rothlin.cu:251-285)
+static int grid_create(sphfem_ctx *ctx, int kind, int num_part, sphfem_double2 bbmin, sphfem_double2 bbmax,
+                       double h, sphfem_grid **out) {
     if (!out || num_part <= 0 || !(h > 0.)) return SPHFEM_ERR_ARG;
     *out = nullptr;
     return guarded(ctx, [&] {
@@ -359,10 +359,33 @@ int sphfem_grid_green_create(sphfem_ctx *ctx, int num_part, sphfem_double2 bbmin
         g->cdest = dalloc<int4>(SPHFEM_RESORT_CAP);
         alloc_particles(ctx->stream, (unsigned)num_part, g->buf);
         g->first_sort = true;
+        g->kind = kind;
+        if (kind == SPHFEM_GRID_ROTHLIN) {
+            g->perm = dalloc<int>(num_part);
+            g->iota = dalloc<int>(num_part);
+            launch_iota(ctx, g->iota, num_part);
+            g->first_sort = false;   // every Rothlin sort is a full sort
+        }
         ctx->c.geom = g->geom;
         *out = g;
         return SPHFEM_OK;
     });
+}
+
+int sphfem_grid_green_create(sphfem_ctx *ctx, int num_part, sphfem_double2 bbmin, sphfem_double2 bbmax, double h,
+                             sphfem_grid **out) {
+    return grid_create(ctx, SPHFEM_GRID_GREEN, num_part, bbmin, bbmax, h, out);
+}
+
+int sphfem_grid_rothlin_create(sphfem_ctx *ctx, int num_part, sphfem_double2 bbmin, sphfem_double2 bbmax,
+                               double h, sphfem_grid **out) {
+    return grid_create(ctx, SPHFEM_GRID_ROTHLIN, num_part, bbmin, bbmax, h, out);
+}
+
+int sphfem_grid_kind(const sphfem_grid *g, int *kind) {
+    if (!g || !kind) return SPHFEM_ERR_ARG;
+    *kind = g->kind;
+    return SPHFEM_OK;
 }
 
 int sphfem_grid_destroy(sphfem_ctx *ctx, sphfem_grid *g) {
@@ -372,6 +395,7 @@ int sphfem_grid_destroy(sphfem_ctx *ctx, sphfem_grid *g) {
         dfree(g->cell_start); dfree(g->cell_end); dfree(g->keys_alt); dfree(g->vals_alt);
         dfree(g->hist); dfree(g->digit_tot);
         dfree(g->cmask); dfree(g->wprefix); dfree(g->bprefix); dfree(g->clist); dfree(g->csorted); dfree(g->cdest);
+        dfree(g->perm); dfree(g->iota);
         free_particles(g->buf);
         delete g;
         return SPHFEM_OK;
@@ -390,7 +414,8 @@ int sphfem_grid_assign_hashes(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles 
     return guarded(ctx, [&] {
         ctx->c.geom = g->geom;
         launch_hash(ctx, g, *p, false, nullptr);
-        g->first_sort = true;   // p.hash no longer holds the keys of the cell table
+        // p.hash no longer holds the keys of the (Green) cell table
+        if (g->kind == SPHFEM_GRID_GREEN) g->first_sort = true;
         return SPHFEM_OK;
     });
 }
@@ -398,6 +423,10 @@ int sphfem_grid_assign_hashes(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles 
 int sphfem_grid_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles *p) {
     if (!g || !p || (int)p->N != g->num_part) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {
+        if (g->kind == SPHFEM_GRID_ROTHLIN) {
+            launch_rothlin_sort(ctx, g, *p);
+            return SPHFEM_OK;
+        }
         launch_radix_sort(ctx, g, *p, false);
         launch_fill_int(ctx, g->cell_start, g->num_cell, -1);
         launch_reorder_full(ctx, g, *p);
@@ -425,9 +454,15 @@ int sphfem_grid_resort_stats(sphfem_ctx *ctx, int *mode, int *changed) {
     });
 }
 
-int sphfem_grid_get_cells(sphfem_ctx *ctx, sphfem_grid *g, int *cell_start, int *cell_end) {
+int sphfem_grid_get_cells(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles *p, int *cell_start,
+                          int *cell_end) {
     if (!g || !cell_start || !cell_end) return SPHFEM_ERR_ARG;
+    if (g->kind == SPHFEM_GRID_ROTHLIN && (!p || (int)p->N != g->num_part)) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {
+        if (g->kind == SPHFEM_GRID_ROTHLIN) {
+            launch_cells_dense(ctx, p->hash, (int)p->N, g->num_cell, cell_start, cell_end);
+            return SPHFEM_OK;
+        }
         SPHFEM_HIP_CHECK(hipMemcpyAsync(cell_start, g->cell_start, sizeof(int) * (size_t)g->num_cell,
                                         hipMemcpyDeviceToDevice, ctx->stream));
         SPHFEM_HIP_CHECK(hipMemcpyAsync(cell_end, g->cell_end, sizeof(int) * (size_t)g->num_cell,
@@ -934,15 +969,24 @@ static void step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p
     ctx->c.geom = g->geom;
     if (t) launch_tool_begin(ctx, t);
     else launch_fill_int(ctx, ctx->d_flags + 1, 2, 0);
-    // the cell table is cleared by the hash pass (cells of the previous keys) except when the
-    // previous keys are unknown
-    if (g->first_sort) launch_fill_int(ctx, g->cell_start, g->num_cell, -1);
-    launch_resort(ctx, g, *p);
-    g->first_sort = false;
-    launch_reorder_predict(ctx, g, *p, lf);
+    // the source of each sorted position: the permutation of the sort (Green), or the identity
+    // after the Rothlin sort has gathered in place
+    sphfem_particles A = *p;
+    if (g->kind == SPHFEM_GRID_ROTHLIN) {
+        launch_hash(ctx, g, *p, false, nullptr);
+        launch_rothlin_sort(ctx, g, *p);    // sorted keys, in-place gathers, dense cell table
+        A.idx = g->iota;
+    } else {
+        // the cell table is cleared by the hash pass (cells of the previous keys) except when the
+        // previous keys are unknown
+        if (g->first_sort) launch_fill_int(ctx, g->cell_start, g->num_cell, -1);
+        launch_resort(ctx, g, *p);
+        g->first_sort = false;
+    }
+    launch_reorder_predict(ctx, g, A, lf);
     sphfem_particles B = g->buf;
     B.hash = p->hash;
-    B.idx = p->idx;
+    B.idx = A.idx;
     B.N = p->N;
     launch_fused_update(ctx, B, *p, lf, g->cell_start, g->cell_end, t);
     if (lf->precision == SPHFEM_FP32C) {

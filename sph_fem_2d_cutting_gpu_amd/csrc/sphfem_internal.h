@@ -88,6 +88,10 @@ struct sphfem_grid {
     unsigned long long *csorted = nullptr; // [SPHFEM_RESORT_CAP] changed (key << 32 | position), sorted
     int4 *cdest = nullptr;                 // [SPHFEM_RESORT_CAP] (destination, key, position)
     int resort_limit = SPHFEM_RESORT_CAP;  // max changed particles for the incremental path (0: off)
+    // grid_gpu_rothlin: in-place gathers and dense cell offsets
+    int kind = SPHFEM_GRID_GREEN;
+    int *perm = nullptr;                   // [N] permutation of the last Rothlin sort
+    int *iota = nullptr;                   // [N] 0..N-1: the identity gather of the step after an in-place sort
 };
 
 struct sphfem_tool {
@@ -172,7 +176,11 @@ __device__ __forceinline__ int xcd_logical_block() {
 
 // sort / cells (sphfem_sort.hip)
 void launch_hash(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, bool clear_old, int *hist0);
-void launch_radix_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool hash_done_with_hist0);
+void launch_radix_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool hash_done_with_hist0,
+                       int *vals_out = nullptr);
+void launch_rothlin_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p);
+void launch_cells_dense(sphfem_ctx *ctx, const int *sorted_keys, int N, int num_cell, int *cell_start, int *cell_end);
+void launch_iota(sphfem_ctx *ctx, int *ptr, int n);
 void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p);
 void launch_fill_int(sphfem_ctx *ctx, int *ptr, long n, int value);
 void launch_reorder_full(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p);

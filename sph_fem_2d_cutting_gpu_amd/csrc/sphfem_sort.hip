@@ -208,17 +208,18 @@ void launch_hash(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, boo
                                             clear_old ? 1 : 0, N, g->num_cell, nullptr, nb, ctx->d_flags);
 }
 
-// Radix sort of (p.hash, index) leaving sorted keys in p.hash and the permutation in p.idx.
+// Radix sort of (p.hash, index) leaving sorted keys in p.hash and the permutation in p.idx
+// (in vals_out when given).
 // If hash_with_hist0, the hash pass (which computes the keys) is run here with the digit-0
 // histogram fused and writes the keys into the buffer the pass parity requires.
-void launch_radix_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool hash_with_hist0) {
+void launch_radix_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool hash_with_hist0, int *vals_out) {
     const int N = (int)p.N;
     const int nb = nblk(N, SORT_TILE);
     const int passes = (g->key_bits + 7) / 8;
     int *kin, *vin = nullptr;
     // choose buffers so that the last pass writes p.hash / p.idx
     int *kbuf[2] = {p.hash, g->keys_alt};
-    int *vbuf[2] = {p.idx, g->vals_alt};
+    int *vbuf[2] = {vals_out ? vals_out : p.idx, g->vals_alt};
     int src = (passes % 2 == 1) ? 1 : 0;   // buffer holding the unsorted keys
     if (hash_with_hist0) {
         KTimer t(ctx, "hash");
@@ -561,6 +562,124 @@ void launch_reorder_full(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles
     KTimer t(ctx, "reorder");
     k_reorder_full<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(p, g->buf, g->cell_start, g->cell_end,
                                                             g->num_cell, ctx->d_flags);
+}
+
+// ---------------------------------------------------------------------------------------------
+// grid_gpu_rothlin.cu:129-249: the stable sort's permutation gathers idx (which therefore keeps
+// particle labels), pos, vel, fc, h, rho, S, eps_pl, eps_pl_dot, fixed, blanked, tool_particle, T,
+// pos_t, vel_t, rho_t, S_t, T_t and on_seg in place; ft and n are not gathered (as in the
+// reference).  The reference's on_seg copy takes the gathered idx buffer (:244-245), which would
+// leave particle labels where segment numbers are expected; on_seg is gathered here.  The gather
+// writes the buffer and a second pass copies it back, so the particle arrays keep their addresses.
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_rothlin_gather(const sphfem_particles p, sphfem_particles b,
+                                                        const int *__restrict__ perm, int *__restrict__ flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = (int)p.N;
+    if (i >= N) return;
+    const int s = perm[i];
+    if ((unsigned)s >= (unsigned)N) {
+        atomicOr(&flags[0], 4);
+        return;
+    }
+    b.idx[i] = p.idx[s];
+    b.pos[i] = p.pos[s];
+    b.vel[i] = p.vel[s];
+    b.fc[i] = p.fc[s];
+    b.h[i] = p.h[s];
+    b.rho[i] = p.rho[s];
+    b.S[i] = p.S[s];
+    b.eps_pl[i] = p.eps_pl[s];
+    b.eps_pl_dot[i] = p.eps_pl_dot[s];
+    b.fixed[i] = p.fixed[s];
+    b.blanked[i] = p.blanked[s];
+    b.tool_particle[i] = p.tool_particle[s];
+    b.T[i] = p.T[s];
+    b.pos_t[i] = p.pos_t[s];
+    b.vel_t[i] = p.vel_t[s];
+    b.rho_t[i] = p.rho_t[s];
+    b.S_t[i] = p.S_t[s];
+    b.T_t[i] = p.T_t[s];
+    b.on_seg[i] = p.on_seg[s];
+}
+
+__global__ void __launch_bounds__(256) k_rothlin_copy_back(sphfem_particles p, const sphfem_particles b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int)p.N) return;
+    p.idx[i] = b.idx[i];
+    p.pos[i] = b.pos[i];
+    p.vel[i] = b.vel[i];
+    p.fc[i] = b.fc[i];
+    p.h[i] = b.h[i];
+    p.rho[i] = b.rho[i];
+    p.S[i] = b.S[i];
+    p.eps_pl[i] = b.eps_pl[i];
+    p.eps_pl_dot[i] = b.eps_pl_dot[i];
+    p.fixed[i] = b.fixed[i];
+    p.blanked[i] = b.blanked[i];
+    p.tool_particle[i] = b.tool_particle[i];
+    p.T[i] = b.T[i];
+    p.pos_t[i] = b.pos_t[i];
+    p.vel_t[i] = b.vel_t[i];
+    p.rho_t[i] = b.rho_t[i];
+    p.S_t[i] = b.S_t[i];
+    p.T_t[i] = b.T_t[i];
+    p.on_seg[i] = b.on_seg[i];
+}
+
+// grid_gpu_rothlin.cu:19-126: unique_by_key + |differences| + scatter + inclusive scan with the
+// last cell fixed up yields offsets[c] = number of sorted keys below c, so every cell gets
+// cell_start[c] = lower_bound(c) and cell_end[c] = lower_bound(c + 1) (start == end when empty).
+// One thread per cell, two binary searches over the sorted keys (L2-resident for these sizes).
+__device__ __forceinline__ int lower_bound_keys(const int *__restrict__ keys, int N, int c) {
+    int lo = 0, n = N;
+    while (n > 0) {
+        const int half = n >> 1;
+        if (keys[lo + half] < c) {
+            lo += half + 1;
+            n -= half + 1;
+        } else {
+            n = half;
+        }
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_cells_dense(const int *__restrict__ keys, int N, int num_cell,
+                                                     int *__restrict__ cell_start, int *__restrict__ cell_end) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= num_cell) return;
+    cell_start[c] = lower_bound_keys(keys, N, c);
+    cell_end[c] = lower_bound_keys(keys, N, c + 1);
+}
+
+void launch_cells_dense(sphfem_ctx *ctx, const int *sorted_keys, int N, int num_cell, int *cell_start, int *cell_end) {
+    KTimer t(ctx, "cells_dense");
+    k_cells_dense<<<nblk(num_cell, 256), 256, 0, ctx->stream>>>(sorted_keys, N, num_cell, cell_start, cell_end);
+}
+
+__global__ void k_iota(int *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+void launch_iota(sphfem_ctx *ctx, int *ptr, int n) {
+    k_iota<<<nblk(n, 256), 256, 0, ctx->stream>>>(ptr, n);
+}
+
+// p.hash must hold the keys; leaves the sorted keys in p.hash, the gathered arrays in place and
+// the dense cell table in the grid
+void launch_rothlin_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p) {
+    launch_radix_sort(ctx, g, p, false, g->perm);
+    {
+        KTimer t(ctx, "rothlin_gather");
+        k_rothlin_gather<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(p, g->buf, g->perm, ctx->d_flags);
+    }
+    {
+        KTimer t(ctx, "rothlin_copy_back");
+        k_rothlin_copy_back<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(p, g->buf);
+    }
+    launch_cells_dense(ctx, p.hash, (int)p.N, g->num_cell, g->cell_start, g->cell_end);
 }
 
 // Fused: boundaries, gather (state and old derivatives), leap_frog init (leap_frog.cu:25-37),

@@ -122,13 +122,16 @@ class particle_gpu:
             self.ctx.call("sphfem_particles_free", C.byref(self.s))
 
 
-class grid_gpu_green:
-    """grid_gpu_green(num_part, bbmin, bbmax, h): locked Green grid (grid.cu:130-149)."""
+class grid_base:
+    """grid_base (grid.h:23-64): locked cell grid of cell size h over [bbmin, bbmax] (grid.cu:130-149)."""
+
+    _create = None
+    kind = None
 
     def __init__(self, ctx: Context, num_part: int, bbmin, bbmax, h: float):
         self.ctx = ctx
         self.h = C.c_void_p()
-        ctx.call("sphfem_grid_green_create", int(num_part), _lib.Double2(*map(float, bbmin)),
+        ctx.call(self._create, int(num_part), _lib.Double2(*map(float, bbmin)),
                  _lib.Double2(*map(float, bbmax)), float(h), C.byref(self.h))
         g = _lib.GeomConstants()
         nc = C.c_int()
@@ -158,21 +161,43 @@ class grid_gpu_green:
         """Largest number of cell changes per step re-sorted incrementally (0: always the full sort)."""
         self.ctx.call("sphfem_grid_set_resort_limit", self.h, int(max_changed))
 
-    def get_cells(self, cell_start_dptr, cell_end_dptr):
-        self.ctx.call("sphfem_grid_get_cells", self.h, C.c_void_p(cell_start_dptr), C.c_void_p(cell_end_dptr))
+    def get_cells(self, p, cell_start_dptr, cell_end_dptr):
+        """grid.h:57 get_cells(particles, cell_start, cell_end); the Green grid does not read p."""
+        ps = C.byref(p.s) if p is not None else None
+        self.ctx.call("sphfem_grid_get_cells", self.h, ps, C.c_void_p(cell_start_dptr), C.c_void_p(cell_end_dptr))
 
-    def download_cells(self):
+    def download_cells(self, p=None):
         a, b = C.c_void_p(), C.c_void_p()
         self.ctx.call("sphfem_malloc", C.byref(a), 4 * self.num_cell)
         self.ctx.call("sphfem_malloc", C.byref(b), 4 * self.num_cell)
         try:
-            self.get_cells(a.value, b.value)
+            self.get_cells(p, a.value, b.value)
             cs = self.ctx.dtoh(a.value, (self.num_cell,), np.int32)
             ce = self.ctx.dtoh(b.value, (self.num_cell,), np.int32)
         finally:
             self.ctx.call("sphfem_free", a)
             self.ctx.call("sphfem_free", b)
         return cs, ce
+
+
+class grid_gpu_green(grid_base):
+    """grid_gpu_green(num_part, bbmin, bbmax, h) (grid_gpu_green.h:84-85): sort gathers into a double
+    buffer and swaps pointers; cell tables mark empty cells with -1."""
+
+    _create = "sphfem_grid_green_create"
+    kind = "green"
+
+
+class grid_gpu_rothlin(grid_base):
+    """grid_gpu_rothlin(num_part, bbmin, bbmax, h) (grid_gpu_rothlin.h:79-80): sort gathers in place
+    (idx then carries particle labels; ft and n are not reordered); get_cells gives dense offsets
+    computed from the particles' sorted hashes."""
+
+    _create = "sphfem_grid_rothlin_create"
+    kind = "rothlin"
+
+
+GRIDS = {"green": grid_gpu_green, "rothlin": grid_gpu_rothlin}
 
 
 def _p(a, dt):
@@ -292,7 +317,7 @@ class leap_frog:
         ctx.call("sphfem_leap_frog_create", int(num_part), int(num_cell), int(node_num), C.byref(self.h))
         ctx.call("sphfem_leap_frog_set_precision", self.h, int(precision))
 
-    def step(self, p: particle_gpu, g: grid_gpu_green, tool: tool_FEM | None = None):
+    def step(self, p: particle_gpu, g: grid_base, tool: tool_FEM | None = None):
         self.ctx.call("sphfem_leap_frog_step", self.h, C.byref(p.s), g.h, tool.h if tool else None)
 
     def set_tuning(self, pair_block: int = 256, pair_chunk: int = 32, update_block: int = 256):
@@ -304,7 +329,7 @@ class leap_frog:
         self.ctx.call("sphfem_leap_frog_get_tuning", self.h, C.byref(t))
         return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}
 
-    def capture(self, p: particle_gpu, g: grid_gpu_green, tool: tool_FEM | None = None, steps: int = 2):
+    def capture(self, p: particle_gpu, g: grid_base, tool: tool_FEM | None = None, steps: int = 2):
         """Record `steps` (even) steps into a HIP graph; returns a StepGraph.  Needs one eager step first."""
         gh = C.c_void_p()
         self.ctx.call("sphfem_leap_frog_graph_create", self.h, C.byref(p.s), g.h, tool.h if tool else None,
@@ -368,7 +393,7 @@ def wear_adjustment(tool: tool_FEM):
     tool.ctx.call("sphfem_wear_adjustment", tool.h)
 
 
-def interactions_setup_geometry_constants(g: grid_gpu_green):
+def interactions_setup_geometry_constants(g: grid_base):
     g.ctx.call("sphfem_interactions_setup_geometry_constants", g.h)
 
 
@@ -392,14 +417,15 @@ def report_force(p: particle_gpu):
 class Simulation:
     """A workload on the device: constants, particles, locked grid, FEM tool and stepper."""
 
-    def __init__(self, case: cases.Case, precision=_lib.FP64, device=0, rr2_min=None, wear=None, ctx=None):
+    def __init__(self, case: cases.Case, precision=_lib.FP64, device=0, rr2_min=None, wear=None, ctx=None,
+                 grid="green"):
         self.case = case
         self.ctx = ctx or Context(device)
         rr2 = cases.default_rr2_min(case.dx) if rr2_min is None else rr2_min
         self.ctx.setup_constants(case.const, case.dt, rr2, case.wear if wear is None else wear)
         init = cases.initial_particles(case)
         self.p = particle_gpu(self.ctx, case.N, **{k: init[k] for k in ("pos", "vel", "rho", "T", "h", "fixed", "S")})
-        self.grid = grid_gpu_green(self.ctx, case.N, case.grid_bbmin, case.grid_bbmax, case.h)
+        self.grid = GRIDS[grid](self.ctx, case.N, case.grid_bbmin, case.grid_bbmax, case.h)
         self.tool = tool_FEM(self.ctx, case) if case.tool is not None else None
         self.stepper = leap_frog(self.ctx, case.N, self.grid.num_cell,
                                  self.tool.n_nodes if self.tool else 0, precision)

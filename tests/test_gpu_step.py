@@ -41,7 +41,7 @@ class Unfused:
         dt = sim.case.dt
         g.assign_hashes(p)
         g.sort(p)
-        g.get_cells(self.cs.value, self.ce.value)
+        g.get_cells(p, self.cs.value, self.ce.value)
         lf.init(p)
         t.reset_FEM_server()
         t.segments_wear_rate_reset()

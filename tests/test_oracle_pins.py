@@ -665,3 +665,100 @@ def test_plasticity_return_mapping():
     heat = case.const.trml_wp["tq"] / (case.const.trml_wp["cp"] * 4.43) * (eps_new - 0.1) * sy
     assert o.P["T"][0] == pytest.approx(500.0 + heat, rel=1e-12)
     assert np.array_equal(o.P["S"][1], [0.002, 0.0, -0.001, -0.001]) and o.P["eps_pl_dot"][1] == 0.0
+
+
+# ---------------------------------------------------------------- Rothlin grid
+def _moving_lattice(nx=24, ny=20, seed=4, grid="green"):
+    """Jittered tool-free lattice translating with a small random velocity spread (a few per cent of
+    the particles change cells per step), grid box padded so that nobody leaves it over a few dozen
+    steps."""
+    dx = 2.5e-4
+    c = cases.make_constants("Ti6Al4V", dx, 0.35)
+    rng = np.random.default_rng(seed)
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    pos = np.stack([ii.ravel() * dx, jj.ravel() * dx], axis=1) + 0.005
+    pos += rng.uniform(-0.3, 0.3, pos.shape) * dx
+    N = len(pos)
+    vel = np.array([0.05, 0.03]) + rng.normal(0.0, 1e-4, (N, 2))
+    case = cases.Case(name="moving_lattice", N=N, dx=dx, h=1.5 * dx, dt=2e-4, pos=pos, vel=vel,
+                      rho=np.full(N, c.phys["rho0"]), T=300.0 + rng.uniform(1.0, 20.0, N), hh=np.full(N, 1.5 * dx),
+                      fixed=np.zeros(N), const=c, grid_bbmin=pos.min(axis=0) - 4 * dx,
+                      grid_bbmax=pos.max(axis=0) + 4 * dx, tool=None, tool_vel=np.zeros(2), wear=False)
+    return O.Oracle(case, grid=grid)
+
+
+def _reference_rothlin_offsets(sorted_keys, num_cell):
+    """grid_gpu_rothlin.cu:19-78 step by step: unique_by_key_copy, |differences| of the first
+    indices (the last occupied cell gets 0), scatter to the cell map, inclusive scan into
+    offsets[1..], offsets[last_hash + 1 ..] = N, offsets[0] = 0."""
+    N = len(sorted_keys)
+    cmap, first = np.unique(sorted_keys, return_index=True)
+    occ = len(cmap)
+    stencil = np.zeros(occ, np.int64)
+    stencil[:occ - 1] = np.abs(first[:-1] - first[1:])
+    scatter = np.zeros(num_cell, np.int64)
+    scatter[cmap] = stencil
+    off = np.zeros(num_cell + 1, np.int64)
+    off[1:] = np.cumsum(scatter)
+    off[sorted_keys[-1] + 1:] = N
+    off[0] = 0
+    return off
+
+
+def test_rothlin_sort_gathers_and_dense_cells():
+    o = _moving_lattice()
+    N = o.case.N
+    rng = np.random.default_rng(9)
+    perm0 = rng.permutation(N)
+    for k in o.P:
+        o.P[k][:] = o.P[k][perm0]
+    labels = rng.permutation(N).astype(np.int32) + 1000
+    o.P["idx"][:] = labels
+    for f in ("fc", "ft", "n", "pos_t", "vel_t", "S_t"):
+        o.P[f][:] = rng.normal(0, 1, o.P[f].shape)
+    for f in ("eps_pl", "eps_pl_dot", "rho_t", "T_t", "fixed", "tool_particle"):
+        o.P[f][:] = rng.normal(0, 1, N)
+    o.P["on_seg"][:] = rng.integers(-1, 50, N)
+    o.P["blanked"][::11] = 1.0
+    o.call("o_compute_hashes", "cp")
+    o.P["hash"][o.P["blanked"] == 1.0] = rng.integers(0, o.st["cell_start"].size, (o.P["blanked"] == 1.0).sum())
+    before = {k: v.copy() for k, v in o.P.items()}
+    keys = before["hash"]
+    o.call("o_sort_rothlin", "cps")
+    ref = np.argsort(keys, kind="stable")
+    assert np.array_equal(o.P["hash"], keys[ref])
+    assert np.array_equal(o.P["idx"], labels[ref])                  # labels travel with the particles
+    for f in ("pos", "vel", "fc", "h", "rho", "S", "eps_pl", "eps_pl_dot", "fixed", "blanked", "tool_particle",
+              "T", "pos_t", "vel_t", "rho_t", "S_t", "T_t", "on_seg"):
+        assert np.array_equal(o.P[f], before[f][ref]), f
+    for f in ("ft", "n", "p", "R", "S_der", "v_der"):                # not gathered by the reference
+        assert np.array_equal(o.P[f], before[f]), f
+    nc = o.st["cell_start"].size
+    sk = keys[ref]
+    lb = np.searchsorted(sk, np.arange(nc + 1), side="left")
+    assert np.array_equal(o.st["cell_start"], lb[:-1])
+    assert np.array_equal(o.st["cell_end"], lb[1:])
+    off = _reference_rothlin_offsets(sk, nc)
+    assert np.array_equal(o.st["cell_start"], off[:-1]) and np.array_equal(o.st["cell_end"], off[1:])
+    empty = np.setdiff1d(np.arange(nc), sk)
+    assert empty.size > 0 and np.all(o.st["cell_start"][empty] == o.st["cell_end"][empty])
+
+
+def test_rothlin_step_equals_green_step_without_tool():
+    """Same keys, same previous order, same stable sort: the particle order and every field agree
+    with the Green grid's step bit for bit (ft and n, the fields the Rothlin sort leaves in place,
+    are only written and read by the tool contact); idx carries labels instead of the permutation."""
+    g, r = _moving_lattice(), _moving_lattice(grid="rothlin")
+    g.step(15)
+    r.step(15)
+    labels = r.P["idx"].copy()
+    g.step(1)
+    r.step(1)
+    perm = g.P["idx"]
+    assert np.array_equal(np.sort(perm), np.arange(g.case.N))
+    assert (perm != np.arange(g.case.N)).sum() > 0                  # the last step did reorder
+    assert np.array_equal(r.P["idx"], labels[perm])
+    for f in g.P:
+        if f != "idx":
+            assert np.array_equal(r.P[f], g.P[f]), f
+    assert np.isfinite(g.P["pos"]).all()
