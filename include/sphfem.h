@@ -133,8 +133,20 @@ int sphfem_grid_green_create(sphfem_ctx *ctx, int num_part, sphfem_double2 bbmin
  * geometry and hashes as the Green grid; sort and get_cells follow grid_gpu_rothlin.cu (below). */
 int sphfem_grid_rothlin_create(sphfem_ctx *ctx, int num_part, sphfem_double2 bbmin, sphfem_double2 bbmax,
                                double h, sphfem_grid **out);
+/* grid_gpu_green(max_cell, N) (grid_gpu_green.cu:258-262) and grid_gpu_rothlin(max_cell, N)
+ * (grid_gpu_rothlin.cu:277-280): geometry not locked; sphfem_grid_update_geometry (called by the
+ * step) sets it from the particles, for up to max_cell - 1 cells. */
+int sphfem_grid_green_create_unlocked(sphfem_ctx *ctx, int max_cell, int num_part, sphfem_grid **out);
+int sphfem_grid_rothlin_create_unlocked(sphfem_ctx *ctx, int max_cell, int num_part, sphfem_grid **out);
+/* grid.cu:88-127: no-op for a locked grid; otherwise bounding box of all particle positions
+ * less/plus 1e-6, cell size max(h) * kernel_width / 2, ceil counts.  SPHFEM_ERR_RANGE when that
+ * needs max_cell cells or more (the reference asserts).  Synchronises the stream (the box is read
+ * back to the host, as in the reference). */
+int sphfem_grid_update_geometry(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles *p, double kernel_width);
 int sphfem_grid_destroy(sphfem_ctx *ctx, sphfem_grid *g);
 int sphfem_grid_geometry(const sphfem_grid *g, sphfem_geom_constants *geom, int *num_cell);
+/* grid.h:45-50: bbmax_x/bbmax_y, is_locked, and the cell capacity */
+int sphfem_grid_bounds(const sphfem_grid *g, sphfem_double2 *bbmax, int *locked, int *max_cell);
 /* SPHFEM_GRID_GREEN or SPHFEM_GRID_ROTHLIN */
 int sphfem_grid_kind(const sphfem_grid *g, int *kind);
 /* grid.cu:80-86 */
@@ -236,7 +248,8 @@ int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particle
 
 /* Step graphs: `steps` (even) steps of sphfem_leap_frog_step recorded into one HIP graph and
  * replayed with sphfem_leap_frog_graph_launch (each launch advances `steps` steps).  Capture
- * records only (the state does not advance) and needs one eager step first; the graph bakes in
+ * records only (the state does not advance) and needs a locked grid and, for a Green grid, one
+ * eager step first; the graph bakes in
  * the constants, dt, wear switch and precision at capture.  No counterpart in the reference
  * (its step is a sequence of launches with host synchronisations). */
 typedef struct sphfem_graph sphfem_graph;

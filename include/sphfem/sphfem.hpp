@@ -213,16 +213,32 @@ class grid_base {
     virtual ~grid_base() {
         if (m_handle && sphfem_shim::ctx_slot()) (void)sphfem_grid_destroy(sphfem_shim::ctx_slot(), m_handle);
     }
-    int nx() const { return m_geom.nx; }
-    int ny() const { return m_geom.ny; }
-    double bbmin_x() const { return m_geom.bbmin_x; }
-    double bbmin_y() const { return m_geom.bbmin_y; }
-    double dx() const { return m_geom.dx; }
-    bool is_locked() const { return true; }
-    int num_cell() const { return m_num_cell; }
+    // the geometry of an unlocked grid changes at every update_geometry: read it from the handle
+    int nx() const { return geom().nx; }
+    int ny() const { return geom().ny; }
+    double bbmin_x() const { return geom().bbmin_x; }
+    double bbmin_y() const { return geom().bbmin_y; }
+    double bbmax_x() const { return bounds().x; }
+    double bbmax_y() const { return bounds().y; }
+    double dx() const { return geom().dx; }
+    bool is_locked() const {
+        int locked = 1;
+        sphfem_shim::check(sphfem_grid_bounds(m_handle, nullptr, &locked, nullptr), "grid is_locked");
+        return locked != 0;
+    }
+    int num_cell() const {
+        int n = 0;
+        sphfem_shim::check(sphfem_grid_geometry(m_handle, nullptr, &n), "grid num_cell");
+        return n;
+    }
 
     void assign_hashes(particle_gpu *particles, tool *) const {
         sphfem_shim::check(sphfem_grid_assign_hashes(sphfem_shim::ctx(), m_handle, particles), "assign_hashes");
+    }
+    // grid.cu:88-127 (no-op for a locked grid)
+    void update_geometry(particle_gpu *particles, tool *, double kernel_width = 2.0) {
+        sphfem_shim::check(sphfem_grid_update_geometry(sphfem_shim::ctx(), m_handle, particles, kernel_width),
+                           "update_geometry");
     }
     virtual void sort(particle_gpu *particles, tool *t) const = 0;
     virtual void get_cells(particle_gpu *particles, int *cell_start, int *cell_end) = 0;
@@ -230,13 +246,18 @@ class grid_base {
 
   protected:
     grid_base() = default;
-    void adopt(sphfem_grid *g) {
-        m_handle = g;
-        sphfem_shim::check(sphfem_grid_geometry(g, &m_geom, &m_num_cell), "grid geometry");
+    void adopt(sphfem_grid *g) { m_handle = g; }
+    geom_constants geom() const {
+        geom_constants gc{};
+        sphfem_shim::check(sphfem_grid_geometry(m_handle, &gc, nullptr), "grid geometry");
+        return gc;
+    }
+    float2_t bounds() const {
+        float2_t b{};
+        sphfem_shim::check(sphfem_grid_bounds(m_handle, &b, nullptr, nullptr), "grid bounds");
+        return b;
     }
     sphfem_grid *m_handle = nullptr;
-    geom_constants m_geom{};
-    int m_num_cell = 0;
 };
 
 class grid_gpu_green : public grid_base {
@@ -245,6 +266,13 @@ class grid_gpu_green : public grid_base {
         sphfem_grid *g = nullptr;
         sphfem_shim::check(sphfem_grid_green_create(sphfem_shim::ctx(), num_part, bbmin, bbmax, h, &g),
                            "grid_gpu_green");
+        adopt(g);
+    }
+    // grid_gpu_green.cu:258-262: geometry from the particles at each update_geometry
+    grid_gpu_green(unsigned int max_cell, unsigned int N) {
+        sphfem_grid *g = nullptr;
+        sphfem_shim::check(sphfem_grid_green_create_unlocked(sphfem_shim::ctx(), (int)max_cell, (int)N, &g),
+                           "grid_gpu_green(max_cell, N)");
         adopt(g);
     }
     void sort(particle_gpu *particles, tool *) const override {
@@ -269,6 +297,13 @@ class grid_gpu_rothlin : public grid_base {
         sphfem_grid *g = nullptr;
         sphfem_shim::check(sphfem_grid_rothlin_create(sphfem_shim::ctx(), num_part, bbmin, bbmax, h, &g),
                            "grid_gpu_rothlin");
+        adopt(g);
+    }
+    // grid_gpu_rothlin.cu:277-280
+    grid_gpu_rothlin(int max_cell, int N) {
+        sphfem_grid *g = nullptr;
+        sphfem_shim::check(sphfem_grid_rothlin_create_unlocked(sphfem_shim::ctx(), max_cell, N, &g),
+                           "grid_gpu_rothlin(max_cell, N)");
         adopt(g);
     }
     void sort(particle_gpu *particles, tool *) const override {

@@ -86,6 +86,8 @@ def lib():
         sig = {
             "o_compute_hashes": (pc, pp), "o_sort_and_cells": (pc, pp, ps), "o_init": (pp, ps),
             "o_sort_rothlin": (pc, pp, ps), "o_run_rothlin": (pc, pp, ps, pt, C.c_int),
+            "o_update_geometry": (pc, pp, _dbl, C.c_int),
+            "o_run_unlocked": (pc, pp, ps, pt, C.c_int, C.c_int, C.c_int),
             "o_predict": (pp, ps, _dbl), "o_correct": (pp, ps, _dbl), "o_material_eos": (pc, pp),
             "o_artificial_stress": (pc, pp), "o_interactions_monaghan": (pc, pp, ps),
             "o_interactions_heat_pse": (pc, pp, ps),
@@ -110,6 +112,8 @@ def lib():
         L.o_sigma_yield.restype = _dbl
         L.o_solve_secant.restype = _dbl
         L.o_num_threads.restype = C.c_int
+        L.o_update_geometry.restype = C.c_int
+        L.o_run_unlocked.restype = C.c_int
         _lib = L
     return _lib
 
@@ -148,17 +152,20 @@ def _ptr(a):
 class Oracle:
     """The reference step on the host, over numpy arrays in the reference layout."""
 
-    def __init__(self, case: cases.Case, rr2_min=None, wear_pro=None, threads=None, grid="green"):
+    def __init__(self, case: cases.Case, rr2_min=None, wear_pro=None, threads=None, grid="green", max_cell=None):
+        """grid: "green" or "rothlin"; max_cell: an unlocked grid of that capacity whose geometry
+        follows the particles each step (grid_base(max_cell, N)), else the case's locked grid."""
         if grid not in ("green", "rothlin"):
             raise ValueError(grid)
         self.grid = grid
+        self.max_cell = max_cell
         self.case = case
         self.c = make_consts(case, rr2_min, wear_pro)
         if threads:
             lib().o_set_num_threads(int(threads))
         self.P = cases.initial_particles(case)
         N = case.N
-        nc = self.c.geom.nx * self.c.geom.ny
+        nc = int(max_cell) if max_cell else self.c.geom.nx * self.c.geom.ny
         self.st = dict(pos_init=np.zeros((N, 2)), vel_init=np.zeros((N, 2)), S_init=np.zeros((N, 4)),
                        rho_init=np.zeros(N), T_init=np.zeros(N),
                        cell_start=np.full(nc, -1, dtype=np.int32), cell_end=np.zeros(nc, dtype=np.int32))
@@ -191,6 +198,7 @@ class Oracle:
         import copy
         o = Oracle.__new__(Oracle)
         o.grid = self.grid
+        o.max_cell = self.max_cell
         o.case = self.case
         o.c = OConsts()
         C.pointer(o.c)[0] = self.c
@@ -218,6 +226,12 @@ class Oracle:
 
     def step(self, n=1):
         tc = C.byref(self.Tc) if self.Tc is not None else None
+        if self.max_cell:
+            done = lib().o_run_unlocked(C.byref(self.c), C.byref(self.Pc), C.byref(self.Sc), tc, int(n),
+                                        int(self.max_cell), int(self.grid == "rothlin"))
+            if done != n:
+                raise RuntimeError(f"unlocked grid outgrew max_cell={self.max_cell} at step {done}")
+            return
         run = lib().o_run_rothlin if self.grid == "rothlin" else lib().o_run
         run(C.byref(self.c), C.byref(self.Pc), C.byref(self.Sc), tc, int(n))
 

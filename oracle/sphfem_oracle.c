@@ -1020,6 +1020,40 @@ void o_run(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t, int nste
     for (int s = 0; s < nsteps; s++) step_impl(c, P, st, t, 0);
 }
 
+/* grid.cu:88-127 for an unlocked grid: bounding box of all positions (min_element /
+ * max_element by x and by y), margins of 1e-6, cell size max(h) * kernel_width / 2, ceil counts.
+ * Returns 0, or -1 when num_cell >= max_cell (the reference's assert). */
+int o_update_geometry(o_consts *c, const o_particles *P, double kernel_width, int max_cell) {
+    double mnx = P->pos[0].x, mny = P->pos[0].y, mxx = P->pos[0].x, mxy = P->pos[0].y, mh = P->h[0];
+    for (int i = 1; i < P->N; i++) {
+        if (P->pos[i].x < mnx) mnx = P->pos[i].x;
+        if (P->pos[i].y < mny) mny = P->pos[i].y;
+        if (P->pos[i].x > mxx) mxx = P->pos[i].x;
+        if (P->pos[i].y > mxy) mxy = P->pos[i].y;
+        if (P->h[i] > mh) mh = P->h[i];
+    }
+    const double bx0 = mnx - 1e-6, by0 = mny - 1e-6, bx1 = mxx + 1e-6, by1 = mxy + 1e-6;
+    const double dx = mh * kernel_width / 2;
+    const double nx = ceil((bx1 - bx0) / dx), ny = ceil((by1 - by0) / dx);
+    if (!(nx * ny < (double)max_cell)) return -1;
+    c->geom.nx = (int)nx;
+    c->geom.ny = (int)ny;
+    c->geom.bbmin_x = bx0;
+    c->geom.bbmin_y = by0;
+    c->geom.dx = dx;
+    return 0;
+}
+
+/* leap_frog::step with an unlocked grid (leap_frog.cu:175): geometry from the particles first.
+ * Returns the number of steps done (fewer than nsteps if the grid outgrew max_cell). */
+int o_run_unlocked(o_consts *c, o_particles *P, o_stepper *st, o_tool *t, int nsteps, int max_cell, int rothlin) {
+    for (int s = 0; s < nsteps; s++) {
+        if (o_update_geometry(c, P, 2., max_cell)) return s;
+        step_impl(c, P, st, t, rothlin);
+    }
+    return nsteps;
+}
+
 /* leap_frog::step with a grid_gpu_rothlin (the same step, the Rothlin sort and cells) */
 void o_run_rothlin(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t, int nsteps) {
     for (int s = 0; s < nsteps; s++) step_impl(c, P, st, t, 1);

@@ -120,6 +120,9 @@ void o_report_force(const o_particles *P, double *fx, double *fy);
 void o_step(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t);
 void o_run(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t, int nsteps);
 void o_run_rothlin(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t, int nsteps);
+/* grid.cu:88-127 and a step on an unlocked grid */
+int o_update_geometry(o_consts *c, const o_particles *P, double kernel_width, int max_cell);
+int o_run_unlocked(o_consts *c, o_particles *P, o_stepper *st, o_tool *t, int nsteps, int max_cell, int rothlin);
 
 int o_num_threads(void);
 void o_set_num_threads(int n);

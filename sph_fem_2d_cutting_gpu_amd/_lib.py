@@ -91,6 +91,10 @@ _SIGS = {
     "sphfem_grid_green_create": (_ctx, C.c_int, Double2, Double2, _dbl, _P(_vp)),
     "sphfem_grid_rothlin_create": (_ctx, C.c_int, Double2, Double2, _dbl, _P(_vp)),
     "sphfem_grid_kind": (_vp, _P(C.c_int)),
+    "sphfem_grid_green_create_unlocked": (_ctx, C.c_int, C.c_int, _P(_vp)),
+    "sphfem_grid_rothlin_create_unlocked": (_ctx, C.c_int, C.c_int, _P(_vp)),
+    "sphfem_grid_update_geometry": (_ctx, _vp, _P(Particles), _dbl),
+    "sphfem_grid_bounds": (_vp, _P(Double2), _P(C.c_int), _P(C.c_int)),
     "sphfem_grid_destroy": (_ctx, _vp),
     "sphfem_grid_set_resort_limit": (_ctx, _vp, C.c_int),
     "sphfem_leap_frog_graph_create": (_ctx, _vp, _P(Particles), _vp, _vp, C.c_int, _P(_vp)),

@@ -314,6 +314,43 @@ int sphfem_particles_free(sphfem_ctx *ctx, sphfem_particles *p) {
 }
 
 // ---- grid (grid.cu:130-149, grid_gpu_green.cu:220-268, grid_gpu_rothlin.cu:251-285)
+// device arrays of a grid with g->max_cell cells at most
+static void grid_alloc(sphfem_ctx *ctx, sphfem_grid *g, int kind, int num_part) {
+    g->num_part = num_part;
+    int bits = 1;
+    while (bits < 31 && (1ll << bits) < (long long)g->max_cell) bits++;
+    g->key_bits = bits;
+    g->cell_start = dalloc<int>(g->max_cell);
+    g->cell_end = dalloc<int>(g->max_cell);
+    SPHFEM_HIP_CHECK(hipMemsetAsync(g->cell_end, 0, sizeof(int) * (size_t)g->max_cell, ctx->stream));
+    launch_fill_int(ctx, g->cell_start, g->max_cell, -1);
+    g->keys_alt = dalloc<int>(num_part);
+    g->vals_alt = dalloc<int>(num_part);
+    g->nblocks = nblk(num_part, 4096);
+    g->hist = dalloc<int>(256ul * g->nblocks);
+    g->digit_tot = dalloc<int>(256);
+    const int nw = nblk(num_part, 64);
+    g->cmask = dalloc<unsigned long long>(nw);
+    g->wprefix = dalloc<int>(nw + 1);
+    g->bprefix = dalloc<int>(g->nblocks + 1);
+    g->clist = dalloc<int2>(SPHFEM_RESORT_CAP);
+    g->csorted = dalloc<unsigned long long>(SPHFEM_RESORT_CAP);
+    g->cdest = dalloc<int4>(SPHFEM_RESORT_CAP);
+    alloc_particles(ctx->stream, (unsigned)num_part, g->buf);
+    g->first_sort = true;
+    g->kind = kind;
+    if (kind == SPHFEM_GRID_ROTHLIN) {
+        g->perm = dalloc<int>(num_part);
+        g->iota = dalloc<int>(num_part);
+        launch_iota(ctx, g->iota, num_part);
+        g->first_sort = false;   // every Rothlin sort is a full sort
+    }
+    if (!g->locked) {
+        g->bbox_part = dalloc<double>(5 * SPHFEM_BBOX_BLOCKS);
+        g->bbox_out = dalloc<double>(5);
+    }
+}
+
 static int grid_create(sphfem_ctx *ctx, int kind, int num_part, sphfem_double2 bbmin, sphfem_double2 bbmax,
                        double h, sphfem_grid **out) {
     if (!out || num_part <= 0 || !(h > 0.)) return SPHFEM_ERR_ARG;
@@ -337,39 +374,80 @@ static int grid_create(sphfem_ctx *ctx, int kind, int num_part, sphfem_double2 b
         g->bbmax_x = bx1;
         g->bbmax_y = by1;
         g->num_cell = g->geom.nx * g->geom.ny;
-        g->num_part = num_part;
-        int bits = 1;
-        while (bits < 31 && (1ll << bits) < (long long)g->num_cell) bits++;
-        g->key_bits = bits;
-        g->cell_start = dalloc<int>(g->num_cell);
-        g->cell_end = dalloc<int>(g->num_cell);
-        SPHFEM_HIP_CHECK(hipMemsetAsync(g->cell_end, 0, sizeof(int) * (size_t)g->num_cell, ctx->stream));
-        launch_fill_int(ctx, g->cell_start, g->num_cell, -1);
-        g->keys_alt = dalloc<int>(num_part);
-        g->vals_alt = dalloc<int>(num_part);
-        g->nblocks = nblk(num_part, 4096);
-        g->hist = dalloc<int>(256ul * g->nblocks);
-        g->digit_tot = dalloc<int>(256);
-        const int nw = nblk(num_part, 64);
-        g->cmask = dalloc<unsigned long long>(nw);
-        g->wprefix = dalloc<int>(nw + 1);
-        g->bprefix = dalloc<int>(g->nblocks + 1);
-        g->clist = dalloc<int2>(SPHFEM_RESORT_CAP);
-        g->csorted = dalloc<unsigned long long>(SPHFEM_RESORT_CAP);
-        g->cdest = dalloc<int4>(SPHFEM_RESORT_CAP);
-        alloc_particles(ctx->stream, (unsigned)num_part, g->buf);
-        g->first_sort = true;
-        g->kind = kind;
-        if (kind == SPHFEM_GRID_ROTHLIN) {
-            g->perm = dalloc<int>(num_part);
-            g->iota = dalloc<int>(num_part);
-            launch_iota(ctx, g->iota, num_part);
-            g->first_sort = false;   // every Rothlin sort is a full sort
-        }
+        g->max_cell = g->num_cell;   // grid.cu:146
+        g->locked = true;
+        grid_alloc(ctx, g, kind, num_part);
         ctx->c.geom = g->geom;
         *out = g;
         return SPHFEM_OK;
     });
+}
+
+// grid_base(max_cell, num_part) (grid.cu:151-153): geometry set by each update_geometry
+static int grid_create_unlocked(sphfem_ctx *ctx, int kind, int max_cell, int num_part, sphfem_grid **out) {
+    if (!out || num_part <= 0 || max_cell <= 0) return SPHFEM_ERR_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        auto *g = new sphfem_grid();
+        g->locked = false;
+        g->max_cell = max_cell;
+        g->num_cell = max_cell;
+        grid_alloc(ctx, g, kind, num_part);
+        *out = g;
+        return SPHFEM_OK;
+    });
+}
+
+// grid.cu:88-127: bounding box of all particles less/plus 1e-6, cell size max(h) * kernel_width / 2,
+// ceil counts; the reference asserts num_cell < max_cell.  Reads the reduction back to the host
+// (as the reference does), so it synchronises the stream.
+static int update_geometry(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, double kernel_width) {
+    if (g->locked) return SPHFEM_OK;
+    launch_bbox(ctx, p, g->bbox_part, g->bbox_out);
+    double b[5];
+    SPHFEM_HIP_CHECK(hipMemcpyAsync(b, g->bbox_out, sizeof(b), hipMemcpyDeviceToHost, ctx->stream));
+    SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const double bx0 = b[0] - 1e-6, by0 = b[1] - 1e-6;
+    const double bx1 = b[2] + 1e-6, by1 = b[3] + 1e-6;
+    const double dx = b[4] * kernel_width / 2;
+    const double nxd = std::ceil((bx1 - bx0) / dx), nyd = std::ceil((by1 - by0) / dx);
+    if (!(dx > 0.) || !std::isfinite(nxd) || !std::isfinite(nyd))
+        return fail(ctx, SPHFEM_ERR_RANGE, "update_geometry: non-finite bounding box or h");
+    if (!(nxd * nyd < (double)g->max_cell))
+        return fail(ctx, SPHFEM_ERR_RANGE, "update_geometry: the bounding box needs max_cell cells or more");
+    sphfem_geom_constants ng{(int)nxd, (int)nyd, bx0, by0, dx};
+    if (ng.nx != g->geom.nx || ng.ny != g->geom.ny || ng.bbmin_x != g->geom.bbmin_x ||
+        ng.bbmin_y != g->geom.bbmin_y || ng.dx != g->geom.dx) {
+        // the previous keys belong to another geometry: the next Green sort is a full one
+        if (g->kind == SPHFEM_GRID_GREEN) g->first_sort = true;
+    }
+    g->geom = ng;
+    g->num_cell = ng.nx * ng.ny;
+    g->bbmax_x = bx1;
+    g->bbmax_y = by1;
+    ctx->c.geom = g->geom;
+    return SPHFEM_OK;
+}
+
+int sphfem_grid_green_create_unlocked(sphfem_ctx *ctx, int max_cell, int num_part, sphfem_grid **out) {
+    return grid_create_unlocked(ctx, SPHFEM_GRID_GREEN, max_cell, num_part, out);
+}
+
+int sphfem_grid_rothlin_create_unlocked(sphfem_ctx *ctx, int max_cell, int num_part, sphfem_grid **out) {
+    return grid_create_unlocked(ctx, SPHFEM_GRID_ROTHLIN, max_cell, num_part, out);
+}
+
+int sphfem_grid_update_geometry(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles *p, double kernel_width) {
+    if (!g || !p || (int)p->N != g->num_part || !(kernel_width > 0.)) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] { return update_geometry(ctx, g, *p, kernel_width); });
+}
+
+int sphfem_grid_bounds(const sphfem_grid *g, sphfem_double2 *bbmax, int *locked, int *max_cell) {
+    if (!g) return SPHFEM_ERR_ARG;
+    if (bbmax) *bbmax = {g->bbmax_x, g->bbmax_y};
+    if (locked) *locked = g->locked ? 1 : 0;
+    if (max_cell) *max_cell = g->max_cell;
+    return SPHFEM_OK;
 }
 
 int sphfem_grid_green_create(sphfem_ctx *ctx, int num_part, sphfem_double2 bbmin, sphfem_double2 bbmax, double h,
@@ -395,7 +473,7 @@ int sphfem_grid_destroy(sphfem_ctx *ctx, sphfem_grid *g) {
         dfree(g->cell_start); dfree(g->cell_end); dfree(g->keys_alt); dfree(g->vals_alt);
         dfree(g->hist); dfree(g->digit_tot);
         dfree(g->cmask); dfree(g->wprefix); dfree(g->bprefix); dfree(g->clist); dfree(g->csorted); dfree(g->cdest);
-        dfree(g->perm); dfree(g->iota);
+        dfree(g->perm); dfree(g->iota); dfree(g->bbox_part); dfree(g->bbox_out);
         free_particles(g->buf);
         delete g;
         return SPHFEM_OK;
@@ -412,6 +490,7 @@ int sphfem_grid_geometry(const sphfem_grid *g, sphfem_geom_constants *geom, int 
 int sphfem_grid_assign_hashes(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles *p) {
     if (!g || !p || (int)p->N != g->num_part) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {
+        if (!(g->geom.dx > 0.)) return fail(ctx, SPHFEM_ERR_STATE, "unlocked grid: update_geometry first");
         ctx->c.geom = g->geom;
         launch_hash(ctx, g, *p, false, nullptr);
         // p.hash no longer holds the keys of the (Green) cell table
@@ -965,7 +1044,9 @@ int sphfem_leap_frog_correct(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_parti
 //  tool_begin | hash+digit-0 histogram | radix passes | reorder+init+predict+EOS+art. stress |
 //  pair sums+Jaumann+continuity+momentum+advection+correct+friction heat+plasticity+far contact+
 //  BC+invalidate | FEM rate+temperature | wear | tool move | near contact s1 (+BC) | contact s2
-static void step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g, sphfem_tool *t) {
+static int step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g, sphfem_tool *t) {
+    // leap_frog.cu:175 (a no-op for a locked grid)
+    if (int rc = update_geometry(ctx, g, *p, 2.)) return rc;
     ctx->c.geom = g->geom;
     if (t) launch_tool_begin(ctx, t);
     else launch_fill_int(ctx, ctx->d_flags + 1, 2, 0);
@@ -1002,6 +1083,7 @@ static void step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p
         launch_tool_move(ctx, t, ctx->c.dt, true, true);
         launch_near_contact(ctx, *p, t, lf);
     }
+    return SPHFEM_OK;
 }
 
 int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,
@@ -1009,8 +1091,7 @@ int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particle
     if (!lf || !p || !g || (int)p->N != g->num_part || p->N != lf->num_part) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {
         if (int rc = need(ctx, SET_PHYS | SET_CORR | SET_TRML_WP | SET_DT | (t ? SET_TOOLC : 0u))) return rc;
-        step_body(ctx, lf, p, g, t);
-        return SPHFEM_OK;
+        return step_body(ctx, lf, p, g, t);
     });
 }
 
@@ -1031,6 +1112,8 @@ int sphfem_leap_frog_graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_
     *out = nullptr;
     return guarded(ctx, [&] {
         if (int rc = need(ctx, SET_PHYS | SET_CORR | SET_TRML_WP | SET_DT | (t ? SET_TOOLC : 0u))) return rc;
+        if (!g->locked)
+            return fail(ctx, SPHFEM_ERR_STATE, "graphs need a locked grid (the geometry update reads back to the host)");
         if (g->first_sort) return fail(ctx, SPHFEM_ERR_STATE, "run one step before capturing (first sort)");
         const bool timing = ctx->timing.enabled;
         ctx->timing.enabled = false;
@@ -1040,7 +1123,7 @@ int sphfem_leap_frog_graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_
             SPHFEM_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             // recording only: the device state does not advance; after an even number of steps the
             // host-side pointer assignment is the one the graph starts from
-            for (int k = 0; k < steps; k++) step_body(ctx, lf, p, g, t);
+            for (int k = 0; k < steps; k++) (void)step_body(ctx, lf, p, g, t);   // locked: cannot fail
             SPHFEM_HIP_CHECK(hipStreamEndCapture(ctx->stream, &gr->graph));
             SPHFEM_HIP_CHECK(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0));
         } catch (...) {

@@ -12,6 +12,8 @@
 
 // largest number of particles that may change cells in one step for the incremental re-sort
 #define SPHFEM_RESORT_CAP 4096
+// blocks of the bounding-box reduction of an unlocked grid
+#define SPHFEM_BBOX_BLOCKS 1024
 
 namespace sphfem {
 
@@ -92,6 +94,11 @@ struct sphfem_grid {
     int kind = SPHFEM_GRID_GREEN;
     int *perm = nullptr;                   // [N] permutation of the last Rothlin sort
     int *iota = nullptr;                   // [N] 0..N-1: the identity gather of the step after an in-place sort
+    // geometry from the particles' bounding box each step (grid_base(max_cell, N), grid.cu:88-127)
+    bool locked = true;
+    int max_cell = 0;
+    double *bbox_part = nullptr;           // [5 * SPHFEM_BBOX_BLOCKS] per-block min x, min y, max x, max y, max h
+    double *bbox_out = nullptr;            // [5]
 };
 
 struct sphfem_tool {
@@ -181,6 +188,7 @@ void launch_radix_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, boo
 void launch_rothlin_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p);
 void launch_cells_dense(sphfem_ctx *ctx, const int *sorted_keys, int N, int num_cell, int *cell_start, int *cell_end);
 void launch_iota(sphfem_ctx *ctx, int *ptr, int n);
+void launch_bbox(sphfem_ctx *ctx, const sphfem_particles &p, double *part, double *out);
 void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p);
 void launch_fill_int(sphfem_ctx *ctx, int *ptr, long n, int value);
 void launch_reorder_full(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p);

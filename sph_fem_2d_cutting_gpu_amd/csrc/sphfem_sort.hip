@@ -667,6 +667,64 @@ void launch_iota(sphfem_ctx *ctx, int *ptr, int n) {
     k_iota<<<nblk(n, 256), 256, 0, ctx->stream>>>(ptr, n);
 }
 
+// grid.cu:88-127: smallest and largest x and y over all particle positions and the largest h
+// (blanked particles included, as the reference's min/max_element over the whole arrays).
+// Positions are finite; min and max of finite values do not depend on the reduction order.
+__device__ __forceinline__ void bbox_combine(double *a, const double *b) {
+    a[0] = fmin(a[0], b[0]);
+    a[1] = fmin(a[1], b[1]);
+    a[2] = fmax(a[2], b[2]);
+    a[3] = fmax(a[3], b[3]);
+    a[4] = fmax(a[4], b[4]);
+}
+
+__device__ void bbox_block_reduce(double *v, double *out) {
+    __shared__ double red[5][256 / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        double o[5];
+        for (int k = 0; k < 5; k++) o[k] = __shfl_xor(v[k], off);
+        bbox_combine(v, o);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+        for (int k = 0; k < 5; k++) red[k][wave] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc[5];
+        for (int k = 0; k < 5; k++) acc[k] = red[k][0];
+        for (int w = 1; w < 256 / 64; w++) {
+            double o[5];
+            for (int k = 0; k < 5; k++) o[k] = red[k][w];
+            bbox_combine(acc, o);
+        }
+        for (int k = 0; k < 5; k++) out[k] = acc[k];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_bbox_partial(const sphfem_double2 *__restrict__ pos,
+                                                      const double *__restrict__ h, int N, double *__restrict__ part) {
+    double v[5] = {INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const sphfem_double2 q = pos[i];
+        const double o[5] = {q.x, q.y, q.x, q.y, h[i]};
+        bbox_combine(v, o);
+    }
+    bbox_block_reduce(v, part + 5 * blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_bbox_final(const double *__restrict__ part, int nb, double *__restrict__ out) {
+    double v[5] = {INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) bbox_combine(v, part + 5 * b);
+    bbox_block_reduce(v, out);
+}
+
+void launch_bbox(sphfem_ctx *ctx, const sphfem_particles &p, double *part, double *out) {
+    const int nb = std::max(1, std::min(nblk(p.N, 256), SPHFEM_BBOX_BLOCKS));
+    KTimer t(ctx, "bbox");
+    k_bbox_partial<<<nb, 256, 0, ctx->stream>>>(p.pos, p.h, (int)p.N, part);
+    k_bbox_final<<<1, 256, 0, ctx->stream>>>(part, nb, out);
+}
+
 // p.hash must hold the keys; leaves the sorted keys in p.hash, the gathered arrays in place and
 // the dense cell table in the grid
 void launch_rothlin_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p) {

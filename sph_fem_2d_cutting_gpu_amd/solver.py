@@ -123,9 +123,12 @@ class particle_gpu:
 
 
 class grid_base:
-    """grid_base (grid.h:23-64): locked cell grid of cell size h over [bbmin, bbmax] (grid.cu:130-149)."""
+    """grid_base (grid.h:23-64).  grid_X(ctx, num_part, bbmin, bbmax, h): locked cell grid of cell
+    size h over [bbmin, bbmax] (grid.cu:130-149); grid_X.unlocked(ctx, max_cell, num_part): the
+    geometry follows the particles at each update_geometry (grid.cu:88-127), which the step calls."""
 
     _create = None
+    _create_unlocked = None
     kind = None
 
     def __init__(self, ctx: Context, num_part: int, bbmin, bbmax, h: float):
@@ -133,17 +136,65 @@ class grid_base:
         self.h = C.c_void_p()
         ctx.call(self._create, int(num_part), _lib.Double2(*map(float, bbmin)),
                  _lib.Double2(*map(float, bbmax)), float(h), C.byref(self.h))
+
+    @classmethod
+    def unlocked(cls, ctx: Context, max_cell: int, num_part: int):
+        g = cls.__new__(cls)
+        g.ctx = ctx
+        g.h = C.c_void_p()
+        ctx.call(cls._create_unlocked, int(max_cell), int(num_part), C.byref(g.h))
+        return g
+
+    def _geometry(self):
         g = _lib.GeomConstants()
         nc = C.c_int()
         check(lib().sphfem_grid_geometry(self.h, C.byref(g), C.byref(nc)), None, "grid_geometry")
-        self.geom = g
-        self.num_cell = nc.value
+        return g, nc.value
+
+    def _bounds(self):
+        bb, locked, mc = _lib.Double2(), C.c_int(), C.c_int()
+        check(lib().sphfem_grid_bounds(self.h, C.byref(bb), C.byref(locked), C.byref(mc)), None, "grid_bounds")
+        return bb, bool(locked.value), mc.value
+
+    @property
+    def geom(self):
+        return self._geometry()[0]
+
+    @property
+    def num_cell(self):
+        return self._geometry()[1]
+
+    @property
+    def max_cell(self):
+        return self._bounds()[2]
+
+    def is_locked(self):
+        return self._bounds()[1]
 
     def nx(self):
         return self.geom.nx
 
     def ny(self):
         return self.geom.ny
+
+    def bbmin_x(self):
+        return self.geom.bbmin_x
+
+    def bbmin_y(self):
+        return self.geom.bbmin_y
+
+    def bbmax_x(self):
+        return self._bounds()[0].x
+
+    def bbmax_y(self):
+        return self._bounds()[0].y
+
+    def dx(self):
+        return self.geom.dx
+
+    def update_geometry(self, p: particle_gpu, kernel_width: float = 2.0):
+        """grid.cu:88-127 (no-op for a locked grid; synchronises the stream)."""
+        self.ctx.call("sphfem_grid_update_geometry", self.h, C.byref(p.s), float(kernel_width))
 
     def assign_hashes(self, p: particle_gpu):
         self.ctx.call("sphfem_grid_assign_hashes", self.h, C.byref(p.s))
@@ -185,6 +236,7 @@ class grid_gpu_green(grid_base):
     buffer and swaps pointers; cell tables mark empty cells with -1."""
 
     _create = "sphfem_grid_green_create"
+    _create_unlocked = "sphfem_grid_green_create_unlocked"
     kind = "green"
 
 
@@ -194,6 +246,7 @@ class grid_gpu_rothlin(grid_base):
     computed from the particles' sorted hashes."""
 
     _create = "sphfem_grid_rothlin_create"
+    _create_unlocked = "sphfem_grid_rothlin_create_unlocked"
     kind = "rothlin"
 
 
@@ -418,14 +471,19 @@ class Simulation:
     """A workload on the device: constants, particles, locked grid, FEM tool and stepper."""
 
     def __init__(self, case: cases.Case, precision=_lib.FP64, device=0, rr2_min=None, wear=None, ctx=None,
-                 grid="green"):
+                 grid="green", max_cell=None):
+        """grid: "green" or "rothlin"; max_cell: an unlocked grid of that capacity (geometry from the
+        particles each step) instead of the case's locked grid."""
         self.case = case
         self.ctx = ctx or Context(device)
         rr2 = cases.default_rr2_min(case.dx) if rr2_min is None else rr2_min
         self.ctx.setup_constants(case.const, case.dt, rr2, case.wear if wear is None else wear)
         init = cases.initial_particles(case)
         self.p = particle_gpu(self.ctx, case.N, **{k: init[k] for k in ("pos", "vel", "rho", "T", "h", "fixed", "S")})
-        self.grid = GRIDS[grid](self.ctx, case.N, case.grid_bbmin, case.grid_bbmax, case.h)
+        if max_cell:
+            self.grid = GRIDS[grid].unlocked(self.ctx, max_cell, case.N)
+        else:
+            self.grid = GRIDS[grid](self.ctx, case.N, case.grid_bbmin, case.grid_bbmax, case.h)
         self.tool = tool_FEM(self.ctx, case) if case.tool is not None else None
         self.stepper = leap_frog(self.ctx, case.N, self.grid.num_cell,
                                  self.tool.n_nodes if self.tool else 0, precision)

@@ -6,6 +6,7 @@ finite differences, conservation laws, exactness on polynomial fields, and brute
 (numpy stable sorting, bisection roots).  Each check is chosen so that a plausible slip
 (a dropped factor, a sign, a transposed index, a wrong branch) fails it.
 """
+import ctypes as C
 import math
 
 import numpy as np
@@ -762,3 +763,54 @@ def test_rothlin_step_equals_green_step_without_tool():
         if f != "idx":
             assert np.array_equal(r.P[f], g.P[f]), f
     assert np.isfinite(g.P["pos"]).all()
+
+
+# ---------------------------------------------------------------- unlocked grid geometry
+def test_update_geometry_rule_and_capacity():
+    """grid.cu:88-127 restated in C gives the same geometry as the locked rule (cases.grid_geometry,
+    grid.cu:130-149) applied to the particles' bounding box and largest h; every particle hashes
+    into the grid; num_cell >= max_cell is refused."""
+    o = _moving_lattice()
+    rng = np.random.default_rng(12)
+    o.P["h"][:] = o.case.h * rng.uniform(0.9, 1.1, o.case.N)
+    P = o.P
+    want = cases.grid_geometry(P["pos"].min(axis=0), P["pos"].max(axis=0), P["h"].max())
+    lib = O.lib()
+    assert lib.o_update_geometry(C.byref(o.c), C.byref(o.Pc), 2.0, want["num_cell"] + 1) == 0
+    g = o.c.geom
+    assert (g.nx, g.ny, g.bbmin_x, g.bbmin_y, g.dx) == (want["nx"], want["ny"], want["bbmin_x"], want["bbmin_y"],
+                                                         want["dx"])
+    o.call("o_compute_hashes", "cp")
+    assert P["hash"].min() >= 0 and P["hash"].max() < g.nx * g.ny
+    # kernel width 4 doubles the cell
+    assert lib.o_update_geometry(C.byref(o.c), C.byref(o.Pc), 4.0, want["num_cell"] + 1) == 0
+    assert o.c.geom.dx == 2 * want["dx"]
+    assert lib.o_update_geometry(C.byref(o.c), C.byref(o.Pc), 2.0, want["num_cell"]) == -1
+
+
+def test_unlocked_step_equals_locked_step_on_the_same_box():
+    """A locked grid built on the particles' exact bounding box and h has the geometry the unlocked
+    grid computes at the first step, so the first step agrees bit for bit; afterwards the unlocked
+    grid is rebuilt at every step from the positions the step starts from."""
+    base = _moving_lattice().case
+    import dataclasses
+    case = dataclasses.replace(base, grid_bbmin=base.pos.min(axis=0), grid_bbmax=base.pos.max(axis=0))
+    for grid in ("green", "rothlin"):
+        lk = O.Oracle(case, grid=grid)
+        un = O.Oracle(case, grid=grid, max_cell=10 ** 6)
+        lk.step(1)
+        un.step(1)
+        assert (un.c.geom.nx, un.c.geom.ny, un.c.geom.dx) == (lk.c.geom.nx, lk.c.geom.ny, lk.c.geom.dx)
+        for f in lk.P:
+            assert np.array_equal(un.P[f], lk.P[f]), (grid, f)
+        un.step(9)
+        pos, h = un.P["pos"].copy(), un.P["h"].max()
+        un.step(1)
+        want = cases.grid_geometry(pos.min(axis=0), pos.max(axis=0), h)
+        g = un.c.geom
+        assert (g.nx, g.ny, g.bbmin_x, g.bbmin_y, g.dx) == (want["nx"], want["ny"], want["bbmin_x"],
+                                                             want["bbmin_y"], want["dx"])
+        assert (g.nx, g.ny) != (lk.c.geom.nx, lk.c.geom.ny) or g.bbmin_x != lk.c.geom.bbmin_x
+    small = O.Oracle(case, max_cell=50)
+    with pytest.raises(RuntimeError):
+        small.step(1)
