@@ -577,8 +577,11 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
                                                           const float *__restrict__ psum_T) {
     const int i = xcd_logical_block() * BS + threadIdx.x;
     if (i >= (int)B.N) return;
-    const double blanked0 = B.blanked[i];
-    const double tool = B.tool_particle[i];
+    // FP32C: the reorder wrote only the neighbour records; every per-particle field is gathered here
+    // from the unsorted arrays A through idx, and the sorted step result is written to B
+    const int src = F32C ? B.idx[i] : i;
+    const double blanked0 = F32C ? A.blanked[src] : B.blanked[i];
+    const double tool = F32C ? A.tool_particle[src] : B.tool_particle[i];
     const bool live = blanked0 != 1.;
     const double thermal_alpha = interaction_alpha(c, tool);
     // pair sums first: the per-particle state below is not live across the neighbour loop
@@ -596,18 +599,61 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
             r = pair_sums_fp64(c, B, i, thermal_alpha, cs, ce);
         }
     }
-    const double fixed = B.fixed[i];
-    // FP32C: the reorder did not write the predicted position and temperature (the pair sums read
-    // the FP32 records); the pre-step state is read through idx from the unsorted arrays A, and the
-    // corrected state is written in place into B (then every array is swapped)
-    const int src = F32C ? B.idx[i] : i;
-    sphfem_double2 pos = F32C ? A.pos[src] : B.pos[i], vel = B.vel[i];
-    sphfem_double4 S = B.S[i];
-    double rho = B.rho[i], T = F32C ? A.T[src] : B.T[i];
-    double eps_pl = B.eps_pl[i], eps_pl_dot = B.eps_pl_dot[i];
+    const double fixed = F32C ? A.fixed[src] : B.fixed[i];
+    // FP32C: pos and T are the pre-step values (the corrector starts from them); velocity, stress and
+    // density are predicted again from the pre-step state and rates (leap_frog.cu:45-69, the same
+    // operations as in the reorder), as is the pressure (actions_gpu.cu:222-232)
+    sphfem_double2 pos, vel;
+    sphfem_double4 S;
+    double rho, T, eps_pl, eps_pl_dot, pr = 0.;
+    sphfem_double2 fc0, ft0, nrm0;
+    int on_seg0;
+    sphfem_double2 v0 = {0., 0.};
+    sphfem_double4 S0 = {0., 0., 0., 0.};
+    double rho0 = 0.;
+    if constexpr (F32C) {
+        pos = A.pos[src];
+        v0 = A.vel[src];
+        S0 = A.S[src];
+        rho0 = A.rho[src];
+        T = A.T[src];
+        vel = v0;
+        S = S0;
+        rho = rho0;
+        if (live) {
+            const double hdt = 0.5 * c.dt;
+            const sphfem_double2 vt = A.vel_t[src];
+            const sphfem_double4 St = A.S_t[src];
+            const double rt = A.rho_t[src];
+            vel.x = vel.x + hdt * vt.x;
+            vel.y = vel.y + hdt * vt.y;
+            S.x = S.x + hdt * St.x;
+            S.y = S.y + hdt * St.y;
+            S.z = S.z + hdt * St.z;
+            S.w = S.w + hdt * St.w;
+            rho = rho + hdt * rt;
+        }
+        if (tool == 0.) pr = eos_pressure(c, rho);
+        eps_pl = A.eps_pl[src];
+        eps_pl_dot = A.eps_pl_dot[src];
+        fc0 = A.fc[src];
+        ft0 = A.ft[src];
+        nrm0 = A.n[src];
+        on_seg0 = A.on_seg[src];
+    } else {
+        pos = B.pos[i];
+        vel = B.vel[i];
+        S = B.S[i];
+        rho = B.rho[i];
+        T = B.T[i];
+        eps_pl = B.eps_pl[i];
+        eps_pl_dot = B.eps_pl_dot[i];
+        fc0 = B.fc[i];
+        ft0 = B.ft[i];
+        nrm0 = B.n[i];
+        on_seg0 = B.on_seg[i];
+    }
     double blanked = blanked0;
-    const sphfem_double2 fc0 = B.fc[i], ft0 = B.ft[i], nrm0 = B.n[i];
-    const int on_seg0 = B.on_seg[i];
     sphfem_double2 fc = fc0, ft = ft0, nrm = nrm0;
     int on_seg = on_seg0;
     bool contact_written = false;
@@ -618,7 +664,7 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
     if (live) {
         pos_t = r.pos_t;
         vel_t = r.vel_t;
-        T_t = thermal_alpha != 0. ? thermal_alpha * r.T_lapl : A.T_t[A.idx[i]];
+        T_t = thermal_alpha != 0. ? thermal_alpha * r.T_lapl : A.T_t[F32C ? src : A.idx[i]];
         if (tool == 0.) {
             S_t = jaumann_rate(c, r.v_der, S);        // actions_gpu.cu:280-314
             rho_t = -rho * (r.v_der.x + r.v_der.w);     // :626-639
@@ -634,9 +680,13 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
         }
         // leap_frog.cu:71-96
         const double dt = c.dt;
-        const sphfem_double2 p0 = F32C ? pos : pos_init[i], v0 = F32C ? A.vel[src] : vel_init[i];
-        const sphfem_double4 S0 = F32C ? A.S[src] : S_init[i];
-        const double rho0 = F32C ? A.rho[src] : rho_init[i], T0 = F32C ? T : T_init[i];
+        const sphfem_double2 p0 = F32C ? pos : pos_init[i];
+        if constexpr (!F32C) {
+            v0 = vel_init[i];
+            S0 = S_init[i];
+            rho0 = rho_init[i];
+        }
+        const double T0 = F32C ? T : T_init[i];
         pos.x = p0.x + dt * pos_t.x;
         pos.y = p0.y + dt * pos_t.y;
         vel.x = v0.x + dt * vel_t.x;
@@ -651,7 +701,7 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
             if (!fric_heat_v21(c, tv, *tv.kin, eps_pl, ft, on_seg, nrm, vel, fc, rho, T)) blanked = 1.;
         }
         if (c.plastic && tool == 0. && blanked != 1.) {
-            if (plasticity_jc(c, S, B.p[i], eps_pl, eps_pl_dot, T, rho)) blanked = 1.;
+            if (plasticity_jc(c, S, F32C ? pr : B.p[i], eps_pl, eps_pl_dot, T, rho)) blanked = 1.;
         }
         // contact s1 for particles outside the 1.05 ref_dist disc of the moved tool
         // (actions_gpu.cu:438-453); the others are finished by the near-tool pass
@@ -703,9 +753,20 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
     O.eps_pl[i] = eps_pl;
     O.eps_pl_dot[i] = eps_pl_dot;
     O.blanked[i] = blanked;
-    // contact fields in place, written only when they change (far from the tool they stay zero)
-    if (contact_written && (fc.x != fc0.x || fc.y != fc0.y || ft.x != ft0.x || ft.y != ft0.y || nrm.x != nrm0.x ||
-                            nrm.y != nrm0.y || on_seg != on_seg0)) {
+    if constexpr (F32C) {
+        // the sorted copy of every remaining field; p of tool particles stays at its array position,
+        // as the reference neither gathers p nor computes it for them
+        B.fc[i] = fc;
+        B.ft[i] = ft;
+        B.n[i] = nrm;
+        B.on_seg[i] = on_seg;
+        B.h[i] = A.h[src];
+        B.fixed[i] = fixed;
+        B.tool_particle[i] = tool;
+        B.p[i] = tool == 0. ? pr : A.p[i];
+    } else if (contact_written && (fc.x != fc0.x || fc.y != fc0.y || ft.x != ft0.x || ft.y != ft0.y ||
+                                   nrm.x != nrm0.x || nrm.y != nrm0.y || on_seg != on_seg0)) {
+        // contact fields in place, written only when they change (far from the tool they stay zero)
         B.fc[i] = fc;
         B.ft[i] = ft;
         B.n[i] = nrm;

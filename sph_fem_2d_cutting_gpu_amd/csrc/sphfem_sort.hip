@@ -740,25 +740,53 @@ void launch_rothlin_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p) {
     launch_cells_dense(ctx, p.hash, (int)p.N, g->num_cell, g->cell_start, g->cell_end);
 }
 
-// Fused: boundaries, gather (state and old derivatives), leap_frog init (leap_frog.cu:25-37),
-// predict (:45-69), EOS (actions_gpu.cu:222-232) and artificial stress (:234-278).
-// Reads the unsorted arrays p via idx, writes the sorted predicted state to b.
-__global__ void __launch_bounds__(256) k_reorder_predict(DevConsts c, const sphfem_particles p, sphfem_particles b,
-                                                         int *__restrict__ cell_start, int *__restrict__ cell_end,
-                                                         sphfem_double2 *__restrict__ pos_init,
-                                                         sphfem_double2 *__restrict__ vel_init,
-                                                         sphfem_double4 *__restrict__ S_init,
-                                                         double *__restrict__ rho_init, double *__restrict__ T_init,
-                                                         int num_cell, int *__restrict__ flags,
-                                                         float4 *__restrict__ rec) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int N = (int)p.N;
-    if (i >= N) return;
-    int h = p.hash[i];
-    const int s = p.idx[i];
+// FP32C neighbour records of one sorted particle: positions relative to the cell corner keep ~1e-7
+// of the cell size; the stress terms are the per-particle factors of the pair sum, (S - p)/rho^2 and
+// S_xy/rho^2; four SoA blocks; the first has 4 padding records that absorb four-wide scan loads
+__device__ __forceinline__ void write_records(const DevConsts &c, float4 *__restrict__ rec, int N, int i, int h,
+                                             sphfem_double2 pos, sphfem_double2 vel, sphfem_double4 S, double rho,
+                                             double T, double blanked, double tool, double hh, double pr,
+                                             sphfem_double4 R) {
+    const int ny = c.geom.ny;
+    const int ix = h / ny, iy = h - ix * ny;
+    const double ox = c.geom.bbmin_x + ix * c.geom.dx;
+    const double oy = c.geom.bbmin_y + iy * c.geom.dx;
+    const double rho21 = 1. / (rho * rho);
+    const float vol = (float)(c.phys.mass / rho);
+    float4 r0, r1, r2, r3;
+    r0.x = blanked == 1. ? 1e30f : (float)(pos.x - ox);     // blanked: never inside a support
+    r0.y = (float)(pos.y - oy);
+    r0.z = (float)iy;
+    r0.w = (float)hh;
+    r1.x = (float)vel.x;
+    r1.y = (float)vel.y;
+    r1.z = tool == 0. ? (float)((S.x - pr) * rho21) : 0.f;
+    r1.w = tool == 0. ? (float)(S.y * rho21) : 0.f;
+    r2.x = tool == 0. ? (float)((S.z - pr) * rho21) : 0.f;
+    r2.y = (float)R.x;
+    r2.z = (float)R.y;
+    r2.w = (float)R.z;
+    r3.x = tool == 1. ? -vol : vol;
+    r3.y = (float)rho;
+    r3.z = (float)sqrtf_d(c.phys.K / rho);
+    r3.w = (float)T;
+    const size_t n4 = (size_t)N + 4;
+    rec[i] = r0;
+    rec[n4 + i] = r1;
+    rec[n4 + (size_t)N + i] = r2;
+    rec[n4 + 2 * (size_t)N + i] = r3;
+}
+
+// cell boundaries of the sorted keys (grid_gpu_green.cu:13-51); false if the key or the source
+// index is out of range (flagged)
+__device__ __forceinline__ bool cell_bounds(const sphfem_particles &p, int i, int N, int num_cell,
+                                            int *__restrict__ cell_start, int *__restrict__ cell_end,
+                                            int *__restrict__ flags, int &h, int &s) {
+    h = p.hash[i];
+    s = p.idx[i];
     if ((unsigned)h >= (unsigned)num_cell || (unsigned)s >= (unsigned)N) {
         atomicOr(&flags[0], 4);
-        return;
+        return false;
     }
     int hp = __shfl_up(h, 1);
     if (__lane_id() == 0 && i > 0) hp = p.hash[i - 1];
@@ -767,26 +795,38 @@ __global__ void __launch_bounds__(256) k_reorder_predict(DevConsts c, const sphf
         if (i > 0) cell_end[hp] = i;
     }
     if (i == N - 1) cell_end[h] = i + 1;
+    return true;
+}
+
+// FP64 form, fused: boundaries, gather (state and old derivatives), leap_frog init
+// (leap_frog.cu:25-37), predict (:45-69), EOS (actions_gpu.cu:222-232) and artificial stress
+// (:234-278).  Reads the unsorted arrays p via idx, writes the sorted predicted state to b.
+__global__ void __launch_bounds__(256) k_reorder_predict(DevConsts c, const sphfem_particles p, sphfem_particles b,
+                                                         int *__restrict__ cell_start, int *__restrict__ cell_end,
+                                                         sphfem_double2 *__restrict__ pos_init,
+                                                         sphfem_double2 *__restrict__ vel_init,
+                                                         sphfem_double4 *__restrict__ S_init,
+                                                         double *__restrict__ rho_init, double *__restrict__ T_init,
+                                                         int num_cell, int *__restrict__ flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = (int)p.N;
+    if (i >= N) return;
+    int h, s;
+    if (!cell_bounds(p, i, N, num_cell, cell_start, cell_end, flags, h, s)) return;
     sphfem_double2 pos = p.pos[s], vel = p.vel[s];
     sphfem_double4 S = p.S[s];
     double rho = p.rho[s], T = p.T[s];
-    double blanked = p.blanked[s], tool = p.tool_particle[s];
-    // FP32C (rec != null): the update kernel reads the pre-step state through idx, and the pair
-    // sums read position and temperature from the records, so the init copies and the FP64
-    // predicted position, temperature and artificial stress are not written
-    const bool f32c = rec != nullptr;
+    const double blanked = p.blanked[s], tool = p.tool_particle[s];
     if (blanked != 1.) {
         const double dt = c.dt;
-        if (!f32c) {
-            pos_init[i] = pos;
-            vel_init[i] = vel;
-            S_init[i] = S;
-            rho_init[i] = rho;
-            T_init[i] = T;
-        }
-        sphfem_double2 pt = p.pos_t[s], vt = p.vel_t[s];
-        sphfem_double4 St = p.S_t[s];
-        double rt = p.rho_t[s], Tt = p.T_t[s];
+        pos_init[i] = pos;
+        vel_init[i] = vel;
+        S_init[i] = S;
+        rho_init[i] = rho;
+        T_init[i] = T;
+        const sphfem_double2 pt = p.pos_t[s], vt = p.vel_t[s];
+        const sphfem_double4 St = p.S_t[s];
+        const double rt = p.rho_t[s], Tt = p.T_t[s];
         pos.x = pos.x + 0.5 * dt * pt.x;
         pos.y = pos.y + 0.5 * dt * pt.y;
         vel.x = vel.x + 0.5 * dt * vt.x;
@@ -798,72 +838,86 @@ __global__ void __launch_bounds__(256) k_reorder_predict(DevConsts c, const sphf
         rho = rho + 0.5 * dt * rt;
         T = T + 0.5 * dt * Tt;
     }
-    if (!f32c) b.pos[i] = pos;
+    b.pos[i] = pos;
     b.vel[i] = vel;
     b.S[i] = S;
     b.rho[i] = rho;
-    if (!f32c) b.T[i] = T;
+    b.T[i] = T;
     b.blanked[i] = blanked;
     b.tool_particle[i] = tool;
     b.fc[i] = p.fc[s];
     b.ft[i] = p.ft[s];
-    const double hh = p.h[s];
-    b.h[i] = hh;
+    b.h[i] = p.h[s];
     b.n[i] = p.n[s];
     b.eps_pl[i] = p.eps_pl[s];
     b.eps_pl_dot[i] = p.eps_pl_dot[s];
     b.fixed[i] = p.fixed[s];
     b.on_seg[i] = p.on_seg[s];
+    if (tool == 0.) {
+        const double pr = eos_pressure(c, rho);
+        b.p[i] = pr;
+        b.R[i] = artificial_stress(c, rho, pr, S);
+    } else {
+        // neither gathered nor computed for tool particles by the reference: they stay in place
+        b.p[i] = p.p[i];
+        b.R[i] = p.R[i];
+    }
+}
+
+// FP32C form: boundaries, predict, EOS and artificial stress of the sorted particle, written only
+// as neighbour records.  The update kernel gathers the pre-step state through idx itself (and
+// repeats the predictor for velocity, stress and density), so no sorted copy of the state is
+// written here.
+__global__ void __launch_bounds__(256) k_reorder_records(DevConsts c, const sphfem_particles p,
+                                                         int *__restrict__ cell_start, int *__restrict__ cell_end,
+                                                         int num_cell, int *__restrict__ flags,
+                                                         float4 *__restrict__ rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = (int)p.N;
+    if (i >= N) return;
+    int h, s;
+    if (!cell_bounds(p, i, N, num_cell, cell_start, cell_end, flags, h, s)) return;
+    sphfem_double2 pos = p.pos[s], vel = p.vel[s];
+    sphfem_double4 S = p.S[s];
+    double rho = p.rho[s], T = p.T[s];
+    const double blanked = p.blanked[s], tool = p.tool_particle[s];
+    if (blanked != 1.) {
+        const double dt = c.dt;
+        const sphfem_double2 pt = p.pos_t[s], vt = p.vel_t[s];
+        const sphfem_double4 St = p.S_t[s];
+        const double rt = p.rho_t[s], Tt = p.T_t[s];
+        pos.x = pos.x + 0.5 * dt * pt.x;
+        pos.y = pos.y + 0.5 * dt * pt.y;
+        vel.x = vel.x + 0.5 * dt * vt.x;
+        vel.y = vel.y + 0.5 * dt * vt.y;
+        S.x = S.x + 0.5 * dt * St.x;
+        S.y = S.y + 0.5 * dt * St.y;
+        S.z = S.z + 0.5 * dt * St.z;
+        S.w = S.w + 0.5 * dt * St.w;
+        rho = rho + 0.5 * dt * rt;
+        T = T + 0.5 * dt * Tt;
+    }
     sphfem_double4 R = {0., 0., 0., 0.};
     double pr = 0.;
     if (tool == 0.) {
         pr = eos_pressure(c, rho);
         R = artificial_stress(c, rho, pr, S);
-        b.p[i] = pr;
-        if (!f32c) b.R[i] = R;
     }
-    if (rec) {
-        // FP32C neighbour records: positions relative to the cell corner keep ~1e-7 of the cell size;
-        // the stress terms are the per-particle factors of the pair sum, (S - p)/rho^2 and S_xy/rho^2;
-        // four SoA blocks; the first has 4 padding records that absorb four-wide scan loads
-        const int ny = c.geom.ny;
-        const int ix = h / ny, iy = h - ix * ny;
-        const double ox = c.geom.bbmin_x + ix * c.geom.dx;
-        const double oy = c.geom.bbmin_y + iy * c.geom.dx;
-        const double rho21 = 1. / (rho * rho);
-        const float vol = (float)(c.phys.mass / rho);
-        float4 r0, r1, r2, r3;
-        r0.x = blanked == 1. ? 1e30f : (float)(pos.x - ox);     // blanked: never inside a support
-        r0.y = (float)(pos.y - oy);
-        r0.z = (float)iy;
-        r0.w = (float)hh;
-        r1.x = (float)vel.x;
-        r1.y = (float)vel.y;
-        r1.z = tool == 0. ? (float)((S.x - pr) * rho21) : 0.f;
-        r1.w = tool == 0. ? (float)(S.y * rho21) : 0.f;
-        r2.x = tool == 0. ? (float)((S.z - pr) * rho21) : 0.f;
-        r2.y = (float)R.x;
-        r2.z = (float)R.y;
-        r2.w = (float)R.z;
-        r3.x = tool == 1. ? -vol : vol;
-        r3.y = (float)rho;
-        r3.z = (float)sqrtf_d(c.phys.K / rho);
-        r3.w = (float)T;
-        const size_t n4 = (size_t)N + 4;
-        rec[i] = r0;
-        rec[n4 + i] = r1;
-        rec[n4 + (size_t)N + i] = r2;
-        rec[n4 + 2 * (size_t)N + i] = r3;
-    }
+    write_records(c, rec, N, i, h, pos, vel, S, rho, T, blanked, tool, p.h[s], pr, R);
 }
 
 void launch_reorder_predict(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, sphfem_leap_frog *lf) {
+    if (lf->precision == SPHFEM_FP32C) {
+        KTimer t(ctx, "reorder_records");
+        k_reorder_records<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, g->cell_start, g->cell_end,
+                                                                   g->num_cell, ctx->d_flags, lf->rec);
+        return;
+    }
     KTimer t(ctx, "reorder_predict");
     k_reorder_predict<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, g->buf, g->cell_start, g->cell_end,
                                                                lf->pos_init, lf->vel_init, lf->S_init,
                                                                lf->rho_init, lf->T_init, g->num_cell,
-                                                               ctx->d_flags,
-                                                               lf->precision == SPHFEM_FP32C ? lf->rec : nullptr);
+                                                               ctx->d_flags);
 }
 
 }  // namespace sphfem
