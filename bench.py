@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-INTERACTION_KERNEL = {"fp64": "fused_update", "fp32c": "pairs_fp32c"}
+INTERACTION_KERNEL = {"fp64": "pairs_fp64", "fp32c": "pairs_fp32c"}
 
 
 def interaction_bytes(N, n_occ_cells, w):
@@ -37,7 +37,8 @@ def measured_traffic(kernel, workload_key):
         return None
     with open(path) as f:
         t = json.load(f)
-    prof_name = {"pairs_fp32c": "sphfem::k_pairs_fp32c", "fused_update": "sphfem::k_fused_update<false>",
+    prof_name = {"pairs_fp32c": "sphfem::k_pairs_fp32c", "pairs_fp64": "sphfem::k_pairs_fp64",
+                 "fused_update": "sphfem::k_fused_update<false>",
                  "update_fp32c": "sphfem::k_fused_update<true>"}.get(kernel)
     v = t.get("per_launch", {}).get(prof_name)
     return None if v is None else v["read_bytes"] + v["write_bytes"]

@@ -233,8 +233,8 @@ int sphfem_leap_frog_init(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particle
 int sphfem_leap_frog_predict(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, double dt);
 int sphfem_leap_frog_correct(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, double dt);
 /* Launch parameters of the step kernels, chosen by the autotuner (sph_fem_2d_cutting_gpu_amd/
- * autotune.py): FP32C pair kernel block size (128 or 256) and neighbour-list chunk (16, 32 or
- * 48 entries per lane), update kernel block size (128 or 256).  Other values are rejected.
+ * autotune.py): pair kernel block size (128 or 256, both forms), FP32C neighbour-list chunk (16,
+ * 32 or 48 entries per lane), update kernel block size (128 or 256).  Other values are rejected.
  * Results do not depend on them beyond the FP32 summation blocks of the FP32C form.  No
  * counterpart in the reference (fixed launch shapes there). */
 typedef struct { int pair_block, pair_chunk, update_block; } sphfem_tuning;

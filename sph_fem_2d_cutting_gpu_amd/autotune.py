@@ -1,7 +1,7 @@
 """Autotuner for the step's launch parameters and the batch's graph depth.
 
 Tunables (all result-neutral except the FP32 summation blocks of the FP32C form):
-  * pair_block   -- FP32C pair kernel workgroup size (128, 256)
+  * pair_block   -- pair kernel workgroup size (128, 256), both forms
   * pair_chunk   -- FP32C neighbour-list entries per lane in LDS (16, 32, 48): larger lists run
                     fewer list flushes in dense regions, smaller ones leave LDS for more waves
   * update_block -- update kernel workgroup size (128, 256)
@@ -29,7 +29,7 @@ def candidates(precision):
     if precision == _lib.FP32C:
         return [dict(pair_block=b, pair_chunk=c, update_block=u)
                 for b, c, u in itertools.product(PAIR_BLOCKS, PAIR_CHUNKS, UPDATE_BLOCKS)]
-    return [dict(pair_block=256, pair_chunk=32, update_block=u) for u in UPDATE_BLOCKS]
+    return [dict(pair_block=b, pair_chunk=32, update_block=u) for b, u in itertools.product(PAIR_BLOCKS, UPDATE_BLOCKS)]
 
 
 def _time_steps(sim, steps):

@@ -958,6 +958,9 @@ int sphfem_leap_frog_create(sphfem_ctx *ctx, unsigned int num_part, unsigned int
         lf->S_init = dalloc<sphfem_double4>(num_part);
         lf->rho_init = dalloc<double>(num_part);
         lf->T_init = dalloc<double>(num_part);
+        lf->pre_vol = dalloc<double>(num_part);
+        lf->pre_s = dalloc<sphfem_double4>(num_part);
+        lf->psum64 = dalloc<sphfem::PairSums>(num_part);
         lf->cell_start = dalloc<int>(std::max(num_cell, 1u));
         lf->cell_end = dalloc<int>(std::max(num_cell, 1u));
         lf->near_list = dalloc<int>(num_part);
@@ -973,7 +976,7 @@ int sphfem_leap_frog_destroy(sphfem_ctx *ctx, sphfem_leap_frog *lf) {
         SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         dfree(lf->pos_init); dfree(lf->vel_init); dfree(lf->S_init); dfree(lf->rho_init); dfree(lf->T_init);
         dfree(lf->cell_start); dfree(lf->cell_end); dfree(lf->near_list); dfree(lf->contact_list);
-        dfree(lf->rec); dfree(lf->psum); dfree(lf->psum_T);
+        dfree(lf->rec); dfree(lf->psum); dfree(lf->psum_T); dfree(lf->pre_vol); dfree(lf->pre_s); dfree(lf->psum64);
         delete lf;
         return SPHFEM_OK;
     });

@@ -44,6 +44,13 @@ typedef sphfem_tool_kinematics ToolKin;
 // Views of the particle arrays passed to kernels.
 typedef sphfem_particles Particles;
 
+// FP64 pair sums of one particle (interactions_gpu.cu:143-379 outputs, T as the Laplacian)
+struct PairSums {
+    sphfem_double4 v_der, S_der;
+    sphfem_double2 vel_t, pos_t;
+    double T_lapl;
+};
+
 struct Timing {
     bool enabled = false;
     std::string only;     // comma-separated kernel names to time (empty: all)
@@ -141,6 +148,12 @@ struct sphfem_leap_frog {
     sphfem_double4 *S_init = nullptr;
     double *rho_init = nullptr, *T_init = nullptr;
     int *cell_start = nullptr, *cell_end = nullptr;   // only used by the unfused path
+    // FP64 form: factors of each sorted particle's neighbour terms that depend on that particle
+    // alone, computed once by the reorder with the operations the pair loop would repeat per pair:
+    // pre_vol = m/rho, pre_s = {(Sxx-p)/rho^2, Sxy/rho^2, (Syy-p)/rho^2, sqrtf(K/rho)}
+    double *pre_vol = nullptr;
+    sphfem_double4 *pre_s = nullptr;
+    sphfem::PairSums *psum64 = nullptr;   // FP64 form: pair sums of the sorted particles
     int *near_list = nullptr;      // [num_part] particles near the tool
     int *contact_list = nullptr;   // [num_part] particles in contact (+ flag bit 30 = invalid)
     bool intermediates = false;    // also write S_der / v_der in the fused step

@@ -15,11 +15,6 @@ namespace sphfem {
 // rows [low_j, high_j) are one contiguous run of sorted indices, so each column is one loop.
 // Pairs outside the kernel support contribute exact zeros in the reference and are skipped.
 // ---------------------------------------------------------------------------------------------
-struct PairSums {
-    sphfem_double4 v_der, S_der;
-    sphfem_double2 vel_t, pos_t;
-    double T_lapl;
-};
 
 __device__ __forceinline__ void column_range(const int *__restrict__ cs, const int *__restrict__ ce, int col0,
                                              int low_j, int high_j, int &first, int &last) {
@@ -35,9 +30,15 @@ __device__ __forceinline__ void column_range(const int *__restrict__ cs, const i
     }
 }
 
+// PRE: the neighbour factors m/rho_j, (S_j - p_j)/rho_j^2, S_xy,j/rho_j^2 and sqrtf(K/rho_j) come
+// from pre_vol/pre_s (computed per particle by the reorder with the same operations, so the sums
+// are bit-identical); otherwise they are computed per pair as in the reference.
+template <bool PRE>
 __device__ __forceinline__ PairSums pair_sums_fp64(const DevConsts &c, const sphfem_particles &P, int pidx,
                                                    double thermal_alpha, const int *__restrict__ cs,
-                                                   const int *__restrict__ ce) {
+                                                   const int *__restrict__ ce,
+                                                   const double *__restrict__ pre_vol = nullptr,
+                                                   const sphfem_double4 *__restrict__ pre_s = nullptr) {
     const double mass = c.phys.mass, K = c.phys.K;
     const double wdeltap = c.corr.wdeltap, alpha = c.corr.alpha, beta = c.corr.beta;
     const double eta = c.corr.eta, eps = c.corr.xspheps;
@@ -51,6 +52,9 @@ __device__ __forceinline__ PairSums pair_sums_fp64(const DevConsts &c, const sph
     if (thermal_alpha != 0.) Ti = P.T[pidx];
     const double rhoi21 = 1. / (rhoi * rhoi);
     const double fourh2 = 4 * hi * hi;
+    const bool unit_wdp = wdeltap == 1.;      // w / 1 == w exactly
+    double ci_pre = 0.;
+    if constexpr (PRE) ci_pre = pre_s[pidx].w;
     const int hashi = P.hash[pidx];
     const int gi = hashi / ny;
     const int gj = hashi - gi * ny;
@@ -81,31 +85,51 @@ __device__ __forceinline__ PairSums pair_sums_fp64(const DevConsts &c, const sph
             const double is_tool_j = P.tool_particle[iter];
             if (!(is_tool_i == 1.) && !(is_tool_j == 1.)) {
                 const sphfem_double2 vj = P.vel[iter];
-                const sphfem_double4 Sj = P.S[iter], Rj = P.R[iter];
-                const double hj = P.h[iter], prsj = P.p[iter];
-                double volj = mass / rhoj;
-                double rhoj21 = 1. / (rhoj * rhoj);
+                const sphfem_double4 Rj = P.R[iter];
+                const double hj = P.h[iter];
+                double volj, sjx, sjy, sjz;
+                if constexpr (PRE) {
+                    volj = pre_vol[iter];
+                    const sphfem_double4 f = pre_s[iter];
+                    sjx = f.x;
+                    sjy = f.y;
+                    sjz = f.z;
+                } else {
+                    const sphfem_double4 Sj = P.S[iter];
+                    const double prsj = P.p[iter];
+                    volj = mass / rhoj;
+                    const double rhoj21 = 1. / (rhoj * rhoj);
+                    sjx = (Sj.x - prsj) * rhoj21;
+                    sjy = Sj.y * rhoj21;
+                    sjz = (Sj.z - prsj) * rhoj21;
+                }
                 r.v_der.x += (vj.x - vi.x) * w_x * volj;
                 r.v_der.y += (vj.x - vi.x) * w_y * volj;
                 r.v_der.z += (vj.y - vi.y) * w_x * volj;
                 r.v_der.w += (vj.y - vi.y) * w_y * volj;
-                double fab = w / wdeltap;
+                double fab = unit_wdp ? w : w / wdeltap;
                 fab *= fab;
                 fab *= fab;
                 double Rx = fab * (Ri.x + Rj.x);
                 double Ry = fab * (Ri.y + Rj.y);
                 double Rz = fab * (Ri.z + Rj.z);
-                r.S_der.x += mass * ((Si.x - prsi) * rhoi21 + (Sj.x - prsj) * rhoj21 + Rx) * w_x;
-                r.S_der.y += mass * (Si.y * rhoi21 + Sj.y * rhoj21 + Ry) * w_y;
-                r.S_der.z += mass * (Si.y * rhoi21 + Sj.y * rhoj21 + Ry) * w_x;
-                r.S_der.w += mass * ((Si.z - prsi) * rhoi21 + (Sj.z - prsj) * rhoj21 + Rz) * w_y;
+                r.S_der.x += mass * ((Si.x - prsi) * rhoi21 + sjx + Rx) * w_x;
+                r.S_der.y += mass * (Si.y * rhoi21 + sjy + Ry) * w_y;
+                r.S_der.z += mass * (Si.y * rhoi21 + sjy + Ry) * w_x;
+                r.S_der.w += mass * ((Si.z - prsi) * rhoi21 + sjz + Rz) * w_y;
                 double vijx = vi.x - vj.x;
                 double vijy = vi.y - vj.y;
                 double vijposij = vijx * xij + vijy * yij;
                 double rhoij = 0.5 * (rhoi + rhoj);
                 if (vijposij < 0.) {
-                    double ci = sqrtf_d(K / rhoi);
-                    double cj = sqrtf_d(K / rhoj);
+                    double ci, cj;
+                    if constexpr (PRE) {
+                        ci = ci_pre;
+                        cj = pre_s[iter].w;
+                    } else {
+                        ci = sqrtf_d(K / rhoi);
+                        cj = sqrtf_d(K / rhoj);
+                    }
                     double cij = 0.5 * (ci + cj);
                     double hij = 0.5 * (hi + hj);
                     double r2ij = xij * xij + yij * yij;
@@ -124,7 +148,8 @@ __device__ __forceinline__ PairSums pair_sums_fp64(const DevConsts &c, const sph
                     double eijx = xij / rij;
                     double eijy = yij / rij;
                     double rij1 = 1. / rij;
-                    r.T_lapl += 2.0 * (mass / rhoj) * (Ti - Tj) * rij1 * (eijx * w_x + eijy * w_y);
+                    const double volT = PRE ? pre_vol[iter] : mass / rhoj;
+                    r.T_lapl += 2.0 * volT * (Ti - Tj) * rij1 * (eijx * w_x + eijy * w_y);
                 }
             }
         }
@@ -375,7 +400,7 @@ __global__ void __launch_bounds__(256) k_interactions_ref(DevConsts c, sphfem_pa
     if (pidx >= (int)P.N) return;
     if (P.blanked[pidx] == 1.) return;
     double thermal_alpha = interaction_alpha(c, P.tool_particle[pidx]);
-    PairSums r = pair_sums_fp64(c, P, pidx, thermal_alpha, cs, ce);
+    PairSums r = pair_sums_fp64<false>(c, P, pidx, thermal_alpha, cs, ce);
     P.S_der[pidx] = r.S_der;
     P.v_der[pidx] = r.v_der;
     P.pos_t[pidx] = r.pos_t;
@@ -564,6 +589,21 @@ void launch_leapfrog(sphfem_ctx *ctx, int which, const sphfem_particles &p, sphf
 // is dead after the reorder except the previous derivatives.  Outputs: corrected state into
 // A.pos/vel/S/rho/T/eps_pl/eps_pl_dot/blanked, derivatives into B.pos_t/vel_t/rho_t/S_t/T_t,
 // contact fields in place in B.fc/ft/n/on_seg; S_der/v_der into A when requested.
+// FP64 pair sums of the sorted predicted state (interactions_gpu.cu:143-379) into psum64, for the
+// update kernel; a kernel of its own so that the pair loop keeps 3 waves per SIMD instead of the
+// update's 2
+template <int BS>
+__global__ void __launch_bounds__(BS) k_pairs_fp64(DevConsts c, sphfem_particles B, const int *__restrict__ cs,
+                                                   const int *__restrict__ ce, const double *__restrict__ pre_vol,
+                                                   const sphfem_double4 *__restrict__ pre_s,
+                                                   PairSums *__restrict__ psum64) {
+    const int i = xcd_logical_block() * BS + threadIdx.x;
+    if (i >= (int)B.N) return;
+    if (B.blanked[i] == 1.) return;
+    const double thermal_alpha = interaction_alpha(c, B.tool_particle[i]);
+    psum64[i] = pair_sums_fp64<true>(c, B, i, thermal_alpha, cs, ce, pre_vol, pre_s);
+}
+
 template <bool F32C, int BS>
 __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particles B, sphfem_particles A,
                                                           const sphfem_double2 *__restrict__ pos_init,
@@ -574,7 +614,8 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
                                                           const int *__restrict__ cs, const int *__restrict__ ce,
                                                           ToolView tv, int write_intermediates,
                                                           const float4 *__restrict__ psum,
-                                                          const float *__restrict__ psum_T) {
+                                                          const float *__restrict__ psum_T,
+                                                          const PairSums *__restrict__ psum64) {
     const int i = xcd_logical_block() * BS + threadIdx.x;
     if (i >= (int)B.N) return;
     // FP32C: the reorder wrote only the neighbour records; every per-particle field is gathered here
@@ -596,7 +637,7 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
             r.pos_t = {a.z, a.w};
             r.T_lapl = psum_T[i];
         } else {
-            r = pair_sums_fp64(c, B, i, thermal_alpha, cs, ce);
+            r = psum64[i];
         }
     }
     const double fixed = F32C ? A.fixed[src] : B.fixed[i];
@@ -810,21 +851,30 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
         if (lf->tuning.update_block == 128)
             k_fused_update<true, 128><<<nblk(B.N, 128), 128, 0, ctx->stream>>>(
                 ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
-                lf->psum, lf->psum_T);
+                lf->psum, lf->psum_T, nullptr);
         else
             k_fused_update<true, 256><<<nblk(B.N, 256), 256, 0, ctx->stream>>>(
                 ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
-                lf->psum, lf->psum_T);
+                lf->psum, lf->psum_T, nullptr);
     } else {
+        {
+            KTimer tp(ctx, "pairs_fp64");
+            if (lf->tuning.pair_block == 128)
+                k_pairs_fp64<128><<<nblk(B.N, 128), 128, 0, ctx->stream>>>(ctx->c, B, cs, ce, lf->pre_vol, lf->pre_s,
+                                                                            lf->psum64);
+            else
+                k_pairs_fp64<256><<<nblk(B.N, 256), 256, 0, ctx->stream>>>(ctx->c, B, cs, ce, lf->pre_vol, lf->pre_s,
+                                                                            lf->psum64);
+        }
         KTimer tm(ctx, "fused_update");
         if (lf->tuning.update_block == 128)
             k_fused_update<false, 128><<<nblk(B.N, 128), 128, 0, ctx->stream>>>(
                 ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
-                nullptr, nullptr);
+                nullptr, nullptr, lf->psum64);
         else
             k_fused_update<false, 256><<<nblk(B.N, 256), 256, 0, ctx->stream>>>(
                 ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
-                nullptr, nullptr);
+                nullptr, nullptr, lf->psum64);
     }
 }
 

@@ -807,7 +807,9 @@ __global__ void __launch_bounds__(256) k_reorder_predict(DevConsts c, const sphf
                                                          sphfem_double2 *__restrict__ vel_init,
                                                          sphfem_double4 *__restrict__ S_init,
                                                          double *__restrict__ rho_init, double *__restrict__ T_init,
-                                                         int num_cell, int *__restrict__ flags) {
+                                                         int num_cell, int *__restrict__ flags,
+                                                         double *__restrict__ pre_vol,
+                                                         sphfem_double4 *__restrict__ pre_s) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int N = (int)p.N;
     if (i >= N) return;
@@ -853,14 +855,19 @@ __global__ void __launch_bounds__(256) k_reorder_predict(DevConsts c, const sphf
     b.eps_pl_dot[i] = p.eps_pl_dot[s];
     b.fixed[i] = p.fixed[s];
     b.on_seg[i] = p.on_seg[s];
+    // the neighbour factors of interactions_gpu.cu:247-252, 283-285 (m/rho_j in the heat term, :352)
+    pre_vol[i] = c.phys.mass / rho;
     if (tool == 0.) {
         const double pr = eos_pressure(c, rho);
         b.p[i] = pr;
         b.R[i] = artificial_stress(c, rho, pr, S);
+        const double rho21 = 1. / (rho * rho);
+        pre_s[i] = {(S.x - pr) * rho21, S.y * rho21, (S.z - pr) * rho21, sqrtf_d(c.phys.K / rho)};
     } else {
         // neither gathered nor computed for tool particles by the reference: they stay in place
         b.p[i] = p.p[i];
         b.R[i] = p.R[i];
+        pre_s[i] = {0., 0., 0., 0.};   // the pair loop skips these terms for tool particles
     }
 }
 
@@ -917,7 +924,7 @@ void launch_reorder_predict(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_partic
     k_reorder_predict<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, g->buf, g->cell_start, g->cell_end,
                                                                lf->pos_init, lf->vel_init, lf->S_init,
                                                                lf->rho_init, lf->T_init, g->num_cell,
-                                                               ctx->d_flags);
+                                                               ctx->d_flags, lf->pre_vol, lf->pre_s);
 }
 
 }  // namespace sphfem

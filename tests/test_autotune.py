@@ -5,8 +5,9 @@ from sph_fem_2d_cutting_gpu_amd import _lib, autotune
 def test_candidate_sets():
     c32 = autotune.candidates(_lib.FP32C)
     assert len(c32) == 12 and {"pair_block": 256, "pair_chunk": 32, "update_block": 256} in c32
-    c64 = autotune.candidates(_lib.FP64)
-    assert all(c["pair_block"] == 256 and c["pair_chunk"] == 32 for c in c64)
+    c64 = autotune.candidates(_lib.FP64)     # the list chunk is an FP32C parameter only
+    assert len(c64) == 4 and all(c["pair_chunk"] == 32 for c in c64)
+    assert {c["pair_block"] for c in c64} == {128, 256}
 
 
 def test_cache_roundtrip(tmp_path):
