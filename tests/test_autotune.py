@@ -5,6 +5,7 @@ from sph_fem_2d_cutting_gpu_amd import _lib, autotune
 def test_candidate_sets():
     c32 = autotune.candidates(_lib.FP32C)
     assert len(c32) == 12 and {"pair_block": 256, "pair_chunk": 32, "update_block": 256} in c32
+    assert {c["pair_chunk"] for c in c32} == {16, 32, 48}
     c64 = autotune.candidates(_lib.FP64)     # the list chunk is an FP32C parameter only
     assert len(c64) == 4 and all(c["pair_chunk"] == 32 for c in c64)
     assert {c["pair_block"] for c in c64} == {128, 256}

@@ -49,7 +49,7 @@ def test_tune_simulation_applies_a_candidate():
     res = autotune.tune_simulation(sim, _lib.FP32C, steps=2, repeats=1)
     assert res["best"] in autotune.candidates(_lib.FP32C)
     assert sim.get_tuning() == res["best"]
-    assert len(res["timings"]) == 12
+    assert len(res["timings"]) == len(autotune.candidates(_lib.FP32C))
     sim.close()
 
 
