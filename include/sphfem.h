@@ -224,6 +224,70 @@ int sphfem_tool_upload(sphfem_ctx *ctx, sphfem_tool *t, const sphfem_double2 *po
                        const double *wear_nodes);
 int sphfem_tool_set_kinematics(sphfem_ctx *ctx, sphfem_tool *t, const sphfem_tool_kinematics *kin);
 
+/* ---- rigid (SPH) tool: tool.h:31-141, tool_gpu.h:19-28, tool_gpu.cuh:23-45 */
+#define SPHFEM_TOOL_MAX_SEG 5
+/* geometric_primitives.h:15-57: y = a x + b, or x = b when vertical (a = DBL_MAX) */
+typedef struct { double a, b; int vertical; } sphfem_line;
+/* left/right ends, line, normal normalize(dy, -dx) of right - left (pointing into the tool) */
+typedef struct { sphfem_double2 left, right; sphfem_line l; sphfem_double2 n; } sphfem_segment;
+/* fillet of radius r about p between the angles t1 and t2 (radians in (0, 2 pi]) */
+typedef struct { double r, t1, t2; sphfem_double2 p; } sphfem_circle_segment;
+/* the host state of class tool (tool.h:109-138): a closed polygon of 4 segments, or 5 when a
+ * fillet replaces the third one (has_fillet), velocity, friction coefficient and front */
+typedef struct {
+    int num_seg, has_fillet;
+    sphfem_segment seg[SPHFEM_TOOL_MAX_SEG];
+    sphfem_circle_segment fillet;
+    sphfem_double2 vel;
+    double mu, front;
+} sphfem_tool_geometry;
+/* bbox (geometric_primitives.h:59-72) */
+typedef struct { double bbmin_x, bbmax_x, bbmin_y, bbmax_y; } sphfem_bbox;
+
+/* Host-only geometry (no device work, no context): the constructors of tool.cpp:316-406,
+ * tool(tl, tr, br, bl[, r], mu) and tool(tl, length, height, rake, clearance[, r], mu) with
+ * angles in degrees; r = 0 builds the sharp tool.  Velocity starts at zero. */
+int sphfem_tool_geometry_from_points(sphfem_double2 tl, sphfem_double2 tr, sphfem_double2 br, sphfem_double2 bl,
+                                     double r, double mu, sphfem_tool_geometry *out);
+int sphfem_tool_geometry_from_angles(sphfem_double2 tl, double length, double height, double rake_angle,
+                                     double clearance_angle, double r, double mu, sphfem_tool_geometry *out);
+/* tool::update_tool (tool.cpp:202-212): ends and fillet centre move by dt * vel, lines rebuilt */
+int sphfem_tool_geometry_move(sphfem_tool_geometry *g, double dt);
+/* tool::low (tool.cpp:178-188), tool::safe_bb (:147-176), tool::inside (:95-110) */
+int sphfem_tool_geometry_low(const sphfem_tool_geometry *g, double *low);
+int sphfem_tool_geometry_safe_bb(const sphfem_tool_geometry *g, double safety, sphfem_bbox *out);
+int sphfem_tool_geometry_inside(const sphfem_tool_geometry *g, sphfem_double2 pos, int *inside);
+/* tool::sample_tool (tool.cpp:112-135): lattice points of spacing dx inside the tool, column by
+ * column.  Writes at most cap points to out (may be NULL) and the full count to *count. */
+int sphfem_tool_geometry_sample(const sphfem_tool_geometry *g, double dx, sphfem_double2 *out, long cap,
+                                long *count);
+
+typedef struct sphfem_rigid_tool sphfem_rigid_tool;
+/* tool_gpu_set_up_tool (tool_gpu.cu:219-266): the device copy of the geometry, velocity and mu,
+ * with contact_alpha and slave_mass = phys->mass */
+int sphfem_tool_gpu_set_up_tool(sphfem_ctx *ctx, const sphfem_tool_geometry *g, double contact_alpha,
+                                const sphfem_phys_constants *phys, sphfem_rigid_tool **out);
+int sphfem_rigid_tool_destroy(sphfem_ctx *ctx, sphfem_rigid_tool *rt);
+/* the device geometry as it stands (blocks until the stream is idle) */
+int sphfem_rigid_tool_get_geometry(sphfem_ctx *ctx, sphfem_rigid_tool *rt, sphfem_tool_geometry *out);
+/* tool::set_vel (tool.cpp:215-217) on the device copy */
+int sphfem_rigid_tool_set_vel(sphfem_ctx *ctx, sphfem_rigid_tool *rt, sphfem_double2 vel);
+/* tool_gpu_update_tool (tool_gpu.cu:270-311): the segments and fillet move by global_dt * vel on
+ * the device (the reference moves its host copy and uploads it) and the tool particles of p by
+ * global_dt * vel (update_tool_particles, :205-216).  p may be NULL (geometry only). */
+int sphfem_tool_gpu_update_tool(sphfem_ctx *ctx, sphfem_rigid_tool *rt, sphfem_particles *p);
+/* compute_contact_forces (tool_gpu.cu:82-202, 314-319): penalty contact and LS-DYNA-style
+ * Coulomb friction against the polygon and fillet, for every non-tool particle */
+int sphfem_compute_contact_forces(sphfem_ctx *ctx, sphfem_particles *p, sphfem_rigid_tool *rt);
+/* material_fric_heat_gen (actions_gpu.cu:316-343, 956-965): T += eta dt |ft| |v_rel| / (cp m)
+ * for non-tool particles; a no-op unless the workpiece's eta is non-zero */
+int sphfem_material_fric_heat_gen(sphfem_ctx *ctx, sphfem_particles *p, sphfem_double2 vel);
+/* interactions_setup_thermal_constants_tool (interactions_gpu.cu:494-502): the thermal block of
+ * tool particles in the pair sums.  Brookshaw conduction reads the device block, which is
+ * written only when alpha != 0; the PSE variant reads the host copy, always written.  Both start
+ * at zero (tool particles then conduct no heat). */
+int sphfem_interactions_setup_thermal_constants_tool(sphfem_ctx *ctx, const sphfem_trml_constants *c);
+
 /* ---- time integration (leap_frog.h:46-49, leap_frog.cu:25-96, 162-248) */
 int sphfem_leap_frog_create(sphfem_ctx *ctx, unsigned int num_part, unsigned int num_cell, int node_num,
                             sphfem_leap_frog **out);
@@ -245,6 +309,12 @@ int sphfem_leap_frog_get_tuning(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_tu
  * carrying labels) and then runs the same fused kernels over the sorted arrays. */
 int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,
                           sphfem_tool *t);
+/* leap_frog::step(particles, grid) (leap_frog.cu:99-160) with global_tool = rt: the same fused
+ * kernels, with friction heat after the corrector, then plasticity, the tool moved by global_dt *
+ * vel (segments, fillet, tool particles) and contact against the moved tool.  rt may be NULL (no
+ * tool: the step of leap_frog.cu:99-160 with global_tool = 0). */
+int sphfem_leap_frog_step_rigid(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,
+                                sphfem_rigid_tool *rt);
 
 /* Step graphs: `steps` (even) steps of sphfem_leap_frog_step recorded into one HIP graph and
  * replayed with sphfem_leap_frog_graph_launch (each launch advances `steps` steps).  Capture
@@ -255,6 +325,9 @@ int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particle
 typedef struct sphfem_graph sphfem_graph;
 int sphfem_leap_frog_graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,
                                   sphfem_tool *t, int steps, sphfem_graph **out);
+/* the same for sphfem_leap_frog_step_rigid (rt may be NULL) */
+int sphfem_leap_frog_graph_create_rigid(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p,
+                                        sphfem_grid *g, sphfem_rigid_tool *rt, int steps, sphfem_graph **out);
 int sphfem_leap_frog_graph_launch(sphfem_ctx *ctx, sphfem_graph *graph, int repeats);
 int sphfem_leap_frog_graph_destroy(sphfem_ctx *ctx, sphfem_graph *graph);
 

@@ -9,7 +9,8 @@
  *   grid_base / grid_gpu_green / grid_gpu_rothlin
  *                       grid.h:23-64, grid_gpu_green.h:84-85, grid_gpu_rothlin.h:79-80
  *   leap_frog           leap_frog.h:31-50 (step(particles, grid[, tool_FEM]))
- *   tool / tool_FEM     tool.h:48-58, tool_FEM.h:34-83 (per-step methods)
+ *   tool / tool_FEM     tool.h:31-141 (rigid tool: constructors, geometry), tool_gpu.h:19-28,
+ *                       tool_FEM.h:34-83 (per-step methods)
  *   free functions      actions_gpu.h:33-72, interactions_gpu.h:27-43, output.h:17
  *   global_dt, wear_pro benchmarks.cpp:12, mfree_gpu.cpp:47
  *
@@ -24,6 +25,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../sphfem.h"
 
@@ -54,6 +56,13 @@ inline int &device_slot() {
 inline sphfem_ctx *&ctx_slot() {
     static sphfem_ctx *c = nullptr;
     return c;
+}
+
+// the device tool of the last tool_gpu_set_up_tool (the reference's __constant__ tool state,
+// tool_gpu.cu:16-22), read by compute_contact_forces
+inline sphfem_rigid_tool *&rigid_slot() {
+    static sphfem_rigid_tool *t = nullptr;
+    return t;
 }
 
 inline void check(int rc, const char *what) {
@@ -139,17 +148,114 @@ struct particle_gpu : sphfem_particles {
 };
 
 // ---------------------------------------------------------------------------------------------
-// tool.h:31-58 / tool_FEM.h:30-83
+// tool.h:31-141 (rigid tool: host geometry + the device copy of tool_gpu_set_up_tool) /
+// tool_FEM.h:30-83
 // ---------------------------------------------------------------------------------------------
+typedef sphfem_bbox bbox;
+
 class tool {
   public:
-    virtual ~tool() = default;
-    virtual void set_vel(vec2_t vel) { m_velocity = vel; }
+    tool() = default;
+    // tool.cpp:316-331
+    tool(vec2_t tl, vec2_t tr, vec2_t br, vec2_t bl, double r, double mu_fric) {
+        sphfem_shim::check(sphfem_tool_geometry_from_points(d2(tl), d2(tr), d2(br), d2(bl), r, mu_fric, &m_geom),
+                           "tool");
+    }
+    tool(vec2_t tl, vec2_t tr, vec2_t br, vec2_t bl, double mu_fric) : tool(tl, tr, br, bl, 0., mu_fric) {}
+    // tool.cpp:335-406 (angles in degrees)
+    tool(vec2_t tl, double length, double height, double rake_angle, double clearance_angle, double r,
+         double mu_fric) {
+        sphfem_shim::check(sphfem_tool_geometry_from_angles(d2(tl), length, height, rake_angle, clearance_angle, r,
+                                                            mu_fric, &m_geom),
+                           "tool");
+    }
+    tool(vec2_t tl, double length, double height, double rake_angle, double clearance_angle, double mu_fric)
+        : tool(tl, length, height, rake_angle, clearance_angle, 0., mu_fric) {}
+    virtual ~tool() {
+        if (m_gpu && sphfem_shim::ctx_slot()) {
+            if (sphfem_shim::rigid_slot() == m_gpu) sphfem_shim::rigid_slot() = nullptr;
+            (void)sphfem_rigid_tool_destroy(sphfem_shim::ctx_slot(), m_gpu);
+        }
+    }
+    tool(const tool &) = delete;
+    tool &operator=(const tool &) = delete;
+
+    bbox safe_bb(double safety = 0.011) const {
+        bbox b{};
+        sphfem_shim::check(sphfem_tool_geometry_safe_bb(&m_geom, safety, &b), "tool::safe_bb");
+        return b;
+    }
+    double low() const {
+        double v = 0.;
+        sphfem_shim::check(sphfem_tool_geometry_low(&m_geom, &v), "tool::low");
+        return v;
+    }
+    double mu() const { return m_geom.mu; }
+    double front() const { return m_geom.front; }
+    // tool.cpp:202-212 (the host copy; the device copy moves in the step / tool_gpu_update_tool)
+    void update_tool(double dt) { sphfem_shim::check(sphfem_tool_geometry_move(&m_geom, dt), "tool::update_tool"); }
+    virtual void set_vel(vec2_t vel) {
+        m_velocity = vel;
+        m_geom.vel = d2(vel);
+        if (m_gpu)
+            sphfem_shim::check(sphfem_rigid_tool_set_vel(sphfem_shim::ctx(), m_gpu, m_geom.vel), "tool::set_vel");
+    }
     vec2_t get_vel() const { return m_velocity; }
+    void set_thermal(bool thermal) { m_thermal = thermal; }
+    bool thermal() const { return m_thermal; }
+    bool inside(float2_t pos) const {
+        int in = 0;
+        sphfem_shim::check(sphfem_tool_geometry_inside(&m_geom, pos, &in), "tool::inside");
+        return in != 0;
+    }
+    // tool.cpp:112-135 (T_init, rho and h are not used there either)
+    std::vector<float2_t> sample_tool(double dx, double, double, double) const {
+        long n = 0;
+        sphfem_shim::check(sphfem_tool_geometry_sample(&m_geom, dx, nullptr, 0, &n), "tool::sample_tool");
+        std::vector<float2_t> out((size_t)n);
+        sphfem_shim::check(sphfem_tool_geometry_sample(&m_geom, dx, out.data(), n, &n), "tool::sample_tool");
+        return out;
+    }
+    const sphfem_tool_geometry &geometry() const { return m_geom; }
+    sphfem_rigid_tool *gpu_handle() const { return m_gpu; }
 
   protected:
+    static sphfem_double2 d2(vec2_t v) { return sphfem_double2{v.x, v.y}; }
     vec2_t m_velocity;
+    sphfem_tool_geometry m_geom{};
+    bool m_thermal = false;
+    sphfem_rigid_tool *m_gpu = nullptr;
+    friend void tool_gpu_set_up_tool(tool *t, double alpha, phys_constants phys);
 };
+
+// mfree_gpu.cpp:40: the rigid tool read by leap_frog::step(particles, grid)
+inline tool *global_tool = nullptr;
+
+// tool_gpu.h:19-28
+inline void tool_gpu_set_up_tool(tool *t, double alpha, phys_constants phys) {
+    sphfem_ctx *c = sphfem_shim::ctx();
+    if (t->m_gpu) {
+        if (sphfem_shim::rigid_slot() == t->m_gpu) sphfem_shim::rigid_slot() = nullptr;
+        sphfem_shim::check(sphfem_rigid_tool_destroy(c, t->m_gpu), "tool_gpu_set_up_tool");
+        t->m_gpu = nullptr;
+    }
+    sphfem_shim::check(sphfem_tool_gpu_set_up_tool(c, &t->m_geom, alpha, &phys, &t->m_gpu), "tool_gpu_set_up_tool");
+    sphfem_shim::rigid_slot() = t->m_gpu;
+}
+inline void tool_gpu_update_tool(tool *t, particle_gpu *particles) {
+    if (!t->gpu_handle()) throw std::runtime_error("tool_gpu_update_tool: tool_gpu_set_up_tool was not called");
+    sphfem_ctx *c = sphfem_shim::ctx();
+    sphfem_shim::check(sphfem_set_dt(c, global_dt), "set_dt");
+    t->update_tool(global_dt);
+    sphfem_shim::check(sphfem_tool_gpu_update_tool(c, t->gpu_handle(), particles), "tool_gpu_update_tool");
+}
+inline void compute_contact_forces(particle_gpu *particles) {
+    if (!sphfem_shim::rigid_slot()) throw std::runtime_error("compute_contact_forces: no tool set up");
+    sphfem_ctx *c = sphfem_shim::ctx();
+    sphfem_shim::check(sphfem_set_dt(c, global_dt), "set_dt");
+    sphfem_shim::check(sphfem_compute_contact_forces(c, particles, sphfem_shim::rigid_slot()),
+                       "compute_contact_forces");
+}
 
 // the segments live inside the tool object; the type exists for signature compatibility
 struct segment_FEM_gpu;
@@ -346,7 +452,16 @@ class leap_frog {
                                                  m_tool ? m_tool->handle() : nullptr),
                            "leap_frog::step");
     }
-    void step(particle_gpu *particles, grid_base *g) { step(particles, g, nullptr); }
+    // leap_frog.cu:99-160 with global_tool (its device copy; a tool never set up on the device, or
+    // none, gives the tool-free step).  The host copy of the tool follows, as tool_gpu_update_tool
+    // moves it in the reference.
+    void step(particle_gpu *particles, grid_base *g) {
+        sphfem_ctx *c = sphfem_shim::ctx();
+        sphfem_shim::check(sphfem_set_dt(c, global_dt), "set_dt");
+        sphfem_rigid_tool *rt = global_tool ? global_tool->gpu_handle() : nullptr;
+        sphfem_shim::check(sphfem_leap_frog_step_rigid(c, m_handle, particles, g->handle(), rt), "leap_frog::step");
+        if (rt) global_tool->update_tool(global_dt);
+    }
 
   private:
     sphfem_leap_frog *m_handle = nullptr;
@@ -382,6 +497,18 @@ inline void interactions_setup_physical_constants(phys_constants c) { actions_se
 inline void interactions_setup_corrector_constants(corr_constants c) { actions_setup_corrector_constants(c); }
 inline void interactions_setup_thermal_constants_workpiece(trml_constants c) { actions_setup_thermal_constants_wp(c); }
 inline void interactions_setup_tool_constants(tool_constants c) { actions_setup_tool_constants(c); }
+// interactions_gpu.cu:494-502
+inline void interactions_setup_thermal_constants_tool(trml_constants c, tool *t) {
+    SPHFEM_SHIM_CALL(sphfem_interactions_setup_thermal_constants_tool(sphfem_shim::ctx(), &c),
+                     "interactions_setup_thermal_constants_tool");
+    if (c.alpha != 0. && t) t->set_thermal(true);
+}
+// actions_gpu.cu:956-965
+inline void material_fric_heat_gen(particle_gpu *p, vec2_t vel) {
+    sphfem_ctx *c = sphfem_shim::ctx();
+    SPHFEM_SHIM_CALL(sphfem_set_dt(c, global_dt), "set_dt");
+    SPHFEM_SHIM_CALL(sphfem_material_fric_heat_gen(c, p, sphfem_double2{vel.x, vel.y}), "material_fric_heat_gen");
+}
 inline void interactions_setup_geometry_constants(grid_base *g) {
     SPHFEM_SHIM_CALL(sphfem_interactions_setup_geometry_constants(sphfem_shim::ctx(), g->handle()),
                      "interactions_setup_geometry_constants");

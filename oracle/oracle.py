@@ -43,7 +43,30 @@ OWearc = _struct("OWearc", ["A", "B", "C", "D"])
 class OConsts(C.Structure):
     _fields_ = [("phys", OPhys), ("trml_wp", OTrml), ("trml_tool", OTrml), ("corr", OCorr),
                 ("joco", OJoco), ("geom", OGeom), ("tc", OToolc), ("wear", OWearc), ("dt", _dbl),
-                ("rr2_min", _dbl), ("wear_pro", C.c_int), ("thermal_bc", C.c_int), ("plastic", C.c_int)]
+                ("rr2_min", _dbl), ("wear_pro", C.c_int), ("thermal_bc", C.c_int), ("plastic", C.c_int),
+                ("fric_heat", C.c_int), ("trml_itool", OTrml), ("itool_alpha_pse", _dbl)]
+
+
+class OD2(C.Structure):
+    _fields_ = [("x", _dbl), ("y", _dbl)]
+
+
+class OLine(C.Structure):
+    _fields_ = [("a", _dbl), ("b", _dbl), ("vertical", C.c_int)]
+
+
+class OSeg(C.Structure):
+    _fields_ = [("left", OD2), ("right", OD2), ("l", OLine), ("n", OD2)]
+
+
+class OCirc(C.Structure):
+    _fields_ = [("r", _dbl), ("t1", _dbl), ("t2", _dbl), ("p", OD2)]
+
+
+class ORTool(C.Structure):
+    """o_rtool: the rigid tool of tool.h / tool_gpu.cu:16-22."""
+    _fields_ = [("num_seg", C.c_int), ("has_fillet", C.c_int), ("seg", OSeg * 5), ("fillet", OCirc), ("vel", OD2),
+                ("mu", _dbl), ("front", _dbl), ("contact_alpha", _dbl), ("slave_mass", _dbl)]
 
 
 _PF = [name for name, _, _ in cases.PARTICLE_FIELDS]
@@ -83,6 +106,7 @@ def lib():
         pp = C.POINTER(OParticles)
         ps = C.POINTER(OStepper)
         pt = C.POINTER(OTool)
+        pr = C.POINTER(ORTool)
         sig = {
             "o_compute_hashes": (pc, pp), "o_sort_and_cells": (pc, pp, ps), "o_init": (pp, ps),
             "o_sort_rothlin": (pc, pp, ps), "o_run_rothlin": (pc, pp, ps, pt, C.c_int),
@@ -104,6 +128,13 @@ def lib():
             "o_sigma_yield": (C.POINTER(OJoco), _dbl, _dbl, _dbl),
             "o_solve_secant": (C.POINTER(OJoco), _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, C.POINTER(C.c_int)),
             "o_set_num_threads": (C.c_int,), "o_num_threads": (),
+            "o_rtool_from_points": (pr, OD2, OD2, OD2, OD2, _dbl, _dbl),
+            "o_rtool_from_angles": (pr, OD2, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl),
+            "o_rtool_move": (pr, _dbl), "o_rtool_low": (pr,), "o_rtool_safe_bb": (pr, _dbl, C.POINTER(_dbl)),
+            "o_rtool_inside": (pr, _dbl, _dbl), "o_rtool_sample": (pr, _dbl, C.c_void_p, C.c_long),
+            "o_rtool_establish_contact": (pr, _dbl, _dbl, C.POINTER(OD2), C.POINTER(OD2)),
+            "o_rtool_contact": (pc, pp, pr), "o_rtool_tool_particles": (pc, pp, pr),
+            "o_rtool_fric_heat": (pc, pp, pr), "o_run_rigid": (pc, pp, ps, pr, C.c_int, C.c_int),
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -114,6 +145,10 @@ def lib():
         L.o_num_threads.restype = C.c_int
         L.o_update_geometry.restype = C.c_int
         L.o_run_unlocked.restype = C.c_int
+        L.o_rtool_low.restype = _dbl
+        L.o_rtool_inside.restype = C.c_int
+        L.o_rtool_sample.restype = C.c_long
+        L.o_rtool_establish_contact.restype = C.c_int
         _lib = L
     return _lib
 
@@ -142,7 +177,21 @@ def make_consts(case: cases.Case, rr2_min=None, wear_pro=None) -> OConsts:
     c.wear_pro = int(case.wear if wear_pro is None else wear_pro)
     c.thermal_bc = int(k.trml_tool["alpha"] != 0.0)
     c.plastic = 1
+    c.fric_heat = int(k.trml_wp["eta"] != 0.0)
+    if case.rigid is not None:
+        _fill(c.trml_itool, case.rigid.trml_itool)
+        c.itool_alpha_pse = case.rigid.trml_itool["alpha"]
     return c
+
+
+def rtool_from_geometry(g, contact_alpha: float, slave_mass: float) -> ORTool:
+    """An oracle rigid tool with the fields of a library ToolGeometry (same layout, plus the two
+    contact constants of tool_gpu_set_up_tool)."""
+    t = ORTool()
+    C.memmove(C.addressof(t), C.addressof(g), C.sizeof(g))
+    t.contact_alpha = contact_alpha
+    t.slave_mass = slave_mass
+    return t
 
 
 def _ptr(a):
@@ -170,6 +219,9 @@ class Oracle:
                        rho_init=np.zeros(N), T_init=np.zeros(N),
                        cell_start=np.full(nc, -1, dtype=np.int32), cell_end=np.zeros(nc, dtype=np.int32))
         self.t = cases.initial_tool(case)
+        self.rt = None
+        if case.rigid is not None:
+            self.rt = rtool_from_geometry(case.rigid.geometry, case.rigid.contact_alpha, case.const.phys["mass"])
         self._bind()
 
     def _bind(self):
@@ -203,6 +255,10 @@ class Oracle:
         o.c = OConsts()
         C.pointer(o.c)[0] = self.c
         o.P = {k: v.copy() for k, v in self.P.items()}
+        o.rt = None
+        if self.rt is not None:
+            o.rt = ORTool()
+            C.pointer(o.rt)[0] = self.rt
         o.st = {k: v.copy() for k, v in self.st.items()}
         if self.t is None:
             o.t = None
@@ -218,7 +274,8 @@ class Oracle:
 
     def _args(self, sig):
         m = {"c": C.byref(self.c), "p": C.byref(self.Pc), "s": C.byref(self.Sc),
-             "t": C.byref(self.Tc) if self.Tc is not None else None}
+             "t": C.byref(self.Tc) if self.Tc is not None else None,
+             "r": C.byref(self.rt) if self.rt is not None else None}
         return [m[ch] for ch in sig]
 
     def call(self, fname, sig, *extra):
@@ -226,6 +283,14 @@ class Oracle:
 
     def step(self, n=1):
         tc = C.byref(self.Tc) if self.Tc is not None else None
+        if self.rt is not None or self.case.tool_p is not None:
+            # leap_frog::step(particles, grid) (leap_frog.cu:99-160)
+            if self.max_cell:
+                raise NotImplementedError("rigid-tool steps on an unlocked grid")
+            rt = C.byref(self.rt) if self.rt is not None else None
+            lib().o_run_rigid(C.byref(self.c), C.byref(self.Pc), C.byref(self.Sc), rt, int(n),
+                              int(self.grid == "rothlin"))
+            return
         if self.max_cell:
             done = lib().o_run_unlocked(C.byref(self.c), C.byref(self.Pc), C.byref(self.Sc), tc, int(n),
                                         int(self.max_cell), int(self.grid == "rothlin"))

@@ -340,7 +340,7 @@ void o_interactions_heat_pse(const o_consts *c, o_particles *P, const o_stepper 
         if (P->blanked[pidx] == 1.) continue;
         o_d2 pi = P->pos[pidx];
         double hi = P->h[pidx], Ti = P->T[pidx];
-        double alpha = (P->tool_particle[pidx] == 1.) ? c->trml_tool.alpha : c->trml_wp.alpha;
+        double alpha = (P->tool_particle[pidx] == 1.) ? c->itool_alpha_pse : c->trml_wp.alpha;
         int hashi = P->hash[pidx];
         int gi = hashi / ny;
         int gj = hashi - gi * ny;
@@ -383,8 +383,9 @@ void o_interactions_monaghan(const o_consts *c, o_particles *P, const o_stepper 
     for (int pidx = 0; pidx < P->N; pidx++) {
         if (P->blanked[pidx] == 1.) continue;
         double is_tool_i = P->tool_particle[pidx];
-        /* the tool block of interactions_gpu.cu:19 is never set by the FEM setups (benchmarks_NZ.cpp:354) */
-        double thermal_alpha = (is_tool_i == 0.) ? c->trml_wp.alpha : 0.;
+        /* interactions_gpu.cu:164: the tool block (:19) is zero unless interactions_setup_thermal_constants_tool
+         * set it (benchmarks.cpp:518; the FEM setups do not, benchmarks_NZ.cpp:354) */
+        double thermal_alpha = (is_tool_i == 0.) ? c->trml_wp.alpha : c->trml_itool.alpha;
         o_d2 pi = P->pos[pidx], vi = P->vel[pidx];
         o_d4 Si = P->S[pidx], Ri = P->R[pidx];
         double hi = P->h[pidx], rhoi = P->rho[pidx], prsi = P->p[pidx];
@@ -979,6 +980,303 @@ void o_report_force(const o_particles *P, double *fx, double *fy) {
 }
 
 /* leap_frog.cu:162-248 */
+/* ---------------------------------------------------------------------------------------------
+ * rigid (SPH) tool: tool.cpp, geometric_primitives.cpp, tool_gpu.cu
+ * ------------------------------------------------------------------------------------------- */
+
+/* geometric_primitives.cpp:61-76 */
+static o_line r_line(o_d2 p1, o_d2 p2) {
+    o_line l;
+    double det = p1.x * 1. - 1. * p2.x;
+    if (fabs(det) < 1e-12) {
+        l.vertical = 1; l.a = DBL_MAX; l.b = p1.x;
+        return l;
+    }
+    l.vertical = 0;
+    l.a = (p1.y * 1. - 1. * p2.y) / det;
+    l.b = (p2.y * p1.x - p2.x * p1.y) / det;
+    return l;
+}
+
+/* geometric_primitives.cpp:40-55 */
+static o_d2 r_closest(o_line l, o_d2 q) {
+    o_d2 r;
+    if (l.vertical) { r.x = l.b; r.y = q.y; return r; }
+    double bb = -1., den = l.a * l.a + bb * bb;
+    r.x = (bb * (bb * q.x - l.a * q.y) - l.a * l.b) / den;
+    r.y = (l.a * (-bb * q.x + l.a * q.y) - bb * l.b) / den;
+    return r;
+}
+
+/* geometric_primitives.cpp:11-38 */
+static o_d2 r_intersect(o_line m, o_line l) {
+    o_d2 r = {DBL_MAX, DBL_MAX};
+    if (m.vertical && l.vertical) return r;
+    if (fabs(m.a - l.a) < 1e-12) return r;
+    if (m.vertical) { r.x = m.b; r.y = l.a * m.b + l.b; return r; }
+    if (l.vertical) { r.x = l.b; r.y = m.a * l.b + m.b; return r; }
+    r.x = (l.b - m.b) / (m.a - l.a);
+    r.y = m.a * r.x + m.b;
+    return r;
+}
+
+static o_d2 r_norm(double x, double y) {        /* glm::normalize */
+    double s = 1. / sqrt(x * x + y * y);
+    o_d2 r = {x * s, y * s};
+    return r;
+}
+
+static double r_atan2(double y, double x) {     /* tool.cpp:18-25 */
+    double t = atan2(y, x);
+    return t > 0. ? t : t + 2 * O_PI;
+}
+
+static void r_segments(o_rtool *t, const o_d2 *pts, int n) {   /* tool.cpp:55-62 + geometric_primitives.cpp:103-113 */
+    t->num_seg = n;
+    for (int i = 0; i < n; i++) {
+        o_d2 a = pts[i], b = pts[(i + 1) % n];
+        t->seg[i].left = a;
+        t->seg[i].right = b;
+        t->seg[i].n = r_norm(b.y - a.y, -(b.x - a.x));
+        t->seg[i].l = r_line(a, b);
+    }
+}
+
+static void r_build(o_rtool *t, o_d2 tl, o_d2 tr, o_d2 br, o_d2 bl, double r, double mu) {
+    memset(t, 0, sizeof(*t));
+    t->mu = mu;
+    t->front = br.x;
+    if (r == 0.) {
+        o_d2 pts[4] = {tl, tr, br, bl};
+        r_segments(t, pts, 4);
+        return;
+    }
+    /* tool.cpp:64-93 with fit_fillet :27-53 and solve_quad :11-16 */
+    o_d2 nt = r_norm(tr.x - br.x, tr.y - br.y), nl = r_norm(bl.x - br.x, bl.y - br.y);
+    o_d2 nm = r_norm(0.5 * (nt.x + nl.x), 0.5 * (nt.y + nl.y));
+    o_d2 q = {br.x + nm.x, br.y + nm.y};
+    o_line lm = r_line(br, q), l1 = r_line(tr, br), l2 = r_line(bl, br);
+    o_d2 p;
+    if (l1.vertical) {
+        o_line par = {DBL_MAX, l1.b - r, 1};
+        p = r_intersect(par, lm);
+    } else {
+        double A = l1.a - lm.a, B = l1.b - lm.b, C = r * sqrt(l1.a * l1.a + 1.);
+        double qa = A * A, qb = 2 * A * B, qc = B * B - C * C;
+        double x1 = (-qb + sqrt(qb * qb - 4 * qa * qc)) / (2 * qa);
+        double x2 = (-qb - sqrt(qb * qb - 4 * qa * qc)) / (2 * qa);
+        p.x = fmin(x1, x2);
+        p.y = lm.a * p.x + lm.b;
+    }
+    o_d2 trc = r_closest(l1, p), blc = r_closest(l2, p);
+    t->has_fillet = 1;
+    t->fillet.r = r;
+    t->fillet.t1 = r_atan2(p.y - trc.y, p.x - trc.x);
+    t->fillet.t2 = r_atan2(p.y - blc.y, p.x - blc.x);
+    t->fillet.p = p;
+    o_d2 pts[5] = {tl, tr, trc, blc, bl};
+    r_segments(t, pts, 5);
+}
+
+void o_rtool_from_points(o_rtool *t, o_d2 tl, o_d2 tr, o_d2 br, o_d2 bl, double r, double mu) {
+    r_build(t, tl, tr, br, bl, r, mu);
+}
+
+/* tool.cpp:335-406: faces through tr and bl along (0, -1) * R(a) = (sin a, -cos a) */
+void o_rtool_from_angles(o_rtool *t, o_d2 tl, double length, double height, double rake, double clear, double r,
+                         double mu) {
+    o_d2 tr = {tl.x + length, tl.y}, bl = {tl.x, tl.y - height};
+    double ar = rake * O_PI / 180., af = (90. - clear) * O_PI / 180.;
+    o_d2 trc = {tr.x + (0. * cos(ar) + -1. * -sin(ar)), tr.y + (0. * sin(ar) + -1. * cos(ar))};
+    o_d2 blc = {bl.x + (0. * cos(af) + -1. * -sin(af)), bl.y + (0. * sin(af) + -1. * cos(af))};
+    o_d2 br = r_intersect(r_line(tr, trc), r_line(bl, blc));
+    r_build(t, tl, tr, br, bl, r, mu);
+}
+
+void o_rtool_move(o_rtool *t, double dt) {     /* tool.cpp:202-212 */
+    for (int i = 0; i < t->num_seg; i++) {
+        o_seg *g = &t->seg[i];
+        g->left.x += dt * t->vel.x; g->left.y += dt * t->vel.y;
+        g->right.x += dt * t->vel.x; g->right.y += dt * t->vel.y;
+        g->l = r_line(g->left, g->right);
+    }
+    if (t->has_fillet) { t->fillet.p.x += dt * t->vel.x; t->fillet.p.y += dt * t->vel.y; }
+}
+
+double o_rtool_low(const o_rtool *t) {
+    if (t->has_fillet) return t->fillet.p.y - t->fillet.r;
+    double y = DBL_MAX;
+    for (int i = 0; i < t->num_seg; i++) y = fmin(y, t->seg[i].left.y);
+    return y;
+}
+
+void o_rtool_safe_bb(const o_rtool *t, double safety, double *out) {
+    double x0 = DBL_MAX, x1 = -DBL_MAX, y0 = DBL_MAX, y1 = -DBL_MAX;
+    for (int i = 0; i < t->num_seg; i++) {
+        x0 = fmin(t->seg[i].left.x, x0); x1 = fmax(t->seg[i].left.x, x1);
+        y0 = fmin(t->seg[i].left.y, y0); y1 = fmax(t->seg[i].left.y, y1);
+    }
+    if (t->has_fillet) { y0 = o_rtool_low(t); x1 = t->fillet.p.x + t->fillet.r; }
+    out[0] = x0 - safety; out[1] = x1 + safety; out[2] = y0 - safety; out[3] = y1 + safety;
+}
+
+static int r_inside(const o_rtool *t, double x, double y, int fillet) {
+    int in = 1;
+    for (int i = 0; i < t->num_seg; i++)
+        in = in && ((t->seg[i].left.x - x) * t->seg[i].n.x + (t->seg[i].left.y - y) * t->seg[i].n.y) < 0.;
+    if (fillet) {
+        double ex = t->fillet.p.x - x, ey = t->fillet.p.y - y;
+        in = in || sqrt(ex * ex + ey * ey) < t->fillet.r;
+    }
+    return in;
+}
+
+int o_rtool_inside(const o_rtool *t, double x, double y) { return r_inside(t, x, y, t->has_fillet); }
+
+long o_rtool_sample(const o_rtool *t, double dx, o_d2 *out, long cap) {
+    double bb[4];
+    o_rtool_safe_bb(t, 0.1 * dx, bb);
+    int nx = (int)((bb[1] - bb[0]) / dx + 1), ny = (int)((bb[3] - bb[2]) / dx + 1);
+    long k = 0;
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+            double x = i * dx + bb[0], y = j * dx + bb[2];
+            if (!o_rtool_inside(t, x, y)) continue;
+            if (out && k < cap) { out[k].x = x; out[k].y = y; }
+            k++;
+        }
+    return k;
+}
+
+/* tool_gpu.cu:82-134 (the fillet counts when there are five segments) */
+int o_rtool_establish_contact(const o_rtool *t, double qx, double qy, o_d2 *cp, o_d2 *n) {
+    int fil = t->num_seg == 5;
+    cp->x = cp->y = n->x = n->y = 0.;
+    if (!r_inside(t, qx, qy, fil)) return 0;
+    double depth = ORACLE_FLT_MAX;
+    for (int i = 0; i < t->num_seg; i++) {
+        const o_seg *g = &t->seg[i];
+        double d;
+        if (g->l.vertical) {
+            d = fabs(g->left.x - qx);
+            if (d < depth) {
+                depth = d;
+                *n = r_norm(g->left.x - qx, 0.);
+                cp->x = g->left.x; cp->y = qy;
+            }
+        } else if (fil && i == 2) {
+            /* circle_segment_distance, tool_gpu.cu:33-58 */
+            const o_circ *f = &t->fillet;
+            double ex = qx - f->p.x, ey = qy - f->p.y;
+            double Cx = f->p.x + f->r * ex / sqrt(ex * ex + ey * ey);
+            double Cy = f->p.y + f->r * ey / sqrt(ex * ex + ey * ey);
+            double a = r_atan2(f->p.y - Cy, f->p.x - Cx);
+            d = (a > fmin(f->t1, f->t2) && a < fmax(f->t1, f->t2))
+                    ? sqrt((Cx - qx) * (Cx - qx) + (Cy - qy) * (Cy - qy)) : ORACLE_FLT_MAX;
+            if (d < depth) {
+                depth = d;
+                *n = r_norm(qx - f->p.x, qy - f->p.y);
+                cp->x = f->p.x + n->x * f->r; cp->y = f->p.y + n->y * f->r;
+            }
+        } else {
+            o_d2 q = {qx, qy}, p = r_closest(g->l, q);
+            d = sqrt((p.x - qx) * (p.x - qx) + (p.y - qy) * (p.y - qy));
+            if (d < depth) {
+                depth = d;
+                n->x = -g->n.x; n->y = -g->n.y;
+                *cp = p;
+            }
+        }
+    }
+    return 1;
+}
+
+/* tool_gpu.cu:146-202 */
+void o_rtool_contact(const o_consts *c, o_particles *P, const o_rtool *t) {
+    const double dt = c->dt, ms = t->slave_mass, ca = t->contact_alpha;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < P->N; i++) {
+        if (P->tool_particle[i] != 0.) continue;
+        o_d2 xs = P->pos[i], vs = P->vel[i], xm, n;
+        if (!o_rtool_establish_contact(t, xs.x, xs.y, &xm, &n)) {
+            P->fc[i].x = P->fc[i].y = P->ft[i].x = P->ft[i].y = P->n[i].x = P->n[i].y = 0.;
+            continue;
+        }
+        double dt2 = dt * dt;
+        double gN = (xs.x - xm.x) * n.x + (xs.y - xm.y) * n.y;
+        o_d2 fN = {-ms * gN * n.x / dt2 * ca, -ms * gN * n.y / dt2 * ca}, fT = {0., 0.};
+        if (t->mu != 0.) {
+            double vx = vs.x - t->vel.x, vy = vs.y - t->vel.y;
+            double vrx = vx - vx * n.x, vry = vy - vy * n.y;         /* glm v - v*n, per component */
+            double kx = ca * ms * vrx / dt, ky = ca * ms * vry / dt;
+            double fy = t->mu * sqrt(fN.x * fN.x + fN.y * fN.y);
+            double sx = P->ft[i].x - kx, sy = P->ft[i].y - ky;
+            double ls = sqrt(sx * sx + sy * sy);
+            if (ls > fy) { fT.x = fy * sx / ls; fT.y = fy * sy / ls; }
+            else { fT.x = sx; fT.y = sy; }
+        }
+        P->fc[i] = fN;
+        P->ft[i] = fT;
+        P->n[i] = n;
+    }
+}
+
+/* tool_gpu.cu:205-216 */
+void o_rtool_tool_particles(const o_consts *c, o_particles *P, const o_rtool *t) {
+    for (int i = 0; i < P->N; i++) {
+        if (P->tool_particle[i] == 0.) continue;
+        P->pos[i].x += c->dt * t->vel.x;
+        P->pos[i].y += c->dt * t->vel.y;
+    }
+}
+
+/* actions_gpu.cu:316-343 (sqrtf of a double: single-precision magnitudes) */
+void o_rtool_fric_heat(const o_consts *c, o_particles *P, const o_rtool *t) {
+    for (int i = 0; i < P->N; i++) {
+        if (P->tool_particle[i] != 0.) continue;
+        o_d2 f = P->ft[i];
+        double fm = (double)sqrtf((float)(f.x * f.x + f.y * f.y));
+        if (fm == 0.) continue;
+        o_d2 nn = P->n[i];
+        double vdx = P->vel[i].x - t->vel.x, vdy = P->vel[i].y - t->vel.y;
+        double vdn = vdx * nn.x + vdy * nn.y;
+        double rx = vdx - vdn * nn.x, ry = vdy - vdn * nn.y;
+        double vm = (double)sqrtf((float)(rx * rx + ry * ry));
+        P->T[i] += c->trml_wp.eta * c->dt * fm * vm / (c->trml_wp.cp * c->phys.mass);
+    }
+}
+
+/* leap_frog.cu:99-160 */
+static void step_rigid(const o_consts *c, o_particles *P, o_stepper *st, o_rtool *t, int rothlin) {
+    o_compute_hashes(c, P);
+    if (rothlin) o_sort_rothlin(c, P, st);
+    else o_sort_and_cells(c, P, st);
+    o_init(P, st);
+    o_predict(P, st, c->dt);
+    o_material_eos(c, P);
+    o_artificial_stress(c, P);
+    o_interactions_monaghan(c, P, st);
+    o_stress_rate_jaumann(c, P);
+    o_continuity(P);
+    o_momentum(c, P);
+    o_advection(P);
+    o_correct(P, st, c->dt);
+    if (t && c->fric_heat) o_rtool_fric_heat(c, P, t);
+    if (c->plastic) o_plasticity(c, P);
+    if (t) {
+        o_rtool_move(t, c->dt);
+        o_rtool_tool_particles(c, P, t);
+        o_rtool_contact(c, P, t);
+    }
+    o_boundary_conditions(P);
+    if (c->thermal_bc) o_boundary_conditions_thermal(c, P);
+    o_invalidate(P);
+}
+
+void o_run_rigid(const o_consts *c, o_particles *P, o_stepper *st, o_rtool *t, int nsteps, int rothlin) {
+    for (int s = 0; s < nsteps; s++) step_rigid(c, P, st, t, rothlin);
+}
+
 static void step_impl(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t, int rothlin) {
     o_compute_hashes(c, P);
     if (rothlin) o_sort_rothlin(c, P, st);

@@ -38,6 +38,9 @@ typedef struct {
     int wear_pro;        /* mfree_gpu.cpp:300 */
     int thermal_bc;      /* actions_gpu.cu:938-943 (tool alpha != 0) */
     int plastic;         /* actions_gpu.cu:921-924 */
+    int fric_heat;       /* actions_gpu.cu:931-934 (workpiece eta != 0) */
+    o_trml trml_itool;   /* interactions_gpu.cu:19 (device block, written for alpha != 0) */
+    double itool_alpha_pse;   /* interactions_gpu.cu:24 host copy's alpha (PSE launch, :453) */
 } o_consts;
 
 /* particle_gpu.h:23-83 */
@@ -67,6 +70,19 @@ typedef struct {
     double vel_x, vel_y, tl_x, tl_y, br_x, br_y, ref_dist, lowest_point, right_point;
     double flank_retreat, flank_contact;   /* __device__ globals tool_FEM.cu:19-20 */
 } o_tool;
+
+/* rigid (SPH) tool: geometric_primitives.h:15-57, tool.h:109-138, tool_gpu.cu:16-22 */
+typedef struct { double a, b; int vertical; } o_line;
+typedef struct { o_d2 left, right; o_line l; o_d2 n; } o_seg;
+typedef struct { double r, t1, t2; o_d2 p; } o_circ;
+typedef struct {
+    int num_seg, has_fillet;
+    o_seg seg[5];
+    o_circ fillet;
+    o_d2 vel;
+    double mu, front;
+    double contact_alpha, slave_mass;    /* tool_gpu_set_up_tool (tool_gpu.cu:219-266) */
+} o_rtool;
 
 /* kernels.cuh:25-65; out[0..2] = w, w_x, w_y */
 void o_cubic_spline(double pix, double piy, double pjx, double pjy, double hi, double rr2_min, double *out);
@@ -113,6 +129,24 @@ void o_segments_update(o_tool *t, double dt);
 void o_tool_update(o_tool *t, double dt);
 void o_wear_adjustment(const o_consts *c, o_tool *t);
 
+/* tool.cpp:316-406 constructors (r = 0: sharp), :202-212 update_tool, :178-188 low,
+ * :147-176 safe_bb (out = bbmin_x, bbmax_x, bbmin_y, bbmax_y), :95-110 inside,
+ * :112-135 sample_tool (writes up to cap points, returns the count) */
+void o_rtool_from_points(o_rtool *t, o_d2 tl, o_d2 tr, o_d2 br, o_d2 bl, double r, double mu);
+void o_rtool_from_angles(o_rtool *t, o_d2 tl, double length, double height, double rake, double clear, double r,
+                         double mu);
+void o_rtool_move(o_rtool *t, double dt);
+double o_rtool_low(const o_rtool *t);
+void o_rtool_safe_bb(const o_rtool *t, double safety, double *out);
+int o_rtool_inside(const o_rtool *t, double x, double y);
+long o_rtool_sample(const o_rtool *t, double dx, o_d2 *out, long cap);
+/* tool_gpu.cu:82-134 for one point: 1 if inside, contact point and normal */
+int o_rtool_establish_contact(const o_rtool *t, double qx, double qy, o_d2 *cp, o_d2 *n);
+/* tool_gpu.cu:146-202, 205-216 and actions_gpu.cu:316-343 over all particles */
+void o_rtool_contact(const o_consts *c, o_particles *P, const o_rtool *t);
+void o_rtool_tool_particles(const o_consts *c, o_particles *P, const o_rtool *t);
+void o_rtool_fric_heat(const o_consts *c, o_particles *P, const o_rtool *t);
+
 /* output.cu:22-30 */
 void o_report_force(const o_particles *P, double *fx, double *fy);
 
@@ -123,6 +157,10 @@ void o_run_rothlin(const o_consts *c, o_particles *P, o_stepper *st, o_tool *t, 
 /* grid.cu:88-127 and a step on an unlocked grid */
 int o_update_geometry(o_consts *c, const o_particles *P, double kernel_width, int max_cell);
 int o_run_unlocked(o_consts *c, o_particles *P, o_stepper *st, o_tool *t, int nsteps, int max_cell, int rothlin);
+
+/* leap_frog.cu:99-160: the step with a rigid tool (t may be NULL: no tool); rothlin selects the
+ * Rothlin sort */
+void o_run_rigid(const o_consts *c, o_particles *P, o_stepper *st, o_rtool *t, int nsteps, int rothlin);
 
 int o_num_threads(void);
 void o_set_num_threads(int n);

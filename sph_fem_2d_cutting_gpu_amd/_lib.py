@@ -65,6 +65,29 @@ class ToolDesc(C.Structure):
                    ("ref_dist", _dbl), ("lowest_point", _dbl), ("right_point", _dbl)])
 
 
+TOOL_MAX_SEG = 5
+
+
+class Line(C.Structure):
+    _fields_ = [("a", _dbl), ("b", _dbl), ("vertical", C.c_int)]
+
+
+class Segment(C.Structure):
+    _fields_ = [("left", Double2), ("right", Double2), ("l", Line), ("n", Double2)]
+
+
+class CircleSegment(C.Structure):
+    _fields_ = [("r", _dbl), ("t1", _dbl), ("t2", _dbl), ("p", Double2)]
+
+
+class ToolGeometry(C.Structure):
+    """sphfem_tool_geometry: the host state of the reference's class tool (tool.h:109-138)."""
+    _fields_ = [("num_seg", C.c_int), ("has_fillet", C.c_int), ("seg", Segment * TOOL_MAX_SEG),
+                ("fillet", CircleSegment), ("vel", Double2), ("mu", _dbl), ("front", _dbl)]
+
+
+BBox = _s("BBox", ["bbmin_x", "bbmax_x", "bbmin_y", "bbmax_y"])
+
 _P = C.POINTER
 _ctx = _vp
 _SIGS = {
@@ -131,6 +154,23 @@ _SIGS = {
     "sphfem_leap_frog_predict": (_ctx, _vp, _P(Particles), _dbl),
     "sphfem_leap_frog_correct": (_ctx, _vp, _P(Particles), _dbl),
     "sphfem_leap_frog_step": (_ctx, _vp, _P(Particles), _vp, _vp),
+    "sphfem_leap_frog_step_rigid": (_ctx, _vp, _P(Particles), _vp, _vp),
+    "sphfem_leap_frog_graph_create_rigid": (_ctx, _vp, _P(Particles), _vp, _vp, C.c_int, _P(_vp)),
+    "sphfem_tool_geometry_from_points": (Double2, Double2, Double2, Double2, _dbl, _dbl, _P(ToolGeometry)),
+    "sphfem_tool_geometry_from_angles": (Double2, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _P(ToolGeometry)),
+    "sphfem_tool_geometry_move": (_P(ToolGeometry), _dbl),
+    "sphfem_tool_geometry_low": (_P(ToolGeometry), _P(_dbl)),
+    "sphfem_tool_geometry_safe_bb": (_P(ToolGeometry), _dbl, _P(BBox)),
+    "sphfem_tool_geometry_inside": (_P(ToolGeometry), Double2, _P(C.c_int)),
+    "sphfem_tool_geometry_sample": (_P(ToolGeometry), _dbl, _vp, C.c_long, _P(C.c_long)),
+    "sphfem_tool_gpu_set_up_tool": (_ctx, _P(ToolGeometry), _dbl, _P(PhysConstants), _P(_vp)),
+    "sphfem_rigid_tool_destroy": (_ctx, _vp),
+    "sphfem_rigid_tool_get_geometry": (_ctx, _vp, _P(ToolGeometry)),
+    "sphfem_rigid_tool_set_vel": (_ctx, _vp, Double2),
+    "sphfem_tool_gpu_update_tool": (_ctx, _vp, _P(Particles)),
+    "sphfem_compute_contact_forces": (_ctx, _P(Particles), _vp),
+    "sphfem_material_fric_heat_gen": (_ctx, _P(Particles), Double2),
+    "sphfem_interactions_setup_thermal_constants_tool": (_ctx, _P(TrmlConstants)),
     "sphfem_report_force": (_ctx, _P(Particles), _P(Double2)),
     "sphfem_timing_enable": (_ctx, C.c_int),
     "sphfem_timing_reset": (_ctx,),

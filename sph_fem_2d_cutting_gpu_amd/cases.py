@@ -586,6 +586,18 @@ class Case:
     wear: bool
     S: np.ndarray | None = None
     extra: dict = field(default_factory=dict)
+    tool_p: np.ndarray | None = None      # tool_particle flags (sampled rigid tools)
+    rigid: "RigidTool | None" = None      # rigid (SPH) tool instead of the FEM tool
+
+
+@dataclass
+class RigidTool:
+    """A rigid tool as tool_gpu_set_up_tool receives it (tool_gpu.cu:219-266): the host geometry of
+    class tool (a ctypes ToolGeometry, velocity included), the contact alpha, and the thermal block
+    that interactions_setup_thermal_constants_tool gives the pair sums (interactions_gpu.cu:494-502)."""
+    geometry: object
+    contact_alpha: float
+    trml_itool: dict
 
 
 def grid_geometry(bbmin, bbmax, h):
@@ -785,6 +797,101 @@ def perturbed(case: Case, seed: int = DEFAULT_SEED + 1, vel_sd: float = 5e-4, S_
     return dataclasses.replace(case, name=case.name + "_perturbed", vel=vel, S=S, rho=rho, T=T)
 
 
+def ref_cut_sph(ny: int = 30, rake: float = 0.0, clear: float = 4.5, chamfer: float = 0.00035,
+                speed: float = 83.333328e-5 / 200 * 300, feed: float = 0.01) -> Case:
+    """setup_ref_cut (benchmarks.cpp:248-530): Ti6Al4V (Johnson 85) cut by a rigid tool sampled with
+    tool particles, at the reference driver's defaults (mfree_gpu.cpp:155-162, 197: rake 0, clearance
+    4.5 deg, fillet 3.5 um, ny 30, feed 100 um, 750 m/min).  The tool geometry comes from the library's
+    host constructors of class tool (tool.cpp:335-406)."""
+    from . import _lib
+    import ctypes as C
+    L = _lib.lib()
+
+    hi_x, hi_y = 0.200, 0.060
+    lo_x, lo_y = 0.005, hi_y - 3 * feed
+    dy = (hi_y - lo_y) / (ny - 1)
+    dx = dy
+    nx = int((hi_x - lo_x) / dx)
+    hdx = 1.5
+    phys = dict(E=1.1, nu=0.35, rho0=4.429998)
+    phys["G"] = phys["E"] / (2. * (1. + phys["nu"]))
+    phys["K"] = 2.0 * phys["G"] * (1 + phys["nu"]) / (3 * (1 - 2 * phys["nu"]))
+    phys["mass"] = dx * dx * phys["rho0"]
+    rho_tool = 15.25
+    phys["mass_tool"] = dx * dx * rho_tool
+    joco = dict(A=0.0085200, B=0.0033890, C=0.02754, m=0.5961, n=0.148, Tref=300., Tmelt=1836.0,
+                eps_dot_ref=1e-6, tanh_a=1.1, tanh_b=0.4, tanh_c=12., tanh_d=1.)
+    trml_wp = dict(cp=580 * 1e-8, tq=0.9, k=7.3 * 1e-13, alpha=0.0, T_init=joco["Tref"], eta=0.9)
+    trml_wp["alpha"] = trml_wp["k"] / (phys["rho0"] * trml_wp["cp"])
+    trml_tool = dict(cp=292 * 1e-08, tq=.0, k=88 * 1e-13, alpha=0.0, T_init=joco["Tref"], eta=0.0)
+    trml_tool["alpha"] = trml_tool["k"] / (rho_tool * trml_tool["cp"])
+    corr = dict(wdeltap=1.0, stresseps=0.3, xspheps=0.5, alpha=1., beta=1., eta=0.1)
+    h1 = 1. / (hdx * dx)
+    q = dx * h1
+    fac = 10 * M_1_PI / 7.0 * h1 * h1
+    corr["wdeltap"] = fac * (1 - 1.5 * q * q * (1 - 0.5 * q))
+    zeros_tool = dict(cp=0., tq=0., k=0., eta=0., rho=0., T0=0., h=0., beta=0., mu=0., contact_alpha=0.,
+                      slave_mass=0., hc=0.)
+    const = Constants(phys, trml_wp, trml_tool, corr, joco, zeros_tool, dict(A=0., B=0., C=0., D=0.))
+
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    pos = np.stack([ii.ravel() * dx + lo_x, jj.ravel() * dx + lo_y], axis=1)
+    n = nx * ny
+    min_x, max_x = pos[:, 0].min(), pos[:, 0].max()
+    min_y = pos[:, 1].min()
+    fixed = pos[:, 1] < min_y + 0.5 * dx
+    fixed |= (pos[:, 0] < min_x + dx / 2) & (pos[:, 1] < min_y + 0.5 * (hi_y - lo_y))
+    fixed |= pos[:, 0] > max_x - dx
+
+    vel_tool = 83.333328 * 1e-5 if speed == 0.0 else speed
+    mu_fric = 0.35
+    nudge = 0.004
+    tl = _lib.Double2(-0.05 + nudge, 0.0986074)
+    l_tool = -0.0086824 - -0.05
+    h_tool = 0.0986074 - 0.0555074
+    g = _lib.ToolGeometry()
+    _lib.check(L.sphfem_tool_geometry_from_angles(tl, l_tool, h_tool, rake, clear, chamfer, mu_fric, C.byref(g)))
+    # move the tool to the workpiece's left edge and to the target feed (benchmarks.cpp:393-408)
+    distance = g.front - lo_x
+    g.vel = _lib.Double2(vel_tool, 0.)
+    _lib.check(L.sphfem_tool_geometry_move(C.byref(g), -(distance / vel_tool)))
+    low = C.c_double()
+    _lib.check(L.sphfem_tool_geometry_low(C.byref(g), C.byref(low)))
+    current_feed = hi_y - low.value
+    correction_time = abs(current_feed - feed) / vel_tool
+    sign = 1. if current_feed > feed else -1.
+    g.vel = _lib.Double2(0., vel_tool)
+    _lib.check(L.sphfem_tool_geometry_move(C.byref(g), correction_time * sign))
+    g.vel = _lib.Double2(vel_tool, 0.)
+
+    cnt = C.c_long()
+    _lib.check(L.sphfem_tool_geometry_sample(C.byref(g), dx, None, 0, C.byref(cnt)))
+    samples = np.zeros((cnt.value, 2))
+    _lib.check(L.sphfem_tool_geometry_sample(C.byref(g), dx, samples.ctypes.data, cnt.value, C.byref(cnt)))
+    tfixed = (samples[:, 0] < samples[:, 0].min() + dx / 2) | (samples[:, 1] > samples[:, 1].max() - dx / 2)
+
+    allpos = np.concatenate([pos, samples])
+    N = len(allpos)
+    bbmin = allpos.min(axis=0) - 1e-8
+    bbmax = allpos.max(axis=0) + 1e-8
+    bb = _lib.BBox()
+    _lib.check(L.sphfem_tool_geometry_safe_bb(C.byref(g), 1e-3, C.byref(bb)))
+    bbmin = np.array([min(bbmin[0], bb.bbmin_x), min(bbmin[1], bb.bbmin_y)])
+    bbmax = np.array([max(bbmax[0], bb.bbmax_x) + 10 * dx, max(bbmax[1], bb.bbmax_y)])
+    bbmax[1] = max(bbmax[1], hi_y + hi_x - lo_x)
+
+    dt = 0.32 * hdx * dx / (math.sqrt(phys["K"] / phys["rho0"]) + math.sqrt(vel_tool))
+    h = hdx * dx
+    rho = np.concatenate([np.full(n, phys["rho0"]), np.full(len(samples), rho_tool)])
+    return Case(name=f"ref_cut_sph_ny{ny}", N=N, dx=dx, h=h, dt=dt, pos=allpos, vel=np.zeros((N, 2)), rho=rho,
+                T=np.full(N, joco["Tref"]), hh=np.full(N, h),
+                fixed=np.concatenate([fixed, tfixed]).astype(np.float64), const=const, grid_bbmin=bbmin,
+                grid_bbmax=bbmax, tool=None, tool_vel=np.array([vel_tool, 0.]), wear=False,
+                tool_p=np.concatenate([np.zeros(n), np.ones(len(samples))]),
+                rigid=RigidTool(geometry=g, contact_alpha=0.075, trml_itool=dict(trml_tool)),
+                extra=dict(t_final=0.1 / vel_tool, n_workpiece=n))
+
+
 def initial_particles(case: Case) -> dict:
     """State after particle_gpu(pos, vel, rho, T, h, fixed, N) (particle_gpu.cu:11-104, 203-206):
     every array zero, idx = 0..N-1, on_seg = 0, then the given fields copied in."""
@@ -797,6 +904,8 @@ def initial_particles(case: Case) -> dict:
     out["T"][:] = case.T
     out["h"][:] = case.hh
     out["fixed"][:] = case.fixed
+    if case.tool_p is not None:
+        out["tool_particle"][:] = case.tool_p
     if case.S is not None:
         out["S"][:] = case.S
     return out

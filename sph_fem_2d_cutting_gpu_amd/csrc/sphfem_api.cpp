@@ -6,6 +6,7 @@
 #include <utility>
 
 #include "sphfem_internal.h"
+#include "sphfem_rigid.h"
 
 using namespace sphfem;
 
@@ -237,7 +238,7 @@ int sphfem_check_error(sphfem_ctx *ctx) {
 SETC(sphfem_setup_physical_constants, phys, SET_PHYS, )
 SETC(sphfem_setup_corrector_constants, corr, SET_CORR, )
 SETC(sphfem_setup_johnson_cook_constants, joco, SET_JOCO, ctx->c.plastic = 1)
-SETC(sphfem_setup_thermal_constants_wp, trml_wp, SET_TRML_WP, )
+SETC(sphfem_setup_thermal_constants_wp, trml_wp, SET_TRML_WP, if (c->eta != 0.) ctx->c.fric_heat = 1)
 SETC(sphfem_setup_thermal_constants_tool, trml_tool, 0, ctx->c.thermal_bc = (c->alpha != 0.) ? 1 : 0)
 SETC(sphfem_setup_tool_constants, tc, SET_TOOLC, )
 SETC(sphfem_setup_wear_constants, wear, 0, )
@@ -648,6 +649,273 @@ int sphfem_wear_adjustment(sphfem_ctx *ctx, sphfem_tool *t) {
     });
 }
 
+// ---- rigid (SPH) tool: host geometry (tool.cpp), device copy and per-step kernels (tool_gpu.cu)
+}  // extern "C"
+
+namespace {
+
+sphfem_double2 d2(double x, double y) { return {x, y}; }
+
+// tool.cpp:11-16
+void solve_quad(double a, double b, double c, double &x1, double &x2) {
+    x1 = (-b + sqrt(b * b - 4 * a * c)) / (2 * a);
+    x2 = (-b - sqrt(b * b - 4 * a * c)) / (2 * a);
+}
+
+// tool.cpp:27-53: centre of the fillet of radius r on the bisector lm, at distance r from l1
+sphfem_double2 fit_fillet(double r, const sphfem_line &lm, const sphfem_line &l1) {
+    if (l1.vertical) {
+        const sphfem_line par = {DBL_MAX, l1.b - r, 1};
+        return rigid_intersect(par, lm);
+    }
+    const double A0 = lm.a, B0 = lm.b, a = l1.a, b = l1.b;
+    const double A = a - A0, B = b - B0, C = r * sqrt(a * a + 1.);
+    double x1, x2;
+    solve_quad(A * A, 2 * A * B, B * B - C * C, x1, x2);
+    const double xm = fmin(x1, x2);
+    return {xm, lm.a * xm + lm.b};
+}
+
+// tool.cpp:55-62: closed polygon through the points
+void construct_segments(sphfem_tool_geometry &g, const sphfem_double2 *pts, int n) {
+    g.num_seg = n;
+    for (int i = 0; i < n; i++) g.seg[i] = rigid_segment(pts[i], pts[i + 1 > n - 1 ? 0 : i + 1]);
+}
+
+// tool.cpp:64-93: the corner br rounded by a fillet tangent to tr-br and bl-br (5 points)
+void construct_points_and_fillet(sphfem_tool_geometry &g, sphfem_double2 tl, sphfem_double2 tr, sphfem_double2 br,
+                                 sphfem_double2 bl, double r, sphfem_double2 *pts) {
+    const sphfem_double2 pm = br;
+    const sphfem_double2 nt = rigid_normalize(d2(tr.x - br.x, tr.y - br.y));
+    const sphfem_double2 nl = rigid_normalize(d2(bl.x - br.x, bl.y - br.y));
+    const sphfem_double2 nm = rigid_normalize(d2(0.5 * (nt.x + nl.x), 0.5 * (nt.y + nl.y)));
+    const sphfem_line lm = rigid_line(pm, d2(pm.x + nm.x, pm.y + nm.y));
+    const sphfem_line l1 = rigid_line(tr, br), l2 = rigid_line(bl, br);
+    const sphfem_double2 p = fit_fillet(r, lm, l1);
+    const sphfem_double2 trc = rigid_closest_point(l1, p), blc = rigid_closest_point(l2, p);
+    g.has_fillet = 1;
+    g.fillet.r = r;
+    g.fillet.t1 = rigid_atan2(p.y - trc.y, p.x - trc.x);
+    g.fillet.t2 = rigid_atan2(p.y - blc.y, p.x - blc.x);
+    g.fillet.p = p;
+    pts[0] = tl; pts[1] = tr; pts[2] = trc; pts[3] = blc; pts[4] = bl;
+}
+
+// tool.cpp:316-331 and the tail of :335-369
+void build_tool(sphfem_tool_geometry &g, sphfem_double2 tl, sphfem_double2 tr, sphfem_double2 br, sphfem_double2 bl,
+                double r, double mu) {
+    std::memset(&g, 0, sizeof(g));
+    g.mu = mu;
+    sphfem_double2 pts[SPHFEM_TOOL_MAX_SEG];
+    if (r == 0.) {
+        pts[0] = tl; pts[1] = tr; pts[2] = br; pts[3] = bl;
+        construct_segments(g, pts, 4);
+    } else {
+        construct_points_and_fillet(g, tl, tr, br, bl, r, pts);
+        construct_segments(g, pts, 5);
+    }
+    g.front = br.x;
+}
+
+bool geometry_ok(const sphfem_tool_geometry *g) {
+    return g && g->num_seg >= 0 && g->num_seg <= SPHFEM_TOOL_MAX_SEG;
+}
+
+// tool.cpp:178-188
+double geometry_low(const sphfem_tool_geometry &g) {
+    if (g.has_fillet) return g.fillet.p.y - g.fillet.r;
+    double y_min = DBL_MAX;
+    for (int i = 0; i < g.num_seg; i++) y_min = fmin(y_min, g.seg[i].left.y);
+    return y_min;
+}
+
+// tool.cpp:147-176
+sphfem_bbox geometry_safe_bb(const sphfem_tool_geometry &g, double safety) {
+    double x_min = DBL_MAX, x_max = -DBL_MAX, y_min = DBL_MAX, y_max = -DBL_MAX;
+    for (int i = 0; i < g.num_seg; i++) {
+        x_min = fmin(g.seg[i].left.x, x_min);
+        x_max = fmax(g.seg[i].left.x, x_max);
+        y_min = fmin(g.seg[i].left.y, y_min);
+        y_max = fmax(g.seg[i].left.y, y_max);
+    }
+    if (g.has_fillet) {
+        y_min = geometry_low(g);
+        x_max = g.fillet.p.x + g.fillet.r;
+    }
+    x_min -= safety;
+    y_min -= safety;
+    x_max += safety;
+    y_max += safety;
+    return {x_min, x_max, y_min, y_max};
+}
+
+}  // namespace
+
+extern "C" {
+
+int sphfem_tool_geometry_from_points(sphfem_double2 tl, sphfem_double2 tr, sphfem_double2 br, sphfem_double2 bl,
+                                     double r, double mu, sphfem_tool_geometry *out) {
+    if (!out || !(r >= 0.)) return SPHFEM_ERR_ARG;
+    build_tool(*out, tl, tr, br, bl, r, mu);
+    return SPHFEM_OK;
+}
+
+// tool.cpp:335-406: tr and bl from the length and height, rake and clearance faces through tr and
+// bl turned from straight down by the rake angle and by 90 - clearance (glm row-vector products
+// down * R = (sin a, -cos a)), br where the two faces meet
+int sphfem_tool_geometry_from_angles(sphfem_double2 tl, double length, double height, double rake_angle,
+                                     double clearance_angle, double r, double mu, sphfem_tool_geometry *out) {
+    if (!out || !(r >= 0.)) return SPHFEM_ERR_ARG;
+    const sphfem_double2 tr = d2(tl.x + length, tl.y), bl = d2(tl.x, tl.y - height);
+    const double alpha_rake = rake_angle * SPHFEM_RIGID_PI / 180.;
+    const double alpha_free = (180 - 90 - clearance_angle) * SPHFEM_RIGID_PI / 180.;
+    // (0, -1) * mat2(cos, -sin, sin, cos): x = 0 cos + (-1)(-sin), y = 0 sin + (-1) cos
+    const double cr = cos(alpha_rake), sr = sin(alpha_rake), cf = cos(alpha_free), sf = sin(alpha_free);
+    const sphfem_double2 dr = d2(0. * cr + (-1.) * (-sr), 0. * sr + (-1.) * cr);
+    const sphfem_double2 df = d2(0. * cf + (-1.) * (-sf), 0. * sf + (-1.) * cf);
+    const sphfem_double2 trc = d2(tr.x + dr.x, tr.y + dr.y), blc = d2(bl.x + df.x, bl.y + df.y);
+    const sphfem_double2 br = rigid_intersect(rigid_line(tr, trc), rigid_line(bl, blc));
+    build_tool(*out, tl, tr, br, bl, r, mu);
+    return SPHFEM_OK;
+}
+
+int sphfem_tool_geometry_move(sphfem_tool_geometry *g, double dt) {
+    if (!geometry_ok(g)) return SPHFEM_ERR_ARG;
+    rigid_move(*g, dt);
+    return SPHFEM_OK;
+}
+
+int sphfem_tool_geometry_low(const sphfem_tool_geometry *g, double *low) {
+    if (!geometry_ok(g) || !low) return SPHFEM_ERR_ARG;
+    *low = geometry_low(*g);
+    return SPHFEM_OK;
+}
+
+int sphfem_tool_geometry_safe_bb(const sphfem_tool_geometry *g, double safety, sphfem_bbox *out) {
+    if (!geometry_ok(g) || !out) return SPHFEM_ERR_ARG;
+    *out = geometry_safe_bb(*g, safety);
+    return SPHFEM_OK;
+}
+
+int sphfem_tool_geometry_inside(const sphfem_tool_geometry *g, sphfem_double2 pos, int *inside) {
+    if (!geometry_ok(g) || !inside) return SPHFEM_ERR_ARG;
+    *inside = rigid_inside(*g, pos.x, pos.y, g->has_fillet != 0) ? 1 : 0;
+    return SPHFEM_OK;
+}
+
+// tool.cpp:112-135 (the lattice spans safe_bb(0.1 dx); counts truncate as the int conversion there)
+int sphfem_tool_geometry_sample(const sphfem_tool_geometry *g, double dx, sphfem_double2 *out, long cap,
+                                long *count) {
+    if (!geometry_ok(g) || !count || !(dx > 0.) || cap < 0) return SPHFEM_ERR_ARG;
+    const sphfem_bbox box = geometry_safe_bb(*g, 0.1 * dx);
+    const double fx = (box.bbmax_x - box.bbmin_x) / dx + 1, fy = (box.bbmax_y - box.bbmin_y) / dx + 1;
+    if (!(fx < 2e9 && fy < 2e9)) return SPHFEM_ERR_ARG;
+    const int nx = (int)fx, ny = (int)fy;
+    long k = 0;
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+            const sphfem_double2 pos = d2(i * dx + box.bbmin_x, j * dx + box.bbmin_y);
+            if (rigid_inside(*g, pos.x, pos.y, g->has_fillet != 0)) {
+                if (out && k < cap) out[k] = pos;
+                k++;
+            }
+        }
+    *count = k;
+    return SPHFEM_OK;
+}
+
+int sphfem_tool_gpu_set_up_tool(sphfem_ctx *ctx, const sphfem_tool_geometry *g, double contact_alpha,
+                                const sphfem_phys_constants *phys, sphfem_rigid_tool **out) {
+    if (!geometry_ok(g) || !phys || !out) return SPHFEM_ERR_ARG;
+    if (g->has_fillet && g->num_seg != SPHFEM_TOOL_MAX_SEG)
+        return fail(ctx, SPHFEM_ERR_ARG, "a fillet replaces the third of five segments");
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        RigidTool h{};
+        h.g = *g;
+        h.contact_alpha = contact_alpha;
+        h.slave_mass = phys->mass;
+        auto *rt = new sphfem_rigid_tool();
+        try {
+            rt->dev = dupload(ctx->stream, &h, 1);
+            SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            dfree(rt->dev);
+            delete rt;
+            throw;
+        }
+        *out = rt;
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_rigid_tool_destroy(sphfem_ctx *ctx, sphfem_rigid_tool *rt) {
+    if (!rt) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        dfree(rt->dev);
+        delete rt;
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_rigid_tool_get_geometry(sphfem_ctx *ctx, sphfem_rigid_tool *rt, sphfem_tool_geometry *out) {
+    if (!rt || !out) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        RigidTool h;
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(&h, rt->dev, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        *out = h.g;
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_rigid_tool_set_vel(sphfem_ctx *ctx, sphfem_rigid_tool *rt, sphfem_double2 vel) {
+    if (!rt) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        const sphfem_double2 v = vel;
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(&rt->dev->g.vel, &v, sizeof(v), hipMemcpyHostToDevice, ctx->stream));
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_tool_gpu_update_tool(sphfem_ctx *ctx, sphfem_rigid_tool *rt, sphfem_particles *p) {
+    if (!rt) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (int rc = need(ctx, SET_DT)) return rc;
+        launch_rigid_advance(ctx, rt);
+        if (p && p->N) launch_rigid_tool_particles(ctx, *p, rt);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_compute_contact_forces(sphfem_ctx *ctx, sphfem_particles *p, sphfem_rigid_tool *rt) {
+    if (!p || !rt) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (int rc = need(ctx, SET_DT)) return rc;
+        if (p->N) launch_rigid_contact(ctx, *p, rt);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_material_fric_heat_gen(sphfem_ctx *ctx, sphfem_particles *p, sphfem_double2 vel) {
+    if (!p) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->c.fric_heat) return SPHFEM_OK;   // actions_gpu.cu:957-959
+        if (int rc = need(ctx, SET_PHYS | SET_TRML_WP | SET_DT)) return rc;
+        if (p->N) launch_rigid_fric_heat(ctx, *p, vel);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_interactions_setup_thermal_constants_tool(sphfem_ctx *ctx, const sphfem_trml_constants *c) {
+    if (!ctx || !c) return SPHFEM_ERR_ARG;
+    ctx->c.itool_alpha_pse = c->alpha;
+    if (c->alpha != 0.) ctx->c.trml_itool = *c;
+    return SPHFEM_OK;
+}
+
 // ---- tool
 int sphfem_tool_create(sphfem_ctx *ctx, const sphfem_tool_desc *d, sphfem_tool **out) {
     if (!d || !out || d->node_num <= 0 || d->ele_num <= 0 || d->nz_num <= 0 || d->seg_num <= 0) return SPHFEM_ERR_ARG;
@@ -1047,12 +1315,17 @@ int sphfem_leap_frog_correct(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_parti
 //  tool_begin | hash+digit-0 histogram | radix passes | reorder+init+predict+EOS+art. stress |
 //  pair sums+Jaumann+continuity+momentum+advection+correct+friction heat+plasticity+far contact+
 //  BC+invalidate | FEM rate+temperature | wear | tool move | near contact s1 (+BC) | contact s2
-static int step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g, sphfem_tool *t) {
+// With a rigid tool (leap_frog.cu:99-160) the tool is moved first, as only the contact of the
+// update kernel reads its geometry and that reads the moved one; the update kernel also applies
+// the friction heat, moves the tool particles and computes the contact.
+static int step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g, sphfem_tool *t,
+                     sphfem_rigid_tool *rt = nullptr) {
     // leap_frog.cu:175 (a no-op for a locked grid)
     if (int rc = update_geometry(ctx, g, *p, 2.)) return rc;
     ctx->c.geom = g->geom;
     if (t) launch_tool_begin(ctx, t);
     else launch_fill_int(ctx, ctx->d_flags + 1, 2, 0);
+    if (rt) launch_rigid_advance(ctx, rt);
     // the source of each sorted position: the permutation of the sort (Green), or the identity
     // after the Rothlin sort has gathered in place
     sphfem_particles A = *p;
@@ -1072,7 +1345,7 @@ static int step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p,
     B.hash = p->hash;
     B.idx = A.idx;
     B.N = p->N;
-    launch_fused_update(ctx, B, *p, lf, g->cell_start, g->cell_end, t);
+    launch_fused_update(ctx, B, *p, lf, g->cell_start, g->cell_end, t, rt);
     if (lf->precision == SPHFEM_FP32C) {
         // the whole new state is in the buffer; R is not maintained by the FP32C form
         swap_gathered(*p, g->buf);
@@ -1098,6 +1371,15 @@ int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particle
     });
 }
 
+int sphfem_leap_frog_step_rigid(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,
+                                sphfem_rigid_tool *rt) {
+    if (!lf || !p || !g || (int)p->N != g->num_part || p->N != lf->num_part) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (int rc = need(ctx, SET_PHYS | SET_CORR | SET_TRML_WP | SET_DT)) return rc;
+        return step_body(ctx, lf, p, g, nullptr, rt);
+    });
+}
+
 }  // extern "C"
 
 struct sphfem_graph {
@@ -1108,8 +1390,8 @@ struct sphfem_graph {
 
 extern "C" {
 
-int sphfem_leap_frog_graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,
-                                  sphfem_tool *t, int steps, sphfem_graph **out) {
+static int graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g, sphfem_tool *t,
+                        sphfem_rigid_tool *rt, int steps, sphfem_graph **out) {
     if (!lf || !p || !g || !out || (int)p->N != g->num_part || p->N != lf->num_part) return SPHFEM_ERR_ARG;
     if (steps < 2 || steps % 2) return fail(ctx, SPHFEM_ERR_ARG, "graph steps must be even (pointer swaps)");
     *out = nullptr;
@@ -1126,7 +1408,7 @@ int sphfem_leap_frog_graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_
             SPHFEM_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             // recording only: the device state does not advance; after an even number of steps the
             // host-side pointer assignment is the one the graph starts from
-            for (int k = 0; k < steps; k++) (void)step_body(ctx, lf, p, g, t);   // locked: cannot fail
+            for (int k = 0; k < steps; k++) (void)step_body(ctx, lf, p, g, t, rt);   // locked: cannot fail
             SPHFEM_HIP_CHECK(hipStreamEndCapture(ctx->stream, &gr->graph));
             SPHFEM_HIP_CHECK(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0));
         } catch (...) {
@@ -1142,6 +1424,16 @@ int sphfem_leap_frog_graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_
         *out = gr;
         return SPHFEM_OK;
     });
+}
+
+int sphfem_leap_frog_graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,
+                                  sphfem_tool *t, int steps, sphfem_graph **out) {
+    return graph_create(ctx, lf, p, g, t, nullptr, steps, out);
+}
+
+int sphfem_leap_frog_graph_create_rigid(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p,
+                                        sphfem_grid *g, sphfem_rigid_tool *rt, int steps, sphfem_graph **out) {
+    return graph_create(ctx, lf, p, g, nullptr, rt, steps, out);
 }
 
 int sphfem_leap_frog_graph_launch(sphfem_ctx *ctx, sphfem_graph *gr, int repeats) {

@@ -34,7 +34,11 @@ struct DevConsts {
     int wear_pro;
     int thermal_bc;   // actions_gpu.cu:938-943
     int plastic;      // actions_gpu.cu:921-924
-    int pad;
+    int fric_heat;    // actions_gpu.cu:931-934 (set once the workpiece's eta is non-zero)
+    // interactions_gpu.cu:19 / :24: the tool-particle thermal block of the pair sums (device copy,
+    // written only for alpha != 0) and the host copy's alpha read by the PSE launch
+    sphfem_trml_constants trml_itool;
+    double itool_alpha_pse;
 };
 
 // Device-resident tool kinematics (the host bookkeeping of tool_FEM.cu:1552-1570 plus the
@@ -140,6 +144,12 @@ struct sphfem_tool {
     sphfem_tool_kinematics host_kin{};
 };
 
+// rigid (SPH) tool of tool_gpu.cu: the geometry and contact constants in device memory
+namespace sphfem { struct RigidTool; }
+struct sphfem_rigid_tool {
+    sphfem::RigidTool *dev = nullptr;
+};
+
 struct sphfem_leap_frog {
     unsigned int num_part = 0, num_cell = 0;
     int precision = SPHFEM_FP64;
@@ -213,7 +223,8 @@ void launch_leapfrog(sphfem_ctx *ctx, int which, const sphfem_particles &p, sphf
 void launch_interactions_ref(sphfem_ctx *ctx, const sphfem_particles &p, const int *cs, const int *ce);
 void launch_interactions_pse(sphfem_ctx *ctx, const sphfem_particles &p, const int *cs, const int *ce);
 void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfem_particles &A,
-                         sphfem_leap_frog *lf, const int *cs, const int *ce, sphfem_tool *t);
+                         sphfem_leap_frog *lf, const int *cs, const int *ce, sphfem_tool *t,
+                         sphfem_rigid_tool *rt = nullptr);
 void launch_report_force(sphfem_ctx *ctx, const sphfem_particles &p, double *d_out);
 
 // tool (sphfem_tool.hip)
@@ -229,6 +240,12 @@ void launch_fem_robin(sphfem_ctx *ctx, sphfem_tool *t);
 void launch_fem_rate(sphfem_ctx *ctx, sphfem_tool *t);
 void launch_wear_reset(sphfem_ctx *ctx, sphfem_tool *t);
 void launch_near_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t, sphfem_leap_frog *lf);
+
+// rigid tool (sphfem_rigid.hip)
+void launch_rigid_advance(sphfem_ctx *ctx, sphfem_rigid_tool *rt);
+void launch_rigid_tool_particles(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_rigid_tool *rt);
+void launch_rigid_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_rigid_tool *rt);
+void launch_rigid_fric_heat(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_double2 vel);
 
 enum { ACT_EOS, ACT_ARTSTRESS, ACT_JAUMANN, ACT_CONTINUITY, ACT_MOMENTUM, ACT_ADVECTION, ACT_PLASTICITY,
        ACT_BC, ACT_BC_THERMAL, ACT_INVALIDATE };

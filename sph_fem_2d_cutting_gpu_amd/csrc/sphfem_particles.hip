@@ -4,6 +4,7 @@
 // plasticity, the far-from-tool contact reset, boundary conditions and NaN invalidation in one
 // pass over the sorted particles.
 #include "sphfem_physics.h"
+#include "sphfem_rigid.h"
 
 #pragma clang fp contract(off)
 
@@ -183,6 +184,7 @@ struct PairI {            // particle i and the per-i kernel constants
     float fxi, fyi, rowi, hi, vxi, vyi, sxi, syi, szi, Rxi, Ryi, Rzi, rhoi, ci, Ti;
     float h1, fac, c_out_w, c_out_g, c_in_g;
     bool thermal;
+    bool mech;            // i is not a tool particle
 };
 
 struct PairAcc {          // FP32 block sums
@@ -212,7 +214,7 @@ __device__ __forceinline__ void pair_term(const PairI &I, float hf, float inv_wd
     const float wx = g * xij, wy = g * yij;
     const float volj = fabsf(q3.x);
     if (I.thermal && rr2 > 1e-16f) a.T += volj * (I.Ti - q3.w) * g;
-    if (q3.x > 0.f) {                                    // j is not a tool particle
+    if (I.mech && q3.x > 0.f) {                          // neither i nor j is a tool particle
         const float dvx = q1.x - I.vxi, dvy = q1.y - I.vyi;
         const float gx = dvx * volj, gy = dvy * volj;
         a.v0 += gx * wx;
@@ -272,9 +274,12 @@ __global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *_
     double d_v0 = 0., d_v1 = 0., d_v2 = 0., d_v3 = 0.;
     double d_ax = 0., d_ay = 0., d_p0 = 0., d_p1 = 0., d_T = 0.;
     const float4 a3 = rv.r3[i];
-    // tool particles (negative volume) have no sums (interactions_gpu.cu:216; their thermal alpha
-    // is zero); blanked particles are skipped
-    if (blanked[i] != 1. && a3.x > 0.f) {
+    // tool particles (negative volume) sum only the Laplacian of T, and only when the tool's
+    // thermal block of the pair sums is set (interactions_gpu.cu:157-164, 260-262); blanked
+    // particles are skipped
+    const bool tool_i = a3.x < 0.f;
+    const double alpha_i = tool_i ? c.trml_itool.alpha : c.trml_wp.alpha;
+    if (blanked[i] != 1. && (!tool_i || alpha_i != 0.)) {
         const int nx = c.geom.nx, ny = c.geom.ny;
         const int hashi = hash[i];
         const int gi = hashi / ny;
@@ -304,7 +309,8 @@ __global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *_
         I.c_out_w = 0.25f * I.fac;
         I.c_out_g = -0.75f * I.h1 * I.fac;
         I.c_in_g = -3.f * I.h1 * I.h1 * I.fac;
-        I.thermal = c.trml_wp.alpha != 0.;     // the workpiece's alpha (interactions_gpu.cu:19)
+        I.thermal = alpha_i != 0.;
+        I.mech = !tool_i;
         const float fourh2 = 4.f * I.hi * I.hi;
         const float rr2min = (float)c.rr2_min;
         const float hf = (float)c.geom.dx;
@@ -389,8 +395,10 @@ __global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *_
 }
 
 __device__ __forceinline__ double interaction_alpha(const DevConsts &c, double is_tool) {
-    // the tool block of interactions_gpu.cu:19 is never set by the FEM setups (benchmarks_NZ.cpp:354)
-    return (is_tool == 0.) ? c.trml_wp.alpha : 0.;
+    // interactions_gpu.cu:164: the tool block (interactions_gpu.cu:19) is zero unless
+    // interactions_setup_thermal_constants_tool set it (the rigid-tool setup does, benchmarks.cpp:518;
+    // the FEM setups do not, benchmarks_NZ.cpp:354)
+    return (is_tool == 0.) ? c.trml_wp.alpha : c.trml_itool.alpha;
 }
 
 // interactions_gpu.cu:143-379 as one kernel (drop-in entry point)
@@ -420,7 +428,9 @@ __global__ void __launch_bounds__(256) k_interactions_pse(DevConsts c, sphfem_pa
     const int nx = c.geom.nx, ny = c.geom.ny;
     const sphfem_double2 pi = P.pos[pidx];
     const double hi = P.h[pidx], Ti = P.T[pidx];
-    const double alpha = (P.tool_particle[pidx] == 1.) ? c.trml_tool.alpha : c.trml_wp.alpha;
+    // the host copies of interactions_gpu.cu:453 (the tool's is zero unless
+    // interactions_setup_thermal_constants_tool was called)
+    const double alpha = (P.tool_particle[pidx] == 1.) ? c.itool_alpha_pse : c.trml_wp.alpha;
     const int hashi = P.hash[pidx];
     const int gi = hashi / ny;
     const int gj = hashi - gi * ny;
@@ -604,7 +614,10 @@ __global__ void __launch_bounds__(BS) k_pairs_fp64(DevConsts c, sphfem_particles
     psum64[i] = pair_sums_fp64<true>(c, B, i, thermal_alpha, cs, ce, pre_vol, pre_s);
 }
 
-template <bool F32C, int BS>
+// RIGID: the rigid-tool step (leap_frog.cu:99-160): friction heat after the corrector
+// (actions_gpu.cu:316-343), tool particles moved and contact against the moved tool
+// (tool_gpu.cu:146-216) after plasticity, for blanked particles too as in the reference.
+template <bool F32C, int BS, bool RIGID>
 __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particles B, sphfem_particles A,
                                                           const sphfem_double2 *__restrict__ pos_init,
                                                           const sphfem_double2 *__restrict__ vel_init,
@@ -615,7 +628,8 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
                                                           ToolView tv, int write_intermediates,
                                                           const float4 *__restrict__ psum,
                                                           const float *__restrict__ psum_T,
-                                                          const PairSums *__restrict__ psum64) {
+                                                          const PairSums *__restrict__ psum64,
+                                                          const RigidTool *__restrict__ rt) {
     const int i = xcd_logical_block() * BS + threadIdx.x;
     if (i >= (int)B.N) return;
     // FP32C: the reorder wrote only the neighbour records; every per-particle field is gathered here
@@ -741,6 +755,9 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
         if (tv.active) {
             if (!fric_heat_v21(c, tv, *tv.kin, eps_pl, ft, on_seg, nrm, vel, fc, rho, T)) blanked = 1.;
         }
+        if constexpr (RIGID) {
+            if (c.fric_heat && tool == 0.) T = rigid_fric_heat(c, rt->g.vel, vel, ft, nrm, T);
+        }
         if (c.plastic && tool == 0. && blanked != 1.) {
             if (plasticity_jc(c, S, F32C ? pr : B.p[i], eps_pl, eps_pl_dot, T, rho)) blanked = 1.;
         }
@@ -760,6 +777,17 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
                 on_seg = -1;
                 contact_written = true;
             }
+        }
+    } else if constexpr (RIGID) {
+        if (c.fric_heat && tool == 0.) T = rigid_fric_heat(c, rt->g.vel, vel, ft, nrm, T);
+    }
+    if constexpr (RIGID) {
+        if (tool != 0.) {
+            pos.x += c.dt * rt->g.vel.x;
+            pos.y += c.dt * rt->g.vel.y;
+        } else {
+            rigid_contact(*rt, c.dt, pos, vel, ft0, fc, ft, nrm);
+            contact_written = true;
         }
     }
     if (!near) {
@@ -816,7 +844,7 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
 }
 
 void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfem_particles &A, sphfem_leap_frog *lf,
-                         const int *cs, const int *ce, sphfem_tool *t) {
+                         const int *cs, const int *ce, sphfem_tool *t, sphfem_rigid_tool *rt) {
     ToolView tv{};
     if (t) {
         tv.T_nodes = t->T;
@@ -830,6 +858,7 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
         tv.active = 1;
     }
     const int wi = lf->intermediates ? 1 : 0;
+    const RigidTool *rtd = rt ? rt->dev : nullptr;
     if (lf->precision == SPHFEM_FP32C) {
         const size_t n = B.N;
         RecView rv{lf->rec, lf->rec + n + 4, lf->rec + 2 * n + 4, lf->rec + 3 * n + 4};
@@ -848,14 +877,18 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
 #undef SPHFEM_PAIRS
         }
         KTimer tm(ctx, "update_fp32c");
-        if (lf->tuning.update_block == 128)
-            k_fused_update<true, 128><<<nblk(B.N, 128), 128, 0, ctx->stream>>>(
-                ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
-                lf->psum, lf->psum_T, nullptr);
-        else
-            k_fused_update<true, 256><<<nblk(B.N, 256), 256, 0, ctx->stream>>>(
-                ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
-                lf->psum, lf->psum_T, nullptr);
+#define SPHFEM_UPDATE(BSZ, RG)                                                                              \
+    k_fused_update<true, BSZ, RG><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(                                  \
+        ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi, lf->psum, \
+        lf->psum_T, nullptr, rtd)
+        if (lf->tuning.update_block == 128) {
+            if (rtd) SPHFEM_UPDATE(128, true);
+            else SPHFEM_UPDATE(128, false);
+        } else {
+            if (rtd) SPHFEM_UPDATE(256, true);
+            else SPHFEM_UPDATE(256, false);
+        }
+#undef SPHFEM_UPDATE
     } else {
         {
             KTimer tp(ctx, "pairs_fp64");
@@ -867,14 +900,18 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
                                                                             lf->psum64);
         }
         KTimer tm(ctx, "fused_update");
-        if (lf->tuning.update_block == 128)
-            k_fused_update<false, 128><<<nblk(B.N, 128), 128, 0, ctx->stream>>>(
-                ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
-                nullptr, nullptr, lf->psum64);
-        else
-            k_fused_update<false, 256><<<nblk(B.N, 256), 256, 0, ctx->stream>>>(
-                ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi,
-                nullptr, nullptr, lf->psum64);
+#define SPHFEM_UPDATE(BSZ, RG)                                                                              \
+    k_fused_update<false, BSZ, RG><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(                                 \
+        ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi, nullptr, \
+        nullptr, lf->psum64, rtd)
+        if (lf->tuning.update_block == 128) {
+            if (rtd) SPHFEM_UPDATE(128, true);
+            else SPHFEM_UPDATE(128, false);
+        } else {
+            if (rtd) SPHFEM_UPDATE(256, true);
+            else SPHFEM_UPDATE(256, false);
+        }
+#undef SPHFEM_UPDATE
     }
 }
 

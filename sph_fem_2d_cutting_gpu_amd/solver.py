@@ -361,6 +361,143 @@ class tool_FEM:
         return out
 
 
+def _d2(v):
+    return _lib.Double2(float(v[0]), float(v[1]))
+
+
+class tool:
+    """The reference's class tool (tool.h:31-141): a rigid tool on the host, a closed polygon of four
+    segments, or five when a fillet of radius r rounds the cutting edge.  The constructors and
+    methods run the library's host geometry (no device work)."""
+
+    def __init__(self, geometry: _lib.ToolGeometry | None = None):
+        self.g = geometry if geometry is not None else _lib.ToolGeometry()
+
+    @classmethod
+    def from_points(cls, tl, tr, br, bl, r: float = 0.0, mu_fric: float = 0.0) -> "tool":
+        """tool(tl, tr, br, bl[, r], mu_fric) (tool.cpp:316-331)."""
+        t = cls()
+        check(lib().sphfem_tool_geometry_from_points(_d2(tl), _d2(tr), _d2(br), _d2(bl), float(r), float(mu_fric),
+                                                     C.byref(t.g)), None, "tool(points)")
+        return t
+
+    @classmethod
+    def from_angles(cls, tl, length: float, height: float, rake_angle: float, clearance_angle: float,
+                    r: float = 0.0, mu_fric: float = 0.0) -> "tool":
+        """tool(tl, length, height, rake, clearance[, r], mu_fric), angles in degrees (tool.cpp:335-406)."""
+        t = cls()
+        check(lib().sphfem_tool_geometry_from_angles(_d2(tl), float(length), float(height), float(rake_angle),
+                                                     float(clearance_angle), float(r), float(mu_fric),
+                                                     C.byref(t.g)), None, "tool(angles)")
+        return t
+
+    def copy(self) -> "tool":
+        g = _lib.ToolGeometry()
+        C.pointer(g)[0] = self.g
+        return tool(g)
+
+    def update_tool(self, dt: float):
+        check(lib().sphfem_tool_geometry_move(C.byref(self.g), float(dt)), None, "update_tool")
+
+    def set_vel(self, vel):
+        self.g.vel = _d2(vel)
+
+    def get_vel(self):
+        return np.array([self.g.vel.x, self.g.vel.y])
+
+    def mu(self) -> float:
+        return self.g.mu
+
+    def front(self) -> float:
+        return self.g.front
+
+    def low(self) -> float:
+        v = C.c_double()
+        check(lib().sphfem_tool_geometry_low(C.byref(self.g), C.byref(v)), None, "low")
+        return v.value
+
+    def safe_bb(self, safety: float = 0.011) -> dict:
+        b = _lib.BBox()
+        check(lib().sphfem_tool_geometry_safe_bb(C.byref(self.g), float(safety), C.byref(b)), None, "safe_bb")
+        return {f: getattr(b, f) for f, _ in _lib.BBox._fields_}
+
+    def inside(self, pos) -> bool:
+        v = C.c_int()
+        check(lib().sphfem_tool_geometry_inside(C.byref(self.g), _d2(pos), C.byref(v)), None, "inside")
+        return bool(v.value)
+
+    def sample_tool(self, dx: float) -> np.ndarray:
+        """Lattice points inside the tool (tool.cpp:112-135)."""
+        n = C.c_long()
+        check(lib().sphfem_tool_geometry_sample(C.byref(self.g), float(dx), None, 0, C.byref(n)), None, "sample")
+        out = np.zeros((n.value, 2))
+        check(lib().sphfem_tool_geometry_sample(C.byref(self.g), float(dx), out.ctypes.data_as(C.c_void_p),
+                                                n.value, C.byref(n)), None, "sample")
+        return out
+
+    def get_segments(self) -> list:
+        return [dict(left=(s.left.x, s.left.y), right=(s.right.x, s.right.y), a=s.l.a, b=s.l.b,
+                     vertical=bool(s.l.vertical), n=(s.n.x, s.n.y)) for s in self.g.seg[: self.g.num_seg]]
+
+    def get_fillet(self) -> dict | None:
+        if not self.g.has_fillet:
+            return None
+        f = self.g.fillet
+        return dict(r=f.r, t1=f.t1, t2=f.t2, p=(f.p.x, f.p.y))
+
+
+class rigid_tool_gpu:
+    """The device copy of a rigid tool made by tool_gpu_set_up_tool (tool_gpu.cu:219-266)."""
+
+    def __init__(self, ctx: Context, t: tool, contact_alpha: float, phys: dict):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx.call("sphfem_tool_gpu_set_up_tool", C.byref(t.g), float(contact_alpha),
+                 C.byref(_lib.PhysConstants(**phys)), C.byref(self.h))
+
+    def geometry(self) -> tool:
+        """The device geometry as it stands (moved by every step and tool_gpu_update_tool)."""
+        g = _lib.ToolGeometry()
+        self.ctx.call("sphfem_rigid_tool_get_geometry", self.h, C.byref(g))
+        return tool(g)
+
+    def set_vel(self, vel):
+        self.ctx.call("sphfem_rigid_tool_set_vel", self.h, _d2(vel))
+
+    def get_vel(self):
+        return self.geometry().get_vel()
+
+    def destroy(self):
+        if self.h:
+            self.ctx.call("sphfem_rigid_tool_destroy", self.h)
+            self.h = C.c_void_p()
+
+
+def tool_gpu_set_up_tool(ctx: Context, t: tool, alpha: float, phys: dict) -> rigid_tool_gpu:
+    """tool_gpu.h:22: the device copy of t with contact alpha and slave mass phys["mass"]."""
+    return rigid_tool_gpu(ctx, t, alpha, phys)
+
+
+def tool_gpu_update_tool(rt: rigid_tool_gpu, p: "particle_gpu | None"):
+    """tool_gpu.h:25: move the device tool and the tool particles by global_dt * vel."""
+    rt.ctx.call("sphfem_tool_gpu_update_tool", rt.h, C.byref(p.s) if p is not None else None)
+
+
+def compute_contact_forces(p: "particle_gpu", rt: rigid_tool_gpu):
+    """tool_gpu.h:28: penalty contact and friction against the rigid tool."""
+    p.ctx.call("sphfem_compute_contact_forces", C.byref(p.s), rt.h)
+
+
+def material_fric_heat_gen(p: "particle_gpu", vel):
+    """actions_gpu.cu:956-965: friction heat of the rigid-tool contact (needs the workpiece's eta)."""
+    p.ctx.call("sphfem_material_fric_heat_gen", C.byref(p.s), _d2(vel))
+
+
+def interactions_setup_thermal_constants_tool(ctx: Context, trml: dict):
+    """interactions_gpu.cu:494-502: the thermal block of tool particles in the pair sums."""
+    ctx.call("sphfem_interactions_setup_thermal_constants_tool", C.byref(_lib.TrmlConstants(**trml)))
+
+
 class leap_frog:
     """leap_frog(num_part, num_cell, node_num) and step(particles, grid, tool) (leap_frog.h:46-49)."""
 
@@ -372,6 +509,10 @@ class leap_frog:
 
     def step(self, p: particle_gpu, g: grid_base, tool: tool_FEM | None = None):
         self.ctx.call("sphfem_leap_frog_step", self.h, C.byref(p.s), g.h, tool.h if tool else None)
+
+    def step_rigid(self, p: particle_gpu, g: grid_base, rt: rigid_tool_gpu | None = None):
+        """step(particles, grid) with global_tool = rt (leap_frog.cu:99-160)."""
+        self.ctx.call("sphfem_leap_frog_step_rigid", self.h, C.byref(p.s), g.h, rt.h if rt else None)
 
     def set_tuning(self, pair_block: int = 256, pair_chunk: int = 32, update_block: int = 256):
         t = _lib.Tuning(int(pair_block), int(pair_chunk), int(update_block))
@@ -386,6 +527,13 @@ class leap_frog:
         """Record `steps` (even) steps into a HIP graph; returns a StepGraph.  Needs one eager step first."""
         gh = C.c_void_p()
         self.ctx.call("sphfem_leap_frog_graph_create", self.h, C.byref(p.s), g.h, tool.h if tool else None,
+                      int(steps), C.byref(gh))
+        return StepGraph(self.ctx, gh, steps)
+
+    def capture_rigid(self, p: particle_gpu, g: grid_base, rt: rigid_tool_gpu | None = None, steps: int = 2):
+        """capture() for step_rigid."""
+        gh = C.c_void_p()
+        self.ctx.call("sphfem_leap_frog_graph_create_rigid", self.h, C.byref(p.s), g.h, rt.h if rt else None,
                       int(steps), C.byref(gh))
         return StepGraph(self.ctx, gh, steps)
 
@@ -479,21 +627,33 @@ class Simulation:
         rr2 = cases.default_rr2_min(case.dx) if rr2_min is None else rr2_min
         self.ctx.setup_constants(case.const, case.dt, rr2, case.wear if wear is None else wear)
         init = cases.initial_particles(case)
-        self.p = particle_gpu(self.ctx, case.N, **{k: init[k] for k in ("pos", "vel", "rho", "T", "h", "fixed", "S")})
+        self.p = particle_gpu(self.ctx, case.N, **{k: init[k] for k in ("pos", "vel", "rho", "T", "h", "fixed", "S",
+                                                                         "tool_particle")})
         if max_cell:
             self.grid = GRIDS[grid].unlocked(self.ctx, max_cell, case.N)
         else:
             self.grid = GRIDS[grid](self.ctx, case.N, case.grid_bbmin, case.grid_bbmax, case.h)
         self.tool = tool_FEM(self.ctx, case) if case.tool is not None else None
+        # a rigid tool (benchmarks.cpp:409-415, 518): device copy and the pair sums' tool block
+        self.rtool = None
+        if case.rigid is not None:
+            interactions_setup_thermal_constants_tool(self.ctx, case.rigid.trml_itool)
+            self.rtool = tool_gpu_set_up_tool(self.ctx, tool(case.rigid.geometry), case.rigid.contact_alpha,
+                                              case.const.phys)
         self.stepper = leap_frog(self.ctx, case.N, self.grid.num_cell,
                                  self.tool.n_nodes if self.tool else 0, precision)
 
     def step(self, n: int = 1):
         for _ in range(n):
-            self.stepper.step(self.p, self.grid, self.tool)
+            if self.rtool is not None:
+                self.stepper.step_rigid(self.p, self.grid, self.rtool)
+            else:
+                self.stepper.step(self.p, self.grid, self.tool)
 
     def capture(self, steps: int = 2) -> StepGraph:
         """Graph of `steps` steps (one eager step must have run)."""
+        if self.rtool is not None:
+            return self.stepper.capture_rigid(self.p, self.grid, self.rtool, steps)
         return self.stepper.capture(self.p, self.grid, self.tool, steps)
 
     def set_tuning(self, **kw):
@@ -517,6 +677,8 @@ class Simulation:
         self.ctx.call("sphfem_leap_frog_destroy", self.stepper.h)
         if self.tool is not None:
             self.ctx.call("sphfem_tool_destroy", self.tool.h)
+        if self.rtool is not None:
+            self.rtool.destroy()
         self.ctx.call("sphfem_grid_destroy", self.grid.h)
         self.p.free()
         self.ctx.close()

@@ -283,13 +283,19 @@ def test_wear_stage(world):
         1e-12, abs(o.tool_scalars["flank_contact"]))
 
 
-def test_heat_pse_tool_particles_and_blanked():
-    """Tool particles take the tool's diffusivity, blanked particles keep their rate; lattice whose
-    grid cells are 2.5 h wide (the window then covers the Gaussian's support)."""
+@pytest.mark.parametrize("tool_block", [False, True])
+def test_heat_pse_tool_particles_and_blanked(tool_block):
+    """Tool particles take the tool diffusivity of interactions_setup_thermal_constants_tool (zero
+    until it is called), blanked particles keep their rate; lattice whose grid cells are 2.5 h wide
+    (the window then covers the Gaussian's support)."""
     from test_oracle_pins import _pse_lattice
     o, hp, inner = _pse_lattice(n=40)
     N = o.case.N
     sim = S.Simulation(o.case)
+    if tool_block:
+        trml = o.case.const.trml_tool
+        S.interactions_setup_thermal_constants_tool(sim.ctx, trml)
+        o.c.itool_alpha_pse = trml["alpha"]
     rng = np.random.default_rng(8)
     x = o.P["pos"] - o.P["pos"].mean(axis=0)
     o.P["T"][:] = 300.0 + (x[:, 0] ** 2 + x[:, 1] ** 2) / hp ** 2 + rng.normal(0, 1.0, N)
@@ -299,6 +305,8 @@ def test_heat_pse_tool_particles_and_blanked():
     _run_unfused_from(sim, o, "heat_pse")
     g = sim.p.download("T_t")
     blank = o.P["blanked"] == 1.0
+    tool = o.P["tool_particle"] == 1.0
     assert np.all(g[blank] == -7.0)
+    assert np.any(g[tool & ~blank] != 0.0) == tool_block
     assert_close("T_t", g, o.P["T_t"], PSE_RTOL, mask=~blank)
     sim.close()

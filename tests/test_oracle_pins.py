@@ -288,8 +288,9 @@ def test_pse_laplacian_closed_forms():
 
 
 def test_pse_tool_alpha_blanked_and_off_switch():
-    """Tool particles use the tool's diffusivity, blanked particles keep their rate, alpha_wp = 0 skips
-    the call (interactions_gpu.cu:436, 486)."""
+    """Tool particles use the tool diffusivity of interactions_setup_thermal_constants_tool (zero until
+    it is called), blanked particles keep their rate, alpha_wp = 0 skips the call
+    (interactions_gpu.cu:436, 453, 486, 494-502)."""
     o, hp, inner = _pse_lattice(n=32, cell_over_h=1.0)
     x = o.P["pos"] - o.P["pos"].mean(axis=0)
     o.P["T"][:] = 300.0 + (x[:, 0] ** 2 + x[:, 1] ** 2) / hp ** 2
@@ -304,9 +305,13 @@ def test_pse_tool_alpha_blanked_and_off_switch():
     o.P["blanked"][:] = blank
     o.P["T_t"][:] = -7.0
     o.call("o_interactions_heat_pse", "cps")
-    aw, at = o.case.const.trml_wp["alpha"], o.case.const.trml_tool["alpha"]
-    assert np.all(o.P["T_t"][blank] == -7.0)
     live = ~blank
+    assert np.all(o.P["T_t"][tool & live] == 0.0)        # the tool block was never set up
+    aw, at = o.case.const.trml_wp["alpha"], o.case.const.trml_tool["alpha"]
+    o.c.itool_alpha_pse = at
+    o.P["T_t"][:] = -7.0
+    o.call("o_interactions_heat_pse", "cps")
+    assert np.all(o.P["T_t"][blank] == -7.0)
     want = np.where(tool, base / aw * at, base)
     # blanked neighbours still contribute (the reference skips only blanked i)
     assert np.allclose(o.P["T_t"][live], want[live], rtol=1e-14, atol=0.0)
