@@ -52,6 +52,12 @@ def make_case(name):
         return cases.config2(), "aisi1045_160k_cutting: 800x200 lattice dx=1.25um, ~8k-triangle FEM tool"
     if name == "ti64_6400":
         return cases.config1(), "ti64_6400_cutting: 160x40 lattice dx=2.5um, ~2k-triangle FEM tool"
+    if name.startswith("ref_cut_sph_ny"):
+        ny = int(name[len("ref_cut_sph_ny"):])
+        c = cases.ref_cut_sph(ny=ny)
+        return c, (f"ref_cut_sph_ny{ny}: setup_ref_cut rigid tool (rake 0, clearance 4.5 deg, fillet 3.5 um, "
+                   f"feed 100 um, 750 m/min), {c.extra['n_workpiece']} workpiece + "
+                   f"{c.N - c.extra['n_workpiece']} tool particles, dx={c.dx * 1e4:.3f}um")
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -249,8 +255,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32 pair sums (FP64 block accumulation) + f64 state",
         "data": "synthetic (seeded jittered lattice and generated tool mesh)",
-        "config": {"workload": workload, "particles": case.N, "tool_triangles": int(len(case.tool.elems)),
-                   "cutting_segments": int(len(case.tool.seg_left)), "wear": bool(case.wear),
+        "config": {"workload": workload, "particles": case.N,
+                   "tool_triangles": int(len(case.tool.elems)) if case.tool is not None else 0,
+                   "cutting_segments": int(len(case.tool.seg_left)) if case.tool is not None else 0,
+                   "rigid_tool": case.rigid is not None, "wear": bool(case.wear),
                    "precision": args.precision, "parallelism": "single GPU", "step_graph": not args.eager,
                    "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
