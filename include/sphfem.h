@@ -334,6 +334,49 @@ int sphfem_leap_frog_graph_destroy(sphfem_ctx *ctx, sphfem_graph *graph);
 /* ---- observables (output.cu:22-30) */
 int sphfem_report_force(sphfem_ctx *ctx, sphfem_particles *p, sphfem_double2 *out);
 
+/* ---- output formats (vtk_writer.cu, tool_wear.cu, mesh_structure_GPU.cu:1259-1296).
+ * Legacy ASCII VTK with the reference's headers, field names, order, scaling and printf
+ * formats (%f, %d), text identical to what vtk_writer.cu writes for the same values. */
+/* vtk_writer_write(particles, step) (vtk_writer.cu:11-219) to `path`: points, vertex cells,
+ * Temperature, on_seg, EquivAccumPlasticStrain, Velocity, fc and ft (x 1e5), normal, SvM.  The
+ * reference never downloads `blanked` and so writes every particle; skip_blanked = 1 leaves
+ * blanked particles out (cells then index the written points).  Downloads and synchronises. */
+int sphfem_vtk_write_particles(sphfem_ctx *ctx, const sphfem_particles *p, const char *path, int skip_blanked);
+/* the same writer over host arrays laid out as the device ones (no device access; the fields
+ * read are pos, vel, p, S, fc, ft, n, blanked, eps_pl, T, on_seg) */
+int sphfem_vtk_write_particles_host(const sphfem_particles *host, const char *path, int skip_blanked);
+/* vtk_writer_write(tool, step) (vtk_writer.cu:222-302): the rigid tool as triangles (two or
+ * three for the body, a 19-triangle fan for the fillet); nothing is written for a tool without
+ * segments.  Host only. */
+int sphfem_vtk_write_rigid_tool(const sphfem_tool_geometry *g, const char *path);
+/* vtk_writer_write_FEM_tool (vtk_writer.cu:304-366): mesh and nodal temperature */
+int sphfem_vtk_write_FEM_tool(sphfem_ctx *ctx, sphfem_tool *t, const char *path);
+/* vtk_writer_write_tool_org (vtk_writer.cu:518-587): the mesh shifted back by tl - tl_ref */
+int sphfem_vtk_write_tool_org(sphfem_ctx *ctx, sphfem_tool *t, sphfem_double2 tl_ref, const char *path);
+/* vtk_writer_write_FEM_segments (vtk_writer.cu:368-516): one triangle per cutting segment with
+ * contact pressure (GPa), sliding velocity (m/s), temperature, frictional stress and wear rate */
+int sphfem_vtk_write_FEM_segments(sphfem_ctx *ctx, sphfem_tool *t, const char *path);
+/* vtk_writer_write_contact_variables (vtk_writer.cu:589-714): segment midpoints with the same
+ * variables; as in the reference, Wear_rate carries three values per point */
+int sphfem_vtk_write_contact_variables(sphfem_ctx *ctx, sphfem_tool *t, const char *path);
+/* mesh_read::mesh_read_temperature (mesh_structure_GPU.cu:1259-1296): the numbers on every
+ * non-empty line after the first "LOOKUP_TABLE default" line of a VTK file (a file written by
+ * sphfem_vtk_write_FEM_tool).  Writes at most cap values to T (may be NULL) and the count to
+ * *count; SPHFEM_ERR_ARG if a line is not a number (the reference throws). */
+int sphfem_read_vtk_temperature(const char *path, double *T, long cap, long *count);
+
+/* tool_wear (tool_wear.h:17-47, tool_wear.cu:14-72): Usui wear rate K exp(alpha / T) p |v_r| of
+ * the particles in contact (p = |fc| / (2 pi h)), min / max / mean over them, and the wear
+ * accumulated as dt * mean per call.  Evaluated by a deterministic device reduction. */
+typedef struct sphfem_tool_wear sphfem_tool_wear;
+int sphfem_tool_wear_create(sphfem_ctx *ctx, double usui_K, double usui_alpha, sphfem_double2 tool_vel,
+                            sphfem_tool_wear **out);
+int sphfem_tool_wear_destroy(sphfem_ctx *ctx, sphfem_tool_wear *w);
+/* tool_wear::eval_usui; blocks until the result is on the host */
+int sphfem_tool_wear_eval_usui(sphfem_ctx *ctx, sphfem_tool_wear *w, const sphfem_particles *p, double dt,
+                               double *usui_min, double *usui_max, double *usui_avg);
+int sphfem_tool_wear_get_accum_wear(const sphfem_tool_wear *w, double *accum);
+
 /* ---- measurement: per-kernel HIP-event timing on the context stream */
 int sphfem_timing_enable(sphfem_ctx *ctx, int enable);
 int sphfem_timing_reset(sphfem_ctx *ctx);

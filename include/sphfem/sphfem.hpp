@@ -11,7 +11,9 @@
  *   leap_frog           leap_frog.h:31-50 (step(particles, grid[, tool_FEM]))
  *   tool / tool_FEM     tool.h:31-141 (rigid tool: constructors, geometry), tool_gpu.h:19-28,
  *                       tool_FEM.h:34-83 (per-step methods)
- *   free functions      actions_gpu.h:33-72, interactions_gpu.h:27-43, output.h:17
+ *   free functions      actions_gpu.h:33-72, interactions_gpu.h:27-43, output.h:17,
+ *                       vtk_writer.h:26-31, mesh_read_temperature
+ *   tool_wear           tool_wear.h:17-47
  *   global_dt, wear_pro benchmarks.cpp:12, mfree_gpu.cpp:47
  *
  * Differences a caller sees: errors throw std::runtime_error (the reference prints and
@@ -23,6 +25,7 @@
 #ifndef SPHFEM_HPP
 #define SPHFEM_HPP
 
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -570,6 +573,84 @@ inline float2_t report_force(particle_gpu *p) {
     SPHFEM_SHIM_CALL(sphfem_report_force(sphfem_shim::ctx(), p, &f), "report_force");
     return f;
 }
+
+// vtk_writer.h:26-31: files under results/ named as the reference names them
+namespace sphfem_shim {
+inline std::string vtk_path(const char *stem, int step) {
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "results/%s_%06d.vtk", stem, step);
+    return buf;
+}
+}  // namespace sphfem_shim
+inline void vtk_writer_write(const particle_gpu *particles, int step) {
+    SPHFEM_SHIM_CALL(sphfem_vtk_write_particles(sphfem_shim::ctx(), particles,
+                                                sphfem_shim::vtk_path("vtk_out", step).c_str(), 0),
+                     "vtk_writer_write(particles)");
+}
+inline void vtk_writer_write(const tool *t, int step) {
+    sphfem_tool_geometry g = t->geometry();
+    if (t->gpu_handle())   // the device copy is the one the steps move
+        SPHFEM_SHIM_CALL(sphfem_rigid_tool_get_geometry(sphfem_shim::ctx(), t->gpu_handle(), &g), "tool geometry");
+    SPHFEM_SHIM_CALL(sphfem_vtk_write_rigid_tool(&g, sphfem_shim::vtk_path("vtk_tool", step).c_str()),
+                     "vtk_writer_write(tool)");
+}
+inline void vtk_writer_write_FEM_tool(const tool_FEM *t, int step) {
+    SPHFEM_SHIM_CALL(sphfem_vtk_write_FEM_tool(sphfem_shim::ctx(), t->handle(),
+                                               sphfem_shim::vtk_path("vtk_tool", step).c_str()),
+                     "vtk_writer_write_FEM_tool");
+}
+inline void vtk_writer_write_FEM_segments(const tool_FEM *t, int step) {
+    SPHFEM_SHIM_CALL(sphfem_vtk_write_FEM_segments(sphfem_shim::ctx(), t->handle(),
+                                                   sphfem_shim::vtk_path("vtk_seg", step).c_str()),
+                     "vtk_writer_write_FEM_segments");
+}
+inline void vtk_writer_write_tool_org(const tool_FEM *t, int step, float2_t tl_ref) {
+    SPHFEM_SHIM_CALL(sphfem_vtk_write_tool_org(sphfem_shim::ctx(), t->handle(), tl_ref,
+                                               sphfem_shim::vtk_path("vtk_tool_org", step).c_str()),
+                     "vtk_writer_write_tool_org");
+}
+inline void vtk_writer_write_contact_variables(const tool_FEM *t, int step) {
+    SPHFEM_SHIM_CALL(sphfem_vtk_write_contact_variables(sphfem_shim::ctx(), t->handle(),
+                                                        sphfem_shim::vtk_path("vtk_phy_wear", step).c_str()),
+                     "vtk_writer_write_contact_variables");
+}
+// mesh_read::mesh_read_temperature (mesh_structure_GPU.cu:1259-1296): the nodal temperatures of a
+// hot-tool restart file, for sphfem_tool_desc::T
+inline std::vector<double> mesh_read_temperature(const std::string &file_name) {
+    long n = 0;
+    SPHFEM_SHIM_CALL(sphfem_read_vtk_temperature(file_name.c_str(), nullptr, 0, &n), "mesh_read_temperature");
+    std::vector<double> T((size_t)n);
+    SPHFEM_SHIM_CALL(sphfem_read_vtk_temperature(file_name.c_str(), T.data(), n, &n), "mesh_read_temperature");
+    return T;
+}
+
+// tool_wear.h:17-47
+class tool_wear {
+  public:
+    tool_wear(double usui_K, double usui_alpha, unsigned int, phys_constants, vec2_t tool_vel) {
+        SPHFEM_SHIM_CALL(sphfem_tool_wear_create(sphfem_shim::ctx(), usui_K, usui_alpha,
+                                                 sphfem_double2{tool_vel.x, tool_vel.y}, &m_handle),
+                         "tool_wear");
+    }
+    ~tool_wear() {
+        if (m_handle && sphfem_shim::ctx_slot()) (void)sphfem_tool_wear_destroy(sphfem_shim::ctx_slot(), m_handle);
+    }
+    tool_wear(const tool_wear &) = delete;
+    tool_wear &operator=(const tool_wear &) = delete;
+    void eval_usui(const particle_gpu *particles, double dt, double &usui_min, double &usui_max, double &usui_avg) {
+        SPHFEM_SHIM_CALL(sphfem_tool_wear_eval_usui(sphfem_shim::ctx(), m_handle, particles, dt, &usui_min,
+                                                    &usui_max, &usui_avg),
+                         "tool_wear::eval_usui");
+    }
+    double get_accum_wear() const {
+        double v = 0.;
+        SPHFEM_SHIM_CALL(sphfem_tool_wear_get_accum_wear(m_handle, &v), "tool_wear::get_accum_wear");
+        return v;
+    }
+
+  private:
+    sphfem_tool_wear *m_handle = nullptr;
+};
 
 // types.cpp:11-37
 inline bool check_cuda_error() { return sphfem_check_error(sphfem_shim::ctx()) == SPHFEM_OK; }

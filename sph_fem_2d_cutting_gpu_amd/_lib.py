@@ -171,6 +171,18 @@ _SIGS = {
     "sphfem_compute_contact_forces": (_ctx, _P(Particles), _vp),
     "sphfem_material_fric_heat_gen": (_ctx, _P(Particles), Double2),
     "sphfem_interactions_setup_thermal_constants_tool": (_ctx, _P(TrmlConstants)),
+    "sphfem_vtk_write_particles": (_ctx, _P(Particles), C.c_char_p, C.c_int),
+    "sphfem_vtk_write_particles_host": (_P(Particles), C.c_char_p, C.c_int),
+    "sphfem_vtk_write_rigid_tool": (_P(ToolGeometry), C.c_char_p),
+    "sphfem_vtk_write_FEM_tool": (_ctx, _vp, C.c_char_p),
+    "sphfem_vtk_write_tool_org": (_ctx, _vp, Double2, C.c_char_p),
+    "sphfem_vtk_write_FEM_segments": (_ctx, _vp, C.c_char_p),
+    "sphfem_vtk_write_contact_variables": (_ctx, _vp, C.c_char_p),
+    "sphfem_read_vtk_temperature": (C.c_char_p, _vp, C.c_long, _P(C.c_long)),
+    "sphfem_tool_wear_create": (_ctx, _dbl, _dbl, Double2, _P(_vp)),
+    "sphfem_tool_wear_destroy": (_ctx, _vp),
+    "sphfem_tool_wear_eval_usui": (_ctx, _vp, _P(Particles), _dbl, _P(_dbl), _P(_dbl), _P(_dbl)),
+    "sphfem_tool_wear_get_accum_wear": (_vp, _P(_dbl)),
     "sphfem_report_force": (_ctx, _P(Particles), _P(Double2)),
     "sphfem_timing_enable": (_ctx, C.c_int),
     "sphfem_timing_reset": (_ctx,),

@@ -1455,6 +1455,188 @@ int sphfem_leap_frog_graph_destroy(sphfem_ctx *ctx, sphfem_graph *gr) {
     });
 }
 
+// ---- output formats
+}  // extern "C"
+
+struct sphfem_tool_wear {
+    double K = 0., alpha = 0.;
+    sphfem_double2 vel{0., 0.};
+    double accum = 0.;
+    double *d_buf = nullptr;    // [4 + 4 * 256]
+};
+
+namespace {
+
+template <class T>
+std::vector<T> download_vec(hipStream_t s, const T *dev, size_t n) {
+    std::vector<T> v(n);
+    if (n) SPHFEM_HIP_CHECK(hipMemcpyAsync(v.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    return v;
+}
+
+FemToolHost download_fem_tool(sphfem_ctx *ctx, sphfem_tool *t, bool mesh, bool segments) {
+    FemToolHost h;
+    h.n_nodes = t->n_nodes;
+    h.n_elem = t->n_elem;
+    h.n_seg = t->n_seg;
+    hipStream_t s = ctx->stream;
+    if (mesh) {
+        h.pos = download_vec(s, t->pos, (size_t)t->n_nodes);
+        h.elem = download_vec(s, t->elem, (size_t)t->n_elem);
+        h.T = download_vec(s, t->T, (size_t)t->n_nodes);
+    }
+    if (segments) {
+        h.left = download_vec(s, t->left, (size_t)t->n_seg);
+        h.right = download_vec(s, t->right, (size_t)t->n_seg);
+        h.n = download_vec(s, t->n, (size_t)t->n_seg);
+        h.physical_para = download_vec(s, t->physical_para, (size_t)t->n_seg);
+        h.sliding_force = download_vec(s, t->sliding_force, (size_t)t->n_seg);
+        h.wear_nodes = download_vec(s, t->wear_nodes, (size_t)t->n_seg);
+    }
+    SPHFEM_HIP_CHECK(hipStreamSynchronize(s));
+    return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sphfem_vtk_write_particles_host(const sphfem_particles *host, const char *path, int skip_blanked) {
+    if (!host || !path) return SPHFEM_ERR_ARG;
+    std::string err;
+    try {
+        return write_particles_vtk(*host, path, skip_blanked != 0, err);
+    } catch (const std::exception &) {
+        return SPHFEM_ERR_ARG;
+    }
+}
+
+int sphfem_vtk_write_particles(sphfem_ctx *ctx, const sphfem_particles *p, const char *path, int skip_blanked) {
+    if (!p || !path) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        const size_t N = p->N;
+        hipStream_t s = ctx->stream;
+        auto pos = download_vec(s, p->pos, N), vel = download_vec(s, p->vel, N);
+        auto fc = download_vec(s, p->fc, N), ft = download_vec(s, p->ft, N), nn = download_vec(s, p->n, N);
+        auto S = download_vec(s, p->S, N);
+        auto pr = download_vec(s, p->p, N), bl = download_vec(s, p->blanked, N), eps = download_vec(s, p->eps_pl, N);
+        auto T = download_vec(s, p->T, N);
+        auto on_seg = download_vec(s, p->on_seg, N);
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(s));
+        sphfem_particles h{};
+        h.N = p->N;
+        h.pos = pos.data(); h.vel = vel.data(); h.fc = fc.data(); h.ft = ft.data(); h.n = nn.data();
+        h.S = S.data(); h.p = pr.data(); h.blanked = bl.data(); h.eps_pl = eps.data(); h.T = T.data();
+        h.on_seg = on_seg.data();
+        return write_particles_vtk(h, path, skip_blanked != 0, ctx->err);
+    });
+}
+
+int sphfem_vtk_write_rigid_tool(const sphfem_tool_geometry *g, const char *path) {
+    if (!g || !path) return SPHFEM_ERR_ARG;
+    std::string err;
+    return write_rigid_tool_vtk(*g, path, err);
+}
+
+int sphfem_vtk_write_FEM_tool(sphfem_ctx *ctx, sphfem_tool *t, const char *path) {
+    if (!t || !path) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        FemToolHost h = download_fem_tool(ctx, t, true, false);
+        return write_fem_tool_vtk(h, sphfem_double2{0., 0.}, path, ctx->err);
+    });
+}
+
+int sphfem_vtk_write_tool_org(sphfem_ctx *ctx, sphfem_tool *t, sphfem_double2 tl_ref, const char *path) {
+    if (!t || !path) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        FemToolHost h = download_fem_tool(ctx, t, true, false);
+        sphfem_tool_kinematics k;
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(&k, t->kin, sizeof(k), hipMemcpyDeviceToHost, ctx->stream));
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return write_fem_tool_vtk(h, sphfem_double2{k.tl_x - tl_ref.x, k.tl_y - tl_ref.y}, path, ctx->err);
+    });
+}
+
+int sphfem_vtk_write_FEM_segments(sphfem_ctx *ctx, sphfem_tool *t, const char *path) {
+    if (!t || !path) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        FemToolHost h = download_fem_tool(ctx, t, false, true);
+        return write_fem_segments_vtk(h, path, ctx->err);
+    });
+}
+
+int sphfem_vtk_write_contact_variables(sphfem_ctx *ctx, sphfem_tool *t, const char *path) {
+    if (!t || !path) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        FemToolHost h = download_fem_tool(ctx, t, false, true);
+        return write_contact_variables_vtk(h, path, ctx->err);
+    });
+}
+
+int sphfem_read_vtk_temperature(const char *path, double *T, long cap, long *count) {
+    if (!path || !count || cap < 0) return SPHFEM_ERR_ARG;
+    std::vector<double> v;
+    std::string err;
+    if (int rc = read_vtk_temperature(path, v, err)) return rc;
+    for (long i = 0; T && i < cap && i < (long)v.size(); i++) T[i] = v[(size_t)i];
+    *count = (long)v.size();
+    return SPHFEM_OK;
+}
+
+int sphfem_tool_wear_create(sphfem_ctx *ctx, double usui_K, double usui_alpha, sphfem_double2 tool_vel,
+                            sphfem_tool_wear **out) {
+    if (!out) return SPHFEM_ERR_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        auto *w = new sphfem_tool_wear();
+        w->K = usui_K;
+        w->alpha = usui_alpha;
+        w->vel = tool_vel;
+        try {
+            w->d_buf = dalloc<double>(4 + 4 * 256);
+        } catch (...) {
+            delete w;
+            throw;
+        }
+        *out = w;
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_tool_wear_destroy(sphfem_ctx *ctx, sphfem_tool_wear *w) {
+    if (!w) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        dfree(w->d_buf);
+        delete w;
+        return SPHFEM_OK;
+    });
+}
+
+// tool_wear.cu:14-57
+int sphfem_tool_wear_eval_usui(sphfem_ctx *ctx, sphfem_tool_wear *w, const sphfem_particles *p, double dt,
+                               double *usui_min, double *usui_max, double *usui_avg) {
+    if (!w || !p || !usui_min || !usui_max || !usui_avg) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        launch_usui(ctx, *p, w->K, w->alpha, w->vel, w->d_buf);
+        double r[4];
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(r, w->d_buf, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        const double n = r[3];
+        *usui_min = (n == 0.) ? 0. : r[1];
+        *usui_max = r[2];
+        *usui_avg = (n == 0.) ? 0. : r[0] / n;
+        w->accum += dt * *usui_avg;
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_tool_wear_get_accum_wear(const sphfem_tool_wear *w, double *accum) {
+    if (!w || !accum) return SPHFEM_ERR_ARG;
+    *accum = w->accum;
+    return SPHFEM_OK;
+}
+
 int sphfem_report_force(sphfem_ctx *ctx, sphfem_particles *p, sphfem_double2 *out) {
     if (!p || !out) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {

@@ -246,6 +246,24 @@ void launch_rigid_advance(sphfem_ctx *ctx, sphfem_rigid_tool *rt);
 void launch_rigid_tool_particles(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_rigid_tool *rt);
 void launch_rigid_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_rigid_tool *rt);
 void launch_rigid_fric_heat(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_double2 vel);
+// d_out: [4 + 1024] doubles
+void launch_usui(sphfem_ctx *ctx, const sphfem_particles &p, double K, double alpha, sphfem_double2 vel,
+                 double *d_out);
+
+// output formats (sphfem_output.cpp): host writers; `err` receives the message of a failure
+int write_particles_vtk(const sphfem_particles &host, const char *path, bool skip_blanked, std::string &err);
+int write_rigid_tool_vtk(const sphfem_tool_geometry &g, const char *path, std::string &err);
+struct FemToolHost {
+    int n_nodes = 0, n_elem = 0, n_seg = 0;
+    std::vector<sphfem_double2> pos, left, right, n;
+    std::vector<sphfem_int4> elem;
+    std::vector<double> T, sliding_force, wear_nodes;
+    std::vector<sphfem_double4> physical_para;
+};
+int write_fem_tool_vtk(const FemToolHost &t, sphfem_double2 shift, const char *path, std::string &err);
+int write_fem_segments_vtk(const FemToolHost &t, const char *path, std::string &err);
+int write_contact_variables_vtk(const FemToolHost &t, const char *path, std::string &err);
+int read_vtk_temperature(const char *path, std::vector<double> &T, std::string &err);
 
 enum { ACT_EOS, ACT_ARTSTRESS, ACT_JAUMANN, ACT_CONTINUITY, ACT_MOMENTUM, ACT_ADVECTION, ACT_PLASTICITY,
        ACT_BC, ACT_BC_THERMAL, ACT_INVALIDATE };

@@ -7,6 +7,7 @@
 // segment left ends).
 //
 // usage: ref_cut_driver NY STEPS FREQ fp64|fp32c OUT_DIR [rake clearance fillet speed_in_ui feed]
+// With REF_CUT_RESULTS=1 it also writes the reference's results/ files every FREQ steps.
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -39,7 +40,7 @@ struct Setup {
     tool *t = nullptr;
 };
 
-// benchmarks.cpp:248-530 (the Usui wear logger of :503-507 is not part of the step and is left out)
+// benchmarks.cpp:248-530 (the Usui wear logger of :503-507 is created by main)
 Setup setup_ref_cut(int ny, double rake, double clear, double chamfer, double speed, double feed) {
     phys_constants phys{};
     corr_constants corr{};
@@ -228,13 +229,43 @@ int main(int argc, char **argv) {
         if (prec == "fp32c") stepper.set_precision(SPHFEM_FP32C);
         std::FILE *ff = std::fopen((out + "/forces.txt").c_str(), "w");
         if (!ff) throw std::runtime_error("cannot write " + out + "/forces.txt");
+        // mfree_gpu.cpp:269-347: dumps every freq steps into results/ (VTK of particles and tool,
+        // force and wear logs) when REF_CUT_RESULTS=1; the Usui logger as benchmarks.cpp:503-507 sets it
+        // up (tool velocity (0, v) as there)
+        const char *res_env = std::getenv("REF_CUT_RESULTS");
+        const bool results = res_env && std::string(res_env) == "1";
+        tool_wear wear(1. / 100 * 7.8, 2500., N, phys_constants{}, vec2_t(0., speed));
+        std::FILE *fp_force = nullptr, *fp_wear = nullptr;
+        if (results) {
+            if (std::system("mkdir -p results") != 0) throw std::runtime_error("cannot create results/");
+            char buf[256];
+            std::snprintf(buf, sizeof(buf), "results/force_%f_%f", fillet, feed);
+            fp_force = std::fopen(buf, "w+");
+            std::snprintf(buf, sizeof(buf), "results/wear_%f_%f", speed, feed);
+            fp_wear = std::fopen(buf, "w+");
+            if (!fp_force || !fp_wear) throw std::runtime_error("cannot open the log files in results/");
+        }
+        int print_iter = 0;
         for (long step = 0; step < steps; step++) {
             if (step % freq == 0) {
                 float2_t F = report_force(&particles);
                 std::fprintf(ff, "%ld %.17g %.17g\n", step, F.x, F.y);
+                if (results) {
+                    vtk_writer_write(&particles, print_iter);
+                    vtk_writer_write(global_tool, print_iter);
+                    std::fprintf(fp_force, "%.9g %.9g\n", F.x, F.y);
+                    std::fflush(fp_force);
+                    double wmin, wmax, wavg;
+                    wear.eval_usui(&particles, freq * global_dt, wmin, wmax, wavg);
+                    std::fprintf(fp_wear, "%e %e %e %e\n", wmin, wmax, wavg, wear.get_accum_wear());
+                    std::fflush(fp_wear);
+                }
+                print_iter++;
             }
             stepper.step(&particles, su.grid);
         }
+        if (fp_force) std::fclose(fp_force);
+        if (fp_wear) std::fclose(fp_wear);
         float2_t F = report_force(&particles);
         std::fprintf(ff, "%ld %.17g %.17g\n", steps, F.x, F.y);
         std::fclose(ff);

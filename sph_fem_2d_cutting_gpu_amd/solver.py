@@ -615,6 +615,92 @@ def report_force(p: particle_gpu):
     return np.array([out.x, out.y])
 
 
+# ---- vtk_writer.h:26-31, tool_wear.h, mesh_read_temperature: output formats
+def _vtk_path(folder, stem, step):
+    import os
+    return os.path.join(folder, f"{stem}_{int(step):06d}.vtk")
+
+
+def vtk_writer_write(obj, step: int, folder: str = "results", skip_blanked: bool = False, path: str | None = None):
+    """vtk_writer_write(particles, step) -> results/vtk_out_%06d.vtk (vtk_writer.cu:11-219), or, for a
+    rigid tool (host `tool` or its device copy), vtk_writer_write(tool, step) -> results/vtk_tool_%06d.vtk
+    (vtk_writer.cu:222-302).  Returns the path written."""
+    if isinstance(obj, particle_gpu):
+        path = path or _vtk_path(folder, "vtk_out", step)
+        obj.ctx.call("sphfem_vtk_write_particles", C.byref(obj.s), path.encode(), int(bool(skip_blanked)))
+        return path
+    t = obj.geometry() if isinstance(obj, rigid_tool_gpu) else obj
+    path = path or _vtk_path(folder, "vtk_tool", step)
+    check(lib().sphfem_vtk_write_rigid_tool(C.byref(t.g), path.encode()), None, "vtk_writer_write(tool)")
+    return path
+
+
+def vtk_writer_write_FEM_tool(tool: tool_FEM, step: int, folder: str = "results", path: str | None = None):
+    """vtk_writer.cu:304-366 -> results/vtk_tool_%06d.vtk"""
+    path = path or _vtk_path(folder, "vtk_tool", step)
+    tool.ctx.call("sphfem_vtk_write_FEM_tool", tool.h, path.encode())
+    return path
+
+
+def vtk_writer_write_tool_org(tool: tool_FEM, step: int, tl_ref, folder: str = "results", path: str | None = None):
+    """vtk_writer.cu:518-587 -> results/vtk_tool_org_%06d.vtk"""
+    path = path or _vtk_path(folder, "vtk_tool_org", step)
+    tool.ctx.call("sphfem_vtk_write_tool_org", tool.h, _d2(tl_ref), path.encode())
+    return path
+
+
+def vtk_writer_write_FEM_segments(tool: tool_FEM, step: int, folder: str = "results", path: str | None = None):
+    """vtk_writer.cu:368-516 -> results/vtk_seg_%06d.vtk"""
+    path = path or _vtk_path(folder, "vtk_seg", step)
+    tool.ctx.call("sphfem_vtk_write_FEM_segments", tool.h, path.encode())
+    return path
+
+
+def vtk_writer_write_contact_variables(tool: tool_FEM, step: int, folder: str = "results",
+                                       path: str | None = None):
+    """vtk_writer.cu:589-714 -> results/vtk_phy_wear_%06d.vtk"""
+    path = path or _vtk_path(folder, "vtk_phy_wear", step)
+    tool.ctx.call("sphfem_vtk_write_contact_variables", tool.h, path.encode())
+    return path
+
+
+def read_vtk_temperature(path: str) -> np.ndarray:
+    """mesh_read::mesh_read_temperature (mesh_structure_GPU.cu:1259-1296)."""
+    n = C.c_long()
+    check(lib().sphfem_read_vtk_temperature(path.encode(), None, 0, C.byref(n)), None, "read_vtk_temperature")
+    out = np.zeros(n.value)
+    check(lib().sphfem_read_vtk_temperature(path.encode(), out.ctypes.data_as(C.c_void_p), n.value, C.byref(n)),
+          None, "read_vtk_temperature")
+    return out
+
+
+class tool_wear:
+    """tool_wear(usui_K, usui_alpha, N, phys, tool_vel) (tool_wear.h:17-47): Usui wear of the particles
+    in contact with a rigid tool, evaluated on the device."""
+
+    def __init__(self, ctx: Context, usui_K: float, usui_alpha: float, tool_vel):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx.call("sphfem_tool_wear_create", float(usui_K), float(usui_alpha), _d2(tool_vel), C.byref(self.h))
+
+    def eval_usui(self, p: particle_gpu, dt: float):
+        """(min, max, avg) wear rate over the particles in contact; accumulates dt * avg."""
+        mn, mx, av = C.c_double(), C.c_double(), C.c_double()
+        self.ctx.call("sphfem_tool_wear_eval_usui", self.h, C.byref(p.s), float(dt), C.byref(mn), C.byref(mx),
+                      C.byref(av))
+        return mn.value, mx.value, av.value
+
+    def get_accum_wear(self) -> float:
+        v = C.c_double()
+        check(lib().sphfem_tool_wear_get_accum_wear(self.h, C.byref(v)), None, "get_accum_wear")
+        return v.value
+
+    def destroy(self):
+        if self.h:
+            self.ctx.call("sphfem_tool_wear_destroy", self.h)
+            self.h = C.c_void_p()
+
+
 class Simulation:
     """A workload on the device: constants, particles, locked grid, FEM tool and stepper."""
 
