@@ -327,10 +327,18 @@ def test_cpp_ref_cut_driver_matches_python(tmp_path):
     if not os.path.exists(exe):
         pytest.fail("ref_cut_driver not built (run __graft_entry__.build())")
     steps, freq = 40, 10
+    env = dict(os.environ, REF_CUT_RESULTS="1")
     r = subprocess.run([exe, "30", str(steps), str(freq), "fp64", str(tmp_path)], capture_output=True, text=True,
-                       timeout=120)
+                       timeout=120, cwd=str(tmp_path), env=env)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
+    # the reference's results/ files (mfree_gpu.cpp:269-347): one VTK pair per dump and the logs
+    res = tmp_path / "results"
+    for k in range(steps // freq):
+        assert (res / f"vtk_out_{k:06d}.vtk").exists() and (res / f"vtk_tool_{k:06d}.vtk").exists()
+    flog = np.loadtxt(res / "force_0.000350_0.010000")
+    wlog = np.loadtxt(res / "wear_0.001250_0.010000")
+    assert flog.shape == (steps // freq, 2) and wlog.shape == (steps // freq, 4)
     case = cases.ref_cut_sph()
     sim = S.Simulation(case)
     forces = []
@@ -341,6 +349,8 @@ def test_cpp_ref_cut_driver_matches_python(tmp_path):
     forces.append(S.report_force(sim.p))
     got = np.loadtxt(tmp_path / "forces.txt")
     assert np.array_equal(got[:, 1:], np.array(forces))
+    assert np.array_equal(flog, np.array([[float("%.9g" % f[0]), float("%.9g" % f[1])] for f in forces[:-1]]))
+    assert np.all(wlog[:, 3] >= 0) and np.all(np.diff(wlog[:, 3]) >= 0)
     N = case.N
     for f, shp in (("pos", (N, 2)), ("vel", (N, 2)), ("T", (N,)), ("fc", (N, 2))):
         a = np.fromfile(tmp_path / f"{f}.bin", dtype=np.float64).reshape(shp)
