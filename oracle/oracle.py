@@ -167,7 +167,8 @@ def make_consts(case: cases.Case, rr2_min=None, wear_pro=None) -> OConsts:
     _fill(c.trml_wp, k.trml_wp)
     _fill(c.trml_tool, k.trml_tool)
     _fill(c.corr, k.corr)
-    _fill(c.joco, k.joco)
+    if k.joco is not None:
+        _fill(c.joco, k.joco)
     _fill(c.tc, k.tool)
     _fill(c.wear, k.wear)
     c.geom.nx, c.geom.ny = g["nx"], g["ny"]
@@ -176,7 +177,7 @@ def make_consts(case: cases.Case, rr2_min=None, wear_pro=None) -> OConsts:
     c.rr2_min = cases.default_rr2_min(case.dx) if rr2_min is None else rr2_min
     c.wear_pro = int(case.wear if wear_pro is None else wear_pro)
     c.thermal_bc = int(k.trml_tool["alpha"] != 0.0)
-    c.plastic = 1
+    c.plastic = int(k.joco is not None)
     c.fric_heat = int(k.trml_wp["eta"] != 0.0)
     if case.rigid is not None:
         _fill(c.trml_itool, case.rigid.trml_itool)
@@ -207,14 +208,14 @@ class Oracle:
         if grid not in ("green", "rothlin"):
             raise ValueError(grid)
         self.grid = grid
-        self.max_cell = max_cell
+        self.max_cell = case.max_cell if max_cell is None else max_cell
         self.case = case
         self.c = make_consts(case, rr2_min, wear_pro)
         if threads:
             lib().o_set_num_threads(int(threads))
         self.P = cases.initial_particles(case)
         N = case.N
-        nc = int(max_cell) if max_cell else self.c.geom.nx * self.c.geom.ny
+        nc = int(self.max_cell) if self.max_cell else self.c.geom.nx * self.c.geom.ny
         self.st = dict(pos_init=np.zeros((N, 2)), vel_init=np.zeros((N, 2)), S_init=np.zeros((N, 4)),
                        rho_init=np.zeros(N), T_init=np.zeros(N),
                        cell_start=np.full(nc, -1, dtype=np.int32), cell_end=np.zeros(nc, dtype=np.int32))

@@ -588,6 +588,7 @@ class Case:
     extra: dict = field(default_factory=dict)
     tool_p: np.ndarray | None = None      # tool_particle flags (sampled rigid tools)
     rigid: "RigidTool | None" = None      # rigid (SPH) tool instead of the FEM tool
+    max_cell: int | None = None           # unlocked grid of this capacity instead of the locked box
 
 
 @dataclass
@@ -890,6 +891,108 @@ def ref_cut_sph(ny: int = 30, rake: float = 0.0, clear: float = 4.5, chamfer: fl
                 tool_p=np.concatenate([np.zeros(n), np.ones(len(samples))]),
                 rigid=RigidTool(geometry=g, contact_alpha=0.075, trml_itool=dict(trml_tool)),
                 extra=dict(t_final=0.1 / vel_tool, n_workpiece=n))
+
+
+# ----------------------------------------------------------------------------
+# textbook validation cases (benchmarks.cpp:16-246), SI units as there
+# ----------------------------------------------------------------------------
+def _zero_trml():
+    return dict(cp=0.0, tq=0.0, k=0.0, alpha=0.0, T_init=0.0, eta=0.0)
+
+
+def _corr_for(hdx, dx):
+    corr = dict(wdeltap=1.0, stresseps=0.3, xspheps=0.5, alpha=1., beta=1., eta=0.1)
+    h1 = 1. / (hdx * dx)
+    q = dx * h1
+    fac = 10 * M_1_PI / 7.0 * h1 * h1
+    corr["wdeltap"] = fac * (1 - 1.5 * q * q * (1 - 0.5 * q))
+    return corr
+
+
+_ZERO_TOOLC = dict(cp=0., tq=0., k=0., eta=0., rho=0., T0=0., h=0., beta=0., mu=0., contact_alpha=0., slave_mass=0.,
+                   hc=0.)
+_ZERO_WEAR = dict(A=0., B=0., C=0., D=0.)
+
+
+def rings(nbox: int = 80) -> Case:
+    """setup_rings (benchmarks.cpp:148-246): two rubber-like rings (inner radius 3 cm, outer 4 cm,
+    E = 10 MPa, nu = 0.4, rho = 1) approaching each other at 200 m/s each on a locked grid; no
+    plasticity (the Johnson-Cook block is never set up), no heat.  particle_gpu(pos, vel, rho, h,
+    n): T and fixed are zero.  The reference driver calls it with nbox = 80 (mfree_gpu.cpp:224)."""
+    ri, ro = 0.03, 0.04
+    spacing = ro + 0.001
+    dx = 2 * ro / (nbox - 1)
+    hdx = 1.7
+    phys = dict(E=1e7, nu=0.4, rho0=1.)
+    phys["G"] = phys["E"] / (2. * (1. + phys["nu"]))
+    phys["K"] = 2.0 * phys["G"] * (1 + phys["nu"]) / (3 * (1 - 2 * phys["nu"]))
+    phys["mass"] = dx * dx * phys["rho0"]
+    phys["mass_tool"] = 0.0
+    pos = []
+    for i in range(nbox):
+        for j in range(nbox):
+            px, py = -ro + i * dx, -ro + j * dx
+            dist = math.sqrt(px * px + py * py)
+            if ro > dist >= ri:
+                pos.append((px - spacing, py))
+                pos.append((px + spacing, py))
+    pos = np.array(pos)
+    n = len(pos)
+    vel_ring = 170.
+    dt = 0.5 * hdx * dx / (math.sqrt(phys["K"] / phys["rho0"]) + math.sqrt(vel_ring))
+    vel = np.zeros((n, 2))
+    vel[:, 0] = np.where(pos[:, 0] < 0, 200., -200.)
+    const = Constants(phys, _zero_trml(), _zero_trml(), _corr_for(hdx, dx), None, dict(_ZERO_TOOLC),
+                      dict(_ZERO_WEAR))
+    h = hdx * dx
+    return Case(name=f"rings_{nbox}", N=n, dx=dx, h=h, dt=dt, pos=pos, vel=vel, rho=np.full(n, phys["rho0"]),
+                T=np.zeros(n), hh=np.full(n, h), fixed=np.zeros(n), const=const,
+                grid_bbmin=np.array([-5 * ro, -2 * ro]), grid_bbmax=np.array([5 * ro, 2 * ro]), tool=None,
+                tool_vel=np.zeros(2), wear=False, extra=dict(t_final=2.5 * ro / vel_ring, ri=ri, ro=ro))
+
+
+def impact(nbox: int = 20) -> Case:
+    """setup_impact (benchmarks.cpp:16-146): a steel 4340 disc (radius 0.2 m) at 1000 m/s against a
+    wall of nbox/2 columns fixed at top and bottom, Johnson-Cook plasticity (no heat: the thermal
+    blocks are never set up), on an unlocked grid grid_gpu_green(10 n, n).  The reference driver
+    calls it with nbox = 20 (mfree_gpu.cpp:225)."""
+    ro = 0.2
+    dx = 2 * ro / (nbox - 1)
+    hdx = 1.7
+    phys = dict(E=200e9, nu=0.29, rho0=7830.0)
+    phys["G"] = phys["E"] / (2. * (1. + phys["nu"]))
+    phys["K"] = 2.0 * phys["G"] * (1 + phys["nu"]) / (3 * (1 - 2 * phys["nu"]))
+    phys["mass"] = dx * dx * phys["rho0"]
+    phys["mass_tool"] = 0.0
+    joco = dict(A=792.0e6, B=510.0e6, C=0.014, m=1.03, n=0.26, Tref=273.0, Tmelt=1793.0, eps_dot_ref=1.,
+                tanh_a=0., tanh_b=0., tanh_c=0., tanh_d=0.)
+    spacing = ro + 0.05
+    pos = []
+    for i in range(nbox):
+        for j in range(nbox):
+            px, py = -ro + i * dx, -ro + j * dx
+            if math.sqrt(px * px + py * py) < ro:
+                pos.append((px - spacing, py))
+    n_flyer = len(pos)
+    for i in range(nbox // 2):
+        for j in range(10 * nbox):
+            if j * dx - 1. > 1.:
+                continue
+            pos.append((i * dx, j * dx - 1.))
+    pos = np.array(pos)
+    n = len(pos)
+    vel_flyer = 1e3
+    dt = 0.3 * hdx * dx / (math.sqrt(phys["K"] / phys["rho0"]) + math.sqrt(vel_flyer))
+    vel = np.zeros((n, 2))
+    vel[:n_flyer, 0] = 1e3
+    fixed = ((pos[:, 1] < -1 + dx / 2) | (pos[:, 1] > 1 - dx / 2)).astype(np.float64)
+    const = Constants(phys, _zero_trml(), _zero_trml(), _corr_for(hdx, dx), joco, dict(_ZERO_TOOLC),
+                      dict(_ZERO_WEAR))
+    h = hdx * dx
+    return Case(name=f"impact_{nbox}", N=n, dx=dx, h=h, dt=dt, pos=pos, vel=vel, rho=np.full(n, phys["rho0"]),
+                T=np.full(n, joco["Tref"]), hh=np.full(n, h), fixed=fixed, const=const,
+                grid_bbmin=pos.min(axis=0), grid_bbmax=pos.max(axis=0), tool=None, tool_vel=np.zeros(2),
+                wear=False, max_cell=10 * n, extra=dict(t_final=2 * ro / vel_flyer, n_flyer=n_flyer))
 
 
 def initial_particles(case: Case) -> dict:

@@ -566,7 +566,7 @@ int sphfem_interactions_monaghan(sphfem_ctx *ctx, sphfem_particles *p, const int
                                  int num_cell) {
     if (!ctx || !p || !cell_start || !cell_end || num_cell != ctx->c.geom.nx * ctx->c.geom.ny) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {
-        if (int rc = need(ctx, SET_PHYS | SET_CORR | SET_TRML_WP)) return rc;
+        if (int rc = need(ctx, SET_PHYS | SET_CORR)) return rc;
         launch_interactions_ref(ctx, *p, cell_start, cell_end);
         return SPHFEM_OK;
     });
@@ -607,7 +607,7 @@ int sphfem_plasticity_johnson_cook(sphfem_ctx *ctx, sphfem_particles *p) {
     if (!p) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {
         if (!ctx->c.plastic) return SPHFEM_OK;   // actions_gpu.cu:875
-        if (int rc = need(ctx, SET_PHYS | SET_TRML_WP | SET_DT)) return rc;
+        if (int rc = need(ctx, SET_PHYS | SET_DT)) return rc;
         launch_actions(ctx, ACT_PLASTICITY, *p);
         return SPHFEM_OK;
     });
@@ -1366,7 +1366,7 @@ int sphfem_leap_frog_step(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particle
                           sphfem_tool *t) {
     if (!lf || !p || !g || (int)p->N != g->num_part || p->N != lf->num_part) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {
-        if (int rc = need(ctx, SET_PHYS | SET_CORR | SET_TRML_WP | SET_DT | (t ? SET_TOOLC : 0u))) return rc;
+        if (int rc = need(ctx, SET_PHYS | SET_CORR | SET_DT | (t ? SET_TOOLC | SET_TRML_WP : 0u))) return rc;
         return step_body(ctx, lf, p, g, t);
     });
 }
@@ -1375,7 +1375,7 @@ int sphfem_leap_frog_step_rigid(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_pa
                                 sphfem_rigid_tool *rt) {
     if (!lf || !p || !g || (int)p->N != g->num_part || p->N != lf->num_part) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {
-        if (int rc = need(ctx, SET_PHYS | SET_CORR | SET_TRML_WP | SET_DT)) return rc;
+        if (int rc = need(ctx, SET_PHYS | SET_CORR | SET_DT)) return rc;
         return step_body(ctx, lf, p, g, nullptr, rt);
     });
 }
@@ -1396,7 +1396,7 @@ static int graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles 
     if (steps < 2 || steps % 2) return fail(ctx, SPHFEM_ERR_ARG, "graph steps must be even (pointer swaps)");
     *out = nullptr;
     return guarded(ctx, [&] {
-        if (int rc = need(ctx, SET_PHYS | SET_CORR | SET_TRML_WP | SET_DT | (t ? SET_TOOLC : 0u))) return rc;
+        if (int rc = need(ctx, SET_PHYS | SET_CORR | SET_DT | (t ? SET_TOOLC | SET_TRML_WP : 0u))) return rc;
         if (!g->locked)
             return fail(ctx, SPHFEM_ERR_STATE, "graphs need a locked grid (the geometry update reads back to the host)");
         if (g->first_sort) return fail(ctx, SPHFEM_ERR_STATE, "run one step before capturing (first sort)");

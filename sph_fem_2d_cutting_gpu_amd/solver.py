@@ -52,7 +52,8 @@ class Context:
         L = _lib
         self.call("sphfem_setup_physical_constants", C.byref(L.PhysConstants(**const.phys)))
         self.call("sphfem_setup_corrector_constants", C.byref(L.CorrConstants(**const.corr)))
-        self.call("sphfem_setup_johnson_cook_constants", C.byref(L.JocoConstants(**const.joco)))
+        if const.joco is not None:           # setups without plasticity never call it (benchmarks.cpp:148-246)
+            self.call("sphfem_setup_johnson_cook_constants", C.byref(L.JocoConstants(**const.joco)))
         self.call("sphfem_setup_thermal_constants_wp", C.byref(L.TrmlConstants(**const.trml_wp)))
         self.call("sphfem_setup_thermal_constants_tool", C.byref(L.TrmlConstants(**const.trml_tool)))
         self.call("sphfem_setup_tool_constants", C.byref(L.ToolConstants(**const.tool)))
@@ -715,6 +716,7 @@ class Simulation:
         init = cases.initial_particles(case)
         self.p = particle_gpu(self.ctx, case.N, **{k: init[k] for k in ("pos", "vel", "rho", "T", "h", "fixed", "S",
                                                                          "tool_particle")})
+        max_cell = case.max_cell if max_cell is None else max_cell
         if max_cell:
             self.grid = GRIDS[grid].unlocked(self.ctx, max_cell, case.N)
         else:
