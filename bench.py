@@ -58,6 +58,15 @@ def make_case(name):
         return c, (f"ref_cut_sph_ny{ny}: setup_ref_cut rigid tool (rake 0, clearance 4.5 deg, fillet 3.5 um, "
                    f"feed 100 um, 750 m/min), {c.extra['n_workpiece']} workpiece + "
                    f"{c.N - c.extra['n_workpiece']} tool particles, dx={c.dx * 1e4:.3f}um")
+    if name.startswith("rings_nbox"):
+        nb = int(name[len("rings_nbox"):])
+        c = cases.rings(nb)
+        return c, f"rings_nbox{nb}: setup_rings rubber rings at 200 m/s, {c.N} particles, dx={c.dx * 1e3:.4f}mm"
+    if name.startswith("impact_nbox"):
+        nb = int(name[len("impact_nbox"):])
+        c = cases.impact(nb)
+        return c, (f"impact_nbox{nb}: setup_impact steel disc at 1000 m/s on a wall, {c.N} particles, "
+                   f"dx={c.dx * 1e3:.3f}mm, unlocked grid (eager steps)")
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -230,7 +239,8 @@ def main():
         run_batch(args)
         return
     case, workload = make_case(args.workload)
-    elapsed, k_avg_s, sim = run_form(case, args.precision, args.steps, args.warmup, args.breakdown, not args.eager,
+    graph = not args.eager and case.max_cell is None      # an unlocked grid reads its geometry back
+    elapsed, k_avg_s, sim = run_form(case, args.precision, args.steps, args.warmup, args.breakdown, graph,
                                      not args.no_tune)
     tuning = sim.bench_tuning
     cs, _ = sim.grid.download_cells()
@@ -259,7 +269,7 @@ def main():
                    "tool_triangles": int(len(case.tool.elems)) if case.tool is not None else 0,
                    "cutting_segments": int(len(case.tool.seg_left)) if case.tool is not None else 0,
                    "rigid_tool": case.rigid is not None, "wear": bool(case.wear),
-                   "precision": args.precision, "parallelism": "single GPU", "step_graph": not args.eager,
+                   "precision": args.precision, "parallelism": "single GPU", "step_graph": graph,
                    "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
@@ -269,8 +279,7 @@ def main():
                      "algorithmic_bytes": b_int, "occupied_cells": n_occ},
     }
     if args.precision == "fp32c" and not args.no_compare:
-        e64, k64, sim64 = run_form(case, "fp64", args.steps, args.warmup, args.breakdown, not args.eager,
-                                   not args.no_tune)
+        e64, k64, sim64 = run_form(case, "fp64", args.steps, args.warmup, args.breakdown, graph, not args.no_tune)
         sim64.close()
         ups64 = case.N * args.steps / e64
         out["fp64_form"] = {"value": ups64, "ms_per_step": e64 / args.steps * 1e3,
