@@ -377,6 +377,45 @@ int sphfem_tool_wear_eval_usui(sphfem_ctx *ctx, sphfem_tool_wear *w, const sphfe
                                double *usui_min, double *usui_max, double *usui_avg);
 int sphfem_tool_wear_get_accum_wear(const sphfem_tool_wear *w, double *accum);
 
+/* ---- tool mesh (mesh_structure_GPU.cu, tool_FEM.cu:771-1039, FEM_thermal_gpu.cu:126-330, 584-659).
+ * Host-side: the .inp reader, element colouring, CSR pattern, mesh VTK file, cutting segments and
+ * the FEM system, ending in a sphfem_tool_desc for sphfem_tool_create. */
+typedef struct sphfem_mesh sphfem_mesh;
+typedef struct {
+    int node_num, ele_num, nz_num, cut_ele_num, fixed_ele_num, color_total, color_cutting_edge;
+} sphfem_mesh_info;
+/* mesh_read(file) (mesh_structure_GPU.cu:784-1257, gmsh_export 0: Abaqus-style *Node / *Element /
+ * CuttingEdge and FixedBoundary node and element sets, coordinates divided by 100) or
+ * mesh_read(file, true) (:213-783, gmsh_export 1: Gmsh export with T3D2 boundary edges and CPS3
+ * triangles, nodes outside the triangles dropped and renumbered).  Colours the cutting-edge
+ * elements from a palette of color_cutting_edge colours, then boundary layer 2, then the rest from
+ * color_total colours (mesh_structure_GPU.h:26-27 uses 18 and 6), each with the first free colour
+ * counted from 1; builds the CSR pattern.  SPHFEM_ERR_ARG on an unreadable file, a malformed line
+ * or a palette too small for the mesh (the reference asserts). */
+int sphfem_mesh_read_inp(const char *path, int gmsh_export, int color_total, int color_cutting_edge,
+                         sphfem_mesh **out);
+int sphfem_mesh_destroy(sphfem_mesh *m);
+int sphfem_mesh_get_info(const sphfem_mesh *m, sphfem_mesh_info *info);
+/* copies of the mesh arrays (any pointer may be NULL): position [node_num], cut/fixed node flags
+ * [node_num], elements (3 nodes + colour) and cut/fixed edge flags (x: edge 0-1, y: 1-2, z: 2-0)
+ * [ele_num], cut/fixed element lists, boundary layers 1 and 2 [2 ele_num], CSR row_ptr
+ * [node_num + 1] and col_ind [nz_num] */
+int sphfem_mesh_get_arrays(const sphfem_mesh *m, sphfem_double2 *position, int *cut_node, int *fixed_node,
+                           sphfem_int4 *elem, sphfem_int4 *cut_flags, sphfem_int4 *fixed_flags, int *cut_elems,
+                           int *fixed_elems, int *layers, int *row_ptr, int *col_ind);
+/* mesh_read_temperature (mesh_structure_GPU.cu:1259-1296): initial nodal temperatures of a hot
+ * tool, n == node_num; T NULL goes back to the uniform T0 */
+int sphfem_mesh_set_temperature(sphfem_mesh *m, const double *T, long n);
+/* write_mesh_file (mesh_structure_GPU.cu:1395-1462): mesh with colour and layer flags */
+int sphfem_mesh_write_vtk(const sphfem_mesh *m, const char *path);
+/* set_up_segments_from_mesh (tool_FEM.cu:771-1039) and the FEM system of FEM_thermal_2D_GPU
+ * (FEM_thermal_gpu.cu:584-659): cutting segments oriented towards tl_point_ref and chained from
+ * the highest point, P1 stiffness and lumped mass assembled colour by colour, Dirichlet rows at
+ * tc->T0 on the fixed boundary.  The arrays `out` points to belong to the mesh object and stay
+ * valid until the next call or sphfem_mesh_destroy. */
+int sphfem_mesh_tool_desc(sphfem_mesh *m, const sphfem_tool_constants *tc, sphfem_double2 tl_point_ref,
+                          sphfem_double2 vel, sphfem_tool_desc *out);
+
 /* ---- measurement: per-kernel HIP-event timing on the context stream */
 int sphfem_timing_enable(sphfem_ctx *ctx, int enable);
 int sphfem_timing_reset(sphfem_ctx *ctx);

@@ -14,11 +14,12 @@
  *   free functions      actions_gpu.h:33-72, interactions_gpu.h:27-43, output.h:17,
  *                       vtk_writer.h:26-31, mesh_read_temperature
  *   tool_wear           tool_wear.h:17-47
+ *   mesh_read           mesh_structure_GPU.h:80-142 (.inp reader, colouring, CSR, write_mesh_file)
  *   global_dt, wear_pro benchmarks.cpp:12, mfree_gpu.cpp:47
  *
  * Differences a caller sees: errors throw std::runtime_error (the reference prints and
  * continues); objects release their device memory in destructors; tool_FEM is built from a
- * host mesh description (sphfem_tool_desc) instead of a Gmsh file; all work is queued on one
+ * mesh_read (an .inp file) or a host mesh description (sphfem_tool_desc); all work is queued on one
  * process-wide context (device 0 unless sphfem_shim::use_device is called first).
  * Header-only, C++17; link with libsphfem_hip.so.
  */
@@ -263,8 +264,65 @@ inline void compute_contact_forces(particle_gpu *particles) {
 // the segments live inside the tool object; the type exists for signature compatibility
 struct segment_FEM_gpu;
 
+// mesh_read (mesh_structure_GPU.h:80-142): a tool mesh read from an .inp file on the host --
+// mesh_read(file) for the Abaqus-style file, mesh_read(file, true) for a Gmsh export -- with its
+// element colours and CSR pattern
+class mesh_read {
+  public:
+    explicit mesh_read(const std::string &file_name, bool gmsh_d = false, int color_total_ = 18,
+                       int color_cutting_edge_ = 6)
+        : color_total(color_total_), color_cutting_edge(color_cutting_edge_) {
+        sphfem_shim::check(sphfem_mesh_read_inp(file_name.c_str(), gmsh_d ? 1 : 0, color_total, color_cutting_edge,
+                                                &m_handle),
+                           ("mesh_read(" + file_name + ")").c_str());
+        sphfem_shim::check(sphfem_mesh_get_info(m_handle, &info), "mesh_read info");
+    }
+    ~mesh_read() {
+        if (m_handle) (void)sphfem_mesh_destroy(m_handle);
+    }
+    mesh_read(const mesh_read &) = delete;
+    mesh_read &operator=(const mesh_read &) = delete;
+
+    int color_total, color_cutting_edge;
+    sphfem_mesh_info info{};
+
+    // mesh_structure_GPU.cu:1259-1296: hot-tool nodal temperatures from a tool VTK file
+    void mesh_read_temperature(const std::string &file_name) {
+        long n = 0;
+        sphfem_shim::check(sphfem_read_vtk_temperature(file_name.c_str(), nullptr, 0, &n), "mesh_read_temperature");
+        std::vector<double> T((size_t)n);
+        sphfem_shim::check(sphfem_read_vtk_temperature(file_name.c_str(), T.data(), n, &n), "mesh_read_temperature");
+        sphfem_shim::check(sphfem_mesh_set_temperature(m_handle, T.data(), n),
+                           "mesh_read_temperature (node count differs)");
+    }
+    // mesh_structure_GPU.cu:1395-1462
+    void write_mesh_file(const std::string &file_name = "Tool_mesh.vtk") const {
+        sphfem_shim::check(sphfem_mesh_write_vtk(m_handle, file_name.c_str()), "write_mesh_file");
+    }
+    sphfem_mesh *handle() const { return m_handle; }
+
+  private:
+    sphfem_mesh *m_handle = nullptr;
+};
+
 class tool_FEM : public tool {
   public:
+    // tool_FEM(tl, ...) + mesh_CPU = mesh + set_up_segments_from_mesh() + set_up_FEM_tool_gpu(tc)
+    // (benchmarks_NZ.cpp:228-245, tool_FEM.cu:771-1039, 1539-1544): segments oriented towards
+    // tl_point_ref, FEM system with the tool constants tc
+    tool_FEM(mesh_read &mesh, const sphfem_tool_constants &tc, vec2_t tl_point_ref, vec2_t vel) {
+        sphfem_tool_desc desc{};
+        sphfem_shim::check(sphfem_mesh_tool_desc(mesh.handle(), &tc, sphfem_double2{tl_point_ref.x, tl_point_ref.y},
+                                                 sphfem_double2{vel.x, vel.y}, &desc),
+                           "set_up_segments_from_mesh");
+        sphfem_shim::check(sphfem_tool_create(sphfem_shim::ctx(), &desc, &m_handle), "tool_FEM");
+        node_number = desc.node_num;
+        element_number = desc.ele_num;
+        cutting_segment_size = desc.seg_num;
+        m_velocity = vel;
+        right_point = desc.right_point;
+        lowest_point = desc.lowest_point;
+    }
     // the device data of set_up_segments_from_mesh + set_up_FEM_tool_gpu, from a host mesh
     explicit tool_FEM(const sphfem_tool_desc &desc) {
         sphfem_shim::check(sphfem_tool_create(sphfem_shim::ctx(), &desc, &m_handle), "tool_FEM");
@@ -272,6 +330,8 @@ class tool_FEM : public tool {
         element_number = desc.ele_num;
         cutting_segment_size = desc.seg_num;
         m_velocity = vec2_t(desc.vel.x, desc.vel.y);
+        right_point = desc.right_point;
+        lowest_point = desc.lowest_point;
     }
     ~tool_FEM() override {
         if (m_handle && sphfem_shim::ctx_slot()) (void)sphfem_tool_destroy(sphfem_shim::ctx_slot(), m_handle);
@@ -282,6 +342,7 @@ class tool_FEM : public tool {
     int node_number = 0;
     int element_number = 0;
     int cutting_segment_size = 0;
+    double right_point = 0., lowest_point = 0.;     // at set-up (tool_FEM.h:54-55)
     segment_FEM_gpu *segments_FEM = nullptr;
 
     void set_vel(vec2_t vel) override {

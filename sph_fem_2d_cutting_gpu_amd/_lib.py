@@ -67,6 +67,9 @@ class ToolDesc(C.Structure):
 
 TOOL_MAX_SEG = 5
 
+MeshInfo = _s("MeshInfo", ["node_num", "ele_num", "nz_num", "cut_ele_num", "fixed_ele_num", "color_total",
+                           "color_cutting_edge"], C.c_int)
+
 
 class Line(C.Structure):
     _fields_ = [("a", _dbl), ("b", _dbl), ("vertical", C.c_int)]
@@ -184,6 +187,13 @@ _SIGS = {
     "sphfem_tool_wear_eval_usui": (_ctx, _vp, _P(Particles), _dbl, _P(_dbl), _P(_dbl), _P(_dbl)),
     "sphfem_tool_wear_get_accum_wear": (_vp, _P(_dbl)),
     "sphfem_report_force": (_ctx, _P(Particles), _P(Double2)),
+    "sphfem_mesh_read_inp": (C.c_char_p, C.c_int, C.c_int, C.c_int, _P(_vp)),
+    "sphfem_mesh_destroy": (_vp,),
+    "sphfem_mesh_get_info": (_vp, _P(MeshInfo)),
+    "sphfem_mesh_get_arrays": (_vp,) + (_vp,) * 11,
+    "sphfem_mesh_set_temperature": (_vp, _vp, C.c_long),
+    "sphfem_mesh_write_vtk": (_vp, C.c_char_p),
+    "sphfem_mesh_tool_desc": (_vp, _P(ToolConstants), Double2, Double2, _P(ToolDesc)),
     "sphfem_timing_enable": (_ctx, C.c_int),
     "sphfem_timing_reset": (_ctx,),
     "sphfem_timing_select": (_ctx, C.c_char_p),

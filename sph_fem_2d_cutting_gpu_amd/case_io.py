@@ -26,7 +26,8 @@ BLOCKS = (("phys", _lib.PhysConstants), ("corr", _lib.CorrConstants), ("joco", _
           ("wear", _lib.WearConstants))
 
 
-def write_case_dir(case: cases.Case, path: str, rr2_min: float | None = None, wear: bool | None = None) -> str:
+def write_case_dir(case: cases.Case, path: str, rr2_min: float | None = None, wear: bool | None = None,
+                   tool_inp: str | None = None) -> str:
     os.makedirs(path, exist_ok=True)
     init = cases.initial_particles(case)
     sc = {"N": case.N, "dt": case.dt, "h": case.h,
@@ -49,6 +50,12 @@ def write_case_dir(case: cases.Case, path: str, rr2_min: float | None = None, we
             sc[f"tool.{k}"] = float(v)
         for name, dt in TOOL_ARRAYS:
             np.ascontiguousarray(t[name], dtype=dt).tofile(os.path.join(path, f"t_{name}.bin"))
+        if tool_inp is not None:
+            # the driver then reads the mesh from tool.inp and sets the tool up from it
+            if tool_inp not in ("gmsh", "abaqus"):
+                raise ValueError("tool_inp must be 'gmsh' or 'abaqus'")
+            cases.write_inp(tm, os.path.join(path, "tool.inp"), gmsh=tool_inp == "gmsh")
+            sc["tool.inp_gmsh"] = int(tool_inp == "gmsh")
     else:
         sc["node_num"] = 0
     with open(os.path.join(path, "scalars.txt"), "w") as f:

@@ -483,6 +483,103 @@ def _mesh_tool(length, height, rake_deg, clear_deg, r, s_max, tc, root_scale=1.0
                     lowest_point=float(ends[:, 1].min()), right_point=float(ends[:, 0].max()))
 
 
+def tool_mesh_edges(tm: ToolMesh):
+    """(cutting edges, fixed edges) of a tool mesh as node pairs: the cutting edges are the segment
+    ends, the fixed edges the boundary edges (edges of one triangle) with both nodes on the fixed
+    boundary that are not cutting edges."""
+    cut = [(int(a), int(b)) for a, b in tm.seg_marks[:, 2:4]]
+    cut_set = {(min(a, b), max(a, b)) for a, b in cut}
+    count = {}
+    for t in tm.elems[:, :3]:
+        for u, v in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0])):
+            k = (int(min(u, v)), int(max(u, v)))
+            count[k] = count.get(k, 0) + 1
+    fx = tm.fixed_edge_node
+    fixed = [k for k, c in count.items() if c == 1 and fx[k[0]] and fx[k[1]] and k not in cut_set]
+    return cut, sorted(fixed)
+
+
+def write_inp(tm: ToolMesh, path: str, gmsh: bool = False, unused_nodes=()):
+    """Write a tool mesh in one of the two .inp dialects the reference reads
+    (mesh_structure_GPU.cu:784-1257 and :213-594), so the same mesh can be fed to both codes.
+
+    gmsh=False: Abaqus style, *Node (coordinates times 100, the reader divides by 100), *Element,
+    node and element sets CuttingEdge / FixedBoundary, *End Part.
+    gmsh=True: Gmsh export, *NODE, T3D2 boundary edges, CPS3 triangles, element sets of edge ids
+    (FixedBoundary before CuttingEdge, as Gmsh orders the physical groups 2 and 67).  unused_nodes
+    lists node positions (0-based, into the written numbering) where a node that no triangle uses is
+    inserted, as Gmsh writes geometry points outside the surface mesh."""
+    cut, fixed = tool_mesh_edges(tm)
+    nn = len(tm.nodes)
+    tri = tm.elems[:, :3]
+    lines = []
+    if not gmsh:
+        lines += ["*Heading", "** tool mesh", "*Part, name=Tool", "*Node"]
+        lines += ["%d, %r, %r" % (i + 1, float(x) * 100, float(y) * 100) for i, (x, y) in enumerate(tm.nodes)]
+        lines.append("*Element, type=CPS3")
+        lines += ["%d, %d, %d, %d" % (e + 1, t[0] + 1, t[1] + 1, t[2] + 1) for e, t in enumerate(tri)]
+
+        def owners(edges):
+            want = {(min(a, b), max(a, b)) for a, b in edges}
+            out = []
+            for e, t in enumerate(tri):
+                for u, v in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0])):
+                    if (min(u, v), max(u, v)) in want:
+                        out.append(e)
+                        break
+            return out
+
+        def ids(values):
+            values = list(values)
+            return [", ".join(str(v) for v in values[k:k + 16]) for k in range(0, len(values), 16)]
+
+        cut_nodes = sorted({a for e in cut for a in e})
+        fixed_nodes = sorted({a for e in fixed for a in e})
+        lines.append("*Nset, nset=CuttingEdge")
+        lines += ids(a + 1 for a in cut_nodes)
+        lines.append("*Elset, elset=CuttingEdge")
+        lines += ids(e + 1 for e in owners(cut))
+        lines.append("*Nset, nset=FixedBoundary")
+        lines += ids(a + 1 for a in fixed_nodes)
+        lines.append("*Elset, elset=FixedBoundary")
+        lines += ids(e + 1 for e in owners(fixed))
+        lines.append("*End Part")
+    else:
+        unused = sorted(int(u) for u in unused_nodes)
+        file_id = np.arange(nn) + 1
+        for u in unused:
+            file_id[file_id >= u + 1] += 1
+        lines += ["*Heading", " tool.inp", "*NODE"]
+        total = nn + len(unused)
+        inv = {int(f): i for i, f in enumerate(file_id)}
+        lo, hi = tm.nodes.min(axis=0), tm.nodes.max(axis=0)
+        for f in range(1, total + 1):
+            if f in inv:
+                x, y = tm.nodes[inv[f]]
+            else:           # a point outside the tool, as the centre of a fillet arc
+                x, y = hi[0] + (f % 7) * 1e-3, lo[1] - (f % 5) * 1e-3
+            lines.append("%d, %r, %r, 0" % (f, float(x), float(y)))
+        lines.append("******* E L E M E N T S *************")
+        lines.append("*ELEMENT, type=T3D2, ELSET=Line1")
+        edges = cut + fixed
+        lines += ["%d, %d, %d" % (k + 1, file_id[a], file_id[b]) for k, (a, b) in enumerate(edges)]
+        lines.append("*ELEMENT, type=CPS3, ELSET=Surface1")
+        ne = len(edges)
+        lines += ["%d, %d, %d, %d" % (ne + e + 1, file_id[t[0]], file_id[t[1]], file_id[t[2]])
+                  for e, t in enumerate(tri)]
+
+        def ids(values):
+            values = list(values)
+            return [", ".join(str(v) for v in values[k:k + 10]) + "," for k in range(0, len(values), 10)]
+
+        lines.append("*ELSET,ELSET=FixedBoundary")
+        lines += ids(range(len(cut) + 1, ne + 1))
+        lines.append("*ELSET,ELSET=CuttingEdge")
+        lines += ids(range(1, len(cut) + 1))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def element_matrices(p0, p1, p2, tc):
     """Lumped mass (per node) and P1 stiffness of one triangle.
 

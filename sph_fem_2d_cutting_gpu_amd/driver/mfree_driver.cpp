@@ -5,7 +5,8 @@
 // sorted state (pos, vel, S, T, idx as raw little-endian arrays).
 //
 // usage: mfree_driver CASE_DIR STEPS FREQ fp64|fp32c OUT_DIR
-//   CASE_DIR as written by sph_fem_2d_cutting_gpu_amd.case_io.write_case_dir
+//   CASE_DIR as written by sph_fem_2d_cutting_gpu_amd.case_io.write_case_dir; with a tool.inp
+//   there, the tool is set up from that mesh file
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -127,14 +128,23 @@ int main(int argc, char **argv) {
         grid_gpu_green grid((int)N, float2_t{get(s, "bbmin_x"), get(s, "bbmin_y")},
                             float2_t{get(s, "bbmax_x"), get(s, "bbmax_y")}, get(s, "h"));
 
-        // tool mesh (replaces the Gmsh read of tool_FEM's constructor)
+        // tool mesh: read from CASE_DIR/tool.inp as benchmarks_NZ.cpp:228-245 does (mesh_read,
+        // set_up_segments_from_mesh, set_up_FEM_tool_gpu), else the set-up arrays of the case
         std::unique_ptr<tool_FEM> tool;
+        std::unique_ptr<mesh_read> mesh;
         const int nn = (int)get(s, "node_num");
         std::vector<sphfem_double2> t_pos, t_left, t_right, t_n;
         std::vector<sphfem_int4> t_elem, t_marks;
         std::vector<int> t_layer, t_rp, t_ci;
         std::vector<double> t_K, t_M, t_f, t_T;
-        if (nn > 0) {
+        const std::string inp = dir + "/tool.inp";
+        if (nn > 0 && std::ifstream(inp).good()) {
+            mesh.reset(new mesh_read(inp, get(s, "tool.inp_gmsh") != 0.));
+            tool.reset(new tool_FEM(*mesh, tc, vec2_t(get(s, "tool.tl_x"), get(s, "tool.tl_y")),
+                                    vec2_t(get(s, "tool.vel_x"), get(s, "tool.vel_y"))));
+            std::printf("tool mesh %s: %d nodes, %d elements, %d cutting segments\n", inp.c_str(),
+                        tool->node_number, tool->element_number, tool->cutting_segment_size);
+        } else if (nn > 0) {
             sphfem_tool_desc d{};
             d.node_num = nn;
             d.ele_num = (int)get(s, "ele_num");

@@ -702,6 +702,94 @@ class tool_wear:
             self.h = C.c_void_p()
 
 
+class mesh_read:
+    """Tool mesh read from an .inp file (mesh_structure_GPU.h:80-142): mesh_read(file) for the
+    Abaqus-style file (coordinates divided by 100, mesh_structure_GPU.cu:784-1257) or
+    mesh_read(file, True) for a Gmsh export (:213-783).  Elements are coloured (cutting edge first
+    from color_cutting_edge colours, then boundary layer 2 and the rest from color_total colours,
+    first free colour counted from 1) and the CSR pattern is built.  Host code only: no device is
+    needed until the tool is created."""
+
+    color_total = 18              # mesh_structure_GPU.h:26-27
+    color_cutting_edge = 6
+
+    def __init__(self, file_name: str, gmsh_d: bool = False, color_total: int = 18, color_cutting_edge: int = 6):
+        self.h = C.c_void_p()
+        check(lib().sphfem_mesh_read_inp(str(file_name).encode(), int(bool(gmsh_d)), int(color_total),
+                                         int(color_cutting_edge), C.byref(self.h)), None, f"mesh_read({file_name})")
+        info = _lib.MeshInfo()
+        check(lib().sphfem_mesh_get_info(self.h, C.byref(info)), None, "mesh_get_info")
+        self.info = {f: getattr(info, f) for f, _ in _lib.MeshInfo._fields_}
+        self.color_total, self.color_cutting_edge = color_total, color_cutting_edge
+
+    def arrays(self) -> dict:
+        i = self.info
+        nn, ne = i["node_num"], i["ele_num"]
+        out = dict(position=np.empty((nn, 2)), cut_node=np.empty(nn, np.int32), fixed_node=np.empty(nn, np.int32),
+                   elem=np.empty((ne, 4), np.int32), cut_flags=np.empty((ne, 4), np.int32),
+                   fixed_flags=np.empty((ne, 4), np.int32), cut_elems=np.empty(i["cut_ele_num"], np.int32),
+                   fixed_elems=np.empty(i["fixed_ele_num"], np.int32), layers=np.empty((ne, 2), np.int32),
+                   row_ptr=np.empty(nn + 1, np.int32), col_ind=np.empty(i["nz_num"], np.int32))
+        check(lib().sphfem_mesh_get_arrays(self.h, *[a.ctypes.data_as(C.c_void_p) for a in out.values()]), None,
+              "mesh_get_arrays")
+        out["cut_flags"] = out["cut_flags"][:, :3].copy()
+        out["fixed_flags"] = out["fixed_flags"][:, :3].copy()
+        return out
+
+    def mesh_read_temperature(self, file_name: str):
+        """Initial nodal temperatures of a hot tool from a tool VTK file (mesh_structure_GPU.cu:1259-1296)."""
+        T = read_vtk_temperature(file_name)
+        self.set_temperature(T)
+
+    def set_temperature(self, T):
+        if T is None:
+            check(lib().sphfem_mesh_set_temperature(self.h, None, 0), None, "mesh_set_temperature")
+            return
+        a = np.ascontiguousarray(T, dtype=np.float64)
+        check(lib().sphfem_mesh_set_temperature(self.h, a.ctypes.data_as(C.c_void_p), len(a)), None,
+              "mesh_set_temperature (the temperature count must equal the node count)")
+
+    def write_mesh_file(self, file_name: str = "Tool_mesh.vtk"):
+        """mesh_structure_GPU.cu:1395-1462"""
+        check(lib().sphfem_mesh_write_vtk(self.h, str(file_name).encode()), None, "write_mesh_file")
+
+    def tool_mesh(self, tc: dict, tl_point_ref, vel=(0.0, 0.0)) -> cases.ToolMesh:
+        """set_up_segments_from_mesh + the FEM system of set_up_FEM_tool_gpu (tool_FEM.cu:771-1039,
+        FEM_thermal_gpu.cu:584-659) as the tool description a workload carries."""
+        d = _lib.ToolDesc()
+        check(lib().sphfem_mesh_tool_desc(self.h, C.byref(_lib.ToolConstants(**tc)), _d2(tl_point_ref), _d2(vel),
+                                          C.byref(d)), None, "set_up_segments_from_mesh")
+        nn, ne, nz, ns = d.node_num, d.ele_num, d.nz_num, d.seg_num
+
+        def get(ptr, n, dt, shape):
+            cnt = n * int(np.prod(shape[1:], dtype=np.int64))
+            buf = (C.c_char * (cnt * np.dtype(dt).itemsize)).from_address(ptr)
+            return np.frombuffer(buf, dtype=dt, count=cnt).reshape(shape).copy()
+
+        a = self.arrays()
+        return cases.ToolMesh(
+            nodes=get(d.position, nn, np.float64, (nn, 2)), elems=get(d.node_num_el, ne, np.int32, (ne, 4)),
+            layers=a["layers"], cut_edge_node=a["cut_node"], fixed_edge_node=a["fixed_node"],
+            row_ptr=get(d.csr_row_ptr, nn + 1, np.int32, (nn + 1,)), col_ind=get(d.csr_col_ind, nz, np.int32, (nz,)),
+            K=get(d.K, nz, np.float64, (nz,)), M=get(d.M, nn, np.float64, (nn,)), f=get(d.f, nn, np.float64, (nn,)),
+            T=get(d.T, nn, np.float64, (nn,)), seg_left=get(d.left, ns, np.float64, (ns, 2)),
+            seg_right=get(d.right, ns, np.float64, (ns, 2)), seg_n=get(d.n, ns, np.float64, (ns, 2)),
+            seg_marks=get(d.marks, ns, np.int32, (ns, 4)), n_colors=d.color_total, n_cut_colors=d.color_cutting_edge,
+            tl=np.array([d.tl_point_ref.x, d.tl_point_ref.y]), br=np.array([d.br_point.x, d.br_point.y]),
+            ref_dist=d.ref_dist, lowest_point=d.lowest_point, right_point=d.right_point)
+
+    def close(self):
+        if self.h:
+            check(lib().sphfem_mesh_destroy(self.h), None, "mesh_destroy")
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Simulation:
     """A workload on the device: constants, particles, locked grid, FEM tool and stepper."""
 
