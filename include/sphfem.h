@@ -81,6 +81,13 @@ typedef struct sphfem_tool_desc {
     sphfem_double2 vel;                  /* tool velocity (tool.h:48-58) */
     sphfem_double2 tl_point_ref, br_point;
     double ref_dist, lowest_point, right_point;
+    /* optional, needed by the remesh entry points (mesh_structure_GPU.h:88-115): per node bit 0 on
+     * the cutting edge, bit 1 on the fixed boundary [node_num]; the fixed-boundary elements and
+     * their edge flags (x: nodes 0-1, y: 1-2, z: 2-0) [fixed_num].  NULL / 0 when not given. */
+    const int *node_flags;
+    int fixed_num;
+    const int *fixed_elems;
+    const sphfem_int4 *fixed_flags;
 } sphfem_tool_desc;
 
 /* tool kinematic state kept on the device (tool_FEM.h:34-60 host fields + tool_FEM.cu:19-20) */
@@ -415,6 +422,32 @@ int sphfem_mesh_write_vtk(const sphfem_mesh *m, const char *path);
  * valid until the next call or sphfem_mesh_destroy. */
 int sphfem_mesh_tool_desc(sphfem_mesh *m, const sphfem_tool_constants *tc, sphfem_double2 tl_point_ref,
                           sphfem_double2 vel, sphfem_tool_desc *out);
+
+/* ---- tool remeshing (actions_gpu.cu:1022-1038).  Both need a tool created with node_flags and
+ * fixed_elems / fixed_flags (SPHFEM_ERR_STATE otherwise). */
+/* remesh_low_level(tool, freq): the nodal temperatures and positions are kept as the mapping
+ * (tool_FEM.cu:1611-1622), the interior nodes are moved by mesh_shift(freq) (tool_FEM.cu:589-616),
+ * K, M and f are assembled again colour by colour with the Dirichlet rows and the inverse mass
+ * (FEM_thermal_gpu.cu:702-720, 797-847), and the temperatures are interpolated from the mapping
+ * (tool_FEM.cu:57-100, 434-443).  Arrays keep their addresses: captured graphs stay valid. */
+int sphfem_tool_remesh_low_level(sphfem_ctx *ctx, sphfem_tool *t, int freq);
+/* read_remesh_gmsh (tool_FEM.cu:1811-1828) from a mesh already read (the .inp Gmsh writes from
+ * the geometry of sphfem_tool_write_remesh_geo): nodal temperatures interpolated from the current
+ * mesh at the new node positions (tool_FEM.cu:445-517; 300 where no element holds the node),
+ * cutting segments rebuilt from the new mesh and oriented towards the current reference point
+ * (tool_FEM.cu:1245-1539), the FEM system of the new mesh (FEM_thermal_gpu.cu:722-795).  The
+ * tool's velocity, reference points and flank retreat are kept; lowest point, right point and
+ * reference distance come from the new segments; the segment accumulators start at zero.  The
+ * tool's device arrays are replaced: graphs captured before are refused afterwards. */
+int sphfem_tool_remesh(sphfem_ctx *ctx, sphfem_tool *t, sphfem_mesh *mesh);
+/* remesh_read_segments (tool_FEM.cu:1708-1810): the Gmsh geometry of the current cutting edge --
+ * the reference point, the chain's points with mesh sizes of six segment lengths (a point is
+ * skipped after a short edge), the closing lines, physical groups CuttingEdge (67) and
+ * FixedBoundary (2) -- written as a .geo script for `gmsh -2 -format inp` */
+int sphfem_tool_write_remesh_geo(sphfem_ctx *ctx, sphfem_tool *t, const char *path);
+/* node, element, non-zero and segment counts of the tool's current mesh (tool_FEM.h:39-41 node_number,
+ * element_number, cutting_segment_size); any pointer may be NULL */
+int sphfem_tool_get_sizes(const sphfem_tool *t, int *node_num, int *ele_num, int *nz_num, int *seg_num);
 
 /* ---- measurement: per-kernel HIP-event timing on the context stream */
 int sphfem_timing_enable(sphfem_ctx *ctx, int enable);

@@ -15,6 +15,8 @@
  *                       vtk_writer.h:26-31, mesh_read_temperature
  *   tool_wear           tool_wear.h:17-47
  *   mesh_read           mesh_structure_GPU.h:80-142 (.inp reader, colouring, CSR, write_mesh_file)
+ *   remesh_low_level, remesh_gmsh, tool_FEM::read_remesh_gmsh / remesh_read_segments
+ *                       actions_gpu.cu:1022-1038, tool_FEM.cu:1708-1828
  *   global_dt, wear_pro benchmarks.cpp:12, mfree_gpu.cpp:47
  *
  * Differences a caller sees: errors throw std::runtime_error (the reference prints and
@@ -371,6 +373,29 @@ class tool_FEM : public tool {
     }
     sphfem_tool *handle() const { return m_handle; }
 
+    // read_remesh_gmsh (tool_FEM.cu:1811-1828): the tool takes the mesh of a Gmsh export (by default
+    // the file the reference's remesh writes), temperatures interpolated from the current mesh;
+    // graphs captured before must be captured again
+    void read_remesh_gmsh(const std::string &file_name = "tool_gmsh_remeshed.inp") {
+        mesh_read mesh(file_name, true);
+        mesh.write_mesh_file();
+        remesh(mesh);
+    }
+    void remesh(mesh_read &mesh) {
+        sphfem_shim::check(sphfem_tool_remesh(sphfem_shim::ctx(), m_handle, mesh.handle()), "tool_FEM::remesh");
+        sphfem_shim::check(sphfem_tool_get_sizes(m_handle, &node_number, &element_number, nullptr,
+                                                 &cutting_segment_size),
+                           "tool sizes");
+        const sphfem_tool_kinematics k = kinematics();
+        right_point = k.right_point;
+        lowest_point = k.lowest_point;
+    }
+    // remesh_read_segments (tool_FEM.cu:1708-1810): the Gmsh geometry of the current cutting edge
+    void remesh_read_segments(const std::string &geo_file = "tool_remesh.geo") {
+        sphfem_shim::check(sphfem_tool_write_remesh_geo(sphfem_shim::ctx(), m_handle, geo_file.c_str()),
+                           "remesh_read_segments");
+    }
+
   private:
     sphfem_tool *m_handle = nullptr;
 };
@@ -626,6 +651,18 @@ inline void compute_contact_forces_FEM_tool_v3(particle_gpu *p, segment_FEM_gpu 
 }
 inline void wear_adjustment(tool_FEM *t) {
     SPHFEM_SHIM_CALL(sphfem_wear_adjustment(sphfem_shim::ctx(), t->handle()), "wear_adjustment");
+}
+
+// actions_gpu.cu:1022-1031
+inline void remesh_low_level(tool_FEM *t, int freq) {
+    SPHFEM_SHIM_CALL(sphfem_tool_remesh_low_level(sphfem_shim::ctx(), t->handle(), freq), "remesh_low_level");
+}
+
+// actions_gpu.cu:1033-1038 with the meshing done outside: the geometry goes to geo_file, and the
+// tool takes the mesh Gmsh made of it (inp_file)
+inline void remesh_gmsh(tool_FEM *t, const std::string &geo_file, const std::string &inp_file) {
+    t->remesh_read_segments(geo_file);
+    t->read_remesh_gmsh(inp_file);
 }
 
 // output.h:17

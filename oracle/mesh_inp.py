@@ -10,7 +10,8 @@ Only tests/ import this module, as the checker of the library's host-side mesh p
 * segments: tool_FEM::set_up_segments_from_mesh (tool_FEM.cu:771-1039);
 * fem_system: the P1 stiffness / lumped mass assembly and the Dirichlet rows of
   FEM_thermal_gpu.cu:71-152, 126-225, 234-324, in colour order;
-* mesh_vtk: write_mesh_file (mesh_structure_GPU.cu:1395-1462).
+* mesh_vtk: write_mesh_file (mesh_structure_GPU.cu:1395-1462);
+* remesh_geo: remesh_read_segments (tool_FEM.cu:1708-1810) as a .geo script.
 
 Pure-Python loops, for meshes of a few thousand elements.
 """
@@ -477,4 +478,48 @@ def mesh_vtk(m: dict, col, layers) -> str:
     out.append("SCALARS layers int 1\nLOOKUP_TABLE default\n")
     out += ["%d\n" % l[1] for l in layers]
     out.append("\n")
+    return "".join(out)
+
+
+def _fdist(a, b):
+    """sqrtf of the double expression, as the distance helpers of tool_FEM.cu do"""
+    return float(np.sqrt(np.float32((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]))))
+
+
+def remesh_geo(left, right, tl) -> str:
+    """remesh_read_segments (tool_FEM.cu:1708-1810): the gmsh::model::geo calls as a .geo script"""
+    ns = len(left)
+    down = left[0][1] > right[0][1]
+    pts, cut = [], []
+    length, prev = 0.004, 0.0
+    i = 0
+    while i < ns:
+        p = left[i] if down else right[i]
+        if i + 1 < ns:
+            judge = _fdist(left[i], right[i + 1]) if down else _fdist(left[i + 1], right[i])
+        else:
+            judge = math.inf
+        buf = _fdist(left[i], right[i])
+        length = 6.0 * buf
+        pts.append((p[0], p[1], length))
+        cut.append(1 + len(pts))
+        if (judge < 1.25 * buf and i != ns - 1 and prev > 0.7 * buf) or judge < 0.00013:
+            i += 1
+        prev = buf
+        i += 1
+    last = right[ns - 1] if down else left[ns - 1]
+    pts.append((last[0], last[1], length))
+    g = lambda v: "%.17g" % v
+    np_ = len(pts) + 1
+    out = ["// tool remesh geometry (remesh_read_segments): mesh with\n",
+           "//   gmsh -2 -format inp -o tool_gmsh_remeshed.inp <this file>\n",
+           "Point(1) = {%s, %s, 0, %s};\n" % (g(tl[0]), g(tl[1]), g(0.075))]
+    out += ["Point(%d) = {%s, %s, 0, %s};\n" % (q + 2, g(x), g(y), g(s)) for q, (x, y, s) in enumerate(pts)]
+    out += ["Line(%d) = {%d, %d};\n" % (q, q, q + 1) for q in range(1, np_)]
+    out.append("Line(%d) = {%d, 1};\n" % (np_, np_))
+    out.append("Curve Loop(1) = {%s};\n" % ", ".join(str(q) for q in range(1, np_ + 1)))
+    out.append("Plane Surface(1) = {1};\n")
+    out.append('Physical Curve("CuttingEdge", 67) = {%s};\n' % ", ".join(str(c) for c in cut))
+    out.append('Physical Curve("FixedBoundary", 2) = {1, %d};\n' % np_)
+    out.append("Mesh.MshFileVersion = 4;\nMesh.SaveAll = 1;\n")
     return "".join(out)

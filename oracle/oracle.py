@@ -90,7 +90,9 @@ _TOOL_SCALARS = ("vel_x", "vel_y", "tl_x", "tl_y", "br_x", "br_y", "ref_dist", "
 
 class OTool(C.Structure):
     _fields_ = ([(n, C.c_int) for n in ("n_nodes", "n_elem", "nnz", "n_seg", "n_colors", "n_cut_colors")]
-                + [(n, _P) for n in _TOOL_ARRAYS] + [(n, _dbl) for n in _TOOL_SCALARS])
+                + [(n, _P) for n in _TOOL_ARRAYS] + [(n, _dbl) for n in _TOOL_SCALARS]
+                + [("node_flags", _P), ("n_fixed", C.c_int), ("fixed_elems", _P), ("fixed_flags", _P),
+                   ("pos_map", _P), ("T_map", _P)])
 
 
 _lib = None
@@ -122,6 +124,7 @@ def lib():
             "o_fem_reset_server": (pt,), "o_fem_calculate_temp": (pt, _dbl), "o_fem_apply_robin": (pc, pt),
             "o_fem_calculate_temp_rate": (pt,), "o_segments_wear_rate_reset": (pt,),
             "o_segments_update": (pt, _dbl), "o_tool_update": (pt, _dbl), "o_wear_adjustment": (pc, pt),
+            "o_remesh_low_level": (pc, pt, C.c_int), "o_interp_new": (pt, C.c_void_p, C.c_int, C.c_void_p),
             "o_step": (pc, pp, ps, pt), "o_run": (pc, pp, ps, pt, C.c_int),
             "o_report_force": (pp, C.POINTER(_dbl), C.POINTER(_dbl)),
             "o_cubic_spline": (_dbl, _dbl, _dbl, _dbl, _dbl, _dbl, C.POINTER(_dbl)),
@@ -245,6 +248,18 @@ class Oracle:
         for name in _TOOL_ARRAYS:
             setattr(self.Tc, name, _ptr(t[name]))
         _fill(self.Tc, t["scalars"])
+        if "fixed_flags4" not in t:
+            nf = len(t["fixed_elems"])
+            t["node_flags"] = np.ascontiguousarray(t["node_flags"], dtype=np.int32)
+            t["fixed_elems"] = np.ascontiguousarray(t["fixed_elems"], dtype=np.int32)
+            t["fixed_flags4"] = np.zeros((nf, 4), np.int32)
+            t["fixed_flags4"][:, :3] = t["fixed_flags"]
+            t["pos_map"] = np.zeros_like(t["node_pos"])
+            t["T_map"] = np.zeros_like(t["T"])
+        self.Tc.node_flags, self.Tc.fixed_elems = _ptr(t["node_flags"]), _ptr(t["fixed_elems"])
+        self.Tc.n_fixed = len(t["fixed_elems"])
+        self.Tc.fixed_flags = _ptr(t["fixed_flags4"])
+        self.Tc.pos_map, self.Tc.T_map = _ptr(t["pos_map"]), _ptr(t["T_map"])
 
     def clone(self) -> "Oracle":
         """Independent copy of the current state (arrays, tool kinematics and constants)."""
@@ -300,6 +315,34 @@ class Oracle:
             return
         run = lib().o_run_rothlin if self.grid == "rothlin" else lib().o_run
         run(C.byref(self.c), C.byref(self.Pc), C.byref(self.Sc), tc, int(n))
+
+    def remesh_low_level(self, freq: int):
+        """remesh_low_level(tool, freq) (actions_gpu.cu:1022-1031)"""
+        lib().o_remesh_low_level(C.byref(self.c), C.byref(self.Tc), int(freq))
+
+    def interp_new(self, new_pos) -> np.ndarray:
+        """do_nodal_value_interpolation (tool_FEM.cu:445-517) from the current mesh at new_pos,
+        300 where no element holds a point"""
+        pos = np.ascontiguousarray(new_pos, dtype=np.float64)
+        out = np.full(len(pos), 300.0)
+        lib().o_interp_new(C.byref(self.Tc), pos.ctypes.data_as(C.c_void_p), len(pos), out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def remesh(self, tm_new):
+        """read_remesh_gmsh (tool_FEM.cu:1811-1828) onto the tool mesh tm_new (cases.ToolMesh, built
+        from the new mesh file at the current reference point): interpolated temperatures, the new
+        mesh's segments and FEM system, kinematics kept except lowest point, right point and
+        reference distance"""
+        import dataclasses
+        T_new = self.interp_new(tm_new.nodes)
+        sc = self.tool_scalars
+        sc.update(lowest_point=float(tm_new.lowest_point), right_point=float(tm_new.right_point),
+                  ref_dist=float(tm_new.ref_dist))
+        t = cases.initial_tool(dataclasses.replace(self.case, tool=tm_new))
+        t["T"], t["T_old"] = T_new.copy(), T_new.copy()
+        t["scalars"] = sc
+        self.t = t
+        self._bind()
 
     def report_force(self):
         fx, fy = _dbl(), _dbl()

@@ -923,6 +923,151 @@ static void mesh_adjust_wear(const o_consts *c, o_tool *t, double sign) {
     }
 }
 
+static double o_cross(o_d2 a, o_d2 b) { return a.x * b.y - a.y * b.x; }
+
+static int o_find(const o_tool *t, int row, int col) {
+    int k = t->row_ptr[row];
+    while (k < t->row_ptr[row + 1] && t->col_ind[k] != col) k++;
+    return k;
+}
+
+/* mesh_shift (tool_FEM.cu:589-616) of one interior node */
+static void o_mesh_shift_node(const o_tool *t, int freq, o_d2 *pp) {
+    o_d2 p = *pp;
+    double shift = t->flank_retreat * 1.0 * freq;
+    shift = fmin(shift, -0.0000065);
+    double wy = (p.x - t->tl_x) / (t->br_x - t->tl_x);
+    double wx = fmin(1.0, (p.y - t->tl_y) / (t->br_y - t->tl_y));
+    double c = 0.94;
+    if (wx <= c)
+        p.x += 1.45 * shift * (pow(c, 0.7) - pow(fabs(wx - c), 0.55)) * wy * wy * wy;
+    else if (wx < 1.1 && wx > c)
+        p.x += 1.45 * shift * (pow(c, 0.7) - pow(fabs(wx - c), 0.55)) * wy * wy;
+    if (wy < 1.01)
+        p.y += -0.9 * shift * fmin(pow(wy, 6), 1.) * pow(wx, 6);
+    else if (wy >= 0.99)
+        p.y += -0.9 * shift * fmin(pow(wy, 6), 1.) * wx * wx * wx * wx;
+    *pp = p;
+}
+
+/* actions_gpu.cu:1022-1031: mapping_data_copy, mesh_shift, reset_FEM_matrix,
+ * stiffness_mass_matrix_construction (colour by colour), dirichlet_bdc (colour by colour, fixed
+ * element by fixed element: FEM_thermal_gpu.cu:234-324), inverse_mass_matrix,
+ * nodal_value_interpolation (the layer variant, tool_FEM.cu:57-100, 434-443) */
+void o_remesh_low_level(const o_consts *c, o_tool *t, int freq) {
+    const int nn = t->n_nodes;
+    for (int i = 0; i < nn; i++) {
+        t->pos_map[i] = t->node_pos[i];
+        t->T_map[i] = t->T[i];
+    }
+    for (int i = 0; i < nn; i++)
+        if (!(t->node_flags[i] & 3)) o_mesh_shift_node(t, freq, &t->node_pos[i]);
+    for (int k = 0; k < t->nnz; k++) t->K[k] = 0.;
+    for (int i = 0; i < nn; i++) t->M[i] = t->f[i] = 0.;
+    int ncol = 0;
+    for (int e = 0; e < t->n_elem; e++) ncol = t->elem[e].w > ncol ? t->elem[e].w : ncol;
+    for (int colour = 1; colour <= ncol; colour++)
+        for (int e = 0; e < t->n_elem; e++) {
+            if (t->elem[e].w != colour) continue;
+            double mass, Ke[3][3];
+            element_matrices(t, e, &c->tc, &mass, Ke);
+            int nd[3] = {t->elem[e].x, t->elem[e].y, t->elem[e].z};
+            for (int a = 0; a < 3; a++) {
+                t->M[nd[a]] += mass;
+                for (int b = 0; b < 3; b++) t->K[o_find(t, nd[a], nd[b])] += Ke[a][b];
+            }
+        }
+    const double T0 = c->tc.T0;
+    for (int colour = 1; colour <= ncol; colour++)
+        for (int j = 0; j < t->n_fixed; j++) {
+            o_i4 el = t->elem[t->fixed_elems[j]];
+            if (el.w != colour) continue;
+            o_i4 fl = t->fixed_flags[j];
+            int nf[3] = {0, 0, 0};
+            if (fl.x) nf[0] = nf[1] = 1;
+            if (fl.y) nf[1] = nf[2] = 1;
+            if (fl.z) nf[2] = nf[0] = 1;
+            int nd[3] = {el.x, el.y, el.z};
+            for (int q = 0; q < 3; q++) {
+                if (!nf[q]) continue;
+                int node = nd[q];
+                for (int k = t->row_ptr[node]; k < t->row_ptr[node + 1]; k++) {
+                    int loc = t->col_ind[k];
+                    t->f[loc] += -t->K[k] * T0;
+                    t->K[k] = 0.;
+                    t->K[o_find(t, loc, node)] = 0.;
+                    if (loc == node) t->K[k] = 1.0;
+                }
+                t->M[node] = 1.0;
+                t->f[node] = T0;
+                t->T_old[node] = T0;
+            }
+        }
+    for (int i = 0; i < nn; i++) t->Minv[i] = 1. / t->M[i];
+    for (int i = 0; i < nn; i++) {
+        o_d2 x = t->node_pos[i];
+        double T = t->T_map[i];
+        for (int e = 0; e < t->n_elem; e++) {
+            o_i4 el = t->elem[e];
+            o_d2 xA = t->pos_map[el.x];
+            o_d2 xB = {t->pos_map[el.y].x - xA.x, t->pos_map[el.y].y - xA.y};
+            o_d2 xC = {t->pos_map[el.z].x - xA.x, t->pos_map[el.z].y - xA.y};
+            o_d2 xP = {x.x - xA.x, x.y - xA.y};
+            o_d2 xBC = {xB.x - xC.x, xB.y - xC.y};
+            double d = o_cross(xB, xC);
+            if (fabs(d) > 1e-13) {
+                double wA = o_cross(xB, xC) + o_cross(xP, xBC);
+                double wB = o_cross(xP, xC);
+                double wC = -o_cross(xP, xB);
+                if (wA >= 0 && wB >= 0 && wC >= 0 && wA <= 1 && wB <= 1 && wC <= 1) {
+                    T = (t->T_map[el.x] * wA + t->T_map[el.y] * wB + t->T_map[el.z] * wC) / d;
+                    break;
+                }
+            }
+        }
+        t->T[i] = T;
+    }
+}
+
+void o_interp_new(const o_tool *t, const o_d2 *new_pos, int n_new, double *T_new) {
+    for (int i = 0; i < n_new; i++) {
+        o_d2 x = new_pos[i];
+        for (int e = 0; e < t->n_elem; e++) {
+            o_i4 el = t->elem[e];
+            o_d2 xA = t->node_pos[el.x], xB = t->node_pos[el.y], xC = t->node_pos[el.z];
+            o_d2 pts[3] = {xA, xB, xC};
+            int ids[3] = {el.x, el.y, el.z}, hit = -1;
+            for (int q = 0; q < 3 && hit < 0; q++) {
+                double dx = x.x - pts[q].x, dy = x.y - pts[q].y;
+                if ((double)sqrtf((float)(dx * dx + dy * dy)) < 1e-10) hit = ids[q];
+            }
+            if (hit >= 0) {
+                T_new[i] = t->T[hit];
+                break;
+            }
+            o_d2 xAC = {xA.x - xC.x, xA.y - xC.y}, xBA = {xB.x - xA.x, xB.y - xA.y};
+            o_d2 xCB = {xC.x - xB.x, xC.y - xB.y};
+            o_d2 xPC = {x.x - xC.x, x.y - xC.y}, xPA = {x.x - xA.x, x.y - xA.y}, xPB = {x.x - xB.x, x.y - xB.y};
+            double d = o_cross(xBA, xAC);
+            if (fabs(d) > 0) {
+                double wC = o_cross(xBA, xPA) / d;
+                double wA = o_cross(xCB, xPB) / d;
+                double wB = o_cross(xAC, xPC) / d;
+                if ((wA >= -1e-8 && wB >= -1e-8 && wC >= -1e-8) || (wA <= 1e-8 && wB <= 1e-8 && wC <= 1e-8)) {
+                    double TA = t->T[el.x], TB = t->T[el.y], TC = t->T[el.z];
+                    double Tn = fabs(TA * wA + TB * wB + TC * wC);
+                    double lo = fmin(fmin(TA, TB), TC);
+                    if (Tn < lo) Tn = lo;
+                    double hi = fmax(fmax(TA, TB), TC);
+                    if (Tn > hi) Tn = hi;
+                    T_new[i] = Tn;
+                    break;
+                }
+            }
+        }
+    }
+}
+
 /* actions_gpu.cu:1009-1020 -> FEM_thermal_gpu.cu:923-951, tool_FEM.cu:299-432 (USUI_RAKE_FLANK,
  * RESTRICT_WEAR_TEMP), :520-548, :559-587, FEM_thermal_gpu.cu:326-332 */
 void o_wear_adjustment(const o_consts *c, o_tool *t) {

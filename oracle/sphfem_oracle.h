@@ -69,6 +69,10 @@ typedef struct {
     /* host-side kinematics of tool_FEM (tool_FEM.cu:1552-1570) */
     double vel_x, vel_y, tl_x, tl_y, br_x, br_y, ref_dist, lowest_point, right_point;
     double flank_retreat, flank_contact;   /* __device__ globals tool_FEM.cu:19-20 */
+    /* remeshing: node flags (bit 0 cutting edge, bit 1 fixed), fixed-boundary elements and their
+     * edge flags (mesh_structure_GPU.h:88-115), mapping buffers (tool_FEM.cu:1611-1622) */
+    int *node_flags; int n_fixed; int *fixed_elems; o_i4 *fixed_flags;
+    o_d2 *pos_map; double *T_map;
 } o_tool;
 
 /* rigid (SPH) tool: geometric_primitives.h:15-57, tool.h:109-138, tool_gpu.cu:16-22 */
@@ -128,6 +132,10 @@ void o_segments_wear_rate_reset(o_tool *t);
 void o_segments_update(o_tool *t, double dt);
 void o_tool_update(o_tool *t, double dt);
 void o_wear_adjustment(const o_consts *c, o_tool *t);
+/* remesh_low_level (actions_gpu.cu:1022-1031) */
+void o_remesh_low_level(const o_consts *c, o_tool *t, int freq);
+/* do_nodal_value_interpolation (tool_FEM.cu:445-517) from the tool's current mesh onto new_pos */
+void o_interp_new(const o_tool *t, const o_d2 *new_pos, int n_new, double *T_new);
 
 /* tool.cpp:316-406 constructors (r = 0: sharp), :202-212 update_tool, :178-188 low,
  * :147-176 safe_bb (out = bbmin_x, bbmax_x, bbmin_y, bbmax_y), :95-110 inside,

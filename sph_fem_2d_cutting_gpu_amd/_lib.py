@@ -62,7 +62,8 @@ class ToolDesc(C.Structure):
                 + [(f, _vp) for f in ("position", "node_num_el", "layer", "csr_row_ptr", "csr_col_ind", "K", "M",
                                       "f", "T", "left", "right", "n", "marks")]
                 + [("vel", Double2), ("tl_point_ref", Double2), ("br_point", Double2),
-                   ("ref_dist", _dbl), ("lowest_point", _dbl), ("right_point", _dbl)])
+                   ("ref_dist", _dbl), ("lowest_point", _dbl), ("right_point", _dbl),
+                   ("node_flags", _vp), ("fixed_num", C.c_int), ("fixed_elems", _vp), ("fixed_flags", _vp)])
 
 
 TOOL_MAX_SEG = 5
@@ -194,6 +195,10 @@ _SIGS = {
     "sphfem_mesh_set_temperature": (_vp, _vp, C.c_long),
     "sphfem_mesh_write_vtk": (_vp, C.c_char_p),
     "sphfem_mesh_tool_desc": (_vp, _P(ToolConstants), Double2, Double2, _P(ToolDesc)),
+    "sphfem_tool_remesh_low_level": (_ctx, _vp, C.c_int),
+    "sphfem_tool_remesh": (_ctx, _vp, _vp),
+    "sphfem_tool_write_remesh_geo": (_ctx, _vp, C.c_char_p),
+    "sphfem_tool_get_sizes": (_vp, _P(C.c_int), _P(C.c_int), _P(C.c_int), _P(C.c_int)),
     "sphfem_timing_enable": (_ctx, C.c_int),
     "sphfem_timing_reset": (_ctx,),
     "sphfem_timing_select": (_ctx, C.c_char_p),

@@ -252,6 +252,8 @@ class ToolMesh:
     ref_dist: float
     lowest_point: float
     right_point: float
+    fixed_elems: np.ndarray | None = None   # fixed-boundary elements and their edge flags (e,3), as read
+    fixed_flags: np.ndarray | None = None   # from a mesh file; None: derived from the boundary edges
 
 
 def _normalize_glm(v):
@@ -497,6 +499,23 @@ def tool_mesh_edges(tm: ToolMesh):
     fx = tm.fixed_edge_node
     fixed = [k for k, c in count.items() if c == 1 and fx[k[0]] and fx[k[1]] and k not in cut_set]
     return cut, sorted(fixed)
+
+
+def fixed_boundary(tm: ToolMesh):
+    """(fixed-boundary elements, their edge flags x: 0-1, y: 1-2, z: 2-0) as mesh_read sets them
+    (mesh_structure_GPU.cu:546-556): the mesh file's when the tool was read from one, else from the
+    boundary edges."""
+    if tm.fixed_elems is not None:
+        return np.asarray(tm.fixed_elems, np.int32), np.asarray(tm.fixed_flags, np.int32)
+    _, fixed = tool_mesh_edges(tm)
+    want = set(fixed)
+    tri = tm.elems[:, :3]
+    fl = np.zeros((len(tri), 3), np.int32)
+    for e, t in enumerate(tri):
+        for k, (u, v) in enumerate(((t[0], t[1]), (t[1], t[2]), (t[2], t[0]))):
+            fl[e, k] = (int(min(u, v)), int(max(u, v))) in want
+    sel = np.nonzero(fl.any(axis=1))[0].astype(np.int32)
+    return sel, fl[sel]
 
 
 def write_inp(tm: ToolMesh, path: str, gmsh: bool = False, unused_nodes=()):
@@ -1128,6 +1147,8 @@ def initial_tool(case: Case) -> dict | None:
         heat_exchange=np.zeros((s, 2)), fric_heat=np.zeros(s), wear_rate=np.zeros((s, 2)),
         wear_nodes=np.zeros(s), physical_para=np.zeros((s, 4)), sliding_force=np.zeros(s),
         marks=tm.seg_marks.copy(),
+        node_flags=(tm.cut_edge_node.astype(np.int32) | (tm.fixed_edge_node.astype(np.int32) << 1)),
+        fixed_elems=fixed_boundary(tm)[0], fixed_flags=fixed_boundary(tm)[1],
         scalars=dict(vel_x=float(case.tool_vel[0]), vel_y=float(case.tool_vel[1]),
                      tl_x=float(tm.tl[0]), tl_y=float(tm.tl[1]), br_x=float(tm.br[0]), br_y=float(tm.br[1]),
                      ref_dist=float(tm.ref_dist), lowest_point=float(tm.lowest_point),

@@ -1037,6 +1037,60 @@ int sphfem_tool_create(sphfem_ctx *ctx, const sphfem_tool_desc *d, sphfem_tool *
         k.flank_retreat = 0.; k.flank_contact = d->lowest_point;
         t->kin = dupload(s, &k, 1);
         t->host_kin = k;
+        if (d->node_flags && d->fixed_num > 0 && d->fixed_elems && d->fixed_flags) {
+            // Dirichlet rows in the order do_dirichlet_bdc_p1q3 applies them (colour, fixed element,
+            // node); a node met again is a no-op.  Free node x gathers the entries (F, x) of the rows
+            // F that reach it, in that order.
+            int max_colour = 0;
+            for (int j = 0; j < d->fixed_num; j++) {
+                const int e = d->fixed_elems[j];
+                if (e < 0 || e >= ne) throw std::runtime_error("fixed-boundary element out of range");
+                max_colour = std::max(max_colour, d->node_num_el[e].w);
+            }
+            std::vector<int> order, dir(nn, 0);
+            for (int c = 1; c <= max_colour; c++)
+                for (int j = 0; j < d->fixed_num; j++) {
+                    const sphfem_int4 el = d->node_num_el[d->fixed_elems[j]];
+                    if (el.w != c) continue;
+                    const sphfem_int4 fl = d->fixed_flags[j];
+                    int nf[3] = {0, 0, 0};
+                    if (fl.x) nf[0] = nf[1] = 1;
+                    if (fl.y) nf[1] = nf[2] = 1;
+                    if (fl.z) nf[2] = nf[0] = 1;
+                    const int nd[3] = {el.x, el.y, el.z};
+                    for (int q = 0; q < 3; q++)
+                        if (nf[q] && !dir[nd[q]]) {
+                            dir[nd[q]] = 1;
+                            order.push_back(nd[q]);
+                        }
+                }
+            // the reassembly writes every element entry: all must be in the pattern
+            for (int e = 0; e < ne; e++) {
+                const sphfem_int4 el = d->node_num_el[e];
+                const int nd[3] = {el.x, el.y, el.z};
+                for (int a = 0; a < 3; a++) {
+                    if (nd[a] < 0 || nd[a] >= nn) throw std::runtime_error("element node out of range");
+                    for (int b = 0; b < 3; b++) (void)find(nd[a], nd[b]);
+                }
+            }
+            std::vector<std::vector<int>> reach(nn);
+            for (int F : order)
+                for (int q = d->csr_row_ptr[F]; q < d->csr_row_ptr[F + 1]; q++)
+                    if (!dir[d->csr_col_ind[q]]) reach[d->csr_col_ind[q]].push_back(q);
+            std::vector<int> dp(nn + 1, 0), de;
+            for (int i = 0; i < nn; i++) {
+                de.insert(de.end(), reach[i].begin(), reach[i].end());
+                dp[i + 1] = (int)de.size();
+            }
+            t->node_flags = dupload(s, d->node_flags, nn);
+            t->dir_node = dupload(s, dir.data(), nn);
+            t->dir_ptr = dupload(s, dp.data(), nn + 1);
+            if (de.empty()) de.push_back(0);
+            t->dir_entry = dupload(s, de.data(), de.size());
+            t->pos_map = dalloc<sphfem_double2>(nn);
+            t->T_map = dalloc<double>(nn);
+            t->remeshable = true;
+        }
         SPHFEM_HIP_CHECK(hipStreamSynchronize(s));
         *out = t;
         return SPHFEM_OK;
@@ -1054,10 +1108,65 @@ int sphfem_tool_destroy(sphfem_ctx *ctx, sphfem_tool *t) {
         dfree(t->physical_para); dfree(t->sliding_force); dfree(t->marks); dfree(t->node_robin);
         dfree(t->node_shift_ptr); dfree(t->node_shift); dfree(t->layer_elems); dfree(t->layer_old);
         dfree(t->layer_new); dfree(t->nnz_layer_ptr); dfree(t->nnz_layer); dfree(t->node_layer_ptr);
-        dfree(t->node_layer); dfree(t->succ); dfree(t->kin);
+        dfree(t->node_layer); dfree(t->kin); dfree(t->node_flags); dfree(t->dir_node); dfree(t->dir_ptr);
+        dfree(t->dir_entry); dfree(t->pos_map); dfree(t->T_map);
         delete t;
         return SPHFEM_OK;
     });
+}
+
+int sphfem_tool_remesh_low_level(sphfem_ctx *ctx, sphfem_tool *t, int freq) {
+    if (!t) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (!t->remeshable)
+            return fail(ctx, SPHFEM_ERR_STATE, "remeshing needs a tool created with node flags and the fixed boundary");
+        if (int rc = need(ctx, SET_TOOLC)) return rc;
+        launch_remesh_low_level(ctx, t, freq);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_tool_remesh(sphfem_ctx *ctx, sphfem_tool *t, sphfem_mesh *mesh) {
+    if (!t || !mesh) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (!t->remeshable)
+            return fail(ctx, SPHFEM_ERR_STATE, "remeshing needs a tool created with node flags and the fixed boundary");
+        if (int rc = need(ctx, SET_TOOLC)) return rc;
+        sphfem_tool_kinematics k;
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(&k, t->kin, sizeof(k), hipMemcpyDeviceToHost, ctx->stream));
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        sphfem_tool_desc d;
+        if (sphfem_mesh_tool_desc(mesh, &ctx->c.tc, sphfem_double2{k.tl_x, k.tl_y}, sphfem_double2{k.vel_x, k.vel_y},
+                                  &d) != SPHFEM_OK)
+            return fail(ctx, SPHFEM_ERR_ARG, "the new mesh has no single cutting-edge chain");
+        sphfem_tool *nt = nullptr;
+        if (int rc = sphfem_tool_create(ctx, &d, &nt)) return rc;
+        // mesh_GPU_m(..., false) starts the new nodes at 300 K (mesh_structure_GPU.cu:43-44)
+        const std::vector<double> t300((size_t)nt->n_nodes, 300.);
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(nt->T, t300.data(), 8ul * nt->n_nodes, hipMemcpyHostToDevice, ctx->stream));
+        launch_interp_new(ctx, t, nt->pos, nt->n_nodes, nt->T);
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(nt->T_old, nt->T, 8ul * nt->n_nodes, hipMemcpyDeviceToDevice, ctx->stream));
+        sphfem_tool_kinematics nk = k;
+        nk.lowest_point = d.lowest_point;
+        nk.right_point = d.right_point;
+        nk.ref_dist = d.ref_dist;
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(nt->kin, &nk, sizeof(nk), hipMemcpyHostToDevice, ctx->stream));
+        nt->host_kin = nk;
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        const unsigned long gen = t->generation + 1;
+        std::swap(*t, *nt);
+        t->generation = gen;
+        return sphfem_tool_destroy(ctx, nt);      // the old arrays
+    });
+}
+
+int sphfem_tool_get_sizes(const sphfem_tool *t, int *node_num, int *ele_num, int *nz_num, int *seg_num) {
+    if (!t) return SPHFEM_ERR_ARG;
+    if (node_num) *node_num = t->n_nodes;
+    if (ele_num) *ele_num = t->n_elem;
+    if (nz_num) *nz_num = t->nnz;
+    if (seg_num) *seg_num = t->n_seg;
+    return SPHFEM_OK;
 }
 
 int sphfem_tool_get_kinematics(sphfem_ctx *ctx, sphfem_tool *t, sphfem_tool_kinematics *out) {
@@ -1386,6 +1495,8 @@ struct sphfem_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int steps = 0;
+    sphfem_tool *tool = nullptr;       // the FEM tool captured and its array generation then
+    unsigned long tool_gen = 0;
 };
 
 extern "C" {
@@ -1404,6 +1515,8 @@ static int graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles 
         ctx->timing.enabled = false;
         auto *gr = new sphfem_graph();
         gr->steps = steps;
+        gr->tool = t;
+        gr->tool_gen = t ? t->generation : 0;
         try {
             SPHFEM_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             // recording only: the device state does not advance; after an even number of steps the
@@ -1439,6 +1552,8 @@ int sphfem_leap_frog_graph_create_rigid(sphfem_ctx *ctx, sphfem_leap_frog *lf, s
 int sphfem_leap_frog_graph_launch(sphfem_ctx *ctx, sphfem_graph *gr, int repeats) {
     if (!gr || repeats < 0) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {
+        if (gr->tool && gr->tool->generation != gr->tool_gen)
+            return fail(ctx, SPHFEM_ERR_STATE, "the tool was remeshed after this graph was captured: capture again");
         for (int r = 0; r < repeats; r++) SPHFEM_HIP_CHECK(hipGraphLaunch(gr->exec, ctx->stream));
         return SPHFEM_OK;
     });
@@ -1536,6 +1651,65 @@ int sphfem_vtk_write_rigid_tool(const sphfem_tool_geometry *g, const char *path)
     if (!g || !path) return SPHFEM_ERR_ARG;
     std::string err;
     return write_rigid_tool_vtk(*g, path, err);
+}
+
+// remesh_read_segments (tool_FEM.cu:1708-1810): the gmsh::model::geo calls written as a .geo script
+int sphfem_tool_write_remesh_geo(sphfem_ctx *ctx, sphfem_tool *t, const char *path) {
+    if (!t || !path) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        FemToolHost h = download_fem_tool(ctx, t, false, true);
+        sphfem_tool_kinematics k;
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(&k, t->kin, sizeof(k), hipMemcpyDeviceToHost, ctx->stream));
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        const int ns = h.n_seg;
+        // the distance of distance-like helpers there: sqrtf of the double expression
+        auto fdist = [](sphfem_double2 a, sphfem_double2 b) {
+            return (double)sqrtf((float)((a.x - b.x) * (a.x - b.x) + (a.y - b.y) * (a.y - b.y)));
+        };
+        struct Pt {
+            double x, y, size;
+        };
+        std::vector<Pt> pts;
+        std::vector<int> cut;
+        double length = 0.004, dist_prev = 0.;
+        const bool down = h.left[0].y > h.right[0].y;    // chain starts at left ends
+        for (int i = 0; i < ns; i++) {
+            const sphfem_double2 p = down ? h.left[(size_t)i] : h.right[(size_t)i];
+            // the reference reads one element past the arrays at the last segment; there the
+            // result cannot matter (the loop ends either way), so no distance is taken
+            const double judge = i + 1 < ns ? (down ? fdist(h.left[(size_t)i], h.right[(size_t)i + 1])
+                                                    : fdist(h.left[(size_t)i + 1], h.right[(size_t)i]))
+                                            : HUGE_VAL;
+            const double buffer = fdist(h.left[(size_t)i], h.right[(size_t)i]);
+            length = 6.0 * buffer;
+            pts.push_back({p.x, p.y, length});
+            cut.push_back(1 + (int)pts.size());
+            if ((judge < 1.25 * buffer && i != ns - 1 && dist_prev > 0.7 * buffer) || judge < 0.00013) i++;
+            dist_prev = buffer;
+        }
+        const sphfem_double2 last = down ? h.right[(size_t)ns - 1] : h.left[(size_t)ns - 1];
+        pts.push_back({last.x, last.y, length});
+        std::FILE *fp = std::fopen(path, "w");
+        if (!fp) return fail(ctx, SPHFEM_ERR_ARG, std::string("cannot write ") + path);
+        const int np = (int)pts.size() + 1;     // with the reference point
+        std::fprintf(fp, "// tool remesh geometry (remesh_read_segments): mesh with\n"
+                         "//   gmsh -2 -format inp -o tool_gmsh_remeshed.inp <this file>\n");
+        std::fprintf(fp, "Point(1) = {%.17g, %.17g, 0, %.17g};\n", k.tl_x, k.tl_y, 0.075);
+        for (size_t q = 0; q < pts.size(); q++)
+            std::fprintf(fp, "Point(%d) = {%.17g, %.17g, 0, %.17g};\n", (int)q + 2, pts[q].x, pts[q].y, pts[q].size);
+        for (int q = 1; q < np; q++) std::fprintf(fp, "Line(%d) = {%d, %d};\n", q, q, q + 1);
+        std::fprintf(fp, "Line(%d) = {%d, 1};\n", np, np);
+        std::fprintf(fp, "Curve Loop(1) = {");
+        for (int q = 1; q <= np; q++) std::fprintf(fp, q < np ? "%d, " : "%d};\n", q);
+        std::fprintf(fp, "Plane Surface(1) = {1};\n");
+        std::fprintf(fp, "Physical Curve(\"CuttingEdge\", 67) = {");
+        for (size_t q = 0; q < cut.size(); q++) std::fprintf(fp, q + 1 < cut.size() ? "%d, " : "%d};\n", cut[q]);
+        std::fprintf(fp, "Physical Curve(\"FixedBoundary\", 2) = {1, %d};\n", np);
+        std::fprintf(fp, "Mesh.MshFileVersion = 4;\nMesh.SaveAll = 1;\n");
+        const bool ok = std::ferror(fp) == 0;
+        if (std::fclose(fp) != 0 || !ok) return fail(ctx, SPHFEM_ERR_ARG, std::string("cannot write ") + path);
+        return SPHFEM_OK;
+    });
 }
 
 int sphfem_vtk_write_FEM_tool(sphfem_ctx *ctx, sphfem_tool *t, const char *path) {

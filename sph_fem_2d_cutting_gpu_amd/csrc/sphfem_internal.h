@@ -139,9 +139,17 @@ struct sphfem_tool {
     int *nnz_layer = nullptr;
     int *node_layer_ptr = nullptr;     // [n_nodes + 1]
     int *node_layer = nullptr;         // layer slot * 4 + a
-    int *succ = nullptr;               // [n_seg] first segment j with left[j] == right[s], else -1
     sphfem::ToolKin *kin = nullptr;    // device
     sphfem_tool_kinematics host_kin{};
+    // remeshing (present when the description carried node flags and the fixed boundary)
+    bool remeshable = false;
+    int *node_flags = nullptr;         // [n_nodes] bit 0 cutting edge, bit 1 fixed boundary
+    int *dir_node = nullptr;           // [n_nodes] 1 if a Dirichlet row
+    int *dir_ptr = nullptr;            // [n_nodes + 1] CSR: entries (F, x) of the Dirichlet rows F
+    int *dir_entry = nullptr;          //   that reach node x, in the order the rows are applied
+    sphfem_double2 *pos_map = nullptr; // [n_nodes] positions and temperatures before a remesh
+    double *T_map = nullptr;
+    unsigned long generation = 0;      // bumped when the device arrays are replaced
 };
 
 // rigid (SPH) tool of tool_gpu.cu: the geometry and contact constants in device memory
@@ -234,6 +242,9 @@ void launch_tool_begin(sphfem_ctx *ctx, sphfem_tool *t);
 void launch_tool_fem_end(sphfem_ctx *ctx, sphfem_tool *t);
 void launch_tool_move(sphfem_ctx *ctx, sphfem_tool *t, double dt, bool segments, bool nodes);
 void launch_wear(sphfem_ctx *ctx, sphfem_tool *t);
+void launch_remesh_low_level(sphfem_ctx *ctx, sphfem_tool *t, int freq);
+void launch_interp_new(sphfem_ctx *ctx, const sphfem_tool *old_t, const sphfem_double2 *new_pos, int n_new,
+                       double *T_new);
 void launch_fem_reset(sphfem_ctx *ctx, sphfem_tool *t);
 void launch_fem_temp(sphfem_ctx *ctx, sphfem_tool *t, double dt);
 void launch_fem_robin(sphfem_ctx *ctx, sphfem_tool *t);

@@ -36,8 +36,8 @@ struct sphfem_mesh {
     std::vector<double> T_init;                    // hot tool (mesh_read_temperature), empty if none
     // arrays of the last sphfem_mesh_tool_desc
     std::vector<sphfem_double2> d_pos, d_left, d_right, d_n;
-    std::vector<sphfem_int4> d_marks;
-    std::vector<int> d_layer;
+    std::vector<sphfem_int4> d_marks, d_fixed_flags;
+    std::vector<int> d_layer, d_node_flags;
     std::vector<double> d_K, d_M, d_f, d_T;
 };
 
@@ -825,6 +825,18 @@ int sphfem_mesh_tool_desc(sphfem_mesh *mp, const sphfem_tool_constants *tc, sphf
     out->ref_dist = dist_max;
     out->lowest_point = y_min;
     out->right_point = x_max;
+    m.d_node_flags.resize((size_t)nn);
+    for (int i = 0; i < nn; i++)
+        m.d_node_flags[(size_t)i] = (m.cut_node[(size_t)i] ? 1 : 0) | (m.fixed_node[(size_t)i] ? 2 : 0);
+    m.d_fixed_flags.resize(m.fixed_elems.size());
+    for (size_t j = 0; j < m.fixed_elems.size(); j++) {
+        const auto &fl = m.fixed_flags[(size_t)m.fixed_elems[j]];
+        m.d_fixed_flags[j] = {fl[0], fl[1], fl[2], 0};
+    }
+    out->node_flags = m.d_node_flags.data();
+    out->fixed_num = (int)m.fixed_elems.size();
+    out->fixed_elems = m.fixed_elems.data();
+    out->fixed_flags = m.d_fixed_flags.data();
     return SPHFEM_OK;
 }
 

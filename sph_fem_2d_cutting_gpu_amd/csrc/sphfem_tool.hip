@@ -335,6 +335,198 @@ void launch_wear(sphfem_ctx *ctx, sphfem_tool *t) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// remeshing (actions_gpu.cu:1022-1038)
+// ---------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ double cross2(sphfem_double2 a, sphfem_double2 b) { return a.x * b.y - a.y * b.x; }
+
+// mapping_data_copy (tool_FEM.cu:1611-1622) fused with mesh_shift (:589-616): the interior nodes
+// (neither cutting edge nor fixed boundary) move towards the worn flank by a profile of the
+// reference points tl / br, scaled by freq times the flank retreat (at least 6.5e-6 inwards)
+__global__ void __launch_bounds__(256) k_map_and_shift(sphfem_tool t, int freq) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= t.n_nodes) return;
+    sphfem_double2 p = t.pos[i];
+    t.pos_map[i] = p;
+    t.T_map[i] = t.T[i];
+    if (t.node_flags[i] & 3) return;
+    const ToolKin k = *t.kin;
+    double shift = k.flank_retreat * 1.0 * freq;
+    shift = fmin(shift, -0.0000065);
+    const double wy = (p.x - k.tl_x) / (k.br_x - k.tl_x);
+    const double wx = fmin(1.0, (p.y - k.tl_y) / (k.br_y - k.tl_y));
+    const double c = 0.94;
+    if (wx <= c)
+        p.x += 1.45 * shift * (pow(c, 0.7) - pow(fabs(wx - c), 0.55)) * wy * wy * wy;
+    else if (wx < 1.1 && wx > c)
+        p.x += 1.45 * shift * (pow(c, 0.7) - pow(fabs(wx - c), 0.55)) * wy * wy;
+    if (wy < 1.01)
+        p.y += -0.9 * shift * fmin(pow(wy, 6.0), 1.) * pow(wx, 6.0);
+    else if (wy >= 0.99)
+        p.y += -0.9 * shift * fmin(pow(wy, 6.0), 1.) * wx * wx * wx * wx;
+    t.pos[i] = p;
+}
+
+// reset_FEM_matrix (FEM_thermal_gpu.cu:702-720)
+__global__ void k_fem_zero(sphfem_tool t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < t.nnz) t.K[i] = 0.;
+    if (i < t.n_nodes) {
+        t.M[i] = 0.;
+        t.f[i] = 0.;
+    }
+}
+
+__device__ __forceinline__ int csr_entry(const sphfem_tool &t, int row, int col) {
+    int k = t.row_ptr[row];
+    const int e = t.row_ptr[row + 1];
+    while (k < e && t.col_ind[k] != col) k++;
+    return k;
+}
+
+// do_stiffness_mass_matrix_construction_p1q3 (FEM_thermal_gpu.cu:126-225) for one colour: elements
+// of one colour share no node, so every entry takes at most one term per launch
+__global__ void __launch_bounds__(256) k_fem_assemble(DevConsts c, sphfem_tool t, int colour) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= t.n_elem) return;
+    const sphfem_int4 el = t.elem[e];
+    if (el.w != colour) return;
+    double m[10];
+    element_matrices(c, t, e, m);
+    const int nd[3] = {el.x, el.y, el.z};
+    for (int a = 0; a < 3; a++) {
+        t.M[nd[a]] += m[0];
+        for (int b = 0; b < 3; b++) t.K[csr_entry(t, nd[a], nd[b])] += m[1 + 3 * a + b];
+    }
+}
+
+// do_dirichlet_bdc_p1q3 (FEM_thermal_gpu.cu:234-324) applied row by row in (colour, element, node)
+// order.  Two passes so no thread reads an entry another clears: the rhs of a free node x takes
+// -K(F, x) T0 of every Dirichlet row F reaching it, in that order (the assembled K(F, x): a free
+// node's column is never cleared); then the Dirichlet rows become identity rows and their
+// columns zero.
+__global__ void __launch_bounds__(256) k_dirichlet_rhs(DevConsts c, sphfem_tool t) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= t.n_nodes) return;
+    const double T0 = c.tc.T0;
+    if (t.dir_node[x]) {
+        t.M[x] = 1.0;
+        t.f[x] = T0;
+        t.T_old[x] = T0;
+        return;
+    }
+    double f = t.f[x];
+    for (int q = t.dir_ptr[x]; q < t.dir_ptr[x + 1]; q++) f += -t.K[t.dir_entry[q]] * T0;
+    t.f[x] = f;
+}
+
+__global__ void __launch_bounds__(256) k_dirichlet_rows(sphfem_tool t) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= t.n_nodes) return;
+    const bool dx = t.dir_node[x] != 0;
+    for (int k = t.row_ptr[x]; k < t.row_ptr[x + 1]; k++) {
+        const int col = t.col_ind[k];
+        if (dx) t.K[k] = col == x ? 1.0 : 0.;
+        else if (t.dir_node[col]) t.K[k] = 0.;
+    }
+}
+
+// inverse_mass_matrix (FEM_thermal_gpu.cu:838-847)
+__global__ void k_fem_minv(sphfem_tool t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < t.n_nodes) t.Minv[i] = 1. / t.M[i];
+}
+
+// do_nodal_value_interpolation_old / value_interpolation_in_layer_element (tool_FEM.cu:57-100,
+// 434-443): the first element of the mapping (in element order, counter-clockwise, weights
+// unnormalised) that holds the node gives its temperature; none keeps the mapped value
+__global__ void __launch_bounds__(256) k_interp_layer(sphfem_tool t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= t.n_nodes) return;
+    const sphfem_double2 x = t.pos[i];
+    double T = t.T_map[i];
+    for (int e = 0; e < t.n_elem; e++) {
+        const sphfem_int4 el = t.elem[e];
+        const sphfem_double2 xA = t.pos_map[el.x];
+        const sphfem_double2 pb = t.pos_map[el.y], pc = t.pos_map[el.z];
+        const sphfem_double2 xB = {pb.x - xA.x, pb.y - xA.y}, xC = {pc.x - xA.x, pc.y - xA.y};
+        const sphfem_double2 xP = {x.x - xA.x, x.y - xA.y}, xBC = {xB.x - xC.x, xB.y - xC.y};
+        const double d = cross2(xB, xC);
+        if (fabs(d) > 1e-13) {
+            const double wA = cross2(xB, xC) + cross2(xP, xBC);
+            const double wB = cross2(xP, xC);
+            const double wC = -cross2(xP, xB);
+            if (wA >= 0 && wB >= 0 && wC >= 0 && wA <= 1 && wB <= 1 && wC <= 1) {
+                T = (t.T_map[el.x] * wA + t.T_map[el.y] * wB + t.T_map[el.z] * wC) / d;
+                break;
+            }
+        }
+    }
+    t.T[i] = T;
+}
+
+void launch_remesh_low_level(sphfem_ctx *ctx, sphfem_tool *t, int freq) {
+    KTimer tm(ctx, "remesh");
+    hipStream_t s = ctx->stream;
+    const int nn = t->n_nodes;
+    k_map_and_shift<<<nblk(nn, 256), 256, 0, s>>>(*t, freq);
+    k_fem_zero<<<nblk(std::max(t->nnz, nn), 256), 256, 0, s>>>(*t);
+    for (int c = 1; c <= t->n_colors; c++) k_fem_assemble<<<nblk(t->n_elem, 256), 256, 0, s>>>(ctx->c, *t, c);
+    k_dirichlet_rhs<<<nblk(nn, 256), 256, 0, s>>>(ctx->c, *t);
+    k_dirichlet_rows<<<nblk(nn, 256), 256, 0, s>>>(*t);
+    k_fem_minv<<<nblk(nn, 256), 256, 0, s>>>(*t);
+    k_interp_layer<<<nblk(nn, 256), 256, 0, s>>>(*t);
+}
+
+// do_nodal_value_interpolation (tool_FEM.cu:445-517): temperature of a new node from the first
+// element of the current mesh that holds it (a coincident node within 1e-10 gives its value;
+// barycentric weights with a 1e-8 tolerance, the result clamped to the element's range);
+// T_new keeps its value where no element holds the node
+__global__ void __launch_bounds__(256) k_interp_new(sphfem_tool t, const sphfem_double2 *new_pos, int n_new,
+                                                    double *T_new) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_new) return;
+    const sphfem_double2 x = new_pos[i];
+    auto near = [&](sphfem_double2 a) {
+        const double dx = x.x - a.x, dy = x.y - a.y;
+        return (double)sqrtf((float)(dx * dx + dy * dy)) < 1e-10;
+    };
+    for (int e = 0; e < t.n_elem; e++) {
+        const sphfem_int4 el = t.elem[e];
+        const sphfem_double2 xA = t.pos[el.x], xB = t.pos[el.y], xC = t.pos[el.z];
+        if (near(xA)) { T_new[i] = t.T[el.x]; return; }
+        if (near(xB)) { T_new[i] = t.T[el.y]; return; }
+        if (near(xC)) { T_new[i] = t.T[el.z]; return; }
+        const sphfem_double2 xAC = {xA.x - xC.x, xA.y - xC.y}, xBA = {xB.x - xA.x, xB.y - xA.y};
+        const sphfem_double2 xCB = {xC.x - xB.x, xC.y - xB.y};
+        const sphfem_double2 xPC = {x.x - xC.x, x.y - xC.y}, xPA = {x.x - xA.x, x.y - xA.y};
+        const sphfem_double2 xPB = {x.x - xB.x, x.y - xB.y};
+        const double d = cross2(xBA, xAC);
+        if (fabs(d) > 0) {
+            const double wC = cross2(xBA, xPA) / d;
+            const double wA = cross2(xCB, xPB) / d;
+            const double wB = cross2(xAC, xPC) / d;
+            if ((wA >= -1e-8 && wB >= -1e-8 && wC >= -1e-8) || (wA <= 1e-8 && wB <= 1e-8 && wC <= 1e-8)) {
+                const double TA = t.T[el.x], TB = t.T[el.y], TC = t.T[el.z];
+                double Tn = fabs(TA * wA + TB * wB + TC * wC);
+                const double lo = fmin(fmin(TA, TB), TC);
+                if (Tn < lo) Tn = lo;
+                const double hi = fmax(fmax(TA, TB), TC);
+                if (Tn > hi) Tn = hi;
+                T_new[i] = Tn;
+                return;
+            }
+        }
+    }
+}
+
+void launch_interp_new(sphfem_ctx *ctx, const sphfem_tool *old_t, const sphfem_double2 *new_pos, int n_new,
+                       double *T_new) {
+    KTimer tm(ctx, "remesh_interp");
+    k_interp_new<<<nblk(std::max(n_new, 1), 256), 256, 0, ctx->stream>>>(*old_t, new_pos, n_new, T_new);
+}
+
+// ---------------------------------------------------------------------------------------------
 // contact
 // ---------------------------------------------------------------------------------------------
 

@@ -294,9 +294,37 @@ class tool_FEM:
         d.tl_point_ref = _lib.Double2(sc["tl_x"], sc["tl_y"])
         d.br_point = _lib.Double2(sc["br_x"], sc["br_y"])
         d.ref_dist, d.lowest_point, d.right_point = sc["ref_dist"], sc["lowest_point"], sc["right_point"]
+        d.node_flags = arr(init["node_flags"], np.int32)
+        d.fixed_num = len(init["fixed_elems"])
+        d.fixed_elems = arr(init["fixed_elems"], np.int32)
+        ff = np.zeros((d.fixed_num, 4), np.int32)
+        ff[:, :3] = init["fixed_flags"]
+        d.fixed_flags = arr(ff, np.int32)
         self.h = C.c_void_p()
         ctx.call("sphfem_tool_create", C.byref(d), C.byref(self.h))
         self.n_nodes, self.n_seg, self.nnz = d.node_num, d.seg_num, d.nz_num
+
+    def remesh_low_level(self, freq: int):
+        """remesh_low_level(tool, freq) (actions_gpu.cu:1022-1031): interior nodes shifted by
+        mesh_shift(freq), the FEM system assembled again, temperatures interpolated"""
+        self.ctx.call("sphfem_tool_remesh_low_level", self.h, int(freq))
+
+    def remesh(self, mesh: "mesh_read"):
+        """read_remesh_gmsh (tool_FEM.cu:1811-1828) from a mesh already read: temperatures
+        interpolated onto the new nodes, segments and FEM system of the new mesh.  Graphs captured
+        before must be captured again."""
+        self.ctx.call("sphfem_tool_remesh", self.h, mesh.h)
+        self.n_nodes, _, self.nnz, self.n_seg = self.sizes()
+
+    def sizes(self):
+        """(node_number, element_number, non-zeros, cutting_segment_size) of the current mesh"""
+        v = [C.c_int() for _ in range(4)]
+        check(lib().sphfem_tool_get_sizes(self.h, *[C.byref(x) for x in v]), None, "tool_get_sizes")
+        return tuple(x.value for x in v)
+
+    def write_remesh_geo(self, path: str):
+        """remesh_read_segments (tool_FEM.cu:1708-1810) as a Gmsh .geo script"""
+        self.ctx.call("sphfem_tool_write_remesh_geo", self.h, str(path).encode())
 
     def get_vel(self):
         k = self.kinematics()

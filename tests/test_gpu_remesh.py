@@ -1,0 +1,199 @@
+"""Tool remeshing on the device (actions_gpu.cu:1022-1038) against the oracle:
+
+* remesh_low_level (mesh_shift, colour-ordered reassembly with the Dirichlet rows, interpolation
+  from the mapping) on the same tool state: positions within a few ulp (the profile's pow calls),
+  K, M, f and T to 1e-12; the steps after it to 1e-6;
+* remesh onto a new mesh read from an .inp file (read_remesh_gmsh): onto the current mesh
+  (identity, no wear) the temperatures are kept exactly and the run continues as without the
+  remesh, to the rounding of the reassembled system; onto a
+  finer mesh the interpolated temperatures equal the oracle's bit for bit and the steps after it
+  match the oracle to 1e-6;
+* a graph captured before a remesh is refused afterwards; capturing again works;
+* the .geo script of remesh_read_segments equals its Python restatement.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from helpers import max_rel_err
+from oracle import mesh_inp as R
+from oracle.oracle import Oracle
+from sph_fem_2d_cutting_gpu_amd import _lib, cases
+from sph_fem_2d_cutting_gpu_amd import solver as S
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("pos", "vel", "S", "rho", "T", "eps_pl", "eps_pl_dot", "p")
+
+
+def _case():
+    return cases.small_cutting_case(nx=48, ny=16, wear=True)
+
+
+def _linear(p):
+    """a linear temperature field above 300 K over the tool (x > 0, y > 0.05 cm)"""
+    return 300.0 + 1e3 * p[:, 0] + 5e2 * (p[:, 1] - 0.05)
+
+
+def _row_rel(a, b, tm_rp):
+    row = np.repeat(np.arange(len(tm_rp) - 1), np.diff(tm_rp))
+    scale = np.zeros(len(tm_rp) - 1)
+    np.maximum.at(scale, row, np.abs(b))
+    return float(np.max(np.abs(a - b) / np.maximum(scale[row], 1e-300)))
+
+
+def _sync_tool(sim, o):
+    """the oracle's tool state and kinematics onto the device tool"""
+    t = o.t
+    sim.tool.upload(dict(node_pos=t["node_pos"], T=t["T"], K=t["K"], M=t["M"], k=t["k"], left=t["left"],
+                         right=t["right"], n=t["n"], scalars=o.tool_scalars))
+
+
+def _compare_particles(sim, o, tol, label):
+    a, b = sim.state(), o.P
+    assert np.array_equal(a["idx"], b["idx"]), label
+    live = b["blanked"] != 1.0
+    errs = {f: max_rel_err(a[f], b[f], live) for f in STATE}
+    assert max(errs.values()) <= tol, (label, errs)
+
+
+def test_remesh_low_level_matches_oracle():
+    case = _case()
+    sim, o = S.Simulation(case), Oracle(case)
+    sim.step(1)
+    o.step(1)
+    o.t["T"][:] = _linear(o.t["node_pos"])
+    o.Tc.flank_retreat = -1e-6          # 40 x -1e-6 beyond the -6.5e-6 cap
+    _sync_tool(sim, o)
+    print("remesh_low_level after 1 step, freq 40", flush=True)
+    sim.tool.remesh_low_level(40)
+    o.remesh_low_level(40)
+    sim.ctx.check_error()
+    g = sim.tool_state()
+    rp = case.tool.row_ptr
+    moved = np.abs(o.t["node_pos"] - case.tool.nodes).max()
+    assert moved > 1e-6
+    assert np.max(np.abs(g["node_pos"] - o.t["node_pos"])) <= 1e-16
+    assert _row_rel(g["K"], o.t["K"], rp) <= 1e-12
+    assert np.allclose(g["M"], o.t["M"], rtol=1e-12, atol=0)
+    assert np.allclose(g["T"], o.t["T"], rtol=1e-12, atol=0)
+    lin = _linear(o.t["node_pos"])
+    assert np.allclose(g["T"], lin, rtol=0, atol=1e-10 * np.ptp(lin))
+    sim.step(5)
+    o.step(5)
+    sim.ctx.check_error()
+    _compare_particles(sim, o, 1e-6, "5 steps after remesh_low_level")
+    g = sim.tool_state()
+    assert np.allclose(g["T"], o.t["T"], rtol=1e-6, atol=0)
+    sim.close()
+
+
+def _mesh_from(tm, pos, path):
+    cases.write_inp(dataclasses.replace(tm, nodes=np.ascontiguousarray(pos)), path, gmsh=True)
+    return S.mesh_read(path, True)
+
+
+def test_remesh_identity_keeps_the_run(tmp_path):
+    # without wear the nodes only translate, so the lowest and right points the remesh takes from
+    # the new segments are the ones the kinematics carried (with wear they follow the worn edge).
+    # The new segments start with zero heat exchange and the new system with a zero temperature
+    # rate (tool_FEM.cu:1453-1500, FEM_thermal_gpu.cu:781), which the twin is given at the same step.
+    case = cases.small_cutting_case(nx=48, ny=16, wear=False)
+    sim, twin = S.Simulation(case), S.Simulation(case)
+    sim.step(6)
+    twin.step(6)
+    g = sim.tool_state()
+    m = _mesh_from(case.tool, g["node_pos"], str(tmp_path / "same.inp"))
+    n_seg = sim.tool.n_seg
+    print("identity remesh after 6 steps", flush=True)
+    sim.tool.remesh(m)
+    sim.ctx.check_error()
+    twin.tool.upload(dict(heat_exchange=np.zeros((n_seg, 2)), fric_heat=np.zeros(n_seg),
+                          k=np.zeros(sim.tool.n_nodes)))
+    h = sim.tool_state()
+    assert sim.tool.n_seg == n_seg and np.array_equal(h["T"], g["T"])
+    assert np.array_equal(h["node_pos"], g["node_pos"])
+    assert _row_rel(h["K"], g["K"], case.tool.row_ptr) <= 1e-12
+    sim.step(10)
+    twin.step(10)
+    a, b = sim.state(), twin.state()
+    assert np.array_equal(a["idx"], b["idx"])
+    for f in STATE:
+        assert max_rel_err(a[f], b[f], b["blanked"] != 1.0) <= 1e-10, f
+    assert np.allclose(sim.tool_state()["T"], twin.tool_state()["T"], rtol=1e-12, atol=0)
+    m.close()
+    sim.close()
+    twin.close()
+
+
+def test_remesh_onto_finer_mesh_matches_oracle(tmp_path):
+    case = _case()
+    sim, o = S.Simulation(case), Oracle(case)
+    sim.step(3)
+    o.step(3)
+    o.t["T"][:] = _linear(o.t["node_pos"]) + 2.0 * np.sin(1e3 * o.t["node_pos"][:, 0])
+    _sync_tool(sim, o)
+    sc = o.tool_scalars
+    L = 2.5 * 1.6e-3
+    fine = cases.mesh_tool(L, L, 0.0, 7.0, 4e-4, 1200, case.const.tool)
+    m = _mesh_from(fine, fine.nodes + (np.array([sc["tl_x"], sc["tl_y"]]) - fine.tl), str(tmp_path / "fine.inp"))
+    new = m.tool_mesh(case.const.tool, (sc["tl_x"], sc["tl_y"]), (sc["vel_x"], sc["vel_y"]))
+    print(f"remesh onto {len(new.nodes)} nodes (was {len(case.tool.nodes)})", flush=True)
+    sim.tool.remesh(m)
+    o.remesh(new)
+    sim.ctx.check_error()
+    g = sim.tool_state()
+    assert sim.tool.n_nodes == len(new.nodes) and sim.tool.n_seg == len(new.seg_left)
+    assert np.array_equal(g["T"], o.t["T"])
+    for f in ("node_pos", "K", "M", "left", "right", "n"):
+        assert np.array_equal(g[f], o.t[f]), f
+    k = sim.tool.kinematics()
+    for f in ("tl_x", "tl_y", "br_x", "br_y", "vel_x", "flank_retreat"):
+        assert k[f] == sc[f], f
+    assert k["lowest_point"] == new.lowest_point and k["ref_dist"] == new.ref_dist
+    sim.step(5)
+    o.step(5)
+    sim.ctx.check_error()
+    _compare_particles(sim, o, 1e-6, "5 steps after the remesh")
+    assert np.allclose(sim.tool_state()["T"], o.t["T"], rtol=1e-6, atol=0)
+    m.close()
+    sim.close()
+
+
+def test_graph_refused_after_remesh(tmp_path):
+    case = _case()
+    sim = S.Simulation(case)
+    sim.step(2)
+    gr = sim.capture(2)
+    gr.launch(1)
+    m = _mesh_from(case.tool, sim.tool_state()["node_pos"], str(tmp_path / "same.inp"))
+    sim.tool.remesh(m)
+    with pytest.raises(_lib.SphfemError, match="remeshed"):
+        gr.launch(1)
+    gr.destroy()
+    gr2 = sim.capture(2)
+    gr2.launch(2)
+    sim.ctx.check_error()
+    gr2.destroy()
+    # remesh_low_level keeps the arrays: a graph stays valid across it
+    gr3 = sim.capture(2)
+    sim.tool.remesh_low_level(20)
+    gr3.launch(1)
+    sim.ctx.check_error()
+    gr3.destroy()
+    assert np.all(np.isfinite(sim.state(("pos",))["pos"]))
+    m.close()
+    sim.close()
+
+
+def test_remesh_geo_file(tmp_path):
+    case = _case()
+    sim = S.Simulation(case)
+    sim.step(4)
+    path = str(tmp_path / "tool_remesh.geo")
+    sim.tool.write_remesh_geo(path)
+    g = sim.tool_state()
+    k = sim.tool.kinematics()
+    assert open(path).read() == R.remesh_geo(g["left"], g["right"], (k["tl_x"], k["tl_y"]))
+    sim.close()

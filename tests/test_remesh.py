@@ -1,0 +1,186 @@
+"""Tool remeshing on the oracle (actions_gpu.cu:1022-1038), pinned by what the mathematics fixes:
+
+* remesh_low_level: cutting-edge and fixed-boundary nodes stay where they are; the interior
+  displacement is linear in the shift max(freq x flank retreat, ...) capped at -6.5e-6; K, M and f
+  equal, bit for bit, an independent assembly (cases.assemble_p1 + dirichlet_rows) at the shifted
+  positions; free rows annihilate linear fields; a linear temperature field comes back exactly at
+  the shifted nodes (barycentric interpolation reproduces linear functions);
+* interpolation onto new nodes (do_nodal_value_interpolation): linear fields reproduced, coincident
+  nodes copied, points outside the tool at 300;
+* oracle.remesh onto the same mesh keeps temperatures and system; onto a finer mesh of the same
+  tool reproduces a linear field;
+* the .geo restatement of remesh_read_segments picks chain points in order, from the top end to
+  the bottom end.
+"""
+import dataclasses
+import re
+
+import numpy as np
+import pytest
+
+from oracle import mesh_inp as R
+from oracle.oracle import Oracle
+from sph_fem_2d_cutting_gpu_amd import cases
+from sph_fem_2d_cutting_gpu_amd import solver as S
+
+
+@pytest.fixture(scope="module")
+def case():
+    return cases.small_cutting_case(nx=48, ny=16, wear=True)
+
+
+def _linear(p):
+    """a linear temperature field above 300 K over the tool (x > 0, y > 0.05 cm)"""
+    return 300.0 + 1e3 * p[:, 0] + 5e2 * (p[:, 1] - 0.05)
+
+
+def _fixed_full(tm):
+    sel, fl = cases.fixed_boundary(tm)
+    full = np.zeros((len(tm.elems), 3), np.int32)
+    full[sel] = fl
+    return sel, full
+
+
+def test_low_level_shift_pins(case):
+    freq = 40
+    outs = []
+    for fr in (0.0, -1e-6):
+        o = Oracle(case)
+        o.step(3)
+        p0 = o.t["node_pos"].copy()
+        o.Tc.flank_retreat = fr
+        o.remesh_low_level(freq)
+        outs.append(o.t["node_pos"] - p0)
+    tm = case.tool
+    bnd = (tm.cut_edge_node | tm.fixed_edge_node) == 1
+    for d in outs:
+        assert np.all(d[bnd] == 0.0)
+        assert np.any(d[~bnd] != 0.0)
+    # flank retreat 0: the shift is capped at -6.5e-6; -1e-6 x 40 = -4e-5.  Each displacement is
+    # a difference of positions rounded to their ulp (7e-18 at 0.06 cm), the scaled one 6x
+    ratio = -4e-5 / -6.5e-6
+    assert np.allclose(outs[1], outs[0] * ratio, rtol=1e-9, atol=1e-16)
+
+
+def test_low_level_reassembly_and_interpolation(case):
+    tm = case.tool
+    tc = case.const.tool
+    o = Oracle(case)
+    o.step(2)
+    o.t["T"][:] = _linear(o.t["node_pos"])
+    o.remesh_low_level(25)
+    pos = o.t["node_pos"]
+    sel, full = _fixed_full(tm)
+    K, M = cases.assemble_p1(pos, tm.elems, tm.row_ptr, tm.col_ind, tc, colour_order=True, n_colors=tm.n_colors)
+    f = np.zeros(len(pos))
+    cases.dirichlet_rows(tm.elems, full, sel, tm.row_ptr, tm.col_ind, K, M, f, np.zeros(len(pos)), tc["T0"],
+                         int(tm.elems[:, 3].max()))
+    assert np.array_equal(o.t["K"], K) and np.array_equal(o.t["M"], M) and np.array_equal(o.t["f"], f)
+    assert np.array_equal(o.t["Minv"], 1.0 / M)
+    # free rows without Dirichlet neighbours annihilate constants and linear fields
+    rp, ci = tm.row_ptr, tm.col_ind
+    dir_nodes = set()
+    for e in sel:
+        for k, (a, b) in enumerate(((0, 1), (1, 2), (2, 0))):
+            if full[e, k]:
+                dir_nodes |= {int(tm.elems[e, a]), int(tm.elems[e, b])}
+    lin = _linear(pos)
+    for i in range(len(pos)):
+        cols = ci[rp[i]:rp[i + 1]]
+        if i in dir_nodes or dir_nodes.intersection(cols.tolist()):
+            continue
+        row = o.t["K"][rp[i]:rp[i + 1]]
+        scale = np.abs(row).max()
+        assert abs(row.sum()) <= 1e-12 * scale
+        if not (tm.cut_edge_node[i] or tm.fixed_edge_node[i]):
+            assert abs(row @ (lin[cols] - lin[i])) <= 1e-10 * scale * np.ptp(lin[cols])
+    # the linear temperature field at the new positions
+    assert np.allclose(o.t["T"], lin, rtol=0, atol=1e-10 * np.ptp(lin))
+
+
+def test_interpolation_onto_new_points(case):
+    tm = case.tool
+    o = Oracle(case)
+    o.t["T"][:] = _linear(o.t["node_pos"])
+    nodes = o.t["node_pos"]
+    tri = tm.elems[:, :3]
+    rng = np.random.default_rng(3)
+    w = rng.dirichlet((1, 1, 1), size=len(tri))
+    pts = np.einsum("ek,ekd->ed", w, nodes[tri])
+    got = o.interp_new(pts)
+    assert np.allclose(got, _linear(pts), rtol=0, atol=1e-10 * np.ptp(_linear(nodes)))
+    same = o.interp_new(nodes)
+    assert np.array_equal(same, o.t["T"])
+    far = o.interp_new(np.array([[nodes[:, 0].max() + 1.0, nodes[:, 1].max() + 1.0]]))
+    assert far[0] == 300.0
+
+
+def _mesh_at(tm, pos, path, case):
+    """the tool mesh with nodes at pos written as a Gmsh export and read back"""
+    moved = dataclasses.replace(tm, nodes=np.ascontiguousarray(pos))
+    cases.write_inp(moved, path, gmsh=True)
+    m = S.mesh_read(path, True)
+    return m
+
+
+def test_oracle_remesh_identity_and_finer_mesh(tmp_path, case):
+    tm = case.tool
+    o = Oracle(case)
+    o.step(4)
+    o.t["T"][:] = _linear(o.t["node_pos"])
+    sc = o.tool_scalars
+    m = _mesh_at(tm, o.t["node_pos"], str(tmp_path / "same.inp"), case)
+    new = m.tool_mesh(case.const.tool, (sc["tl_x"], sc["tl_y"]), (sc["vel_x"], sc["vel_y"]))
+    T_before = o.t["T"].copy()
+    o.remesh(new)
+    assert np.array_equal(o.t["T"], T_before)
+    assert np.array_equal(o.t["node_pos"], new.nodes)
+    for k in ("vel_x", "tl_x", "tl_y", "br_x", "br_y", "flank_retreat"):
+        assert o.tool_scalars[k] == sc[k]
+    assert o.tool_scalars["lowest_point"] == new.lowest_point
+    # the run goes on as one whose segment heat exchange and temperature rate were reset then
+    twin = Oracle(case)
+    twin.step(4)
+    twin.t["T"][:] = T_before
+    twin.t["heat_exchange"][:] = 0.0
+    twin.t["fric_heat"][:] = 0.0
+    twin.t["k"][:] = 0.0
+    o2 = o.clone()
+    o2.step(6)
+    twin.step(6)
+    assert np.allclose(o2.t["T"], twin.t["T"], rtol=1e-12, atol=0)
+    assert np.allclose(o2.P["T"], twin.P["T"], rtol=1e-12, atol=0)
+    # a finer mesh of the same tool moved to where the tool is now
+    L = 2.5 * 1.6e-3
+    fine = cases.mesh_tool(L, L, 0.0, 7.0, 4e-4, 1200, case.const.tool)
+    off = np.array([sc["tl_x"], sc["tl_y"]]) - fine.tl
+    fpos = fine.nodes + off
+    m2 = _mesh_at(fine, fpos, str(tmp_path / "fine.inp"), case)
+    new2 = m2.tool_mesh(case.const.tool, (sc["tl_x"], sc["tl_y"]), (sc["vel_x"], sc["vel_y"]))
+    assert len(new2.nodes) > len(new.nodes)
+    o.remesh(new2)
+    lin = _linear(new2.nodes)
+    assert np.allclose(o.t["T"], lin, rtol=0, atol=1e-9 * np.ptp(lin))
+    o.step(2)
+    assert np.all(np.isfinite(o.P["pos"])) and np.all(np.isfinite(o.t["T"]))
+
+
+def test_remesh_geo_restatement_picks_chain_points(case):
+    tm = case.tool
+    text = R.remesh_geo(tm.seg_left, tm.seg_right, tm.tl)
+    pts = [tuple(map(float, m.groups())) for m in
+           re.finditer(r"Point\((\d+)\) = \{([^,]+), ([^,]+), 0, ([^}]+)\};", text)]
+    ids = [int(p[0]) for p in pts]
+    assert ids == list(range(1, len(pts) + 1))
+    xy = np.array([[p[1], p[2]] for p in pts])
+    assert np.array_equal(xy[0], tm.tl)
+    ends = np.concatenate([tm.seg_left, tm.seg_right[-1:]])
+    # every chain point is a segment end, in chain order, first and last ends included
+    k = [int(np.nonzero((ends == q).all(axis=1))[0][0]) for q in xy[1:]]
+    assert k[0] == 0 and k[-1] == len(ends) - 1 and all(b > a for a, b in zip(k, k[1:]))
+    assert all(b - a <= 2 for a, b in zip(k, k[1:]))
+    n = len(pts)
+    assert f"Line({n}) = {{{n}, 1}};" in text
+    assert 'Physical Curve("FixedBoundary", 2) = {1, %d};' % n in text
+    cut = re.search(r'Physical Curve\("CuttingEdge", 67\) = \{([^}]*)\}', text).group(1)
+    assert [int(v) for v in cut.split(",")] == list(range(2, n))
