@@ -98,7 +98,7 @@ def test_remesh_identity_keeps_the_run(tmp_path):
     # without wear the nodes only translate, so the lowest and right points the remesh takes from
     # the new segments are the ones the kinematics carried (with wear they follow the worn edge).
     # The new segments start with zero heat exchange and the new system with a zero temperature
-    # rate (tool_FEM.cu:1453-1500, FEM_thermal_gpu.cu:781), which the twin is given at the same step.
+    # rate (tool_FEM.cu:1496, FEM_thermal_gpu.cu:782), which the twin is given at the same step.
     case = cases.small_cutting_case(nx=48, ny=16, wear=False)
     sim, twin = S.Simulation(case), S.Simulation(case)
     sim.step(6)
