@@ -27,14 +27,15 @@ BLOCKS = (("phys", _lib.PhysConstants), ("corr", _lib.CorrConstants), ("joco", _
 
 
 def write_case_dir(case: cases.Case, path: str, rr2_min: float | None = None, wear: bool | None = None,
-                   tool_inp: str | None = None) -> str:
+                   tool_inp: str | None = None, remesh_low_level: bool = False) -> str:
     os.makedirs(path, exist_ok=True)
     init = cases.initial_particles(case)
     sc = {"N": case.N, "dt": case.dt, "h": case.h,
           "bbmin_x": case.grid_bbmin[0], "bbmin_y": case.grid_bbmin[1],
           "bbmax_x": case.grid_bbmax[0], "bbmax_y": case.grid_bbmax[1],
           "rr2_min": cases.default_rr2_min(case.dx) if rr2_min is None else rr2_min,
-          "wear": int(case.wear if wear is None else wear)}
+          "wear": int(case.wear if wear is None else wear),
+          "remesh_low_level": int(bool(remesh_low_level))}
     for name, dt in PARTICLE_ARRAYS:
         np.ascontiguousarray(init[name], dtype=dt).tofile(os.path.join(path, f"p_{name}.bin"))
     for block, struct in BLOCKS:

@@ -191,6 +191,12 @@ int main(int argc, char **argv) {
         leap_frog stepper(particles.N, (unsigned)grid.num_cell(), tool ? tool->node_number : 0);
         if (prec == "fp32c") stepper.set_precision(SPHFEM_FP32C);
 
+        // mfree_gpu.cpp:259-260, 351-357: with wear, the tool is remeshed every freq / 20 steps
+        // (remesh_low_level; the Gmsh remesh needs the meshing program between its two halves)
+        const bool remesh = s.count("remesh_low_level") && get(s, "remesh_low_level") != 0.;
+        const long freq_remesh = freq / 20;
+        if (remesh && freq_remesh <= 0) throw std::runtime_error("remeshing needs FREQ >= 20 (freq / 20 steps)");
+
         // mfree_gpu.cpp:289-366 without the dumps
         std::FILE *ff = std::fopen((out + "/forces.txt").c_str(), "w");
         if (!ff) throw std::runtime_error("cannot write " + out + "/forces.txt");
@@ -199,6 +205,7 @@ int main(int argc, char **argv) {
                 float2_t F = report_force(&particles);
                 std::fprintf(ff, "%ld %.17g %.17g\n", step, F.x, F.y);
             }
+            if (remesh && tool && wear_pro && step % freq_remesh == 0) remesh_low_level(tool.get(), (int)freq);
             stepper.step(&particles, &grid, tool.get());
         }
         float2_t F = report_force(&particles);

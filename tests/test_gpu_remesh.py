@@ -197,3 +197,37 @@ def test_remesh_geo_file(tmp_path):
     k = sim.tool.kinematics()
     assert open(path).read() == R.remesh_geo(g["left"], g["right"], (k["tl_x"], k["tl_y"]))
     sim.close()
+
+
+def test_cpp_driver_remesh_low_level(tmp_path):
+    """the driver loop of mfree_gpu.cpp:351-357 (remesh_low_level every freq / 20 steps, with wear)
+    in the C++ driver and in the Python host, on the same case"""
+    import os
+    import subprocess
+    from sph_fem_2d_cutting_gpu_amd import case_io
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sph_fem_2d_cutting_gpu_amd",
+                       "driver", "mfree_driver")
+    if not os.path.exists(exe):
+        pytest.fail("driver not built (run __graft_entry__.build())")
+    case = _case()
+    steps, freq = 30, 40
+    cdir = case_io.write_case_dir(case, str(tmp_path / "case"), remesh_low_level=True)
+    out = tmp_path / "out"
+    out.mkdir()
+    print(f"driver with remesh_low_level every {freq // 20} steps", flush=True)
+    r = subprocess.run([exe, cdir, str(steps), str(freq), "fp64", str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = case_io.read_driver_output(str(out), case.N)
+    sim = S.Simulation(case)
+    for step in range(steps):
+        if step % (freq // 20) == 0:
+            sim.tool.remesh_low_level(freq)
+        sim.step(1)
+    sim.ctx.check_error()
+    ref = sim.state(("pos", "vel", "S", "T", "idx"))
+    assert np.array_equal(got["idx"], ref["idx"])
+    for f in ("pos", "vel", "S", "T"):
+        scale = max(float(np.max(np.abs(ref[f]))), 1e-300)
+        assert float(np.max(np.abs(got[f] - ref[f]))) / scale <= 1e-11, f
+    sim.close()
