@@ -445,6 +445,26 @@ int sphfem_tool_remesh(sphfem_ctx *ctx, sphfem_tool *t, sphfem_mesh *mesh);
  * skipped after a short edge), the closing lines, physical groups CuttingEdge (67) and
  * FixedBoundary (2) -- written as a .geo script for `gmsh -2 -format inp` */
 int sphfem_tool_write_remesh_geo(sphfem_ctx *ctx, sphfem_tool *t, const char *path);
+/* the tool's device arrays under the reference's struct views (mesh_structure_GPU.h:80-142 mesh_GPU_m,
+ * FEM_thermal_gpu.h:35-93 FEM_thermal_2D_GPU, segment_FEM_gpu.h:31-56): device pointers owned by
+ * the tool, valid until it is remeshed onto a new mesh or destroyed.  The mapping arrays and the
+ * node flags are NULL for a tool created without the remeshing description; fixed_ele_num is 0
+ * then. */
+typedef struct {
+    int node_num, ele_num, nz_num, seg_num, color_total, color_cutting_edge, fixed_ele_num;
+    sphfem_double2 *position, *position_mapping;
+    sphfem_int4 *node_num_el;
+    int *csr_row_ptr, *csr_col_ind, *node_flags;
+    double *K, *K_temp, *M, *M_inv, *f, *f_temp, *k, *T, *T_old, *T_mapping;
+    sphfem_double2 *left, *right, *n, *heat_exchange;
+    sphfem_int4 *marks;
+    double *fric_heat;
+    sphfem_double2 *wear_rate;
+    double *wear_nodes;
+    sphfem_double4 *physical_para;
+    double *sliding_force;
+} sphfem_tool_arrays;
+int sphfem_tool_get_arrays(const sphfem_tool *t, sphfem_tool_arrays *out);
 /* node, element, non-zero and segment counts of the tool's current mesh (tool_FEM.h:39-41 node_number,
  * element_number, cutting_segment_size); any pointer may be NULL */
 int sphfem_tool_get_sizes(const sphfem_tool *t, int *node_num, int *ele_num, int *nz_num, int *seg_num);

@@ -98,6 +98,8 @@ inline void upload(T *dst, const T *src, unsigned int n) {
 
 // benchmarks.cpp:12 and mfree_gpu.cpp:47: read by leap_frog::step on every call
 inline double global_dt = 0.;
+inline double global_t_final = 0.;       // benchmarks.cpp:13
+inline int global_step = 0;              // mfree_gpu.cpp:39
 inline bool wear_pro = false;
 
 // ---------------------------------------------------------------------------------------------
@@ -263,8 +265,46 @@ inline void compute_contact_forces(particle_gpu *particles) {
                        "compute_contact_forces");
 }
 
-// the segments live inside the tool object; the type exists for signature compatibility
-struct segment_FEM_gpu;
+// Views of a tool_FEM's device arrays under the reference's names (segment_FEM_gpu.h:31-56,
+// FEM_thermal_gpu.h:35-93, mesh_structure_GPU.h:80-142): plain device pointers owned by the tool,
+// refreshed when the tool is remeshed.  Kernels and writers that read these fields in the
+// reference read the same data here.
+struct segment_FEM_gpu {
+    int segment_num = 0;
+    float2_t *left = nullptr, *right = nullptr, *n = nullptr;
+    sphfem_int4 *marks = nullptr;          // element, colour, left node, right node
+    float2_t *heat_exchange = nullptr;     // x: q_T * length, y: length
+    double *fric_heat = nullptr;
+    float2_t *wear_rate = nullptr;
+    double *wear_nodes = nullptr;
+    float4_t *physical_para = nullptr;     // contact count, force, sliding velocity, temperature
+    double *sliding_force = nullptr;
+};
+
+struct FEM_thermal_2D_GPU {
+    int node_num = 0, ele_num = 0, nz_num = 0, color_total = 0, color_cutting_edge = 0, fixed_ele_num = 0;
+    double T_env = 300.;
+    int *m_row_ptr = nullptr, *m_col_ind = nullptr;
+    double *m_stiffness_matrix = nullptr, *m_stiffness_matrix_temp = nullptr;
+    double *m_mass_matrix = nullptr, *m_mass_matrix_inverse = nullptr;
+    double *m_f = nullptr, *m_f_temp = nullptr, *m_k = nullptr, *m_T = nullptr, *m_T_old = nullptr;
+    double *m_T_mapping = nullptr;
+};
+
+struct mesh_GPU_m {
+    sphfem_int4 *node_num = nullptr;       // 3 node ids + colour per element
+    float2_t *position = nullptr;
+    float2_t *position_mapping = nullptr;
+    int *csr_row_ptr = nullptr, *csr_col_ind = nullptr;
+    int *node_flags = nullptr;             // bit 0 cutting edge, bit 1 fixed boundary
+    int get_node_number() const { return no_num; }
+    int get_element_number() const { return ele_num_total; }
+    int get_non_zero_number() const { return non_zeros; }
+    int get_color_number() const { return color_total; }
+    int get_cutting_edge_color_number() const { return cutting_edge_color; }
+    int get_fixed_edge_element_number() const { return ele_num_fixed; }
+    int no_num = 0, ele_num_total = 0, non_zeros = 0, color_total = 0, cutting_edge_color = 0, ele_num_fixed = 0;
+};
 
 // mesh_read (mesh_structure_GPU.h:80-142): a tool mesh read from an .inp file on the host --
 // mesh_read(file) for the Abaqus-style file, mesh_read(file, true) for a Gmsh export -- with its
@@ -324,6 +364,7 @@ class tool_FEM : public tool {
         m_velocity = vel;
         right_point = desc.right_point;
         lowest_point = desc.lowest_point;
+        refresh_views();
     }
     // the device data of set_up_segments_from_mesh + set_up_FEM_tool_gpu, from a host mesh
     explicit tool_FEM(const sphfem_tool_desc &desc) {
@@ -334,6 +375,7 @@ class tool_FEM : public tool {
         m_velocity = vec2_t(desc.vel.x, desc.vel.y);
         right_point = desc.right_point;
         lowest_point = desc.lowest_point;
+        refresh_views();
     }
     ~tool_FEM() override {
         if (m_handle && sphfem_shim::ctx_slot()) (void)sphfem_tool_destroy(sphfem_shim::ctx_slot(), m_handle);
@@ -345,7 +387,9 @@ class tool_FEM : public tool {
     int element_number = 0;
     int cutting_segment_size = 0;
     double right_point = 0., lowest_point = 0.;     // at set-up (tool_FEM.h:54-55)
-    segment_FEM_gpu *segments_FEM = nullptr;
+    segment_FEM_gpu *segments_FEM = &m_seg;         // tool_FEM.h:34-40 members
+    FEM_thermal_2D_GPU *FEM_solver = &m_fem;
+    mesh_GPU_m *mesh_GPU = &m_mesh;
 
     void set_vel(vec2_t vel) override {
         m_velocity = vel;
@@ -389,6 +433,7 @@ class tool_FEM : public tool {
         const sphfem_tool_kinematics k = kinematics();
         right_point = k.right_point;
         lowest_point = k.lowest_point;
+        refresh_views();
     }
     // remesh_read_segments (tool_FEM.cu:1708-1810): the Gmsh geometry of the current cutting edge
     void remesh_read_segments(const std::string &geo_file = "tool_remesh.geo") {
@@ -397,7 +442,31 @@ class tool_FEM : public tool {
     }
 
   private:
+    void refresh_views() {
+        sphfem_tool_arrays a{};
+        sphfem_shim::check(sphfem_tool_get_arrays(m_handle, &a), "tool arrays");
+        m_seg.segment_num = a.seg_num;
+        m_seg.left = a.left; m_seg.right = a.right; m_seg.n = a.n; m_seg.marks = a.marks;
+        m_seg.heat_exchange = a.heat_exchange; m_seg.fric_heat = a.fric_heat; m_seg.wear_rate = a.wear_rate;
+        m_seg.wear_nodes = a.wear_nodes; m_seg.physical_para = a.physical_para; m_seg.sliding_force = a.sliding_force;
+        m_fem.node_num = a.node_num; m_fem.ele_num = a.ele_num; m_fem.nz_num = a.nz_num;
+        m_fem.color_total = a.color_total; m_fem.color_cutting_edge = a.color_cutting_edge;
+        m_fem.fixed_ele_num = a.fixed_ele_num;
+        m_fem.m_row_ptr = a.csr_row_ptr; m_fem.m_col_ind = a.csr_col_ind;
+        m_fem.m_stiffness_matrix = a.K; m_fem.m_stiffness_matrix_temp = a.K_temp;
+        m_fem.m_mass_matrix = a.M; m_fem.m_mass_matrix_inverse = a.M_inv;
+        m_fem.m_f = a.f; m_fem.m_f_temp = a.f_temp; m_fem.m_k = a.k; m_fem.m_T = a.T; m_fem.m_T_old = a.T_old;
+        m_fem.m_T_mapping = a.T_mapping;
+        m_mesh.node_num = a.node_num_el; m_mesh.position = a.position; m_mesh.position_mapping = a.position_mapping;
+        m_mesh.csr_row_ptr = a.csr_row_ptr; m_mesh.csr_col_ind = a.csr_col_ind; m_mesh.node_flags = a.node_flags;
+        m_mesh.no_num = a.node_num; m_mesh.ele_num_total = a.ele_num; m_mesh.non_zeros = a.nz_num;
+        m_mesh.color_total = a.color_total; m_mesh.cutting_edge_color = a.color_cutting_edge;
+        m_mesh.ele_num_fixed = a.fixed_ele_num;
+    }
     sphfem_tool *m_handle = nullptr;
+    segment_FEM_gpu m_seg;
+    FEM_thermal_2D_GPU m_fem;
+    mesh_GPU_m m_mesh;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -749,6 +818,10 @@ class tool_wear {
   private:
     sphfem_tool_wear *m_handle = nullptr;
 };
+
+// mfree_gpu.cpp:41-42: the driver's FEM tool and wear objects
+inline tool_FEM *global_tool_FEM = nullptr;
+inline tool_wear *global_wear = nullptr;
 
 // types.cpp:11-37
 inline bool check_cuda_error() { return sphfem_check_error(sphfem_shim::ctx()) == SPHFEM_OK; }

@@ -68,6 +68,15 @@ class ToolDesc(C.Structure):
 
 TOOL_MAX_SEG = 5
 
+
+class ToolArrays(C.Structure):
+    _fields_ = ([(f, C.c_int) for f in ("node_num", "ele_num", "nz_num", "seg_num", "color_total",
+                                        "color_cutting_edge", "fixed_ele_num")]
+                + [(f, _vp) for f in ("position", "position_mapping", "node_num_el", "csr_row_ptr", "csr_col_ind",
+                                      "node_flags", "K", "K_temp", "M", "M_inv", "f", "f_temp", "k", "T", "T_old",
+                                      "T_mapping", "left", "right", "n", "heat_exchange", "marks", "fric_heat",
+                                      "wear_rate", "wear_nodes", "physical_para", "sliding_force")])
+
 MeshInfo = _s("MeshInfo", ["node_num", "ele_num", "nz_num", "cut_ele_num", "fixed_ele_num", "color_total",
                            "color_cutting_edge"], C.c_int)
 
@@ -199,6 +208,7 @@ _SIGS = {
     "sphfem_tool_remesh": (_ctx, _vp, _vp),
     "sphfem_tool_write_remesh_geo": (_ctx, _vp, C.c_char_p),
     "sphfem_tool_get_sizes": (_vp, _P(C.c_int), _P(C.c_int), _P(C.c_int), _P(C.c_int)),
+    "sphfem_tool_get_arrays": (_vp, _P(ToolArrays)),
     "sphfem_timing_enable": (_ctx, C.c_int),
     "sphfem_timing_reset": (_ctx,),
     "sphfem_timing_select": (_ctx, C.c_char_p),

@@ -1089,6 +1089,7 @@ int sphfem_tool_create(sphfem_ctx *ctx, const sphfem_tool_desc *d, sphfem_tool *
             t->dir_entry = dupload(s, de.data(), de.size());
             t->pos_map = dalloc<sphfem_double2>(nn);
             t->T_map = dalloc<double>(nn);
+            t->n_fixed = d->fixed_num;
             t->remeshable = true;
         }
         SPHFEM_HIP_CHECK(hipStreamSynchronize(s));
@@ -1158,6 +1159,21 @@ int sphfem_tool_remesh(sphfem_ctx *ctx, sphfem_tool *t, sphfem_mesh *mesh) {
         t->generation = gen;
         return sphfem_tool_destroy(ctx, nt);      // the old arrays
     });
+}
+
+int sphfem_tool_get_arrays(const sphfem_tool *t, sphfem_tool_arrays *o) {
+    if (!t || !o) return SPHFEM_ERR_ARG;
+    std::memset(o, 0, sizeof(*o));
+    o->node_num = t->n_nodes; o->ele_num = t->n_elem; o->nz_num = t->nnz; o->seg_num = t->n_seg;
+    o->color_total = t->n_colors; o->color_cutting_edge = t->n_cut_colors; o->fixed_ele_num = t->n_fixed;
+    o->position = t->pos; o->position_mapping = t->pos_map; o->node_num_el = t->elem;
+    o->csr_row_ptr = t->row_ptr; o->csr_col_ind = t->col_ind; o->node_flags = t->node_flags;
+    o->K = t->K; o->K_temp = t->K_temp; o->M = t->M; o->M_inv = t->Minv; o->f = t->f; o->f_temp = t->f_temp;
+    o->k = t->k; o->T = t->T; o->T_old = t->T_old; o->T_mapping = t->T_map;
+    o->left = t->left; o->right = t->right; o->n = t->n; o->heat_exchange = t->heat_exchange;
+    o->marks = t->marks; o->fric_heat = t->fric_heat; o->wear_rate = t->wear_rate; o->wear_nodes = t->wear_nodes;
+    o->physical_para = t->physical_para; o->sliding_force = t->sliding_force;
+    return SPHFEM_OK;
 }
 
 int sphfem_tool_get_sizes(const sphfem_tool *t, int *node_num, int *ele_num, int *nz_num, int *seg_num) {

@@ -143,6 +143,7 @@ struct sphfem_tool {
     sphfem_tool_kinematics host_kin{};
     // remeshing (present when the description carried node flags and the fixed boundary)
     bool remeshable = false;
+    int n_fixed = 0;                   // fixed-boundary elements
     int *node_flags = nullptr;         // [n_nodes] bit 0 cutting edge, bit 1 fixed boundary
     int *dir_node = nullptr;           // [n_nodes] 1 if a Dirichlet row
     int *dir_ptr = nullptr;            // [n_nodes + 1] CSR: entries (F, x) of the Dirichlet rows F

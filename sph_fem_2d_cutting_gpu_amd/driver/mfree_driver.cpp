@@ -213,6 +213,14 @@ int main(int argc, char **argv) {
         std::fclose(ff);
         if (!check_cuda_error()) throw std::runtime_error(sphfem_last_error(sphfem_shim::ctx()));
 
+        if (tool) {
+            // the tool's temperatures through the reference's view (FEM_solver->m_T, FEM_thermal_gpu.h:58)
+            const auto T = download(tool->FEM_solver->m_T, (size_t)tool->FEM_solver->node_num);
+            double Tmax = T.empty() ? 0. : T[0];
+            for (double v : T) Tmax = v > Tmax ? v : Tmax;
+            std::printf("tool nodes %d, segments %d, max T %.17g\n", tool->mesh_GPU->get_node_number(),
+                        tool->segments_FEM->segment_num, Tmax);
+        }
         write_array(out + "/pos.bin", download(particles.pos, N));
         write_array(out + "/vel.bin", download(particles.vel, N));
         write_array(out + "/S.bin", download(particles.S, N));

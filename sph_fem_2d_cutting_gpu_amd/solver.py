@@ -316,6 +316,13 @@ class tool_FEM:
         self.ctx.call("sphfem_tool_remesh", self.h, mesh.h)
         self.n_nodes, _, self.nnz, self.n_seg = self.sizes()
 
+    def arrays(self) -> dict:
+        """device pointers and sizes of the tool's arrays (the reference's mesh_GPU_m,
+        FEM_thermal_2D_GPU and segment_FEM_gpu fields); valid until a remesh onto a new mesh"""
+        a = _lib.ToolArrays()
+        check(lib().sphfem_tool_get_arrays(self.h, C.byref(a)), None, "tool_get_arrays")
+        return {f: getattr(a, f) for f, _ in _lib.ToolArrays._fields_}
+
     def sizes(self):
         """(node_number, element_number, non-zeros, cutting_segment_size) of the current mesh"""
         v = [C.c_int() for _ in range(4)]

@@ -95,4 +95,10 @@ def test_cpp_driver_reads_inp(tmp_path):
     F = np.array(forces)
     fscale = max(float(np.max(np.abs(F[:, 1:]))), 1e-300)
     assert float(np.max(np.abs(got["forces"][:, 1:] - F[:, 1:]))) / fscale <= 1e-11
+    # the driver read the tool temperatures through the FEM_solver / mesh_GPU / segments_FEM views
+    line = next(l for l in r.stdout.splitlines() if l.startswith("tool nodes"))
+    _, _, nodes, _, segs, _, _, tmax = line.replace(",", "").split()
+    tg = sim.tool_state()
+    assert int(nodes) == sim.tool.n_nodes and int(segs) == sim.tool.n_seg
+    assert float(tmax) == pytest.approx(float(tg["T"].max()), rel=1e-12)
     sim.close()

@@ -231,3 +231,29 @@ def test_cpp_driver_remesh_low_level(tmp_path):
         scale = max(float(np.max(np.abs(ref[f]))), 1e-300)
         assert float(np.max(np.abs(got[f] - ref[f]))) / scale <= 1e-11, f
     sim.close()
+
+
+def test_tool_array_views_follow_a_remesh(tmp_path):
+    """sphfem_tool_get_arrays (the mesh_GPU_m / FEM_thermal_2D_GPU / segment_FEM_gpu views): the
+    pointers read the tool's data, before and after a remesh onto a finer mesh"""
+    case = _case()
+    sim = S.Simulation(case)
+    sim.step(2)
+    for stage in ("created", "remeshed"):
+        a = sim.tool.arrays()
+        g = sim.tool_state()
+        nn, ns, nz = a["node_num"], a["seg_num"], a["nz_num"]
+        assert (nn, ns, nz) == (sim.tool.n_nodes, sim.tool.n_seg, sim.tool.nnz)
+        assert a["fixed_ele_num"] > 0 and a["node_flags"] and a["T_mapping"]
+        assert np.array_equal(sim.ctx.dtoh(a["T"], (nn,), np.float64), g["T"])
+        assert np.array_equal(sim.ctx.dtoh(a["K"], (nz,), np.float64), g["K"])
+        assert np.array_equal(sim.ctx.dtoh(a["position"], (nn, 2), np.float64), g["node_pos"])
+        assert np.array_equal(sim.ctx.dtoh(a["left"], (ns, 2), np.float64), g["left"])
+        if stage == "created":
+            L = 2.5 * 1.6e-3
+            k = sim.tool.kinematics()
+            fine = cases.mesh_tool(L, L, 0.0, 7.0, 4e-4, 1200, case.const.tool)
+            m = _mesh_from(fine, fine.nodes + (np.array([k["tl_x"], k["tl_y"]]) - fine.tl), str(tmp_path / "f.inp"))
+            sim.tool.remesh(m)
+            m.close()
+    sim.close()
