@@ -1142,18 +1142,25 @@ int sphfem_tool_remesh(sphfem_ctx *ctx, sphfem_tool *t, sphfem_mesh *mesh) {
             return fail(ctx, SPHFEM_ERR_ARG, "the new mesh has no single cutting-edge chain");
         sphfem_tool *nt = nullptr;
         if (int rc = sphfem_tool_create(ctx, &d, &nt)) return rc;
-        // mesh_GPU_m(..., false) starts the new nodes at 300 K (mesh_structure_GPU.cu:43-44)
-        const std::vector<double> t300((size_t)nt->n_nodes, 300.);
-        SPHFEM_HIP_CHECK(hipMemcpyAsync(nt->T, t300.data(), 8ul * nt->n_nodes, hipMemcpyHostToDevice, ctx->stream));
-        launch_interp_new(ctx, t, nt->pos, nt->n_nodes, nt->T);
-        SPHFEM_HIP_CHECK(hipMemcpyAsync(nt->T_old, nt->T, 8ul * nt->n_nodes, hipMemcpyDeviceToDevice, ctx->stream));
-        sphfem_tool_kinematics nk = k;
-        nk.lowest_point = d.lowest_point;
-        nk.right_point = d.right_point;
-        nk.ref_dist = d.ref_dist;
-        SPHFEM_HIP_CHECK(hipMemcpyAsync(nt->kin, &nk, sizeof(nk), hipMemcpyHostToDevice, ctx->stream));
-        nt->host_kin = nk;
-        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        try {
+            // mesh_GPU_m(..., false) starts the new nodes at 300 K (mesh_structure_GPU.cu:43-44)
+            const std::vector<double> t300((size_t)nt->n_nodes, 300.);
+            SPHFEM_HIP_CHECK(hipMemcpyAsync(nt->T, t300.data(), 8ul * nt->n_nodes, hipMemcpyHostToDevice,
+                                            ctx->stream));
+            launch_interp_new(ctx, t, nt->pos, nt->n_nodes, nt->T);
+            SPHFEM_HIP_CHECK(hipMemcpyAsync(nt->T_old, nt->T, 8ul * nt->n_nodes, hipMemcpyDeviceToDevice,
+                                            ctx->stream));
+            sphfem_tool_kinematics nk = k;
+            nk.lowest_point = d.lowest_point;
+            nk.right_point = d.right_point;
+            nk.ref_dist = d.ref_dist;
+            SPHFEM_HIP_CHECK(hipMemcpyAsync(nt->kin, &nk, sizeof(nk), hipMemcpyHostToDevice, ctx->stream));
+            nt->host_kin = nk;
+            SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            (void)sphfem_tool_destroy(ctx, nt);     // the tool keeps its old mesh
+            throw;
+        }
         const unsigned long gen = t->generation + 1;
         std::swap(*t, *nt);
         t->generation = gen;
