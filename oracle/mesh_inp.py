@@ -13,7 +13,12 @@ Only tests/ import this module, as the checker of the library's host-side mesh p
 * mesh_vtk: write_mesh_file (mesh_structure_GPU.cu:1395-1462);
 * remesh_geo: remesh_read_segments (tool_FEM.cu:1708-1810) as a .geo script.
 
-Pure-Python loops, for meshes of a few thousand elements.
+Pure-Python loops, for meshes of a few thousand elements.  Pins (tests/test_mesh_inp.py,
+tests/test_remesh.py): round trips of generated meshes in both dialects, the closed-form second
+renumbering of Gmsh boundary edges, a hand-worked correction example, colouring / layer / CSR
+invariants by brute force, P1 patch tests and lumped-mass areas, the chain and normals of the
+generated mesh, and the chain-point properties of remesh_geo.  What Gmsh makes of the remesh_geo
+script is parity unpinned (Gmsh is not available).
 """
 from __future__ import annotations
 
