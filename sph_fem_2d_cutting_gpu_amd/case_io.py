@@ -51,6 +51,13 @@ def write_case_dir(case: cases.Case, path: str, rr2_min: float | None = None, we
             sc[f"tool.{k}"] = float(v)
         for name, dt in TOOL_ARRAYS:
             np.ascontiguousarray(t[name], dtype=dt).tofile(os.path.join(path, f"t_{name}.bin"))
+        # remeshing description: node flags, fixed-boundary elements and their edge flags (int4)
+        np.ascontiguousarray(t["node_flags"], dtype=np.int32).tofile(os.path.join(path, "t_node_flags.bin"))
+        np.ascontiguousarray(t["fixed_elems"], dtype=np.int32).tofile(os.path.join(path, "t_fixed_elems.bin"))
+        ff = np.zeros((len(t["fixed_elems"]), 4), np.int32)
+        ff[:, :3] = t["fixed_flags"]
+        ff.tofile(os.path.join(path, "t_fixed_flags.bin"))
+        sc["fixed_num"] = len(t["fixed_elems"])
         if tool_inp is not None:
             # the driver then reads the mesh from tool.inp and sets the tool up from it
             if tool_inp not in ("gmsh", "abaqus"):

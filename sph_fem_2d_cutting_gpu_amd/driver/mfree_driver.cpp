@@ -135,7 +135,8 @@ int main(int argc, char **argv) {
         const int nn = (int)get(s, "node_num");
         std::vector<sphfem_double2> t_pos, t_left, t_right, t_n;
         std::vector<sphfem_int4> t_elem, t_marks;
-        std::vector<int> t_layer, t_rp, t_ci;
+        std::vector<int> t_layer, t_rp, t_ci, t_nflags, t_fixed;
+        std::vector<sphfem_int4> t_fflags;
         std::vector<double> t_K, t_M, t_f, t_T;
         const std::string inp = dir + "/tool.inp";
         if (nn > 0 && std::ifstream(inp).good()) {
@@ -184,6 +185,15 @@ int main(int argc, char **argv) {
             d.ref_dist = get(s, "tool.ref_dist");
             d.lowest_point = get(s, "tool.lowest_point");
             d.right_point = get(s, "tool.right_point");
+            if (s.count("fixed_num")) {      // the remeshing description
+                d.fixed_num = (int)get(s, "fixed_num");
+                t_nflags = read_array<int>(dir + "/t_node_flags.bin", nn);
+                t_fixed = read_array<int>(dir + "/t_fixed_elems.bin", d.fixed_num);
+                t_fflags = read_array<sphfem_int4>(dir + "/t_fixed_flags.bin", d.fixed_num);
+                d.node_flags = t_nflags.data();
+                d.fixed_elems = t_fixed.data();
+                d.fixed_flags = t_fflags.data();
+            }
             tool.reset(new tool_FEM(d));
         }
 
