@@ -20,7 +20,7 @@
 #include <string>
 #include <vector>
 
-#include "sphfem_internal.h"
+#include "../../include/sphfem.h"
 
 #pragma clang fp contract(off)
 
@@ -336,6 +336,7 @@ void read_gmsh(sphfem_mesh &m, const std::string &text) {
             if (line.find("FixedBoundary") != std::string::npos) break;
             if (!elem_start) continue;
             const auto tok = split(line, ',');
+            if (tok.size() < 4) throw ParseError{"element line: " + line};
             int a[3];
             for (int k = 0; k < 3; k++) {
                 if (!to_int(tok[(size_t)k + 1], a[k])) throw ParseError{"element line: " + line};
