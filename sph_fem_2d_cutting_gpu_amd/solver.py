@@ -262,7 +262,8 @@ def _p(a, dt):
 class tool_FEM:
     """Rigid FEM tool with its thermal solver and cutting-edge segments (tool_FEM.h:34-83)."""
 
-    def __init__(self, ctx: Context, case: cases.Case):
+    def __init__(self, ctx: Context, case: cases.Case, remeshable: bool = True):
+        """remeshable: pass the node flags and fixed boundary the remesh entry points need"""
         self.ctx = ctx
         tm = case.tool
         init = cases.initial_tool(case)
@@ -294,8 +295,8 @@ class tool_FEM:
         d.tl_point_ref = _lib.Double2(sc["tl_x"], sc["tl_y"])
         d.br_point = _lib.Double2(sc["br_x"], sc["br_y"])
         d.ref_dist, d.lowest_point, d.right_point = sc["ref_dist"], sc["lowest_point"], sc["right_point"]
-        d.node_flags = arr(init["node_flags"], np.int32)
-        d.fixed_num = len(init["fixed_elems"])
+        d.node_flags = arr(init["node_flags"], np.int32) if remeshable else None
+        d.fixed_num = len(init["fixed_elems"]) if remeshable else 0
         d.fixed_elems = arr(init["fixed_elems"], np.int32)
         ff = np.zeros((d.fixed_num, 4), np.int32)
         ff[:, :3] = init["fixed_flags"]

@@ -257,3 +257,23 @@ def test_tool_array_views_follow_a_remesh(tmp_path):
             sim.tool.remesh(m)
             m.close()
     sim.close()
+
+
+def test_remesh_needs_the_remeshing_description(tmp_path):
+    """a tool created without node flags and fixed boundary refuses both remesh entry points with
+    SPHFEM_ERR_STATE and keeps working"""
+    case = _case()
+    ctx = S.Context()
+    ctx.setup_constants(case.const, case.dt, cases.default_rr2_min(case.dx), case.wear)
+    t = S.tool_FEM(ctx, case, remeshable=False)
+    assert t.arrays()["fixed_ele_num"] == 0 and not t.arrays()["node_flags"]
+    with pytest.raises(_lib.SphfemError, match="remeshing needs"):
+        t.remesh_low_level(20)
+    m = _mesh_from(case.tool, case.tool.nodes, str(tmp_path / "same.inp"))
+    with pytest.raises(_lib.SphfemError, match="remeshing needs"):
+        t.remesh(m)
+    g = t.download()
+    assert np.array_equal(g["node_pos"], case.tool.nodes)
+    m.close()
+    ctx.call("sphfem_tool_destroy", t.h)
+    ctx.close()
