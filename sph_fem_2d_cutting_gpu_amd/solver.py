@@ -298,7 +298,7 @@ class tool_FEM:
         d.node_flags = arr(init["node_flags"], np.int32) if remeshable else None
         d.fixed_num = len(init["fixed_elems"]) if remeshable else 0
         d.fixed_elems = arr(init["fixed_elems"], np.int32)
-        ff = np.zeros((d.fixed_num, 4), np.int32)
+        ff = np.zeros((len(init["fixed_elems"]), 4), np.int32)
         ff[:, :3] = init["fixed_flags"]
         d.fixed_flags = arr(ff, np.int32)
         self.h = C.c_void_p()
