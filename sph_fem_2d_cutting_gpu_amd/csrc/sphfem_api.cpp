@@ -1,5 +1,6 @@
 // C ABI of libsphfem_hip: context, constants, memory, and the per-step orchestration.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -1089,6 +1090,8 @@ int sphfem_tool_create(sphfem_ctx *ctx, const sphfem_tool_desc *d, sphfem_tool *
             t->dir_entry = dupload(s, de.data(), de.size());
             t->pos_map = dalloc<sphfem_double2>(nn);
             t->T_map = dalloc<double>(nn);
+            t->h_elem = new sphfem_int4[(size_t)ne];
+            std::copy(d->node_num_el, d->node_num_el + ne, t->h_elem);
             t->n_fixed = d->fixed_num;
             t->remeshable = true;
         }
@@ -1110,11 +1113,100 @@ int sphfem_tool_destroy(sphfem_ctx *ctx, sphfem_tool *t) {
         dfree(t->node_shift_ptr); dfree(t->node_shift); dfree(t->layer_elems); dfree(t->layer_old);
         dfree(t->layer_new); dfree(t->nnz_layer_ptr); dfree(t->nnz_layer); dfree(t->node_layer_ptr);
         dfree(t->node_layer); dfree(t->kin); dfree(t->node_flags); dfree(t->dir_node); dfree(t->dir_ptr);
-        dfree(t->dir_entry); dfree(t->pos_map); dfree(t->T_map);
+        dfree(t->dir_entry); dfree(t->pos_map); dfree(t->T_map); dfree(t->bk_ptr); dfree(t->bk_list);
+        delete[] t->h_elem;
         delete t;
         return SPHFEM_OK;
     });
 }
+
+}  // extern "C"
+
+// bins of the tool's elements at its current node positions, in the tool's bin buffers
+static ElemBuckets build_elem_buckets(sphfem_ctx *ctx, sphfem_tool *t) {
+    const int nn = t->n_nodes, ne = t->n_elem;
+    std::vector<sphfem_double2> pos((size_t)nn);
+    SPHFEM_HIP_CHECK(hipMemcpyAsync(pos.data(), t->pos, 16ul * nn, hipMemcpyDeviceToHost, ctx->stream));
+    SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    double ext = 0.;
+    {
+        double x0 = DBL_MAX, y0 = DBL_MAX, x1 = -DBL_MAX, y1 = -DBL_MAX;
+        for (const auto &p : pos) {
+            x0 = std::min(x0, p.x); y0 = std::min(y0, p.y); x1 = std::max(x1, p.x); y1 = std::max(y1, p.y);
+        }
+        ext = std::max(x1 - x0, y1 - y0);
+    }
+    // element boxes padded by 1e-6 of their size and 1e-9 of the tool's: far beyond the 1e-8
+    // barycentric tolerance and the 1e-10 coincidence distance of the interpolations
+    std::vector<double> box((size_t)4 * ne), diag((size_t)ne);
+    double gx0 = DBL_MAX, gy0 = DBL_MAX, gx1 = -DBL_MAX, gy1 = -DBL_MAX;
+    for (int e = 0; e < ne; e++) {
+        const sphfem_int4 el = t->h_elem[e];
+        const sphfem_double2 a = pos[(size_t)el.x], b = pos[(size_t)el.y], c = pos[(size_t)el.z];
+        double lx = std::min(a.x, std::min(b.x, c.x)), hx = std::max(a.x, std::max(b.x, c.x));
+        double ly = std::min(a.y, std::min(b.y, c.y)), hy = std::max(a.y, std::max(b.y, c.y));
+        diag[(size_t)e] = std::hypot(hx - lx, hy - ly);
+        const double pad = 1e-6 * diag[(size_t)e] + 1e-9 * ext;
+        box[4 * (size_t)e] = lx - pad; box[4 * (size_t)e + 1] = ly - pad;
+        box[4 * (size_t)e + 2] = hx + pad; box[4 * (size_t)e + 3] = hy + pad;
+        gx0 = std::min(gx0, lx - pad); gy0 = std::min(gy0, ly - pad);
+        gx1 = std::max(gx1, hx + pad); gy1 = std::max(gy1, hy + pad);
+    }
+    std::vector<double> sorted(diag);
+    std::nth_element(sorted.begin(), sorted.begin() + ne / 2, sorted.end());
+    double h = std::max(2. * sorted[(size_t)ne / 2], 1e-12 * std::max(ext, 1e-300));
+    ElemBuckets bk;
+    auto cell = [&](double v, double o) { return (int)std::floor((v - o) * bk.inv); };
+    long total = 0;
+    for (int tries = 0;; tries++) {
+        bk.inv = 1. / h;
+        bk.x0 = gx0; bk.y0 = gy0;
+        bk.nx = cell(gx1, gx0) + 1;
+        bk.ny = cell(gy1, gy0) + 1;
+        total = 0;
+        for (int e = 0; e < ne; e++)
+            total += (long)(cell(box[4 * (size_t)e + 2], gx0) - cell(box[4 * (size_t)e], gx0) + 1) *
+                     (cell(box[4 * (size_t)e + 3], gy0) - cell(box[4 * (size_t)e + 1], gy0) + 1);
+        if ((total <= 16l * ne && (long)bk.nx * bk.ny <= 4l * ne + 16) || tries > 60) break;
+        h *= 1.5;
+    }
+    const size_t ncell = (size_t)bk.nx * bk.ny;
+    std::vector<int> ptr(ncell + 1, 0), list((size_t)total), fill;
+    for (int pass = 0; pass < 2; pass++) {      // counts, then the lists in element order
+        if (pass) fill.assign(ptr.begin(), ptr.end() - 1);
+        for (int e = 0; e < ne; e++) {
+            const int cx0 = cell(box[4 * (size_t)e], gx0), cx1 = cell(box[4 * (size_t)e + 2], gx0);
+            const int cy0 = cell(box[4 * (size_t)e + 1], gy0), cy1 = cell(box[4 * (size_t)e + 3], gy0);
+            for (int cy = cy0; cy <= cy1; cy++)
+                for (int cx = cx0; cx <= cx1; cx++) {
+                    const size_t c = (size_t)cy * bk.nx + cx;
+                    if (pass == 0) ptr[c + 1]++;
+                    else list[(size_t)fill[c]++] = e;
+                }
+        }
+        if (pass == 0)
+            for (size_t c = 0; c < ncell; c++) ptr[c + 1] += ptr[c];
+    }
+    if (t->bk_ptr_cap < ncell + 1) {
+        dfree(t->bk_ptr);
+        t->bk_ptr = dalloc<int>(ncell + 1);
+        t->bk_ptr_cap = ncell + 1;
+    }
+    if (t->bk_list_cap < std::max<size_t>(list.size(), 1)) {
+        dfree(t->bk_list);
+        t->bk_list = dalloc<int>(std::max<size_t>(list.size(), 1));
+        t->bk_list_cap = std::max<size_t>(list.size(), 1);
+    }
+    SPHFEM_HIP_CHECK(hipMemcpyAsync(t->bk_ptr, ptr.data(), 4 * (ncell + 1), hipMemcpyHostToDevice, ctx->stream));
+    if (!list.empty())
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(t->bk_list, list.data(), 4 * list.size(), hipMemcpyHostToDevice, ctx->stream));
+    SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));    // the host vectors go out of scope
+    bk.ptr = t->bk_ptr;
+    bk.list = t->bk_list;
+    return bk;
+}
+
+extern "C" {
 
 int sphfem_tool_remesh_low_level(sphfem_ctx *ctx, sphfem_tool *t, int freq) {
     if (!t) return SPHFEM_ERR_ARG;
@@ -1122,7 +1214,8 @@ int sphfem_tool_remesh_low_level(sphfem_ctx *ctx, sphfem_tool *t, int freq) {
         if (!t->remeshable)
             return fail(ctx, SPHFEM_ERR_STATE, "remeshing needs a tool created with node flags and the fixed boundary");
         if (int rc = need(ctx, SET_TOOLC)) return rc;
-        launch_remesh_low_level(ctx, t, freq);
+        const ElemBuckets bk = build_elem_buckets(ctx, t);     // the mesh before the shift
+        launch_remesh_low_level(ctx, t, freq, bk);
         return SPHFEM_OK;
     });
 }
@@ -1147,7 +1240,8 @@ int sphfem_tool_remesh(sphfem_ctx *ctx, sphfem_tool *t, sphfem_mesh *mesh) {
             const std::vector<double> t300((size_t)nt->n_nodes, 300.);
             SPHFEM_HIP_CHECK(hipMemcpyAsync(nt->T, t300.data(), 8ul * nt->n_nodes, hipMemcpyHostToDevice,
                                             ctx->stream));
-            launch_interp_new(ctx, t, nt->pos, nt->n_nodes, nt->T);
+            const ElemBuckets bk = build_elem_buckets(ctx, t);    // the current mesh
+            launch_interp_new(ctx, t, nt->pos, nt->n_nodes, nt->T, bk);
             SPHFEM_HIP_CHECK(hipMemcpyAsync(nt->T_old, nt->T, 8ul * nt->n_nodes, hipMemcpyDeviceToDevice,
                                             ctx->stream));
             sphfem_tool_kinematics nk = k;

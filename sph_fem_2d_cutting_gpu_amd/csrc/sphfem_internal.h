@@ -151,6 +151,18 @@ struct sphfem_tool {
     sphfem_double2 *pos_map = nullptr; // [n_nodes] positions and temperatures before a remesh
     double *T_map = nullptr;
     unsigned long generation = 0;      // bumped when the device arrays are replaced
+    sphfem_int4 *h_elem = nullptr;     // host copy of the elements (new[]), for the element bins
+    int *bk_ptr = nullptr, *bk_list = nullptr;   // element bins of the last remesh (device)
+    size_t bk_ptr_cap = 0, bk_list_cap = 0;
+};
+
+// The tool's elements binned on a uniform grid by their bounding boxes (padded well beyond the
+// interpolation tolerances), each bin listing its elements in increasing order: the first
+// element of a point's bin that holds the point is the first element of the whole mesh that does.
+struct ElemBuckets {
+    int nx = 0, ny = 0;
+    double x0 = 0., y0 = 0., inv = 0.;
+    const int *ptr = nullptr, *list = nullptr;
 };
 
 // rigid (SPH) tool of tool_gpu.cu: the geometry and contact constants in device memory
@@ -243,9 +255,9 @@ void launch_tool_begin(sphfem_ctx *ctx, sphfem_tool *t);
 void launch_tool_fem_end(sphfem_ctx *ctx, sphfem_tool *t);
 void launch_tool_move(sphfem_ctx *ctx, sphfem_tool *t, double dt, bool segments, bool nodes);
 void launch_wear(sphfem_ctx *ctx, sphfem_tool *t);
-void launch_remesh_low_level(sphfem_ctx *ctx, sphfem_tool *t, int freq);
+void launch_remesh_low_level(sphfem_ctx *ctx, sphfem_tool *t, int freq, const ElemBuckets &b);
 void launch_interp_new(sphfem_ctx *ctx, const sphfem_tool *old_t, const sphfem_double2 *new_pos, int n_new,
-                       double *T_new);
+                       double *T_new, const ElemBuckets &b);
 void launch_fem_reset(sphfem_ctx *ctx, sphfem_tool *t);
 void launch_fem_temp(sphfem_ctx *ctx, sphfem_tool *t, double dt);
 void launch_fem_robin(sphfem_ctx *ctx, sphfem_tool *t);

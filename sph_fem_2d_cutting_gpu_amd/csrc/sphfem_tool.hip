@@ -437,15 +437,25 @@ __global__ void k_fem_minv(sphfem_tool t) {
     if (i < t.n_nodes) t.Minv[i] = 1. / t.M[i];
 }
 
+__device__ __forceinline__ int bucket_of(const ElemBuckets &b, sphfem_double2 x) {
+    int cx = (int)floor((x.x - b.x0) * b.inv), cy = (int)floor((x.y - b.y0) * b.inv);
+    cx = cx < 0 ? 0 : (cx >= b.nx ? b.nx - 1 : cx);
+    cy = cy < 0 ? 0 : (cy >= b.ny ? b.ny - 1 : cy);
+    return cy * b.nx + cx;
+}
+
 // do_nodal_value_interpolation_old / value_interpolation_in_layer_element (tool_FEM.cu:57-100,
 // 434-443): the first element of the mapping (in element order, counter-clockwise, weights
-// unnormalised) that holds the node gives its temperature; none keeps the mapped value
-__global__ void __launch_bounds__(256) k_interp_layer(sphfem_tool t) {
+// unnormalised) that holds the node gives its temperature; none keeps the mapped value.  The
+// candidates are the elements of the node's bin, in element order.
+__global__ void __launch_bounds__(256) k_interp_layer(sphfem_tool t, ElemBuckets b) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= t.n_nodes) return;
     const sphfem_double2 x = t.pos[i];
     double T = t.T_map[i];
-    for (int e = 0; e < t.n_elem; e++) {
+    const int cell = bucket_of(b, x);
+    for (int q = b.ptr[cell]; q < b.ptr[cell + 1]; q++) {
+        const int e = b.list[q];
         const sphfem_int4 el = t.elem[e];
         const sphfem_double2 xA = t.pos_map[el.x];
         const sphfem_double2 pb = t.pos_map[el.y], pc = t.pos_map[el.z];
@@ -465,7 +475,7 @@ __global__ void __launch_bounds__(256) k_interp_layer(sphfem_tool t) {
     t.T[i] = T;
 }
 
-void launch_remesh_low_level(sphfem_ctx *ctx, sphfem_tool *t, int freq) {
+void launch_remesh_low_level(sphfem_ctx *ctx, sphfem_tool *t, int freq, const ElemBuckets &b) {
     KTimer tm(ctx, "remesh");
     hipStream_t s = ctx->stream;
     const int nn = t->n_nodes;
@@ -475,15 +485,15 @@ void launch_remesh_low_level(sphfem_ctx *ctx, sphfem_tool *t, int freq) {
     k_dirichlet_rhs<<<nblk(nn, 256), 256, 0, s>>>(ctx->c, *t);
     k_dirichlet_rows<<<nblk(nn, 256), 256, 0, s>>>(*t);
     k_fem_minv<<<nblk(nn, 256), 256, 0, s>>>(*t);
-    k_interp_layer<<<nblk(nn, 256), 256, 0, s>>>(*t);
+    k_interp_layer<<<nblk(nn, 256), 256, 0, s>>>(*t, b);
 }
 
 // do_nodal_value_interpolation (tool_FEM.cu:445-517): temperature of a new node from the first
 // element of the current mesh that holds it (a coincident node within 1e-10 gives its value;
 // barycentric weights with a 1e-8 tolerance, the result clamped to the element's range);
-// T_new keeps its value where no element holds the node
+// T_new keeps its value where no element holds the node.  Candidates: the node's bin.
 __global__ void __launch_bounds__(256) k_interp_new(sphfem_tool t, const sphfem_double2 *new_pos, int n_new,
-                                                    double *T_new) {
+                                                    double *T_new, ElemBuckets b) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_new) return;
     const sphfem_double2 x = new_pos[i];
@@ -491,7 +501,9 @@ __global__ void __launch_bounds__(256) k_interp_new(sphfem_tool t, const sphfem_
         const double dx = x.x - a.x, dy = x.y - a.y;
         return (double)sqrtf((float)(dx * dx + dy * dy)) < 1e-10;
     };
-    for (int e = 0; e < t.n_elem; e++) {
+    const int cell = bucket_of(b, x);
+    for (int q = b.ptr[cell]; q < b.ptr[cell + 1]; q++) {
+        const int e = b.list[q];
         const sphfem_int4 el = t.elem[e];
         const sphfem_double2 xA = t.pos[el.x], xB = t.pos[el.y], xC = t.pos[el.z];
         if (near(xA)) { T_new[i] = t.T[el.x]; return; }
@@ -521,9 +533,9 @@ __global__ void __launch_bounds__(256) k_interp_new(sphfem_tool t, const sphfem_
 }
 
 void launch_interp_new(sphfem_ctx *ctx, const sphfem_tool *old_t, const sphfem_double2 *new_pos, int n_new,
-                       double *T_new) {
+                       double *T_new, const ElemBuckets &b) {
     KTimer tm(ctx, "remesh_interp");
-    k_interp_new<<<nblk(std::max(n_new, 1), 256), 256, 0, ctx->stream>>>(*old_t, new_pos, n_new, T_new);
+    k_interp_new<<<nblk(std::max(n_new, 1), 256), 256, 0, ctx->stream>>>(*old_t, new_pos, n_new, T_new, b);
 }
 
 // ---------------------------------------------------------------------------------------------
