@@ -10,7 +10,10 @@
 // Neighbour lookups go through a node -> element table (the reference scans every element).
 #include <algorithm>
 #include <array>
+#include <cerrno>
 #include <cfloat>
+#include <climits>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -43,19 +46,32 @@ struct sphfem_mesh {
 
 namespace {
 
+// the tokens std::getline(ss, tok, sep) yields: no empty token after a final separator
 std::vector<std::string> split(const std::string &line, char sep) {
     std::vector<std::string> out;
-    std::string tok;
-    std::istringstream ss(line);
-    while (std::getline(ss, tok, sep)) out.push_back(tok);
+    size_t a = 0;
+    while (a < line.size()) {
+        const size_t b = line.find(sep, a);
+        if (b == std::string::npos) {
+            out.push_back(line.substr(a));
+            break;
+        }
+        out.push_back(line.substr(a, b - a));
+        a = b + 1;
+    }
     return out;
 }
 
-// the integer at the start of a token (as `stringstream >> int`); false if there is none
+// the integer at the start of a token (as `stringstream >> int`: leading blanks, a sign, digits);
+// false if there is none or it does not fit an int
 bool to_int(const std::string &tok, int &v) {
-    std::istringstream s(tok);
-    s >> v;
-    return !s.fail();
+    const char *b = tok.c_str();
+    char *e = nullptr;
+    errno = 0;
+    const long x = std::strtol(b, &e, 10);
+    if (e == b || errno == ERANGE || x < INT_MIN || x > INT_MAX) return false;
+    v = (int)x;
+    return true;
 }
 
 // the double at the start of a string (as std::stod)
@@ -323,10 +339,13 @@ void read_gmsh(sphfem_mesh &m, const std::string &text) {
         std::istringstream in(text);
         std::string line;
         bool elem_start = false;
-        auto matches = [](const std::vector<std::array<int, 2>> &edges, int u, int v) {
-            for (const auto &e : edges)
-                if ((e[0] == u && e[1] == v) || (e[0] == v && e[1] == u)) return true;
-            return false;
+        // the boundary edges as unordered node pairs
+        auto key = [](int u, int v) { return ((long long)std::min(u, v) << 32) | (unsigned)std::max(u, v); };
+        std::set<long long> cut_set, fixed_set;
+        for (const auto &e : cut_edges) cut_set.insert(key(e[0], e[1]));
+        for (const auto &e : fixed_edges) fixed_set.insert(key(e[0], e[1]));
+        auto matches = [&](const std::vector<std::array<int, 2>> &edges, int u, int v) {
+            return (&edges == &cut_edges ? cut_set : fixed_set).count(key(u, v)) > 0;
         };
         while (std::getline(in, line)) {
             if (line.find("type=CPS3") != std::string::npos) {
