@@ -277,3 +277,33 @@ def test_remesh_needs_the_remeshing_description(tmp_path):
     m.close()
     ctx.call("sphfem_tool_destroy", t.h)
     ctx.close()
+
+
+def test_long_run_with_remesh_cadence_matches_oracle():
+    """300 steps with wear and remesh_low_level every 20 steps (the driver's freq / 20 with
+    freq = 400), the tool one lattice spacing into the workpiece, on the device and on the oracle:
+    the tool's temperature field and the particles' peak temperature agree within 1e-3 of the
+    temperature rise, the tool's nodes within 1e-9 cm (the single-precision libm calls of the
+    artificial stress make the two runs drift apart slowly)"""
+    case = cases.small_cutting_case(nx=48, ny=16, wear=True, engage=1.0)
+    sim, o = S.Simulation(case), Oracle(case)
+    freq = 400
+    print("300 steps, remesh every 20", flush=True)
+    for step in range(300):
+        if step % (freq // 20) == 0:
+            sim.tool.remesh_low_level(freq)
+            o.remesh_low_level(freq)
+        sim.step(1)
+        o.step(1)
+    sim.ctx.check_error()
+    g = sim.tool_state()
+    T0 = case.const.tool["T0"]
+    rise = o.t["T"].max() - T0
+    print("tool T rise", g["T"].max() - T0, rise)
+    assert rise > 10.0
+    assert np.max(np.abs(g["T"] - o.t["T"])) <= 1e-3 * rise
+    assert np.max(np.abs(g["node_pos"] - o.t["node_pos"])) <= 1e-9
+    a = sim.state(("T", "blanked"))
+    assert np.array_equal(a["blanked"] == 1.0, o.P["blanked"] == 1.0)
+    assert abs(a["T"].max() - o.P["T"].max()) <= 1e-3 * (o.P["T"].max() - 300.0)
+    sim.close()
