@@ -1161,6 +1161,17 @@ static ElemBuckets build_elem_buckets(sphfem_ctx *ctx, sphfem_tool *t) {
     for (int tries = 0;; tries++) {
         bk.inv = 1. / h;
         bk.x0 = gx0; bk.y0 = gy0;
+        const double span = std::max(gx1 - gx0, gy1 - gy0) * bk.inv;
+        if (!(span < 1e7)) {            // degenerate extents: widen the bins first
+            h = std::max(h * 1.5, std::max(gx1 - gx0, gy1 - gy0) * 1e-6);
+            if (!(h > 0.) || tries > 200) {
+                bk.inv = 0.;            // one bin holding every element
+                bk.nx = bk.ny = 1;
+                total = ne;
+                break;
+            }
+            continue;
+        }
         bk.nx = cell(gx1, gx0) + 1;
         bk.ny = cell(gy1, gy0) + 1;
         total = 0;
