@@ -34,12 +34,15 @@ __device__ __forceinline__ void column_range(const int *__restrict__ cs, const i
 // PRE: the neighbour factors m/rho_j, (S_j - p_j)/rho_j^2, S_xy,j/rho_j^2 and sqrtf(K/rho_j) come
 // from pre_vol/pre_s (computed per particle by the reorder with the same operations, so the sums
 // are bit-identical); otherwise they are computed per pair as in the reference.
-template <bool PRE>
+// LIST_BS > 0: the in-support neighbours go through a per-lane list of CHUNK entries in LDS
+// (list = this lane's first entry, stride LIST_BS) before their terms are evaluated.
+template <bool PRE, int LIST_BS = 0, int CHUNK = 0>
 __device__ __forceinline__ PairSums pair_sums_fp64(const DevConsts &c, const sphfem_particles &P, int pidx,
                                                    double thermal_alpha, const int *__restrict__ cs,
                                                    const int *__restrict__ ce,
                                                    const double *__restrict__ pre_vol = nullptr,
-                                                   const sphfem_double4 *__restrict__ pre_s = nullptr) {
+                                                   const sphfem_double4 *__restrict__ pre_s = nullptr,
+                                                   int *__restrict__ list = nullptr) {
     const double mass = c.phys.mass, K = c.phys.K;
     const double wdeltap = c.corr.wdeltap, alpha = c.corr.alpha, beta = c.corr.beta;
     const double eta = c.corr.eta, eps = c.corr.xspheps;
@@ -69,91 +72,127 @@ __device__ __forceinline__ PairSums pair_sums_fp64(const DevConsts &c, const sph
     r.vel_t = {0., 0.};
     r.pos_t = {0., 0.};
     r.T_lapl = 0.;
-    for (int ii = low_i; ii < high_i; ii++) {
-        int first, last;
-        column_range(cs, ce, ii * ny, low_j, high_j, first, last);
-        if (first < 0 || last > (int)P.N) continue;   // empty column (or an inconsistent cell table)
-        for (int iter = first; iter < last; iter++) {
-            if (P.blanked[iter] == 1.) continue;
-            const sphfem_double2 pj = P.pos[iter];
-            double xij = pi.x - pj.x;
-            double yij = pi.y - pj.y;
-            double rr2 = xij * xij + yij * yij;
-            if (rr2 >= fourh2 || rr2 < rr2_min) continue;
-            double w, w_x, w_y;
-            cubic_spline(pi.x, pi.y, pj.x, pj.y, hi, rr2_min, w, w_x, w_y);
-            const double rhoj = P.rho[iter];
-            const double is_tool_j = P.tool_particle[iter];
-            if (!(is_tool_i == 1.) && !(is_tool_j == 1.)) {
-                const sphfem_double2 vj = P.vel[iter];
-                const sphfem_double4 Rj = P.R[iter];
-                const double hj = P.h[iter];
-                double volj, sjx, sjy, sjz;
-                if constexpr (PRE) {
-                    volj = pre_vol[iter];
-                    const sphfem_double4 f = pre_s[iter];
-                    sjx = f.x;
-                    sjy = f.y;
-                    sjz = f.z;
-                } else {
-                    const sphfem_double4 Sj = P.S[iter];
-                    const double prsj = P.p[iter];
-                    volj = mass / rhoj;
-                    const double rhoj21 = 1. / (rhoj * rhoj);
-                    sjx = (Sj.x - prsj) * rhoj21;
-                    sjy = Sj.y * rhoj21;
-                    sjz = (Sj.z - prsj) * rhoj21;
-                }
-                r.v_der.x += (vj.x - vi.x) * w_x * volj;
-                r.v_der.y += (vj.x - vi.x) * w_y * volj;
-                r.v_der.z += (vj.y - vi.y) * w_x * volj;
-                r.v_der.w += (vj.y - vi.y) * w_y * volj;
-                double fab = unit_wdp ? w : w / wdeltap;
-                fab *= fab;
-                fab *= fab;
-                double Rx = fab * (Ri.x + Rj.x);
-                double Ry = fab * (Ri.y + Rj.y);
-                double Rz = fab * (Ri.z + Rj.z);
-                r.S_der.x += mass * ((Si.x - prsi) * rhoi21 + sjx + Rx) * w_x;
-                r.S_der.y += mass * (Si.y * rhoi21 + sjy + Ry) * w_y;
-                r.S_der.z += mass * (Si.y * rhoi21 + sjy + Ry) * w_x;
-                r.S_der.w += mass * ((Si.z - prsi) * rhoi21 + sjz + Rz) * w_y;
-                double vijx = vi.x - vj.x;
-                double vijy = vi.y - vj.y;
-                double vijposij = vijx * xij + vijy * yij;
-                double rhoij = 0.5 * (rhoi + rhoj);
-                if (vijposij < 0.) {
-                    double ci, cj;
-                    if constexpr (PRE) {
-                        ci = ci_pre;
-                        cj = pre_s[iter].w;
-                    } else {
-                        ci = sqrtf_d(K / rhoi);
-                        cj = sqrtf_d(K / rhoj);
-                    }
-                    double cij = 0.5 * (ci + cj);
-                    double hij = 0.5 * (hi + hj);
-                    double r2ij = xij * xij + yij * yij;
-                    double muij = (hij * vijposij) / (r2ij + eta * eta * hij * hij);
-                    double piij = (-alpha * cij * muij + beta * muij * muij) / rhoij;
-                    r.vel_t.x += -mass * piij * w_x;
-                    r.vel_t.y += -mass * piij * w_y;
-                }
-                r.pos_t.x += -eps * w * mass / rhoij * vijx;
-                r.pos_t.y += -eps * w * mass / rhoij * vijy;
+    // one in-support pair (xij, yij from the sorted positions); the order of the calls is the
+    // reference's neighbour order, so the sums are the same whichever loop drives them
+    auto pair = [&](int iter, const sphfem_double2 pj, double xij, double yij) {
+        double w, w_x, w_y;
+        cubic_spline(pi.x, pi.y, pj.x, pj.y, hi, rr2_min, w, w_x, w_y);
+        const double rhoj = P.rho[iter];
+        const double is_tool_j = P.tool_particle[iter];
+        if (!(is_tool_i == 1.) && !(is_tool_j == 1.)) {
+            const sphfem_double2 vj = P.vel[iter];
+            const sphfem_double4 Rj = P.R[iter];
+            const double hj = P.h[iter];
+            double volj, sjx, sjy, sjz;
+            if constexpr (PRE) {
+                volj = pre_vol[iter];
+                const sphfem_double4 f = pre_s[iter];
+                sjx = f.x;
+                sjy = f.y;
+                sjz = f.z;
+            } else {
+                const sphfem_double4 Sj = P.S[iter];
+                const double prsj = P.p[iter];
+                volj = mass / rhoj;
+                const double rhoj21 = 1. / (rhoj * rhoj);
+                sjx = (Sj.x - prsj) * rhoj21;
+                sjy = Sj.y * rhoj21;
+                sjz = (Sj.z - prsj) * rhoj21;
             }
-            if (thermal_alpha != 0.) {
-                double rij = sqrt(xij * xij + yij * yij);
-                if (rij > 1e-8) {
-                    const double Tj = P.T[iter];
-                    double eijx = xij / rij;
-                    double eijy = yij / rij;
-                    double rij1 = 1. / rij;
-                    const double volT = PRE ? pre_vol[iter] : mass / rhoj;
-                    r.T_lapl += 2.0 * volT * (Ti - Tj) * rij1 * (eijx * w_x + eijy * w_y);
+            r.v_der.x += (vj.x - vi.x) * w_x * volj;
+            r.v_der.y += (vj.x - vi.x) * w_y * volj;
+            r.v_der.z += (vj.y - vi.y) * w_x * volj;
+            r.v_der.w += (vj.y - vi.y) * w_y * volj;
+            double fab = unit_wdp ? w : w / wdeltap;
+            fab *= fab;
+            fab *= fab;
+            double Rx = fab * (Ri.x + Rj.x);
+            double Ry = fab * (Ri.y + Rj.y);
+            double Rz = fab * (Ri.z + Rj.z);
+            r.S_der.x += mass * ((Si.x - prsi) * rhoi21 + sjx + Rx) * w_x;
+            r.S_der.y += mass * (Si.y * rhoi21 + sjy + Ry) * w_y;
+            r.S_der.z += mass * (Si.y * rhoi21 + sjy + Ry) * w_x;
+            r.S_der.w += mass * ((Si.z - prsi) * rhoi21 + sjz + Rz) * w_y;
+            double vijx = vi.x - vj.x;
+            double vijy = vi.y - vj.y;
+            double vijposij = vijx * xij + vijy * yij;
+            double rhoij = 0.5 * (rhoi + rhoj);
+            if (vijposij < 0.) {
+                double ci, cj;
+                if constexpr (PRE) {
+                    ci = ci_pre;
+                    cj = pre_s[iter].w;
+                } else {
+                    ci = sqrtf_d(K / rhoi);
+                    cj = sqrtf_d(K / rhoj);
                 }
+                double cij = 0.5 * (ci + cj);
+                double hij = 0.5 * (hi + hj);
+                double r2ij = xij * xij + yij * yij;
+                double muij = (hij * vijposij) / (r2ij + eta * eta * hij * hij);
+                double piij = (-alpha * cij * muij + beta * muij * muij) / rhoij;
+                r.vel_t.x += -mass * piij * w_x;
+                r.vel_t.y += -mass * piij * w_y;
+            }
+            r.pos_t.x += -eps * w * mass / rhoij * vijx;
+            r.pos_t.y += -eps * w * mass / rhoij * vijy;
+        }
+        if (thermal_alpha != 0.) {
+            double rij = sqrt(xij * xij + yij * yij);
+            if (rij > 1e-8) {
+                const double Tj = P.T[iter];
+                double eijx = xij / rij;
+                double eijy = yij / rij;
+                double rij1 = 1. / rij;
+                const double volT = PRE ? pre_vol[iter] : mass / rhoj;
+                r.T_lapl += 2.0 * volT * (Ti - Tj) * rij1 * (eijx * w_x + eijy * w_y);
             }
         }
+    };
+    if constexpr (LIST_BS == 0) {
+        for (int ii = low_i; ii < high_i; ii++) {
+            int first, last;
+            column_range(cs, ce, ii * ny, low_j, high_j, first, last);
+            if (first < 0 || last > (int)P.N) continue;   // empty column (or an inconsistent cell table)
+            for (int iter = first; iter < last; iter++) {
+                if (P.blanked[iter] == 1.) continue;
+                const sphfem_double2 pj = P.pos[iter];
+                double xij = pi.x - pj.x;
+                double yij = pi.y - pj.y;
+                double rr2 = xij * xij + yij * yij;
+                if (rr2 >= fourh2 || rr2 < rr2_min) continue;
+                pair(iter, pj, xij, yij);
+            }
+        }
+    } else {
+        // the in-support neighbours are listed first (in the same order) and the pair terms are
+        // evaluated from the list, so the lanes of a wavefront evaluate pairs together instead of
+        // idling on candidates outside the support
+        int n = 0;
+        auto run = [&]() {
+            for (int k = 0; k < n; k++) {
+                const int iter = list[k * LIST_BS];
+                const sphfem_double2 pj = P.pos[iter];
+                pair(iter, pj, pi.x - pj.x, pi.y - pj.y);
+            }
+            n = 0;
+        };
+        for (int ii = low_i; ii < high_i; ii++) {
+            int first, last;
+            column_range(cs, ce, ii * ny, low_j, high_j, first, last);
+            if (first < 0 || last > (int)P.N) continue;
+            for (int iter = first; iter < last; iter++) {
+                if (P.blanked[iter] == 1.) continue;
+                const sphfem_double2 pj = P.pos[iter];
+                double xij = pi.x - pj.x;
+                double yij = pi.y - pj.y;
+                double rr2 = xij * xij + yij * yij;
+                if (rr2 >= fourh2 || rr2 < rr2_min) continue;
+                list[n * LIST_BS] = iter;
+                if (++n == CHUNK) run();
+            }
+        }
+        run();
     }
     return r;
 }
@@ -607,11 +646,14 @@ __global__ void __launch_bounds__(BS) k_pairs_fp64(DevConsts c, sphfem_particles
                                                    const int *__restrict__ ce, const double *__restrict__ pre_vol,
                                                    const sphfem_double4 *__restrict__ pre_s,
                                                    PairSums *__restrict__ psum64) {
+    constexpr int CHUNK = 32;
+    __shared__ int nb_list[CHUNK * BS];
     const int i = xcd_logical_block() * BS + threadIdx.x;
     if (i >= (int)B.N) return;
     if (B.blanked[i] == 1.) return;
     const double thermal_alpha = interaction_alpha(c, B.tool_particle[i]);
-    psum64[i] = pair_sums_fp64<true>(c, B, i, thermal_alpha, cs, ce, pre_vol, pre_s);
+    psum64[i] = pair_sums_fp64<true, BS, CHUNK>(c, B, i, thermal_alpha, cs, ce, pre_vol, pre_s,
+                                                nb_list + threadIdx.x);
 }
 
 // RIGID: the rigid-tool step (leap_frog.cu:99-160): friction heat after the corrector
