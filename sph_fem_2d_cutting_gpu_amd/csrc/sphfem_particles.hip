@@ -189,7 +189,10 @@ __device__ __forceinline__ PairSums pair_sums_fp64(const DevConsts &c, const sph
                 double rr2 = xij * xij + yij * yij;
                 if (rr2 >= fourh2 || rr2 < rr2_min) continue;
                 list[n * LIST_BS] = iter;
-                if (++n == CHUNK) run();
+                ++n;
+                // a full list empties every active lane's list at once: lanes that flushed at
+                // their own points would each run their list with the rest of the wavefront idle
+                if (__any(n == CHUNK)) run();
             }
         }
         run();
@@ -443,11 +446,13 @@ __device__ __forceinline__ double interaction_alpha(const DevConsts &c, double i
 // interactions_gpu.cu:143-379 as one kernel (drop-in entry point)
 __global__ void __launch_bounds__(256) k_interactions_ref(DevConsts c, sphfem_particles P, const int *__restrict__ cs,
                                                           const int *__restrict__ ce) {
+    __shared__ int nb_list[48 * 256];
     int pidx = blockIdx.x * blockDim.x + threadIdx.x;
     if (pidx >= (int)P.N) return;
     if (P.blanked[pidx] == 1.) return;
     double thermal_alpha = interaction_alpha(c, P.tool_particle[pidx]);
-    PairSums r = pair_sums_fp64<false>(c, P, pidx, thermal_alpha, cs, ce);
+    PairSums r = pair_sums_fp64<false, 256, 48>(c, P, pidx, thermal_alpha, cs, ce, nullptr, nullptr,
+                                                nb_list + threadIdx.x);
     P.S_der[pidx] = r.S_der;
     P.v_der[pidx] = r.v_der;
     P.pos_t[pidx] = r.pos_t;
@@ -646,7 +651,7 @@ __global__ void __launch_bounds__(BS) k_pairs_fp64(DevConsts c, sphfem_particles
                                                    const int *__restrict__ ce, const double *__restrict__ pre_vol,
                                                    const sphfem_double4 *__restrict__ pre_s,
                                                    PairSums *__restrict__ psum64) {
-    constexpr int CHUNK = 32;
+    constexpr int CHUNK = 48;
     __shared__ int nb_list[CHUNK * BS];
     const int i = xcd_logical_block() * BS + threadIdx.x;
     if (i >= (int)B.N) return;
