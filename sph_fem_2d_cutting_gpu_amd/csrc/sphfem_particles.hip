@@ -421,7 +421,9 @@ __global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *_
                         n++;
                     }
                 }
-                if (n > CHUNK - 4) {                                     // dense region: empty the list
+                // dense region: every active lane empties its list at once (a lane emptying its
+                // list alone runs it with the rest of the wavefront idle)
+                if (__any(n > CHUNK - 4)) {
                     run_list(n);
                     n = 0;
                 }
