@@ -184,3 +184,21 @@ def test_remesh_geo_restatement_picks_chain_points(case):
     assert 'Physical Curve("FixedBoundary", 2) = {1, %d};' % n in text
     cut = re.search(r'Physical Curve\("CuttingEdge", 67\) = \{([^}]*)\}', text).group(1)
     assert [int(v) for v in cut.split(",")] == list(range(2, n))
+
+
+def test_remesh_geo_skips_the_point_before_a_short_edge():
+    """hand-worked chain (tool_FEM.cu:1736-1757): after two unit segments, the segment ending
+    0.1 before the next point makes |left[1] - right[2]| = 1.1 < 1.25 x 1 with the previous length
+    1 > 0.7, so the left end of segment 2 is skipped; the last point is right[3]; mesh sizes are
+    6 x the segment lengths"""
+    left = np.array([[0.0, 4.0], [0.0, 3.0], [0.0, 2.0], [0.0, 1.9]])
+    right = np.array([[0.0, 3.0], [0.0, 2.0], [0.0, 1.9], [0.0, 0.0]])
+    text = R.remesh_geo(left, right, (-1.0, 4.0))
+    pts = re.findall(r"Point\((\d+)\) = \{([^,]+), ([^,]+), 0, ([^}]+)\};", text)
+    xy = [(float(p[1]), float(p[2])) for p in pts]
+    assert xy == [(-1.0, 4.0), (0.0, 4.0), (0.0, 3.0), (0.0, 1.9), (0.0, 0.0)]
+    sizes = [float(p[3]) for p in pts]
+    assert sizes[1] == 6.0 and sizes[2] == 6.0
+    assert sizes[3] == pytest.approx(6 * 1.9, rel=1e-7) and sizes[4] == sizes[3]
+    assert 'Physical Curve("CuttingEdge", 67) = {2, 3, 4};' in text
+    assert 'Physical Curve("FixedBoundary", 2) = {1, 5};' in text
