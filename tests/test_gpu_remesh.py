@@ -307,3 +307,23 @@ def test_long_run_with_remesh_cadence_matches_oracle():
     assert np.array_equal(a["blanked"] == 1.0, o.P["blanked"] == 1.0)
     assert abs(a["T"].max() - o.P["T"].max()) <= 1e-3 * (o.P["T"].max() - 300.0)
     sim.close()
+
+
+def test_remesh_geo_with_a_short_segment(tmp_path):
+    """the library's remesh geometry on a chain with one segment shortened to 5 % (so the point
+    before it is skipped, tool_FEM.cu:1743-1747) equals the restatement"""
+    case = _case()
+    sim = S.Simulation(case)
+    g = sim.tool_state()
+    left, right = g["left"].copy(), g["right"].copy()
+    k = len(left) // 2
+    right[k] = left[k] + 0.05 * (right[k] - left[k])
+    left[k + 1] = right[k]
+    sim.tool.upload(dict(left=left, right=right))
+    path = str(tmp_path / "short.geo")
+    sim.tool.write_remesh_geo(path)
+    kin = sim.tool.kinematics()
+    text = open(path).read()
+    assert text == R.remesh_geo(left, right, (kin["tl_x"], kin["tl_y"]))
+    assert text.count("Point(") == len(left) + 1     # tl, the chain less one point, the end
+    sim.close()
