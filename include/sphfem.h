@@ -185,6 +185,14 @@ int sphfem_grid_get_cells(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particle
 int sphfem_interactions_setup_geometry_constants(sphfem_ctx *ctx, const sphfem_grid *g);
 int sphfem_interactions_monaghan(sphfem_ctx *ctx, sphfem_particles *p, const int *cell_start,
                                  const int *cell_end, int num_cell);
+/* interactions_gpu.cu:466-474 in the FP32-compensated form (the pair sums of
+ * sphfem_leap_frog_step with SPHFEM_FP32C, lf set to that precision): FP32 terms on cell-relative
+ * positions, FP32 blocks of 8 terms accumulated in FP64, from the sorted state p (p and R as left
+ * by material_eos and corrector_artificial_stress).  Writes v_der, pos_t and T_t as the reference
+ * does; the stress-divergence and viscous accelerations come summed per axis as S_der = {ax, 0,
+ * ay, 0} with vel_t = 0, the two sums the momentum equation (actions_gpu.cu:641-671) uses. */
+int sphfem_interactions_monaghan_fp32c(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p,
+                                       const int *cell_start, const int *cell_end, int num_cell);
 /* interactions_gpu.cu:84-140, 434-464 (PSE heat conduction, compiled out of the reference's
  * flag set): T_t = alpha sum (T_j - T_i) 4/(pi h^4) exp(-r^2/h^2) m/rho_j over the 5x5 cells */
 int sphfem_interactions_heat_pse(sphfem_ctx *ctx, sphfem_particles *p, const int *cell_start,

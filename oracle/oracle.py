@@ -116,6 +116,7 @@ def lib():
             "o_run_unlocked": (pc, pp, ps, pt, C.c_int, C.c_int, C.c_int),
             "o_predict": (pp, ps, _dbl), "o_correct": (pp, ps, _dbl), "o_material_eos": (pc, pp),
             "o_artificial_stress": (pc, pp), "o_interactions_monaghan": (pc, pp, ps),
+            "o_interactions_monaghan_range": (pc, pp, ps, C.c_int, C.c_int),
             "o_interactions_heat_pse": (pc, pp, ps),
             "o_stress_rate_jaumann": (pc, pp), "o_continuity": (pp,), "o_momentum": (pc, pp),
             "o_advection": (pp,), "o_fric_heat_v21": (pc, pp, pt), "o_plasticity": (pc, pp),

@@ -374,13 +374,18 @@ void o_interactions_heat_pse(const o_consts *c, o_particles *P, const o_stepper 
 /* interactions_gpu.cu:143-379 (Brookshaw on, SURFACE off); neighbours in
  * the reference's cell order ii (column), jj (row), then sorted index. */
 void o_interactions_monaghan(const o_consts *c, o_particles *P, const o_stepper *st) {
+    o_interactions_monaghan_range(c, P, st, 0, P->N);
+}
+
+/* the same sums for the sorted particles i0 <= pidx < i1 only (sub-windows of large inputs) */
+void o_interactions_monaghan_range(const o_consts *c, o_particles *P, const o_stepper *st, int i0, int i1) {
     const double mass = c->phys.mass, K = c->phys.K;
     const double wdeltap = c->corr.wdeltap, alpha = c->corr.alpha, beta = c->corr.beta;
     const double eta = c->corr.eta, eps = c->corr.xspheps;
     const int nx = c->geom.nx, ny = c->geom.ny;
     const double rr2_min = c->rr2_min;
 #pragma omp parallel for schedule(dynamic, 128)
-    for (int pidx = 0; pidx < P->N; pidx++) {
+    for (int pidx = i0; pidx < i1; pidx++) {
         if (P->blanked[pidx] == 1.) continue;
         double is_tool_i = P->tool_particle[pidx];
         /* interactions_gpu.cu:164: the tool block (:19) is zero unless interactions_setup_thermal_constants_tool

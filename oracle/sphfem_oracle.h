@@ -110,6 +110,7 @@ void o_correct(o_particles *P, o_stepper *st, double dt);
 void o_material_eos(const o_consts *c, o_particles *P);
 void o_artificial_stress(const o_consts *c, o_particles *P);
 void o_interactions_monaghan(const o_consts *c, o_particles *P, const o_stepper *st);
+void o_interactions_monaghan_range(const o_consts *c, o_particles *P, const o_stepper *st, int i0, int i1);
 void o_interactions_heat_pse(const o_consts *c, o_particles *P, const o_stepper *st);
 void o_stress_rate_jaumann(const o_consts *c, o_particles *P);
 void o_continuity(o_particles *P);

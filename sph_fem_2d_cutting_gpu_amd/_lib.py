@@ -146,6 +146,7 @@ _SIGS = {
     "sphfem_interactions_setup_geometry_constants": (_ctx, _vp),
     "sphfem_interactions_monaghan": (_ctx, _P(Particles), _vp, _vp, C.c_int),
     "sphfem_interactions_heat_pse": (_ctx, _P(Particles), _vp, _vp, C.c_int),
+    "sphfem_interactions_monaghan_fp32c": (_ctx, _vp, _P(Particles), _vp, _vp, C.c_int),
     "sphfem_tool_create": (_ctx, _P(ToolDesc), _P(_vp)),
     "sphfem_tool_destroy": (_ctx, _vp),
     "sphfem_tool_set_vel": (_ctx, _vp, Double2),

@@ -825,11 +825,11 @@ def chip_case(n_target=16_000_000, seed=DEFAULT_SEED, dx=1e-4, material="Ti6Al4V
     deviatoric stress ~ N(0, 300 MPa) with Szz = -(Sxx + Syy), rho = rho0, T = 300.
     """
     const = make_constants(material, dx, 0.35)
-    # slab of aspect 4:1 with a chip band of the same area on top; solve for the lattice size
+    # slab of aspect 4:1 (4 base^2 lattice points) with a chip band of about 0.35 base^2 points on
+    # top, so base = sqrt(n_target / 4.35) gives N just above n_target.
     # region A: slab [0, Lx] x [0, Ly]; region B: band along the rake at 45 degrees.
-    # density factor rho_n(x,y) = 1 + 9 * exp(-(d/0.15 Ly)^2) around the chip root.
     ratio = 4.0
-    base = int(math.sqrt(n_target / (ratio * 1.0 * 2.2)))
+    base = int(math.ceil(math.sqrt(n_target / 4.35)))
     nxs, nys = int(ratio * base), base
     Lx, Ly = nxs * dx, nys * dx
     root = np.array([0.6 * Lx, Ly])

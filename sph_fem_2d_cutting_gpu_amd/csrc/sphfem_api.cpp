@@ -573,6 +573,19 @@ int sphfem_interactions_monaghan(sphfem_ctx *ctx, sphfem_particles *p, const int
     });
 }
 
+int sphfem_interactions_monaghan_fp32c(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p,
+                                       const int *cell_start, const int *cell_end, int num_cell) {
+    if (!ctx || !lf || !p || !cell_start || !cell_end || num_cell != ctx->c.geom.nx * ctx->c.geom.ny)
+        return SPHFEM_ERR_ARG;
+    if (lf->precision != SPHFEM_FP32C || !lf->rec) return fail(ctx, SPHFEM_ERR_STATE, "leap_frog is not in the FP32C form");
+    if (p->N > lf->num_part) return fail(ctx, SPHFEM_ERR_ARG, "more particles than the leap_frog was made for");
+    return guarded(ctx, [&] {
+        if (int rc = need(ctx, SET_PHYS | SET_CORR)) return rc;
+        launch_interactions_fp32c(ctx, *p, lf, cell_start, cell_end, num_cell);
+        return SPHFEM_OK;
+    });
+}
+
 int sphfem_interactions_heat_pse(sphfem_ctx *ctx, sphfem_particles *p, const int *cell_start, const int *cell_end,
                                  int num_cell) {
     if (!ctx || !p || !cell_start || !cell_end || num_cell != ctx->c.geom.nx * ctx->c.geom.ny) return SPHFEM_ERR_ARG;

@@ -237,12 +237,15 @@ void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p);
 void launch_fill_int(sphfem_ctx *ctx, int *ptr, long n, int value);
 void launch_reorder_full(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p);
 void launch_reorder_predict(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, sphfem_leap_frog *lf);
+void launch_records_state(sphfem_ctx *ctx, const sphfem_particles &p, int num_cell, float4 *rec);
 
 // particles (sphfem_particles.hip)
 void launch_actions(sphfem_ctx *ctx, int which, const sphfem_particles &p);
 void launch_leapfrog(sphfem_ctx *ctx, int which, const sphfem_particles &p, sphfem_leap_frog *lf, double dt);
 void launch_interactions_ref(sphfem_ctx *ctx, const sphfem_particles &p, const int *cs, const int *ce);
 void launch_interactions_pse(sphfem_ctx *ctx, const sphfem_particles &p, const int *cs, const int *ce);
+void launch_interactions_fp32c(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_leap_frog *lf, const int *cs,
+                               const int *ce, int num_cell);
 void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfem_particles &A,
                          sphfem_leap_frog *lf, const int *cs, const int *ce, sphfem_tool *t,
                          sphfem_rigid_tool *rt = nullptr);

@@ -892,6 +892,55 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
     }
 }
 
+// FP32C pair sums written into the reference's interaction outputs (interactions_gpu.cu:370-378):
+// v_der; the stress-divergence plus viscous acceleration per axis as S_der = {ax, 0, ay, 0} with
+// vel_t = 0 (the momentum equation only uses S_der.x + S_der.y + vel_t.x and S_der.z + S_der.w +
+// vel_t.y); XSPH in pos_t; alpha times the Laplacian in T_t where the thermal block is set.
+// Blanked particles are left untouched, as by the reference.
+__global__ void __launch_bounds__(256) k_psum_expand(DevConsts c, sphfem_particles P, const float4 *__restrict__ psum,
+                                                     const float *__restrict__ psum_T) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = (int)P.N;
+    if (i >= N) return;
+    if (P.blanked[i] == 1.) return;
+    const double alpha = interaction_alpha(c, P.tool_particle[i]);
+    const float4 g = psum[i], a = psum[(size_t)N + i];
+    P.v_der[i] = {g.x, g.y, g.z, g.w};
+    P.S_der[i] = {a.x, 0., a.y, 0.};
+    P.vel_t[i] = {0., 0.};
+    P.pos_t[i] = {a.z, a.w};
+    if (alpha != 0.) P.T_t[i] = alpha * (double)psum_T[i];
+}
+
+template <int BSZ, int CH>
+static void launch_pairs_fp32c(sphfem_ctx *ctx, const sphfem_particles &B, sphfem_leap_frog *lf, const int *cs,
+                               const int *ce) {
+    const size_t n = B.N;
+    RecView rv{lf->rec, lf->rec + n + 4, lf->rec + 2 * n + 4, lf->rec + 3 * n + 4};
+    k_pairs_fp32c<BSZ, CH><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(ctx->c, B.blanked, B.hash, (int)B.N, rv, cs, ce,
+                                                                    lf->psum, lf->psum_T);
+}
+
+static void launch_pairs_fp32c_tuned(sphfem_ctx *ctx, const sphfem_particles &B, sphfem_leap_frog *lf, const int *cs,
+                                     const int *ce) {
+    KTimer tm(ctx, "pairs_fp32c");
+    const int pb = lf->tuning.pair_block, pc = lf->tuning.pair_chunk;
+    if (pb == 128 && pc == 16) launch_pairs_fp32c<128, 16>(ctx, B, lf, cs, ce);
+    else if (pb == 128 && pc == 32) launch_pairs_fp32c<128, 32>(ctx, B, lf, cs, ce);
+    else if (pb == 128 && pc == 48) launch_pairs_fp32c<128, 48>(ctx, B, lf, cs, ce);
+    else if (pb == 256 && pc == 16) launch_pairs_fp32c<256, 16>(ctx, B, lf, cs, ce);
+    else if (pb == 256 && pc == 48) launch_pairs_fp32c<256, 48>(ctx, B, lf, cs, ce);
+    else launch_pairs_fp32c<256, 32>(ctx, B, lf, cs, ce);
+}
+
+void launch_interactions_fp32c(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_leap_frog *lf, const int *cs,
+                               const int *ce, int num_cell) {
+    launch_records_state(ctx, p, num_cell, lf->rec);
+    launch_pairs_fp32c_tuned(ctx, p, lf, cs, ce);
+    KTimer t(ctx, "psum_expand");
+    k_psum_expand<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, lf->psum, lf->psum_T);
+}
+
 void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfem_particles &A, sphfem_leap_frog *lf,
                          const int *cs, const int *ce, sphfem_tool *t, sphfem_rigid_tool *rt) {
     ToolView tv{};
@@ -909,22 +958,7 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
     const int wi = lf->intermediates ? 1 : 0;
     const RigidTool *rtd = rt ? rt->dev : nullptr;
     if (lf->precision == SPHFEM_FP32C) {
-        const size_t n = B.N;
-        RecView rv{lf->rec, lf->rec + n + 4, lf->rec + 2 * n + 4, lf->rec + 3 * n + 4};
-        {
-            KTimer tm(ctx, "pairs_fp32c");
-            const int pb = lf->tuning.pair_block, pc = lf->tuning.pair_chunk;
-#define SPHFEM_PAIRS(BSZ, CH)                                                                              \
-    k_pairs_fp32c<BSZ, CH><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(ctx->c, B.blanked, B.hash, (int)B.N, rv, cs, \
-                                                                    ce, lf->psum, lf->psum_T)
-            if (pb == 128 && pc == 16) SPHFEM_PAIRS(128, 16);
-            else if (pb == 128 && pc == 32) SPHFEM_PAIRS(128, 32);
-            else if (pb == 128 && pc == 48) SPHFEM_PAIRS(128, 48);
-            else if (pb == 256 && pc == 16) SPHFEM_PAIRS(256, 16);
-            else if (pb == 256 && pc == 48) SPHFEM_PAIRS(256, 48);
-            else SPHFEM_PAIRS(256, 32);
-#undef SPHFEM_PAIRS
-        }
+        launch_pairs_fp32c_tuned(ctx, B, lf, cs, ce);
         KTimer tm(ctx, "update_fp32c");
 #define SPHFEM_UPDATE(BSZ, RG)                                                                              \
     k_fused_update<true, BSZ, RG><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(                                  \

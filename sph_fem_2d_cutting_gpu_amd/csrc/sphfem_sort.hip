@@ -777,6 +777,30 @@ __device__ __forceinline__ void write_records(const DevConsts &c, float4 *__rest
     rec[n4 + 2 * (size_t)N + i] = r3;
 }
 
+// FP32C records of an already sorted state as it stands (no predictor): the inputs of
+// interactions_monaghan, with p and R as material_eos and corrector_artificial_stress left them
+// (interactions_gpu.cu:143-379 reads them from the particle arrays)
+__global__ void __launch_bounds__(256) k_records_state(DevConsts c, const sphfem_particles p, int num_cell,
+                                                       int *__restrict__ flags, float4 *__restrict__ rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = (int)p.N;
+    if (i >= N) return;
+    int h = p.hash[i];
+    if ((unsigned)h >= (unsigned)num_cell) {
+        atomicOr(&flags[0], 4);
+        h = 0;
+    }
+    const double tool = p.tool_particle[i];
+    const double pr = tool == 0. ? p.p[i] : 0.;
+    const sphfem_double4 R = tool == 0. ? p.R[i] : sphfem_double4{0., 0., 0., 0.};
+    write_records(c, rec, N, i, h, p.pos[i], p.vel[i], p.S[i], p.rho[i], p.T[i], p.blanked[i], tool, p.h[i], pr, R);
+}
+
+void launch_records_state(sphfem_ctx *ctx, const sphfem_particles &p, int num_cell, float4 *rec) {
+    KTimer t(ctx, "records_state");
+    k_records_state<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, num_cell, ctx->d_flags, rec);
+}
+
 // cell boundaries of the sorted keys (grid_gpu_green.cu:13-51); false if the key or the source
 // index is out of range (flagged)
 __device__ __forceinline__ bool cell_bounds(const sphfem_particles &p, int i, int N, int num_cell,

@@ -640,6 +640,14 @@ def interactions_monaghan(p: particle_gpu, cell_start_dptr, cell_end_dptr, num_c
                C.c_void_p(cell_end_dptr), int(num_cell))
 
 
+def interactions_monaghan_fp32c(lf: "leap_frog", p: particle_gpu, cell_start_dptr, cell_end_dptr, num_cell):
+    """interactions_monaghan in the FP32-compensated form of the step's pair sums (lf in FP32C):
+    v_der, pos_t and T_t as the reference writes them; stress divergence plus viscosity summed per
+    axis in S_der = (ax, 0, ay, 0), vel_t = 0."""
+    p.ctx.call("sphfem_interactions_monaghan_fp32c", lf.h, C.byref(p.s), C.c_void_p(cell_start_dptr),
+               C.c_void_p(cell_end_dptr), int(num_cell))
+
+
 def interactions_heat_pse(p: particle_gpu, cell_start_dptr, cell_end_dptr, num_cell):
     """interactions_gpu.cu:434-464: PSE heat conduction into T_t."""
     p.ctx.call("sphfem_interactions_heat_pse", C.byref(p.s), C.c_void_p(cell_start_dptr),

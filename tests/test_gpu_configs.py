@@ -1,0 +1,229 @@
+"""The benchmark configurations at their full sizes against the oracle.
+
+* 160k particles of AISI 1045 with an ~8k-triangle FEM tool (FEM_thermal_gpu.cu:342-468,
+  benchmarks_NZ.cpp:51-367): 2000 steps in the FP64 and the FP32-compensated form; cutting force
+  within 0.5 % of the oracle's running maximum, peak temperature rise and peak/mean plastic strain
+  within 0.5 %.
+* 4M particles of Ti6Al4V with a ~50k-triangle tool, FEM heat and Usui wear: a 10-step window of the
+  FP64 form against the oracle (hash, sorted index and blanked flags exact, fields within 1e-6), and
+  one step of the FP32C form against the FP64 form from a perturbed state (every pair term non-zero).
+* 16.3M particles on the chip-shaped domain with a 10:1 density range (kernel-only): the hash and
+  stable sort against numpy, then one interaction pass in both forms against the oracle on four
+  windows of 60k sorted particles (grid_gpu_green.cu:184-213, interactions_gpu.cu:143-379).
+* the 64-case parameter sweep (20k-300k particles per case): every case of the batch equals the same
+  case stepped alone, and one case follows the oracle.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from helpers import assert_close, assert_equal
+from oracle.oracle import Oracle
+from sph_fem_2d_cutting_gpu_amd import _lib, cases
+from sph_fem_2d_cutting_gpu_amd import solver as S
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("pos", "vel", "S", "rho", "T", "eps_pl", "eps_pl_dot", "fc", "ft", "n")
+DERIVS = ("pos_t", "vel_t", "rho_t", "S_t", "T_t")
+
+
+def normwise(a, b, mask=None):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if mask is not None:
+        a, b = a[mask], b[mask]
+    scale = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (scale if scale > 0 else 1.0))
+
+
+# ------------------------------------------------------------------------------- 160k, 2000 steps
+@pytest.fixture(scope="module")
+def config2_oracle():
+    case = cases.config2()
+    print(f"config2 oracle: {case.N} particles, {len(case.tool.elems)} triangles, 2000 steps", flush=True)
+    o = Oracle(case)
+    F = []
+    for k in range(100):
+        o.step(20)
+        F.append(o.report_force())
+    return case, o, np.array(F)
+
+
+@pytest.mark.parametrize("prec", [_lib.FP64, _lib.FP32C])
+def test_config2_observables_2000_steps(config2_oracle, prec):
+    case, o, F_o = config2_oracle
+    print(f"config2 GPU, precision {prec}", flush=True)
+    sim = S.Simulation(case, precision=prec)
+    F_g = []
+    for k in range(100):
+        sim.step(20)
+        F_g.append(S.report_force(sim.p))
+    sim.ctx.check_error()
+    F_g = np.array(F_g)
+    scale = np.max(np.abs(F_o), axis=0)
+    ferr = np.max(np.abs(F_g - F_o), axis=0) / scale
+    a = sim.state(("T", "eps_pl", "blanked"))
+    T0 = case.const.joco["Tref"]
+    dT_o, dT_g = o.P["T"].max() - T0, a["T"].max() - T0
+    live_o, live_g = o.P["blanked"] != 1.0, a["blanked"] != 1.0
+    eps_o, eps_g = o.P["eps_pl"][live_o], a["eps_pl"][live_g]
+    print(f"force error {ferr}, dT {dT_g:.8g} vs {dT_o:.8g}, eps max {eps_g.max():.6g} vs {eps_o.max():.6g}, "
+          f"mean {eps_g.mean():.6g} vs {eps_o.mean():.6g}, blanked {(~live_g).sum()} vs {(~live_o).sum()}")
+    assert np.all(scale > 0) and dT_o > 0.0
+    assert np.all(ferr <= 5e-3), ferr
+    assert abs(dT_g - dT_o) <= 5e-3 * dT_o
+    assert abs(eps_g.max() - eps_o.max()) <= 5e-3 * max(eps_o.max(), 1e-12)
+    assert abs(eps_g.mean() - eps_o.mean()) <= 5e-3 * max(eps_o.mean(), 1e-12)
+    sim.close()
+
+
+# ------------------------------------------------------------------------------- 4M window
+def test_config3_fp64_ten_steps_vs_oracle():
+    case = cases.config3()
+    print(f"config3: {case.N} particles, {len(case.tool.elems)} triangles, 10 steps", flush=True)
+    o = Oracle(case)
+    sim = S.Simulation(case)
+    o.step(10)
+    sim.step(10)
+    sim.ctx.check_error()
+    a, b = sim.state(), o.P
+    for f in ("hash", "idx", "blanked"):
+        assert_equal(f, a[f], b[f])
+    live = b["blanked"] != 1.0
+    errs = {f: normwise(a[f], b[f], live) for f in STATE + DERIVS}
+    print({k: f"{v:.2e}" for k, v in errs.items()})
+    assert all(v <= 1e-6 for v in errs.values()), errs
+    assert np.mean(a["on_seg"][live] == b["on_seg"][live]) >= 0.999
+    tg = sim.tool_state()
+    for f in ("T", "node_pos", "left", "right", "n", "K", "M"):
+        assert_close("tool " + f, tg[f], o.t[f], 1e-6)
+    sim.close()
+
+
+def test_config3_fp32c_vs_fp64_derivatives():
+    case = cases.perturbed(cases.config3())
+    print(f"config3 perturbed: {case.N} particles, FP32C vs FP64, 1 step", flush=True)
+    out = {}
+    for prec in (_lib.FP64, _lib.FP32C):
+        sim = S.Simulation(case, precision=prec)
+        sim.step(1)
+        sim.ctx.check_error()
+        out[prec] = sim.state(("hash", "idx", "blanked") + DERIVS + ("pos", "vel", "S", "rho", "T"))
+        sim.close()
+    a, b = out[_lib.FP32C], out[_lib.FP64]
+    for f in ("hash", "idx", "blanked"):
+        assert_equal(f, a[f], b[f])
+    live = b["blanked"] != 1.0
+    for f in DERIVS:
+        assert np.max(np.abs(b[f][live])) > 0.0, f
+    errs = {f: normwise(a[f], b[f], live) for f in DERIVS}
+    errs_s = {f: normwise(a[f], b[f], live) for f in ("pos", "vel", "S", "rho", "T")}
+    print("derivatives", {k: f"{v:.2e}" for k, v in errs.items()}, "state", {k: f"{v:.2e}" for k, v in errs_s.items()})
+    assert all(v <= 1e-5 for v in errs.values()), errs
+    assert all(v <= 1e-6 for v in errs_s.values()), errs_s
+
+
+# ------------------------------------------------------------------------------- 16M kernel-only
+def test_config4_chip16m_sort_and_interaction_pass():
+    case = cases.chip_case(n_target=16_000_000)
+    N = case.N
+    assert N >= 16_000_000
+    print(f"chip domain: {N} particles, hash + stable sort", flush=True)
+    sim = S.Simulation(case, precision=_lib.FP32C)
+    p, g, ctx = sim.p, sim.grid, sim.ctx
+    geo = g.geom
+    ix = ((case.pos[:, 0] - geo.bbmin_x) / geo.dx).astype(np.int32)
+    iy = ((case.pos[:, 1] - geo.bbmin_y) / geo.dx).astype(np.int32)
+    keys = (ix * geo.ny + iy).astype(np.int32)
+    g.assign_hashes(p)
+    g.sort(p)
+    ctx.check_error()
+    perm = np.argsort(keys, kind="stable").astype(np.int32)
+    assert_equal("idx", p.download("idx"), perm)
+    skeys = keys[perm]
+    assert_equal("sorted hash", p.download("hash"), skeys)
+    del ix, iy, keys
+    nc = g.num_cell
+    cs_d, ce_d = C.c_void_p(), C.c_void_p()
+    ctx.call("sphfem_malloc", C.byref(cs_d), 4 * nc)
+    ctx.call("sphfem_malloc", C.byref(ce_d), 4 * nc)
+    g.get_cells(p, cs_d.value, ce_d.value)
+    S.interactions_setup_geometry_constants(g)
+    S.material_eos(p)
+    S.corrector_artificial_stress(p)
+    outs = ("v_der", "S_der", "vel_t", "pos_t", "T_t")
+    print("one FP64 interaction pass", flush=True)
+    S.interactions_monaghan(p, cs_d.value, ce_d.value, nc)
+    ctx.check_error()
+    r64 = {f: p.download(f) for f in outs}
+    print("one FP32C interaction pass", flush=True)
+    S.interactions_monaghan_fp32c(sim.stepper, p, cs_d.value, ce_d.value, nc)
+    ctx.check_error()
+    r32 = {f: p.download(f) for f in outs}
+    # the oracle on the sorted state the GPU produced (the sort is checked above)
+    o = Oracle(case)
+    for f in ("pos", "vel", "h", "rho", "p", "S", "R", "T", "blanked", "tool_particle", "hash"):
+        o.P[f][:] = p.download(f)
+    cs_h = np.empty(nc, np.int32)
+    ce_h = np.empty(nc, np.int32)
+    ctx.call("sphfem_memcpy_dtoh", cs_h.ctypes.data_as(C.c_void_p), cs_d, 4 * nc)
+    ctx.call("sphfem_memcpy_dtoh", ce_h.ctypes.data_as(C.c_void_p), ce_d, 4 * nc)
+    o.st["cell_start"][:] = cs_h
+    o.st["cell_end"][:] = ce_h
+    ctx.call("sphfem_free", cs_d)
+    ctx.call("sphfem_free", ce_d)
+    W = 60_000
+    for frac in (0.1, 0.45, 0.6, 0.9):            # slab, chip root (10:1 density), chip band
+        i0 = int(frac * N)
+        i1 = i0 + W
+        print(f"oracle window [{i0}, {i1})", flush=True)
+        o.call("o_interactions_monaghan_range", "cps", i0, i1)
+        w = slice(i0, i1)
+        for f in outs:
+            assert_close(f"fp64 {f} [{i0}:{i1}]", r64[f][w], o.P[f][w], 1e-15)
+        acc_o = np.stack([o.P["S_der"][w, 0] + o.P["S_der"][w, 1] + o.P["vel_t"][w, 0],
+                          o.P["S_der"][w, 2] + o.P["S_der"][w, 3] + o.P["vel_t"][w, 1]], axis=1)
+        acc_g = np.stack([r32["S_der"][w, 0] + r32["vel_t"][w, 0], r32["S_der"][w, 2] + r32["vel_t"][w, 1]], axis=1)
+        errs = {"accel": normwise(acc_g, acc_o), "v_der": normwise(r32["v_der"][w], o.P["v_der"][w]),
+                "pos_t": normwise(r32["pos_t"][w], o.P["pos_t"][w])}
+        print("fp32c vs oracle", {k: f"{v:.2e}" for k, v in errs.items()})
+        assert all(v <= 1e-5 for v in errs.values()), errs
+    sim.close()
+
+
+# ------------------------------------------------------------------------------- 64-case sweep
+def test_config5_batch64_cases_equal_alone_and_oracle():
+    cs = cases.batch_cases(64)
+    total = sum(c.N for c in cs)
+    print(f"batch: 64 cases, {total} particles, sizes {min(c.N for c in cs)}..{max(c.N for c in cs)}", flush=True)
+    batch = S.BatchedSimulation(cs, precision=_lib.FP32C)
+    batch.step(1)
+    batch.capture()
+    batch.launch(2)                      # 5 steps, all cases in flight
+    batch.synchronize()
+    batch.check_error()
+    fields = ("hash", "idx", "blanked", "pos", "vel", "S", "rho", "T", "fc", "ft")
+    for k, c in enumerate(cs):
+        alone = S.Simulation(c, precision=_lib.FP32C)
+        alone.step(5)
+        alone.ctx.check_error()
+        a, b = batch.sims[k].state(fields), alone.state(fields)
+        for f in ("hash", "idx", "blanked"):
+            assert_equal(f"{c.name} {f}", a[f], b[f])
+        for f in fields[3:]:
+            e = normwise(a[f], b[f])
+            assert e <= 1e-11, f"{c.name} {f}: {e:.3e}"
+        alone.close()
+    c = min(cs, key=lambda x: x.N)
+    k = cs.index(c)
+    o = Oracle(c)
+    o.step(5)
+    a = batch.sims[k].state(("hash", "idx", "pos", "vel", "T"))
+    assert_equal("hash", a["hash"], o.P["hash"])
+    live = o.P["blanked"] != 1.0
+    for f in ("pos", "vel", "T"):
+        e = normwise(a[f], o.P[f], live)
+        assert e <= 1e-6, f"{c.name} {f} vs oracle: {e:.3e}"
+    batch.close()
