@@ -14,6 +14,11 @@
 #define SPHFEM_RESORT_CAP 4096
 // blocks of the bounding-box reduction of an unlocked grid
 #define SPHFEM_BBOX_BLOCKS 1024
+// d_flags slot set by the reorder when a live particle carries a non-finite pair input (the pair
+// kernels then use the reference's full 5x5-cell window) and cleared by the update kernel
+#define SPHFEM_FLAG_NONFINITE 5
+// blocks of the grid-stride window form of the pair kernels
+#define SPHFEM_WINDOW_BLOCKS 512
 
 namespace sphfem {
 
@@ -74,7 +79,8 @@ struct sphfem_ctx {
     unsigned set_mask = 0;
     std::string err;
     int *d_flags = nullptr;       // [0]: error bits, [1]: near-list count, [2]: contact count,
-                                  // [3]: re-sort mode (1 full, 2 incremental), [4]: changed count
+                                  // [3]: re-sort mode (1 full, 2 incremental), [4]: changed count,
+                                  // [5]: non-finite pair inputs present (SPHFEM_FLAG_NONFINITE)
     sphfem::Timing timing;
 };
 

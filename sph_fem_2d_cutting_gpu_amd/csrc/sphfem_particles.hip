@@ -36,7 +36,11 @@ __device__ __forceinline__ void column_range(const int *__restrict__ cs, const i
 // are bit-identical); otherwise they are computed per pair as in the reference.
 // LIST_BS > 0: the in-support neighbours go through a per-lane list of CHUNK entries in LDS
 // (list = this lane's first entry, stride LIST_BS) before their terms are evaluated.
-template <bool PRE, int LIST_BS = 0, int CHUNK = 0>
+// WIN: every non-blanked particle of the 5x5 cells is evaluated, as the reference does, instead of
+// the in-support ones only.  The kernel's zeros outside the support make the two the same sums for
+// finite inputs; a neighbour with a non-finite input turns the sums NaN (0 * NaN) in the reference
+// wherever it lies in the window, and only the WIN form reproduces that.
+template <bool PRE, int LIST_BS = 0, int CHUNK = 0, bool WIN = false>
 __device__ __forceinline__ PairSums pair_sums_fp64(const DevConsts &c, const sphfem_particles &P, int pidx,
                                                    double thermal_alpha, const int *__restrict__ cs,
                                                    const int *__restrict__ ce,
@@ -160,7 +164,7 @@ __device__ __forceinline__ PairSums pair_sums_fp64(const DevConsts &c, const sph
                 double xij = pi.x - pj.x;
                 double yij = pi.y - pj.y;
                 double rr2 = xij * xij + yij * yij;
-                if (rr2 >= fourh2 || rr2 < rr2_min) continue;
+                if (!WIN && (rr2 >= fourh2 || rr2 < rr2_min)) continue;
                 pair(iter, pj, xij, yij);
             }
         }
@@ -187,7 +191,7 @@ __device__ __forceinline__ PairSums pair_sums_fp64(const DevConsts &c, const sph
                 double xij = pi.x - pj.x;
                 double yij = pi.y - pj.y;
                 double rr2 = xij * xij + yij * yij;
-                if (rr2 >= fourh2 || rr2 < rr2_min) continue;
+                if (!WIN && (rr2 >= fourh2 || rr2 < rr2_min)) continue;
                 list[n * LIST_BS] = iter;
                 ++n;
                 // a full list empties every active lane's list at once: lanes that flushed at
@@ -225,6 +229,7 @@ __device__ __forceinline__ float4 ldrec(const float4 *__restrict__ base, unsigne
 struct PairI {            // particle i and the per-i kernel constants
     float fxi, fyi, rowi, hi, vxi, vyi, sxi, syi, szi, Rxi, Ryi, Rzi, rhoi, ci, Ti;
     float h1, fac, c_out_w, c_out_g, c_in_g;
+    float fourh2, rr2min;     // the support test (used by the window form)
     bool thermal;
     bool mech;            // i is not a tool particle
 };
@@ -233,6 +238,7 @@ struct PairAcc {          // FP32 block sums
     float v0, v1, v2, v3, ax, ay, p0, p1, T;
 };
 
+template <bool WIN = false>
 __device__ __forceinline__ void pair_term(const PairI &I, float hf, float inv_wdp, float alpha, float beta,
                                           float eta2, const float4 q0, const float4 q1, const float4 q2,
                                           const float4 q3, int e, PairAcc &a) {
@@ -252,6 +258,13 @@ __device__ __forceinline__ void pair_term(const PairI &I, float hf, float inv_wd
     } else {
         w = I.fac * (1.f - 1.5f * q * q * (1.f - 0.5f * q));
         g = I.c_in_g * (1.f - 0.75f * q);
+    }
+    if constexpr (WIN) {
+        // a window candidate outside the support: the reference's kernel value and gradient are 0
+        if (!(rr2 < I.fourh2 && rr2 >= I.rr2min)) {
+            w = 0.f;
+            g = 0.f;
+        }
     }
     const float wx = g * xij, wy = g * yij;
     const float volj = fabsf(q3.x);
@@ -303,16 +316,15 @@ __device__ __forceinline__ void pair_term(const PairI &I, float hf, float inv_wd
 //    rounding on output.
 // Outputs (FP32): psum[i] = velocity gradient (xx, xy, yx, yy); psum[N + i] = (mass * stress
 // divergence - mass * viscous term) per axis, XSPH velocity per axis; psum_T[i] = Laplacian of T.
-template <int BS, int CHUNK>
-__global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *__restrict__ blanked,
-                                                         const int *__restrict__ hash, int N, RecView rv,
-                                                         const int *__restrict__ cs, const int *__restrict__ ce,
-                                                         float4 *__restrict__ psum, float *__restrict__ psum_T) {
+// WIN: every non-blanked record of the 5x5 cells (blanked records sit at x = 1e30) is evaluated with
+// the kernel set to 0 outside the support, as the reference does; see pair_sums_fp64.
+template <int BS, int CHUNK, bool WIN>
+__device__ __forceinline__ void pairs_fp32c_one(const DevConsts &c, const double *__restrict__ blanked,
+                                                const int *__restrict__ hash, int N, const RecView &rv,
+                                                const int *__restrict__ cs, const int *__restrict__ ce,
+                                                float4 *__restrict__ psum, float *__restrict__ psum_T, int i,
+                                                int *__restrict__ list) {
 #pragma clang fp contract(fast)
-    __shared__ int nb_list[CHUNK * BS];
-    const int i = xcd_logical_block() * BS + threadIdx.x;
-    if (i >= N) return;
-    int *__restrict__ list = nb_list + threadIdx.x;
     double d_v0 = 0., d_v1 = 0., d_v2 = 0., d_v3 = 0.;
     double d_ax = 0., d_ay = 0., d_p0 = 0., d_p1 = 0., d_T = 0.;
     const float4 a3 = rv.r3[i];
@@ -355,6 +367,8 @@ __global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *_
         I.mech = !tool_i;
         const float fourh2 = 4.f * I.hi * I.hi;
         const float rr2min = (float)c.rr2_min;
+        I.fourh2 = fourh2;
+        I.rr2min = rr2min;
         const float hf = (float)c.geom.dx;
         const float inv_wdp = (float)(1. / c.corr.wdeltap);
         const float alpha = (float)c.corr.alpha, beta = (float)c.corr.beta;
@@ -378,12 +392,12 @@ __global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *_
                 const unsigned oB = (unsigned)(eB & 0x0fffffff) << 4;
                 const float4 B0 = ldrec(rv.r0, oB), B1 = ldrec(rv.r1, oB), B2 = ldrec(rv.r2, oB),
                              B3 = ldrec(rv.r3, oB);
-                pair_term(I, hf, inv_wdp, alpha, beta, eta2, A0, A1, A2, A3, eA, acc);
+                pair_term<WIN>(I, hf, inv_wdp, alpha, beta, eta2, A0, A1, A2, A3, eA, acc);
                 if (k + 1 < n) {
                     eA = list[(k + 2 < n ? k + 2 : n - 1) * BS];
                     oA = (unsigned)(eA & 0x0fffffff) << 4;
                     A0 = ldrec(rv.r0, oA); A1 = ldrec(rv.r1, oA); A2 = ldrec(rv.r2, oA); A3 = ldrec(rv.r3, oA);
-                    pair_term(I, hf, inv_wdp, alpha, beta, eta2, B0, B1, B2, B3, eB, acc);
+                    pair_term<WIN>(I, hf, inv_wdp, alpha, beta, eta2, B0, B1, B2, B3, eB, acc);
                 }
                 if (((k + 2) & 7) == 0) flush(acc);
             }
@@ -416,7 +430,8 @@ __global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *_
                     const float xij = (I.fxi - bb[u].x) + cofs;
                     const float yij = (I.fyi - bb[u].y) + (I.rowi - bb[u].z) * hf;
                     const float rr2 = xij * xij + yij * yij;
-                    if (j + u < last && rr2 < fourh2 && rr2 >= rr2min) {
+                    const bool take = WIN ? rr2 < 1e30f : (rr2 < fourh2 && rr2 >= rr2min);
+                    if (j + u < last && take) {
                         list[n * BS] = (j + u) | tag;
                         n++;
                     }
@@ -438,6 +453,34 @@ __global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *_
     psum_T[i] = (float)(2. * d_T);
 }
 
+// in-support pair sums, one lane per sorted particle; nothing when nonfinite[0] is set
+template <int BS, int CHUNK>
+__global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *__restrict__ blanked,
+                                                         const int *__restrict__ hash, int N, RecView rv,
+                                                         const int *__restrict__ cs, const int *__restrict__ ce,
+                                                         float4 *__restrict__ psum, float *__restrict__ psum_T,
+                                                         const int *__restrict__ nonfinite) {
+    __shared__ int nb_list[CHUNK * BS];
+    const int i = xcd_logical_block() * BS + threadIdx.x;
+    if (i >= N) return;
+    if (*nonfinite != 0) return;
+    pairs_fp32c_one<BS, CHUNK, false>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, nb_list + threadIdx.x);
+}
+
+// the window form over every particle (grid-stride loop) when nonfinite[0] is set, else nothing
+template <int BS, int CHUNK>
+__global__ void __launch_bounds__(BS) k_pairs_fp32c_window(DevConsts c, const double *__restrict__ blanked,
+                                                                const int *__restrict__ hash, int N, RecView rv,
+                                                                const int *__restrict__ cs,
+                                                                const int *__restrict__ ce, float4 *__restrict__ psum,
+                                                                float *__restrict__ psum_T,
+                                                                const int *__restrict__ nonfinite) {
+    __shared__ int nb_list[CHUNK * BS];
+    if (*nonfinite == 0) return;
+    for (int i = (int)blockIdx.x * BS + (int)threadIdx.x; i < N; i += (int)gridDim.x * BS)
+        pairs_fp32c_one<BS, CHUNK, true>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, nb_list + threadIdx.x);
+}
+
 __device__ __forceinline__ double interaction_alpha(const DevConsts &c, double is_tool) {
     // interactions_gpu.cu:164: the tool block (interactions_gpu.cu:19) is zero unless
     // interactions_setup_thermal_constants_tool set it (the rigid-tool setup does, benchmarks.cpp:518;
@@ -445,7 +488,8 @@ __device__ __forceinline__ double interaction_alpha(const DevConsts &c, double i
     return (is_tool == 0.) ? c.trml_wp.alpha : c.trml_itool.alpha;
 }
 
-// interactions_gpu.cu:143-379 as one kernel (drop-in entry point)
+// interactions_gpu.cu:143-379 as one kernel (drop-in entry point): every particle of the 5x5 cells,
+// as the reference visits them
 __global__ void __launch_bounds__(256) k_interactions_ref(DevConsts c, sphfem_particles P, const int *__restrict__ cs,
                                                           const int *__restrict__ ce) {
     __shared__ int nb_list[48 * 256];
@@ -453,7 +497,7 @@ __global__ void __launch_bounds__(256) k_interactions_ref(DevConsts c, sphfem_pa
     if (pidx >= (int)P.N) return;
     if (P.blanked[pidx] == 1.) return;
     double thermal_alpha = interaction_alpha(c, P.tool_particle[pidx]);
-    PairSums r = pair_sums_fp64<false, 256, 48>(c, P, pidx, thermal_alpha, cs, ce, nullptr, nullptr,
+    PairSums r = pair_sums_fp64<false, 256, 48, true>(c, P, pidx, thermal_alpha, cs, ce, nullptr, nullptr,
                                                 nb_list + threadIdx.x);
     P.S_der[pidx] = r.S_der;
     P.v_der[pidx] = r.v_der;
@@ -648,19 +692,25 @@ void launch_leapfrog(sphfem_ctx *ctx, int which, const sphfem_particles &p, sphf
 // FP64 pair sums of the sorted predicted state (interactions_gpu.cu:143-379) into psum64, for the
 // update kernel; a kernel of its own so that the pair loop keeps 3 waves per SIMD instead of the
 // update's 2
-template <int BS>
+// nonfinite[0] is set by the reorder when a live particle carries a non-finite pair input: then the
+// in-support kernel does nothing and the window kernel (a grid-stride loop, which returns at once
+// otherwise) evaluates every particle with the reference's full window.
+template <int BS, bool WIN>
 __global__ void __launch_bounds__(BS) k_pairs_fp64(DevConsts c, sphfem_particles B, const int *__restrict__ cs,
                                                    const int *__restrict__ ce, const double *__restrict__ pre_vol,
                                                    const sphfem_double4 *__restrict__ pre_s,
-                                                   PairSums *__restrict__ psum64) {
+                                                   PairSums *__restrict__ psum64, const int *__restrict__ nonfinite) {
     constexpr int CHUNK = 48;
     __shared__ int nb_list[CHUNK * BS];
-    const int i = xcd_logical_block() * BS + threadIdx.x;
-    if (i >= (int)B.N) return;
-    if (B.blanked[i] == 1.) return;
-    const double thermal_alpha = interaction_alpha(c, B.tool_particle[i]);
-    psum64[i] = pair_sums_fp64<true, BS, CHUNK>(c, B, i, thermal_alpha, cs, ce, pre_vol, pre_s,
-                                                nb_list + threadIdx.x);
+    if ((*nonfinite != 0) != WIN) return;
+    const int N = (int)B.N;
+    const int stride = WIN ? (int)gridDim.x * BS : N;
+    for (int i = (WIN ? (int)blockIdx.x : xcd_logical_block()) * BS + (int)threadIdx.x; i < N; i += stride) {
+        if (B.blanked[i] == 1.) continue;
+        const double thermal_alpha = interaction_alpha(c, B.tool_particle[i]);
+        psum64[i] = pair_sums_fp64<true, BS, CHUNK, WIN>(c, B, i, thermal_alpha, cs, ce, pre_vol, pre_s,
+                                                         nb_list + threadIdx.x);
+    }
 }
 
 // RIGID: the rigid-tool step (leap_frog.cu:99-160): friction heat after the corrector
@@ -678,8 +728,10 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
                                                           const float4 *__restrict__ psum,
                                                           const float *__restrict__ psum_T,
                                                           const PairSums *__restrict__ psum64,
-                                                          const RigidTool *__restrict__ rt) {
+                                                          const RigidTool *__restrict__ rt,
+                                                          int *__restrict__ nonfinite) {
     const int i = xcd_logical_block() * BS + threadIdx.x;
+    if (i == 0) *nonfinite = 0;      // the pair kernels of this step have read it
     if (i >= (int)B.N) return;
     // FP32C: the reorder wrote only the neighbour records; every per-particle field is gathered here
     // from the unsorted arrays A through idx, and the sorted step result is written to B
@@ -898,9 +950,10 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
 // vel_t.y); XSPH in pos_t; alpha times the Laplacian in T_t where the thermal block is set.
 // Blanked particles are left untouched, as by the reference.
 __global__ void __launch_bounds__(256) k_psum_expand(DevConsts c, sphfem_particles P, const float4 *__restrict__ psum,
-                                                     const float *__restrict__ psum_T) {
+                                                     const float *__restrict__ psum_T, int *__restrict__ nonfinite) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int N = (int)P.N;
+    if (i == 0) *nonfinite = 0;
     if (i >= N) return;
     if (P.blanked[i] == 1.) return;
     const double alpha = interaction_alpha(c, P.tool_particle[i]);
@@ -918,19 +971,27 @@ static void launch_pairs_fp32c(sphfem_ctx *ctx, const sphfem_particles &B, sphfe
     const size_t n = B.N;
     RecView rv{lf->rec, lf->rec + n + 4, lf->rec + 2 * n + 4, lf->rec + 3 * n + 4};
     k_pairs_fp32c<BSZ, CH><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(ctx->c, B.blanked, B.hash, (int)B.N, rv, cs, ce,
-                                                                    lf->psum, lf->psum_T);
+                                                                    lf->psum, lf->psum_T,
+                                                                    ctx->d_flags + SPHFEM_FLAG_NONFINITE);
 }
 
 static void launch_pairs_fp32c_tuned(sphfem_ctx *ctx, const sphfem_particles &B, sphfem_leap_frog *lf, const int *cs,
                                      const int *ce) {
-    KTimer tm(ctx, "pairs_fp32c");
-    const int pb = lf->tuning.pair_block, pc = lf->tuning.pair_chunk;
-    if (pb == 128 && pc == 16) launch_pairs_fp32c<128, 16>(ctx, B, lf, cs, ce);
-    else if (pb == 128 && pc == 32) launch_pairs_fp32c<128, 32>(ctx, B, lf, cs, ce);
-    else if (pb == 128 && pc == 48) launch_pairs_fp32c<128, 48>(ctx, B, lf, cs, ce);
-    else if (pb == 256 && pc == 16) launch_pairs_fp32c<256, 16>(ctx, B, lf, cs, ce);
-    else if (pb == 256 && pc == 48) launch_pairs_fp32c<256, 48>(ctx, B, lf, cs, ce);
-    else launch_pairs_fp32c<256, 32>(ctx, B, lf, cs, ce);
+    {
+        KTimer tm(ctx, "pairs_fp32c");
+        const int pb = lf->tuning.pair_block, pc = lf->tuning.pair_chunk;
+        if (pb == 128 && pc == 16) launch_pairs_fp32c<128, 16>(ctx, B, lf, cs, ce);
+        else if (pb == 128 && pc == 32) launch_pairs_fp32c<128, 32>(ctx, B, lf, cs, ce);
+        else if (pb == 128 && pc == 48) launch_pairs_fp32c<128, 48>(ctx, B, lf, cs, ce);
+        else if (pb == 256 && pc == 16) launch_pairs_fp32c<256, 16>(ctx, B, lf, cs, ce);
+        else if (pb == 256 && pc == 48) launch_pairs_fp32c<256, 48>(ctx, B, lf, cs, ce);
+        else launch_pairs_fp32c<256, 32>(ctx, B, lf, cs, ce);
+    }
+    KTimer tw(ctx, "pairs_fp32c_window");
+    const size_t n = B.N;
+    RecView rv{lf->rec, lf->rec + n + 4, lf->rec + 2 * n + 4, lf->rec + 3 * n + 4};
+    k_pairs_fp32c_window<256, 32><<<SPHFEM_WINDOW_BLOCKS, 256, 0, ctx->stream>>>(
+        ctx->c, B.blanked, B.hash, (int)B.N, rv, cs, ce, lf->psum, lf->psum_T, ctx->d_flags + SPHFEM_FLAG_NONFINITE);
 }
 
 void launch_interactions_fp32c(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_leap_frog *lf, const int *cs,
@@ -938,7 +999,8 @@ void launch_interactions_fp32c(sphfem_ctx *ctx, const sphfem_particles &p, sphfe
     launch_records_state(ctx, p, num_cell, lf->rec);
     launch_pairs_fp32c_tuned(ctx, p, lf, cs, ce);
     KTimer t(ctx, "psum_expand");
-    k_psum_expand<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, lf->psum, lf->psum_T);
+    k_psum_expand<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, lf->psum, lf->psum_T,
+                                                           ctx->d_flags + SPHFEM_FLAG_NONFINITE);
 }
 
 void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfem_particles &A, sphfem_leap_frog *lf,
@@ -963,7 +1025,7 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
 #define SPHFEM_UPDATE(BSZ, RG)                                                                              \
     k_fused_update<true, BSZ, RG><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(                                  \
         ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi, lf->psum, \
-        lf->psum_T, nullptr, rtd)
+        lf->psum_T, nullptr, rtd, ctx->d_flags + SPHFEM_FLAG_NONFINITE)
         if (lf->tuning.update_block == 128) {
             if (rtd) SPHFEM_UPDATE(128, true);
             else SPHFEM_UPDATE(128, false);
@@ -975,18 +1037,25 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
     } else {
         {
             KTimer tp(ctx, "pairs_fp64");
+            const int *nf = ctx->d_flags + SPHFEM_FLAG_NONFINITE;
             if (lf->tuning.pair_block == 128)
-                k_pairs_fp64<128><<<nblk(B.N, 128), 128, 0, ctx->stream>>>(ctx->c, B, cs, ce, lf->pre_vol, lf->pre_s,
-                                                                            lf->psum64);
+                k_pairs_fp64<128, false><<<nblk(B.N, 128), 128, 0, ctx->stream>>>(ctx->c, B, cs, ce, lf->pre_vol,
+                                                                                   lf->pre_s, lf->psum64, nf);
             else
-                k_pairs_fp64<256><<<nblk(B.N, 256), 256, 0, ctx->stream>>>(ctx->c, B, cs, ce, lf->pre_vol, lf->pre_s,
-                                                                            lf->psum64);
+                k_pairs_fp64<256, false><<<nblk(B.N, 256), 256, 0, ctx->stream>>>(ctx->c, B, cs, ce, lf->pre_vol,
+                                                                                   lf->pre_s, lf->psum64, nf);
+        }
+        {
+            KTimer tp(ctx, "pairs_fp64_window");
+            k_pairs_fp64<256, true><<<SPHFEM_WINDOW_BLOCKS, 256, 0, ctx->stream>>>(ctx->c, B, cs, ce, lf->pre_vol,
+                                                                                  lf->pre_s, lf->psum64,
+                                                                                  ctx->d_flags + SPHFEM_FLAG_NONFINITE);
         }
         KTimer tm(ctx, "fused_update");
 #define SPHFEM_UPDATE(BSZ, RG)                                                                              \
     k_fused_update<false, BSZ, RG><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(                                 \
         ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi, nullptr, \
-        nullptr, lf->psum64, rtd)
+        nullptr, lf->psum64, rtd, ctx->d_flags + SPHFEM_FLAG_NONFINITE)
         if (lf->tuning.update_block == 128) {
             if (rtd) SPHFEM_UPDATE(128, true);
             else SPHFEM_UPDATE(128, false);

@@ -740,6 +740,18 @@ void launch_rothlin_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p) {
     launch_cells_dense(ctx, p.hash, (int)p.N, g->num_cell, g->cell_start, g->cell_end);
 }
 
+// a live particle with a non-finite pair input (interactions_gpu.cu:143-379 reads pos, vel, h, rho, p,
+// S, R and T of both particles): the pair kernels then evaluate the reference's full 5x5-cell window,
+// where such a neighbour turns the sums NaN even outside the kernel support.  A finite sum that
+// overflows only sends the step down that (exact, slower) path.
+__device__ __forceinline__ void flag_nonfinite(int *__restrict__ flags, double blanked, sphfem_double2 pos,
+                                               sphfem_double2 vel, sphfem_double4 S, double rho, double T, double h,
+                                               double pr, sphfem_double4 R) {
+    const double s = ((pos.x + pos.y) + (vel.x + vel.y)) + ((S.x + S.y) + (S.z + rho)) + ((T + h) + (pr + R.x)) +
+                     (R.y + R.z);
+    if (blanked != 1. && !isfinite(s)) atomicOr(&flags[SPHFEM_FLAG_NONFINITE], 1);
+}
+
 // FP32C neighbour records of one sorted particle: positions relative to the cell corner keep ~1e-7
 // of the cell size; the stress terms are the per-particle factors of the pair sum, (S - p)/rho^2 and
 // S_xy/rho^2; four SoA blocks; the first has 4 padding records that absorb four-wide scan loads
@@ -793,7 +805,11 @@ __global__ void __launch_bounds__(256) k_records_state(DevConsts c, const sphfem
     const double tool = p.tool_particle[i];
     const double pr = tool == 0. ? p.p[i] : 0.;
     const sphfem_double4 R = tool == 0. ? p.R[i] : sphfem_double4{0., 0., 0., 0.};
-    write_records(c, rec, N, i, h, p.pos[i], p.vel[i], p.S[i], p.rho[i], p.T[i], p.blanked[i], tool, p.h[i], pr, R);
+    const sphfem_double2 pos = p.pos[i], vel = p.vel[i];
+    const sphfem_double4 S = p.S[i];
+    const double rho = p.rho[i], T = p.T[i], blanked = p.blanked[i], hh = p.h[i];
+    flag_nonfinite(flags, blanked, pos, vel, S, rho, T, hh, pr, R);
+    write_records(c, rec, N, i, h, pos, vel, S, rho, T, blanked, tool, hh, pr, R);
 }
 
 void launch_records_state(sphfem_ctx *ctx, const sphfem_particles &p, int num_cell, float4 *rec) {
@@ -887,11 +903,14 @@ __global__ void __launch_bounds__(256) k_reorder_predict(DevConsts c, const sphf
         b.R[i] = artificial_stress(c, rho, pr, S);
         const double rho21 = 1. / (rho * rho);
         pre_s[i] = {(S.x - pr) * rho21, S.y * rho21, (S.z - pr) * rho21, sqrtf_d(c.phys.K / rho)};
+        flag_nonfinite(flags, blanked, pos, vel, S, rho, T, p.h[s], pr, b.R[i]);
     } else {
         // neither gathered nor computed for tool particles by the reference: they stay in place
         b.p[i] = p.p[i];
         b.R[i] = p.R[i];
         pre_s[i] = {0., 0., 0., 0.};   // the pair loop skips these terms for tool particles
+        flag_nonfinite(flags, blanked, pos, vel, sphfem_double4{0., 0., 0., 0.}, rho, T, p.h[s], 0.,
+                       sphfem_double4{0., 0., 0., 0.});
     }
 }
 
@@ -934,7 +953,9 @@ __global__ void __launch_bounds__(256) k_reorder_records(DevConsts c, const sphf
         pr = eos_pressure(c, rho);
         R = artificial_stress(c, rho, pr, S);
     }
-    write_records(c, rec, N, i, h, pos, vel, S, rho, T, blanked, tool, p.h[s], pr, R);
+    const double hh = p.h[s];
+    flag_nonfinite(flags, blanked, pos, vel, tool == 0. ? S : sphfem_double4{0., 0., 0., 0.}, rho, T, hh, pr, R);
+    write_records(c, rec, N, i, h, pos, vel, S, rho, T, blanked, tool, hh, pr, R);
 }
 
 void launch_reorder_predict(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, sphfem_leap_frog *lf) {
