@@ -196,7 +196,7 @@ struct sphfem_leap_frog {
     bool intermediates = false;    // also write S_der / v_der in the fused step
     // FP32C neighbour records of the sorted predicted state, SoA blocks r0 [num_part + 4],
     // r1, r2, r3 [num_part]: {x, y relative to the cell corner, cell row, h},
-    // {vx, vy, (Sxx-p)/rho^2, Sxy/rho^2}, {(Syy-p)/rho^2, R}, {+-mass/rho, rho, sqrt(K/rho), T}
+    // {vx, vy, (Sxx-p)/rho^2, Sxy/rho^2}, {(Syy-p)/rho^2, R}, {+-mass/rho, rho, sqrt(K/rho), T - T_init}
     float4 *rec = nullptr;
     // FP32C pair sums of the sorted particles: [0] velocity gradient (xx, xy, yx, yy),
     // [1] stress-divergence + viscous acceleration (x, y), XSPH velocity (x, y); psum_T: Laplacian of T

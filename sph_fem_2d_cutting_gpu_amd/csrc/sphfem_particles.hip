@@ -211,7 +211,7 @@ __device__ __forceinline__ PairSums pair_sums_fp64(const DevConsts &c, const sph
 // FP32C neighbour records (written by the reorder kernel), four SoA blocks of float4:
 //   r0[j] = {x - cell corner x, y - cell corner y, cell row, h}   (block padded by 4 records)
 //   r1[j] = {vx, vy, (Sxx - p)/rho^2, Sxy/rho^2}     r2[j] = {(Syy - p)/rho^2, Rxx, Rxy, Ryy}
-//   r3[j] = {+-mass/rho, rho, sqrt(K/rho), T}
+//   r3[j] = {+-mass/rho, rho, sqrt(K/rho), T - T_init}
 // The volume is stored negative for tool particles; blanked particles have x = 1e30.  SoA keeps a
 // wavefront's gather of one field within few cache lines.
 struct RecView {

@@ -754,7 +754,9 @@ __device__ __forceinline__ void flag_nonfinite(int *__restrict__ flags, double b
 
 // FP32C neighbour records of one sorted particle: positions relative to the cell corner keep ~1e-7
 // of the cell size; the stress terms are the per-particle factors of the pair sum, (S - p)/rho^2 and
-// S_xy/rho^2; four SoA blocks; the first has 4 padding records that absorb four-wide scan loads
+// S_xy/rho^2; the temperature is stored as T - T_init, so that the FP32 differences T_i - T_j of a
+// nearly uniform field keep ~1e-7 of the deviation instead of ~1e-7 of T; four SoA blocks; the
+// first has 4 padding records that absorb four-wide scan loads
 __device__ __forceinline__ void write_records(const DevConsts &c, float4 *__restrict__ rec, int N, int i, int h,
                                              sphfem_double2 pos, sphfem_double2 vel, sphfem_double4 S, double rho,
                                              double T, double blanked, double tool, double hh, double pr,
@@ -781,7 +783,7 @@ __device__ __forceinline__ void write_records(const DevConsts &c, float4 *__rest
     r3.x = tool == 1. ? -vol : vol;
     r3.y = (float)rho;
     r3.z = (float)sqrtf_d(c.phys.K / rho);
-    r3.w = (float)T;
+    r3.w = (float)(T - c.trml_wp.T_init);    // only differences enter the sums: offset from T_init
     const size_t n4 = (size_t)N + 4;
     rec[i] = r0;
     rec[n4 + i] = r1;
