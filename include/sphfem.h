@@ -348,6 +348,13 @@ int sphfem_leap_frog_graph_destroy(sphfem_ctx *ctx, sphfem_graph *graph);
 
 /* ---- observables (output.cu:22-30) */
 int sphfem_report_force(sphfem_ctx *ctx, sphfem_particles *p, sphfem_double2 *out);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops): the sums of sphfem_report_force, bit for bit, written
+ * asynchronously on the context's stream to `out`, a device pointer to 2 doubles, without
+ * allocating anything.  The block partials live in a fixed device array of 256 x 2 doubles per GPU
+ * (4 KiB, held until the process ends), shared by every call on that GPU: calls that overlap on
+ * one GPU (two contexts on different streams) race.  N above 2^30 is rejected (SPHFEM_ERR_ARG). */
+int sphfem_report_force_noalloc(sphfem_ctx *ctx, const sphfem_particles *p, sphfem_double2 *out);
 
 /* ---- output formats (vtk_writer.cu, tool_wear.cu, mesh_structure_GPU.cu:1259-1296).
  * Legacy ASCII VTK with the reference's headers, field names, order, scaling and printf

@@ -198,6 +198,7 @@ _SIGS = {
     "sphfem_tool_wear_eval_usui": (_ctx, _vp, _P(Particles), _dbl, _P(_dbl), _P(_dbl), _P(_dbl)),
     "sphfem_tool_wear_get_accum_wear": (_vp, _P(_dbl)),
     "sphfem_report_force": (_ctx, _P(Particles), _P(Double2)),
+    "sphfem_report_force_noalloc": (_ctx, _P(Particles), _vp),
     "sphfem_mesh_read_inp": (C.c_char_p, C.c_int, C.c_int, C.c_int, _P(_vp)),
     "sphfem_mesh_destroy": (_vp,),
     "sphfem_mesh_get_info": (_vp, _P(MeshInfo)),

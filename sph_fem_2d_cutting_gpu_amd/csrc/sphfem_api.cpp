@@ -1,8 +1,11 @@
 // C ABI of libsphfem_hip: context, constants, memory, and the per-step orchestration.
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <utility>
 
@@ -174,6 +177,31 @@ static void timing_collect(sphfem_ctx *ctx) {
     }
 }
 
+// Block partials of sphfem_report_force_noalloc: one (x, y) pair per reduction block, in a
+// file-scope device array instead of memory allocated per call.  Each GPU holds its own copy until
+// the process ends.  Host code never passes the array's name to a kernel (on the host it is a
+// stand-in, not a device address): it uses the address hipGetSymbolAddress returns for the current
+// device, looked up once per device when a context is created there.
+static __device__ __align__(256) double scratch[2 * SPHFEM_FORCE_BLOCKS];
+
+static std::mutex g_scratch_mu;
+static std::map<int, double *> g_scratch_ptr;   // device -> address of `scratch` on that device
+
+// with ctx->device current
+static int lookup_scratch(sphfem_ctx *ctx) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(ctx->device);
+    if (it == g_scratch_ptr.end()) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(ctx, SPHFEM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        it = g_scratch_ptr.emplace(ctx->device, static_cast<double *>(ptr)).first;
+    }
+    ctx->force_scratch = it->second;
+    return SPHFEM_OK;
+}
+
 extern "C" {
 
 int sphfem_create(int device, sphfem_ctx **out) {
@@ -186,7 +214,7 @@ int sphfem_create(int device, sphfem_ctx **out) {
         ctx->d_flags = dalloc<int>(8);
         SPHFEM_HIP_CHECK(hipMemsetAsync(ctx->d_flags, 0, 8 * sizeof(int), ctx->stream));
         ctx->c.rr2_min = 1e-8;
-        return SPHFEM_OK;
+        return lookup_scratch(ctx);
     });
     if (rc != SPHFEM_OK) {
         delete ctx;
@@ -1955,11 +1983,22 @@ int sphfem_tool_wear_get_accum_wear(const sphfem_tool_wear *w, double *accum) {
 int sphfem_report_force(sphfem_ctx *ctx, sphfem_particles *p, sphfem_double2 *out) {
     if (!p || !out) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {
-        double *d = dalloc<double>(2 + 2 * 256);
-        launch_report_force(ctx, *p, d);
+        double *d = dalloc<double>(2 + 2 * SPHFEM_FORCE_BLOCKS);
+        launch_report_force(ctx, *p, d + 2, d);
         SPHFEM_HIP_CHECK(hipMemcpyAsync(out, d, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         dfree(d);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_report_force_noalloc(sphfem_ctx *ctx, const sphfem_particles *p, sphfem_double2 *out) {
+    if (!ctx || !p || !out) return SPHFEM_ERR_ARG;
+    if (p->N > SPHFEM_FORCE_MAX_N)
+        return fail(ctx, SPHFEM_ERR_ARG, "report_force_noalloc: more than 2^30 particles (32-bit reduction indices)");
+    if (!ctx->force_scratch) return fail(ctx, SPHFEM_ERR_STATE, "report_force_noalloc: no scratch address");
+    return guarded(ctx, [&] {
+        launch_report_force(ctx, *p, ctx->force_scratch, reinterpret_cast<double *>(out));
         return SPHFEM_OK;
     });
 }

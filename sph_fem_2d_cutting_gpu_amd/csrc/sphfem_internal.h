@@ -19,6 +19,10 @@
 #define SPHFEM_FLAG_NONFINITE 5
 // blocks of the grid-stride window form of the pair kernels
 #define SPHFEM_WINDOW_BLOCKS 512
+// blocks of the report_force reduction (its partials: 2 doubles per block) and the largest particle
+// count of the allocation-free variant
+#define SPHFEM_FORCE_BLOCKS 256
+#define SPHFEM_FORCE_MAX_N (1u << 30)
 
 namespace sphfem {
 
@@ -81,6 +85,7 @@ struct sphfem_ctx {
     int *d_flags = nullptr;       // [0]: error bits, [1]: near-list count, [2]: contact count,
                                   // [3]: re-sort mode (1 full, 2 incremental), [4]: changed count,
                                   // [5]: non-finite pair inputs present (SPHFEM_FLAG_NONFINITE)
+    double *force_scratch = nullptr;  // this device's `scratch` array (sphfem_api.cpp)
     sphfem::Timing timing;
 };
 
@@ -255,7 +260,8 @@ void launch_interactions_fp32c(sphfem_ctx *ctx, const sphfem_particles &p, sphfe
 void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfem_particles &A,
                          sphfem_leap_frog *lf, const int *cs, const int *ce, sphfem_tool *t,
                          sphfem_rigid_tool *rt = nullptr);
-void launch_report_force(sphfem_ctx *ctx, const sphfem_particles &p, double *d_out);
+// part: 2 * SPHFEM_FORCE_BLOCKS doubles of block partials; out: the 2 sums
+void launch_report_force(sphfem_ctx *ctx, const sphfem_particles &p, double *part, double *out);
 
 // tool (sphfem_tool.hip)
 void launch_fric_heat(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t);

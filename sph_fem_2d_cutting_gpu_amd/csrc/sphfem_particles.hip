@@ -1105,11 +1105,10 @@ __global__ void k_force_final(const double *part, int n, double *out) {
     out[1] = y;
 }
 
-void launch_report_force(sphfem_ctx *ctx, const sphfem_particles &p, double *d_out) {
-    const int blocks = 256;
+void launch_report_force(sphfem_ctx *ctx, const sphfem_particles &p, double *part, double *out) {
     KTimer t(ctx, "report_force");
-    k_force_partial<<<blocks, 256, 0, ctx->stream>>>(p, d_out + 2);
-    k_force_final<<<1, 64, 0, ctx->stream>>>(d_out + 2, blocks, d_out);
+    k_force_partial<<<SPHFEM_FORCE_BLOCKS, 256, 0, ctx->stream>>>(p, part);
+    k_force_final<<<1, 64, 0, ctx->stream>>>(part, SPHFEM_FORCE_BLOCKS, out);
 }
 
 }  // namespace sphfem
