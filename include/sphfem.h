@@ -335,7 +335,8 @@ int sphfem_leap_frog_step_rigid(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_pa
  * replayed with sphfem_leap_frog_graph_launch (each launch advances `steps` steps).  Capture
  * records only (the state does not advance) and needs a locked grid and, for a Green grid, one
  * eager step first; the graph bakes in
- * the constants, dt, wear switch and precision at capture.  No counterpart in the reference
+ * the constants, dt, wear switch, precision and tuning at capture, and a launch after any of them
+ * changed (or after the tool was remeshed onto a new mesh) is refused with SPHFEM_ERR_STATE.  No counterpart in the reference
  * (its step is a sequence of launches with host synchronisations). */
 typedef struct sphfem_graph sphfem_graph;
 int sphfem_leap_frog_graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g,

@@ -28,7 +28,10 @@
 #ifndef SPHFEM_HPP
 #define SPHFEM_HPP
 
+#include <sys/stat.h>
+
 #include <cstdio>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -727,10 +730,29 @@ inline void remesh_low_level(tool_FEM *t, int freq) {
     SPHFEM_SHIM_CALL(sphfem_tool_remesh_low_level(sphfem_shim::ctx(), t->handle(), freq), "remesh_low_level");
 }
 
-// actions_gpu.cu:1033-1038 with the meshing done outside: the geometry goes to geo_file, and the
-// tool takes the mesh Gmsh made of it (inp_file)
-inline void remesh_gmsh(tool_FEM *t, const std::string &geo_file, const std::string &inp_file) {
+namespace sphfem_shim {
+// true if `newer` was modified after `older` (both must exist)
+inline bool file_is_newer(const std::string &newer, const std::string &older) {
+    struct stat a {}, b {};
+    if (stat(newer.c_str(), &a) != 0 || stat(older.c_str(), &b) != 0) return false;
+    if (a.st_mtim.tv_sec != b.st_mtim.tv_sec) return a.st_mtim.tv_sec > b.st_mtim.tv_sec;
+    return a.st_mtim.tv_nsec > b.st_mtim.tv_nsec;
+}
+}  // namespace sphfem_shim
+
+// actions_gpu.cu:1033-1038 with the meshing step supplied by the caller (the reference calls the Gmsh
+// library between the two halves): the geometry of the current cutting edge goes to geo_file,
+// mesh(geo_file, inp_file) must write the new mesh to inp_file, and the tool takes that mesh.  An
+// inp_file that is missing or not newer than the geo_file just written is refused (it would be an
+// older mesh that undoes the wear).
+// (A template over the tool type only so that the host logic can be exercised with a stand-in tool.)
+template <class Tool>
+inline void remesh_gmsh(Tool *t, const std::string &geo_file, const std::string &inp_file,
+                        const std::function<void(const std::string &, const std::string &)> &mesh) {
     t->remesh_read_segments(geo_file);
+    if (mesh) mesh(geo_file, inp_file);
+    if (!sphfem_shim::file_is_newer(inp_file, geo_file))
+        throw std::runtime_error("remesh_gmsh: " + inp_file + " is missing or older than " + geo_file);
     t->read_remesh_gmsh(inp_file);
 }
 

@@ -18,6 +18,11 @@ namespace {
 
 enum : unsigned { SET_PHYS = 1, SET_CORR = 2, SET_JOCO = 4, SET_TRML_WP = 8, SET_TOOLC = 16, SET_DT = 32 };
 
+// an input out of the range an entry point can handle (reported as SPHFEM_ERR_RANGE)
+struct RangeError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
 template <class F>
 int guarded(sphfem_ctx *ctx, F &&f) {
     if (!ctx) return SPHFEM_ERR_ARG;
@@ -32,6 +37,9 @@ int guarded(sphfem_ctx *ctx, F &&f) {
             }
         }
         return rc;
+    } catch (const RangeError &e) {
+        ctx->err = e.what();
+        return SPHFEM_ERR_RANGE;
     } catch (const std::exception &e) {
         ctx->err = e.what();
         return SPHFEM_ERR_HIP;
@@ -1169,6 +1177,28 @@ static ElemBuckets build_elem_buckets(sphfem_ctx *ctx, sphfem_tool *t) {
     std::vector<sphfem_double2> pos((size_t)nn);
     SPHFEM_HIP_CHECK(hipMemcpyAsync(pos.data(), t->pos, 16ul * nn, hipMemcpyDeviceToHost, ctx->stream));
     SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (const auto &p : pos)
+        if (!std::isfinite(p.x) || !std::isfinite(p.y)) throw RangeError("a tool node position is not finite");
+    ElemBuckets bk;
+    if (ne == 0) {                      // no elements: one empty bin
+        if (t->bk_ptr_cap < 2) {
+            dfree(t->bk_ptr);
+            t->bk_ptr = dalloc<int>(2);
+            t->bk_ptr_cap = 2;
+        }
+        if (t->bk_list_cap < 1) {
+            dfree(t->bk_list);
+            t->bk_list = dalloc<int>(1);
+            t->bk_list_cap = 1;
+        }
+        const int zero[2] = {0, 0};
+        SPHFEM_HIP_CHECK(hipMemcpyAsync(t->bk_ptr, zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
+        SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        bk.nx = bk.ny = 1;
+        bk.ptr = t->bk_ptr;
+        bk.list = t->bk_list;
+        return bk;
+    }
     double ext = 0.;
     {
         double x0 = DBL_MAX, y0 = DBL_MAX, x1 = -DBL_MAX, y1 = -DBL_MAX;
@@ -1196,7 +1226,6 @@ static ElemBuckets build_elem_buckets(sphfem_ctx *ctx, sphfem_tool *t) {
     std::vector<double> sorted(diag);
     std::nth_element(sorted.begin(), sorted.begin() + ne / 2, sorted.end());
     double h = std::max(2. * sorted[(size_t)ne / 2], 1e-12 * std::max(ext, 1e-300));
-    ElemBuckets bk;
     auto cell = [&](double v, double o) { return (int)std::floor((v - o) * bk.inv); };
     long total = 0;
     for (int tries = 0;; tries++) {
@@ -1666,7 +1695,22 @@ struct sphfem_graph {
     int steps = 0;
     sphfem_tool *tool = nullptr;       // the FEM tool captured and its array generation then
     unsigned long tool_gen = 0;
+    // what the recorded launches baked in: the constant blocks (passed by value: dt, wear switch,
+    // cut-off, geometry, ...) and the stepper's precision, tuning and intermediates switch
+    DevConsts consts{};
+    sphfem_leap_frog *lf = nullptr;
+    int precision = 0;
+    sphfem_tuning tuning{};
+    bool intermediates = false;
 };
+
+// the settings a graph was captured with still hold (same values, whatever was called since)
+static bool graph_settings_hold(const sphfem_ctx *ctx, const sphfem_graph *gr) {
+    const sphfem_leap_frog *lf = gr->lf;
+    return std::memcmp(&ctx->c, &gr->consts, sizeof(DevConsts)) == 0 && lf->precision == gr->precision &&
+           lf->tuning.pair_block == gr->tuning.pair_block && lf->tuning.pair_chunk == gr->tuning.pair_chunk &&
+           lf->tuning.update_block == gr->tuning.update_block && lf->intermediates == gr->intermediates;
+}
 
 extern "C" {
 
@@ -1693,6 +1737,11 @@ static int graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles 
             for (int k = 0; k < steps; k++) (void)step_body(ctx, lf, p, g, t, rt);   // locked: cannot fail
             SPHFEM_HIP_CHECK(hipStreamEndCapture(ctx->stream, &gr->graph));
             SPHFEM_HIP_CHECK(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0));
+            gr->consts = ctx->c;
+            gr->lf = lf;
+            gr->precision = lf->precision;
+            gr->tuning = lf->tuning;
+            gr->intermediates = lf->intermediates;
         } catch (...) {
             hipGraph_t dead = nullptr;
             (void)hipStreamEndCapture(ctx->stream, &dead);
@@ -1723,6 +1772,10 @@ int sphfem_leap_frog_graph_launch(sphfem_ctx *ctx, sphfem_graph *gr, int repeats
     return guarded(ctx, [&] {
         if (gr->tool && gr->tool->generation != gr->tool_gen)
             return fail(ctx, SPHFEM_ERR_STATE, "the tool was remeshed after this graph was captured: capture again");
+        if (!graph_settings_hold(ctx, gr))
+            return fail(ctx, SPHFEM_ERR_STATE,
+                        "constants, dt, wear switch, precision or tuning changed after this graph was captured: "
+                        "capture again");
         for (int r = 0; r < repeats; r++) SPHFEM_HIP_CHECK(hipGraphLaunch(gr->exec, ctx->stream));
         return SPHFEM_OK;
     });

@@ -64,3 +64,32 @@ def test_graph_needs_an_eager_step_and_even_steps():
     g.launch(1)
     sim.ctx.check_error()
     g.destroy()
+
+
+def test_graph_refused_after_settings_change():
+    """A graph bakes in the constants, dt, wear switch, precision and tuning: after any of them changes
+    its launch is refused (the reference driver turns wear on mid-run, mfree_gpu.cpp:306); setting the
+    captured values back makes it valid again, and a new capture follows the new settings."""
+    case = cases.small_cutting_case(nx=24, ny=8, wear=True)
+    sim = S.Simulation(case, wear=False)
+    sim.step(1)
+    g = sim.capture(steps=2)
+    g.launch(1)
+    sim.ctx.call("sphfem_set_wear", 1)
+    with pytest.raises(_lib.SphfemError, match="captured"):
+        g.launch(1)
+    sim.ctx.call("sphfem_set_wear", 0)
+    g.launch(1)
+    sim.ctx.call("sphfem_set_dt", case.dt * 0.5)
+    with pytest.raises(_lib.SphfemError, match="captured"):
+        g.launch(1)
+    sim.ctx.call("sphfem_set_dt", case.dt)
+    sim.set_tuning(update_block=128 if sim.get_tuning()["update_block"] == 256 else 256)
+    with pytest.raises(_lib.SphfemError, match="captured"):
+        g.launch(1)
+    g.destroy()
+    sim.ctx.call("sphfem_set_wear", 1)
+    g2 = sim.capture(steps=2)                          # wear on from here
+    g2.launch(2)
+    sim.ctx.check_error()
+    g2.destroy()
