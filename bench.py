@@ -70,7 +70,21 @@ def make_case(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def cpu_model():
+    """The host CPU's model name (what lscpu reports), from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(case, budget_s):
+    """The FP64 host oracle on the host's cores: a bounded sample of steps with the OpenMP threads the
+    environment gives (the reported value), then one step on one thread."""
     from oracle.oracle import Oracle, lib
     o = Oracle(case)
     threads = lib().o_num_threads()
@@ -81,9 +95,20 @@ def cpu_baseline(case, budget_s):
     t0 = time.perf_counter()
     o.step(n)
     dt = time.perf_counter() - t0
+    lib().o_set_num_threads(1)
+    t0 = time.perf_counter()
+    o.step(1)
+    d1 = time.perf_counter() - t0
+    lib().o_set_num_threads(threads)
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
     return {"value": case.N * n / dt, "unit": "particle-updates/s", "cores": threads, "kind": "port",
             "sample": f"{n} steps (after 1 warm-up step) of the same workload on the FP64 host oracle, "
-                      f"{threads} OpenMP threads"}
+                      f"{threads} OpenMP threads; then 1 step on 1 thread",
+            "single_thread": {"value": case.N / d1, "steps": 1},
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity": affinity}
 
 
 def run_form(case, precision, steps, warmup, breakdown=False, graph=True, tune=True):
