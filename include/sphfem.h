@@ -308,6 +308,15 @@ int sphfem_leap_frog_create(sphfem_ctx *ctx, unsigned int num_part, unsigned int
                             sphfem_leap_frog **out);
 int sphfem_leap_frog_destroy(sphfem_ctx *ctx, sphfem_leap_frog *lf);
 int sphfem_leap_frog_set_precision(sphfem_ctx *ctx, sphfem_leap_frog *lf, int precision);
+/* The FP32C form keeps the derivative carry (pos_t, vel_t, S_t, rho_t, T_t) inside the stepper in
+ * FP32 (the pair sums it comes from are FP32; positions, velocities, stresses, density and
+ * temperature are integrated in FP64 from it): its steps neither read nor write those five arrays
+ * of sphfem_particles.  export_rates writes the current carry into them (sorted order, as after
+ * the reference's step); import_rates reads them in after a caller changed them between steps.
+ * The first FP32C step imports them, and the first FP64 step after FP32C steps exports them, by
+ * itself.  Both calls do nothing in the FP64 form.  No counterpart in the reference. */
+int sphfem_leap_frog_export_rates(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p);
+int sphfem_leap_frog_import_rates(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p);
 int sphfem_leap_frog_init(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p);
 int sphfem_leap_frog_predict(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, double dt);
 int sphfem_leap_frog_correct(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, double dt);

@@ -600,6 +600,16 @@ class leap_frog {
     leap_frog &operator=(const leap_frog &) = delete;
 
     // SPHFEM_FP64 (default) or SPHFEM_FP32C
+    // the FP32C form keeps pos_t, vel_t, S_t, rho_t and T_t in FP32 inside the stepper: these write
+    // them to / read them from the particle arrays (no-ops in the FP64 form)
+    void export_rates(particle_gpu *particles) {
+        sphfem_shim::check(sphfem_leap_frog_export_rates(sphfem_shim::ctx(), m_handle, particles),
+                           "leap_frog::export_rates");
+    }
+    void import_rates(particle_gpu *particles) {
+        sphfem_shim::check(sphfem_leap_frog_import_rates(sphfem_shim::ctx(), m_handle, particles),
+                           "leap_frog::import_rates");
+    }
     void set_precision(int precision) {
         sphfem_shim::check(sphfem_leap_frog_set_precision(sphfem_shim::ctx(), m_handle, precision),
                            "leap_frog::set_precision");

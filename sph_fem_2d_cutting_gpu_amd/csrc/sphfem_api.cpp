@@ -135,6 +135,15 @@ void swap_gathered(sphfem_particles &p, sphfem_particles &b) {
     std::swap(p.T_t, b.T_t); std::swap(p.on_seg, b.on_seg);
 }
 
+// FP32C step: every gathered array but the rates (the FP32 derivative carry holds those)
+void swap_gathered_state(sphfem_particles &p, sphfem_particles &b) {
+    std::swap(p.pos, b.pos); std::swap(p.vel, b.vel); std::swap(p.fc, b.fc); std::swap(p.ft, b.ft);
+    std::swap(p.h, b.h); std::swap(p.n, b.n); std::swap(p.rho, b.rho); std::swap(p.S, b.S);
+    std::swap(p.eps_pl, b.eps_pl); std::swap(p.eps_pl_dot, b.eps_pl_dot); std::swap(p.T, b.T);
+    std::swap(p.fixed, b.fixed); std::swap(p.blanked, b.blanked); std::swap(p.tool_particle, b.tool_particle);
+    std::swap(p.on_seg, b.on_seg);
+}
+
 // after the fused step: the arrays written in the sorted buffer become the particle arrays
 void swap_fused(sphfem_particles &p, sphfem_particles &b) {
     std::swap(p.pos_t, b.pos_t); std::swap(p.vel_t, b.vel_t); std::swap(p.rho_t, b.rho_t); std::swap(p.S_t, b.S_t);
@@ -1552,6 +1561,7 @@ int sphfem_leap_frog_destroy(sphfem_ctx *ctx, sphfem_leap_frog *lf) {
         dfree(lf->pos_init); dfree(lf->vel_init); dfree(lf->S_init); dfree(lf->rho_init); dfree(lf->T_init);
         dfree(lf->cell_start); dfree(lf->cell_end); dfree(lf->near_list); dfree(lf->contact_list);
         dfree(lf->rec); dfree(lf->psum); dfree(lf->psum_T); dfree(lf->pre_vol); dfree(lf->pre_s); dfree(lf->psum64);
+        dfree(lf->rate32[0]); dfree(lf->rate32[1]);
         delete lf;
         return SPHFEM_OK;
     });
@@ -1571,6 +1581,8 @@ int sphfem_leap_frog_set_precision(sphfem_ctx *ctx, sphfem_leap_frog *lf, int pr
             }
             if (!lf->psum) lf->psum = dalloc<float4>(2ul * lf->num_part);
             if (!lf->psum_T) lf->psum_T = dalloc<float>(lf->num_part);
+            for (int k = 0; k < 2; k++)
+                if (!lf->rate32[k]) lf->rate32[k] = dalloc<float>(10ul * lf->num_part);
         }
         lf->precision = precision;
         return SPHFEM_OK;
@@ -1586,6 +1598,22 @@ int sphfem_leap_frog_set_tuning(sphfem_ctx *ctx, sphfem_leap_frog *lf, const sph
     if (!ok) return fail(ctx, SPHFEM_ERR_ARG, "unsupported tuning");
     lf->tuning = t;
     return SPHFEM_OK;
+}
+
+int sphfem_leap_frog_export_rates(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p) {
+    if (!lf || !p || p->N != lf->num_part) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (lf->rates_fp32) launch_rates_export(ctx, rate_view(lf->rate32[lf->rate_cur], lf->num_part), *p);
+        return SPHFEM_OK;
+    });
+}
+
+int sphfem_leap_frog_import_rates(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p) {
+    if (!lf || !p || p->N != lf->num_part) return SPHFEM_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (lf->rates_fp32) launch_rates_import(ctx, *p, rate_view(lf->rate32[lf->rate_cur], lf->num_part));
+        return SPHFEM_OK;
+    });
 }
 
 int sphfem_leap_frog_get_tuning(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_tuning *tuning) {
@@ -1625,6 +1653,17 @@ int sphfem_leap_frog_correct(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_parti
 // With a rigid tool (leap_frog.cu:99-160) the tool is moved first, as only the contact of the
 // update kernel reads its geometry and that reads the moved one; the update kernel also applies
 // the friction heat, moves the tool particles and computes the contact.
+// where the rates live follows the stepper's form: the FP32 carry is filled from the particle arrays
+// when FP32C steps begin, and written back to them when FP64 steps follow FP32C ones
+static void sync_rate_form(sphfem_ctx *ctx, sphfem_leap_frog *lf, const sphfem_particles &p) {
+    const bool f32c = lf->precision == SPHFEM_FP32C;
+    if (f32c == lf->rates_fp32) return;
+    const RateView r = rate_view(lf->rate32[lf->rate_cur], lf->num_part);
+    if (f32c) launch_rates_import(ctx, p, r);
+    else launch_rates_export(ctx, r, p);
+    lf->rates_fp32 = f32c;
+}
+
 static int step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p, sphfem_grid *g, sphfem_tool *t,
                      sphfem_rigid_tool *rt = nullptr) {
     // leap_frog.cu:175 (a no-op for a locked grid)
@@ -1636,9 +1675,17 @@ static int step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p,
     // the source of each sorted position: the permutation of the sort (Green), or the identity
     // after the Rothlin sort has gathered in place
     sphfem_particles A = *p;
+    const bool f32c = lf->precision == SPHFEM_FP32C;
+    const size_t nr = lf->num_part;
+    sync_rate_form(ctx, lf, *p);
     if (g->kind == SPHFEM_GRID_ROTHLIN) {
         launch_hash(ctx, g, *p, false, nullptr);
         launch_rothlin_sort(ctx, g, *p);    // sorted keys, in-place gathers, dense cell table
+        if (f32c) {                         // the carry follows the in-place gathers
+            launch_rates_gather(ctx, rate_view(lf->rate32[lf->rate_cur], nr), rate_view(lf->rate32[lf->rate_cur ^ 1], nr),
+                                g->perm, (int)p->N);
+            lf->rate_cur ^= 1;
+        }
         A.idx = g->iota;
     } else {
         // the cell table is cleared by the hash pass (cells of the previous keys) except when the
@@ -1647,16 +1694,24 @@ static int step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p,
         launch_resort(ctx, g, *p);
         g->first_sort = false;
     }
-    launch_reorder_predict(ctx, g, A, lf);
+    const RateView r_old = rate_view(lf->rate32[lf->rate_cur], nr);
+    const RateView r_new = rate_view(lf->rate32[lf->rate_cur ^ 1], nr);
+    launch_reorder_predict(ctx, g, A, lf, f32c ? &r_old : nullptr);
     sphfem_particles B = g->buf;
     B.hash = p->hash;
     B.idx = A.idx;
     B.N = p->N;
-    launch_fused_update(ctx, B, *p, lf, g->cell_start, g->cell_end, t, rt);
-    if (lf->precision == SPHFEM_FP32C) {
-        // the whole new state is in the buffer; R is not maintained by the FP32C form
-        swap_gathered(*p, g->buf);
+    // far from the FEM tool the contact fields are zero and need not be read (not with the Rothlin
+    // grid, whose step leaves ft and n ungathered as the reference does, nor with a rigid tool)
+    const bool skip_contact = t && !rt && g->kind == SPHFEM_GRID_GREEN;
+    launch_fused_update(ctx, B, *p, lf, g->cell_start, g->cell_end, t, rt, f32c ? &r_old : nullptr,
+                        f32c ? &r_new : nullptr, skip_contact);
+    if (f32c) {
+        // the whole new state is in the buffer, the new rates in the other carry buffer; R is not
+        // maintained by the FP32C form
+        swap_gathered_state(*p, g->buf);
         std::swap(p->p, g->buf.p);
+        lf->rate_cur ^= 1;
     } else {
         swap_fused(*p, g->buf);
     }
@@ -1724,6 +1779,7 @@ static int graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles 
         if (!g->locked)
             return fail(ctx, SPHFEM_ERR_STATE, "graphs need a locked grid (the geometry update reads back to the host)");
         if (g->first_sort) return fail(ctx, SPHFEM_ERR_STATE, "run one step before capturing (first sort)");
+        sync_rate_form(ctx, lf, *p);      // eagerly: the graph records the steady state only
         const bool timing = ctx->timing.enabled;
         ctx->timing.enabled = false;
         auto *gr = new sphfem_graph();

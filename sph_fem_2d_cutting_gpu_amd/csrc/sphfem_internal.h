@@ -57,6 +57,23 @@ typedef sphfem_tool_kinematics ToolKin;
 // Views of the particle arrays passed to kernels.
 typedef sphfem_particles Particles;
 
+// FP32 derivative carry of the FP32C form: SoA blocks of float2 pos_t, float2 vel_t, float4 S_t,
+// float rho_t, float T_t over n particles from one base pointer (10 n floats)
+struct RateView {
+    float2 *pos_t, *vel_t;
+    float4 *S_t;
+    float *rho_t, *T_t;
+};
+inline RateView rate_view(float *base, size_t n) {
+    RateView r;
+    r.pos_t = reinterpret_cast<float2 *>(base);
+    r.vel_t = reinterpret_cast<float2 *>(base + 2 * n);
+    r.S_t = reinterpret_cast<float4 *>(base + 4 * n);
+    r.rho_t = base + 8 * n;
+    r.T_t = base + 9 * n;
+    return r;
+}
+
 // FP64 pair sums of one particle (interactions_gpu.cu:143-379 outputs, T as the Laplacian)
 struct PairSums {
     sphfem_double4 v_der, S_der;
@@ -207,6 +224,13 @@ struct sphfem_leap_frog {
     // [1] stress-divergence + viscous acceleration (x, y), XSPH velocity (x, y); psum_T: Laplacian of T
     float4 *psum = nullptr;
     float *psum_T = nullptr;
+    // FP32C form: the derivative carry (pos_t, vel_t, S_t, rho_t, T_t) in FP32, two buffers of 10
+    // floats per particle (SoA, see rate_view); rate32[rate_cur] is in the particles' current order.
+    // While rates_fp32 is set the carry, not the particle arrays' FP64 rates, holds the rates:
+    // sphfem_leap_frog_export_rates writes them out, sphfem_leap_frog_import_rates reads them in.
+    float *rate32[2] = {nullptr, nullptr};
+    int rate_cur = 0;
+    bool rates_fp32 = false;
 };
 
 namespace sphfem {
@@ -247,7 +271,9 @@ void launch_bbox(sphfem_ctx *ctx, const sphfem_particles &p, double *part, doubl
 void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p);
 void launch_fill_int(sphfem_ctx *ctx, int *ptr, long n, int value);
 void launch_reorder_full(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p);
-void launch_reorder_predict(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, sphfem_leap_frog *lf);
+// FP32C: the pre-step rates are read from the carry buffer rates (in the order of p)
+void launch_reorder_predict(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, sphfem_leap_frog *lf,
+                            const RateView *rates = nullptr);
 void launch_records_state(sphfem_ctx *ctx, const sphfem_particles &p, int num_cell, float4 *rec);
 
 // particles (sphfem_particles.hip)
@@ -257,11 +283,20 @@ void launch_interactions_ref(sphfem_ctx *ctx, const sphfem_particles &p, const i
 void launch_interactions_pse(sphfem_ctx *ctx, const sphfem_particles &p, const int *cs, const int *ce);
 void launch_interactions_fp32c(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_leap_frog *lf, const int *cs,
                                const int *ce, int num_cell);
+// FP32C: rates_old holds the pre-step rates in the order of A, rates_new receives the new rates in
+// sorted order; skip_contact: the contact fields of a particle with on_seg = -1 are zero (FEM tool,
+// Green grid) and are not read
 void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfem_particles &A,
                          sphfem_leap_frog *lf, const int *cs, const int *ce, sphfem_tool *t,
-                         sphfem_rigid_tool *rt = nullptr);
+                         sphfem_rigid_tool *rt = nullptr, const RateView *rates_old = nullptr,
+                         const RateView *rates_new = nullptr, bool skip_contact = false);
 // part: 2 * SPHFEM_FORCE_BLOCKS doubles of block partials; out: the 2 sums
 void launch_report_force(sphfem_ctx *ctx, const sphfem_particles &p, double *part, double *out);
+// FP32C derivative carry: the particle arrays' FP64 rates into / out of a carry buffer; a gather
+// dst[i] = src[perm[i]] of a carry buffer
+void launch_rates_import(sphfem_ctx *ctx, const sphfem_particles &p, const RateView &dst);
+void launch_rates_export(sphfem_ctx *ctx, const RateView &src, const sphfem_particles &p);
+void launch_rates_gather(sphfem_ctx *ctx, const RateView &src, const RateView &dst, const int *perm, int n);
 
 // tool (sphfem_tool.hip)
 void launch_fric_heat(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t);

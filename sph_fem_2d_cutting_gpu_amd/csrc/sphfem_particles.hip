@@ -729,7 +729,8 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
                                                           const float *__restrict__ psum_T,
                                                           const PairSums *__restrict__ psum64,
                                                           const RigidTool *__restrict__ rt,
-                                                          int *__restrict__ nonfinite) {
+                                                          int *__restrict__ nonfinite, const RateView r_old,
+                                                          const RateView r_new, int skip_contact) {
     const int i = xcd_logical_block() * BS + threadIdx.x;
     if (i == 0) *nonfinite = 0;      // the pair kernels of this step have read it
     if (i >= (int)B.N) return;
@@ -778,9 +779,9 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
         rho = rho0;
         if (live) {
             const double hdt = 0.5 * c.dt;
-            const sphfem_double2 vt = A.vel_t[src];
-            const sphfem_double4 St = A.S_t[src];
-            const double rt = A.rho_t[src];
+            const float2 vt = r_old.vel_t[src];          // the FP32 derivative carry
+            const float4 St = r_old.S_t[src];
+            const double rt = r_old.rho_t[src];
             vel.x = vel.x + hdt * vt.x;
             vel.y = vel.y + hdt * vt.y;
             S.x = S.x + hdt * St.x;
@@ -792,10 +793,19 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
         if (tool == 0.) pr = eos_pressure(c, rho);
         eps_pl = A.eps_pl[src];
         eps_pl_dot = A.eps_pl_dot[src];
-        fc0 = A.fc[src];
-        ft0 = A.ft[src];
-        nrm0 = A.n[src];
         on_seg0 = A.on_seg[src];
+        if (skip_contact && on_seg0 == -1) {
+            // every writer of on_seg = -1 (contact s1, the far-from-tool reset) zeroes fc, ft and n
+            // with it (actions_gpu.cu:455-491): nothing to read far from the tool
+            const sphfem_double2 z = {0., 0.};
+            fc0 = z;
+            ft0 = z;
+            nrm0 = z;
+        } else {
+            fc0 = A.fc[src];
+            ft0 = A.ft[src];
+            nrm0 = A.n[src];
+        }
     } else {
         pos = B.pos[i];
         vel = B.vel[i];
@@ -820,7 +830,8 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
     if (live) {
         pos_t = r.pos_t;
         vel_t = r.vel_t;
-        T_t = thermal_alpha != 0. ? thermal_alpha * r.T_lapl : A.T_t[F32C ? src : A.idx[i]];
+        if constexpr (F32C) T_t = thermal_alpha != 0. ? thermal_alpha * r.T_lapl : (double)r_old.T_t[src];
+        else T_t = thermal_alpha != 0. ? thermal_alpha * r.T_lapl : A.T_t[A.idx[i]];
         if (tool == 0.) {
             S_t = jaumann_rate(c, r.v_der, S);        // actions_gpu.cu:280-314
             rho_t = -rho * (r.v_der.x + r.v_der.w);     // :626-639
@@ -909,11 +920,20 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
             if (bad) blanked = 1.;
         }
     }
-    B.pos_t[i] = pos_t;
-    B.vel_t[i] = vel_t;
-    B.S_t[i] = S_t;
-    B.rho_t[i] = rho_t;
-    B.T_t[i] = T_t;
+    if constexpr (F32C) {
+        // the new rates in sorted order, FP32 (the pair sums they come from are FP32)
+        r_new.pos_t[i] = make_float2((float)pos_t.x, (float)pos_t.y);
+        r_new.vel_t[i] = make_float2((float)vel_t.x, (float)vel_t.y);
+        r_new.S_t[i] = make_float4((float)S_t.x, (float)S_t.y, (float)S_t.z, (float)S_t.w);
+        r_new.rho_t[i] = (float)rho_t;
+        r_new.T_t[i] = (float)T_t;
+    } else {
+        B.pos_t[i] = pos_t;
+        B.vel_t[i] = vel_t;
+        B.S_t[i] = S_t;
+        B.rho_t[i] = rho_t;
+        B.T_t[i] = T_t;
+    }
     sphfem_particles &O = F32C ? B : A;
     O.pos[i] = pos;
     O.vel[i] = vel;
@@ -1004,7 +1024,11 @@ void launch_interactions_fp32c(sphfem_ctx *ctx, const sphfem_particles &p, sphfe
 }
 
 void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfem_particles &A, sphfem_leap_frog *lf,
-                         const int *cs, const int *ce, sphfem_tool *t, sphfem_rigid_tool *rt) {
+                         const int *cs, const int *ce, sphfem_tool *t, sphfem_rigid_tool *rt,
+                         const RateView *rates_old, const RateView *rates_new, bool skip_contact) {
+    const RateView none{};
+    const RateView r_old = rates_old ? *rates_old : none, r_new = rates_new ? *rates_new : none;
+    const int skip = skip_contact ? 1 : 0;
     ToolView tv{};
     if (t) {
         tv.T_nodes = t->T;
@@ -1025,7 +1049,7 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
 #define SPHFEM_UPDATE(BSZ, RG)                                                                              \
     k_fused_update<true, BSZ, RG><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(                                  \
         ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi, lf->psum, \
-        lf->psum_T, nullptr, rtd, ctx->d_flags + SPHFEM_FLAG_NONFINITE)
+        lf->psum_T, nullptr, rtd, ctx->d_flags + SPHFEM_FLAG_NONFINITE, r_old, r_new, skip)
         if (lf->tuning.update_block == 128) {
             if (rtd) SPHFEM_UPDATE(128, true);
             else SPHFEM_UPDATE(128, false);
@@ -1055,7 +1079,7 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
 #define SPHFEM_UPDATE(BSZ, RG)                                                                              \
     k_fused_update<false, BSZ, RG><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(                                 \
         ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi, nullptr, \
-        nullptr, lf->psum64, rtd, ctx->d_flags + SPHFEM_FLAG_NONFINITE)
+        nullptr, lf->psum64, rtd, ctx->d_flags + SPHFEM_FLAG_NONFINITE, r_old, r_new, 0)
         if (lf->tuning.update_block == 128) {
             if (rtd) SPHFEM_UPDATE(128, true);
             else SPHFEM_UPDATE(128, false);
@@ -1065,6 +1089,58 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
         }
 #undef SPHFEM_UPDATE
     }
+}
+
+// FP32C derivative carry <-> the particle arrays' FP64 rates, and its gather by a permutation
+__global__ void __launch_bounds__(256) k_rates_import(sphfem_particles P, RateView d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int)P.N) return;
+    const sphfem_double2 pt = P.pos_t[i], vt = P.vel_t[i];
+    const sphfem_double4 St = P.S_t[i];
+    d.pos_t[i] = make_float2((float)pt.x, (float)pt.y);
+    d.vel_t[i] = make_float2((float)vt.x, (float)vt.y);
+    d.S_t[i] = make_float4((float)St.x, (float)St.y, (float)St.z, (float)St.w);
+    d.rho_t[i] = (float)P.rho_t[i];
+    d.T_t[i] = (float)P.T_t[i];
+}
+
+__global__ void __launch_bounds__(256) k_rates_export(RateView r, sphfem_particles P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int)P.N) return;
+    const float2 pt = r.pos_t[i], vt = r.vel_t[i];
+    const float4 St = r.S_t[i];
+    P.pos_t[i] = {pt.x, pt.y};
+    P.vel_t[i] = {vt.x, vt.y};
+    P.S_t[i] = {St.x, St.y, St.z, St.w};
+    P.rho_t[i] = r.rho_t[i];
+    P.T_t[i] = r.T_t[i];
+}
+
+__global__ void __launch_bounds__(256) k_rates_gather(RateView a, RateView b, const int *__restrict__ perm, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = perm[i];
+    if ((unsigned)s >= (unsigned)n) return;
+    b.pos_t[i] = a.pos_t[s];
+    b.vel_t[i] = a.vel_t[s];
+    b.S_t[i] = a.S_t[s];
+    b.rho_t[i] = a.rho_t[s];
+    b.T_t[i] = a.T_t[s];
+}
+
+void launch_rates_import(sphfem_ctx *ctx, const sphfem_particles &p, const RateView &dst) {
+    KTimer t(ctx, "rates_import");
+    k_rates_import<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(p, dst);
+}
+
+void launch_rates_export(sphfem_ctx *ctx, const RateView &src, const sphfem_particles &p) {
+    KTimer t(ctx, "rates_export");
+    k_rates_export<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(src, p);
+}
+
+void launch_rates_gather(sphfem_ctx *ctx, const RateView &src, const RateView &dst, const int *perm, int n) {
+    KTimer t(ctx, "rates_gather");
+    k_rates_gather<<<nblk(n, 256), 256, 0, ctx->stream>>>(src, dst, perm, n);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -923,7 +923,7 @@ __global__ void __launch_bounds__(256) k_reorder_predict(DevConsts c, const sphf
 __global__ void __launch_bounds__(256) k_reorder_records(DevConsts c, const sphfem_particles p,
                                                          int *__restrict__ cell_start, int *__restrict__ cell_end,
                                                          int num_cell, int *__restrict__ flags,
-                                                         float4 *__restrict__ rec) {
+                                                         float4 *__restrict__ rec, const RateView rates) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int N = (int)p.N;
     if (i >= N) return;
@@ -935,9 +935,10 @@ __global__ void __launch_bounds__(256) k_reorder_records(DevConsts c, const sphf
     const double blanked = p.blanked[s], tool = p.tool_particle[s];
     if (blanked != 1.) {
         const double dt = c.dt;
-        const sphfem_double2 pt = p.pos_t[s], vt = p.vel_t[s];
-        const sphfem_double4 St = p.S_t[s];
-        const double rt = p.rho_t[s], Tt = p.T_t[s];
+        // the FP32 derivative carry of the FP32C form, in the order of p
+        const float2 pt = rates.pos_t[s], vt = rates.vel_t[s];
+        const float4 St = rates.S_t[s];
+        const double rt = rates.rho_t[s], Tt = rates.T_t[s];
         pos.x = pos.x + 0.5 * dt * pt.x;
         pos.y = pos.y + 0.5 * dt * pt.y;
         vel.x = vel.x + 0.5 * dt * vt.x;
@@ -960,11 +961,12 @@ __global__ void __launch_bounds__(256) k_reorder_records(DevConsts c, const sphf
     write_records(c, rec, N, i, h, pos, vel, S, rho, T, blanked, tool, hh, pr, R);
 }
 
-void launch_reorder_predict(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, sphfem_leap_frog *lf) {
+void launch_reorder_predict(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, sphfem_leap_frog *lf,
+                            const RateView *rates) {
     if (lf->precision == SPHFEM_FP32C) {
         KTimer t(ctx, "reorder_records");
         k_reorder_records<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, g->cell_start, g->cell_end,
-                                                                   g->num_cell, ctx->d_flags, lf->rec);
+                                                                   g->num_cell, ctx->d_flags, lf->rec, *rates);
         return;
     }
     KTimer t(ctx, "reorder_predict");

@@ -18,6 +18,7 @@ from . import cases
 
 # trailing shapes and dtypes of the particle arrays (particle_gpu.h:23-83)
 _FIELD_SPEC = {name: (shp, dt) for name, shp, dt in cases.PARTICLE_FIELDS}
+RATE_FIELDS = ("pos_t", "vel_t", "rho_t", "S_t", "T_t")
 
 
 class Context:
@@ -99,6 +100,9 @@ class particle_gpu:
         self.s = _lib.Particles()
         ctx.call("sphfem_particles_alloc", int(N), C.byref(self.s))
         self.N = int(N)
+        # a stepper in the FP32C form keeps the rates in FP32 (sphfem_leap_frog_export_rates /
+        # import_rates): set by leap_frog.attach, None otherwise
+        self.stepper = None
         for k, v in fields.items():
             if v is not None:
                 self.upload(k, v)
@@ -110,8 +114,12 @@ class particle_gpu:
         shp, dt = _FIELD_SPEC[field]
         a = np.asarray(arr, dtype=dt).reshape((self.N,) + shp)
         self.ctx.htod(self.ptr(field), a)
+        if field in RATE_FIELDS and self.stepper is not None:
+            self.ctx.call("sphfem_leap_frog_import_rates", self.stepper.h, C.byref(self.s))
 
     def download(self, field) -> np.ndarray:
+        if field in RATE_FIELDS and self.stepper is not None:
+            self.ctx.call("sphfem_leap_frog_export_rates", self.stepper.h, C.byref(self.s))
         shp, dt = _FIELD_SPEC[field]
         return self.ctx.dtoh(self.ptr(field), (self.N,) + shp, dt)
 
@@ -862,6 +870,7 @@ class Simulation:
                                               case.const.phys)
         self.stepper = leap_frog(self.ctx, case.N, self.grid.num_cell,
                                  self.tool.n_nodes if self.tool else 0, precision)
+        self.p.stepper = self.stepper      # the rates of an FP32C stepper live in FP32 inside it
 
     def step(self, n: int = 1):
         for _ in range(n):
