@@ -627,9 +627,11 @@ void launch_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t) 
 // Near-tool particles of the fused step: contact s1, mechanical boundary condition, NaN check;
 // particles left in contact go to the contact list (bit 30 = invalidate after s2), the others
 // also get the thermal boundary condition and their invalidation here.
+// use32: the rates are the FP32 derivative carry r32 of the FP32C form (the particle arrays' FP64
+// rates are not maintained by its steps)
 __global__ void __launch_bounds__(256) k_near_contact(DevConsts c, sphfem_particles P, sphfem_tool t,
                                                       const int *__restrict__ near_list, int *counters,
-                                                      int *contact_list) {
+                                                      int *contact_list, RateView r32, int use32) {
     // one particle of the near list per wavefront (wave-uniform loop); the segment loops of
     // contact s1 run over the lanes
     const int count = counters[1];
@@ -644,15 +646,28 @@ __global__ void __launch_bounds__(256) k_near_contact(DevConsts c, sphfem_partic
                                              t.n_seg, lane);
         if (lane != 0) continue;
         double blanked = blank ? 1. : P.blanked[i];
-        sphfem_double2 pos_t = P.pos_t[i], vel_t = P.vel_t[i];
+        sphfem_double2 pos_t, vel_t;
+        if (use32) {
+            const float2 a = r32.pos_t[i], b = r32.vel_t[i];
+            pos_t = {a.x, a.y};
+            vel_t = {b.x, b.y};
+        } else {
+            pos_t = P.pos_t[i];
+            vel_t = P.vel_t[i];
+        }
         if (P.tool_particle[i] == 0. && P.fixed[i] != 0.) {
             sphfem_double2 z = {0., 0.};
             P.vel[i] = z;
             fc = z;
             pos_t = z;
             vel_t = z;
-            P.pos_t[i] = z;
-            P.vel_t[i] = z;
+            if (use32) {
+                r32.pos_t[i] = make_float2(0.f, 0.f);
+                r32.vel_t[i] = make_float2(0.f, 0.f);
+            } else {
+                P.pos_t[i] = z;
+                P.vel_t[i] = z;
+            }
         }
         P.fc[i] = fc;
         P.ft[i] = ft;
@@ -660,10 +675,21 @@ __global__ void __launch_bounds__(256) k_near_contact(DevConsts c, sphfem_partic
         P.on_seg[i] = seg;
         bool bad = false;
         if (blanked != 1.) {
-            sphfem_double4 st = P.S_t[i];
+            sphfem_double4 st;
+            double rt, Tt;
+            if (use32) {
+                const float4 a = r32.S_t[i];
+                st = {a.x, a.y, a.z, a.w};
+                rt = r32.rho_t[i];
+                Tt = r32.T_t[i];
+            } else {
+                st = P.S_t[i];
+                rt = P.rho_t[i];
+                Tt = P.T_t[i];
+            }
             bad = isnaninf(pos_t.x) || isnaninf(pos_t.y) || isnaninf(vel_t.x) || isnaninf(vel_t.y) ||
-                  isnaninf(st.x) || isnaninf(st.y) || isnaninf(st.z) || isnaninf(st.w) || isnaninf(P.rho_t[i]) ||
-                  isnaninf(P.T_t[i]);
+                  isnaninf(st.x) || isnaninf(st.y) || isnaninf(st.z) || isnaninf(st.w) || isnaninf(rt) ||
+                  isnaninf(Tt);
         }
         if (seg != -1) {
             int slot = atomicAdd(&counters[2], 1);
@@ -694,8 +720,10 @@ void launch_near_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool
     int blocks = std::max(1, std::min(nblk(p.N, 256), 1024));
     {
         KTimer tm(ctx, "near_contact");
+        const bool use32 = lf->rates_fp32;
+        const RateView r32 = use32 ? rate_view(lf->rate32[lf->rate_cur], lf->num_part) : RateView{};
         k_near_contact<<<blocks, 256, 0, ctx->stream>>>(ctx->c, p, *t, lf->near_list, ctx->d_flags,
-                                                        lf->contact_list);
+                                                        lf->contact_list, r32, use32 ? 1 : 0);
     }
     KTimer tm(ctx, "contact_list_s2");
     k_contact_list_s2<<<std::max(1, std::min(blocks, 64)), 256, 0, ctx->stream>>>(ctx->c, p, *t, ctx->d_flags,

@@ -210,3 +210,37 @@ def _assert_forced_branches_fired(case, P):
     inside = P["pos"][:, 0] < case.tool.seg_left[0][0] - 2.5e-4 + case.tool_vel[0] * case.dt
     assert np.any(bl & inside), "deep penetration"
     assert bl.sum() >= 10, "the NaN particle's neighbours"
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_fixed_particles_near_the_tool(precision):
+    """Fixed particles among the near-tool ones (the near-contact pass applies their boundary condition
+    and the invalidation check): after every step their velocity, contact force and position/velocity
+    rates are zero and their temperature is T_init, and the run follows the oracle."""
+    base = cases.small_cutting_case(nx=48, ny=16, wear=True)
+    fixed = base.fixed.copy()
+    tip = base.tool.seg_left[0][0]
+    near = np.nonzero(base.pos[:, 0] < tip + 12 * base.dx)[0]
+    fixed[near[::3]] = 1.0
+    case = dataclasses.replace(base, fixed=fixed)
+    o = Oracle(case)
+    prec = _lib.FP32C if precision == "fp32c" else _lib.FP64
+    sim = S.Simulation(case, precision=prec)
+    T0 = case.const.trml_wp["T_init"]
+    for k in range(4):
+        print(f"near-tool fixed particles, {precision}, steps {3 * k + 1}..{3 * k + 3}", flush=True)
+        o.step(3)
+        sim.step(3)
+        sim.ctx.check_error()
+        a = sim.state()
+        fx = a["fixed"] == 1.0
+        assert fx.sum() == fixed.sum()
+        for f in ("vel", "fc", "pos_t", "vel_t"):
+            assert np.all(a[f][fx] == 0.0), f
+        assert np.all(a["T"][fx] == T0)
+        for f in ("hash", "idx", "blanked"):
+            assert_equal(f, a[f], o.P[f])
+        live = o.P["blanked"] != 1.0
+        for f in ("pos", "vel", "S", "rho", "T"):
+            assert_close(f"{f} after {3 * k + 3} steps", a[f], o.P[f], 1e-6, mask=live)
+    assert (o.P["on_seg"] >= 0).sum() > 0
