@@ -1153,6 +1153,8 @@ int sphfem_tool_create(sphfem_ctx *ctx, const sphfem_tool_desc *d, sphfem_tool *
             t->n_fixed = d->fixed_num;
             t->remeshable = true;
         }
+        t->sink_keys = dalloc<unsigned long long>(2ul * SPHFEM_SEG_SINK_CAP);
+        t->sink_vals = dalloc<double>(8ul * SPHFEM_SEG_SINK_CAP);
         SPHFEM_HIP_CHECK(hipStreamSynchronize(s));
         *out = t;
         return SPHFEM_OK;
@@ -1172,6 +1174,7 @@ int sphfem_tool_destroy(sphfem_ctx *ctx, sphfem_tool *t) {
         dfree(t->layer_new); dfree(t->nnz_layer_ptr); dfree(t->nnz_layer); dfree(t->node_layer_ptr);
         dfree(t->node_layer); dfree(t->kin); dfree(t->node_flags); dfree(t->dir_node); dfree(t->dir_ptr);
         dfree(t->dir_entry); dfree(t->pos_map); dfree(t->T_map); dfree(t->bk_ptr); dfree(t->bk_list);
+        dfree(t->sink_keys); dfree(t->sink_vals);
         delete[] t->h_elem;
         delete t;
         return SPHFEM_OK;

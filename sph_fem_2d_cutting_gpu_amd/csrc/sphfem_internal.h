@@ -19,6 +19,11 @@
 #define SPHFEM_FLAG_NONFINITE 5
 // blocks of the grid-stride window form of the pair kernels
 #define SPHFEM_WINDOW_BLOCKS 512
+// records per step of each ordered segment sum (friction heat, contact heat exchange); the
+// reduction sorts them in LDS
+#define SPHFEM_SEG_SINK_CAP 4096
+#define SPHFEM_FLAG_SINK_FRIC 6
+#define SPHFEM_FLAG_SINK_S2 7
 // blocks of the report_force reduction (its partials: 2 doubles per block) and the largest particle
 // count of the allocation-free variant
 #define SPHFEM_FORCE_BLOCKS 256
@@ -182,6 +187,10 @@ struct sphfem_tool {
     sphfem_int4 *h_elem = nullptr;     // host copy of the elements (new[]), for the element bins
     int *bk_ptr = nullptr, *bk_list = nullptr;   // element bins of the last remesh (device)
     size_t bk_ptr_cap = 0, bk_list_cap = 0;
+    // records of the ordered segment sums (SegSink): friction heat (6 values) and the contact heat
+    // exchange (2 values), SPHFEM_SEG_SINK_CAP each; their counters are d_flags[6] and d_flags[7]
+    unsigned long long *sink_keys = nullptr;     // [2 * cap]
+    double *sink_vals = nullptr;                 // [8 * cap]
 };
 
 // The tool's elements binned on a uniform grid by their bounding boxes (padded well beyond the
@@ -314,6 +323,8 @@ void launch_fem_robin(sphfem_ctx *ctx, sphfem_tool *t);
 void launch_fem_rate(sphfem_ctx *ctx, sphfem_tool *t);
 void launch_wear_reset(sphfem_ctx *ctx, sphfem_tool *t);
 void launch_near_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t, sphfem_leap_frog *lf);
+// the ordered segment sums of the friction-heat records (after the pass that produced them)
+void launch_fric_heat_sums(sphfem_ctx *ctx, sphfem_tool *t);
 
 // rigid tool (sphfem_rigid.hip)
 void launch_rigid_advance(sphfem_ctx *ctx, sphfem_rigid_tool *rt);

@@ -865,7 +865,7 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
         rho = rho0 + dt * rho_t;
         T = T0 + dt * T_t;
         if (tv.active) {
-            if (!fric_heat_v21(c, tv, *tv.kin, eps_pl, ft, on_seg, nrm, vel, fc, rho, T)) blanked = 1.;
+            if (!fric_heat_v21(c, tv, *tv.kin, i, eps_pl, ft, on_seg, nrm, vel, fc, rho, T)) blanked = 1.;
         }
         if constexpr (RIGID) {
             if (c.fric_heat && tool == 0.) T = rigid_fric_heat(c, rt->g.vel, vel, ft, nrm, T);
@@ -1040,6 +1040,7 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
         tv.near_list = lf->near_list;
         tv.near_count = ctx->d_flags + 1;
         tv.active = 1;
+        tv.sink = SegSink{ctx->d_flags + SPHFEM_FLAG_SINK_FRIC, t->sink_keys, t->sink_vals, SPHFEM_SEG_SINK_CAP};
     }
     const int wi = lf->intermediates ? 1 : 0;
     const RigidTool *rtd = rt ? rt->dev : nullptr;
@@ -1089,6 +1090,7 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
         }
 #undef SPHFEM_UPDATE
     }
+    if (t) launch_fric_heat_sums(ctx, t);     // the friction-heat segment sums, in particle order
 }
 
 // FP32C derivative carry <-> the particle arrays' FP64 rates, and its gather by a permutation

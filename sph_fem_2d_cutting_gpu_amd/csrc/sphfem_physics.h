@@ -484,7 +484,7 @@ __device__ __forceinline__ bool isnaninf(double v) { return isnan(v) || isinf(v)
 // particle still carries segment 0 (particle_gpu.cu:98) and would hit one address.
 template <int NV>
 __device__ __forceinline__ void wave_atomic_add(bool active, int key, const double (&v)[NV], double *const (&dst)[NV],
-                                                int stride) {
+                                                const int (&stride)[NV]) {
     uint64_t pending = __ballot(active);
     while (pending) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
@@ -500,9 +500,41 @@ __device__ __forceinline__ void wave_atomic_add(bool active, int key, const doub
             g &= g - 1;
         }
         if (__lane_id() == leader)
-            for (int q = 0; q < NV; q++) unsafeAtomicAdd(dst[q] + (long)lkey * stride, s[q]);
+            for (int q = 0; q < NV; q++) unsafeAtomicAdd(dst[q] + (long)lkey * stride[q], s[q]);
         pending &= ~group;
     }
+}
+
+// Segment sums in a fixed order.  Each contributing particle appends one record (segment, sorted
+// particle index, values); k_segment_reduce then adds the values to the segment arrays in
+// increasing particle index per segment, the order of the reference's particle loop
+// (actions_gpu.cu:398-418, 607-608), which its atomics leave open.  Contributions beyond `cap` in
+// one step (the first step of a large case, where every particle still carries segment 0) are
+// added directly with wave-aggregated atomics: still correct, no longer reproducible bit for bit.
+struct SegSink {
+    int *count;
+    unsigned long long *keys;   // [cap] segment << 32 | particle index
+    double *vals;               // [NV][cap]
+    int cap;
+};
+
+template <int NV>
+__device__ __forceinline__ void seg_sink_add(const SegSink &s, bool active, int seg, int i, const double (&v)[NV],
+                                             double *const (&dst)[NV], const int (&stride)[NV]) {
+    const uint64_t m = __ballot(active);
+    if (!m) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (__lane_id() == leader) base = atomicAdd(s.count, __popcll(m));
+    base = __shfl(base, leader);
+    if (!active) return;
+    const int slot = base + __popcll(m & ((1ull << __lane_id()) - 1ull));
+    const bool fits = slot < s.cap;
+    if (fits) {
+        s.keys[slot] = ((unsigned long long)(unsigned)seg << 32) | (unsigned)i;
+        for (int q = 0; q < NV; q++) s.vals[(size_t)q * s.cap + slot] = v[q];
+    }
+    wave_atomic_add<NV>(!fits, seg, v, dst, stride);
 }
 
 // tool-side arrays the particle kernels accumulate into
@@ -516,10 +548,11 @@ struct ToolView {
     int *near_list;
     int *near_count;
     int active;
+    SegSink sink;          // the friction-heat segment sums (fric_heat, physical_para, sliding_force)
 };
 
 // actions_gpu.cu:345-420 for one particle; returns false if the particle was blanked
-__device__ __forceinline__ bool fric_heat_v21(const DevConsts &c, const ToolView &tv, const ToolKin &kin,
+__device__ __forceinline__ bool fric_heat_v21(const DevConsts &c, const ToolView &tv, const ToolKin &kin, int pidx,
                                               double eps_pl, sphfem_double2 ft, int seg, sphfem_double2 normal,
                                               sphfem_double2 vp, sphfem_double2 fcv, double rho, double &T) {
     bool keep = true, add_heat = false, add_wear = false;
@@ -560,15 +593,14 @@ __device__ __forceinline__ bool fric_heat_v21(const DevConsts &c, const ToolView
         pp[3] = T_seg;
         slide = f_fric_mag / length;
     } while (false);
-    const double hv[1] = {heat};
-    double *const hd[1] = {tv.fric_heat};
-    wave_atomic_add<1>(add_heat, seg, hv, hd, 1);
-    double *const pd[4] = {&tv.physical_para[0].x, &tv.physical_para[0].y, &tv.physical_para[0].z,
-                           &tv.physical_para[0].w};
-    wave_atomic_add<4>(add_wear, seg, pp, pd, 4);
-    const double sv[1] = {slide};
-    double *const sd[1] = {tv.sliding_force};
-    wave_atomic_add<1>(add_wear, seg, sv, sd, 1);
+    // one record per heated particle; the Usui inputs and the sliding force are zero unless add_wear
+    // (adding +0 to the segment sums leaves them unchanged)
+    const double v[6] = {heat, pp[0], pp[1], pp[2], pp[3], slide};
+    double *const dst[6] = {tv.fric_heat, &tv.physical_para[0].x, &tv.physical_para[0].y, &tv.physical_para[0].z,
+                            &tv.physical_para[0].w, tv.sliding_force};
+    const int stride[6] = {1, 4, 4, 4, 4, 1};
+    (void)add_wear;
+    seg_sink_add<6>(tv.sink, add_heat, seg, pidx, v, dst, stride);
     return keep;
 }
 

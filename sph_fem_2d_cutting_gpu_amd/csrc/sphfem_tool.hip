@@ -35,6 +35,8 @@ __global__ void __launch_bounds__(256) k_tool_begin(sphfem_tool t, double half_d
     if (i == 0) {
         counters[1] = 0;
         counters[2] = 0;
+        counters[SPHFEM_FLAG_SINK_FRIC] = 0;
+        counters[SPHFEM_FLAG_SINK_S2] = 0;
     }
     if (i < t.n_nodes) {
         double T = t.T[i];
@@ -543,7 +545,16 @@ void launch_interp_new(sphfem_ctx *ctx, const sphfem_tool *old_t, const sphfem_d
 // ---------------------------------------------------------------------------------------------
 
 // material_fric_heat_gen_FEM_v2 (actions_gpu.cu:345-420, 967-981) over all particles
-__global__ void __launch_bounds__(256) k_fric_heat(DevConsts c, sphfem_particles P, sphfem_tool t) {
+// the two record lists of a tool's ordered segment sums
+__device__ __forceinline__ SegSink fric_sink(const sphfem_tool &t, int *counters) {
+    return SegSink{counters + SPHFEM_FLAG_SINK_FRIC, t.sink_keys, t.sink_vals, SPHFEM_SEG_SINK_CAP};
+}
+__device__ __forceinline__ SegSink s2_sink(const sphfem_tool &t, int *counters) {
+    return SegSink{counters + SPHFEM_FLAG_SINK_S2, t.sink_keys + SPHFEM_SEG_SINK_CAP,
+                   t.sink_vals + 6ul * SPHFEM_SEG_SINK_CAP, SPHFEM_SEG_SINK_CAP};
+}
+
+__global__ void __launch_bounds__(256) k_fric_heat(DevConsts c, sphfem_particles P, sphfem_tool t, int *counters) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int)P.N) return;
     if (P.blanked[i] == 1.) return;
@@ -553,15 +564,20 @@ __global__ void __launch_bounds__(256) k_fric_heat(DevConsts c, sphfem_particles
     tv.fric_heat = t.fric_heat;
     tv.physical_para = t.physical_para;
     tv.sliding_force = t.sliding_force;
+    tv.sink = fric_sink(t, counters);
     double T = P.T[i];
-    if (!fric_heat_v21(c, tv, *t.kin, P.eps_pl[i], P.ft[i], P.on_seg[i], P.n[i], P.vel[i], P.fc[i], P.rho[i], T))
+    if (!fric_heat_v21(c, tv, *t.kin, i, P.eps_pl[i], P.ft[i], P.on_seg[i], P.n[i], P.vel[i], P.fc[i], P.rho[i], T))
         P.blanked[i] = 1.;
     P.T[i] = T;
 }
 
 void launch_fric_heat(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t) {
-    KTimer tm(ctx, "fric_heat");
-    k_fric_heat<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, *t);
+    launch_fill_int(ctx, ctx->d_flags + SPHFEM_FLAG_SINK_FRIC, 1, 0);
+    {
+        KTimer tm(ctx, "fric_heat");
+        k_fric_heat<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, *t, ctx->d_flags);
+    }
+    launch_fric_heat_sums(ctx, t);
 }
 
 // compute_contact_forces_FEM_tool_v3 step 1 (actions_gpu.cu:423-583) over all particles
@@ -589,8 +605,8 @@ __global__ void __launch_bounds__(256) k_contact_s1(DevConsts c, sphfem_particle
 }
 
 // step 2 (actions_gpu.cu:585-617) for one particle
-__device__ __forceinline__ void contact_s2_one(const DevConsts &c, const sphfem_tool &t, int seg_no, double rho,
-                                               double &T) {
+__device__ __forceinline__ void contact_s2_one(const DevConsts &c, const sphfem_tool &t, const SegSink &sink, int pidx,
+                                               int seg_no, double rho, double &T) {
     sphfem_int4 mk = t.marks[seg_no];
     double T_ = T;
     double T_seg = 0.5 * (t.T[mk.z] + t.T[mk.w]);
@@ -598,21 +614,105 @@ __device__ __forceinline__ void contact_s2_one(const DevConsts &c, const sphfem_
     double length = sqrtf_d(c.tc.slave_mass / rho);
     const double hv[2] = {T_ * length, length};
     double *const hd[2] = {&t.heat_exchange[0].x, &t.heat_exchange[0].y};
-    wave_atomic_add<2>(true, seg_no, hv, hd, 2);
+    const int hs[2] = {2, 2};
+    seg_sink_add<2>(sink, true, seg_no, pidx, hv, hd, hs);
     double dTp = delta_T * c.tc.h * length / c.trml_wp.cp / c.tc.slave_mass * c.dt;
     if (dTp < 0) T += fmax(dTp, delta_T);
     else T += fmin(dTp, delta_T);
 }
 
-__global__ void __launch_bounds__(256) k_contact_s2(DevConsts c, sphfem_particles P, sphfem_tool t) {
+__global__ void __launch_bounds__(256) k_contact_s2(DevConsts c, sphfem_particles P, sphfem_tool t, int *counters) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int)P.N) return;
     if (P.blanked[i] == 1.) return;
     int seg_no = P.on_seg[i];
     if (seg_no == -1) return;
     double T = P.T[i];
-    contact_s2_one(c, t, seg_no, P.rho[i], T);
+    contact_s2_one(c, t, s2_sink(t, counters), i, seg_no, P.rho[i], T);
     P.T[i] = T;
+}
+
+// The ordered segment sums: sort the records (segment << 32 | particle index, unique) in LDS and add
+// each segment's values in increasing particle index to the segment arrays (dst[q][seg * stride[q]]).
+struct SegDst {
+    double *p[6];
+    int stride[6];
+    int nv;
+};
+
+__global__ void __launch_bounds__(1024) k_segment_reduce(SegSink s, SegDst d) {
+    __shared__ unsigned long long sk[SPHFEM_SEG_SINK_CAP];
+    __shared__ int sl[SPHFEM_SEG_SINK_CAP];
+    const int n = min(*s.count, s.cap);
+    if (n <= 0) return;
+    int np = 1;
+    while (np < n) np <<= 1;
+    for (int k = threadIdx.x; k < np; k += blockDim.x) {
+        sk[k] = k < n ? s.keys[k] : ~0ull;
+        sl[k] = k;
+    }
+    __syncthreads();
+    for (int size = 2; size <= np; size <<= 1) {          // bitonic sort of (key, slot), ascending
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int k = threadIdx.x; k < np; k += blockDim.x) {
+                const int j = k ^ stride;
+                if (j > k) {
+                    const bool up = (k & size) == 0;
+                    const unsigned long long a = sk[k], b = sk[j];
+                    if ((a > b) == up) {
+                        sk[k] = b;
+                        sk[j] = a;
+                        const int x = sl[k];
+                        sl[k] = sl[j];
+                        sl[j] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const unsigned seg = (unsigned)(sk[k] >> 32);
+        if (k > 0 && (unsigned)(sk[k - 1] >> 32) == seg) continue;     // not the first of its segment
+        for (int q = 0; q < d.nv; q++) {
+            double *dst = d.p[q] + (size_t)seg * d.stride[q];
+            double acc = *dst;
+            for (int r = k; r < n && (unsigned)(sk[r] >> 32) == seg; r++) acc += s.vals[(size_t)q * s.cap + sl[r]];
+            *dst = acc;
+        }
+    }
+}
+
+static SegSink host_sink(sphfem_ctx *ctx, sphfem_tool *t, bool s2) {
+    if (s2)
+        return SegSink{ctx->d_flags + SPHFEM_FLAG_SINK_S2, t->sink_keys + SPHFEM_SEG_SINK_CAP,
+                       t->sink_vals + 6ul * SPHFEM_SEG_SINK_CAP, SPHFEM_SEG_SINK_CAP};
+    return SegSink{ctx->d_flags + SPHFEM_FLAG_SINK_FRIC, t->sink_keys, t->sink_vals, SPHFEM_SEG_SINK_CAP};
+}
+
+void launch_fric_heat_sums(sphfem_ctx *ctx, sphfem_tool *t) {
+    SegDst d{};
+    d.nv = 6;
+    d.p[0] = t->fric_heat;
+    d.p[1] = &t->physical_para[0].x;
+    d.p[2] = &t->physical_para[0].y;
+    d.p[3] = &t->physical_para[0].z;
+    d.p[4] = &t->physical_para[0].w;
+    d.p[5] = t->sliding_force;
+    const int st[6] = {1, 4, 4, 4, 4, 1};
+    for (int q = 0; q < 6; q++) d.stride[q] = st[q];
+    KTimer tm(ctx, "fric_heat_sums");
+    k_segment_reduce<<<1, 1024, 0, ctx->stream>>>(host_sink(ctx, t, false), d);
+}
+
+static void launch_s2_sums(sphfem_ctx *ctx, sphfem_tool *t) {
+    SegDst d{};
+    d.nv = 2;
+    d.p[0] = &t->heat_exchange[0].x;
+    d.p[1] = &t->heat_exchange[0].y;
+    d.stride[0] = d.stride[1] = 2;
+    KTimer tm(ctx, "contact_s2_sums");
+    k_segment_reduce<<<1, 1024, 0, ctx->stream>>>(host_sink(ctx, t, true), d);
 }
 
 void launch_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t) {
@@ -620,8 +720,12 @@ void launch_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t) 
         KTimer tm(ctx, "contact_s1");
         k_contact_s1<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, *t);
     }
-    KTimer tm(ctx, "contact_s2");
-    k_contact_s2<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, *t);
+    launch_fill_int(ctx, ctx->d_flags + SPHFEM_FLAG_SINK_S2, 1, 0);
+    {
+        KTimer tm(ctx, "contact_s2");
+        k_contact_s2<<<nblk(p.N, 256), 256, 0, ctx->stream>>>(ctx->c, p, *t, ctx->d_flags);
+    }
+    launch_s2_sums(ctx, t);
 }
 
 // Near-tool particles of the fused step: contact s1, mechanical boundary condition, NaN check;
@@ -703,13 +807,19 @@ __global__ void __launch_bounds__(256) k_near_contact(DevConsts c, sphfem_partic
 }
 
 __global__ void __launch_bounds__(256) k_contact_list_s2(DevConsts c, sphfem_particles P, sphfem_tool t,
-                                                         const int *counters, const int *contact_list) {
+                                                         int *counters, const int *contact_list) {
     const int count = counters[2];
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+    const SegSink sink = s2_sink(t, counters);
+    // a wave-uniform trip count: the record slots are allocated with wave ballots
+    const int stride = gridDim.x * blockDim.x;
+    const int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+    for (int q0 = base; q0 < count; q0 += stride) {
+        const int q = q0 + (threadIdx.x & 63);
+        if (q >= count) break;
         const int code = contact_list[q];
         const int i = code & ((1 << 30) - 1);
         double T = P.T[i];
-        contact_s2_one(c, t, P.on_seg[i], P.rho[i], T);
+        contact_s2_one(c, t, sink, i, P.on_seg[i], P.rho[i], T);
         if (c.thermal_bc && P.fixed[i] == 1.) T = c.trml_wp.T_init;
         P.T[i] = T;
         if (code & (1 << 30)) P.blanked[i] = 1.;
@@ -725,9 +835,12 @@ void launch_near_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool
         k_near_contact<<<blocks, 256, 0, ctx->stream>>>(ctx->c, p, *t, lf->near_list, ctx->d_flags,
                                                         lf->contact_list, r32, use32 ? 1 : 0);
     }
-    KTimer tm(ctx, "contact_list_s2");
-    k_contact_list_s2<<<std::max(1, std::min(blocks, 64)), 256, 0, ctx->stream>>>(ctx->c, p, *t, ctx->d_flags,
-                                                                                 lf->contact_list);
+    {
+        KTimer tm(ctx, "contact_list_s2");
+        k_contact_list_s2<<<std::max(1, std::min(blocks, 64)), 256, 0, ctx->stream>>>(ctx->c, p, *t, ctx->d_flags,
+                                                                                     lf->contact_list);
+    }
+    launch_s2_sums(ctx, t);
 }
 
 }  // namespace sphfem

@@ -128,7 +128,7 @@ def test_fric_heat_nan_velocity_branch(world):
     assert_equal("T", sim.p.download("T"), o.P["T"])
     tg = sim.tool.download()
     for f in ("fric_heat", "physical_para", "sliding_force"):
-        assert_close(f, tg[f], o.t[f], 1e-13)
+        assert_equal(f, tg[f], o.t[f])
 
 
 def test_interactions_skip_blanked_neighbours(world):

@@ -3,9 +3,9 @@
 Every stage of the reference step (leap_frog.cu:162-248) is run on the GPU from exactly
 the oracle's state before that stage, and compared with the oracle's state after it.
 Stages whose arithmetic is the same IEEE operations in the same order must match
-bit for bit; the tolerances below name the few places where results depend on the
-device's libm (atan2f/cosf/sinf in the artificial stress, pow/log/exp in Johnson-Cook
-and Usui) or on the order of atomic sums into the tool segments.
+bit for bit -- the tool's segment sums included, which both sides add in increasing particle
+index; the tolerances below name the few places where results depend on the device's libm
+(atan2f/cosf/sinf in the artificial stress, pow/log/exp in Johnson-Cook and Usui).
 """
 import copy
 
@@ -19,7 +19,6 @@ from sph_fem_2d_cutting_gpu_amd import solver as S
 
 pytestmark = pytest.mark.gpu
 
-ATOMIC_RTOL = 1e-13    # sums of the same terms in another order
 LIBM_RTOL = 1e-6       # float trig of the artificial stress (ulp-level differences, relative to max|R|)
 POW_RTOL = 1e-10       # double pow/log/exp of Johnson-Cook / Usui
 PSE_RTOL = 1e-12       # double exp of the PSE kernel, summed in the same order
@@ -195,7 +194,7 @@ def test_stage_bitexact(world, stage):
     if stage in ("contact", "fric_heat"):
         tg = sim.tool.download()
         for f in ("heat_exchange", "fric_heat", "physical_para", "sliding_force"):
-            assert_close(f, tg[f], o.t[f], ATOMIC_RTOL)
+            assert_equal(f, tg[f], o.t[f])
 
 
 def test_stage_heat_pse(world):
