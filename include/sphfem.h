@@ -353,6 +353,14 @@ int sphfem_leap_frog_graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_
 /* the same for sphfem_leap_frog_step_rigid (rt may be NULL) */
 int sphfem_leap_frog_graph_create_rigid(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p,
                                         sphfem_grid *g, sphfem_rigid_tool *rt, int steps, sphfem_graph **out);
+/* Independent cases (a parameter sweep) in one HIP graph: `steps` (even) steps of every case k --
+ * its own context ctxs[k] (stream and constants), stepper, particles, locked grid and FEM tool (ts
+ * may be NULL, or hold NULL entries, for tool-free cases) -- recorded as parallel branches of a
+ * single graph, which sphfem_leap_frog_graph_launch(ctxs[0], ...) replays: one launch advances all
+ * cases.  Every case needs its own context and one eager step first; the same settings checks as
+ * for a step graph apply to every case.  No counterpart in the reference. */
+int sphfem_batch_graph_create(sphfem_ctx *const *ctxs, sphfem_leap_frog *const *lfs, sphfem_particles *const *ps,
+                              sphfem_grid *const *gs, sphfem_tool *const *ts, int n, int steps, sphfem_graph **out);
 int sphfem_leap_frog_graph_launch(sphfem_ctx *ctx, sphfem_graph *graph, int repeats);
 int sphfem_leap_frog_graph_destroy(sphfem_ctx *ctx, sphfem_graph *graph);
 

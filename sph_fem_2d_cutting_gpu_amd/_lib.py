@@ -134,6 +134,7 @@ _SIGS = {
     "sphfem_grid_destroy": (_ctx, _vp),
     "sphfem_grid_set_resort_limit": (_ctx, _vp, C.c_int),
     "sphfem_leap_frog_graph_create": (_ctx, _vp, _P(Particles), _vp, _vp, C.c_int, _P(_vp)),
+    "sphfem_batch_graph_create": (_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, _P(_vp)),
     "sphfem_leap_frog_graph_launch": (_ctx, _vp, C.c_int),
     "sphfem_leap_frog_export_rates": (_ctx, _vp, _P(Particles)),
     "sphfem_leap_frog_import_rates": (_ctx, _vp, _P(Particles)),

@@ -57,29 +57,32 @@ def tune_simulation(sim, precision, steps=6, repeats=2) -> dict:
 
 
 def tune_batch(batch, precision, steps=4, launches=2) -> dict:
-    """Per-case kernel tuning, then the graph depth of the batch (recaptures the graphs)."""
+    """Per-case kernel tuning, then how the batch is replayed: one graph holding every case as a
+    branch, or one graph per case queued on the cases' own streams, each at graph depths
+    GRAPH_STEPS (recaptures the graphs; the fastest is left captured)."""
     per_case = [tune_simulation(s, precision, steps=steps, repeats=1)["best"] for s in batch.sims]
     timings = {}
-    for gs in GRAPH_STEPS:
-        if batch.graphs:
-            for g in batch.graphs:
-                g.destroy()
-        batch.graph_steps = gs
-        batch.capture()
-        batch.launch(1)
-        batch.synchronize()
-        reps = max(1, (launches * max(GRAPH_STEPS)) // gs)
-        t0 = time.perf_counter()
-        batch.launch(reps)
-        batch.synchronize()
-        timings[gs] = (time.perf_counter() - t0) / (reps * gs)
-    best = min(timings, key=timings.get)
-    for g in batch.graphs:
-        g.destroy()
+    for one in (True, False):
+        for gs in GRAPH_STEPS:
+            batch.release_graphs()
+            batch.graph_steps = gs
+            batch.capture(one_graph=one)
+            batch.launch(1)
+            batch.synchronize()
+            reps = max(1, (launches * max(GRAPH_STEPS)) // gs)
+            t0 = time.perf_counter()
+            batch.launch(reps)
+            batch.synchronize()
+            timings[(one, gs)] = (time.perf_counter() - t0) / (reps * gs)
+    one, best = min(timings, key=timings.get)
+    batch.release_graphs()
     batch.graph_steps = best
-    batch.capture()
+    batch.capture(one_graph=one)
     batch.check_error()
-    return {"graph_steps": best, "ms_per_step": {k: v * 1e3 for k, v in timings.items()}, "per_case": per_case}
+    return {"graph_steps": best, "one_graph": one,
+            "ms_per_step": {f"{'one graph' if o else 'graph per case'}, {g} steps": v * 1e3
+                            for (o, g), v in timings.items()},
+            "per_case": per_case}
 
 
 def cache_key(n_particles, precision, device_name=""):

@@ -7,7 +7,9 @@
 #include <map>
 #include <mutex>
 #include <stdexcept>
+#include <string>
 #include <utility>
+#include <vector>
 
 #include "sphfem_internal.h"
 #include "sphfem_rigid.h"
@@ -1747,14 +1749,13 @@ int sphfem_leap_frog_step_rigid(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_pa
 
 }  // extern "C"
 
-struct sphfem_graph {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int steps = 0;
-    sphfem_tool *tool = nullptr;       // the FEM tool captured and its array generation then
+// what the recorded launches of one case baked in: the constant blocks (passed by value: dt, wear
+// switch, cut-off, geometry, ...), the stepper's precision, tuning and intermediates switch, and the
+// FEM tool's array generation
+struct GraphCase {
+    sphfem_ctx *ctx = nullptr;
+    sphfem_tool *tool = nullptr;
     unsigned long tool_gen = 0;
-    // what the recorded launches baked in: the constant blocks (passed by value: dt, wear switch,
-    // cut-off, geometry, ...) and the stepper's precision, tuning and intermediates switch
     DevConsts consts{};
     sphfem_leap_frog *lf = nullptr;
     int precision = 0;
@@ -1762,12 +1763,36 @@ struct sphfem_graph {
     bool intermediates = false;
 };
 
+struct sphfem_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int steps = 0;
+    std::vector<GraphCase> cases;     // one for a step graph, one per case for a batch graph
+    // batch graph: a launch on the first context's stream first waits for the work queued on the
+    // other cases' streams (join[k]), and those streams then wait for the launch (done)
+    std::vector<hipEvent_t> join;
+    hipEvent_t done = nullptr;
+};
+
+static GraphCase graph_case(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_tool *t) {
+    GraphCase g;
+    g.ctx = ctx;
+    g.tool = t;
+    g.tool_gen = t ? t->generation : 0;
+    g.consts = ctx->c;
+    g.lf = lf;
+    g.precision = lf->precision;
+    g.tuning = lf->tuning;
+    g.intermediates = lf->intermediates;
+    return g;
+}
+
 // the settings a graph was captured with still hold (same values, whatever was called since)
-static bool graph_settings_hold(const sphfem_ctx *ctx, const sphfem_graph *gr) {
-    const sphfem_leap_frog *lf = gr->lf;
-    return std::memcmp(&ctx->c, &gr->consts, sizeof(DevConsts)) == 0 && lf->precision == gr->precision &&
-           lf->tuning.pair_block == gr->tuning.pair_block && lf->tuning.pair_chunk == gr->tuning.pair_chunk &&
-           lf->tuning.update_block == gr->tuning.update_block && lf->intermediates == gr->intermediates;
+static bool graph_settings_hold(const GraphCase &g) {
+    const sphfem_leap_frog *lf = g.lf;
+    return std::memcmp(&g.ctx->c, &g.consts, sizeof(DevConsts)) == 0 && lf->precision == g.precision &&
+           lf->tuning.pair_block == g.tuning.pair_block && lf->tuning.pair_chunk == g.tuning.pair_chunk &&
+           lf->tuning.update_block == g.tuning.update_block && lf->intermediates == g.intermediates;
 }
 
 extern "C" {
@@ -1787,8 +1812,6 @@ static int graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles 
         ctx->timing.enabled = false;
         auto *gr = new sphfem_graph();
         gr->steps = steps;
-        gr->tool = t;
-        gr->tool_gen = t ? t->generation : 0;
         try {
             SPHFEM_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             // recording only: the device state does not advance; after an even number of steps the
@@ -1796,11 +1819,7 @@ static int graph_create(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles 
             for (int k = 0; k < steps; k++) (void)step_body(ctx, lf, p, g, t, rt);   // locked: cannot fail
             SPHFEM_HIP_CHECK(hipStreamEndCapture(ctx->stream, &gr->graph));
             SPHFEM_HIP_CHECK(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0));
-            gr->consts = ctx->c;
-            gr->lf = lf;
-            gr->precision = lf->precision;
-            gr->tuning = lf->tuning;
-            gr->intermediates = lf->intermediates;
+            gr->cases.push_back(graph_case(ctx, lf, t));
         } catch (...) {
             hipGraph_t dead = nullptr;
             (void)hipStreamEndCapture(ctx->stream, &dead);
@@ -1826,16 +1845,103 @@ int sphfem_leap_frog_graph_create_rigid(sphfem_ctx *ctx, sphfem_leap_frog *lf, s
     return graph_create(ctx, lf, p, g, nullptr, rt, steps, out);
 }
 
+int sphfem_batch_graph_create(sphfem_ctx *const *ctxs, sphfem_leap_frog *const *lfs, sphfem_particles *const *ps,
+                              sphfem_grid *const *gs, sphfem_tool *const *ts, int n, int steps, sphfem_graph **out) {
+    if (!ctxs || !lfs || !ps || !gs || !out || n < 1 || !ctxs[0]) return SPHFEM_ERR_ARG;
+    sphfem_ctx *ctx0 = ctxs[0];
+    if (steps < 2 || steps % 2) return fail(ctx0, SPHFEM_ERR_ARG, "graph steps must be even (pointer swaps)");
+    *out = nullptr;
+    for (int k = 0; k < n; k++) {
+        sphfem_ctx *c = ctxs[k];
+        sphfem_leap_frog *lf = lfs[k];
+        sphfem_particles *p = ps[k];
+        sphfem_grid *g = gs[k];
+        if (!c || !lf || !p || !g || (int)p->N != g->num_part || p->N != lf->num_part)
+            return fail(ctx0, SPHFEM_ERR_ARG, "batch graph: case " + std::to_string(k) + " is incomplete");
+        if (c->device != ctx0->device) return fail(ctx0, SPHFEM_ERR_ARG, "batch graph: cases on different devices");
+        for (int j = 0; j < k; j++)
+            if (ctxs[j] == c) return fail(ctx0, SPHFEM_ERR_ARG, "batch graph: each case needs its own context");
+        sphfem_tool *t = ts ? ts[k] : nullptr;
+        if (need(c, SET_PHYS | SET_CORR | SET_DT | (t ? SET_TOOLC | SET_TRML_WP : 0u)) != SPHFEM_OK)
+            return fail(ctx0, SPHFEM_ERR_STATE, "batch graph: case " + std::to_string(k) + " lacks constants");
+        if (!g->locked || g->first_sort)
+            return fail(ctx0, SPHFEM_ERR_STATE,
+                        "batch graph: every case needs a locked grid and one eager step before capture");
+    }
+    return guarded(ctx0, [&] {
+        for (int k = 0; k < n; k++) sync_rate_form(ctxs[k], lfs[k], *ps[k]);
+        std::vector<bool> timing((size_t)n);
+        for (int k = 0; k < n; k++) {
+            timing[(size_t)k] = ctxs[k]->timing.enabled;
+            ctxs[k]->timing.enabled = false;
+        }
+        auto *gr = new sphfem_graph();
+        gr->steps = steps;
+        std::vector<hipEvent_t> ev((size_t)n + 1, nullptr);
+        try {
+            for (auto &e : ev) SPHFEM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            // every case's steps as one branch: the other streams join the capture of the first
+            // context's stream through an event and it waits for all of them at the end
+            SPHFEM_HIP_CHECK(hipStreamBeginCapture(ctx0->stream, hipStreamCaptureModeThreadLocal));
+            SPHFEM_HIP_CHECK(hipEventRecord(ev[(size_t)n], ctx0->stream));
+            for (int k = 1; k < n; k++) SPHFEM_HIP_CHECK(hipStreamWaitEvent(ctxs[k]->stream, ev[(size_t)n], 0));
+            for (int k = 0; k < n; k++)
+                for (int st = 0; st < steps; st++)
+                    (void)step_body(ctxs[k], lfs[k], ps[k], gs[k], ts ? ts[k] : nullptr);   // locked: cannot fail
+            for (int k = 1; k < n; k++) {
+                SPHFEM_HIP_CHECK(hipEventRecord(ev[(size_t)k], ctxs[k]->stream));
+                SPHFEM_HIP_CHECK(hipStreamWaitEvent(ctx0->stream, ev[(size_t)k], 0));
+            }
+            SPHFEM_HIP_CHECK(hipStreamEndCapture(ctx0->stream, &gr->graph));
+            SPHFEM_HIP_CHECK(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0));
+            for (int k = 0; k < n; k++) gr->cases.push_back(graph_case(ctxs[k], lfs[k], ts ? ts[k] : nullptr));
+            gr->join.assign((size_t)n, nullptr);
+            for (int k = 1; k < n; k++) SPHFEM_HIP_CHECK(hipEventCreateWithFlags(&gr->join[(size_t)k], hipEventDisableTiming));
+            SPHFEM_HIP_CHECK(hipEventCreateWithFlags(&gr->done, hipEventDisableTiming));
+        } catch (...) {
+            for (auto e : gr->join)
+                if (e) (void)hipEventDestroy(e);
+            if (gr->done) (void)hipEventDestroy(gr->done);
+            hipGraph_t dead = nullptr;
+            (void)hipStreamEndCapture(ctx0->stream, &dead);
+            if (dead) (void)hipGraphDestroy(dead);
+            if (gr->graph) (void)hipGraphDestroy(gr->graph);
+            delete gr;
+            for (auto e : ev)
+                if (e) (void)hipEventDestroy(e);
+            for (int k = 0; k < n; k++) ctxs[k]->timing.enabled = timing[(size_t)k];
+            throw;
+        }
+        for (auto e : ev) (void)hipEventDestroy(e);
+        for (int k = 0; k < n; k++) ctxs[k]->timing.enabled = timing[(size_t)k];
+        *out = gr;
+        return SPHFEM_OK;
+    });
+}
+
 int sphfem_leap_frog_graph_launch(sphfem_ctx *ctx, sphfem_graph *gr, int repeats) {
     if (!gr || repeats < 0) return SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {
-        if (gr->tool && gr->tool->generation != gr->tool_gen)
-            return fail(ctx, SPHFEM_ERR_STATE, "the tool was remeshed after this graph was captured: capture again");
-        if (!graph_settings_hold(ctx, gr))
-            return fail(ctx, SPHFEM_ERR_STATE,
-                        "constants, dt, wear switch, precision or tuning changed after this graph was captured: "
-                        "capture again");
+        if (gr->cases.empty() || gr->cases[0].ctx != ctx)
+            return fail(ctx, SPHFEM_ERR_ARG, "launch a graph on the (first) context it was captured with");
+        for (const GraphCase &g : gr->cases) {
+            if (g.tool && g.tool->generation != g.tool_gen)
+                return fail(ctx, SPHFEM_ERR_STATE, "the tool was remeshed after this graph was captured: capture again");
+            if (!graph_settings_hold(g))
+                return fail(ctx, SPHFEM_ERR_STATE,
+                            "constants, dt, wear switch, precision or tuning changed after this graph was captured: "
+                            "capture again");
+        }
+        const size_t nc = gr->cases.size();
+        for (size_t k = 1; k < nc; k++) {       // batch: after what the other cases' streams hold
+            SPHFEM_HIP_CHECK(hipEventRecord(gr->join[k], gr->cases[k].ctx->stream));
+            SPHFEM_HIP_CHECK(hipStreamWaitEvent(ctx->stream, gr->join[k], 0));
+        }
         for (int r = 0; r < repeats; r++) SPHFEM_HIP_CHECK(hipGraphLaunch(gr->exec, ctx->stream));
+        if (nc > 1) {                            // and before what they are given next
+            SPHFEM_HIP_CHECK(hipEventRecord(gr->done, ctx->stream));
+            for (size_t k = 1; k < nc; k++) SPHFEM_HIP_CHECK(hipStreamWaitEvent(gr->cases[k].ctx->stream, gr->done, 0));
+        }
         return SPHFEM_OK;
     });
 }
@@ -1846,6 +1952,9 @@ int sphfem_leap_frog_graph_destroy(sphfem_ctx *ctx, sphfem_graph *gr) {
         SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
         if (gr->graph) (void)hipGraphDestroy(gr->graph);
+        for (auto e : gr->join)
+            if (e) (void)hipEventDestroy(e);
+        if (gr->done) (void)hipEventDestroy(gr->done);
         delete gr;
         return SPHFEM_OK;
     });

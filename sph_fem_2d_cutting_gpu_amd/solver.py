@@ -914,16 +914,18 @@ class Simulation:
 
 
 class BatchedSimulation:
-    """Independent cases advanced together: one context (HIP stream + constants) and one captured
-    two-step graph per case.  `launch` queues one graph replay per case without synchronising, so
-    the cases run concurrently on the GPU; `launch_sequential` runs them one after another (the
-    baseline a per-case loop would give)."""
+    """Independent cases advanced together, each with its own context (HIP stream + constants).
+    `capture` records `graph_steps` steps of every case into ONE HIP graph (the cases are parallel
+    branches of it, sphfem_batch_graph_create), so one `launch` advances the whole sweep;
+    `capture(one_graph=False)` records one graph per case instead (queued case by case).
+    `launch_sequential` runs the cases one after another (the baseline a per-case loop would give)."""
 
     def __init__(self, case_list, precision=_lib.FP64, device=0, graph_steps=2):
         self.cases = list(case_list)
         self.sims = [Simulation(c, precision=precision, device=device) for c in self.cases]
         self.graph_steps = graph_steps
-        self.graphs = None
+        self.graph = None          # the batch graph
+        self.graphs = None         # per-case graphs (one_graph=False, and launch_sequential)
         self.total_particles = sum(c.N for c in self.cases)
 
     def step(self, n: int = 1):
@@ -932,17 +934,35 @@ class BatchedSimulation:
             for s in self.sims:
                 s.step(1)
 
-    def capture(self):
-        """One graph per case (each case must have run an eager step)."""
-        self.graphs = [s.capture(self.graph_steps) for s in self.sims]
+    def capture(self, one_graph: bool = True):
+        """Record the graph(s); every case must have run an eager step."""
+        if one_graph:
+            n = len(self.sims)
+            arr = lambda vals: (C.c_void_p * n)(*vals)
+            ctxs = arr([s.ctx.h.value for s in self.sims])
+            lfs = arr([s.stepper.h.value for s in self.sims])
+            ps = arr([C.addressof(s.p.s) for s in self.sims])
+            gs = arr([s.grid.h.value for s in self.sims])
+            ts = arr([s.tool.h.value if s.tool else None for s in self.sims])
+            gh = C.c_void_p()
+            check(lib().sphfem_batch_graph_create(ctxs, lfs, ps, gs, ts, n, int(self.graph_steps), C.byref(gh)),
+                  self.sims[0].ctx.h, "sphfem_batch_graph_create")
+            self.graph = StepGraph(self.sims[0].ctx, gh, self.graph_steps)
+        else:
+            self.graphs = [s.capture(self.graph_steps) for s in self.sims]
 
     def launch(self, repeats: int = 1):
         """Advance every case by repeats * graph_steps steps, all cases in flight together."""
+        if self.graph is not None:
+            self.graph.launch(repeats)
+            return
         for _ in range(repeats):
             for g in self.graphs:
                 g.launch(1)
 
     def launch_sequential(self, repeats: int = 1):
+        if self.graphs is None:
+            self.graphs = [s.capture(self.graph_steps) for s in self.sims]
         for s, g in zip(self.sims, self.graphs):
             g.launch(repeats)
             s.ctx.synchronize()
@@ -955,10 +975,16 @@ class BatchedSimulation:
         for s in self.sims:
             s.ctx.check_error()
 
-    def close(self):
+    def release_graphs(self):
+        if self.graph is not None:
+            self.graph.destroy()
+            self.graph = None
         if self.graphs:
             for g in self.graphs:
                 g.destroy()
             self.graphs = None
+
+    def close(self):
+        self.release_graphs()
         for s in self.sims:
             s.close()
