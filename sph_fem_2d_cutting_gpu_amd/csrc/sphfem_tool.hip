@@ -641,8 +641,9 @@ struct SegDst {
 };
 
 __global__ void __launch_bounds__(1024) k_segment_reduce(SegSink s, SegDst d) {
-    __shared__ unsigned long long sk[SPHFEM_SEG_SINK_CAP];
-    __shared__ int sl[SPHFEM_SEG_SINK_CAP];
+    __shared__ unsigned long long sk[SPHFEM_SEG_SINK_CAP];   // keys; after the sort, one value per record
+    __shared__ int sl[SPHFEM_SEG_SINK_CAP];                  // record slots
+    __shared__ int sseg[SPHFEM_SEG_SINK_CAP];                // segments in sorted order
     const int n = min(*s.count, s.cap);
     if (n <= 0) return;
     int np = 1;
@@ -671,15 +672,23 @@ __global__ void __launch_bounds__(1024) k_segment_reduce(SegSink s, SegDst d) {
             __syncthreads();
         }
     }
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const unsigned seg = (unsigned)(sk[k] >> 32);
-        if (k > 0 && (unsigned)(sk[k - 1] >> 32) == seg) continue;     // not the first of its segment
-        for (int q = 0; q < d.nv; q++) {
+    for (int k = threadIdx.x; k < n; k += blockDim.x) sseg[k] = (int)(unsigned)(sk[k] >> 32);
+    __syncthreads();
+    double *vbuf = reinterpret_cast<double *>(sk);
+    for (int q = 0; q < d.nv; q++) {
+        // this value of every record, gathered in sorted order, then each segment's run added in
+        // order by the run's first thread (from LDS: long runs stay cheap)
+        for (int k = threadIdx.x; k < n; k += blockDim.x) vbuf[k] = s.vals[(size_t)q * s.cap + sl[k]];
+        __syncthreads();
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            const int seg = sseg[k];
+            if (k > 0 && sseg[k - 1] == seg) continue;          // not the first of its segment
             double *dst = d.p[q] + (size_t)seg * d.stride[q];
             double acc = *dst;
-            for (int r = k; r < n && (unsigned)(sk[r] >> 32) == seg; r++) acc += s.vals[(size_t)q * s.cap + sl[r]];
+            for (int r = k; r < n && sseg[r] == seg; r++) acc += vbuf[r];
             *dst = acc;
         }
+        __syncthreads();
     }
 }
 
