@@ -294,7 +294,29 @@ struct FEM_thermal_2D_GPU {
     double *m_T_mapping = nullptr;
 };
 
+// mesh_structure_GPU.h:37-63: host copies of a mesh read from an .inp file (mesh_read fills them;
+// int3 flags of the reference are the x, y, z of an int4 here, w unused)
+struct elements_read {
+    std::vector<sphfem_int4> node_num;                 // 3 node ids + colour
+    std::vector<sphfem_int4> cutting_boundary_flags, fixed_boundary_flags;
+    std::vector<int> cutting_boundary_elements, fixed_boundary_elements;
+    std::vector<int> boundary_layers;                  // 2 per element: layer 1, layer 2
+};
+struct nodes_read {
+    std::vector<double> x_init, y_init;
+    std::vector<int> cutting_edge_flag, fixed_edge_flag, row_ptr, col_ind;
+    const sphfem_mesh *source = nullptr;               // the mesh object the arrays were read from
+};
+
 struct mesh_GPU_m {
+    mesh_GPU_m() = default;
+    // mesh_structure_GPU.h:135 (benchmarks_NZ.cpp:241): names the mesh whose device copy the tool
+    // makes in tool_FEM::set_up_FEM_tool_gpu; the views below are filled then
+    mesh_GPU_m(const nodes_read &nodes, const elements_read &elements, int color_number, int cut_edge_col_no,
+               bool /*ini*/)
+        : no_num((int)nodes.x_init.size()), ele_num_total((int)elements.node_num.size()),
+          non_zeros((int)nodes.col_ind.size()), color_total(color_number), cutting_edge_color(cut_edge_col_no),
+          ele_num_fixed((int)elements.fixed_boundary_elements.size()), source(nodes.source) {}
     sphfem_int4 *node_num = nullptr;       // 3 node ids + colour per element
     float2_t *position = nullptr;
     float2_t *position_mapping = nullptr;
@@ -307,6 +329,7 @@ struct mesh_GPU_m {
     int get_cutting_edge_color_number() const { return cutting_edge_color; }
     int get_fixed_edge_element_number() const { return ele_num_fixed; }
     int no_num = 0, ele_num_total = 0, non_zeros = 0, color_total = 0, cutting_edge_color = 0, ele_num_fixed = 0;
+    const sphfem_mesh *source = nullptr;
 };
 
 // mesh_read (mesh_structure_GPU.h:80-142): a tool mesh read from an .inp file on the host --
@@ -321,6 +344,7 @@ class mesh_read {
                                                 &m_handle),
                            ("mesh_read(" + file_name + ")").c_str());
         sphfem_shim::check(sphfem_mesh_get_info(m_handle, &info), "mesh_read info");
+        fill_host_copies();
     }
     ~mesh_read() {
         if (m_handle) (void)sphfem_mesh_destroy(m_handle);
@@ -330,6 +354,8 @@ class mesh_read {
 
     int color_total, color_cutting_edge;
     sphfem_mesh_info info{};
+    nodes_read nodes;              // mesh_structure_GPU.h:82-83
+    elements_read elements;
 
     // mesh_structure_GPU.cu:1259-1296: hot-tool nodal temperatures from a tool VTK file
     void mesh_read_temperature(const std::string &file_name) {
@@ -347,39 +373,61 @@ class mesh_read {
     sphfem_mesh *handle() const { return m_handle; }
 
   private:
+    void fill_host_copies() {
+        const size_t nn = (size_t)info.node_num, ne = (size_t)info.ele_num;
+        std::vector<sphfem_double2> pos(nn);
+        nodes.cutting_edge_flag.resize(nn);
+        nodes.fixed_edge_flag.resize(nn);
+        nodes.row_ptr.resize(nn + 1);
+        nodes.col_ind.resize((size_t)info.nz_num);
+        elements.node_num.resize(ne);
+        elements.cutting_boundary_flags.resize(ne);
+        elements.fixed_boundary_flags.resize(ne);
+        elements.cutting_boundary_elements.resize((size_t)info.cut_ele_num);
+        elements.fixed_boundary_elements.resize((size_t)info.fixed_ele_num);
+        elements.boundary_layers.resize(2 * ne);
+        sphfem_shim::check(sphfem_mesh_get_arrays(m_handle, pos.data(), nodes.cutting_edge_flag.data(),
+                                                  nodes.fixed_edge_flag.data(), elements.node_num.data(),
+                                                  elements.cutting_boundary_flags.data(),
+                                                  elements.fixed_boundary_flags.data(),
+                                                  elements.cutting_boundary_elements.data(),
+                                                  elements.fixed_boundary_elements.data(),
+                                                  elements.boundary_layers.data(), nodes.row_ptr.data(),
+                                                  nodes.col_ind.data()),
+                           "mesh_read arrays");
+        nodes.x_init.resize(nn);
+        nodes.y_init.resize(nn);
+        for (size_t i = 0; i < nn; i++) {
+            nodes.x_init[i] = pos[i].x;
+            nodes.y_init[i] = pos[i].y;
+        }
+        nodes.source = m_handle;
+    }
     sphfem_mesh *m_handle = nullptr;
 };
 
 class tool_FEM : public tool {
   public:
-    // tool_FEM(tl, ...) + mesh_CPU = mesh + set_up_segments_from_mesh() + set_up_FEM_tool_gpu(tc)
-    // (benchmarks_NZ.cpp:228-245, tool_FEM.cu:771-1039, 1539-1544): segments oriented towards
-    // tl_point_ref, FEM system with the tool constants tc
-    tool_FEM(mesh_read &mesh, const sphfem_tool_constants &tc, vec2_t tl_point_ref, vec2_t vel) {
-        sphfem_tool_desc desc{};
-        sphfem_shim::check(sphfem_mesh_tool_desc(mesh.handle(), &tc, sphfem_double2{tl_point_ref.x, tl_point_ref.y},
-                                                 sphfem_double2{vel.x, vel.y}, &desc),
-                           "set_up_segments_from_mesh");
-        sphfem_shim::check(sphfem_tool_create(sphfem_shim::ctx(), &desc, &m_handle), "tool_FEM");
-        node_number = desc.node_num;
-        element_number = desc.ele_num;
-        cutting_segment_size = desc.seg_num;
+    // tool_FEM.cu:623-661, 663-695 (angles in degrees): the tool body's outline and friction
+    // coefficient; tl orients the cutting segments.  The mesh is attached afterwards as
+    // benchmarks_NZ.cpp:228-245 does: mesh_CPU = new mesh_read(file); mesh_GPU = new
+    // mesh_GPU_m(mesh_CPU->nodes, mesh_CPU->elements, ...); set_up_segments_from_mesh();
+    // set_up_FEM_tool_gpu(tc)
+    tool_FEM(vec2_t tl, double length, double height, double rake_angle, double clearance_angle, double r,
+             double mu_fric)
+        : tool(tl, length, height, rake_angle, clearance_angle, r, mu_fric), tl_point_ref(tl) {}
+    tool_FEM(vec2_t tl, double length, double height, double rake_angle, double clearance_angle, double mu_fric)
+        : tool(tl, length, height, rake_angle, clearance_angle, mu_fric), tl_point_ref(tl) {}
+    // the same four steps in one call: segments oriented towards tl_point_ref, FEM system with the
+    // tool constants tc
+    tool_FEM(mesh_read &mesh, const sphfem_tool_constants &tc, vec2_t tl_point_ref_, vec2_t vel)
+        : tl_point_ref(tl_point_ref_) {
         m_velocity = vel;
-        right_point = desc.right_point;
-        lowest_point = desc.lowest_point;
-        refresh_views();
+        mesh_CPU = &mesh;
+        set_up_FEM_tool_gpu(tc);
     }
     // the device data of set_up_segments_from_mesh + set_up_FEM_tool_gpu, from a host mesh
-    explicit tool_FEM(const sphfem_tool_desc &desc) {
-        sphfem_shim::check(sphfem_tool_create(sphfem_shim::ctx(), &desc, &m_handle), "tool_FEM");
-        node_number = desc.node_num;
-        element_number = desc.ele_num;
-        cutting_segment_size = desc.seg_num;
-        m_velocity = vec2_t(desc.vel.x, desc.vel.y);
-        right_point = desc.right_point;
-        lowest_point = desc.lowest_point;
-        refresh_views();
-    }
+    explicit tool_FEM(const sphfem_tool_desc &desc) { create(desc); }
     ~tool_FEM() override {
         if (m_handle && sphfem_shim::ctx_slot()) (void)sphfem_tool_destroy(sphfem_shim::ctx_slot(), m_handle);
     }
@@ -389,22 +437,45 @@ class tool_FEM : public tool {
     int node_number = 0;
     int element_number = 0;
     int cutting_segment_size = 0;
-    double right_point = 0., lowest_point = 0.;     // at set-up (tool_FEM.h:54-55)
-    segment_FEM_gpu *segments_FEM = &m_seg;         // tool_FEM.h:34-40 members
+    vec2_t tl_point_ref, br_point;                  // tool_FEM.h:38-48, 54-55: host copies, moved by
+    double ref_dist = 0.;                           // tool_update_FEM_gpu like the device's
+    double right_point = 0., lowest_point = 0.;
+    segment_FEM_gpu *segments_FEM = &m_seg;         // tool_FEM.h:34-47 members
     FEM_thermal_2D_GPU *FEM_solver = &m_fem;
     mesh_GPU_m *mesh_GPU = &m_mesh;
+    mesh_read *mesh_CPU = nullptr;                  // set by the caller before set_up_segments_from_mesh
+
+    // tool_FEM.cu:771-1039: cutting segments of mesh_CPU oriented towards tl_point_ref and chained
+    // from the highest point; sets the extent (tl_point_ref, br_point, ref_dist, lowest_point,
+    // right_point) and the counts.  The device arrays are made by set_up_FEM_tool_gpu.
+    void set_up_segments_from_mesh() {
+        const sphfem_tool_constants none{};
+        m_orient = tl_point_ref;
+        m_oriented = true;
+        set_up_host(describe(none));
+    }
+    // tool_FEM.cu:1539-1544: the tool's device data (segments as set up above, FEM system with the
+    // tool constants tc, flank retreat 0); a tool set up before is replaced
+    void set_up_FEM_tool_gpu(const sphfem_tool_constants &tc) { create(describe(tc)); }
 
     void set_vel(vec2_t vel) override {
         m_velocity = vel;
-        sphfem_shim::check(sphfem_tool_set_vel(sphfem_shim::ctx(), m_handle, sphfem_double2{vel.x, vel.y}),
-                           "tool_FEM::set_vel");
+        if (m_handle)
+            sphfem_shim::check(sphfem_tool_set_vel(sphfem_shim::ctx(), m_handle, sphfem_double2{vel.x, vel.y}),
+                               "tool_FEM::set_vel");
     }
     void apply_heat_boundary_condition() {
         sphfem_shim::check(sphfem_tool_apply_heat_boundary_condition(sphfem_shim::ctx(), m_handle),
                            "apply_heat_boundary_condition");
     }
     void tool_update_FEM_gpu(double dt) {
-        sphfem_shim::check(sphfem_tool_update_FEM_gpu(sphfem_shim::ctx(), m_handle, dt), "tool_update_FEM_gpu");
+        sphfem_shim::check(sphfem_tool_update_FEM_gpu(sphfem_shim::ctx(), handle_or_throw(), dt),
+                           "tool_update_FEM_gpu");
+        // tool_FEM.cu:1551-1558
+        tl_point_ref = vec2_t(tl_point_ref.x + m_velocity.x * dt, tl_point_ref.y + m_velocity.y * dt);
+        br_point = vec2_t(br_point.x + m_velocity.x * dt, br_point.y + m_velocity.y * dt);
+        lowest_point += m_velocity.y * dt;
+        right_point += m_velocity.x * dt;
     }
     void segments_update(double dt) {
         sphfem_shim::check(sphfem_tool_segments_update(sphfem_shim::ctx(), m_handle, dt), "segments_update");
@@ -445,6 +516,42 @@ class tool_FEM : public tool {
     }
 
   private:
+    sphfem_tool *handle_or_throw() const {
+        if (!m_handle) throw std::runtime_error("tool_FEM: set_up_FEM_tool_gpu was not called");
+        return m_handle;
+    }
+    sphfem_tool_desc describe(const sphfem_tool_constants &tc) {
+        if (!mesh_CPU) throw std::runtime_error("tool_FEM: mesh_CPU is not set");
+        if (mesh_GPU != &m_mesh && mesh_GPU && mesh_GPU->source && mesh_GPU->source != mesh_CPU->handle())
+            throw std::runtime_error("tool_FEM: mesh_GPU was made from another mesh than mesh_CPU");
+        // the segments are oriented by tl_point_ref as it was when set_up_segments_from_mesh ran
+        // (that call replaces tl_point_ref by the mesh's own corner)
+        const vec2_t o = m_oriented ? m_orient : tl_point_ref;
+        sphfem_tool_desc desc{};
+        sphfem_shim::check(sphfem_mesh_tool_desc(mesh_CPU->handle(), &tc, sphfem_double2{o.x, o.y},
+                                                 sphfem_double2{m_velocity.x, m_velocity.y}, &desc),
+                           "set_up_segments_from_mesh");
+        return desc;
+    }
+    void set_up_host(const sphfem_tool_desc &desc) {
+        node_number = desc.node_num;
+        element_number = desc.ele_num;
+        cutting_segment_size = desc.seg_num;
+        m_velocity = vec2_t(desc.vel.x, desc.vel.y);
+        tl_point_ref = vec2_t(desc.tl_point_ref.x, desc.tl_point_ref.y);
+        br_point = vec2_t(desc.br_point.x, desc.br_point.y);
+        ref_dist = desc.ref_dist;
+        right_point = desc.right_point;
+        lowest_point = desc.lowest_point;
+    }
+    void create(const sphfem_tool_desc &desc) {
+        sphfem_tool *made = nullptr;
+        sphfem_shim::check(sphfem_tool_create(sphfem_shim::ctx(), &desc, &made), "tool_FEM");
+        if (m_handle) (void)sphfem_tool_destroy(sphfem_shim::ctx(), m_handle);
+        m_handle = made;
+        set_up_host(desc);
+        refresh_views();
+    }
     void refresh_views() {
         sphfem_tool_arrays a{};
         sphfem_shim::check(sphfem_tool_get_arrays(m_handle, &a), "tool arrays");
@@ -460,13 +567,16 @@ class tool_FEM : public tool {
         m_fem.m_mass_matrix = a.M; m_fem.m_mass_matrix_inverse = a.M_inv;
         m_fem.m_f = a.f; m_fem.m_f_temp = a.f_temp; m_fem.m_k = a.k; m_fem.m_T = a.T; m_fem.m_T_old = a.T_old;
         m_fem.m_T_mapping = a.T_mapping;
-        m_mesh.node_num = a.node_num_el; m_mesh.position = a.position; m_mesh.position_mapping = a.position_mapping;
-        m_mesh.csr_row_ptr = a.csr_row_ptr; m_mesh.csr_col_ind = a.csr_col_ind; m_mesh.node_flags = a.node_flags;
-        m_mesh.no_num = a.node_num; m_mesh.ele_num_total = a.ele_num; m_mesh.non_zeros = a.nz_num;
-        m_mesh.color_total = a.color_total; m_mesh.cutting_edge_color = a.color_cutting_edge;
-        m_mesh.ele_num_fixed = a.fixed_ele_num;
+        mesh_GPU_m &g = mesh_GPU ? *mesh_GPU : m_mesh;
+        g.node_num = a.node_num_el; g.position = a.position; g.position_mapping = a.position_mapping;
+        g.csr_row_ptr = a.csr_row_ptr; g.csr_col_ind = a.csr_col_ind; g.node_flags = a.node_flags;
+        g.no_num = a.node_num; g.ele_num_total = a.ele_num; g.non_zeros = a.nz_num;
+        g.color_total = a.color_total; g.cutting_edge_color = a.color_cutting_edge;
+        g.ele_num_fixed = a.fixed_ele_num;
     }
     sphfem_tool *m_handle = nullptr;
+    vec2_t m_orient;
+    bool m_oriented = false;
     segment_FEM_gpu m_seg;
     FEM_thermal_2D_GPU m_fem;
     mesh_GPU_m m_mesh;

@@ -64,6 +64,10 @@ def write_case_dir(case: cases.Case, path: str, rr2_min: float | None = None, we
                 raise ValueError("tool_inp must be 'gmsh' or 'abaqus'")
             cases.write_inp(tm, os.path.join(path, "tool.inp"), gmsh=tool_inp == "gmsh")
             sc["tool.inp_gmsh"] = int(tool_inp == "gmsh")
+            if tm.outline is not None:
+                # the tool_FEM(tl, length, height, rake, clearance, r, mu) constructor's arguments
+                for k, v in zip(("length", "height", "rake", "clearance", "r"), tm.outline):
+                    sc[f"tool_outline.{k}"] = float(v)
     else:
         sc["node_num"] = 0
     with open(os.path.join(path, "scalars.txt"), "w") as f:

@@ -254,6 +254,7 @@ class ToolMesh:
     right_point: float
     fixed_elems: np.ndarray | None = None   # fixed-boundary elements and their edge flags (e,3), as read
     fixed_flags: np.ndarray | None = None   # from a mesh file; None: derived from the boundary edges
+    outline: tuple | None = None    # (length, height, rake, clearance, edge radius) of a generated tool
 
 
 def _normalize_glm(v):
@@ -272,7 +273,9 @@ def mesh_tool(length, height, rake_deg, clear_deg, r, target_tris, tc) -> ToolMe
         ntri = 2 * len(_tool_points(length, height, rake_deg, clear_deg, r, s_max, 2.0 ** u)[0])
         if best is None or abs(ntri - target_tris) < abs(best[1] - target_tris):
             best = (u, ntri)
-    return _mesh_tool(length, height, rake_deg, clear_deg, r, s_max, tc, 2.0 ** best[0])
+    tm = _mesh_tool(length, height, rake_deg, clear_deg, r, s_max, tc, 2.0 ** best[0])
+    tm.outline = (float(length), float(height), float(rake_deg), float(clear_deg), float(r))
+    return tm
 
 
 def _tool_points(length, height, rake_deg, clear_deg, r, s_max, root_scale=1.0):

@@ -139,7 +139,22 @@ int main(int argc, char **argv) {
         std::vector<sphfem_int4> t_fflags;
         std::vector<double> t_K, t_M, t_f, t_T;
         const std::string inp = dir + "/tool.inp";
-        if (nn > 0 && std::ifstream(inp).good()) {
+        std::unique_ptr<mesh_GPU_m> mesh_gpu;
+        if (nn > 0 && std::ifstream(inp).good() && s.count("tool_outline.length")) {
+            // benchmarks_NZ.cpp:228-245 step by step
+            tool.reset(new tool_FEM(vec2_t(get(s, "tool.tl_x"), get(s, "tool.tl_y")), get(s, "tool_outline.length"),
+                                    get(s, "tool_outline.height"), get(s, "tool_outline.rake"),
+                                    get(s, "tool_outline.clearance"), get(s, "tool_outline.r"), tc.mu));
+            mesh.reset(new mesh_read(inp, get(s, "tool.inp_gmsh") != 0.));
+            tool->mesh_CPU = mesh.get();
+            mesh_gpu.reset(new mesh_GPU_m(mesh->nodes, mesh->elements, mesh->color_total, mesh->color_cutting_edge, 1));
+            tool->mesh_GPU = mesh_gpu.get();
+            tool->set_up_segments_from_mesh();
+            tool->set_up_FEM_tool_gpu(tc);
+            tool->set_vel(vec2_t(get(s, "tool.vel_x"), get(s, "tool.vel_y")));
+            std::printf("tool mesh %s (tool_FEM set-up sequence): %d nodes, %d elements, %d cutting segments\n",
+                        inp.c_str(), tool->node_number, tool->element_number, tool->cutting_segment_size);
+        } else if (nn > 0 && std::ifstream(inp).good()) {
             mesh.reset(new mesh_read(inp, get(s, "tool.inp_gmsh") != 0.));
             tool.reset(new tool_FEM(*mesh, tc, vec2_t(get(s, "tool.tl_x"), get(s, "tool.tl_y")),
                                     vec2_t(get(s, "tool.vel_x"), get(s, "tool.vel_y"))));
