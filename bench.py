@@ -58,6 +58,10 @@ def make_case(name):
         return cases.config2(), "aisi1045_160k_cutting: 800x200 lattice dx=1.25um, ~8k-triangle FEM tool"
     if name == "ti64_6400":
         return cases.config1(), "ti64_6400_cutting: 160x40 lattice dx=2.5um, ~2k-triangle FEM tool"
+    if name == "chip16m":
+        c = cases.chip_case(n_target=16_000_000)
+        return c, (f"chip16m: chip-shaped domain with a 10:1 density range (slab, chip root, chip band), "
+                   f"{c.N} particles, dx=1um, no tool (full tool-free step)")
     if name.startswith("ref_cut_sph_ny"):
         ny = int(name[len("ref_cut_sph_ny"):])
         c = cases.ref_cut_sph(ny=ny)

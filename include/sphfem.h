@@ -325,7 +325,11 @@ int sphfem_leap_frog_correct(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_parti
  * 32 or 48 entries per lane), update kernel block size (128 or 256).  Other values are rejected.
  * Results do not depend on them beyond the FP32 summation blocks of the FP32C form.  No
  * counterpart in the reference (fixed launch shapes there). */
-typedef struct { int pair_block, pair_chunk, update_block; } sphfem_tuning;
+/* pair_kernel (FP32C): SPHFEM_PAIRS_LIST gathers every neighbour record through L1;
+ * SPHFEM_PAIRS_STAGED first copies each wavefront's window positions into LDS (pair_block unused,
+ * one wavefront per workgroup).  At the same pair_chunk both give the same sums bit for bit. */
+enum { SPHFEM_PAIRS_LIST = 0, SPHFEM_PAIRS_STAGED = 1 };
+typedef struct { int pair_block, pair_chunk, update_block, pair_kernel; } sphfem_tuning;
 int sphfem_leap_frog_set_tuning(sphfem_ctx *ctx, sphfem_leap_frog *lf, const sphfem_tuning *tuning);
 int sphfem_leap_frog_get_tuning(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_tuning *tuning);
 /* leap_frog.cu:162-248: one full step (fused kernels); tool may be NULL for a tool-free workload.

@@ -5,6 +5,9 @@ Tunables (all result-neutral except the FP32 summation blocks of the FP32C form)
   * pair_chunk   -- FP32C neighbour-list entries per lane in LDS (16, 32, 48): larger lists run
                     fewer list flushes in dense regions, smaller ones leave LDS for more waves
   * update_block -- update kernel workgroup size (128, 256)
+  * pair_kernel  -- FP32C pair kernel: the list kernel (every neighbour record gathered through L1)
+                    or the staged kernel (each wavefront's window positions copied into LDS first;
+                    one wavefront per workgroup, so pair_block does not apply); same sums
   * graph_steps  -- steps per captured graph in a batch (2, 4, 8): fewer host launches per step
 The tuner times each candidate on the live workload (it advances the simulation: run it in the
 warm-up), keeps the fastest and can store the choice in a JSON cache keyed by workload size,
@@ -27,9 +30,13 @@ GRAPH_STEPS = (2, 4, 8)
 
 def candidates(precision):
     if precision == _lib.FP32C:
-        return [dict(pair_block=b, pair_chunk=c, update_block=u)
-                for b, c, u in itertools.product(PAIR_BLOCKS, PAIR_CHUNKS, UPDATE_BLOCKS)]
-    return [dict(pair_block=b, pair_chunk=32, update_block=u) for b, u in itertools.product(PAIR_BLOCKS, UPDATE_BLOCKS)]
+        listed = [dict(pair_block=b, pair_chunk=c, update_block=u, pair_kernel=_lib.PAIRS_LIST)
+                  for b, c, u in itertools.product(PAIR_BLOCKS, PAIR_CHUNKS, UPDATE_BLOCKS)]
+        staged = [dict(pair_block=256, pair_chunk=c, update_block=u, pair_kernel=_lib.PAIRS_STAGED)
+                  for c, u in itertools.product(PAIR_CHUNKS, UPDATE_BLOCKS)]
+        return listed + staged
+    return [dict(pair_block=b, pair_chunk=32, update_block=u, pair_kernel=_lib.PAIRS_LIST)
+            for b, u in itertools.product(PAIR_BLOCKS, UPDATE_BLOCKS)]
 
 
 def _time_steps(sim, steps):

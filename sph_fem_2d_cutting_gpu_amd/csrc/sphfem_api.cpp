@@ -1599,7 +1599,8 @@ int sphfem_leap_frog_set_tuning(sphfem_ctx *ctx, sphfem_leap_frog *lf, const sph
     const sphfem_tuning &t = *tuning;
     const bool ok = (t.pair_block == 128 || t.pair_block == 256) &&
                     (t.pair_chunk == 16 || t.pair_chunk == 32 || t.pair_chunk == 48) &&
-                    (t.update_block == 128 || t.update_block == 256);
+                    (t.update_block == 128 || t.update_block == 256) &&
+                    (t.pair_kernel == SPHFEM_PAIRS_LIST || t.pair_kernel == SPHFEM_PAIRS_STAGED);
     if (!ok) return fail(ctx, SPHFEM_ERR_ARG, "unsupported tuning");
     lf->tuning = t;
     return SPHFEM_OK;
@@ -1792,7 +1793,8 @@ static bool graph_settings_hold(const GraphCase &g) {
     const sphfem_leap_frog *lf = g.lf;
     return std::memcmp(&g.ctx->c, &g.consts, sizeof(DevConsts)) == 0 && lf->precision == g.precision &&
            lf->tuning.pair_block == g.tuning.pair_block && lf->tuning.pair_chunk == g.tuning.pair_chunk &&
-           lf->tuning.update_block == g.tuning.update_block && lf->intermediates == g.intermediates;
+           lf->tuning.update_block == g.tuning.update_block && lf->tuning.pair_kernel == g.tuning.pair_kernel &&
+           lf->intermediates == g.intermediates;
 }
 
 extern "C" {

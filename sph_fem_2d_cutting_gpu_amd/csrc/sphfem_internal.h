@@ -24,6 +24,8 @@
 #define SPHFEM_SEG_SINK_CAP 4096
 #define SPHFEM_FLAG_SINK_FRIC 6
 #define SPHFEM_FLAG_SINK_S2 7
+// position records a wavefront of the staged FP32C pair kernel copies into LDS (8 KiB)
+#define SPHFEM_STAGE_CAP 512
 // blocks of the report_force reduction (its partials: 2 doubles per block) and the largest particle
 // count of the allocation-free variant
 #define SPHFEM_FORCE_BLOCKS 256
@@ -211,7 +213,7 @@ struct sphfem_rigid_tool {
 struct sphfem_leap_frog {
     unsigned int num_part = 0, num_cell = 0;
     int precision = SPHFEM_FP64;
-    sphfem_tuning tuning = {256, 32, 256};
+    sphfem_tuning tuning = {256, 32, 256, SPHFEM_PAIRS_LIST};
     sphfem_double2 *pos_init = nullptr, *vel_init = nullptr;
     sphfem_double4 *S_init = nullptr;
     double *rho_init = nullptr, *T_init = nullptr;

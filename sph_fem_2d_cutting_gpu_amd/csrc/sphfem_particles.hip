@@ -226,6 +226,45 @@ __device__ __forceinline__ float4 ldrec(const float4 *__restrict__ base, unsigne
     return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + byte_off);
 }
 
+// Where the pair loops read the neighbours' position records (r0) from: HBM through L1 (the list
+// kernel), or a wavefront's copy of its five columns' windows in LDS (the staged kernel).  Both give
+// the same float4 values, so the sums do not depend on the source.
+// A neighbour-list entry names the candidate's column d (0..4) and where its records are.
+struct GlobalPos {
+    using Entry = int;                                      // sorted index | column << 29
+    const float4 *__restrict__ r0;
+    // candidate j of column d during the scan
+    __device__ __forceinline__ float4 scan(int, int j) const { return ldrec(r0, (unsigned)j << 4); }
+    __device__ __forceinline__ Entry encode(int d, int j) const { return j | (d << 29); }
+    __device__ __forceinline__ int column(Entry e) const { return (int)((unsigned)e >> 29); }
+    __device__ __forceinline__ unsigned index(Entry e) const { return (unsigned)(e & 0x0fffffff); }
+    __device__ __forceinline__ float4 entry(Entry e) const { return ldrec(r0, index(e) << 4); }
+    // the range a lane scans in column d, limited to what the source holds (everything here)
+    __device__ __forceinline__ bool clip(int, int &, int &) const { return true; }
+};
+
+struct StagedPos {
+    using Entry = unsigned short;                           // LDS slot | column << 12
+    const float4 *spos;       // LDS: the union windows of columns 0..4, one after the other
+    const int *sdelta;        // LDS: per column, sorted index minus LDS slot
+    int F[5], L[5], delta[5]; // per column: staged sorted range [F, L) and its delta (uniform)
+    __device__ __forceinline__ float4 scan(int d, int j) const { return spos[j - delta[d]]; }
+    __device__ __forceinline__ Entry encode(int d, int j) const { return (Entry)((j - delta[d]) | (d << 12)); }
+    __device__ __forceinline__ int column(Entry e) const { return (int)(e >> 12); }
+    __device__ __forceinline__ unsigned index(Entry e) const { return (unsigned)((e & 0xfff) + sdelta[e >> 12]); }
+    __device__ __forceinline__ float4 entry(Entry e) const { return spos[e & 0xfff]; }
+    // a lane's window lies inside the wavefront's union by construction when the hashes are sorted;
+    // otherwise the range is cut to the staged part (no access outside the LDS copy) and false
+    // reports the inconsistent table
+    __device__ __forceinline__ bool clip(int d, int &first, int &last) const {
+        const bool inside = first >= F[d] && last <= L[d];
+        first = first < F[d] ? F[d] : first;
+        last = last > L[d] ? L[d] : last;
+        if (last < first) last = first;
+        return inside;
+    }
+};
+
 struct PairI {            // particle i and the per-i kernel constants
     float fxi, fyi, rowi, hi, vxi, vyi, sxi, syi, szi, Rxi, Ryi, Rzi, rhoi, ci, Ti;
     float h1, fac, c_out_w, c_out_g, c_in_g;
@@ -241,9 +280,9 @@ struct PairAcc {          // FP32 block sums
 template <bool WIN = false>
 __device__ __forceinline__ void pair_term(const PairI &I, float hf, float inv_wdp, float alpha, float beta,
                                           float eta2, const float4 q0, const float4 q1, const float4 q2,
-                                          const float4 q3, int e, PairAcc &a) {
+                                          const float4 q3, int d, PairAcc &a) {
 #pragma clang fp contract(fast)
-    const float dcol = (float)((int)((unsigned)e >> 29) - 2);     // column of j minus column of i
+    const float dcol = (float)(d - 2);                          // column of j minus column of i
     const float xij = (I.fxi - q0.x) - dcol * hf;
     const float yij = (I.fyi - q0.y) + (I.rowi - q0.z) * hf;
     const float rr2 = xij * xij + yij * yij;
@@ -318,12 +357,13 @@ __device__ __forceinline__ void pair_term(const PairI &I, float hf, float inv_wd
 // divergence - mass * viscous term) per axis, XSPH velocity per axis; psum_T[i] = Laplacian of T.
 // WIN: every non-blanked record of the 5x5 cells (blanked records sit at x = 1e30) is evaluated with
 // the kernel set to 0 outside the support, as the reference does; see pair_sums_fp64.
-template <int BS, int CHUNK, bool WIN>
+template <int BS, int CHUNK, bool WIN, class Pos = GlobalPos>
 __device__ __forceinline__ void pairs_fp32c_one(const DevConsts &c, const double *__restrict__ blanked,
                                                 const int *__restrict__ hash, int N, const RecView &rv,
                                                 const int *__restrict__ cs, const int *__restrict__ ce,
                                                 float4 *__restrict__ psum, float *__restrict__ psum_T, int i,
-                                                int *__restrict__ list) {
+                                                typename Pos::Entry *__restrict__ list, const Pos &pos,
+                                                int *__restrict__ flags) {
 #pragma clang fp contract(fast)
     double d_v0 = 0., d_v1 = 0., d_v2 = 0., d_v3 = 0.;
     double d_ax = 0., d_ay = 0., d_p0 = 0., d_p1 = 0., d_T = 0.;
@@ -384,20 +424,20 @@ __device__ __forceinline__ void pairs_fp32c_one(const DevConsts &c, const double
         auto run_list = [&](int n) {
             if (n <= 0) return;
             PairAcc acc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            int eA = list[0];
-            unsigned oA = (unsigned)(eA & 0x0fffffff) << 4;
-            float4 A0 = ldrec(rv.r0, oA), A1 = ldrec(rv.r1, oA), A2 = ldrec(rv.r2, oA), A3 = ldrec(rv.r3, oA);
+            typename Pos::Entry eA = list[0];
+            unsigned oA = pos.index(eA) << 4;
+            float4 A0 = pos.entry(eA), A1 = ldrec(rv.r1, oA), A2 = ldrec(rv.r2, oA), A3 = ldrec(rv.r3, oA);
             for (int k = 0; k < n; k += 2) {
-                const int eB = list[(k + 1 < n ? k + 1 : n - 1) * BS];
-                const unsigned oB = (unsigned)(eB & 0x0fffffff) << 4;
-                const float4 B0 = ldrec(rv.r0, oB), B1 = ldrec(rv.r1, oB), B2 = ldrec(rv.r2, oB),
+                const typename Pos::Entry eB = list[(k + 1 < n ? k + 1 : n - 1) * BS];
+                const unsigned oB = pos.index(eB) << 4;
+                const float4 B0 = pos.entry(eB), B1 = ldrec(rv.r1, oB), B2 = ldrec(rv.r2, oB),
                              B3 = ldrec(rv.r3, oB);
-                pair_term<WIN>(I, hf, inv_wdp, alpha, beta, eta2, A0, A1, A2, A3, eA, acc);
+                pair_term<WIN>(I, hf, inv_wdp, alpha, beta, eta2, A0, A1, A2, A3, pos.column(eA), acc);
                 if (k + 1 < n) {
                     eA = list[(k + 2 < n ? k + 2 : n - 1) * BS];
-                    oA = (unsigned)(eA & 0x0fffffff) << 4;
-                    A0 = ldrec(rv.r0, oA); A1 = ldrec(rv.r1, oA); A2 = ldrec(rv.r2, oA); A3 = ldrec(rv.r3, oA);
-                    pair_term<WIN>(I, hf, inv_wdp, alpha, beta, eta2, B0, B1, B2, B3, eB, acc);
+                    oA = pos.index(eA) << 4;
+                    A0 = pos.entry(eA); A1 = ldrec(rv.r1, oA); A2 = ldrec(rv.r2, oA); A3 = ldrec(rv.r3, oA);
+                    pair_term<WIN>(I, hf, inv_wdp, alpha, beta, eta2, B0, B1, B2, B3, pos.column(eB), acc);
                 }
                 if (((k + 2) & 7) == 0) flush(acc);
             }
@@ -417,13 +457,12 @@ __device__ __forceinline__ void pairs_fp32c_one(const DevConsts &c, const double
             for (int jr = high_j - 2; last == -1 && jr >= low_j; jr--)
                 if (cs[c0 + jr] != -1) last = ce[c0 + jr];
             if (last > N || first < 0) continue;                        // inconsistent cell table
+            if (!pos.clip(d, first, last)) atomicOr(&flags[0], 4);
             const float cofs = (float)(gi - col) * hf;
-            const int tag = d << 29;
             for (int j = first; j < last; j += 4) {
-                // four scan records; up to three past `last` are read (the scan block is padded)
-                const unsigned off = (unsigned)j << 4;
-                const float4 b0 = ldrec(rv.r0, off), b1 = ldrec(rv.r0, off + 16u), b2 = ldrec(rv.r0, off + 32u),
-                             b3 = ldrec(rv.r0, off + 48u);
+                // four scan records; up to three past `last` are read (both sources are padded)
+                const float4 b0 = pos.scan(d, j), b1 = pos.scan(d, j + 1), b2 = pos.scan(d, j + 2),
+                             b3 = pos.scan(d, j + 3);
                 const float4 bb[4] = {b0, b1, b2, b3};
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
@@ -432,7 +471,7 @@ __device__ __forceinline__ void pairs_fp32c_one(const DevConsts &c, const double
                     const float rr2 = xij * xij + yij * yij;
                     const bool take = WIN ? rr2 < 1e30f : (rr2 < fourh2 && rr2 >= rr2min);
                     if (j + u < last && take) {
-                        list[n * BS] = (j + u) | tag;
+                        list[n * BS] = pos.encode(d, j + u);
                         n++;
                     }
                 }
@@ -464,7 +503,108 @@ __global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *_
     const int i = xcd_logical_block() * BS + threadIdx.x;
     if (i >= N) return;
     if (*nonfinite != 0) return;
-    pairs_fp32c_one<BS, CHUNK, false>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, nb_list + threadIdx.x);
+    pairs_fp32c_one<BS, CHUNK, false>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, nb_list + threadIdx.x,
+                                      GlobalPos{rv.r0}, nullptr);
+}
+
+// LDS-staged form of k_pairs_fp32c, one wavefront per workgroup.  When the wavefront's 64
+// particles lie in one cell column and the union of their 5x5-cell windows holds at most CAP
+// records, the wavefront first copies the position records of that union -- five sorted ranges,
+// one per neighbour column, rows [first row - 2, last row + 2] -- into LDS with coalesced loads;
+// the candidate scan and the pair terms then read positions from LDS and only the three other
+// records of each pair from L2/HBM.  Otherwise (a wavefront across two columns, a larger union, or
+// a long run of empty cells) it runs the list kernel's path.  The per-lane loops, votes and FP32
+// summation blocks are those of k_pairs_fp32c and both kernels group particles into wavefronts
+// alike, so the two give the same sums bit for bit.  All 64 lanes take part in the staging (lanes
+// past N copy records but sum nothing); the union comes from the hashes of the wavefront's first and
+// last particles, read by every lane from memory, so no lane depends on another lane's registers.
+#define SPHFEM_STAGE_ROW_SCAN 64
+template <int CHUNK, int CAP>
+__global__ void __launch_bounds__(64) k_pairs_fp32c_staged(DevConsts c, const double *__restrict__ blanked,
+                                                           const int *__restrict__ hash, int N, RecView rv,
+                                                           const int *__restrict__ cs, const int *__restrict__ ce,
+                                                           float4 *__restrict__ psum, float *__restrict__ psum_T,
+                                                           const int *__restrict__ nonfinite, int *__restrict__ flags) {
+    // LDS: staged -- positions [CAP + 4] (+4: the scan reads up to three records past a range) and
+    // 16-bit list entries; list path -- 32-bit list entries in the same bytes
+    constexpr int POS_BYTES = (CAP + 4) * 16, LIST16 = CHUNK * 64 * 2, LIST32 = CHUNK * 64 * 4;
+    constexpr int LDS_BYTES = POS_BYTES + LIST16 > LIST32 ? POS_BYTES + LIST16 : LIST32;
+    __shared__ __align__(16) unsigned char lds[LDS_BYTES];
+    __shared__ int sdelta[8];
+    float4 *spos = reinterpret_cast<float4 *>(lds);
+    unsigned short *list16 = reinterpret_cast<unsigned short *>(lds + POS_BYTES);
+    int *list32 = reinterpret_cast<int *>(lds);
+    const int base = xcd_logical_block() * 64;
+    if (base >= N || *nonfinite != 0) return;           // the whole wavefront
+    const int lane = (int)threadIdx.x;
+    const int i = base + lane;
+    const int last_i = base + 63 < N ? base + 63 : N - 1;
+    const int nx = c.geom.nx, ny = c.geom.ny;
+    const int h0 = hash[base], h1 = hash[last_i];
+    const int col0 = h0 / ny;
+    StagedPos sp;
+    sp.spos = spos;
+    sp.sdelta = sdelta;
+    bool staged = h0 >= 0 && h1 >= h0 && h1 / ny == col0 && col0 < nx;
+    int total = 0;
+    if (staged) {
+        const int r_lo = h0 - col0 * ny - 2 < 0 ? 0 : h0 - col0 * ny - 2;
+        const int r_hi = h1 - col0 * ny + 2 > ny - 1 ? ny - 1 : h1 - col0 * ny + 2;
+        // the end cells of the five row ranges, loaded together; runs of empty cells are scanned
+        // only where an end cell is empty
+        int lo_s[5], hi_s[5], hi_e[5];
+#pragma unroll
+        for (int d = 0; d < 5; d++) {
+            const int col = col0 - 2 + d;
+            const bool ok = col >= 0 && col < nx;
+            const int c0 = (ok ? col : col0) * ny;
+            lo_s[d] = ok ? cs[c0 + r_lo] : -1;
+            hi_s[d] = ok ? cs[c0 + r_hi] : -1;
+            hi_e[d] = ok ? ce[c0 + r_hi] : -1;
+        }
+#pragma unroll
+        for (int d = 0; d < 5; d++) {
+            const int col = col0 - 2 + d;
+            int F = 0, L = 0;
+            if (col >= 0 && col < nx) {
+                const int c0 = col * ny;
+                // first and last non-empty cell of the rows [r_lo, r_hi] (empty cells hold -1)
+                int r = r_lo, f = lo_s[d];
+                while (f == -1 && r < r_hi && r - r_lo < SPHFEM_STAGE_ROW_SCAN) f = cs[c0 + ++r];
+                if (f != -1) {
+                    int l = hi_s[d] != -1 ? hi_e[d] : -1, r2 = r_hi;
+                    while (l == -1 && r2 > r && r_hi - r2 < SPHFEM_STAGE_ROW_SCAN)
+                        if (cs[c0 + --r2] != -1) l = ce[c0 + r2];
+                    if (l == -1) staged = false;                 // the downward scan gave up
+                    F = f;
+                    L = l;
+                } else if (r < r_hi) {
+                    staged = false;                              // the upward scan gave up
+                }
+                if (F < 0 || L > N || L < F) staged = false;     // inconsistent cell table
+            }
+            sp.F[d] = F;
+            sp.L[d] = L;
+            sp.delta[d] = F - total;
+            total += L - F;
+        }
+        if (total > CAP) staged = false;
+    }
+    if (staged) {
+#pragma unroll
+        for (int d = 0; d < 5; d++) {
+            if (lane == d) sdelta[d] = sp.delta[d];
+            const int n = sp.L[d] - sp.F[d], off = sp.F[d] - sp.delta[d];
+            for (int k = lane; k < n; k += 64) spos[off + k] = rv.r0[sp.F[d] + k];
+        }
+        __syncthreads();
+        if (i < N)
+            pairs_fp32c_one<64, CHUNK, false, StagedPos>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i,
+                                                         list16 + lane, sp, flags);
+    } else if (i < N) {
+        pairs_fp32c_one<64, CHUNK, false>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, list32 + lane,
+                                          GlobalPos{rv.r0}, nullptr);
+    }
 }
 
 // the window form over every particle (grid-stride loop) when nonfinite[0] is set, else nothing
@@ -478,7 +618,8 @@ __global__ void __launch_bounds__(BS) k_pairs_fp32c_window(DevConsts c, const do
     __shared__ int nb_list[CHUNK * BS];
     if (*nonfinite == 0) return;
     for (int i = (int)blockIdx.x * BS + (int)threadIdx.x; i < N; i += (int)gridDim.x * BS)
-        pairs_fp32c_one<BS, CHUNK, true>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, nb_list + threadIdx.x);
+        pairs_fp32c_one<BS, CHUNK, true>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, nb_list + threadIdx.x,
+                                         GlobalPos{rv.r0}, nullptr);
 }
 
 __device__ __forceinline__ double interaction_alpha(const DevConsts &c, double is_tool) {
@@ -995,12 +1136,26 @@ static void launch_pairs_fp32c(sphfem_ctx *ctx, const sphfem_particles &B, sphfe
                                                                     ctx->d_flags + SPHFEM_FLAG_NONFINITE);
 }
 
+template <int CH>
+static void launch_pairs_fp32c_staged(sphfem_ctx *ctx, const sphfem_particles &B, sphfem_leap_frog *lf, const int *cs,
+                                      const int *ce) {
+    const size_t n = B.N;
+    RecView rv{lf->rec, lf->rec + n + 4, lf->rec + 2 * n + 4, lf->rec + 3 * n + 4};
+    k_pairs_fp32c_staged<CH, SPHFEM_STAGE_CAP><<<nblk(B.N, 64), 64, 0, ctx->stream>>>(
+        ctx->c, B.blanked, B.hash, (int)B.N, rv, cs, ce, lf->psum, lf->psum_T, ctx->d_flags + SPHFEM_FLAG_NONFINITE,
+        ctx->d_flags);
+}
+
 static void launch_pairs_fp32c_tuned(sphfem_ctx *ctx, const sphfem_particles &B, sphfem_leap_frog *lf, const int *cs,
                                      const int *ce) {
     {
         KTimer tm(ctx, "pairs_fp32c");
         const int pb = lf->tuning.pair_block, pc = lf->tuning.pair_chunk;
-        if (pb == 128 && pc == 16) launch_pairs_fp32c<128, 16>(ctx, B, lf, cs, ce);
+        if (lf->tuning.pair_kernel == SPHFEM_PAIRS_STAGED) {
+            if (pc == 16) launch_pairs_fp32c_staged<16>(ctx, B, lf, cs, ce);
+            else if (pc == 48) launch_pairs_fp32c_staged<48>(ctx, B, lf, cs, ce);
+            else launch_pairs_fp32c_staged<32>(ctx, B, lf, cs, ce);
+        } else if (pb == 128 && pc == 16) launch_pairs_fp32c<128, 16>(ctx, B, lf, cs, ce);
         else if (pb == 128 && pc == 32) launch_pairs_fp32c<128, 32>(ctx, B, lf, cs, ce);
         else if (pb == 128 && pc == 48) launch_pairs_fp32c<128, 48>(ctx, B, lf, cs, ce);
         else if (pb == 256 && pc == 16) launch_pairs_fp32c<256, 16>(ctx, B, lf, cs, ce);

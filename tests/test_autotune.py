@@ -4,10 +4,13 @@ from sph_fem_2d_cutting_gpu_amd import _lib, autotune
 
 def test_candidate_sets():
     c32 = autotune.candidates(_lib.FP32C)
-    assert len(c32) == 12 and {"pair_block": 256, "pair_chunk": 32, "update_block": 256} in c32
+    assert len(c32) == 18
+    assert {"pair_block": 256, "pair_chunk": 32, "update_block": 256, "pair_kernel": _lib.PAIRS_LIST} in c32
     assert {c["pair_chunk"] for c in c32} == {16, 32, 48}
+    staged = [c for c in c32 if c["pair_kernel"] == _lib.PAIRS_STAGED]
+    assert len(staged) == 6 and {c["pair_chunk"] for c in staged} == {16, 32, 48}
     c64 = autotune.candidates(_lib.FP64)     # the list chunk is an FP32C parameter only
-    assert len(c64) == 4 and all(c["pair_chunk"] == 32 for c in c64)
+    assert len(c64) == 4 and all(c["pair_chunk"] == 32 and c["pair_kernel"] == _lib.PAIRS_LIST for c in c64)
     assert {c["pair_block"] for c in c64} == {128, 256}
 
 
