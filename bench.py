@@ -16,6 +16,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -61,7 +63,7 @@ def make_case(name):
     if name == "chip16m":
         c = cases.chip_case(n_target=16_000_000)
         return c, (f"chip16m: chip-shaped domain with a 10:1 density range (slab, chip root, chip band), "
-                   f"{c.N} particles, dx=1um, no tool (full tool-free step)")
+                   f"{c.N} particles, dx=1um, no tool; kernel-only pass")
     if name.startswith("ref_cut_sph_ny"):
         ny = int(name[len("ref_cut_sph_ny"):])
         c = cases.ref_cut_sph(ny=ny)
@@ -78,6 +80,151 @@ def make_case(name):
         return c, (f"impact_nbox{nb}: setup_impact steel disc at 1000 m/s on a wall, {c.N} particles, "
                    f"dx={c.dx * 1e3:.3f}mm, unlocked grid (eager steps)")
     raise SystemExit(f"unknown workload {name}")
+
+
+KERNEL_PASS = {"fp64": "interactions_ref", "fp32c": "pairs_fp32c"}
+
+
+def run_kernel_pass(case, precision, steps, warmup, tune=True, pair_kernel="auto"):
+    """Kernel-only step (the 16M chip domain): hash, stable sort with the gather of every particle
+    field (grid_gpu_green::sort), cell table, then one interaction pass -- the FP32C pass of the
+    step's pair kernel (records, pair sums, expansion into the reference's fields) or the FP64
+    interactions_monaghan of interactions_gpu.cu:143-379 -- on particles that do not move (the
+    domain's velocities would carry them out of its tight grid).  Eager launches.  Returns
+    (seconds for `steps` passes, interaction-kernel seconds per launch, occupied cells, tuning)."""
+    import ctypes as C
+    from sph_fem_2d_cutting_gpu_amd import _lib, autotune
+    from sph_fem_2d_cutting_gpu_amd import solver as S
+    prec = {"fp64": _lib.FP64, "fp32c": _lib.FP32C}[precision]
+    sim = S.Simulation(case, precision=prec)
+    p, g, ctx = sim.p, sim.grid, sim.ctx
+    nc = g.num_cell
+    cs, ce = C.c_void_p(), C.c_void_p()
+    ctx.call("sphfem_malloc", C.byref(cs), 4 * nc)
+    ctx.call("sphfem_malloc", C.byref(ce), 4 * nc)
+    S.interactions_setup_geometry_constants(g)
+    S.material_eos(p)
+    S.corrector_artificial_stress(p)
+
+    def passes(n):
+        for _ in range(n):
+            g.assign_hashes(p)
+            g.sort(p)
+            g.get_cells(p, cs.value, ce.value)
+            if prec == _lib.FP32C:
+                S.interactions_monaghan_fp32c(sim.stepper, p, cs.value, ce.value, nc)
+            else:
+                S.interactions_monaghan(p, cs.value, ce.value, nc)
+
+    passes(max(warmup, 1))
+    ctx.synchronize()
+    ctx.check_error()
+    tuning = None
+    if tune and prec == _lib.FP32C:
+        rows = []
+        for cand in autotune.candidates(prec):
+            if pair_kernel != "auto" and cand["pair_kernel"] != {"list": _lib.PAIRS_LIST,
+                                                                  "staged": _lib.PAIRS_STAGED}[pair_kernel]:
+                continue
+            sim.set_tuning(**cand)
+            passes(1)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            passes(3)
+            ctx.synchronize()
+            rows.append((time.perf_counter() - t0, cand))
+        tuning = min(rows, key=lambda r: r[0])[1]
+        sim.set_tuning(**tuning)
+    kname = KERNEL_PASS[precision]
+    ctx.timing_select(kname)
+    ctx.timing(True)
+    ctx.timing_reset()
+    passes(min(steps, 10))
+    ctx.synchronize()
+    ctx.timing(False)
+    k_ms, k_n = ctx.timing_get(kname)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    passes(steps)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.check_error()
+    cs_h = np.empty(nc, np.int32)
+    ctx.call("sphfem_memcpy_dtoh", cs_h.ctypes.data_as(C.c_void_p), cs, 4 * nc)
+    ctx.call("sphfem_free", cs)
+    ctx.call("sphfem_free", ce)
+    sim.close()
+    return elapsed, k_ms / max(k_n, 1) / 1e3, int(np.count_nonzero(cs_h != -1)), tuning
+
+
+def cpu_kernel_pass(case, budget_s):
+    """The FP64 host oracle's kernel-only pass (hash, stable sort and cells, interactions) on this
+    host's cores, a bounded number of passes; the one-thread figure times the interaction pass
+    over the first 500,000 sorted particles and scales it to N."""
+    from oracle.oracle import Oracle, lib
+    o = Oracle(case)
+    threads = lib().o_num_threads()
+    o.call("o_material_eos", "cp")
+    o.call("o_artificial_stress", "cp")
+
+    def one():
+        o.call("o_compute_hashes", "cp")
+        o.call("o_sort_and_cells", "cps")
+        o.call("o_interactions_monaghan", "cps")
+
+    t0 = time.perf_counter()
+    one()
+    t1 = time.perf_counter() - t0
+    n = int(max(1, min(10, budget_s / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    for _ in range(n):
+        one()
+    dt = time.perf_counter() - t0
+    w = min(500_000, case.N)
+    lib().o_set_num_threads(1)
+    t0 = time.perf_counter()
+    o.call("o_interactions_monaghan_range", "cps", 0, w)
+    d1 = time.perf_counter() - t0
+    lib().o_set_num_threads(threads)
+    return {"value": case.N * n / dt, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{n} kernel-only passes (hash, stable sort and cells, FP64 interactions; after one "
+                      f"warm-up pass) on the FP64 host oracle, {threads} OpenMP threads; one-thread figure: "
+                      f"the interaction pass over {w} particles, scaled",
+            "single_thread": {"value": w / d1, "interactions_only": True},
+            "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+
+
+def main_kernel_pass(args, case, workload):
+    elapsed, k_s, n_occ, tuning = run_kernel_pass(case, args.precision, args.steps, args.warmup,
+                                                  not args.no_tune, args.pair_kernel)
+    w = 8 if args.precision == "fp64" else 4
+    b_int = interaction_bytes(case.N, n_occ, w)
+    achieved = b_int / k_s / 1e9
+    ups = case.N * args.steps / elapsed
+    out = {
+        "metric": "particle-updates/s of the kernel-only pass (sort + interactions), FP32-comp vs FP64",
+        "value": ups, "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32 pair sums (FP64 block accumulation) + f64 state",
+        "data": "synthetic (seeded chip-shaped point set)",
+        "config": {"workload": workload, "particles": case.N, "precision": args.precision,
+                   "parallelism": "single GPU", "step": "hash + sort + cells + one interaction pass, eager",
+                   "tuning": tuning},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": KERNEL_PASS[args.precision],
+                     "kernel_ms": k_s * 1e3, "algorithmic_bytes": b_int, "occupied_cells": n_occ},
+    }
+    if args.precision == "fp32c" and not args.no_compare:
+        e64, k64, _, _ = run_kernel_pass(case, "fp64", args.steps, args.warmup, False)
+        ups64 = case.N * args.steps / e64
+        out["fp64_form"] = {"value": ups64, "ms_per_step": e64 / args.steps * 1e3,
+                            "kernel": KERNEL_PASS["fp64"], "kernel_ms": k64 * 1e3,
+                            "roofline_frac": interaction_bytes(case.N, n_occ, 8) / k64 / 1e9 / HBM_PEAK_GBS}
+        out["speedup_vs_fp64"] = ups / ups64
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_kernel_pass(case, args.cpu_budget)
+    print(json.dumps(out))
 
 
 def cpu_model():
@@ -121,7 +268,7 @@ def cpu_baseline(case, budget_s):
             "cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity": affinity}
 
 
-def run_form(case, precision, steps, warmup, breakdown=False, graph=True, tune=True):
+def run_form(case, precision, steps, warmup, breakdown=False, graph=True, tune=True, pair_kernel="auto"):
     """Time `steps` full steps of one arithmetic form; returns (seconds, interaction-kernel seconds, sim).
 
     The timed loop replays a captured two-step HIP graph (an odd step count ends with one eager
@@ -139,7 +286,8 @@ def run_form(case, precision, steps, warmup, breakdown=False, graph=True, tune=T
     tuning = None
     if tune:                                           # untimed: advances the warm-up
         from sph_fem_2d_cutting_gpu_amd import autotune
-        tuning = autotune.tune_simulation(sim, prec, steps=4, repeats=2)["best"]
+        kernels = None if pair_kernel == "auto" else {"list": _lib.PAIRS_LIST, "staged": _lib.PAIRS_STAGED}[pair_kernel]
+        tuning = autotune.tune_simulation(sim, prec, steps=4, repeats=2, pair_kernel=kernels)["best"]
     # interaction-kernel time (eager steps, events around that kernel only)
     kname = INTERACTION_KERNEL[precision]
     ctx.timing_select(kname)
@@ -252,6 +400,8 @@ def main():
     ap.add_argument("--eager", action="store_true", help="time eager steps instead of the captured step graph")
     ap.add_argument("--batch-cases", type=int, default=64, help="cases of the batch64 workload")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch parameters)")
+    ap.add_argument("--pair-kernel", default="auto", choices=["auto", "list", "staged"],
+                    help="FP32C pair kernel the tuner may choose: both (auto) or only one")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -272,15 +422,16 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-
     if args.workload == "batch64":
         run_batch(args)
         return
     case, workload = make_case(args.workload)
+    if args.workload in ("chip16m",):
+        main_kernel_pass(args, case, workload)
+        return
     graph = not args.eager and case.max_cell is None      # an unlocked grid reads its geometry back
     elapsed, k_avg_s, sim = run_form(case, args.precision, args.steps, args.warmup, args.breakdown, graph,
-                                     not args.no_tune)
+                                     not args.no_tune, args.pair_kernel)
     tuning = sim.bench_tuning
     cs, _ = sim.grid.download_cells()
     n_occ = int(np.count_nonzero(cs != -1))

@@ -47,12 +47,14 @@ def _time_steps(sim, steps):
     return (time.perf_counter() - t0) / steps
 
 
-def tune_simulation(sim, precision, steps=6, repeats=2) -> dict:
+def tune_simulation(sim, precision, steps=6, repeats=2, pair_kernel=None) -> dict:
     """Time every candidate for `steps` steps (best of `repeats`), apply the fastest; returns the
-    choice and all timings (ms per step).  Advances the simulation by
-    len(candidates) * steps * repeats steps."""
+    choice and all timings (ms per step).  pair_kernel limits the candidates to one FP32C pair
+    kernel.  Advances the simulation by len(candidates) * steps * repeats steps."""
     rows = []
     for cand in candidates(precision):
+        if pair_kernel is not None and cand["pair_kernel"] != pair_kernel:
+            continue
         sim.set_tuning(**cand)
         t = min(_time_steps(sim, steps) for _ in range(repeats))
         rows.append((t, cand))
