@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsphfem_hip.so")
 
 OK = 0
+ERR_HIP, ERR_ARG, ERR_STATE, ERR_RANGE = 1, 2, 3, 4      # SPHFEM_ERR_* of include/sphfem.h
 FP64 = 0
 FP32C = 1
 

@@ -8,7 +8,7 @@ Tunables (all result-neutral except the FP32 summation blocks of the FP32C form)
   * pair_kernel  -- FP32C pair kernel: the list kernel (every neighbour record gathered through L1)
                     or the staged kernel (each wavefront's window positions copied into LDS first;
                     one wavefront per workgroup, so pair_block does not apply); same sums
-  * graph_steps  -- steps per captured graph in a batch (2, 4, 8): fewer host launches per step
+  * graph_steps  -- steps per captured graph in a batch (2 ... 32): fewer graph launches per step
 The tuner times each candidate on the live workload (it advances the simulation: run it in the
 warm-up), keeps the fastest and can store the choice in a JSON cache keyed by workload size,
 precision and device.
@@ -25,7 +25,7 @@ from . import _lib
 PAIR_BLOCKS = (128, 256)
 PAIR_CHUNKS = (16, 32, 48)
 UPDATE_BLOCKS = (128, 256)
-GRAPH_STEPS = (2, 4, 8)
+GRAPH_STEPS = (2, 4, 8, 16, 32)
 
 
 def candidates(precision):

@@ -327,3 +327,24 @@ def test_remesh_geo_with_a_short_segment(tmp_path):
     assert text == R.remesh_geo(left, right, (kin["tl_x"], kin["tl_y"]))
     assert text.count("Point(") == len(left) + 1     # tl, the chain less one point, the end
     sim.close()
+
+
+@pytest.mark.parametrize("which", ["low_level", "new_mesh"])
+def test_remesh_refuses_a_non_finite_node(which):
+    """A tool node at a non-finite position: both remeshes are refused with SPHFEM_ERR_RANGE before the
+    element bins are built (binning would cast the coordinate to an integer cell index)."""
+    case = _case()
+    sim = S.Simulation(case)
+    sim.step(1)
+    sim.ctx.check_error()
+    pos = sim.tool.download()["node_pos"].copy()
+    pos[len(pos) // 2] = [np.nan, pos[len(pos) // 2][1]]
+    sim.tool.upload(dict(node_pos=pos))
+    with pytest.raises(_lib.SphfemError, match=f"status {_lib.ERR_RANGE}.*not finite"):
+        if which == "low_level":
+            sim.tool.remesh_low_level(20)
+        else:
+            path = f"/tmp/sphfem_remesh_nonfinite_{id(sim)}.inp"
+            cases.write_inp(case.tool, path, gmsh=True)
+            sim.tool.remesh(S.mesh_read(path, True))
+    sim.close()

@@ -63,3 +63,12 @@ def test_cpp_driver_links_and_checks_arguments():
         pytest.skip("driver not built")
     r = subprocess.run([exe], capture_output=True, text=True)      # usage: no device work
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_status_codes_match_the_header():
+    """The status codes the Python binding names are the header's SPHFEM_OK / SPHFEM_ERR_* values."""
+    text = open(os.path.join(ROOT, "include", "sphfem.h")).read()
+    codes = {m.group(1): int(m.group(2)) for m in re.finditer(r"#define SPHFEM_(OK|ERR_[A-Z]+)\s+(\d+)", text)}
+    assert codes["OK"] == _lib.OK
+    for name in ("HIP", "ARG", "STATE", "RANGE"):
+        assert codes[f"ERR_{name}"] == getattr(_lib, f"ERR_{name}"), name
