@@ -794,9 +794,10 @@ def config2(seed=DEFAULT_SEED):
                         5e-4, 8000, wear=False, seed=seed)
 
 
-def config3(seed=DEFAULT_SEED, nx=4000, ny=1000, tool_tris=50000, wear=True):
-    """4M-particle Ti6Al4V case: 4x1 mm, dx 1 um, ~50k-triangle tool, FEM heat + Usui wear."""
-    return cutting_case(f"ti64_{nx * ny}", nx, ny, 1e-4, "Ti6Al4V", 70.0, 5e-3, 0.35, 0.0, 7.0,
+def config3(seed=DEFAULT_SEED, nx=4000, ny=1000, tool_tris=50000, wear=True, speed_m_min=70.0):
+    """4M-particle Ti6Al4V case: 4x1 mm, dx 1 um, ~50k-triangle tool, FEM heat + Usui wear, cutting
+    speed 70 m/min unless given."""
+    return cutting_case(f"ti64_{nx * ny}", nx, ny, 1e-4, "Ti6Al4V", speed_m_min, 5e-3, 0.35, 0.0, 7.0,
                         5e-4, tool_tris, wear=wear, seed=seed)
 
 

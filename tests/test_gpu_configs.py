@@ -6,7 +6,8 @@
   within 0.5 %.
 * 4M particles of Ti6Al4V with a ~50k-triangle tool, FEM heat and Usui wear: a 10-step window of the
   FP64 form against the oracle (hash, sorted index and blanked flags exact, fields within 1e-6), and
-  one step of the FP32C form against the FP64 form from a perturbed state (every pair term non-zero).
+  one step of the FP32C form against the FP64 form from a perturbed state (every pair term non-zero),
+  and 2000 steps of both forms (cutting force, peak temperature rise, plastic strain within 0.5 %).
 * 16.3M particles on the chip-shaped domain with a 10:1 density range (kernel-only): the hash and
   stable sort against numpy, then one interaction pass in both forms against the oracle on four
   windows of 60k sorted particles (grid_gpu_green.cu:184-213, interactions_gpu.cu:143-379).
@@ -123,6 +124,41 @@ def test_config3_fp32c_vs_fp64_derivatives():
     print("derivatives", {k: f"{v:.2e}" for k, v in errs.items()}, "state", {k: f"{v:.2e}" for k, v in errs_s.items()})
     assert all(v <= 1e-5 for v in errs.values()), errs
     assert all(v <= 1e-6 for v in errs_s.values()), errs_s
+
+
+def test_config3_fp32c_vs_fp64_2000_steps():
+    """The bench workload at 700 m/min (at 70 m/min 2000 steps move the tool by less than one
+    lattice spacing of 1 um and the contact barely forms; at 700 m/min by ~7 um), 2000 steps in both
+    forms on the GPU (the FP64 form follows the oracle above): cutting force within 0.5 % of the
+    FP64 form's running maximum, peak temperature rise and peak / mean plastic strain within 0.5 %,
+    plastic flow present."""
+    case = cases.config3(speed_m_min=700.0)
+    out = {}
+    for prec in (_lib.FP64, _lib.FP32C):
+        print(f"config3 2000 steps, precision {prec}", flush=True)
+        sim = S.Simulation(case, precision=prec)
+        F = []
+        for k in range(100):
+            sim.step(20)
+            F.append(S.report_force(sim.p))
+        sim.ctx.check_error()
+        out[prec] = (np.array(F), sim.state(("T", "eps_pl", "blanked")))
+        sim.close()
+    (F64, a64), (F32, a32) = out[_lib.FP64], out[_lib.FP32C]
+    scale = np.max(np.abs(F64), axis=0)
+    ferr = np.max(np.abs(F32 - F64), axis=0) / scale
+    T0 = case.const.joco["Tref"]
+    dT64, dT32 = a64["T"].max() - T0, a32["T"].max() - T0
+    e64 = a64["eps_pl"][a64["blanked"] != 1.0]
+    e32 = a32["eps_pl"][a32["blanked"] != 1.0]
+    print(f"force error {ferr}, dT {dT32:.8g} vs {dT64:.8g}, eps max {e32.max():.6g} vs {e64.max():.6g}, "
+          f"mean {e32.mean():.6g} vs {e64.mean():.6g}, blanked {(a32['blanked'] == 1.0).sum()} vs "
+          f"{(a64['blanked'] == 1.0).sum()}")
+    assert np.all(scale > 0) and dT64 > 0.0 and e64.max() > 0.0
+    assert np.all(ferr <= 5e-3), ferr
+    assert abs(dT32 - dT64) <= 5e-3 * dT64
+    assert abs(e32.max() - e64.max()) <= 5e-3 * e64.max()
+    assert abs(e32.mean() - e64.mean()) <= 5e-3 * e64.mean()
 
 
 # ------------------------------------------------------------------------------- 16M kernel-only

@@ -43,14 +43,17 @@ def test_all_candidates_agree(prec, which):
             assert err <= tol, f"{cand} {f}: {err:.3e}"
 
 
-@pytest.mark.parametrize("which", ["cutting", "chip50k", "chip1m"])
+@pytest.mark.parametrize("which", ["cutting", "cutting_rothlin", "chip50k", "chip1m"])
 def test_staged_pair_kernel_equals_list_kernel(which):
     """The LDS-staged FP32C pair kernel against the list kernel at each list chunk: the same sums bit
     for bit, on cases where most wavefronts stage their windows (the chip domain, 92 % at 1M) and on
-    a cutting case where most run the list path (64 particles span several short columns), with
-    partial last wavefronts (N mod 64 = 0, 50 and 16)."""
-    if which == "cutting":
+    a cutting case where most run the list path (64 particles span several short columns), on the
+    Green and the Rothlin (dense) cell table, with partial last wavefronts (N mod 64 = 0, 50, 16)."""
+    grid = "green"
+    if which.startswith("cutting"):
         case, steps = cases.perturbed(cases.small_cutting_case(nx=160, ny=40, wear=True)), 3
+        if which == "cutting_rothlin":           # dense cell table: empty cells hold offsets, not -1
+            grid = "rothlin"
     elif which == "chip50k":
         case, steps = cases.chip_case(n_target=50_000), 2
     else:
@@ -60,7 +63,7 @@ def test_staged_pair_kernel_equals_list_kernel(which):
     for chunk in autotune.PAIR_CHUNKS:
         out = {}
         for kernel in (_lib.PAIRS_LIST, _lib.PAIRS_STAGED):
-            sim = S.Simulation(case, precision=_lib.FP32C)
+            sim = S.Simulation(case, precision=_lib.FP32C, grid=grid)
             sim.set_tuning(pair_block=256, pair_chunk=chunk, update_block=256, pair_kernel=kernel)
             sim.step(steps)
             sim.ctx.check_error()          # also the staged kernel's containment check
