@@ -77,6 +77,9 @@ def test_cpp_driver_reads_inp(tmp_path):
                        timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "tool mesh" in r.stdout and "tool.inp" in r.stdout
+    # the driver set the tool up as benchmarks_NZ.cpp:228-245 does: tool_FEM(tl, length, height, rake,
+    # clearance, r, mu), mesh_CPU / mesh_GPU, set_up_segments_from_mesh(), set_up_FEM_tool_gpu(tc)
+    assert "tool_FEM set-up sequence" in r.stdout, r.stdout
     got = case_io.read_driver_output(str(out), case.N)
     sim = S.Simulation(case)
     forces = []
