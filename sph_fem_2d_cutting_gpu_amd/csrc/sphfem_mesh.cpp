@@ -64,14 +64,20 @@ std::vector<std::string> split(const std::string &line, char sep) {
 
 // the integer at the start of a token (as `stringstream >> int`: leading blanks, a sign, digits);
 // false if there is none or it does not fit an int
+// ids and counts of a mesh file stay within +-2^30, so that id arithmetic (id - 1 - jumps) cannot
+// overflow; larger values are parse errors
 bool to_int(const std::string &tok, int &v) {
     const char *b = tok.c_str();
     char *e = nullptr;
     errno = 0;
     const long x = std::strtol(b, &e, 10);
-    if (e == b || errno == ERANGE || x < INT_MIN || x > INT_MAX) return false;
+    if (e == b || errno == ERANGE || x < -(1L << 30) || x > (1L << 30)) return false;
     v = (int)x;
     return true;
+}
+
+bool is_blank(const std::string &tok) {
+    return tok.find_first_not_of(" \t\r") == std::string::npos;
 }
 
 // the double at the start of a string (as std::stod)
@@ -187,8 +193,10 @@ void read_abaqus(sphfem_mesh &m, std::istream &in) {
         auto ids = [&](const std::string &l) {
             std::vector<int> v;
             for (const auto &t : split(l, ',')) {
+                if (is_blank(t)) continue;
                 int a;
-                if (to_int(t, a)) v.push_back(a);
+                if (!to_int(t, a)) throw ParseError{"set id: " + l};
+                v.push_back(a);
             }
             return v;
         };
@@ -318,8 +326,9 @@ void read_gmsh(sphfem_mesh &m, const std::string &text) {
             }
             if (cut_start || fixed_start) {
                 for (const auto &t : split(line, ',')) {
+                    if (is_blank(t)) continue;
                     int a;
-                    if (!to_int(t, a)) continue;
+                    if (!to_int(t, a)) throw ParseError{"edge set id: " + line};
                     if (a < 1 || a > (int)boundary_edge.size()) throw ParseError{"edge set id out of range: " + line};
                     const auto be = boundary_edge[(size_t)a - 1];
                     // the ids are renumbered a second time here, as in the reference (:424-427); a no-op
