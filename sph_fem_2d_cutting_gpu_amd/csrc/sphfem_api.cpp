@@ -978,9 +978,49 @@ int sphfem_interactions_setup_thermal_constants_tool(sphfem_ctx *ctx, const sphf
 }
 
 // ---- tool
+}  // extern "C"
+
+// every index of a tool description inside its array before anything is copied or indexed: the
+// host set-up below and the device kernels index with them unchecked
+static bool tool_desc_valid(const sphfem_tool_desc *d, std::string &why) {
+    const int nn = d->node_num, ne = d->ele_num, nz = d->nz_num, ns = d->seg_num;
+    if (nn <= 0 || ne <= 0 || nz <= 0 || ns <= 0) return why = "empty tool description", false;
+    if (!d->position || !d->node_num_el || !d->csr_row_ptr || !d->csr_col_ind || !d->K || !d->M || !d->f ||
+        !d->T || !d->left || !d->right || !d->n || !d->marks)
+        return why = "tool description: a required array is NULL", false;
+    auto node = [&](int v) { return v >= 0 && v < nn; };
+    for (int i = 0; i < nn; i++)
+        if (!std::isfinite(d->position[i].x) || !std::isfinite(d->position[i].y) || !(d->M[i] > 0.))
+            return why = "tool node " + std::to_string(i) + ": position not finite or mass not positive", false;
+    for (int e = 0; e < ne; e++) {
+        const sphfem_int4 el = d->node_num_el[e];
+        if (!node(el.x) || !node(el.y) || !node(el.z) || el.w < 0)
+            return why = "tool element " + std::to_string(e) + ": node id or colour out of range", false;
+    }
+    if (d->csr_row_ptr[0] != 0 || d->csr_row_ptr[nn] != nz) return why = "CSR row pointers do not span nz_num", false;
+    for (int i = 0; i < nn; i++)
+        if (d->csr_row_ptr[i + 1] < d->csr_row_ptr[i]) return why = "CSR row pointers decrease", false;
+    for (int k = 0; k < nz; k++)
+        if (!node(d->csr_col_ind[k])) return why = "CSR column index out of range", false;
+    for (int q = 0; q < ns; q++) {
+        const sphfem_int4 m = d->marks[q];
+        if (m.x < 0 || m.x >= ne || !node(m.z) || !node(m.w))
+            return why = "cutting segment " + std::to_string(q) + ": element or node out of range", false;
+    }
+    if (d->fixed_num < 0 || (d->fixed_num > 0 && (!d->fixed_elems || !d->fixed_flags)))
+        return why = "fixed-boundary description incomplete", false;
+    for (int j = 0; d->fixed_elems && j < d->fixed_num; j++)
+        if (d->fixed_elems[j] < 0 || d->fixed_elems[j] >= ne) return why = "fixed-boundary element out of range", false;
+    return true;
+}
+
+extern "C" {
+
 int sphfem_tool_create(sphfem_ctx *ctx, const sphfem_tool_desc *d, sphfem_tool **out) {
-    if (!d || !out || d->node_num <= 0 || d->ele_num <= 0 || d->nz_num <= 0 || d->seg_num <= 0) return SPHFEM_ERR_ARG;
+    if (!d || !out) return SPHFEM_ERR_ARG;
     *out = nullptr;
+    std::string why;
+    if (!tool_desc_valid(d, why)) return ctx ? fail(ctx, SPHFEM_ERR_ARG, why) : SPHFEM_ERR_ARG;
     return guarded(ctx, [&] {
         auto *t = new sphfem_tool();
         hipStream_t s = ctx->stream;

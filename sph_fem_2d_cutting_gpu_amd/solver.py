@@ -267,48 +267,55 @@ def _p(a, dt):
     return a, a.ctypes.data_as(C.c_void_p)
 
 
+def tool_desc(case: cases.Case, remeshable: bool = True):
+    """The sphfem_tool_desc of a case's tool (host arrays of set_up_segments_from_mesh and
+    set_up_FEM_tool_gpu) and the arrays it points into, which must outlive its use."""
+    tm = case.tool
+    init = cases.initial_tool(case)
+    keep = []
+
+    def arr(a, dt):
+        a2, ptr = _p(a, dt)
+        keep.append(a2)
+        return ptr
+
+    d = _lib.ToolDesc()
+    d.node_num, d.ele_num, d.nz_num = len(tm.nodes), len(tm.elems), len(tm.col_ind)
+    d.seg_num, d.color_total, d.color_cutting_edge = len(tm.seg_left), tm.n_colors, tm.n_cut_colors
+    d.position = arr(init["node_pos"], np.float64)
+    d.node_num_el = arr(init["elem"], np.int32)
+    d.layer = arr(init["layer"], np.int32)
+    d.csr_row_ptr = arr(init["row_ptr"], np.int32)
+    d.csr_col_ind = arr(init["col_ind"], np.int32)
+    d.K = arr(init["K"], np.float64)
+    d.M = arr(init["M"], np.float64)
+    d.f = arr(init["f"], np.float64)
+    d.T = arr(init["T"], np.float64)
+    d.left = arr(init["left"], np.float64)
+    d.right = arr(init["right"], np.float64)
+    d.n = arr(init["n"], np.float64)
+    d.marks = arr(init["marks"], np.int32)
+    sc = init["scalars"]
+    d.vel = _lib.Double2(sc["vel_x"], sc["vel_y"])
+    d.tl_point_ref = _lib.Double2(sc["tl_x"], sc["tl_y"])
+    d.br_point = _lib.Double2(sc["br_x"], sc["br_y"])
+    d.ref_dist, d.lowest_point, d.right_point = sc["ref_dist"], sc["lowest_point"], sc["right_point"]
+    d.node_flags = arr(init["node_flags"], np.int32) if remeshable else None
+    d.fixed_num = len(init["fixed_elems"]) if remeshable else 0
+    d.fixed_elems = arr(init["fixed_elems"], np.int32)
+    ff = np.zeros((len(init["fixed_elems"]), 4), np.int32)
+    ff[:, :3] = init["fixed_flags"]
+    d.fixed_flags = arr(ff, np.int32)
+    return d, keep
+
+
 class tool_FEM:
     """Rigid FEM tool with its thermal solver and cutting-edge segments (tool_FEM.h:34-83)."""
 
     def __init__(self, ctx: Context, case: cases.Case, remeshable: bool = True):
         """remeshable: pass the node flags and fixed boundary the remesh entry points need"""
         self.ctx = ctx
-        tm = case.tool
-        init = cases.initial_tool(case)
-        keep = []
-
-        def arr(a, dt):
-            a2, ptr = _p(a, dt)
-            keep.append(a2)
-            return ptr
-
-        d = _lib.ToolDesc()
-        d.node_num, d.ele_num, d.nz_num = len(tm.nodes), len(tm.elems), len(tm.col_ind)
-        d.seg_num, d.color_total, d.color_cutting_edge = len(tm.seg_left), tm.n_colors, tm.n_cut_colors
-        d.position = arr(init["node_pos"], np.float64)
-        d.node_num_el = arr(init["elem"], np.int32)
-        d.layer = arr(init["layer"], np.int32)
-        d.csr_row_ptr = arr(init["row_ptr"], np.int32)
-        d.csr_col_ind = arr(init["col_ind"], np.int32)
-        d.K = arr(init["K"], np.float64)
-        d.M = arr(init["M"], np.float64)
-        d.f = arr(init["f"], np.float64)
-        d.T = arr(init["T"], np.float64)
-        d.left = arr(init["left"], np.float64)
-        d.right = arr(init["right"], np.float64)
-        d.n = arr(init["n"], np.float64)
-        d.marks = arr(init["marks"], np.int32)
-        sc = init["scalars"]
-        d.vel = _lib.Double2(sc["vel_x"], sc["vel_y"])
-        d.tl_point_ref = _lib.Double2(sc["tl_x"], sc["tl_y"])
-        d.br_point = _lib.Double2(sc["br_x"], sc["br_y"])
-        d.ref_dist, d.lowest_point, d.right_point = sc["ref_dist"], sc["lowest_point"], sc["right_point"]
-        d.node_flags = arr(init["node_flags"], np.int32) if remeshable else None
-        d.fixed_num = len(init["fixed_elems"]) if remeshable else 0
-        d.fixed_elems = arr(init["fixed_elems"], np.int32)
-        ff = np.zeros((len(init["fixed_elems"]), 4), np.int32)
-        ff[:, :3] = init["fixed_flags"]
-        d.fixed_flags = arr(ff, np.int32)
+        d, keep = tool_desc(case, remeshable)
         self.h = C.c_void_p()
         ctx.call("sphfem_tool_create", C.byref(d), C.byref(self.h))
         self.n_nodes, self.n_seg, self.nnz = d.node_num, d.seg_num, d.nz_num
