@@ -329,7 +329,11 @@ int sphfem_leap_frog_correct(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_parti
  * SPHFEM_PAIRS_STAGED first copies each wavefront's window positions into LDS (pair_block unused,
  * one wavefront per workgroup).  At the same pair_chunk both give the same sums bit for bit. */
 enum { SPHFEM_PAIRS_LIST = 0, SPHFEM_PAIRS_STAGED = 1 };
-typedef struct { int pair_block, pair_chunk, update_block, pair_kernel; } sphfem_tuning;
+/* resort_one_block: the step's full-sort fallback (first step, or more particles changed cells than
+ * the incremental re-sort takes) by one workgroup in one launch instead of the multi-block radix
+ * sort's per-pass launches, which are queued every step and return at once on the incremental
+ * path; the same permutation.  Only for N <= 2^19 (rejected above). */
+typedef struct { int pair_block, pair_chunk, update_block, pair_kernel, resort_one_block; } sphfem_tuning;
 int sphfem_leap_frog_set_tuning(sphfem_ctx *ctx, sphfem_leap_frog *lf, const sphfem_tuning *tuning);
 int sphfem_leap_frog_get_tuning(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_tuning *tuning);
 /* leap_frog.cu:162-248: one full step (fused kernels); tool may be NULL for a tool-free workload.

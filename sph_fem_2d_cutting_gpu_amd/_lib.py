@@ -40,7 +40,8 @@ ToolKinematics = _s("ToolKinematics", ["vel_x", "vel_y", "tl_x", "tl_y", "br_x",
                                        "lowest_point", "right_point", "flank_retreat", "flank_contact"])
 
 
-Tuning = _s("Tuning", ["pair_block", "pair_chunk", "update_block", "pair_kernel"], C.c_int)
+Tuning = _s("Tuning", ["pair_block", "pair_chunk", "update_block", "pair_kernel", "resort_one_block"], C.c_int)
+SORT_ONE_BLOCK_MAX = 1 << 19
 PAIRS_LIST, PAIRS_STAGED = 0, 1
 
 

@@ -8,6 +8,9 @@ Tunables (all result-neutral except the FP32 summation blocks of the FP32C form)
   * pair_kernel  -- FP32C pair kernel: the list kernel (every neighbour record gathered through L1)
                     or the staged kernel (each wavefront's window positions copied into LDS first;
                     one wavefront per workgroup, so pair_block does not apply); same sums
+  * resort_one_block -- the step's full-sort fallback by one workgroup in one launch instead of the
+                    multi-block radix sort's eight per-step launches (N <= 2^19; tried with the best
+                    of the others: it pays when launches dominate and full sorts are rare)
   * graph_steps  -- steps per captured graph in a batch (2 ... 32): fewer graph launches per step
 The tuner times each candidate on the live workload (it advances the simulation: run it in the
 warm-up), keeps the fastest and can store the choice in a JSON cache keyed by workload size,
@@ -30,12 +33,12 @@ GRAPH_STEPS = (2, 4, 8, 16, 32)
 
 def candidates(precision):
     if precision == _lib.FP32C:
-        listed = [dict(pair_block=b, pair_chunk=c, update_block=u, pair_kernel=_lib.PAIRS_LIST)
+        listed = [dict(pair_block=b, pair_chunk=c, update_block=u, pair_kernel=_lib.PAIRS_LIST, resort_one_block=0)
                   for b, c, u in itertools.product(PAIR_BLOCKS, PAIR_CHUNKS, UPDATE_BLOCKS)]
-        staged = [dict(pair_block=256, pair_chunk=c, update_block=u, pair_kernel=_lib.PAIRS_STAGED)
+        staged = [dict(pair_block=256, pair_chunk=c, update_block=u, pair_kernel=_lib.PAIRS_STAGED, resort_one_block=0)
                   for c, u in itertools.product(PAIR_CHUNKS, UPDATE_BLOCKS)]
         return listed + staged
-    return [dict(pair_block=b, pair_chunk=32, update_block=u, pair_kernel=_lib.PAIRS_LIST)
+    return [dict(pair_block=b, pair_chunk=32, update_block=u, pair_kernel=_lib.PAIRS_LIST, resort_one_block=0)
             for b, u in itertools.product(PAIR_BLOCKS, UPDATE_BLOCKS)]
 
 
@@ -60,6 +63,14 @@ def tune_simulation(sim, precision, steps=6, repeats=2, pair_kernel=None) -> dic
         rows.append((t, cand))
     sim.ctx.check_error()
     t_best, best = min(rows, key=lambda r: r[0])
+    if sim.case.N <= _lib.SORT_ONE_BLOCK_MAX:
+        cand = dict(best, resort_one_block=1)
+        sim.set_tuning(**cand)
+        t = min(_time_steps(sim, steps) for _ in range(repeats))
+        rows.append((t, cand))
+        sim.ctx.check_error()
+        if t < t_best:
+            t_best, best = t, cand
     sim.set_tuning(**best)
     return {"best": best, "ms_per_step": t_best * 1e3,
             "timings": [{**c, "ms_per_step": t * 1e3} for t, c in rows]}

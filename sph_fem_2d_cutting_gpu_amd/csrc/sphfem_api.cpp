@@ -1640,7 +1640,9 @@ int sphfem_leap_frog_set_tuning(sphfem_ctx *ctx, sphfem_leap_frog *lf, const sph
     const bool ok = (t.pair_block == 128 || t.pair_block == 256) &&
                     (t.pair_chunk == 16 || t.pair_chunk == 32 || t.pair_chunk == 48) &&
                     (t.update_block == 128 || t.update_block == 256) &&
-                    (t.pair_kernel == SPHFEM_PAIRS_LIST || t.pair_kernel == SPHFEM_PAIRS_STAGED);
+                    (t.pair_kernel == SPHFEM_PAIRS_LIST || t.pair_kernel == SPHFEM_PAIRS_STAGED) &&
+                    (t.resort_one_block == 0 ||
+                     (t.resort_one_block == 1 && lf->num_part <= (unsigned)SPHFEM_SORT_ONE_BLOCK_MAX));
     if (!ok) return fail(ctx, SPHFEM_ERR_ARG, "unsupported tuning");
     lf->tuning = t;
     return SPHFEM_OK;
@@ -1737,7 +1739,7 @@ static int step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p,
         // the cell table is cleared by the hash pass (cells of the previous keys) except when the
         // previous keys are unknown
         if (g->first_sort) launch_fill_int(ctx, g->cell_start, g->num_cell, -1);
-        launch_resort(ctx, g, *p);
+        launch_resort(ctx, g, *p, lf->tuning.resort_one_block != 0);
         g->first_sort = false;
     }
     const RateView r_old = rate_view(lf->rate32[lf->rate_cur], nr);
@@ -1834,7 +1836,7 @@ static bool graph_settings_hold(const GraphCase &g) {
     return std::memcmp(&g.ctx->c, &g.consts, sizeof(DevConsts)) == 0 && lf->precision == g.precision &&
            lf->tuning.pair_block == g.tuning.pair_block && lf->tuning.pair_chunk == g.tuning.pair_chunk &&
            lf->tuning.update_block == g.tuning.update_block && lf->tuning.pair_kernel == g.tuning.pair_kernel &&
-           lf->intermediates == g.intermediates;
+           lf->tuning.resort_one_block == g.tuning.resort_one_block && lf->intermediates == g.intermediates;
 }
 
 extern "C" {

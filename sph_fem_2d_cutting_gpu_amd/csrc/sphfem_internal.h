@@ -12,6 +12,8 @@
 
 // largest number of particles that may change cells in one step for the incremental re-sort
 #define SPHFEM_RESORT_CAP 4096
+// largest N whose full-sort fallback may run in one workgroup (tuning resort_one_block)
+#define SPHFEM_SORT_ONE_BLOCK_MAX (1 << 19)
 // blocks of the bounding-box reduction of an unlocked grid
 #define SPHFEM_BBOX_BLOCKS 1024
 // d_flags slot set by the reorder when a live particle carries a non-finite pair input (the pair
@@ -213,7 +215,7 @@ struct sphfem_rigid_tool {
 struct sphfem_leap_frog {
     unsigned int num_part = 0, num_cell = 0;
     int precision = SPHFEM_FP64;
-    sphfem_tuning tuning = {256, 32, 256, SPHFEM_PAIRS_LIST};
+    sphfem_tuning tuning = {256, 32, 256, SPHFEM_PAIRS_LIST, 0};
     sphfem_double2 *pos_init = nullptr, *vel_init = nullptr;
     sphfem_double4 *S_init = nullptr;
     double *rho_init = nullptr, *T_init = nullptr;
@@ -279,7 +281,7 @@ void launch_rothlin_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p);
 void launch_cells_dense(sphfem_ctx *ctx, const int *sorted_keys, int N, int num_cell, int *cell_start, int *cell_end);
 void launch_iota(sphfem_ctx *ctx, int *ptr, int n);
 void launch_bbox(sphfem_ctx *ctx, const sphfem_particles &p, double *part, double *out);
-void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p);
+void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool one_block_fallback);
 void launch_fill_int(sphfem_ctx *ctx, int *ptr, long n, int value);
 void launch_reorder_full(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p);
 // FP32C: the pre-step rates are read from the carry buffer rates (in the order of p)

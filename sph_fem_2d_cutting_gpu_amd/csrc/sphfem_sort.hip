@@ -185,6 +185,113 @@ __global__ void __launch_bounds__(SORT_BS) k_downsweep(const int *__restrict__ k
     }
 }
 
+// The full stable LSD radix sort of the step's fallback (mode 1) by one workgroup, for up to
+// SPHFEM_SORT_ONE_BLOCK_MAX particles: one launch, which returns at once on the incremental path,
+// in place of the per-pass upsweep / digit-scan / downsweep launches.  The passes, the digit
+// offsets and the stable order within a pass (tiles in increasing index; within a tile waves, item
+// rows and lanes in index order, as k_downsweep ranks them) are those of the multi-block sort, so
+// the permutation is the same.  Unsorted keys in keys_in (values: their positions); passes ping-pong
+// between (ka, va) and (kb, vb) and the result ends in (ka, va).  kb may be keys_in: pass 0 reads
+// keys_in and writes ka, later passes no longer need it.  Every lane runs every loop (bounds are
+// uniform), so the ballots see whole wavefronts.
+__global__ void __launch_bounds__(SORT_BS) k_sort_one_block(const int *keys_in, int *ka, int *va, int *kb, int *vb,
+                                                            int N, int passes, int *flags, const int *mode) {
+    if (*mode != 1) return;
+    __shared__ int off[RADIX];
+    __shared__ int cnt[RADIX];
+    __shared__ int wcount[SORT_BS / WAVE][RADIX];
+    const int tid = threadIdx.x;
+    const int w = tid / WAVE;
+    const int lane = tid % WAVE;
+    const uint64_t lt = lanemask_lt();
+    const int *kin = keys_in, *vin = nullptr;
+    int *kout = ka, *vout = va;
+    for (int ps = 0; ps < passes; ps++) {
+        const int shift = 8 * ps;
+        // digit totals of the pass, then their exclusive scan
+        cnt[tid] = 0;
+        __syncthreads();
+        for (int i = tid; i < N; i += SORT_BS) atomicAdd(&cnt[(kin[i] >> shift) & (RADIX - 1)], 1);
+        __syncthreads();
+        for (int o = 1; o < RADIX; o <<= 1) {
+            const int v = tid >= o ? cnt[tid - o] : 0;
+            __syncthreads();
+            cnt[tid] += v;
+            __syncthreads();
+        }
+        off[tid] = tid ? cnt[tid - 1] : 0;
+        __syncthreads();
+        for (int t0 = 0; t0 < N; t0 += SORT_TILE) {
+            for (int k = 0; k < SORT_BS / WAVE; k++) wcount[k][tid] = 0;
+            __syncthreads();
+            int key[SORT_ITEMS], val[SORT_ITEMS], rank[SORT_ITEMS];
+            const int base = t0 + w * (SORT_ITEMS * WAVE);
+#pragma unroll
+            for (int r = 0; r < SORT_ITEMS; r++) {
+                const int i = base + r * WAVE + lane;
+                const bool valid = i < N;
+                const int k = valid ? kin[i] : 0;
+                key[r] = k;
+                val[r] = valid ? (vin ? vin[i] : i) : 0;
+                const int d = (k >> shift) & (RADIX - 1);
+                uint64_t peers = __ballot(valid);
+#pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    const uint64_t m = __ballot((d >> b) & 1);
+                    peers &= ((d >> b) & 1) ? m : ~m;
+                }
+                const int before = __popcll(peers & lt);
+                const int c = __popcll(peers);
+                int old = 0;
+                if (valid) old = wcount[w][d];
+                __builtin_amdgcn_wave_barrier();
+                if (valid && before == 0) wcount[w][d] = old + c;
+                __builtin_amdgcn_wave_barrier();
+                rank[r] = valid ? old + before : -1;
+            }
+            __syncthreads();
+            {   // per digit: prefix over the waves of the tile, and the tile's count
+                int run = 0;
+                for (int k = 0; k < SORT_BS / WAVE; k++) {
+                    const int v = wcount[k][tid];
+                    wcount[k][tid] = run;
+                    run += v;
+                }
+                cnt[tid] = run;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < SORT_ITEMS; r++) {
+                if (rank[r] < 0) continue;
+                const int d = (key[r] >> shift) & (RADIX - 1);
+                const int dst = off[d] + wcount[w][d] + rank[r];
+                if ((unsigned)dst >= (unsigned)N) {   // cannot happen for consistent counts
+                    atomicOr(&flags[0], 2);
+                    continue;
+                }
+                kout[dst] = key[r];
+                vout[dst] = val[r];
+            }
+            __syncthreads();
+            off[tid] += cnt[tid];
+            __syncthreads();
+        }
+        // this pass's output, written by all waves, is the next pass's input
+        __threadfence();
+        __syncthreads();
+        kin = kout;
+        vin = vout;
+        kout = kout == ka ? kb : ka;
+        vout = vout == va ? vb : va;
+    }
+    if (kin != ka) {
+        for (int i = tid; i < N; i += SORT_BS) {
+            ka[i] = kin[i];
+            va[i] = vin[i];
+        }
+    }
+}
+
 __global__ void k_fill_int(int *p, long n, int v) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long stride = (long)gridDim.x * blockDim.x;
@@ -449,8 +556,12 @@ __global__ void k_copy_keys_if_full(const int *__restrict__ src, int *__restrict
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) dst[i] = src[i];
 }
 
-// Re-sort of the step: new keys and the permutation end in p.hash / p.idx (as launch_radix_sort)
-void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p) {
+static void launch_resort_radix(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, int passes);
+
+// Re-sort of the step: new keys and the permutation end in p.hash / p.idx (as launch_radix_sort);
+// the full-sort fallback by one workgroup (one launch) when asked and N allows, else the multi-block
+// radix sort
+void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool one_block_fallback) {
     const int N = (int)p.N;
     const int nb = nblk(N, SORT_TILE);
     int *flags = ctx->d_flags;
@@ -469,6 +580,37 @@ void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p) {
     }
     // full stable radix sort (mode 1): unsorted keys are in keys_alt
     const int passes = (g->key_bits + 7) / 8;
+    if (one_block_fallback && N <= SPHFEM_SORT_ONE_BLOCK_MAX) {
+        KTimer t(ctx, "radix_one_block");
+        k_sort_one_block<<<1, SORT_BS, 0, ctx->stream>>>(g->keys_alt, p.hash, p.idx, g->keys_alt, g->vals_alt, N,
+                                                         passes, flags, mode);
+    } else
+        launch_resort_radix(ctx, g, p, passes);
+    // incremental path (mode 2)
+    {
+        KTimer t(ctx, "resort_compact");
+        k_resort_compact<<<nblk(N, 256), 256, 0, ctx->stream>>>(g->keys_alt, N, g->cmask, g->wprefix, g->bprefix,
+                                                                g->clist, flags);
+    }
+    {
+        KTimer t(ctx, "resort_rank");
+        k_resort_rank<<<1, RESORT_SCAN_BS, 0, ctx->stream>>>(g->clist, p.hash, N, g->cmask, g->wprefix, g->bprefix,
+                                                              g->csorted, g->cdest, flags);
+    }
+    {
+        KTimer t(ctx, "resort_scatter");
+        k_resort_scatter<<<nblk(N, 256), 256, 0, ctx->stream>>>(g->keys_alt, N, g->cmask, g->wprefix, g->bprefix,
+                                                                g->csorted, g->cdest, p.hash, p.idx, flags);
+    }
+}
+
+// the multi-block fallback of launch_resort: per pass an upsweep (pass 0's histogram came with the
+// hash), the digit scan and the downsweep, each returning at once on the incremental path
+static void launch_resort_radix(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, int passes) {
+    const int N = (int)p.N;
+    const int nb = nblk(N, SORT_TILE);
+    int *flags = ctx->d_flags;
+    const int *mode = flags + 3;
     int *kbuf[2] = {p.hash, g->keys_alt};
     int *vbuf[2] = {p.idx, g->vals_alt};
     int src = 1;
@@ -497,22 +639,6 @@ void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p) {
         kin = kbuf[dst];
         vin = vbuf[dst];
         src = dst;
-    }
-    // incremental path (mode 2)
-    {
-        KTimer t(ctx, "resort_compact");
-        k_resort_compact<<<nblk(N, 256), 256, 0, ctx->stream>>>(g->keys_alt, N, g->cmask, g->wprefix, g->bprefix,
-                                                                g->clist, flags);
-    }
-    {
-        KTimer t(ctx, "resort_rank");
-        k_resort_rank<<<1, RESORT_SCAN_BS, 0, ctx->stream>>>(g->clist, p.hash, N, g->cmask, g->wprefix, g->bprefix,
-                                                              g->csorted, g->cdest, flags);
-    }
-    {
-        KTimer t(ctx, "resort_scatter");
-        k_resort_scatter<<<nblk(N, 256), 256, 0, ctx->stream>>>(g->keys_alt, N, g->cmask, g->wprefix, g->bprefix,
-                                                                g->csorted, g->cdest, p.hash, p.idx, flags);
     }
 }
 

@@ -5,7 +5,8 @@ from sph_fem_2d_cutting_gpu_amd import _lib, autotune
 def test_candidate_sets():
     c32 = autotune.candidates(_lib.FP32C)
     assert len(c32) == 18
-    assert {"pair_block": 256, "pair_chunk": 32, "update_block": 256, "pair_kernel": _lib.PAIRS_LIST} in c32
+    assert {"pair_block": 256, "pair_chunk": 32, "update_block": 256, "pair_kernel": _lib.PAIRS_LIST,
+            "resort_one_block": 0} in c32
     assert {c["pair_chunk"] for c in c32} == {16, 32, 48}
     staged = [c for c in c32 if c["pair_kernel"] == _lib.PAIRS_STAGED]
     assert len(staged) == 6 and {c["pair_chunk"] for c in staged} == {16, 32, 48}

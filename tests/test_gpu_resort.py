@@ -21,11 +21,14 @@ def fast_case():
     return cases.perturbed(cases.small_cutting_case(nx=48, ny=16, wear=True), vel_sd=0.05)
 
 
+@pytest.mark.parametrize("one_block", [0, 1])
 @pytest.mark.parametrize("limit,expect_mode", [(4096, 2), (0, 1), (2, 1)])
-def test_resort_paths_match_oracle(limit, expect_mode):
+def test_resort_paths_match_oracle(limit, expect_mode, one_block):
+    """one_block: the full-sort fallback by one workgroup instead of the multi-block radix sort."""
     case = fast_case()
     o = Oracle(case)
     sim = S.Simulation(case)
+    sim.set_tuning(resort_one_block=one_block)
     sim.grid.set_resort_limit(limit)
     modes, changed = [], []
     for k in range(10):
@@ -55,17 +58,23 @@ def test_incremental_equals_full_on_200k_with_thousands_of_changes():
     inc = S.Simulation(case, precision=_lib.FP32C)
     full = S.Simulation(case, precision=_lib.FP32C)
     full.grid.set_resort_limit(0)
+    one = S.Simulation(case, precision=_lib.FP32C)      # the full sort by one workgroup every step
+    one.set_tuning(resort_one_block=1)
+    one.grid.set_resort_limit(0)
     seen = []
     for k in range(6):
         inc.step(1)
         full.step(1)
+        one.step(1)
         seen.append(inc.grid.resort_stats())
-    inc.ctx.check_error()
-    full.ctx.check_error()
+        assert one.grid.resort_stats()[0] == 1
+    for s_ in (inc, full, one):
+        s_.ctx.check_error()
     print("incremental (mode, changed):", seen)
     assert all(m == 2 for m, _ in seen[1:]), seen
     assert max(c for _, c in seen[1:]) > 500, seen
-    a = inc.state(("hash", "idx", "pos", "vel", "S", "rho", "T"))
-    b = full.state(("hash", "idx", "pos", "vel", "S", "rho", "T"))
-    for f in a:
+    fields = ("hash", "idx", "pos", "vel", "S", "rho", "T")
+    a, b, c = inc.state(fields), full.state(fields), one.state(fields)
+    for f in fields:
         assert_equal(f, a[f], b[f])
+        assert_equal(f"{f} (one-workgroup sort)", c[f], b[f])

@@ -33,7 +33,7 @@ def test_all_candidates_agree(prec, which):
         case, steps = cases.chip_case(n_target=50_000), 1      # 51,570 particles: a partial last wavefront
     ref = _run(case, prec, {"pair_block": 256, "pair_chunk": 32, "update_block": 256}, steps)
     tol = 1e-11 if prec == _lib.FP64 else 1e-6
-    for cand in autotune.candidates(prec):
+    for cand in autotune.candidates(prec) + [dict(autotune.candidates(prec)[0], resort_one_block=1)]:
         got = _run(case, prec, cand, steps)
         assert_equal(f"{cand} hash", got["hash"], ref["hash"])
         assert_equal(f"{cand} idx", got["idx"], ref["idx"])
@@ -77,9 +77,9 @@ def test_tune_simulation_applies_a_candidate():
     sim = S.Simulation(cases.small_cutting_case(nx=32, ny=12), precision=_lib.FP32C)
     sim.step(1)
     res = autotune.tune_simulation(sim, _lib.FP32C, steps=2, repeats=1)
-    assert res["best"] in autotune.candidates(_lib.FP32C)
+    assert dict(res["best"], resort_one_block=0) in autotune.candidates(_lib.FP32C)
     assert sim.get_tuning() == res["best"]
-    assert len(res["timings"]) == len(autotune.candidates(_lib.FP32C))
+    assert len(res["timings"]) == len(autotune.candidates(_lib.FP32C)) + 1     # + the one-block fallback
     sim.close()
 
 
