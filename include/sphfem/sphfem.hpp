@@ -720,6 +720,16 @@ class leap_frog {
         sphfem_shim::check(sphfem_leap_frog_import_rates(sphfem_shim::ctx(), m_handle, particles),
                            "leap_frog::import_rates");
     }
+    // launch parameters of the step kernels (no counterpart in the reference): pair block, FP32C list
+    // chunk, update block, FP32C pair kernel (list or LDS-staged), one-workgroup sort fallback
+    void set_tuning(const sphfem_tuning &tuning) {
+        sphfem_shim::check(sphfem_leap_frog_set_tuning(sphfem_shim::ctx(), m_handle, &tuning), "leap_frog::set_tuning");
+    }
+    sphfem_tuning get_tuning() const {
+        sphfem_tuning t{};
+        sphfem_shim::check(sphfem_leap_frog_get_tuning(sphfem_shim::ctx(), m_handle, &t), "leap_frog::get_tuning");
+        return t;
+    }
     void set_precision(int precision) {
         sphfem_shim::check(sphfem_leap_frog_set_precision(sphfem_shim::ctx(), m_handle, precision),
                            "leap_frog::set_precision");
