@@ -19,8 +19,6 @@
 // d_flags slot set by the reorder when a live particle carries a non-finite pair input (the pair
 // kernels then use the reference's full 5x5-cell window) and cleared by the update kernel
 #define SPHFEM_FLAG_NONFINITE 5
-// blocks of the grid-stride window form of the pair kernels
-#define SPHFEM_WINDOW_BLOCKS 512
 // records per step of each ordered segment sum (friction heat, contact heat exchange); the
 // reduction sorts them in LDS
 #define SPHFEM_SEG_SINK_CAP 4096

@@ -492,7 +492,9 @@ __device__ __forceinline__ void pairs_fp32c_one(const DevConsts &c, const double
     psum_T[i] = (float)(2. * d_T);
 }
 
-// in-support pair sums, one lane per sorted particle; nothing when nonfinite[0] is set
+// in-support pair sums, one lane per sorted particle; when nonfinite[0] is set (a live particle
+// with a non-finite input), the window form: every record of the 5x5 cells, the kernel zeroed
+// outside the support, so the non-finite values spread as in the reference
 template <int BS, int CHUNK>
 __global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *__restrict__ blanked,
                                                          const int *__restrict__ hash, int N, RecView rv,
@@ -502,9 +504,12 @@ __global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *_
     __shared__ int nb_list[CHUNK * BS];
     const int i = xcd_logical_block() * BS + threadIdx.x;
     if (i >= N) return;
-    if (*nonfinite != 0) return;
-    pairs_fp32c_one<BS, CHUNK, false>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, nb_list + threadIdx.x,
-                                      GlobalPos{rv.r0}, nullptr);
+    if (*nonfinite != 0)
+        pairs_fp32c_one<BS, CHUNK, true>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, nb_list + threadIdx.x,
+                                         GlobalPos{rv.r0}, nullptr);
+    else
+        pairs_fp32c_one<BS, CHUNK, false>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, nb_list + threadIdx.x,
+                                          GlobalPos{rv.r0}, nullptr);
 }
 
 // LDS-staged form of k_pairs_fp32c, one wavefront per workgroup.  When the wavefront's 64
@@ -535,9 +540,15 @@ __global__ void __launch_bounds__(64) k_pairs_fp32c_staged(DevConsts c, const do
     unsigned short *list16 = reinterpret_cast<unsigned short *>(lds + POS_BYTES);
     int *list32 = reinterpret_cast<int *>(lds);
     const int base = xcd_logical_block() * 64;
-    if (base >= N || *nonfinite != 0) return;           // the whole wavefront
+    if (base >= N) return;                              // the whole wavefront
     const int lane = (int)threadIdx.x;
     const int i = base + lane;
+    if (*nonfinite != 0) {                              // the window form, as k_pairs_fp32c
+        if (i < N)
+            pairs_fp32c_one<64, CHUNK, true>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, list32 + lane,
+                                             GlobalPos{rv.r0}, nullptr);
+        return;
+    }
     const int last_i = base + 63 < N ? base + 63 : N - 1;
     const int nx = c.geom.nx, ny = c.geom.ny;
     const int h0 = hash[base], h1 = hash[last_i];
@@ -605,21 +616,6 @@ __global__ void __launch_bounds__(64) k_pairs_fp32c_staged(DevConsts c, const do
         pairs_fp32c_one<64, CHUNK, false>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, list32 + lane,
                                           GlobalPos{rv.r0}, nullptr);
     }
-}
-
-// the window form over every particle (grid-stride loop) when nonfinite[0] is set, else nothing
-template <int BS, int CHUNK>
-__global__ void __launch_bounds__(BS) k_pairs_fp32c_window(DevConsts c, const double *__restrict__ blanked,
-                                                                const int *__restrict__ hash, int N, RecView rv,
-                                                                const int *__restrict__ cs,
-                                                                const int *__restrict__ ce, float4 *__restrict__ psum,
-                                                                float *__restrict__ psum_T,
-                                                                const int *__restrict__ nonfinite) {
-    __shared__ int nb_list[CHUNK * BS];
-    if (*nonfinite == 0) return;
-    for (int i = (int)blockIdx.x * BS + (int)threadIdx.x; i < N; i += (int)gridDim.x * BS)
-        pairs_fp32c_one<BS, CHUNK, true>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, nb_list + threadIdx.x,
-                                         GlobalPos{rv.r0}, nullptr);
 }
 
 __device__ __forceinline__ double interaction_alpha(const DevConsts &c, double is_tool) {
@@ -833,25 +829,26 @@ void launch_leapfrog(sphfem_ctx *ctx, int which, const sphfem_particles &p, sphf
 // FP64 pair sums of the sorted predicted state (interactions_gpu.cu:143-379) into psum64, for the
 // update kernel; a kernel of its own so that the pair loop keeps 3 waves per SIMD instead of the
 // update's 2
-// nonfinite[0] is set by the reorder when a live particle carries a non-finite pair input: then the
-// in-support kernel does nothing and the window kernel (a grid-stride loop, which returns at once
-// otherwise) evaluates every particle with the reference's full window.
-template <int BS, bool WIN>
+// nonfinite[0] is set by the reorder when a live particle carries a non-finite pair input: then
+// every particle is evaluated with the reference's full window.
+template <int BS>
 __global__ void __launch_bounds__(BS) k_pairs_fp64(DevConsts c, sphfem_particles B, const int *__restrict__ cs,
                                                    const int *__restrict__ ce, const double *__restrict__ pre_vol,
                                                    const sphfem_double4 *__restrict__ pre_s,
                                                    PairSums *__restrict__ psum64, const int *__restrict__ nonfinite) {
     constexpr int CHUNK = 48;
     __shared__ int nb_list[CHUNK * BS];
-    if ((*nonfinite != 0) != WIN) return;
     const int N = (int)B.N;
-    const int stride = WIN ? (int)gridDim.x * BS : N;
-    for (int i = (WIN ? (int)blockIdx.x : xcd_logical_block()) * BS + (int)threadIdx.x; i < N; i += stride) {
-        if (B.blanked[i] == 1.) continue;
-        const double thermal_alpha = interaction_alpha(c, B.tool_particle[i]);
-        psum64[i] = pair_sums_fp64<true, BS, CHUNK, WIN>(c, B, i, thermal_alpha, cs, ce, pre_vol, pre_s,
-                                                         nb_list + threadIdx.x);
-    }
+    const int i = xcd_logical_block() * BS + (int)threadIdx.x;
+    if (i >= N || B.blanked[i] == 1.) return;
+    const double thermal_alpha = interaction_alpha(c, B.tool_particle[i]);
+    // the window form when a live particle carries a non-finite input (see pair_sums_fp64)
+    if (*nonfinite != 0)
+        psum64[i] = pair_sums_fp64<true, BS, CHUNK, true>(c, B, i, thermal_alpha, cs, ce, pre_vol, pre_s,
+                                                          nb_list + threadIdx.x);
+    else
+        psum64[i] = pair_sums_fp64<true, BS, CHUNK, false>(c, B, i, thermal_alpha, cs, ce, pre_vol, pre_s,
+                                                           nb_list + threadIdx.x);
 }
 
 // RIGID: the rigid-tool step (leap_frog.cu:99-160): friction heat after the corrector
@@ -1162,11 +1159,6 @@ static void launch_pairs_fp32c_tuned(sphfem_ctx *ctx, const sphfem_particles &B,
         else if (pb == 256 && pc == 48) launch_pairs_fp32c<256, 48>(ctx, B, lf, cs, ce);
         else launch_pairs_fp32c<256, 32>(ctx, B, lf, cs, ce);
     }
-    KTimer tw(ctx, "pairs_fp32c_window");
-    const size_t n = B.N;
-    RecView rv{lf->rec, lf->rec + n + 4, lf->rec + 2 * n + 4, lf->rec + 3 * n + 4};
-    k_pairs_fp32c_window<256, 32><<<SPHFEM_WINDOW_BLOCKS, 256, 0, ctx->stream>>>(
-        ctx->c, B.blanked, B.hash, (int)B.N, rv, cs, ce, lf->psum, lf->psum_T, ctx->d_flags + SPHFEM_FLAG_NONFINITE);
 }
 
 void launch_interactions_fp32c(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_leap_frog *lf, const int *cs,
@@ -1219,17 +1211,11 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
             KTimer tp(ctx, "pairs_fp64");
             const int *nf = ctx->d_flags + SPHFEM_FLAG_NONFINITE;
             if (lf->tuning.pair_block == 128)
-                k_pairs_fp64<128, false><<<nblk(B.N, 128), 128, 0, ctx->stream>>>(ctx->c, B, cs, ce, lf->pre_vol,
-                                                                                   lf->pre_s, lf->psum64, nf);
+                k_pairs_fp64<128><<<nblk(B.N, 128), 128, 0, ctx->stream>>>(ctx->c, B, cs, ce, lf->pre_vol, lf->pre_s,
+                                                                            lf->psum64, nf);
             else
-                k_pairs_fp64<256, false><<<nblk(B.N, 256), 256, 0, ctx->stream>>>(ctx->c, B, cs, ce, lf->pre_vol,
-                                                                                   lf->pre_s, lf->psum64, nf);
-        }
-        {
-            KTimer tp(ctx, "pairs_fp64_window");
-            k_pairs_fp64<256, true><<<SPHFEM_WINDOW_BLOCKS, 256, 0, ctx->stream>>>(ctx->c, B, cs, ce, lf->pre_vol,
-                                                                                  lf->pre_s, lf->psum64,
-                                                                                  ctx->d_flags + SPHFEM_FLAG_NONFINITE);
+                k_pairs_fp64<256><<<nblk(B.N, 256), 256, 0, ctx->stream>>>(ctx->c, B, cs, ce, lf->pre_vol, lf->pre_s,
+                                                                            lf->psum64, nf);
         }
         KTimer tm(ctx, "fused_update");
 #define SPHFEM_UPDATE(BSZ, RG)                                                                              \

@@ -194,13 +194,42 @@ __global__ void __launch_bounds__(SORT_BS) k_downsweep(const int *__restrict__ k
 // between (ka, va) and (kb, vb) and the result ends in (ka, va).  kb may be keys_in: pass 0 reads
 // keys_in and writes ka, later passes no longer need it.  Every lane runs every loop (bounds are
 // uniform), so the ballots see whole wavefronts.
+// It also does k_resort_scan's work first (the changed counts of the hash pass -> re-sort mode), so the
+// step's sort path is one launch fewer.
 __global__ void __launch_bounds__(SORT_BS) k_sort_one_block(const int *keys_in, int *ka, int *va, int *kb, int *vb,
-                                                            int N, int passes, int *flags, const int *mode) {
-    if (*mode != 1) return;
+                                                            int N, int passes, int *flags, int *bprefix, int nb,
+                                                            int force_full, int cap) {
     __shared__ int off[RADIX];
     __shared__ int cnt[RADIX];
     __shared__ int wcount[SORT_BS / WAVE][RADIX];
+    __shared__ int s_mode;
     const int tid = threadIdx.x;
+    {   // exclusive scan of the tiles' changed counts, as k_resort_scan
+        int carry = 0;
+        for (int c0 = 0; c0 < nb; c0 += SORT_BS) {
+            const int j = c0 + tid;
+            const int v = j < nb ? bprefix[j] : 0;
+            cnt[tid] = v;
+            __syncthreads();
+            for (int o = 1; o < SORT_BS; o <<= 1) {
+                const int u = tid >= o ? cnt[tid - o] : 0;
+                __syncthreads();
+                cnt[tid] += u;
+                __syncthreads();
+            }
+            if (j < nb) bprefix[j] = carry + cnt[tid] - v;
+            carry += cnt[SORT_BS - 1];
+            __syncthreads();
+        }
+        if (tid == 0) {
+            bprefix[nb] = carry;
+            flags[4] = carry;
+            s_mode = (force_full || carry > cap) ? 1 : 2;
+            flags[3] = s_mode;
+        }
+        __syncthreads();
+    }
+    if (s_mode != 1) return;
     const int w = tid / WAVE;
     const int lane = tid % WAVE;
     const uint64_t lt = lanemask_lt();
@@ -565,27 +594,28 @@ void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool on
     const int N = (int)p.N;
     const int nb = nblk(N, SORT_TILE);
     int *flags = ctx->d_flags;
-    const int *mode = flags + 3;
     {
         KTimer t(ctx, "hash");
         k_hash_mark<<<nb, SORT_BS, 0, ctx->stream>>>(ctx->c, p.blanked, p.pos, p.hash, g->keys_alt, g->cell_start,
                                                      g->first_sort ? 0 : 1, N, g->num_cell, g->hist, nb, flags,
                                                      g->cmask, g->wprefix, g->bprefix);
     }
-    {
-        KTimer t(ctx, "resort_scan");
-        const int cap = std::min(g->resort_limit, SPHFEM_RESORT_CAP);
-        k_resort_scan<<<1, RESORT_SCAN_BS, 0, ctx->stream>>>(g->bprefix, nb, flags,
-                                                              (g->first_sort || cap <= 0) ? 1 : 0, cap);
-    }
-    // full stable radix sort (mode 1): unsorted keys are in keys_alt
+    // re-sort mode (full or incremental), then the full stable radix sort in mode 1: unsorted keys
+    // are in keys_alt
+    const int cap = std::min(g->resort_limit, SPHFEM_RESORT_CAP);
+    const int force_full = (g->first_sort || cap <= 0) ? 1 : 0;
     const int passes = (g->key_bits + 7) / 8;
     if (one_block_fallback && N <= SPHFEM_SORT_ONE_BLOCK_MAX) {
-        KTimer t(ctx, "radix_one_block");
+        KTimer t(ctx, "resort_scan_one_block");
         k_sort_one_block<<<1, SORT_BS, 0, ctx->stream>>>(g->keys_alt, p.hash, p.idx, g->keys_alt, g->vals_alt, N,
-                                                         passes, flags, mode);
-    } else
+                                                         passes, flags, g->bprefix, nb, force_full, cap);
+    } else {
+        {
+            KTimer t(ctx, "resort_scan");
+            k_resort_scan<<<1, RESORT_SCAN_BS, 0, ctx->stream>>>(g->bprefix, nb, flags, force_full, cap);
+        }
         launch_resort_radix(ctx, g, p, passes);
+    }
     // incremental path (mode 2)
     {
         KTimer t(ctx, "resort_compact");
