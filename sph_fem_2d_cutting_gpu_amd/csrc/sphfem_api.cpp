@@ -1717,8 +1717,10 @@ static int step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p,
     // leap_frog.cu:175 (a no-op for a locked grid)
     if (int rc = update_geometry(ctx, g, *p, 2.)) return rc;
     ctx->c.geom = g->geom;
-    if (t) launch_tool_begin(ctx, t);
-    else launch_fill_int(ctx, ctx->d_flags + 1, 2, 0);
+    // the FEM tool's step start runs inside the hash pass of the Green re-sort (below), else here
+    const bool tool_in_hash = t && g->kind != SPHFEM_GRID_ROTHLIN;
+    if (t && !tool_in_hash) launch_tool_begin(ctx, t);
+    else if (!t) launch_fill_int(ctx, ctx->d_flags + 1, 2, 0);
     if (rt) launch_rigid_advance(ctx, rt);
     // the source of each sorted position: the permutation of the sort (Green), or the identity
     // after the Rothlin sort has gathered in place
@@ -1739,7 +1741,7 @@ static int step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p,
         // the cell table is cleared by the hash pass (cells of the previous keys) except when the
         // previous keys are unknown
         if (g->first_sort) launch_fill_int(ctx, g->cell_start, g->num_cell, -1);
-        launch_resort(ctx, g, *p, lf->tuning.resort_one_block != 0);
+        launch_resort(ctx, g, *p, lf->tuning.resort_one_block != 0, tool_in_hash ? t : nullptr);
         g->first_sort = false;
     }
     const RateView r_old = rate_view(lf->rate32[lf->rate_cur], nr);

@@ -279,7 +279,9 @@ void launch_rothlin_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p);
 void launch_cells_dense(sphfem_ctx *ctx, const int *sorted_keys, int N, int num_cell, int *cell_start, int *cell_end);
 void launch_iota(sphfem_ctx *ctx, int *ptr, int n);
 void launch_bbox(sphfem_ctx *ctx, const sphfem_particles &p, double *part, double *out);
-void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool one_block_fallback);
+// t: the FEM tool whose step start (tool_begin_item) runs in extra workgroups of the hash pass, or NULL
+void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool one_block_fallback,
+                   sphfem_tool *t = nullptr);
 void launch_fill_int(sphfem_ctx *ctx, int *ptr, long n, int value);
 void launch_reorder_full(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p);
 // FP32C: the pre-step rates are read from the carry buffer rates (in the order of p)
@@ -313,6 +315,29 @@ void launch_rates_gather(sphfem_ctx *ctx, const RateView &src, const RateView &d
 void launch_fric_heat(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t);
 void launch_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t);
 void launch_tool_begin(sphfem_ctx *ctx, sphfem_tool *t);
+
+// the start of a step for the FEM tool, item i of max(nodes, segments): reset_FEM_server's
+// T_old = T and calculate_temp(0.5 dt) for the nodes; zero the wear accumulators
+// (segments_wear_rate_reset) and, item 0, the contact-list and segment-sink counters
+__device__ __forceinline__ void tool_begin_item(const sphfem_tool &t, double half_dt, int *counters, int i) {
+    if (i == 0) {
+        counters[1] = 0;
+        counters[2] = 0;
+        counters[SPHFEM_FLAG_SINK_FRIC] = 0;
+        counters[SPHFEM_FLAG_SINK_S2] = 0;
+    }
+    if (i < t.n_nodes) {
+        double T = t.T[i];
+        t.T_old[i] = T;
+        t.T[i] = half_dt * t.k[i] + T;
+    }
+    if (i < t.n_seg) {
+        t.wear_rate[i] = {0., 0.};
+        t.wear_nodes[i] = 0.;
+        t.physical_para[i] = {0., 0., 0., 0.};
+        t.sliding_force[i] = 0.;
+    }
+}
 void launch_tool_fem_end(sphfem_ctx *ctx, sphfem_tool *t);
 void launch_tool_move(sphfem_ctx *ctx, sphfem_tool *t, double dt, bool segments, bool nodes);
 void launch_wear(sphfem_ctx *ctx, sphfem_tool *t);

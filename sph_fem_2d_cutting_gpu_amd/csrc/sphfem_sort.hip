@@ -413,13 +413,21 @@ __device__ __forceinline__ int changed_before(const unsigned long long *__restri
 // grid.cu:25-42 hash (as k_hash, with the digit-0 histogram for the full-sort fallback), plus the
 // changed bit of every position, the changed count before each 64 positions within the tile
 // (wlocal) and the tile's total (bcount)
+// Workgroups past nblocks (tool_blocks of them, when a FEM tool is given) do the tool's step start
+// instead (tool_begin_item), which shares no data with the hash pass: one launch fewer per step.
 __global__ void __launch_bounds__(SORT_BS) k_hash_mark(DevConsts c, const double *__restrict__ blanked,
                                                        const sphfem_double2 *__restrict__ pos,
                                                        const int *__restrict__ old_hash, int *__restrict__ hash_out,
                                                        int *__restrict__ cell_start, int clear_old, int N,
                                                        int num_cell, int *__restrict__ hist0, int nblocks,
-                                                       int *__restrict__ flags, unsigned long long *__restrict__ cmask,
-                                                       int *__restrict__ wlocal, int *__restrict__ bcount) {
+                                                       int *flags, unsigned long long *__restrict__ cmask,
+                                                       int *__restrict__ wlocal, int *__restrict__ bcount,
+                                                       sphfem_tool tool, int tool_blocks, double half_dt) {
+    if ((int)blockIdx.x >= nblocks) {
+        if (tool_blocks > 0)
+            tool_begin_item(tool, half_dt, flags, ((int)blockIdx.x - nblocks) * SORT_BS + (int)threadIdx.x);
+        return;
+    }
     __shared__ int lh[RADIX];
     __shared__ int wc[SORT_TILE / 64];
     for (int d = threadIdx.x; d < RADIX; d += SORT_BS) lh[d] = 0;
@@ -590,15 +598,18 @@ static void launch_resort_radix(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particle
 // Re-sort of the step: new keys and the permutation end in p.hash / p.idx (as launch_radix_sort);
 // the full-sort fallback by one workgroup (one launch) when asked and N allows, else the multi-block
 // radix sort
-void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool one_block_fallback) {
+void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool one_block_fallback,
+                   sphfem_tool *tool) {
     const int N = (int)p.N;
     const int nb = nblk(N, SORT_TILE);
     int *flags = ctx->d_flags;
     {
-        KTimer t(ctx, "hash");
-        k_hash_mark<<<nb, SORT_BS, 0, ctx->stream>>>(ctx->c, p.blanked, p.pos, p.hash, g->keys_alt, g->cell_start,
-                                                     g->first_sort ? 0 : 1, N, g->num_cell, g->hist, nb, flags,
-                                                     g->cmask, g->wprefix, g->bprefix);
+        KTimer t(ctx, tool ? "hash_tool_begin" : "hash");
+        const int tb = tool ? nblk(std::max(std::max(tool->n_nodes, tool->n_seg), 1), SORT_BS) : 0;
+        k_hash_mark<<<nb + tb, SORT_BS, 0, ctx->stream>>>(ctx->c, p.blanked, p.pos, p.hash, g->keys_alt,
+                                                          g->cell_start, g->first_sort ? 0 : 1, N, g->num_cell,
+                                                          g->hist, nb, flags, g->cmask, g->wprefix, g->bprefix,
+                                                          tool ? *tool : sphfem_tool{}, tb, 0.5 * ctx->c.dt);
     }
     // re-sort mode (full or incremental), then the full stable radix sort in mode 1: unsorted keys
     // are in keys_alt

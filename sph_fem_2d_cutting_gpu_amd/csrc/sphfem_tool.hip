@@ -28,27 +28,8 @@ __device__ __forceinline__ bool robin_term(const DevConsts &c, const sphfem_tool
     return true;
 }
 
-// reset_FEM_server's T_old = T and calculate_temp(0.5 dt) for the nodes; zero the wear
-// accumulators (segments_wear_rate_reset) and the contact-list counters.
 __global__ void __launch_bounds__(256) k_tool_begin(sphfem_tool t, double half_dt, int *counters) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        counters[1] = 0;
-        counters[2] = 0;
-        counters[SPHFEM_FLAG_SINK_FRIC] = 0;
-        counters[SPHFEM_FLAG_SINK_S2] = 0;
-    }
-    if (i < t.n_nodes) {
-        double T = t.T[i];
-        t.T_old[i] = T;
-        t.T[i] = half_dt * t.k[i] + T;
-    }
-    if (i < t.n_seg) {
-        t.wear_rate[i] = {0., 0.};
-        t.wear_nodes[i] = 0.;
-        t.physical_para[i] = {0., 0., 0., 0.};
-        t.sliding_force[i] = 0.;
-    }
+    tool_begin_item(t, half_dt, counters, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 void launch_tool_begin(sphfem_ctx *ctx, sphfem_tool *t) {
