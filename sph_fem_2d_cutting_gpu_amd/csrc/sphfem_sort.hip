@@ -194,15 +194,24 @@ __global__ void __launch_bounds__(SORT_BS) k_downsweep(const int *__restrict__ k
 // between (ka, va) and (kb, vb) and the result ends in (ka, va).  kb may be keys_in: pass 0 reads
 // keys_in and writes ka, later passes no longer need it.  Every lane runs every loop (bounds are
 // uniform), so the ballots see whole wavefronts.
-// It also does k_resort_scan's work first (the changed counts of the hash pass -> re-sort mode), so the
-// step's sort path is one launch fewer.
+// It also does k_resort_scan's work first (the changed counts of the hash pass -> re-sort mode) and,
+// on the incremental path, k_resort_compact's and k_resort_rank's (the changed particles gathered
+// into LDS from the changed bits, then ranked): the step's sort is then three launches (hash pass,
+// this, scatter) instead of five.
+__device__ __forceinline__ int changed_before(const unsigned long long *__restrict__ cmask,
+                                              const int *__restrict__ wlocal, const int *__restrict__ bprefix,
+                                              int p, int N, int m);
+
 __global__ void __launch_bounds__(SORT_BS) k_sort_one_block(const int *keys_in, int *ka, int *va, int *kb, int *vb,
                                                             int N, int passes, int *flags, int *bprefix, int nb,
-                                                            int force_full, int cap) {
+                                                            int force_full, int cap,
+                                                            const unsigned long long *cmask, const int *wlocal,
+                                                            unsigned long long *csorted, int4 *cdest) {
     __shared__ int off[RADIX];
     __shared__ int cnt[RADIX];
     __shared__ int wcount[SORT_BS / WAVE][RADIX];
-    __shared__ int s_mode;
+    __shared__ unsigned long long comp[SPHFEM_RESORT_CAP];
+    __shared__ int s_mode, s_m;
     const int tid = threadIdx.x;
     {   // exclusive scan of the tiles' changed counts, as k_resort_scan
         int carry = 0;
@@ -225,9 +234,52 @@ __global__ void __launch_bounds__(SORT_BS) k_sort_one_block(const int *keys_in, 
             bprefix[nb] = carry;
             flags[4] = carry;
             s_mode = (force_full || carry > cap) ? 1 : 2;
+            s_m = carry;
             flags[3] = s_mode;
         }
+        __threadfence();
         __syncthreads();
+    }
+    if (s_mode == 2) {
+        // the changed particles in position order (k_resort_compact), as (new key, position)
+        const int m = s_m;
+        const int words = (N + 63) >> 6;
+        for (int w = tid; w < words; w += SORT_BS) {
+            const unsigned long long bits = cmask[w];
+            unsigned long long rest = bits;
+            while (rest) {
+                const int b = __ffsll((long long)rest) - 1;
+                rest &= rest - 1;
+                const int i = (w << 6) + b;
+                const int k = changed_before(cmask, wlocal, bprefix, i, N, m);
+                if ((unsigned)k < (unsigned)m)
+                    comp[k] = ((unsigned long long)(unsigned)keys_in[i] << 32) | (unsigned)i;
+                else
+                    atomicOr(&flags[0], 2);
+            }
+        }
+        __syncthreads();
+        // their ranks and destinations (k_resort_rank), the old keys being ka (the sorted keys)
+        for (int k = tid; k < m; k += SORT_BS) {
+            const unsigned long long ck = comp[k];
+            int r = 0;
+            for (int q = 0; q < m; q++) r += comp[q] < ck;
+            int lo = 0, hi = N;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                const unsigned long long cm = ((unsigned long long)(unsigned)ka[mid] << 32) | (unsigned)mid;
+                if (cm < ck) lo = mid + 1;
+                else hi = mid;
+            }
+            const int dest = r + lo - changed_before(cmask, wlocal, bprefix, lo, N, m);
+            if ((unsigned)dest >= (unsigned)N || r >= m) {
+                atomicOr(&flags[0], 2);
+                continue;
+            }
+            csorted[r] = ck;
+            cdest[r] = make_int4(dest, (int)(ck >> 32), (int)(ck & 0xffffffffu), 0);
+        }
+        return;
     }
     if (s_mode != 1) return;
     const int w = tid / WAVE;
@@ -617,26 +669,27 @@ void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool on
     const int force_full = (g->first_sort || cap <= 0) ? 1 : 0;
     const int passes = (g->key_bits + 7) / 8;
     if (one_block_fallback && N <= SPHFEM_SORT_ONE_BLOCK_MAX) {
-        KTimer t(ctx, "resort_scan_one_block");
+        KTimer t(ctx, "resort_one_block");
         k_sort_one_block<<<1, SORT_BS, 0, ctx->stream>>>(g->keys_alt, p.hash, p.idx, g->keys_alt, g->vals_alt, N,
-                                                         passes, flags, g->bprefix, nb, force_full, cap);
+                                                         passes, flags, g->bprefix, nb, force_full, cap, g->cmask,
+                                                         g->wprefix, g->csorted, g->cdest);
     } else {
         {
             KTimer t(ctx, "resort_scan");
             k_resort_scan<<<1, RESORT_SCAN_BS, 0, ctx->stream>>>(g->bprefix, nb, flags, force_full, cap);
         }
         launch_resort_radix(ctx, g, p, passes);
-    }
-    // incremental path (mode 2)
-    {
-        KTimer t(ctx, "resort_compact");
-        k_resort_compact<<<nblk(N, 256), 256, 0, ctx->stream>>>(g->keys_alt, N, g->cmask, g->wprefix, g->bprefix,
-                                                                g->clist, flags);
-    }
-    {
-        KTimer t(ctx, "resort_rank");
-        k_resort_rank<<<1, RESORT_SCAN_BS, 0, ctx->stream>>>(g->clist, p.hash, N, g->cmask, g->wprefix, g->bprefix,
-                                                              g->csorted, g->cdest, flags);
+        // incremental path (mode 2)
+        {
+            KTimer t(ctx, "resort_compact");
+            k_resort_compact<<<nblk(N, 256), 256, 0, ctx->stream>>>(g->keys_alt, N, g->cmask, g->wprefix,
+                                                                    g->bprefix, g->clist, flags);
+        }
+        {
+            KTimer t(ctx, "resort_rank");
+            k_resort_rank<<<1, RESORT_SCAN_BS, 0, ctx->stream>>>(g->clist, p.hash, N, g->cmask, g->wprefix,
+                                                                  g->bprefix, g->csorted, g->cdest, flags);
+        }
     }
     {
         KTimer t(ctx, "resort_scatter");
