@@ -271,6 +271,33 @@ __device__ __forceinline__ int xcd_logical_block() {
     return x < r ? x * (q + 1) + local : r * (q + 1) + (x - r) * q + local;
 }
 
+// Streaming (nontemporal) loads for per-particle state that a kernel reads once and that is not
+// read again before the step's other streams have passed through the caches: the load does not
+// displace cache lines that are reused soon.  Same values as plain loads.
+typedef double sphfem_v2d __attribute__((ext_vector_type(2)));
+typedef float sphfem_v2f __attribute__((ext_vector_type(2)));
+typedef float sphfem_v4f __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ double ld_stream(const double *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float ld_stream(const float *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ int ld_stream(const int *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ sphfem_double2 ld_stream(const sphfem_double2 *p) {
+    const sphfem_v2d v = __builtin_nontemporal_load(reinterpret_cast<const sphfem_v2d *>(p));
+    return {v.x, v.y};
+}
+__device__ __forceinline__ sphfem_double4 ld_stream(const sphfem_double4 *p) {
+    const sphfem_v2d a = __builtin_nontemporal_load(reinterpret_cast<const sphfem_v2d *>(p));
+    const sphfem_v2d b = __builtin_nontemporal_load(reinterpret_cast<const sphfem_v2d *>(p) + 1);
+    return {a.x, a.y, b.x, b.y};
+}
+__device__ __forceinline__ float2 ld_stream(const float2 *p) {
+    const sphfem_v2f v = __builtin_nontemporal_load(reinterpret_cast<const sphfem_v2f *>(p));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float4 ld_stream(const float4 *p) {
+    const sphfem_v4f v = __builtin_nontemporal_load(reinterpret_cast<const sphfem_v4f *>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // sort / cells (sphfem_sort.hip)
 void launch_hash(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, bool clear_old, int *hist0);
 void launch_radix_sort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool hash_done_with_hist0,

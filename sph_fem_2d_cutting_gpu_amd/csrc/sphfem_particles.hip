@@ -873,31 +873,11 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
     if (i == 0) *nonfinite = 0;      // the pair kernels of this step have read it
     if (i >= (int)B.N) return;
     // FP32C: the reorder wrote only the neighbour records; every per-particle field is gathered here
-    // from the unsorted arrays A through idx, and the sorted step result is written to B
+    // from the unsorted arrays A through idx, and the sorted step result is written to B.  All loads
+    // of the particle are issued before any branch on loaded values, so a wavefront waits for
+    // memory once (the kernel holds 3 wavefronts per SIMD); the rates and pair sums of blanked
+    // particles are read and not used.
     const int src = F32C ? B.idx[i] : i;
-    const double blanked0 = F32C ? A.blanked[src] : B.blanked[i];
-    const double tool = F32C ? A.tool_particle[src] : B.tool_particle[i];
-    const bool live = blanked0 != 1.;
-    const double thermal_alpha = interaction_alpha(c, tool);
-    // pair sums first: the per-particle state below is not live across the neighbour loop
-    PairSums r;
-    if (live) {
-        if constexpr (F32C) {
-            // sums of k_pairs_fp32c: stress divergence and viscous acceleration come summed per axis
-            const float4 g = psum[i], a = psum[(size_t)B.N + i];
-            r.v_der = {g.x, g.y, g.z, g.w};
-            r.S_der = {a.x, 0., a.y, 0.};
-            r.vel_t = {0., 0.};
-            r.pos_t = {a.z, a.w};
-            r.T_lapl = psum_T[i];
-        } else {
-            r = psum64[i];
-        }
-    }
-    const double fixed = F32C ? A.fixed[src] : B.fixed[i];
-    // FP32C: pos and T are the pre-step values (the corrector starts from them); velocity, stress and
-    // density are predicted again from the pre-step state and rates (leap_frog.cu:45-69, the same
-    // operations as in the reorder), as is the pressure (actions_gpu.cu:222-232)
     sphfem_double2 pos, vel;
     sphfem_double4 S;
     double rho, T, eps_pl, eps_pl_dot, pr = 0.;
@@ -906,32 +886,68 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
     sphfem_double2 v0 = {0., 0.};
     sphfem_double4 S0 = {0., 0., 0., 0.};
     double rho0 = 0.;
+    double blanked0, tool, fixed, h_src = 0.;
+    float2 vt_old = make_float2(0.f, 0.f);
+    float4 St_old = make_float4(0.f, 0.f, 0.f, 0.f), g_sum = St_old, a_sum = St_old;
+    float rt_old = 0.f, Tt_old = 0.f, T_lapl32 = 0.f;
     if constexpr (F32C) {
-        pos = A.pos[src];
-        v0 = A.vel[src];
-        S0 = A.S[src];
-        rho0 = A.rho[src];
-        T = A.T[src];
+        blanked0 = ld_stream(&A.blanked[src]);
+        tool = ld_stream(&A.tool_particle[src]);
+        fixed = ld_stream(&A.fixed[src]);
+        pos = ld_stream(&A.pos[src]);
+        v0 = ld_stream(&A.vel[src]);
+        S0 = ld_stream(&A.S[src]);
+        rho0 = ld_stream(&A.rho[src]);
+        T = ld_stream(&A.T[src]);
+        eps_pl = ld_stream(&A.eps_pl[src]);
+        eps_pl_dot = ld_stream(&A.eps_pl_dot[src]);
+        on_seg0 = ld_stream(&A.on_seg[src]);
+        h_src = ld_stream(&A.h[src]);
+        vt_old = ld_stream(&r_old.vel_t[src]);          // the FP32 derivative carry
+        St_old = ld_stream(&r_old.S_t[src]);
+        rt_old = ld_stream(&r_old.rho_t[src]);
+        Tt_old = ld_stream(&r_old.T_t[src]);
+        g_sum = psum[i];                             // sums of k_pairs_fp32c
+        a_sum = psum[(size_t)B.N + i];
+        T_lapl32 = psum_T[i];
+    } else {
+        blanked0 = B.blanked[i];
+        tool = B.tool_particle[i];
+        fixed = B.fixed[i];
+    }
+    const bool live = blanked0 != 1.;
+    const double thermal_alpha = interaction_alpha(c, tool);
+    PairSums r;
+    if (live) {
+        if constexpr (F32C) {
+            // stress divergence and viscous acceleration come summed per axis
+            r.v_der = {g_sum.x, g_sum.y, g_sum.z, g_sum.w};
+            r.S_der = {a_sum.x, 0., a_sum.y, 0.};
+            r.vel_t = {0., 0.};
+            r.pos_t = {a_sum.z, a_sum.w};
+            r.T_lapl = T_lapl32;
+        } else {
+            r = psum64[i];
+        }
+    }
+    // FP32C: pos and T are the pre-step values (the corrector starts from them); velocity, stress and
+    // density are predicted again from the pre-step state and rates (leap_frog.cu:45-69, the same
+    // operations as in the reorder), as is the pressure (actions_gpu.cu:222-232)
+    if constexpr (F32C) {
         vel = v0;
         S = S0;
         rho = rho0;
         if (live) {
             const double hdt = 0.5 * c.dt;
-            const float2 vt = r_old.vel_t[src];          // the FP32 derivative carry
-            const float4 St = r_old.S_t[src];
-            const double rt = r_old.rho_t[src];
-            vel.x = vel.x + hdt * vt.x;
-            vel.y = vel.y + hdt * vt.y;
-            S.x = S.x + hdt * St.x;
-            S.y = S.y + hdt * St.y;
-            S.z = S.z + hdt * St.z;
-            S.w = S.w + hdt * St.w;
-            rho = rho + hdt * rt;
+            vel.x = vel.x + hdt * vt_old.x;
+            vel.y = vel.y + hdt * vt_old.y;
+            S.x = S.x + hdt * St_old.x;
+            S.y = S.y + hdt * St_old.y;
+            S.z = S.z + hdt * St_old.z;
+            S.w = S.w + hdt * St_old.w;
+            rho = rho + hdt * (double)rt_old;
         }
         if (tool == 0.) pr = eos_pressure(c, rho);
-        eps_pl = A.eps_pl[src];
-        eps_pl_dot = A.eps_pl_dot[src];
-        on_seg0 = A.on_seg[src];
         if (skip_contact && on_seg0 == -1) {
             // every writer of on_seg = -1 (contact s1, the far-from-tool reset) zeroes fc, ft and n
             // with it (actions_gpu.cu:455-491): nothing to read far from the tool
@@ -940,9 +956,9 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
             ft0 = z;
             nrm0 = z;
         } else {
-            fc0 = A.fc[src];
-            ft0 = A.ft[src];
-            nrm0 = A.n[src];
+            fc0 = ld_stream(&A.fc[src]);
+            ft0 = ld_stream(&A.ft[src]);
+            nrm0 = ld_stream(&A.n[src]);
         }
     } else {
         pos = B.pos[i];
@@ -968,7 +984,7 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
     if (live) {
         pos_t = r.pos_t;
         vel_t = r.vel_t;
-        if constexpr (F32C) T_t = thermal_alpha != 0. ? thermal_alpha * r.T_lapl : (double)r_old.T_t[src];
+        if constexpr (F32C) T_t = thermal_alpha != 0. ? thermal_alpha * r.T_lapl : (double)Tt_old;
         else T_t = thermal_alpha != 0. ? thermal_alpha * r.T_lapl : A.T_t[A.idx[i]];
         if (tool == 0.) {
             S_t = jaumann_rate(c, r.v_der, S);        // actions_gpu.cu:280-314
@@ -1088,7 +1104,7 @@ __global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particl
         B.ft[i] = ft;
         B.n[i] = nrm;
         B.on_seg[i] = on_seg;
-        B.h[i] = A.h[src];
+        B.h[i] = h_src;
         B.fixed[i] = fixed;
         B.tool_particle[i] = tool;
         B.p[i] = tool == 0. ? pr : A.p[i];

@@ -467,7 +467,9 @@ __device__ __forceinline__ int changed_before(const unsigned long long *__restri
 // (wlocal) and the tile's total (bcount)
 // Workgroups past nblocks (tool_blocks of them, when a FEM tool is given) do the tool's step start
 // instead (tool_begin_item), which shares no data with the hash pass: one launch fewer per step.
-__global__ void __launch_bounds__(SORT_BS) k_hash_mark(DevConsts c, const double *__restrict__ blanked,
+constexpr int HASH_BS = 1024;                       // 16 wavefronts per tile, 4 items per lane
+constexpr int HASH_ITEMS = SORT_TILE / HASH_BS;
+__global__ void __launch_bounds__(HASH_BS) k_hash_mark(DevConsts c, const double *__restrict__ blanked,
                                                        const sphfem_double2 *__restrict__ pos,
                                                        const int *__restrict__ old_hash, int *__restrict__ hash_out,
                                                        int *__restrict__ cell_start, int clear_old, int N,
@@ -477,22 +479,34 @@ __global__ void __launch_bounds__(SORT_BS) k_hash_mark(DevConsts c, const double
                                                        sphfem_tool tool, int tool_blocks, double half_dt) {
     if ((int)blockIdx.x >= nblocks) {
         if (tool_blocks > 0)
-            tool_begin_item(tool, half_dt, flags, ((int)blockIdx.x - nblocks) * SORT_BS + (int)threadIdx.x);
+            tool_begin_item(tool, half_dt, flags, ((int)blockIdx.x - nblocks) * HASH_BS + (int)threadIdx.x);
         return;
     }
     __shared__ int lh[RADIX];
     __shared__ int wc[SORT_TILE / 64];
-    for (int d = threadIdx.x; d < RADIX; d += SORT_BS) lh[d] = 0;
+    for (int d = threadIdx.x; d < RADIX; d += HASH_BS) lh[d] = 0;
     __syncthreads();
     const int base = blockIdx.x * SORT_TILE;
-    for (int it = 0; it < SORT_ITEMS; it++) {
-        const int i = base + it * SORT_BS + threadIdx.x;
+    // every load of the lane's items first, then the hashes: the pass waits for memory once
+    int ohs[HASH_ITEMS];
+    double bls[HASH_ITEMS];
+    sphfem_double2 pss[HASH_ITEMS];
+#pragma unroll
+    for (int it = 0; it < HASH_ITEMS; it++) {
+        const int i = base + it * HASH_BS + threadIdx.x;
+        ohs[it] = i < N ? old_hash[i] : 0;
+        bls[it] = i < N ? blanked[i] : 1.;
+        pss[it] = i < N ? pos[i] : sphfem_double2{0., 0.};
+    }
+#pragma unroll
+    for (int it = 0; it < HASH_ITEMS; it++) {
+        const int i = base + it * HASH_BS + threadIdx.x;
         bool changed = false;
         if (i < N) {
-            const int oh = old_hash[i];
+            const int oh = ohs[it];
             if (clear_old && oh >= 0 && oh < num_cell) cell_start[oh] = -1;
             int h = oh;
-            if (blanked[i] != 1.) h = hash_of(c, pos[i]);
+            if (bls[it] != 1.) h = hash_of(c, pss[it]);
             if (h < 0 || h >= num_cell) {
                 atomicOr(&flags[0], 1);
                 h = h < 0 ? 0 : num_cell - 1;
@@ -508,7 +522,7 @@ __global__ void __launch_bounds__(SORT_BS) k_hash_mark(DevConsts c, const double
         }
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < RADIX; d += SORT_BS) hist0[d * nblocks + blockIdx.x] = lh[d];
+    for (int d = threadIdx.x; d < RADIX; d += HASH_BS) hist0[d * nblocks + blockIdx.x] = lh[d];
     if (threadIdx.x == 0) {
         int run = 0;
         for (int k = 0; k < SORT_TILE / 64; k++) {
@@ -657,8 +671,8 @@ void launch_resort(sphfem_ctx *ctx, sphfem_grid *g, sphfem_particles &p, bool on
     int *flags = ctx->d_flags;
     {
         KTimer t(ctx, tool ? "hash_tool_begin" : "hash");
-        const int tb = tool ? nblk(std::max(std::max(tool->n_nodes, tool->n_seg), 1), SORT_BS) : 0;
-        k_hash_mark<<<nb + tb, SORT_BS, 0, ctx->stream>>>(ctx->c, p.blanked, p.pos, p.hash, g->keys_alt,
+        const int tb = tool ? nblk(std::max(std::max(tool->n_nodes, tool->n_seg), 1), HASH_BS) : 0;
+        k_hash_mark<<<nb + tb, HASH_BS, 0, ctx->stream>>>(ctx->c, p.blanked, p.pos, p.hash, g->keys_alt,
                                                           g->cell_start, g->first_sort ? 0 : 1, N, g->num_cell,
                                                           g->hist, nb, flags, g->cmask, g->wprefix, g->bprefix,
                                                           tool ? *tool : sphfem_tool{}, tb, 0.5 * ctx->c.dt);
@@ -1149,16 +1163,16 @@ __global__ void __launch_bounds__(256) k_reorder_records(DevConsts c, const sphf
     if (i >= N) return;
     int h, s;
     if (!cell_bounds(p, i, N, num_cell, cell_start, cell_end, flags, h, s)) return;
-    sphfem_double2 pos = p.pos[s], vel = p.vel[s];
-    sphfem_double4 S = p.S[s];
-    double rho = p.rho[s], T = p.T[s];
-    const double blanked = p.blanked[s], tool = p.tool_particle[s];
+    sphfem_double2 pos = ld_stream(&p.pos[s]), vel = ld_stream(&p.vel[s]);
+    sphfem_double4 S = ld_stream(&p.S[s]);
+    double rho = ld_stream(&p.rho[s]), T = ld_stream(&p.T[s]);
+    const double blanked = ld_stream(&p.blanked[s]), tool = ld_stream(&p.tool_particle[s]);
     if (blanked != 1.) {
         const double dt = c.dt;
         // the FP32 derivative carry of the FP32C form, in the order of p
-        const float2 pt = rates.pos_t[s], vt = rates.vel_t[s];
-        const float4 St = rates.S_t[s];
-        const double rt = rates.rho_t[s], Tt = rates.T_t[s];
+        const float2 pt = ld_stream(&rates.pos_t[s]), vt = ld_stream(&rates.vel_t[s]);
+        const float4 St = ld_stream(&rates.S_t[s]);
+        const double rt = ld_stream(&rates.rho_t[s]), Tt = ld_stream(&rates.T_t[s]);
         pos.x = pos.x + 0.5 * dt * pt.x;
         pos.y = pos.y + 0.5 * dt * pt.y;
         vel.x = vel.x + 0.5 * dt * vt.x;
@@ -1176,7 +1190,7 @@ __global__ void __launch_bounds__(256) k_reorder_records(DevConsts c, const sphf
         pr = eos_pressure(c, rho);
         R = artificial_stress(c, rho, pr, S);
     }
-    const double hh = p.h[s];
+    const double hh = ld_stream(&p.h[s]);
     flag_nonfinite(flags, blanked, pos, vel, tool == 0. ? S : sphfem_double4{0., 0., 0., 0.}, rho, T, hh, pr, R);
     write_records(c, rec, N, i, h, pos, vel, S, rho, T, blanked, tool, hh, pr, R);
 }
