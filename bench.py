@@ -23,12 +23,27 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 INTERACTION_KERNEL = {"fp64": "pairs_fp64", "fp32c": "pairs_fp32c"}
+FUSED_KERNEL = "pairs_update_fp32c"      # the FP32C pair kernel that also runs the update
+# Algorithmic bytes per particle of the FP32C update: reads idx 4, blanked, tool flag, fixed 3x8,
+# pos 16, vel 16, S 32, rho 8, T 8, eps_pl 8, eps_pl_dot 8, on_seg 4, h 8, the FP32 rate carry
+# (vel_t 8, S_t 16, rho_t 4, T_t 4) = 176; writes the sorted state (pos 16, vel 16, S 32, rho, T,
+# eps_pl, eps_pl_dot, blanked 5x8, fc, ft, n 3x16, on_seg 4, h, fixed, tool flag, p 4x8) = 192 and
+# the new carry (pos_t 8, vel_t 8, S_t 16, rho_t 4, T_t 4) = 40.  The contact fields of particles
+# far from the tool are known to be zero and not read.
+UPDATE_BYTES_FP32C = 176 + 192 + 40
 
 
 def interaction_bytes(N, n_occ_cells, w):
     """Algorithmic bytes of the fused pair-sum stage: 14 input and 10 output scalars of w bytes,
     2 flag bytes per particle, 8 bytes per occupied cell."""
     return N * (24 * w + 2) + 8 * n_occ_cells
+
+
+def pairs_update_bytes(N, n_occ_cells):
+    """Algorithmic bytes of the FP32C pair kernel that runs each particle's update after its sums:
+    the pair stage's inputs (14 FP32 scalars, 2 flag bytes per particle, 8 bytes per occupied cell;
+    its sums stay in registers) plus the update's reads and writes."""
+    return N * (14 * 4 + 2 + UPDATE_BYTES_FP32C) + 8 * n_occ_cells
 
 
 def measured_traffic(kernel, workload_key):
@@ -39,13 +54,17 @@ def measured_traffic(kernel, workload_key):
         return None
     with open(path) as f:
         t = json.load(f)
-    prefix = {"pairs_fp32c": "sphfem::k_pairs_fp32c<", "pairs_fp64": "sphfem::k_pairs_fp64<",
-              "fused_update": "sphfem::k_fused_update<false,",
-              "update_fp32c": "sphfem::k_fused_update<true,"}.get(kernel)
-    if prefix is None:
+    # (name prefix, template-argument tail) of the kernel's instantiations
+    sel = {"pairs_fp32c": ("sphfem::k_pairs_fp32c<", ", false, false>"),
+           "pairs_update_fp32c": ("sphfem::k_pairs_fp32c<", ", true, false>"),
+           "pairs_fp64": ("sphfem::k_pairs_fp64<", ">"),
+           "fused_update": ("sphfem::k_fused_update<false,", ", false>"),
+           "update_fp32c": ("sphfem::k_fused_update<true,", ", false>")}.get(kernel)
+    if sel is None:
         return None
+    prefix, tail = sel
     # the launch shapes of one kernel move the same bytes: take the most profiled one
-    hits = [v for k, v in t.get("per_launch", {}).items() if k.startswith(prefix) and "true>" not in k[len(prefix):]]
+    hits = [v for k, v in t.get("per_launch", {}).items() if k.startswith(prefix) and k.endswith(tail)]
     if not hits:
         return None
     v = max(hits, key=lambda x: x["dispatches"])
@@ -83,6 +102,9 @@ def make_case(name):
 
 
 KERNEL_PASS = {"fp64": "interactions_ref", "fp32c": "pairs_fp32c"}
+
+
+PAIR_KERNELS = {"list": 0, "staged": 1, "list_fused": 2, "staged_fused": 3}   # the library's pair_kernel values
 
 
 def run_kernel_pass(case, precision, steps, warmup, tune=True, pair_kernel="auto"):
@@ -123,8 +145,10 @@ def run_kernel_pass(case, precision, steps, warmup, tune=True, pair_kernel="auto
     if tune and prec == _lib.FP32C:
         rows = []
         for cand in autotune.candidates(prec):
-            if pair_kernel != "auto" and cand["pair_kernel"] != {"list": _lib.PAIRS_LIST,
-                                                                  "staged": _lib.PAIRS_STAGED}[pair_kernel]:
+            # the pass has no update: the fused forms would run the same pair kernels
+            if cand["pair_kernel"] in (_lib.PAIRS_LIST_FUSED, _lib.PAIRS_STAGED_FUSED):
+                continue
+            if pair_kernel != "auto" and cand["pair_kernel"] != PAIR_KERNELS[pair_kernel.replace("_fused", "")]:
                 continue
             sim.set_tuning(**cand)
             passes(1)
@@ -286,10 +310,17 @@ def run_form(case, precision, steps, warmup, breakdown=False, graph=True, tune=T
     tuning = None
     if tune:                                           # untimed: advances the warm-up
         from sph_fem_2d_cutting_gpu_amd import autotune
-        kernels = None if pair_kernel == "auto" else {"list": _lib.PAIRS_LIST, "staged": _lib.PAIRS_STAGED}[pair_kernel]
+        kernels = None if pair_kernel == "auto" else PAIR_KERNELS[pair_kernel]
         tuning = autotune.tune_simulation(sim, prec, steps=4, repeats=2, pair_kernel=kernels)["best"]
-    # interaction-kernel time (eager steps, events around that kernel only)
+    elif pair_kernel != "auto":
+        sim.set_tuning(pair_kernel=PAIR_KERNELS[pair_kernel])
+    # interaction-kernel time (eager steps, events around that kernel only): the FP32C pair kernel,
+    # or the pair kernel that also runs the update (not with a rigid tool, whose step runs the
+    # update kernel)
     kname = INTERACTION_KERNEL[precision]
+    if (prec == _lib.FP32C and case.rigid is None
+            and sim.get_tuning()["pair_kernel"] in (_lib.PAIRS_LIST_FUSED, _lib.PAIRS_STAGED_FUSED)):
+        kname = FUSED_KERNEL
     ctx.timing_select(kname)
     ctx.timing(True)
     ctx.timing_reset()
@@ -330,6 +361,7 @@ def run_form(case, precision, steps, warmup, breakdown=False, graph=True, tune=T
         for ms, n, name in rows:
             print(f"  {name:28s} {ms:9.4f}  x{n:.0f}", file=sys.stderr)
     sim.bench_tuning = tuning
+    sim.bench_kernel = kname
     return elapsed, (k_ms / 1e3) / max(k_n, 1), sim
 
 
@@ -400,8 +432,9 @@ def main():
     ap.add_argument("--eager", action="store_true", help="time eager steps instead of the captured step graph")
     ap.add_argument("--batch-cases", type=int, default=64, help="cases of the batch64 workload")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch parameters)")
-    ap.add_argument("--pair-kernel", default="auto", choices=["auto", "list", "staged"],
-                    help="FP32C pair kernel the tuner may choose: both (auto) or only one")
+    ap.add_argument("--pair-kernel", default="auto", choices=["auto"] + list(PAIR_KERNELS),
+                    help="FP32C pair kernel the tuner may choose: any (auto) or only one; the _fused forms "
+                         "also run the update")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -433,10 +466,11 @@ def main():
     elapsed, k_avg_s, sim = run_form(case, args.precision, args.steps, args.warmup, args.breakdown, graph,
                                      not args.no_tune, args.pair_kernel)
     tuning = sim.bench_tuning
+    kname = sim.bench_kernel
     cs, _ = sim.grid.download_cells()
     n_occ = int(np.count_nonzero(cs != -1))
     w = 8 if args.precision == "fp64" else 4
-    b_int = interaction_bytes(case.N, n_occ, w)
+    b_int = pairs_update_bytes(case.N, n_occ) if kname == FUSED_KERNEL else interaction_bytes(case.N, n_occ, w)
     achieved = b_int / k_avg_s / 1e9
     ups = case.N * args.steps / elapsed
     sim.close()
@@ -463,9 +497,8 @@ def main():
                    "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(INTERACTION_KERNEL[args.precision],
-                                                 {"ti64_4m": "ti64_4m"}.get(args.workload, args.workload)),
-                     "kernel": INTERACTION_KERNEL[args.precision], "kernel_ms": k_avg_s * 1e3,
+                     "traffic": measured_traffic(kname, args.workload),
+                     "kernel": kname, "kernel_ms": k_avg_s * 1e3,
                      "algorithmic_bytes": b_int, "occupied_cells": n_occ},
     }
     if args.precision == "fp32c" and not args.no_compare:

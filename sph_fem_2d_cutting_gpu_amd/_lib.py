@@ -42,7 +42,7 @@ ToolKinematics = _s("ToolKinematics", ["vel_x", "vel_y", "tl_x", "tl_y", "br_x",
 
 Tuning = _s("Tuning", ["pair_block", "pair_chunk", "update_block", "pair_kernel", "resort_one_block"], C.c_int)
 SORT_ONE_BLOCK_MAX = 1 << 19
-PAIRS_LIST, PAIRS_STAGED = 0, 1
+PAIRS_LIST, PAIRS_STAGED, PAIRS_LIST_FUSED, PAIRS_STAGED_FUSED = 0, 1, 2, 3
 
 
 class GeomConstants(C.Structure):

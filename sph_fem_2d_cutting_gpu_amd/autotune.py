@@ -7,7 +7,9 @@ Tunables (all result-neutral except the FP32 summation blocks of the FP32C form)
   * update_block -- update kernel workgroup size (128, 256)
   * pair_kernel  -- FP32C pair kernel: the list kernel (every neighbour record gathered through L1)
                     or the staged kernel (each wavefront's window positions copied into LDS first;
-                    one wavefront per workgroup, so pair_block does not apply); same sums
+                    one wavefront per workgroup, so pair_block does not apply); same sums.  Each
+                    also fused with the update (the particle's update right after its sums: no
+                    pair-sum round trip, update_block does not apply); same step result
   * resort_one_block -- the step's full-sort fallback by one workgroup in one launch instead of the
                     multi-block radix sort's eight per-step launches (N <= 2^19; tried with the best
                     of the others: it pays when launches dominate and full sorts are rare)
@@ -37,7 +39,11 @@ def candidates(precision):
                   for b, c, u in itertools.product(PAIR_BLOCKS, PAIR_CHUNKS, UPDATE_BLOCKS)]
         staged = [dict(pair_block=256, pair_chunk=c, update_block=u, pair_kernel=_lib.PAIRS_STAGED, resort_one_block=0)
                   for c, u in itertools.product(PAIR_CHUNKS, UPDATE_BLOCKS)]
-        return listed + staged
+        fused = [dict(pair_block=b, pair_chunk=c, update_block=256, pair_kernel=_lib.PAIRS_LIST_FUSED,
+                      resort_one_block=0) for b, c in itertools.product(PAIR_BLOCKS, PAIR_CHUNKS)]
+        fused += [dict(pair_block=256, pair_chunk=c, update_block=256, pair_kernel=_lib.PAIRS_STAGED_FUSED,
+                       resort_one_block=0) for c in PAIR_CHUNKS]
+        return listed + staged + fused
     return [dict(pair_block=b, pair_chunk=32, update_block=u, pair_kernel=_lib.PAIRS_LIST, resort_one_block=0)
             for b, u in itertools.product(PAIR_BLOCKS, UPDATE_BLOCKS)]
 

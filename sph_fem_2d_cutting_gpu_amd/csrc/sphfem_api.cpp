@@ -230,8 +230,8 @@ int sphfem_create(int device, sphfem_ctx **out) {
     ctx->device = device;
     int rc = guarded(ctx, [&] {
         SPHFEM_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        ctx->d_flags = dalloc<int>(8);
-        SPHFEM_HIP_CHECK(hipMemsetAsync(ctx->d_flags, 0, 8 * sizeof(int), ctx->stream));
+        ctx->d_flags = dalloc<int>(SPHFEM_NUM_FLAGS);
+        SPHFEM_HIP_CHECK(hipMemsetAsync(ctx->d_flags, 0, SPHFEM_NUM_FLAGS * sizeof(int), ctx->stream));
         ctx->c.rr2_min = 1e-8;
         return lookup_scratch(ctx);
     });
@@ -1640,7 +1640,7 @@ int sphfem_leap_frog_set_tuning(sphfem_ctx *ctx, sphfem_leap_frog *lf, const sph
     const bool ok = (t.pair_block == 128 || t.pair_block == 256) &&
                     (t.pair_chunk == 16 || t.pair_chunk == 32 || t.pair_chunk == 48) &&
                     (t.update_block == 128 || t.update_block == 256) &&
-                    (t.pair_kernel == SPHFEM_PAIRS_LIST || t.pair_kernel == SPHFEM_PAIRS_STAGED) &&
+                    (t.pair_kernel >= SPHFEM_PAIRS_LIST && t.pair_kernel <= SPHFEM_PAIRS_STAGED_FUSED) &&
                     (t.resort_one_block == 0 ||
                      (t.resort_one_block == 1 && lf->num_part <= (unsigned)SPHFEM_SORT_ONE_BLOCK_MAX));
     if (!ok) return fail(ctx, SPHFEM_ERR_ARG, "unsupported tuning");
@@ -1720,7 +1720,7 @@ static int step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p,
     // the FEM tool's step start runs inside the hash pass of the Green re-sort (below), else here
     const bool tool_in_hash = t && g->kind != SPHFEM_GRID_ROTHLIN;
     if (t && !tool_in_hash) launch_tool_begin(ctx, t);
-    else if (!t) launch_fill_int(ctx, ctx->d_flags + 1, 2, 0);
+    else if (!t) launch_fill_int(ctx, ctx->d_flags + 1, SPHFEM_FLAG_NONFINITE, 0);   // [1] .. [5]
     if (rt) launch_rigid_advance(ctx, rt);
     // the source of each sorted position: the permutation of the sort (Green), or the identity
     // after the Rothlin sort has gathered in place

@@ -24,6 +24,7 @@
 #define SPHFEM_SEG_SINK_CAP 4096
 #define SPHFEM_FLAG_SINK_FRIC 6
 #define SPHFEM_FLAG_SINK_S2 7
+#define SPHFEM_NUM_FLAGS 8
 // position records a wavefront of the staged FP32C pair kernel copies into LDS (8 KiB)
 #define SPHFEM_STAGE_CAP 512
 // blocks of the report_force reduction (its partials: 2 doubles per block) and the largest particle
@@ -213,7 +214,7 @@ struct sphfem_rigid_tool {
 struct sphfem_leap_frog {
     unsigned int num_part = 0, num_cell = 0;
     int precision = SPHFEM_FP64;
-    sphfem_tuning tuning = {256, 32, 256, SPHFEM_PAIRS_LIST, 0};
+    sphfem_tuning tuning = {256, 32, 256, SPHFEM_PAIRS_LIST_FUSED, 0};
     sphfem_double2 *pos_init = nullptr, *vel_init = nullptr;
     sphfem_double4 *S_init = nullptr;
     double *rho_init = nullptr, *T_init = nullptr;
@@ -297,6 +298,28 @@ __device__ __forceinline__ float4 ld_stream(const float4 *p) {
     const sphfem_v4f v = __builtin_nontemporal_load(reinterpret_cast<const sphfem_v4f *>(p));
     return make_float4(v.x, v.y, v.z, v.w);
 }
+// streaming stores, for results not read again before the caches have turned over
+__device__ __forceinline__ void st_stream(double *p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_stream(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_stream(int *p, int v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_stream(sphfem_double2 *p, sphfem_double2 v) {
+    __builtin_nontemporal_store(sphfem_v2d{v.x, v.y}, reinterpret_cast<sphfem_v2d *>(p));
+}
+__device__ __forceinline__ void st_stream(sphfem_double4 *p, sphfem_double4 v) {
+    __builtin_nontemporal_store(sphfem_v2d{v.x, v.y}, reinterpret_cast<sphfem_v2d *>(p));
+    __builtin_nontemporal_store(sphfem_v2d{v.z, v.w}, reinterpret_cast<sphfem_v2d *>(p) + 1);
+}
+__device__ __forceinline__ void st_stream(float2 *p, float2 v) {
+    __builtin_nontemporal_store(sphfem_v2f{v.x, v.y}, reinterpret_cast<sphfem_v2f *>(p));
+}
+__device__ __forceinline__ void st_stream(float4 *p, float4 v) {
+    __builtin_nontemporal_store(sphfem_v4f{v.x, v.y, v.z, v.w}, reinterpret_cast<sphfem_v4f *>(p));
+}
+// a store that streams when NT
+template <bool NT, class T> __device__ __forceinline__ void st_maybe_stream(T *p, T v) {
+    if constexpr (NT) st_stream(p, v);
+    else *p = v;
+}
 
 // sort / cells (sphfem_sort.hip)
 void launch_hash(sphfem_ctx *ctx, sphfem_grid *g, const sphfem_particles &p, bool clear_old, int *hist0);
@@ -345,11 +368,13 @@ void launch_tool_begin(sphfem_ctx *ctx, sphfem_tool *t);
 
 // the start of a step for the FEM tool, item i of max(nodes, segments): reset_FEM_server's
 // T_old = T and calculate_temp(0.5 dt) for the nodes; zero the wear accumulators
-// (segments_wear_rate_reset) and, item 0, the contact-list and segment-sink counters
+// (segments_wear_rate_reset) and, item 0, the contact-list and segment-sink counters and the
+// window-form switch of the pair kernels
 __device__ __forceinline__ void tool_begin_item(const sphfem_tool &t, double half_dt, int *counters, int i) {
     if (i == 0) {
         counters[1] = 0;
         counters[2] = 0;
+        counters[SPHFEM_FLAG_NONFINITE] = 0;
         counters[SPHFEM_FLAG_SINK_FRIC] = 0;
         counters[SPHFEM_FLAG_SINK_S2] = 0;
     }

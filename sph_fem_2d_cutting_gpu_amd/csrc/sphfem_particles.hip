@@ -357,13 +357,299 @@ __device__ __forceinline__ void pair_term(const PairI &I, float hf, float inv_wd
 // divergence - mass * viscous term) per axis, XSPH velocity per axis; psum_T[i] = Laplacian of T.
 // WIN: every non-blanked record of the 5x5 cells (blanked records sit at x = 1e30) is evaluated with
 // the kernel set to 0 outside the support, as the reference does; see pair_sums_fp64.
+__device__ __forceinline__ double interaction_alpha(const DevConsts &c, double is_tool) {
+    // interactions_gpu.cu:164: the tool block (interactions_gpu.cu:19) is zero unless
+    // interactions_setup_thermal_constants_tool set it (the rigid-tool setup does, benchmarks.cpp:518;
+    // the FEM setups do not, benchmarks_NZ.cpp:354)
+    return (is_tool == 0.) ? c.trml_wp.alpha : c.trml_itool.alpha;
+}
+
+// Arguments of the update (everything after the pair sums, leap_frog.cu:71-96 with the actions of
+// the step), shared by the update kernel and the FP32C pair kernels that run it after their sums
+struct UpdateArgs {
+    sphfem_particles B, A;
+    const sphfem_double2 *pos_init, *vel_init;
+    const sphfem_double4 *S_init;
+    const double *rho_init, *T_init;
+    ToolView tv;
+    int write_intermediates;
+    const PairSums *psum64;
+    const RigidTool *rt;
+    RateView r_old, r_new;
+    int skip_contact;
+};
+
+// the update of sorted particle i; g_sum, a_sum, T_lapl32 are its FP32C pair sums (the FP64 form
+// reads psum64 instead)
+template <bool F32C, bool RIGID, bool NT = false>
+__device__ __forceinline__ void update_item(const DevConsts &c, const UpdateArgs &u, int i, float4 g_sum,
+                                            float4 a_sum, float T_lapl32) {
+    const sphfem_particles &B = u.B, &A = u.A;
+    const sphfem_double2 *__restrict__ pos_init = u.pos_init;
+    const sphfem_double2 *__restrict__ vel_init = u.vel_init;
+    const sphfem_double4 *__restrict__ S_init = u.S_init;
+    const double *__restrict__ rho_init = u.rho_init;
+    const double *__restrict__ T_init = u.T_init;
+    const ToolView &tv = u.tv;
+    const int write_intermediates = u.write_intermediates;
+    const PairSums *__restrict__ psum64 = u.psum64;
+    const RigidTool *__restrict__ rt = u.rt;
+    const RateView &r_old = u.r_old, &r_new = u.r_new;
+    const int skip_contact = u.skip_contact;
+    // FP32C: the reorder wrote only the neighbour records; every per-particle field is gathered here
+    // from the unsorted arrays A through idx, and the sorted step result is written to B.  All loads
+    // of the particle are issued before any branch on loaded values, so a wavefront waits for
+    // memory once (the kernel holds 3 wavefronts per SIMD); the rates and pair sums of blanked
+    // particles are read and not used.
+    const int src = F32C ? B.idx[i] : i;
+    sphfem_double2 pos, vel;
+    sphfem_double4 S;
+    double rho, T, eps_pl, eps_pl_dot, pr = 0.;
+    sphfem_double2 fc0, ft0, nrm0;
+    int on_seg0;
+    sphfem_double2 v0 = {0., 0.};
+    sphfem_double4 S0 = {0., 0., 0., 0.};
+    double rho0 = 0.;
+    double blanked0, tool, fixed, h_src = 0.;
+    float2 vt_old = make_float2(0.f, 0.f);
+    float4 St_old = make_float4(0.f, 0.f, 0.f, 0.f);
+    float rt_old = 0.f, Tt_old = 0.f;
+    if constexpr (F32C) {
+        blanked0 = ld_stream(&A.blanked[src]);
+        tool = ld_stream(&A.tool_particle[src]);
+        fixed = ld_stream(&A.fixed[src]);
+        pos = ld_stream(&A.pos[src]);
+        v0 = ld_stream(&A.vel[src]);
+        S0 = ld_stream(&A.S[src]);
+        rho0 = ld_stream(&A.rho[src]);
+        T = ld_stream(&A.T[src]);
+        eps_pl = ld_stream(&A.eps_pl[src]);
+        eps_pl_dot = ld_stream(&A.eps_pl_dot[src]);
+        on_seg0 = ld_stream(&A.on_seg[src]);
+        h_src = ld_stream(&A.h[src]);
+        vt_old = ld_stream(&r_old.vel_t[src]);          // the FP32 derivative carry
+        St_old = ld_stream(&r_old.S_t[src]);
+        rt_old = ld_stream(&r_old.rho_t[src]);
+        Tt_old = ld_stream(&r_old.T_t[src]);
+    } else {
+        blanked0 = B.blanked[i];
+        tool = B.tool_particle[i];
+        fixed = B.fixed[i];
+    }
+    const bool live = blanked0 != 1.;
+    const double thermal_alpha = interaction_alpha(c, tool);
+    PairSums r;
+    if (live) {
+        if constexpr (F32C) {
+            // stress divergence and viscous acceleration come summed per axis
+            r.v_der = {g_sum.x, g_sum.y, g_sum.z, g_sum.w};
+            r.S_der = {a_sum.x, 0., a_sum.y, 0.};
+            r.vel_t = {0., 0.};
+            r.pos_t = {a_sum.z, a_sum.w};
+            r.T_lapl = T_lapl32;
+        } else {
+            r = psum64[i];
+        }
+    }
+    // FP32C: pos and T are the pre-step values (the corrector starts from them); velocity, stress and
+    // density are predicted again from the pre-step state and rates (leap_frog.cu:45-69, the same
+    // operations as in the reorder), as is the pressure (actions_gpu.cu:222-232)
+    if constexpr (F32C) {
+        vel = v0;
+        S = S0;
+        rho = rho0;
+        if (live) {
+            const double hdt = 0.5 * c.dt;
+            vel.x = vel.x + hdt * vt_old.x;
+            vel.y = vel.y + hdt * vt_old.y;
+            S.x = S.x + hdt * St_old.x;
+            S.y = S.y + hdt * St_old.y;
+            S.z = S.z + hdt * St_old.z;
+            S.w = S.w + hdt * St_old.w;
+            rho = rho + hdt * (double)rt_old;
+        }
+        if (tool == 0.) pr = eos_pressure(c, rho);
+        if (skip_contact && on_seg0 == -1) {
+            // every writer of on_seg = -1 (contact s1, the far-from-tool reset) zeroes fc, ft and n
+            // with it (actions_gpu.cu:455-491): nothing to read far from the tool
+            const sphfem_double2 z = {0., 0.};
+            fc0 = z;
+            ft0 = z;
+            nrm0 = z;
+        } else {
+            fc0 = ld_stream(&A.fc[src]);
+            ft0 = ld_stream(&A.ft[src]);
+            nrm0 = ld_stream(&A.n[src]);
+        }
+    } else {
+        pos = B.pos[i];
+        vel = B.vel[i];
+        S = B.S[i];
+        rho = B.rho[i];
+        T = B.T[i];
+        eps_pl = B.eps_pl[i];
+        eps_pl_dot = B.eps_pl_dot[i];
+        fc0 = B.fc[i];
+        ft0 = B.ft[i];
+        nrm0 = B.n[i];
+        on_seg0 = B.on_seg[i];
+    }
+    double blanked = blanked0;
+    sphfem_double2 fc = fc0, ft = ft0, nrm = nrm0;
+    int on_seg = on_seg0;
+    bool contact_written = false;
+    bool near = false;
+    sphfem_double2 pos_t = {0., 0.}, vel_t = {0., 0.};
+    sphfem_double4 S_t = {0., 0., 0., 0.};
+    double rho_t = 0., T_t = 0.;
+    if (live) {
+        pos_t = r.pos_t;
+        vel_t = r.vel_t;
+        if constexpr (F32C) T_t = thermal_alpha != 0. ? thermal_alpha * r.T_lapl : (double)Tt_old;
+        else T_t = thermal_alpha != 0. ? thermal_alpha * r.T_lapl : A.T_t[A.idx[i]];
+        if (tool == 0.) {
+            S_t = jaumann_rate(c, r.v_der, S);        // actions_gpu.cu:280-314
+            rho_t = -rho * (r.v_der.x + r.v_der.w);     // :626-639
+            const double mass = c.phys.mass;            // :641-671
+            vel_t.x += r.S_der.x + r.S_der.y + fc.x / mass + ft.x / mass;
+            vel_t.y += r.S_der.z + r.S_der.w + fc.y / mass + ft.y / mass;
+            pos_t.x = pos_t.x + vel.x;                  // :673-687
+            pos_t.y = pos_t.y + vel.y;
+        }
+        if (write_intermediates) {
+            A.S_der[i] = r.S_der;
+            A.v_der[i] = r.v_der;
+        }
+        // leap_frog.cu:71-96
+        const double dt = c.dt;
+        const sphfem_double2 p0 = F32C ? pos : pos_init[i];
+        if constexpr (!F32C) {
+            v0 = vel_init[i];
+            S0 = S_init[i];
+            rho0 = rho_init[i];
+        }
+        const double T0 = F32C ? T : T_init[i];
+        pos.x = p0.x + dt * pos_t.x;
+        pos.y = p0.y + dt * pos_t.y;
+        vel.x = v0.x + dt * vel_t.x;
+        vel.y = v0.y + dt * vel_t.y;
+        S.x = S0.x + dt * S_t.x;
+        S.y = S0.y + dt * S_t.y;
+        S.z = S0.z + dt * S_t.z;
+        S.w = S0.w + dt * S_t.w;
+        rho = rho0 + dt * rho_t;
+        T = T0 + dt * T_t;
+        if (tv.active) {
+            if (!fric_heat_v21(c, tv, *tv.kin, i, eps_pl, ft, on_seg, nrm, vel, fc, rho, T)) blanked = 1.;
+        }
+        if constexpr (RIGID) {
+            if (c.fric_heat && tool == 0.) T = rigid_fric_heat(c, rt->g.vel, vel, ft, nrm, T);
+        }
+        if (c.plastic && tool == 0. && blanked != 1.) {
+            if (plasticity_jc(c, S, F32C ? pr : B.p[i], eps_pl, eps_pl_dot, T, rho)) blanked = 1.;
+        }
+        // contact s1 for particles outside the 1.05 ref_dist disc of the moved tool
+        // (actions_gpu.cu:438-453); the others are finished by the near-tool pass
+        if (tv.active && blanked != 1.) {
+            ToolKin k = *tv.kin;
+            k.tl_x += k.vel_x * dt;
+            k.tl_y += k.vel_y * dt;
+            if (near_tool(k, pos)) {
+                near = true;
+                int slot = atomicAdd(tv.near_count, 1);
+                tv.near_list[slot] = i;
+            } else {
+                sphfem_double2 z = {0., 0.};
+                fc = z; ft = z; nrm = z;
+                on_seg = -1;
+                contact_written = true;
+            }
+        }
+    } else if constexpr (RIGID) {
+        if (c.fric_heat && tool == 0.) T = rigid_fric_heat(c, rt->g.vel, vel, ft, nrm, T);
+    }
+    if constexpr (RIGID) {
+        if (tool != 0.) {
+            pos.x += c.dt * rt->g.vel.x;
+            pos.y += c.dt * rt->g.vel.y;
+        } else {
+            rigid_contact(*rt, c.dt, pos, vel, ft0, fc, ft, nrm);
+            contact_written = true;
+        }
+    }
+    if (!near) {
+        // actions_gpu.cu:765-781, 756-763, 787-824
+        if (tool == 0. && fixed != 0.) {
+            sphfem_double2 z = {0., 0.};
+            vel = z;
+            fc = z;
+            pos_t = z;
+            vel_t = z;
+            contact_written = true;
+        }
+        if (c.thermal_bc && fixed == 1.) T = c.trml_wp.T_init;
+        if (blanked != 1. && live) {
+            bool bad = isnaninf(pos_t.x) || isnaninf(pos_t.y) || isnaninf(vel_t.x) || isnaninf(vel_t.y) ||
+                       isnaninf(S_t.x) || isnaninf(S_t.y) || isnaninf(S_t.z) || isnaninf(S_t.w) ||
+                       isnaninf(rho_t) || isnaninf(T_t);
+            if (bad) blanked = 1.;
+        }
+    }
+    if constexpr (F32C) {
+        // the new rates in sorted order, FP32 (the pair sums they come from are FP32)
+        st_maybe_stream<NT>(&r_new.pos_t[i], make_float2((float)pos_t.x, (float)pos_t.y));
+        st_maybe_stream<NT>(&r_new.vel_t[i], make_float2((float)vel_t.x, (float)vel_t.y));
+        st_maybe_stream<NT>(&r_new.S_t[i], make_float4((float)S_t.x, (float)S_t.y, (float)S_t.z, (float)S_t.w));
+        st_maybe_stream<NT>(&r_new.rho_t[i], (float)rho_t);
+        st_maybe_stream<NT>(&r_new.T_t[i], (float)T_t);
+    } else {
+        B.pos_t[i] = pos_t;
+        B.vel_t[i] = vel_t;
+        B.S_t[i] = S_t;
+        B.rho_t[i] = rho_t;
+        B.T_t[i] = T_t;
+    }
+    const sphfem_particles &O = F32C ? B : A;
+    st_maybe_stream<NT>(&O.pos[i], pos);
+    st_maybe_stream<NT>(&O.vel[i], vel);
+    st_maybe_stream<NT>(&O.S[i], S);
+    st_maybe_stream<NT>(&O.rho[i], rho);
+    st_maybe_stream<NT>(&O.T[i], T);
+    st_maybe_stream<NT>(&O.eps_pl[i], eps_pl);
+    st_maybe_stream<NT>(&O.eps_pl_dot[i], eps_pl_dot);
+    st_maybe_stream<NT>(&O.blanked[i], blanked);
+    if constexpr (F32C) {
+        // the sorted copy of every remaining field; p of tool particles stays at its array position,
+        // as the reference neither gathers p nor computes it for them
+        st_maybe_stream<NT>(&B.fc[i], fc);
+        st_maybe_stream<NT>(&B.ft[i], ft);
+        st_maybe_stream<NT>(&B.n[i], nrm);
+        st_maybe_stream<NT>(&B.on_seg[i], on_seg);
+        st_maybe_stream<NT>(&B.h[i], h_src);
+        st_maybe_stream<NT>(&B.fixed[i], fixed);
+        st_maybe_stream<NT>(&B.tool_particle[i], tool);
+        st_maybe_stream<NT>(&B.p[i], tool == 0. ? pr : A.p[i]);
+    } else if (contact_written && (fc.x != fc0.x || fc.y != fc0.y || ft.x != ft0.x || ft.y != ft0.y ||
+                                   nrm.x != nrm0.x || nrm.y != nrm0.y || on_seg != on_seg0)) {
+        // contact fields in place, written only when they change (far from the tool they stay zero)
+        B.fc[i] = fc;
+        B.ft[i] = ft;
+        B.n[i] = nrm;
+        B.on_seg[i] = on_seg;
+    }
+}
+
+struct PairOut {           // one particle's FP32C sums: psum[i], psum[N + i], psum_T[i]
+    float4 g, a;
+    float T;
+};
+
 template <int BS, int CHUNK, bool WIN, class Pos = GlobalPos>
-__device__ __forceinline__ void pairs_fp32c_one(const DevConsts &c, const double *__restrict__ blanked,
-                                                const int *__restrict__ hash, int N, const RecView &rv,
-                                                const int *__restrict__ cs, const int *__restrict__ ce,
-                                                float4 *__restrict__ psum, float *__restrict__ psum_T, int i,
-                                                typename Pos::Entry *__restrict__ list, const Pos &pos,
-                                                int *__restrict__ flags) {
+__device__ __forceinline__ PairOut pairs_fp32c_one(const DevConsts &c, const double *__restrict__ blanked,
+                                                   const int *__restrict__ hash, int N, const RecView &rv,
+                                                   const int *__restrict__ cs, const int *__restrict__ ce, int i,
+                                                   typename Pos::Entry *__restrict__ list, const Pos &pos,
+                                                   int *__restrict__ flags) {
 #pragma clang fp contract(fast)
     double d_v0 = 0., d_v1 = 0., d_v2 = 0., d_v3 = 0.;
     double d_ax = 0., d_ay = 0., d_p0 = 0., d_p1 = 0., d_T = 0.;
@@ -486,30 +772,48 @@ __device__ __forceinline__ void pairs_fp32c_one(const DevConsts &c, const double
         run_list(n);
     }
     const double mass = c.phys.mass;
-    psum[i] = {(float)d_v0, (float)d_v1, (float)d_v2, (float)d_v3};
-    psum[(size_t)N + i] = {(float)(mass * d_ax), (float)(mass * d_ay), (float)(-c.corr.xspheps * mass * d_p0),
-                           (float)(-c.corr.xspheps * mass * d_p1)};
-    psum_T[i] = (float)(2. * d_T);
+    PairOut o;
+    o.g = make_float4((float)d_v0, (float)d_v1, (float)d_v2, (float)d_v3);
+    o.a = make_float4((float)(mass * d_ax), (float)(mass * d_ay), (float)(-c.corr.xspheps * mass * d_p0),
+                      (float)(-c.corr.xspheps * mass * d_p1));
+    o.T = (float)(2. * d_T);
+    return o;
+}
+
+// Where a pair kernel's sums go: psum / psum_T, read by the update kernel (or the stage outputs), or,
+// FUSE, straight into the particle's update (the update needs no other particle's sums)
+template <bool FUSE, bool RIGID>
+__device__ __forceinline__ void emit_pairs(const DevConsts &c, const UpdateArgs &u, float4 *__restrict__ psum,
+                                           float *__restrict__ psum_T, int N, int i, const PairOut &o) {
+    if constexpr (FUSE) {
+        update_item<true, RIGID, true>(c, u, i, o.g, o.a, o.T);
+    } else {
+        psum[i] = o.g;
+        psum[(size_t)N + i] = o.a;
+        psum_T[i] = o.T;
+    }
 }
 
 // in-support pair sums, one lane per sorted particle; when nonfinite[0] is set (a live particle
 // with a non-finite input), the window form: every record of the 5x5 cells, the kernel zeroed
-// outside the support, so the non-finite values spread as in the reference
-template <int BS, int CHUNK>
+// outside the support, so the non-finite values spread as in the reference.  FUSE: each lane then
+// runs its particle's update (u) instead of writing psum.
+template <int BS, int CHUNK, bool FUSE = false, bool RIGID = false>
 __global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *__restrict__ blanked,
                                                          const int *__restrict__ hash, int N, RecView rv,
                                                          const int *__restrict__ cs, const int *__restrict__ ce,
                                                          float4 *__restrict__ psum, float *__restrict__ psum_T,
-                                                         const int *__restrict__ nonfinite) {
+                                                         const int *__restrict__ nonfinite, UpdateArgs u) {
     __shared__ int nb_list[CHUNK * BS];
+    const int win = *nonfinite;
     const int i = xcd_logical_block() * BS + threadIdx.x;
     if (i >= N) return;
-    if (*nonfinite != 0)
-        pairs_fp32c_one<BS, CHUNK, true>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, nb_list + threadIdx.x,
-                                         GlobalPos{rv.r0}, nullptr);
-    else
-        pairs_fp32c_one<BS, CHUNK, false>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, nb_list + threadIdx.x,
-                                          GlobalPos{rv.r0}, nullptr);
+    const PairOut o =
+        win != 0 ? pairs_fp32c_one<BS, CHUNK, true>(c, blanked, hash, N, rv, cs, ce, i, nb_list + threadIdx.x,
+                                                    GlobalPos{rv.r0}, nullptr)
+                 : pairs_fp32c_one<BS, CHUNK, false>(c, blanked, hash, N, rv, cs, ce, i, nb_list + threadIdx.x,
+                                                     GlobalPos{rv.r0}, nullptr);
+    emit_pairs<FUSE, RIGID>(c, u, psum, psum_T, N, i, o);
 }
 
 // LDS-staged form of k_pairs_fp32c, one wavefront per workgroup.  When the wavefront's 64
@@ -524,12 +828,13 @@ __global__ void __launch_bounds__(BS) k_pairs_fp32c(DevConsts c, const double *_
 // past N copy records but sum nothing); the union comes from the hashes of the wavefront's first and
 // last particles, read by every lane from memory, so no lane depends on another lane's registers.
 #define SPHFEM_STAGE_ROW_SCAN 64
-template <int CHUNK, int CAP>
+template <int CHUNK, int CAP, bool FUSE = false, bool RIGID = false>
 __global__ void __launch_bounds__(64) k_pairs_fp32c_staged(DevConsts c, const double *__restrict__ blanked,
                                                            const int *__restrict__ hash, int N, RecView rv,
                                                            const int *__restrict__ cs, const int *__restrict__ ce,
                                                            float4 *__restrict__ psum, float *__restrict__ psum_T,
-                                                           const int *__restrict__ nonfinite, int *__restrict__ flags) {
+                                                           const int *__restrict__ nonfinite, int *__restrict__ flags,
+                                                           UpdateArgs u) {
     // LDS: staged -- positions [CAP + 4] (+4: the scan reads up to three records past a range) and
     // 16-bit list entries; list path -- 32-bit list entries in the same bytes
     constexpr int POS_BYTES = (CAP + 4) * 16, LIST16 = CHUNK * 64 * 2, LIST32 = CHUNK * 64 * 4;
@@ -539,14 +844,16 @@ __global__ void __launch_bounds__(64) k_pairs_fp32c_staged(DevConsts c, const do
     float4 *spos = reinterpret_cast<float4 *>(lds);
     unsigned short *list16 = reinterpret_cast<unsigned short *>(lds + POS_BYTES);
     int *list32 = reinterpret_cast<int *>(lds);
+    const int win = *nonfinite;
     const int base = xcd_logical_block() * 64;
     if (base >= N) return;                              // the whole wavefront
     const int lane = (int)threadIdx.x;
     const int i = base + lane;
-    if (*nonfinite != 0) {                              // the window form, as k_pairs_fp32c
+    if (win != 0) {                                     // the window form, as k_pairs_fp32c
         if (i < N)
-            pairs_fp32c_one<64, CHUNK, true>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, list32 + lane,
-                                             GlobalPos{rv.r0}, nullptr);
+            emit_pairs<FUSE, RIGID>(c, u, psum, psum_T, N, i,
+                                    pairs_fp32c_one<64, CHUNK, true>(c, blanked, hash, N, rv, cs, ce, i,
+                                                                     list32 + lane, GlobalPos{rv.r0}, nullptr));
         return;
     }
     const int last_i = base + 63 < N ? base + 63 : N - 1;
@@ -610,20 +917,16 @@ __global__ void __launch_bounds__(64) k_pairs_fp32c_staged(DevConsts c, const do
         }
         __syncthreads();
         if (i < N)
-            pairs_fp32c_one<64, CHUNK, false, StagedPos>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i,
-                                                         list16 + lane, sp, flags);
+            emit_pairs<FUSE, RIGID>(c, u, psum, psum_T, N, i,
+                                    pairs_fp32c_one<64, CHUNK, false, StagedPos>(c, blanked, hash, N, rv, cs, ce, i,
+                                                                                 list16 + lane, sp, flags));
     } else if (i < N) {
-        pairs_fp32c_one<64, CHUNK, false>(c, blanked, hash, N, rv, cs, ce, psum, psum_T, i, list32 + lane,
-                                          GlobalPos{rv.r0}, nullptr);
+        emit_pairs<FUSE, RIGID>(c, u, psum, psum_T, N, i,
+                                pairs_fp32c_one<64, CHUNK, false>(c, blanked, hash, N, rv, cs, ce, i, list32 + lane,
+                                                                  GlobalPos{rv.r0}, nullptr));
     }
 }
 
-__device__ __forceinline__ double interaction_alpha(const DevConsts &c, double is_tool) {
-    // interactions_gpu.cu:164: the tool block (interactions_gpu.cu:19) is zero unless
-    // interactions_setup_thermal_constants_tool set it (the rigid-tool setup does, benchmarks.cpp:518;
-    // the FEM setups do not, benchmarks_NZ.cpp:354)
-    return (is_tool == 0.) ? c.trml_wp.alpha : c.trml_itool.alpha;
-}
 
 // interactions_gpu.cu:143-379 as one kernel (drop-in entry point): every particle of the 5x5 cells,
 // as the reference visits them
@@ -854,268 +1157,21 @@ __global__ void __launch_bounds__(BS) k_pairs_fp64(DevConsts c, sphfem_particles
 // RIGID: the rigid-tool step (leap_frog.cu:99-160): friction heat after the corrector
 // (actions_gpu.cu:316-343), tool particles moved and contact against the moved tool
 // (tool_gpu.cu:146-216) after plasticity, for blanked particles too as in the reference.
+// the update as its own kernel, after k_pairs_fp64 or a k_pairs_fp32c that wrote psum
 template <bool F32C, int BS, bool RIGID>
-__global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, sphfem_particles B, sphfem_particles A,
-                                                          const sphfem_double2 *__restrict__ pos_init,
-                                                          const sphfem_double2 *__restrict__ vel_init,
-                                                          const sphfem_double4 *__restrict__ S_init,
-                                                          const double *__restrict__ rho_init,
-                                                          const double *__restrict__ T_init,
-                                                          const int *__restrict__ cs, const int *__restrict__ ce,
-                                                          ToolView tv, int write_intermediates,
-                                                          const float4 *__restrict__ psum,
-                                                          const float *__restrict__ psum_T,
-                                                          const PairSums *__restrict__ psum64,
-                                                          const RigidTool *__restrict__ rt,
-                                                          int *__restrict__ nonfinite, const RateView r_old,
-                                                          const RateView r_new, int skip_contact) {
+__global__ void __launch_bounds__(BS) k_fused_update(DevConsts c, UpdateArgs u, const float4 *__restrict__ psum,
+                                                     const float *__restrict__ psum_T, int *__restrict__ nonfinite) {
     const int i = xcd_logical_block() * BS + threadIdx.x;
     if (i == 0) *nonfinite = 0;      // the pair kernels of this step have read it
-    if (i >= (int)B.N) return;
-    // FP32C: the reorder wrote only the neighbour records; every per-particle field is gathered here
-    // from the unsorted arrays A through idx, and the sorted step result is written to B.  All loads
-    // of the particle are issued before any branch on loaded values, so a wavefront waits for
-    // memory once (the kernel holds 3 wavefronts per SIMD); the rates and pair sums of blanked
-    // particles are read and not used.
-    const int src = F32C ? B.idx[i] : i;
-    sphfem_double2 pos, vel;
-    sphfem_double4 S;
-    double rho, T, eps_pl, eps_pl_dot, pr = 0.;
-    sphfem_double2 fc0, ft0, nrm0;
-    int on_seg0;
-    sphfem_double2 v0 = {0., 0.};
-    sphfem_double4 S0 = {0., 0., 0., 0.};
-    double rho0 = 0.;
-    double blanked0, tool, fixed, h_src = 0.;
-    float2 vt_old = make_float2(0.f, 0.f);
-    float4 St_old = make_float4(0.f, 0.f, 0.f, 0.f), g_sum = St_old, a_sum = St_old;
-    float rt_old = 0.f, Tt_old = 0.f, T_lapl32 = 0.f;
+    if (i >= (int)u.B.N) return;
+    float4 g = make_float4(0.f, 0.f, 0.f, 0.f), a = g;
+    float tl = 0.f;
     if constexpr (F32C) {
-        blanked0 = ld_stream(&A.blanked[src]);
-        tool = ld_stream(&A.tool_particle[src]);
-        fixed = ld_stream(&A.fixed[src]);
-        pos = ld_stream(&A.pos[src]);
-        v0 = ld_stream(&A.vel[src]);
-        S0 = ld_stream(&A.S[src]);
-        rho0 = ld_stream(&A.rho[src]);
-        T = ld_stream(&A.T[src]);
-        eps_pl = ld_stream(&A.eps_pl[src]);
-        eps_pl_dot = ld_stream(&A.eps_pl_dot[src]);
-        on_seg0 = ld_stream(&A.on_seg[src]);
-        h_src = ld_stream(&A.h[src]);
-        vt_old = ld_stream(&r_old.vel_t[src]);          // the FP32 derivative carry
-        St_old = ld_stream(&r_old.S_t[src]);
-        rt_old = ld_stream(&r_old.rho_t[src]);
-        Tt_old = ld_stream(&r_old.T_t[src]);
-        g_sum = psum[i];                             // sums of k_pairs_fp32c
-        a_sum = psum[(size_t)B.N + i];
-        T_lapl32 = psum_T[i];
-    } else {
-        blanked0 = B.blanked[i];
-        tool = B.tool_particle[i];
-        fixed = B.fixed[i];
+        g = psum[i];                 // sums of k_pairs_fp32c
+        a = psum[(size_t)u.B.N + i];
+        tl = psum_T[i];
     }
-    const bool live = blanked0 != 1.;
-    const double thermal_alpha = interaction_alpha(c, tool);
-    PairSums r;
-    if (live) {
-        if constexpr (F32C) {
-            // stress divergence and viscous acceleration come summed per axis
-            r.v_der = {g_sum.x, g_sum.y, g_sum.z, g_sum.w};
-            r.S_der = {a_sum.x, 0., a_sum.y, 0.};
-            r.vel_t = {0., 0.};
-            r.pos_t = {a_sum.z, a_sum.w};
-            r.T_lapl = T_lapl32;
-        } else {
-            r = psum64[i];
-        }
-    }
-    // FP32C: pos and T are the pre-step values (the corrector starts from them); velocity, stress and
-    // density are predicted again from the pre-step state and rates (leap_frog.cu:45-69, the same
-    // operations as in the reorder), as is the pressure (actions_gpu.cu:222-232)
-    if constexpr (F32C) {
-        vel = v0;
-        S = S0;
-        rho = rho0;
-        if (live) {
-            const double hdt = 0.5 * c.dt;
-            vel.x = vel.x + hdt * vt_old.x;
-            vel.y = vel.y + hdt * vt_old.y;
-            S.x = S.x + hdt * St_old.x;
-            S.y = S.y + hdt * St_old.y;
-            S.z = S.z + hdt * St_old.z;
-            S.w = S.w + hdt * St_old.w;
-            rho = rho + hdt * (double)rt_old;
-        }
-        if (tool == 0.) pr = eos_pressure(c, rho);
-        if (skip_contact && on_seg0 == -1) {
-            // every writer of on_seg = -1 (contact s1, the far-from-tool reset) zeroes fc, ft and n
-            // with it (actions_gpu.cu:455-491): nothing to read far from the tool
-            const sphfem_double2 z = {0., 0.};
-            fc0 = z;
-            ft0 = z;
-            nrm0 = z;
-        } else {
-            fc0 = ld_stream(&A.fc[src]);
-            ft0 = ld_stream(&A.ft[src]);
-            nrm0 = ld_stream(&A.n[src]);
-        }
-    } else {
-        pos = B.pos[i];
-        vel = B.vel[i];
-        S = B.S[i];
-        rho = B.rho[i];
-        T = B.T[i];
-        eps_pl = B.eps_pl[i];
-        eps_pl_dot = B.eps_pl_dot[i];
-        fc0 = B.fc[i];
-        ft0 = B.ft[i];
-        nrm0 = B.n[i];
-        on_seg0 = B.on_seg[i];
-    }
-    double blanked = blanked0;
-    sphfem_double2 fc = fc0, ft = ft0, nrm = nrm0;
-    int on_seg = on_seg0;
-    bool contact_written = false;
-    bool near = false;
-    sphfem_double2 pos_t = {0., 0.}, vel_t = {0., 0.};
-    sphfem_double4 S_t = {0., 0., 0., 0.};
-    double rho_t = 0., T_t = 0.;
-    if (live) {
-        pos_t = r.pos_t;
-        vel_t = r.vel_t;
-        if constexpr (F32C) T_t = thermal_alpha != 0. ? thermal_alpha * r.T_lapl : (double)Tt_old;
-        else T_t = thermal_alpha != 0. ? thermal_alpha * r.T_lapl : A.T_t[A.idx[i]];
-        if (tool == 0.) {
-            S_t = jaumann_rate(c, r.v_der, S);        // actions_gpu.cu:280-314
-            rho_t = -rho * (r.v_der.x + r.v_der.w);     // :626-639
-            const double mass = c.phys.mass;            // :641-671
-            vel_t.x += r.S_der.x + r.S_der.y + fc.x / mass + ft.x / mass;
-            vel_t.y += r.S_der.z + r.S_der.w + fc.y / mass + ft.y / mass;
-            pos_t.x = pos_t.x + vel.x;                  // :673-687
-            pos_t.y = pos_t.y + vel.y;
-        }
-        if (write_intermediates) {
-            A.S_der[i] = r.S_der;
-            A.v_der[i] = r.v_der;
-        }
-        // leap_frog.cu:71-96
-        const double dt = c.dt;
-        const sphfem_double2 p0 = F32C ? pos : pos_init[i];
-        if constexpr (!F32C) {
-            v0 = vel_init[i];
-            S0 = S_init[i];
-            rho0 = rho_init[i];
-        }
-        const double T0 = F32C ? T : T_init[i];
-        pos.x = p0.x + dt * pos_t.x;
-        pos.y = p0.y + dt * pos_t.y;
-        vel.x = v0.x + dt * vel_t.x;
-        vel.y = v0.y + dt * vel_t.y;
-        S.x = S0.x + dt * S_t.x;
-        S.y = S0.y + dt * S_t.y;
-        S.z = S0.z + dt * S_t.z;
-        S.w = S0.w + dt * S_t.w;
-        rho = rho0 + dt * rho_t;
-        T = T0 + dt * T_t;
-        if (tv.active) {
-            if (!fric_heat_v21(c, tv, *tv.kin, i, eps_pl, ft, on_seg, nrm, vel, fc, rho, T)) blanked = 1.;
-        }
-        if constexpr (RIGID) {
-            if (c.fric_heat && tool == 0.) T = rigid_fric_heat(c, rt->g.vel, vel, ft, nrm, T);
-        }
-        if (c.plastic && tool == 0. && blanked != 1.) {
-            if (plasticity_jc(c, S, F32C ? pr : B.p[i], eps_pl, eps_pl_dot, T, rho)) blanked = 1.;
-        }
-        // contact s1 for particles outside the 1.05 ref_dist disc of the moved tool
-        // (actions_gpu.cu:438-453); the others are finished by the near-tool pass
-        if (tv.active && blanked != 1.) {
-            ToolKin k = *tv.kin;
-            k.tl_x += k.vel_x * dt;
-            k.tl_y += k.vel_y * dt;
-            if (near_tool(k, pos)) {
-                near = true;
-                int slot = atomicAdd(tv.near_count, 1);
-                tv.near_list[slot] = i;
-            } else {
-                sphfem_double2 z = {0., 0.};
-                fc = z; ft = z; nrm = z;
-                on_seg = -1;
-                contact_written = true;
-            }
-        }
-    } else if constexpr (RIGID) {
-        if (c.fric_heat && tool == 0.) T = rigid_fric_heat(c, rt->g.vel, vel, ft, nrm, T);
-    }
-    if constexpr (RIGID) {
-        if (tool != 0.) {
-            pos.x += c.dt * rt->g.vel.x;
-            pos.y += c.dt * rt->g.vel.y;
-        } else {
-            rigid_contact(*rt, c.dt, pos, vel, ft0, fc, ft, nrm);
-            contact_written = true;
-        }
-    }
-    if (!near) {
-        // actions_gpu.cu:765-781, 756-763, 787-824
-        if (tool == 0. && fixed != 0.) {
-            sphfem_double2 z = {0., 0.};
-            vel = z;
-            fc = z;
-            pos_t = z;
-            vel_t = z;
-            contact_written = true;
-        }
-        if (c.thermal_bc && fixed == 1.) T = c.trml_wp.T_init;
-        if (blanked != 1. && live) {
-            bool bad = isnaninf(pos_t.x) || isnaninf(pos_t.y) || isnaninf(vel_t.x) || isnaninf(vel_t.y) ||
-                       isnaninf(S_t.x) || isnaninf(S_t.y) || isnaninf(S_t.z) || isnaninf(S_t.w) ||
-                       isnaninf(rho_t) || isnaninf(T_t);
-            if (bad) blanked = 1.;
-        }
-    }
-    if constexpr (F32C) {
-        // the new rates in sorted order, FP32 (the pair sums they come from are FP32)
-        r_new.pos_t[i] = make_float2((float)pos_t.x, (float)pos_t.y);
-        r_new.vel_t[i] = make_float2((float)vel_t.x, (float)vel_t.y);
-        r_new.S_t[i] = make_float4((float)S_t.x, (float)S_t.y, (float)S_t.z, (float)S_t.w);
-        r_new.rho_t[i] = (float)rho_t;
-        r_new.T_t[i] = (float)T_t;
-    } else {
-        B.pos_t[i] = pos_t;
-        B.vel_t[i] = vel_t;
-        B.S_t[i] = S_t;
-        B.rho_t[i] = rho_t;
-        B.T_t[i] = T_t;
-    }
-    sphfem_particles &O = F32C ? B : A;
-    O.pos[i] = pos;
-    O.vel[i] = vel;
-    O.S[i] = S;
-    O.rho[i] = rho;
-    O.T[i] = T;
-    O.eps_pl[i] = eps_pl;
-    O.eps_pl_dot[i] = eps_pl_dot;
-    O.blanked[i] = blanked;
-    if constexpr (F32C) {
-        // the sorted copy of every remaining field; p of tool particles stays at its array position,
-        // as the reference neither gathers p nor computes it for them
-        B.fc[i] = fc;
-        B.ft[i] = ft;
-        B.n[i] = nrm;
-        B.on_seg[i] = on_seg;
-        B.h[i] = h_src;
-        B.fixed[i] = fixed;
-        B.tool_particle[i] = tool;
-        B.p[i] = tool == 0. ? pr : A.p[i];
-    } else if (contact_written && (fc.x != fc0.x || fc.y != fc0.y || ft.x != ft0.x || ft.y != ft0.y ||
-                                   nrm.x != nrm0.x || nrm.y != nrm0.y || on_seg != on_seg0)) {
-        // contact fields in place, written only when they change (far from the tool they stay zero)
-        B.fc[i] = fc;
-        B.ft[i] = ft;
-        B.n[i] = nrm;
-        B.on_seg[i] = on_seg;
-    }
+    update_item<F32C, RIGID>(c, u, i, g, a, tl);
 }
 
 // FP32C pair sums written into the reference's interaction outputs (interactions_gpu.cu:370-378):
@@ -1139,41 +1195,51 @@ __global__ void __launch_bounds__(256) k_psum_expand(DevConsts c, sphfem_particl
     if (alpha != 0.) P.T_t[i] = alpha * (double)psum_T[i];
 }
 
-template <int BSZ, int CH>
+template <int BSZ, int CH, bool FUSE>
 static void launch_pairs_fp32c(sphfem_ctx *ctx, const sphfem_particles &B, sphfem_leap_frog *lf, const int *cs,
-                               const int *ce) {
+                               const int *ce, const UpdateArgs &u) {
     const size_t n = B.N;
     RecView rv{lf->rec, lf->rec + n + 4, lf->rec + 2 * n + 4, lf->rec + 3 * n + 4};
-    k_pairs_fp32c<BSZ, CH><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(ctx->c, B.blanked, B.hash, (int)B.N, rv, cs, ce,
-                                                                    lf->psum, lf->psum_T,
-                                                                    ctx->d_flags + SPHFEM_FLAG_NONFINITE);
+    k_pairs_fp32c<BSZ, CH, FUSE><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(
+        ctx->c, B.blanked, B.hash, (int)B.N, rv, cs, ce, lf->psum, lf->psum_T, ctx->d_flags + SPHFEM_FLAG_NONFINITE, u);
 }
 
-template <int CH>
+template <int CH, bool FUSE>
 static void launch_pairs_fp32c_staged(sphfem_ctx *ctx, const sphfem_particles &B, sphfem_leap_frog *lf, const int *cs,
-                                      const int *ce) {
+                                      const int *ce, const UpdateArgs &u) {
     const size_t n = B.N;
     RecView rv{lf->rec, lf->rec + n + 4, lf->rec + 2 * n + 4, lf->rec + 3 * n + 4};
-    k_pairs_fp32c_staged<CH, SPHFEM_STAGE_CAP><<<nblk(B.N, 64), 64, 0, ctx->stream>>>(
+    k_pairs_fp32c_staged<CH, SPHFEM_STAGE_CAP, FUSE><<<nblk(B.N, 64), 64, 0, ctx->stream>>>(
         ctx->c, B.blanked, B.hash, (int)B.N, rv, cs, ce, lf->psum, lf->psum_T, ctx->d_flags + SPHFEM_FLAG_NONFINITE,
-        ctx->d_flags);
+        ctx->d_flags, u);
 }
 
+template <bool FUSE>
+static void launch_pairs_fp32c_shape(sphfem_ctx *ctx, const sphfem_particles &B, sphfem_leap_frog *lf, const int *cs,
+                                     const int *ce, const UpdateArgs &u) {
+    const int pb = lf->tuning.pair_block, pc = lf->tuning.pair_chunk;
+    const int k = lf->tuning.pair_kernel;
+    if (k == SPHFEM_PAIRS_STAGED || k == SPHFEM_PAIRS_STAGED_FUSED) {
+        if (pc == 16) launch_pairs_fp32c_staged<16, FUSE>(ctx, B, lf, cs, ce, u);
+        else if (pc == 48) launch_pairs_fp32c_staged<48, FUSE>(ctx, B, lf, cs, ce, u);
+        else launch_pairs_fp32c_staged<32, FUSE>(ctx, B, lf, cs, ce, u);
+    } else if (pb == 128 && pc == 16) launch_pairs_fp32c<128, 16, FUSE>(ctx, B, lf, cs, ce, u);
+    else if (pb == 128 && pc == 32) launch_pairs_fp32c<128, 32, FUSE>(ctx, B, lf, cs, ce, u);
+    else if (pb == 128 && pc == 48) launch_pairs_fp32c<128, 48, FUSE>(ctx, B, lf, cs, ce, u);
+    else if (pb == 256 && pc == 16) launch_pairs_fp32c<256, 16, FUSE>(ctx, B, lf, cs, ce, u);
+    else if (pb == 256 && pc == 48) launch_pairs_fp32c<256, 48, FUSE>(ctx, B, lf, cs, ce, u);
+    else launch_pairs_fp32c<256, 32, FUSE>(ctx, B, lf, cs, ce, u);
+}
+
+// the tuned FP32C pair kernel; fuse: it also runs every particle's update, with these arguments
 static void launch_pairs_fp32c_tuned(sphfem_ctx *ctx, const sphfem_particles &B, sphfem_leap_frog *lf, const int *cs,
-                                     const int *ce) {
-    {
+                                     const int *ce, const UpdateArgs *fuse = nullptr) {
+    if (fuse) {
+        KTimer tm(ctx, "pairs_update_fp32c");
+        launch_pairs_fp32c_shape<true>(ctx, B, lf, cs, ce, *fuse);
+    } else {
         KTimer tm(ctx, "pairs_fp32c");
-        const int pb = lf->tuning.pair_block, pc = lf->tuning.pair_chunk;
-        if (lf->tuning.pair_kernel == SPHFEM_PAIRS_STAGED) {
-            if (pc == 16) launch_pairs_fp32c_staged<16>(ctx, B, lf, cs, ce);
-            else if (pc == 48) launch_pairs_fp32c_staged<48>(ctx, B, lf, cs, ce);
-            else launch_pairs_fp32c_staged<32>(ctx, B, lf, cs, ce);
-        } else if (pb == 128 && pc == 16) launch_pairs_fp32c<128, 16>(ctx, B, lf, cs, ce);
-        else if (pb == 128 && pc == 32) launch_pairs_fp32c<128, 32>(ctx, B, lf, cs, ce);
-        else if (pb == 128 && pc == 48) launch_pairs_fp32c<128, 48>(ctx, B, lf, cs, ce);
-        else if (pb == 256 && pc == 16) launch_pairs_fp32c<256, 16>(ctx, B, lf, cs, ce);
-        else if (pb == 256 && pc == 48) launch_pairs_fp32c<256, 48>(ctx, B, lf, cs, ce);
-        else launch_pairs_fp32c<256, 32>(ctx, B, lf, cs, ce);
+        launch_pairs_fp32c_shape<false>(ctx, B, lf, cs, ce, UpdateArgs{});
     }
 }
 
@@ -1205,15 +1271,35 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
         tv.active = 1;
         tv.sink = SegSink{ctx->d_flags + SPHFEM_FLAG_SINK_FRIC, t->sink_keys, t->sink_vals, SPHFEM_SEG_SINK_CAP};
     }
-    const int wi = lf->intermediates ? 1 : 0;
     const RigidTool *rtd = rt ? rt->dev : nullptr;
+    UpdateArgs u{};
+    u.B = B;
+    u.A = A;
+    u.pos_init = lf->pos_init;
+    u.vel_init = lf->vel_init;
+    u.S_init = lf->S_init;
+    u.rho_init = lf->rho_init;
+    u.T_init = lf->T_init;
+    u.tv = tv;
+    u.write_intermediates = lf->intermediates ? 1 : 0;
+    u.rt = rtd;
+    u.r_old = r_old;
+    u.r_new = r_new;
+    int *nf = ctx->d_flags + SPHFEM_FLAG_NONFINITE;
     if (lf->precision == SPHFEM_FP32C) {
+        u.skip_contact = skip;
+        // the fused pair kernels run the update themselves (not with a rigid tool, whose update has
+        // its own kernels)
+        const int pk = lf->tuning.pair_kernel;
+        if ((pk == SPHFEM_PAIRS_LIST_FUSED || pk == SPHFEM_PAIRS_STAGED_FUSED) && !rtd) {
+            launch_pairs_fp32c_tuned(ctx, B, lf, cs, ce, &u);
+            if (t) launch_fric_heat_sums(ctx, t);
+            return;
+        }
         launch_pairs_fp32c_tuned(ctx, B, lf, cs, ce);
         KTimer tm(ctx, "update_fp32c");
 #define SPHFEM_UPDATE(BSZ, RG)                                                                              \
-    k_fused_update<true, BSZ, RG><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(                                  \
-        ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi, lf->psum, \
-        lf->psum_T, nullptr, rtd, ctx->d_flags + SPHFEM_FLAG_NONFINITE, r_old, r_new, skip)
+    k_fused_update<true, BSZ, RG><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(ctx->c, u, lf->psum, lf->psum_T, nf)
         if (lf->tuning.update_block == 128) {
             if (rtd) SPHFEM_UPDATE(128, true);
             else SPHFEM_UPDATE(128, false);
@@ -1223,9 +1309,9 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
         }
 #undef SPHFEM_UPDATE
     } else {
+        u.psum64 = lf->psum64;
         {
             KTimer tp(ctx, "pairs_fp64");
-            const int *nf = ctx->d_flags + SPHFEM_FLAG_NONFINITE;
             if (lf->tuning.pair_block == 128)
                 k_pairs_fp64<128><<<nblk(B.N, 128), 128, 0, ctx->stream>>>(ctx->c, B, cs, ce, lf->pre_vol, lf->pre_s,
                                                                             lf->psum64, nf);
@@ -1235,9 +1321,7 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
         }
         KTimer tm(ctx, "fused_update");
 #define SPHFEM_UPDATE(BSZ, RG)                                                                              \
-    k_fused_update<false, BSZ, RG><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(                                 \
-        ctx->c, B, A, lf->pos_init, lf->vel_init, lf->S_init, lf->rho_init, lf->T_init, cs, ce, tv, wi, nullptr, \
-        nullptr, lf->psum64, rtd, ctx->d_flags + SPHFEM_FLAG_NONFINITE, r_old, r_new, 0)
+    k_fused_update<false, BSZ, RG><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(ctx->c, u, nullptr, nullptr, nf)
         if (lf->tuning.update_block == 128) {
             if (rtd) SPHFEM_UPDATE(128, true);
             else SPHFEM_UPDATE(128, false);

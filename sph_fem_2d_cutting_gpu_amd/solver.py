@@ -567,7 +567,7 @@ class leap_frog:
         self.ctx.call("sphfem_leap_frog_step_rigid", self.h, C.byref(p.s), g.h, rt.h if rt else None)
 
     def set_tuning(self, pair_block: int = 256, pair_chunk: int = 32, update_block: int = 256,
-                   pair_kernel: int = _lib.PAIRS_LIST, resort_one_block: int = 0):
+                   pair_kernel: int = _lib.PAIRS_LIST_FUSED, resort_one_block: int = 0):
         t = _lib.Tuning(int(pair_block), int(pair_chunk), int(update_block), int(pair_kernel), int(resort_one_block))
         self.ctx.call("sphfem_leap_frog_set_tuning", self.h, C.byref(t))
 

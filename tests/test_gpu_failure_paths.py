@@ -202,6 +202,33 @@ def test_step_from_forced_state(precision):
             _assert_forced_branches_fired(case, b)
 
 
+def test_fused_pair_kernels_equal_unfused_from_forced_state():
+    """The fused FP32C pair kernels (each particle's update run right after its sums) against the
+    separate pair and update kernels, from the forced state: the first step runs the window form
+    (a live particle with a NaN stress), the later ones the in-support form again once the step
+    start has cleared the switch; every field equal bit for bit after each step."""
+    case, _ = _forced_case()
+    fields = ("hash", "idx", "blanked", "pos", "vel", "S", "rho", "T", "eps_pl", "fc", "ft",
+              "pos_t", "vel_t", "rho_t", "S_t", "T_t")
+    sims = {}
+    for kernel in (_lib.PAIRS_LIST, _lib.PAIRS_LIST_FUSED, _lib.PAIRS_STAGED_FUSED):
+        sims[kernel] = S.Simulation(case, precision=_lib.FP32C)
+        sims[kernel].set_tuning(pair_block=256, pair_chunk=32, update_block=256, pair_kernel=kernel)
+    for step in range(1, 4):
+        print(f"fused vs separate pair kernels, forced state, step {step}", flush=True)
+        out = {}
+        for kernel, sim in sims.items():
+            sim.step(1)
+            sim.ctx.check_error()
+            out[kernel] = sim.state(fields)
+        for kernel in (_lib.PAIRS_LIST_FUSED, _lib.PAIRS_STAGED_FUSED):
+            for f in fields:
+                np.testing.assert_array_equal(out[kernel][f], out[_lib.PAIRS_LIST][f],
+                                              err_msg=f"kernel {kernel} {f} after step {step}")
+    for sim in sims.values():
+        sim.close()
+
+
 def _assert_forced_branches_fired(case, P):
     """The oracle's state after the first step holds the three forced failures."""
     bl = P["blanked"] == 1.0

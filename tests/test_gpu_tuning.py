@@ -48,7 +48,8 @@ def test_staged_pair_kernel_equals_list_kernel(which):
     """The LDS-staged FP32C pair kernel against the list kernel at each list chunk: the same sums bit
     for bit, on cases where most wavefronts stage their windows (the chip domain, 92 % at 1M) and on
     a cutting case where most run the list path (64 particles span several short columns), on the
-    Green and the Rothlin (dense) cell table, with partial last wavefronts (N mod 64 = 0, 50, 16)."""
+    Green and the Rothlin (dense) cell table, with partial last wavefronts (N mod 64 = 0, 50, 16).
+    Both fused forms (each particle's update run by the pair kernel) give the same step bit for bit."""
     grid = "green"
     if which.startswith("cutting"):
         case, steps = cases.perturbed(cases.small_cutting_case(nx=160, ny=40, wear=True)), 3
@@ -62,15 +63,16 @@ def test_staged_pair_kernel_equals_list_kernel(which):
     fields = ("hash", "idx", "blanked") + DERIVS + ("pos", "vel", "S", "rho", "T")
     for chunk in autotune.PAIR_CHUNKS:
         out = {}
-        for kernel in (_lib.PAIRS_LIST, _lib.PAIRS_STAGED):
+        for kernel in (_lib.PAIRS_LIST, _lib.PAIRS_STAGED, _lib.PAIRS_LIST_FUSED, _lib.PAIRS_STAGED_FUSED):
             sim = S.Simulation(case, precision=_lib.FP32C, grid=grid)
             sim.set_tuning(pair_block=256, pair_chunk=chunk, update_block=256, pair_kernel=kernel)
             sim.step(steps)
             sim.ctx.check_error()          # also the staged kernel's containment check
             out[kernel] = sim.state(fields)
             sim.close()
-        for f in fields:
-            assert_equal(f"chunk {chunk} {f}", out[_lib.PAIRS_STAGED][f], out[_lib.PAIRS_LIST][f])
+        for kernel in (_lib.PAIRS_STAGED, _lib.PAIRS_LIST_FUSED, _lib.PAIRS_STAGED_FUSED):
+            for f in fields:
+                assert_equal(f"chunk {chunk} kernel {kernel} {f}", out[kernel][f], out[_lib.PAIRS_LIST][f])
 
 
 def test_tune_simulation_applies_a_candidate():
