@@ -315,11 +315,9 @@ def run_form(case, precision, steps, warmup, breakdown=False, graph=True, tune=T
     elif pair_kernel != "auto":
         sim.set_tuning(pair_kernel=PAIR_KERNELS[pair_kernel])
     # interaction-kernel time (eager steps, events around that kernel only): the FP32C pair kernel,
-    # or the pair kernel that also runs the update (not with a rigid tool, whose step runs the
-    # update kernel)
+    # or the pair kernel that also runs the update
     kname = INTERACTION_KERNEL[precision]
-    if (prec == _lib.FP32C and case.rigid is None
-            and sim.get_tuning()["pair_kernel"] in (_lib.PAIRS_LIST_FUSED, _lib.PAIRS_STAGED_FUSED)):
+    if prec == _lib.FP32C and sim.get_tuning()["pair_kernel"] in (_lib.PAIRS_LIST_FUSED, _lib.PAIRS_STAGED_FUSED):
         kname = FUSED_KERNEL
     ctx.timing_select(kname)
     ctx.timing(True)

@@ -1195,51 +1195,53 @@ __global__ void __launch_bounds__(256) k_psum_expand(DevConsts c, sphfem_particl
     if (alpha != 0.) P.T_t[i] = alpha * (double)psum_T[i];
 }
 
-template <int BSZ, int CH, bool FUSE>
+template <int BSZ, int CH, bool FUSE, bool RIGID>
 static void launch_pairs_fp32c(sphfem_ctx *ctx, const sphfem_particles &B, sphfem_leap_frog *lf, const int *cs,
                                const int *ce, const UpdateArgs &u) {
     const size_t n = B.N;
     RecView rv{lf->rec, lf->rec + n + 4, lf->rec + 2 * n + 4, lf->rec + 3 * n + 4};
-    k_pairs_fp32c<BSZ, CH, FUSE><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(
+    k_pairs_fp32c<BSZ, CH, FUSE, RIGID><<<nblk(B.N, BSZ), BSZ, 0, ctx->stream>>>(
         ctx->c, B.blanked, B.hash, (int)B.N, rv, cs, ce, lf->psum, lf->psum_T, ctx->d_flags + SPHFEM_FLAG_NONFINITE, u);
 }
 
-template <int CH, bool FUSE>
+template <int CH, bool FUSE, bool RIGID>
 static void launch_pairs_fp32c_staged(sphfem_ctx *ctx, const sphfem_particles &B, sphfem_leap_frog *lf, const int *cs,
                                       const int *ce, const UpdateArgs &u) {
     const size_t n = B.N;
     RecView rv{lf->rec, lf->rec + n + 4, lf->rec + 2 * n + 4, lf->rec + 3 * n + 4};
-    k_pairs_fp32c_staged<CH, SPHFEM_STAGE_CAP, FUSE><<<nblk(B.N, 64), 64, 0, ctx->stream>>>(
+    k_pairs_fp32c_staged<CH, SPHFEM_STAGE_CAP, FUSE, RIGID><<<nblk(B.N, 64), 64, 0, ctx->stream>>>(
         ctx->c, B.blanked, B.hash, (int)B.N, rv, cs, ce, lf->psum, lf->psum_T, ctx->d_flags + SPHFEM_FLAG_NONFINITE,
         ctx->d_flags, u);
 }
 
-template <bool FUSE>
+template <bool FUSE, bool RIGID>
 static void launch_pairs_fp32c_shape(sphfem_ctx *ctx, const sphfem_particles &B, sphfem_leap_frog *lf, const int *cs,
                                      const int *ce, const UpdateArgs &u) {
     const int pb = lf->tuning.pair_block, pc = lf->tuning.pair_chunk;
     const int k = lf->tuning.pair_kernel;
     if (k == SPHFEM_PAIRS_STAGED || k == SPHFEM_PAIRS_STAGED_FUSED) {
-        if (pc == 16) launch_pairs_fp32c_staged<16, FUSE>(ctx, B, lf, cs, ce, u);
-        else if (pc == 48) launch_pairs_fp32c_staged<48, FUSE>(ctx, B, lf, cs, ce, u);
-        else launch_pairs_fp32c_staged<32, FUSE>(ctx, B, lf, cs, ce, u);
-    } else if (pb == 128 && pc == 16) launch_pairs_fp32c<128, 16, FUSE>(ctx, B, lf, cs, ce, u);
-    else if (pb == 128 && pc == 32) launch_pairs_fp32c<128, 32, FUSE>(ctx, B, lf, cs, ce, u);
-    else if (pb == 128 && pc == 48) launch_pairs_fp32c<128, 48, FUSE>(ctx, B, lf, cs, ce, u);
-    else if (pb == 256 && pc == 16) launch_pairs_fp32c<256, 16, FUSE>(ctx, B, lf, cs, ce, u);
-    else if (pb == 256 && pc == 48) launch_pairs_fp32c<256, 48, FUSE>(ctx, B, lf, cs, ce, u);
-    else launch_pairs_fp32c<256, 32, FUSE>(ctx, B, lf, cs, ce, u);
+        if (pc == 16) launch_pairs_fp32c_staged<16, FUSE, RIGID>(ctx, B, lf, cs, ce, u);
+        else if (pc == 48) launch_pairs_fp32c_staged<48, FUSE, RIGID>(ctx, B, lf, cs, ce, u);
+        else launch_pairs_fp32c_staged<32, FUSE, RIGID>(ctx, B, lf, cs, ce, u);
+    } else if (pb == 128 && pc == 16) launch_pairs_fp32c<128, 16, FUSE, RIGID>(ctx, B, lf, cs, ce, u);
+    else if (pb == 128 && pc == 32) launch_pairs_fp32c<128, 32, FUSE, RIGID>(ctx, B, lf, cs, ce, u);
+    else if (pb == 128 && pc == 48) launch_pairs_fp32c<128, 48, FUSE, RIGID>(ctx, B, lf, cs, ce, u);
+    else if (pb == 256 && pc == 16) launch_pairs_fp32c<256, 16, FUSE, RIGID>(ctx, B, lf, cs, ce, u);
+    else if (pb == 256 && pc == 48) launch_pairs_fp32c<256, 48, FUSE, RIGID>(ctx, B, lf, cs, ce, u);
+    else launch_pairs_fp32c<256, 32, FUSE, RIGID>(ctx, B, lf, cs, ce, u);
 }
 
 // the tuned FP32C pair kernel; fuse: it also runs every particle's update, with these arguments
+// (the rigid-tool update when fuse->rt is set)
 static void launch_pairs_fp32c_tuned(sphfem_ctx *ctx, const sphfem_particles &B, sphfem_leap_frog *lf, const int *cs,
                                      const int *ce, const UpdateArgs *fuse = nullptr) {
     if (fuse) {
         KTimer tm(ctx, "pairs_update_fp32c");
-        launch_pairs_fp32c_shape<true>(ctx, B, lf, cs, ce, *fuse);
+        if (fuse->rt) launch_pairs_fp32c_shape<true, true>(ctx, B, lf, cs, ce, *fuse);
+        else launch_pairs_fp32c_shape<true, false>(ctx, B, lf, cs, ce, *fuse);
     } else {
         KTimer tm(ctx, "pairs_fp32c");
-        launch_pairs_fp32c_shape<false>(ctx, B, lf, cs, ce, UpdateArgs{});
+        launch_pairs_fp32c_shape<false, false>(ctx, B, lf, cs, ce, UpdateArgs{});
     }
 }
 
@@ -1288,10 +1290,9 @@ void launch_fused_update(sphfem_ctx *ctx, const sphfem_particles &B, const sphfe
     int *nf = ctx->d_flags + SPHFEM_FLAG_NONFINITE;
     if (lf->precision == SPHFEM_FP32C) {
         u.skip_contact = skip;
-        // the fused pair kernels run the update themselves (not with a rigid tool, whose update has
-        // its own kernels)
+        // the fused pair kernels run the update themselves
         const int pk = lf->tuning.pair_kernel;
-        if ((pk == SPHFEM_PAIRS_LIST_FUSED || pk == SPHFEM_PAIRS_STAGED_FUSED) && !rtd) {
+        if (pk == SPHFEM_PAIRS_LIST_FUSED || pk == SPHFEM_PAIRS_STAGED_FUSED) {
             launch_pairs_fp32c_tuned(ctx, B, lf, cs, ce, &u);
             if (t) launch_fric_heat_sums(ctx, t);
             return;

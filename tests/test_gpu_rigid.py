@@ -297,6 +297,31 @@ def test_rigid_step_fp32c_matches_fp64():
         s.close()
 
 
+def test_rigid_step_fused_pair_kernels_equal_separate():
+    """The FP32C pair kernels running the rigid-tool update (friction heat, tool-particle motion,
+    polygon and fillet contact) against the separate pair and update kernels, 10 steps from a
+    developed cut: every field and the tool equal bit for bit."""
+    case = cases.ref_cut_sph()
+    o = Oracle(case)
+    o.step(300)
+    fields = ("hash", "idx", "blanked", "pos", "vel", "S", "rho", "T", "eps_pl", "fc", "ft", "n") + DERIVS
+    out, geo = {}, {}
+    for kernel in (_lib.PAIRS_LIST, _lib.PAIRS_LIST_FUSED, _lib.PAIRS_STAGED_FUSED):
+        print(f"rigid cut, pair kernel {kernel}", flush=True)
+        sim = S.Simulation(case, precision=_lib.FP32C)
+        sim.set_tuning(pair_block=256, pair_chunk=32, update_block=256, pair_kernel=kernel)
+        _load_developed(sim, o, case)
+        sim.step(10)
+        sim.ctx.check_error()
+        out[kernel] = sim.state(fields)
+        geo[kernel] = bytes(sim.rtool.geometry().g)
+        sim.close()
+    for kernel in (_lib.PAIRS_LIST_FUSED, _lib.PAIRS_STAGED_FUSED):
+        for f in fields:
+            np.testing.assert_array_equal(out[kernel][f], out[_lib.PAIRS_LIST][f], err_msg=f"kernel {kernel} {f}")
+        assert geo[kernel] == geo[_lib.PAIRS_LIST]
+
+
 @pytest.mark.parametrize("prec", [_lib.FP64, _lib.FP32C])
 def test_rigid_graph_replays_match_eager(prec):
     case = cases.ref_cut_sph()
