@@ -330,7 +330,8 @@ int sphfem_leap_frog_correct(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_parti
  * one wavefront per workgroup).  At the same pair_chunk both give the same sums bit for bit.
  * The _FUSED forms are the same kernels running each particle's update right after its sums (no
  * pair-sum round trip through memory, one launch fewer; update_block unused); the same step
- * result bit for bit, with the FEM tool and with a rigid tool. */
+ * result bit for bit, with the FEM tool and with a rigid tool.  The FP64 form always runs its
+ * pair and update kernels separately (pair_kernel does not apply to it). */
 enum { SPHFEM_PAIRS_LIST = 0, SPHFEM_PAIRS_STAGED = 1, SPHFEM_PAIRS_LIST_FUSED = 2, SPHFEM_PAIRS_STAGED_FUSED = 3 };
 /* resort_one_block: the step's full-sort fallback (first step, or more particles changed cells than
  * the incremental re-sort takes) by one workgroup in one launch instead of the multi-block radix
