@@ -380,7 +380,8 @@ struct UpdateArgs {
 };
 
 // the update of sorted particle i; g_sum, a_sum, T_lapl32 are its FP32C pair sums (the FP64 form
-// reads psum64 instead)
+// reads psum64 instead).  NT: streaming stores, for the update run inside a pair kernel, whose
+// neighbour records other wavefronts are still reading from L2 (plain stores would displace them)
 template <bool F32C, bool RIGID, bool NT = false>
 __device__ __forceinline__ void update_item(const DevConsts &c, const UpdateArgs &u, int i, float4 g_sum,
                                             float4 a_sum, float T_lapl32) {
