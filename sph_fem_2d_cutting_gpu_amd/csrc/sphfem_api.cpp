@@ -274,6 +274,24 @@ int sphfem_check_error(sphfem_ctx *ctx) {
     });
 }
 
+// The bounds of the plasticity's early exit (DevConsts::jc_svm_below): with A > 0 and B, C, n, m >= 0,
+// every factor of plasticity.cuh:21-39 is bounded below for eps_pl, eps_pl_dot >= 0 -- A + B eps^n >= A,
+// the rate factor >= 1 -- and the thermal factor 1 - theta^m >= 1 - 0.99^m / 2 > 1/2 for theta up to
+// 0.99 * 0.5^(1/m), so sigma_yield > A / 2 there; 0.49 A leaves a margin far beyond the rounding of
+// the evaluation.  Off for constants outside that range.
+static void joco_early_exit_bounds(sphfem::DevConsts &c) {
+    const sphfem_joco_constants &j = c.joco;
+    c.jc_svm_below = -1.;
+    const bool ok = j.A > 0. && j.B >= 0. && j.C >= 0. && j.n > 0. && j.m > 0. && j.eps_dot_ref > 0. &&
+                    j.Tmelt > j.Tref && std::isfinite(j.A) && std::isfinite(j.B) && std::isfinite(j.C) &&
+                    std::isfinite(j.n) && std::isfinite(j.m) && std::isfinite(j.Tmelt) && std::isfinite(j.Tref);
+    if (!ok) return;
+    const double theta = 0.99 * std::pow(0.5, 1. / j.m);
+    c.jc_T_lo = j.Tref;
+    c.jc_T_hi = j.Tref + theta * (j.Tmelt - j.Tref);
+    c.jc_svm_below = 0.49 * j.A;
+}
+
 #define SETC(fn, field, bit, extra)                                           \
     int fn(sphfem_ctx *ctx, const decltype(sphfem::DevConsts::field) *c) {    \
         if (!ctx || !c) return SPHFEM_ERR_ARG;                                \
@@ -285,7 +303,7 @@ int sphfem_check_error(sphfem_ctx *ctx) {
 
 SETC(sphfem_setup_physical_constants, phys, SET_PHYS, )
 SETC(sphfem_setup_corrector_constants, corr, SET_CORR, )
-SETC(sphfem_setup_johnson_cook_constants, joco, SET_JOCO, ctx->c.plastic = 1)
+SETC(sphfem_setup_johnson_cook_constants, joco, SET_JOCO, ctx->c.plastic = 1; joco_early_exit_bounds(ctx->c))
 SETC(sphfem_setup_thermal_constants_wp, trml_wp, SET_TRML_WP, if (c->eta != 0.) ctx->c.fric_heat = 1)
 SETC(sphfem_setup_thermal_constants_tool, trml_tool, 0, ctx->c.thermal_bc = (c->alpha != 0.) ? 1 : 0)
 SETC(sphfem_setup_tool_constants, tc, SET_TOOLC, )

@@ -56,6 +56,9 @@ struct DevConsts {
     // written only for alpha != 0) and the host copy's alpha read by the PSE launch
     sphfem_trml_constants trml_itool;
     double itool_alpha_pse;
+    // a von Mises stress below jc_svm_below at a temperature in [jc_T_lo, jc_T_hi] (with eps_pl and
+    // its rate >= 0) lies below the Johnson-Cook yield stress whatever the plastic state; -1: off
+    double jc_svm_below = -1., jc_T_lo = 0., jc_T_hi = 0.;
 };
 
 // Device-resident tool kinematics (the host bookkeeping of tool_FEM.cu:1552-1570 plus the

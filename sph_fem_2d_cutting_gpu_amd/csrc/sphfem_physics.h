@@ -162,6 +162,12 @@ __device__ __forceinline__ bool plasticity_jc(const DevConsts &c, sphfem_double4
     double norm_Strial = sqrtf_d(Sxx * Sxx + Syy * Syy + Szz * Szz + 2. * Sxy * Sxy);
     double cxx = Sxx - p, cyy = Syy - p, czz = Szz - p, cxy = Sxy;
     double svm = sqrtf_d((cxx * cxx + cyy * cyy + czz * czz) - cxx * cyy - cxx * czz - cyy * czz + 3.0 * cxy * cxy);
+    // well below yield (a lower bound of the yield stress, DevConsts::jc_svm_below): the same outcome
+    // as the comparison below, without the three pow/log evaluations of the yield stress
+    if (svm < c.jc_svm_below && eps_pl >= 0. && eps_pl_dot >= 0. && T >= c.jc_T_lo && T <= c.jc_T_hi) {
+        eps_pl_dot = 0.;
+        return false;
+    }
     double sigma_Y = sigma_yield(c.joco, eps_pl, eps_pl_dot, T);
     if (svm < sigma_Y) {
         eps_pl_dot = 0.;
