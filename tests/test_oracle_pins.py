@@ -73,6 +73,28 @@ def test_spline_antisymmetric_gradient():
 
 
 # ---------------------------------------------------------------- plasticity.cuh:21-162
+def test_plasticity_early_exit_bound_is_below_the_yield_stress():
+    """The plasticity kernel skips the yield stress when the von Mises stress lies below 0.49 A at
+    Tref <= T <= Tref + 0.99 * 0.5^(1/m) * (Tmelt - Tref) with eps_pl, eps_pl_dot >= 0 (the bound the
+    library derives from the Johnson-Cook constants).  The yield stress exceeds that bound over the
+    whole range, for the workload materials and the rigid-tool setup's steel, down to the end of the
+    temperature range and at zero strain and rate; it does not just above the range's end."""
+    mats = [dict(A=m["A"], B=m["B"], C=m["C"], n=m["n"], m=m["m"], Tmelt=m["Tmelt"], Tref=300.0, eps_dot_ref=1e-6,
+                 tanh_a=0.0, tanh_b=0.0, tanh_c=0.0, tanh_d=0.0) for m in cases.MATERIALS.values()]
+    mats.append(cases.ref_cut_sph().const.joco)
+    for jc in mats:
+        A, m, Tr, Tm, e0 = jc["A"], jc["m"], jc["Tref"], jc["Tmelt"], jc["eps_dot_ref"]
+        bound = 0.49 * A
+        T_hi = Tr + 0.99 * 0.5 ** (1.0 / m) * (Tm - Tr)
+        for eps in (0.0, 1e-12, 1e-6, 1e-3, 0.1, 1.0, 10.0):
+            for rate in (0.0, 1e-12, 0.5 * e0, e0, 2.0 * e0, 1e3 * e0, 1e12 * e0):
+                for T in np.linspace(Tr, T_hi, 41):
+                    assert O.sigma_yield(jc, eps, rate, float(T)) > bound, (jc["A"], eps, rate, T)
+        # the bound is not loose: at the hottest end of a wider range the yield stress falls below it
+        T_far = Tr + 0.5 ** (0.1 / m) * (Tm - Tr)
+        assert O.sigma_yield(jc, 0.0, 0.0, T_far) < bound
+
+
 def test_sigma_yield_closed_forms():
     jc = TI.joco
     A, B, C, n, m = jc["A"], jc["B"], jc["C"], jc["n"], jc["m"]
