@@ -301,7 +301,10 @@ static void joco_early_exit_bounds(sphfem::DevConsts &c) {
         return SPHFEM_OK;                                                     \
     }
 
-SETC(sphfem_setup_physical_constants, phys, SET_PHYS, )
+SETC(sphfem_setup_physical_constants, phys, SET_PHYS, {
+    const double c0 = (double)sqrtf((float)(c->K / c->rho0));
+    ctx->c.eos_c02 = c0 * c0;
+})
 SETC(sphfem_setup_corrector_constants, corr, SET_CORR, )
 SETC(sphfem_setup_johnson_cook_constants, joco, SET_JOCO, ctx->c.plastic = 1; joco_early_exit_bounds(ctx->c))
 SETC(sphfem_setup_thermal_constants_wp, trml_wp, SET_TRML_WP, if (c->eta != 0.) ctx->c.fric_heat = 1)

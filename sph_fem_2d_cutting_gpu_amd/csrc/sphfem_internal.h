@@ -59,6 +59,9 @@ struct DevConsts {
     // a von Mises stress below jc_svm_below at a temperature in [jc_T_lo, jc_T_hi] (with eps_pl and
     // its rate >= 0) lies below the Johnson-Cook yield stress whatever the plastic state; -1: off
     double jc_svm_below = -1., jc_T_lo = 0., jc_T_hi = 0.;
+    // actions_gpu.cu:222-232's c0 * c0, c0 = sqrtf(K / rho0) (single-precision square root), per
+    // physical-constant block
+    double eos_c02 = 0.;
 };
 
 // Device-resident tool kinematics (the host bookkeeping of tool_FEM.cu:1552-1570 plus the

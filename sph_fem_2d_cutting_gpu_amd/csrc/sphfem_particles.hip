@@ -512,8 +512,15 @@ __device__ __forceinline__ void update_item(const DevConsts &c, const UpdateArgs
             S_t = jaumann_rate(c, r.v_der, S);        // actions_gpu.cu:280-314
             rho_t = -rho * (r.v_der.x + r.v_der.w);     // :626-639
             const double mass = c.phys.mass;            // :641-671
-            vel_t.x += r.S_der.x + r.S_der.y + fc.x / mass + ft.x / mass;
-            vel_t.y += r.S_der.z + r.S_der.w + fc.y / mass + ft.y / mass;
+            // x / mass is x itself for x = +-0 (mass > 0): far from the tool the contact forces are
+            // zero and their divisions are skipped, with the same sums
+            sphfem_double2 fcm = fc, ftm = ft;
+            if (fc.x != 0. || fc.y != 0. || ft.x != 0. || ft.y != 0.) {
+                fcm = {fc.x / mass, fc.y / mass};
+                ftm = {ft.x / mass, ft.y / mass};
+            }
+            vel_t.x += r.S_der.x + r.S_der.y + fcm.x + ftm.x;
+            vel_t.y += r.S_der.z + r.S_der.w + fcm.y + ftm.y;
             pos_t.x = pos_t.x + vel.x;                  // :673-687
             pos_t.y = pos_t.y + vel.y;
         }

@@ -57,9 +57,8 @@ __device__ __forceinline__ bool cubic_spline(double pix, double piy, double pjx,
 
 // actions_gpu.cu:222-232
 __device__ __forceinline__ double eos_pressure(const DevConsts &c, double rho) {
-    double rho0 = c.phys.rho0;
-    double c0 = sqrtf_d(c.phys.K / rho0);
-    return c0 * c0 * (rho - rho0);
+    // c0 * c0 with c0 = sqrtf(K / rho0), computed once when the constants are set (DevConsts::eos_c02)
+    return c.eos_c02 * (rho - c.phys.rho0);
 }
 
 // actions_gpu.cu:45-49, 234-278
@@ -478,7 +477,11 @@ __device__ __forceinline__ int contact_s1_near_wave(const DevConsts &c, const To
 
 __device__ __forceinline__ bool near_tool(const ToolKin &kin, sphfem_double2 pos) {
     double ddx = pos.x - kin.tl_x, ddy = pos.y - kin.tl_y;
-    return sqrt(ddx * ddx + ddy * ddy) < 1.05 * kin.ref_dist;
+    const double d2 = ddx * ddx + ddy * ddy, r = 1.05 * kin.ref_dist;
+    // beyond the radius by a relative 1e-10 (far above the rounding of the square root and of r * r)
+    // the comparison below is false: skip the square root there
+    if (d2 > r * r * (1. + 1e-10)) return false;
+    return sqrt(d2) < r;
 }
 
 __device__ __forceinline__ bool isnaninf(double v) { return isnan(v) || isinf(v); }

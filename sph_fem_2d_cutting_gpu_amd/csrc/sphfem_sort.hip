@@ -504,7 +504,6 @@ __global__ void __launch_bounds__(HASH_BS) k_hash_mark(DevConsts c, const double
         bool changed = false;
         if (i < N) {
             const int oh = ohs[it];
-            if (clear_old && oh >= 0 && oh < num_cell) cell_start[oh] = -1;
             int h = oh;
             if (bls[it] != 1.) h = hash_of(c, pss[it]);
             if (h < 0 || h >= num_cell) {
@@ -514,6 +513,9 @@ __global__ void __launch_bounds__(HASH_BS) k_hash_mark(DevConsts c, const double
             hash_out[i] = h;
             atomicAdd(&lh[h & (RADIX - 1)], 1);
             changed = h != oh;
+            // a cell that keeps one of its particles stays occupied (the reorder rewrites its start);
+            // one that all of them left is cleared by each of them
+            if (clear_old && changed && oh >= 0 && oh < num_cell) cell_start[oh] = -1;
         }
         const unsigned long long bits = __ballot(changed);
         if (__lane_id() == 0) {
