@@ -714,24 +714,28 @@ __device__ __forceinline__ PairOut pairs_fp32c_one(const DevConsts &c, const dou
             a = PairAcc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         };
         // dense list: two entries per iteration, the records of the next entry loaded before the
-        // current one is evaluated (ping-pong registers, no copies)
+        // current one is evaluated (ping-pong registers, no copies); each list entry is read from LDS
+        // one pair term before its records are loaded, so the LDS latency passes under that term
         auto run_list = [&](int n) {
             if (n <= 0) return;
             PairAcc acc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             typename Pos::Entry eA = list[0];
+            typename Pos::Entry eB = list[(1 < n ? 1 : n - 1) * BS];
             unsigned oA = pos.index(eA) << 4;
             float4 A0 = pos.entry(eA), A1 = ldrec(rv.r1, oA), A2 = ldrec(rv.r2, oA), A3 = ldrec(rv.r3, oA);
             for (int k = 0; k < n; k += 2) {
-                const typename Pos::Entry eB = list[(k + 1 < n ? k + 1 : n - 1) * BS];
+                const typename Pos::Entry eA_next = list[(k + 2 < n ? k + 2 : n - 1) * BS];
                 const unsigned oB = pos.index(eB) << 4;
                 const float4 B0 = pos.entry(eB), B1 = ldrec(rv.r1, oB), B2 = ldrec(rv.r2, oB),
                              B3 = ldrec(rv.r3, oB);
                 pair_term<WIN>(I, hf, inv_wdp, alpha, beta, eta2, A0, A1, A2, A3, pos.column(eA), acc);
                 if (k + 1 < n) {
-                    eA = list[(k + 2 < n ? k + 2 : n - 1) * BS];
+                    const typename Pos::Entry eB_next = list[(k + 3 < n ? k + 3 : n - 1) * BS];
+                    eA = eA_next;
                     oA = pos.index(eA) << 4;
                     A0 = pos.entry(eA); A1 = ldrec(rv.r1, oA); A2 = ldrec(rv.r2, oA); A3 = ldrec(rv.r3, oA);
                     pair_term<WIN>(I, hf, inv_wdp, alpha, beta, eta2, B0, B1, B2, B3, pos.column(eB), acc);
+                    eB = eB_next;
                 }
                 if (((k + 2) & 7) == 0) flush(acc);
             }
