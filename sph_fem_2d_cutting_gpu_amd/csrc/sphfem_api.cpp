@@ -1787,9 +1787,13 @@ static int step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p,
         swap_fused(*p, g->buf);
     }
     if (t) {
-        launch_tool_fem_end(ctx, t);
-        if (ctx->c.wear_pro) launch_wear(ctx, t);
-        launch_tool_move(ctx, t, ctx->c.dt, true, true);
+        if (ctx->c.wear_pro) {
+            launch_tool_fem_end(ctx, t);
+            launch_wear(ctx, t);                   // reads the new temperatures, before the motion
+            launch_tool_move(ctx, t, ctx->c.dt, true, true);
+        } else {
+            launch_tool_fem_end_move(ctx, t);
+        }
         launch_near_contact(ctx, *p, t, lf);
     }
     return SPHFEM_OK;

@@ -134,9 +134,10 @@ void launch_wear_reset(sphfem_ctx *ctx, sphfem_tool *t) {
 // ---------------------------------------------------------------------------------------------
 // tool motion: segments_update (tool_FEM.cu:119-139) and tool_update_FEM_gpu (:109-117, :1552-1570)
 // ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_tool_move(sphfem_tool t, double dt, int do_segments, int do_nodes) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const ToolKin k = *t.kin;
+// tool_FEM.cu:1552-1570 for segment and node i with the step's kinematics k (read by every item
+// before item 0 rewrites them; the motion uses vel and flank_retreat, which the rewrite keeps)
+__device__ __forceinline__ void tool_move_item(const sphfem_tool &t, const ToolKin &k, double dt, int do_segments,
+                                               int do_nodes, int i) {
     if (do_segments && i < t.n_seg) {
         sphfem_double2 l = t.left[i], r = t.right[i];
         t.left[i] = {l.x + k.vel_x * dt, l.y + k.vel_y * dt + k.flank_retreat};
@@ -150,7 +151,10 @@ __global__ void __launch_bounds__(256) k_tool_move(sphfem_tool t, double dt, int
         p.y += k.vel_y * dt + k.flank_retreat;
         t.pos[i] = p;
     }
-    if (do_nodes && i == 0) {
+}
+
+__device__ __forceinline__ void tool_move_kinematics(const sphfem_tool &t, const ToolKin &k, double dt) {
+    {
         ToolKin n = k;
         n.tl_x += k.vel_x * dt;
         n.tl_y += k.vel_y * dt;
@@ -161,6 +165,33 @@ __global__ void __launch_bounds__(256) k_tool_move(sphfem_tool t, double dt, int
         n.flank_contact = n.lowest_point;
         *t.kin = n;
     }
+}
+
+__global__ void __launch_bounds__(256) k_tool_move(sphfem_tool t, double dt, int do_segments, int do_nodes) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const ToolKin k = *t.kin;
+    tool_move_item(t, k, dt, do_segments, do_nodes, i);
+    if (do_nodes && i == 0) tool_move_kinematics(t, k, dt);
+}
+
+// k_fem_temp and k_tool_move in one launch (the step without wear: nothing between them reads the
+// temperatures or moves the tool otherwise); the same per-item arithmetic
+__global__ void __launch_bounds__(256) k_fem_temp_tool_move(sphfem_tool t, double dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const ToolKin k = *t.kin;
+    if (i < t.n_nodes) t.T[i] = dt * t.k[i] + t.T_old[i];
+    tool_move_item(t, k, dt, 1, 1, i);
+    if (i == 0) tool_move_kinematics(t, k, dt);
+}
+
+void launch_tool_fem_end_move(sphfem_ctx *ctx, sphfem_tool *t) {
+    {
+        KTimer tm(ctx, "fem_rate");
+        k_fem_rate_fused<<<nblk(t->n_nodes, 256), 256, 0, ctx->stream>>>(ctx->c, *t);
+    }
+    KTimer tm(ctx, "fem_temp_tool_move");
+    const int n = std::max(std::max(t->n_nodes, t->n_seg), 1);
+    k_fem_temp_tool_move<<<nblk(n, 256), 256, 0, ctx->stream>>>(*t, ctx->c.dt);
 }
 
 void launch_tool_move(sphfem_ctx *ctx, sphfem_tool *t, double dt, bool segments, bool nodes) {
