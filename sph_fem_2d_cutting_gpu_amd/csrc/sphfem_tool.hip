@@ -649,10 +649,14 @@ __global__ void __launch_bounds__(256) k_contact_s2(DevConsts c, sphfem_particle
 struct SegDst {
     double *p[6];
     int stride[6];
-    int nv;
 };
 
-__global__ void __launch_bounds__(1024) k_segment_reduce(SegSink s, SegDst d) {
+constexpr int SEG_REDUCE_BS = 1024;
+constexpr int SEG_REDUCE_PER = SPHFEM_SEG_SINK_CAP / SEG_REDUCE_BS;   // records per thread, at most
+static_assert(SPHFEM_SEG_SINK_CAP % SEG_REDUCE_BS == 0, "sink capacity: whole records per thread");
+
+template <int NV>
+__global__ void __launch_bounds__(SEG_REDUCE_BS) k_segment_reduce(SegSink s, SegDst d) {
     __shared__ unsigned long long sk[SPHFEM_SEG_SINK_CAP];   // keys; after the sort, one value per record
     __shared__ int sl[SPHFEM_SEG_SINK_CAP];                  // record slots
     __shared__ int sseg[SPHFEM_SEG_SINK_CAP];                // segments in sorted order
@@ -660,14 +664,14 @@ __global__ void __launch_bounds__(1024) k_segment_reduce(SegSink s, SegDst d) {
     if (n <= 0) return;
     int np = 1;
     while (np < n) np <<= 1;
-    for (int k = threadIdx.x; k < np; k += blockDim.x) {
+    for (int k = threadIdx.x; k < np; k += SEG_REDUCE_BS) {
         sk[k] = k < n ? s.keys[k] : ~0ull;
         sl[k] = k;
     }
     __syncthreads();
     for (int size = 2; size <= np; size <<= 1) {          // bitonic sort of (key, slot), ascending
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int k = threadIdx.x; k < np; k += blockDim.x) {
+            for (int k = threadIdx.x; k < np; k += SEG_REDUCE_BS) {
                 const int j = k ^ stride;
                 if (j > k) {
                     const bool up = (k & size) == 0;
@@ -684,21 +688,46 @@ __global__ void __launch_bounds__(1024) k_segment_reduce(SegSink s, SegDst d) {
             __syncthreads();
         }
     }
-    for (int k = threadIdx.x; k < n; k += blockDim.x) sseg[k] = (int)(unsigned)(sk[k] >> 32);
+    for (int k = threadIdx.x; k < n; k += SEG_REDUCE_BS) sseg[k] = (int)(unsigned)(sk[k] >> 32);
     __syncthreads();
+    // every global load of the additions is issued here, before the first is used: the values of
+    // this thread's records (sorted positions threadIdx.x + u * block) and, for a record that
+    // starts its segment's run, the segment's current sums (one memory wait instead of one per value)
+    double v[NV][SEG_REDUCE_PER], acc0[NV][SEG_REDUCE_PER];
+    int seg_of[SEG_REDUCE_PER];
+    bool head[SEG_REDUCE_PER];
+#pragma unroll
+    for (int u = 0; u < SEG_REDUCE_PER; u++) {
+        const int k = threadIdx.x + u * SEG_REDUCE_BS;
+        const bool in = k < n;
+        const int slot = in ? sl[k] : 0;
+        const int seg = in ? sseg[k] : 0;
+        seg_of[u] = seg;
+        head[u] = in && (k == 0 || sseg[k - 1] != seg);
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            v[q][u] = in ? s.vals[(size_t)q * s.cap + slot] : 0.;
+            acc0[q][u] = head[u] ? d.p[q][(size_t)seg * d.stride[q]] : 0.;
+        }
+    }
     double *vbuf = reinterpret_cast<double *>(sk);
-    for (int q = 0; q < d.nv; q++) {
-        // this value of every record, gathered in sorted order, then each segment's run added in
-        // order by the run's first thread (from LDS: long runs stay cheap)
-        for (int k = threadIdx.x; k < n; k += blockDim.x) vbuf[k] = s.vals[(size_t)q * s.cap + sl[k]];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        // this value of every record in sorted order, then each segment's run added in order by
+        // the run's first thread (from LDS: long runs stay cheap)
+#pragma unroll
+        for (int u = 0; u < SEG_REDUCE_PER; u++) {
+            const int k = threadIdx.x + u * SEG_REDUCE_BS;
+            if (k < n) vbuf[k] = v[q][u];
+        }
         __syncthreads();
-        for (int k = threadIdx.x; k < n; k += blockDim.x) {
-            const int seg = sseg[k];
-            if (k > 0 && sseg[k - 1] == seg) continue;          // not the first of its segment
-            double *dst = d.p[q] + (size_t)seg * d.stride[q];
-            double acc = *dst;
-            for (int r = k; r < n && sseg[r] == seg; r++) acc += vbuf[r];
-            *dst = acc;
+#pragma unroll
+        for (int u = 0; u < SEG_REDUCE_PER; u++) {
+            if (!head[u]) continue;
+            const int k = threadIdx.x + u * SEG_REDUCE_BS;
+            double acc = acc0[q][u];
+            for (int r = k; r < n && sseg[r] == seg_of[u]; r++) acc += vbuf[r];
+            d.p[q][(size_t)seg_of[u] * d.stride[q]] = acc;
         }
         __syncthreads();
     }
@@ -713,7 +742,6 @@ static SegSink host_sink(sphfem_ctx *ctx, sphfem_tool *t, bool s2) {
 
 void launch_fric_heat_sums(sphfem_ctx *ctx, sphfem_tool *t) {
     SegDst d{};
-    d.nv = 6;
     d.p[0] = t->fric_heat;
     d.p[1] = &t->physical_para[0].x;
     d.p[2] = &t->physical_para[0].y;
@@ -723,17 +751,16 @@ void launch_fric_heat_sums(sphfem_ctx *ctx, sphfem_tool *t) {
     const int st[6] = {1, 4, 4, 4, 4, 1};
     for (int q = 0; q < 6; q++) d.stride[q] = st[q];
     KTimer tm(ctx, "fric_heat_sums");
-    k_segment_reduce<<<1, 1024, 0, ctx->stream>>>(host_sink(ctx, t, false), d);
+    k_segment_reduce<6><<<1, SEG_REDUCE_BS, 0, ctx->stream>>>(host_sink(ctx, t, false), d);
 }
 
 static void launch_s2_sums(sphfem_ctx *ctx, sphfem_tool *t) {
     SegDst d{};
-    d.nv = 2;
     d.p[0] = &t->heat_exchange[0].x;
     d.p[1] = &t->heat_exchange[0].y;
     d.stride[0] = d.stride[1] = 2;
     KTimer tm(ctx, "contact_s2_sums");
-    k_segment_reduce<<<1, 1024, 0, ctx->stream>>>(host_sink(ctx, t, true), d);
+    k_segment_reduce<2><<<1, SEG_REDUCE_BS, 0, ctx->stream>>>(host_sink(ctx, t, true), d);
 }
 
 void launch_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t) {
