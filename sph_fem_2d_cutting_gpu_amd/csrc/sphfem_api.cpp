@@ -1614,7 +1614,6 @@ int sphfem_leap_frog_create(sphfem_ctx *ctx, unsigned int num_part, unsigned int
         lf->cell_start = dalloc<int>(std::max(num_cell, 1u));
         lf->cell_end = dalloc<int>(std::max(num_cell, 1u));
         lf->near_list = dalloc<int>(num_part);
-        lf->contact_list = dalloc<int>(num_part);
         *out = lf;
         return SPHFEM_OK;
     });
@@ -1625,7 +1624,7 @@ int sphfem_leap_frog_destroy(sphfem_ctx *ctx, sphfem_leap_frog *lf) {
     return guarded(ctx, [&] {
         SPHFEM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         dfree(lf->pos_init); dfree(lf->vel_init); dfree(lf->S_init); dfree(lf->rho_init); dfree(lf->T_init);
-        dfree(lf->cell_start); dfree(lf->cell_end); dfree(lf->near_list); dfree(lf->contact_list);
+        dfree(lf->cell_start); dfree(lf->cell_end); dfree(lf->near_list);
         dfree(lf->rec); dfree(lf->psum); dfree(lf->psum_T); dfree(lf->pre_vol); dfree(lf->pre_s); dfree(lf->psum64);
         dfree(lf->rate32[0]); dfree(lf->rate32[1]);
         delete lf;

@@ -113,7 +113,7 @@ struct sphfem_ctx {
     sphfem::DevConsts c{};
     unsigned set_mask = 0;
     std::string err;
-    int *d_flags = nullptr;       // [0]: error bits, [1]: near-list count, [2]: contact count,
+    int *d_flags = nullptr;       // [0]: error bits, [1]: near-list count, [2]: unused,
                                   // [3]: re-sort mode (1 full, 2 incremental), [4]: changed count,
                                   // [5]: non-finite pair inputs present (SPHFEM_FLAG_NONFINITE)
     double *force_scratch = nullptr;  // this device's `scratch` array (sphfem_api.cpp)
@@ -232,7 +232,6 @@ struct sphfem_leap_frog {
     sphfem_double4 *pre_s = nullptr;
     sphfem::PairSums *psum64 = nullptr;   // FP64 form: pair sums of the sorted particles
     int *near_list = nullptr;      // [num_part] particles near the tool
-    int *contact_list = nullptr;   // [num_part] particles in contact (+ flag bit 30 = invalid)
     bool intermediates = false;    // also write S_der / v_der in the fused step
     // FP32C neighbour records of the sorted predicted state, SoA blocks r0 [num_part + 4],
     // r1, r2, r3 [num_part]: {x, y relative to the cell corner, cell row, h},
@@ -374,12 +373,11 @@ void launch_tool_begin(sphfem_ctx *ctx, sphfem_tool *t);
 
 // the start of a step for the FEM tool, item i of max(nodes, segments): reset_FEM_server's
 // T_old = T and calculate_temp(0.5 dt) for the nodes; zero the wear accumulators
-// (segments_wear_rate_reset) and, item 0, the contact-list and segment-sink counters and the
+// (segments_wear_rate_reset) and, item 0, the near-list and segment-sink counters and the
 // window-form switch of the pair kernels
 __device__ __forceinline__ void tool_begin_item(const sphfem_tool &t, double half_dt, int *counters, int i) {
     if (i == 0) {
         counters[1] = 0;
-        counters[2] = 0;
         counters[SPHFEM_FLAG_NONFINITE] = 0;
         counters[SPHFEM_FLAG_SINK_FRIC] = 0;
         counters[SPHFEM_FLAG_SINK_S2] = 0;

@@ -776,14 +776,15 @@ void launch_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool *t) 
     launch_s2_sums(ctx, t);
 }
 
-// Near-tool particles of the fused step: contact s1, mechanical boundary condition, NaN check;
-// particles left in contact go to the contact list (bit 30 = invalidate after s2), the others
-// also get the thermal boundary condition and their invalidation here.
+// Near-tool particles of the fused step: contact s1, mechanical boundary condition, NaN check, then
+// for particles left in contact contact s2 (its heat exchange reads only the particle itself and
+// the tool's temperatures, so it follows s1 in the same lane), and the thermal boundary condition
+// and the invalidation.
 // use32: the rates are the FP32 derivative carry r32 of the FP32C form (the particle arrays' FP64
 // rates are not maintained by its steps)
 __global__ void __launch_bounds__(256) k_near_contact(DevConsts c, sphfem_particles P, sphfem_tool t,
                                                       const int *__restrict__ near_list, int *counters,
-                                                      int *contact_list, RateView r32, int use32) {
+                                                      RateView r32, int use32) {
     // one particle of the near list per wavefront (wave-uniform loop); the segment loops of
     // contact s1 run over the lanes
     const int count = counters[1];
@@ -844,33 +845,16 @@ __global__ void __launch_bounds__(256) k_near_contact(DevConsts c, sphfem_partic
                   isnaninf(Tt);
         }
         if (seg != -1) {
-            int slot = atomicAdd(&counters[2], 1);
-            contact_list[slot] = i | (bad ? (1 << 30) : 0);
-        } else {
-            if (c.thermal_bc && P.fixed[i] == 1.) P.T[i] = c.trml_wp.T_init;
-            if (bad) blanked = 1.;
+            // contact s2 (actions_gpu.cu:585-617; one lane, its sink record slot from a one-lane ballot)
+            double T = P.T[i];
+            contact_s2_one(c, t, s2_sink(t, counters), i, seg, P.rho[i], T);
+            if (c.thermal_bc && P.fixed[i] == 1.) T = c.trml_wp.T_init;
+            P.T[i] = T;
+        } else if (c.thermal_bc && P.fixed[i] == 1.) {
+            P.T[i] = c.trml_wp.T_init;
         }
+        if (bad) blanked = 1.;
         if (blanked == 1.) P.blanked[i] = 1.;
-    }
-}
-
-__global__ void __launch_bounds__(256) k_contact_list_s2(DevConsts c, sphfem_particles P, sphfem_tool t,
-                                                         int *counters, const int *contact_list) {
-    const int count = counters[2];
-    const SegSink sink = s2_sink(t, counters);
-    // a wave-uniform trip count: the record slots are allocated with wave ballots
-    const int stride = gridDim.x * blockDim.x;
-    const int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63);
-    for (int q0 = base; q0 < count; q0 += stride) {
-        const int q = q0 + (threadIdx.x & 63);
-        if (q >= count) break;
-        const int code = contact_list[q];
-        const int i = code & ((1 << 30) - 1);
-        double T = P.T[i];
-        contact_s2_one(c, t, sink, i, P.on_seg[i], P.rho[i], T);
-        if (c.thermal_bc && P.fixed[i] == 1.) T = c.trml_wp.T_init;
-        P.T[i] = T;
-        if (code & (1 << 30)) P.blanked[i] = 1.;
     }
 }
 
@@ -880,13 +864,8 @@ void launch_near_contact(sphfem_ctx *ctx, const sphfem_particles &p, sphfem_tool
         KTimer tm(ctx, "near_contact");
         const bool use32 = lf->rates_fp32;
         const RateView r32 = use32 ? rate_view(lf->rate32[lf->rate_cur], lf->num_part) : RateView{};
-        k_near_contact<<<blocks, 256, 0, ctx->stream>>>(ctx->c, p, *t, lf->near_list, ctx->d_flags,
-                                                        lf->contact_list, r32, use32 ? 1 : 0);
-    }
-    {
-        KTimer tm(ctx, "contact_list_s2");
-        k_contact_list_s2<<<std::max(1, std::min(blocks, 64)), 256, 0, ctx->stream>>>(ctx->c, p, *t, ctx->d_flags,
-                                                                                     lf->contact_list);
+        k_near_contact<<<blocks, 256, 0, ctx->stream>>>(ctx->c, p, *t, lf->near_list, ctx->d_flags, r32,
+                                                        use32 ? 1 : 0);
     }
     launch_s2_sums(ctx, t);
 }
