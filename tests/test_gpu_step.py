@@ -97,7 +97,9 @@ def test_fused_matches_unfused(wear):
     live = b["blanked"] != 1.0
     _compare_states(a, b, live, 1e-11, STATE + DERIVS, f"fused vs unfused (wear={wear})")
     ta, tb = fused.tool_state(), ref.tool_state()
-    for f in ("T", "node_pos", "left", "right", "K", "M"):
+    # heat_exchange: the contact heat-exchange sums of the last step (contact s2, which the fused
+    # step runs in the near-tool pass and the unfused step over all particles)
+    for f in ("T", "node_pos", "left", "right", "K", "M", "heat_exchange"):
         assert_close("tool " + f, ta[f], tb[f], 1e-11)
 
 
@@ -122,6 +124,8 @@ def test_fused_matches_oracle(nsteps):
     tg = sim.tool_state()
     for f in ("T", "node_pos", "left", "right", "n", "K", "M"):
         assert_close("tool " + f, tg[f], o.t[f], tol)
+    if nsteps == 1:     # the contact set is the oracle's exactly after one step
+        assert_close("tool heat_exchange", tg["heat_exchange"], o.t["heat_exchange"], tol)
 
 
 def test_observables_2000_steps():
