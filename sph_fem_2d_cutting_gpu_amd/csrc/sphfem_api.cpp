@@ -1787,8 +1787,7 @@ static int step_body(sphfem_ctx *ctx, sphfem_leap_frog *lf, sphfem_particles *p,
     }
     if (t) {
         if (ctx->c.wear_pro) {
-            launch_tool_fem_end(ctx, t);
-            launch_wear(ctx, t);                   // reads the new temperatures, before the motion
+            launch_tool_fem_end_wear(ctx, t);      // the wear reads the new temperatures, before the motion
             launch_tool_move(ctx, t, ctx->c.dt, true, true);
         } else {
             launch_tool_fem_end_move(ctx, t);

@@ -399,6 +399,7 @@ void launch_tool_move(sphfem_ctx *ctx, sphfem_tool *t, double dt, bool segments,
 // launch_tool_fem_end then launch_tool_move(dt, segments, nodes), the temperature update and the motion
 // in one launch (the step without wear)
 void launch_tool_fem_end_move(sphfem_ctx *ctx, sphfem_tool *t);
+void launch_tool_fem_end_wear(sphfem_ctx *ctx, sphfem_tool *t);
 void launch_wear(sphfem_ctx *ctx, sphfem_tool *t);
 void launch_remesh_low_level(sphfem_ctx *ctx, sphfem_tool *t, int freq, const ElemBuckets &b);
 void launch_interp_new(sphfem_ctx *ctx, const sphfem_tool *old_t, const sphfem_double2 *new_pos, int n_new,

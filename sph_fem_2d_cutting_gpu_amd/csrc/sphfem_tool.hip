@@ -338,14 +338,48 @@ __global__ void __launch_bounds__(1024) k_segment_wear_update(sphfem_tool t) {
     if (threadIdx.x == 0) t.kin->flank_retreat = t.kin->flank_contact - red[0];
 }
 
-void launch_wear(sphfem_ctx *ctx, sphfem_tool *t) {
-    KTimer tm(ctx, "wear");
+// the wear step after the old boundary-layer matrices: node shift, new matrices, K/M update,
+// segment geometry
+static void launch_wear_rest(sphfem_ctx *ctx, sphfem_tool *t) {
     const int nl = std::max(t->n_layer, 1);
-    k_layer_mats<<<nblk(nl, 128), 128, 0, ctx->stream>>>(ctx->c, *t, t->layer_old);
     k_nodal_shift<<<nblk(std::max(t->n_nodes, t->n_seg), 256), 256, 0, ctx->stream>>>(ctx->c, *t);
     k_layer_mats<<<nblk(nl, 128), 128, 0, ctx->stream>>>(ctx->c, *t, t->layer_new);
     k_wear_assemble<<<nblk(std::max(t->nnz, t->n_nodes), 256), 256, 0, ctx->stream>>>(*t);
     k_segment_wear_update<<<1, 1024, 0, ctx->stream>>>(*t);
+}
+
+void launch_wear(sphfem_ctx *ctx, sphfem_tool *t) {
+    KTimer tm(ctx, "wear");
+    const int nl = std::max(t->n_layer, 1);
+    k_layer_mats<<<nblk(nl, 128), 128, 0, ctx->stream>>>(ctx->c, *t, t->layer_old);
+    launch_wear_rest(ctx, t);
+}
+
+// k_fem_temp and the wear step's old boundary-layer matrices in one launch: the matrices read the
+// node positions and constants only, the temperature update writes the temperatures only (the
+// same per-item arithmetic as the two kernels)
+__global__ void __launch_bounds__(256) k_fem_temp_layer_mats(DevConsts c, sphfem_tool t, double dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < t.n_nodes) t.T[i] = dt * t.k[i] + t.T_old[i];
+    if (i < t.n_layer) {
+        double m[10];
+        element_matrices(c, t, t.layer_elems[i], m);
+        for (int q = 0; q < 10; q++) t.layer_old[(long)i * 10 + q] = m[q];
+    }
+}
+
+// the FEM tool's end of a step with wear: temperature rate, then the temperature update with the
+// old layer matrices, then the rest of the wear step (launch_tool_fem_end + launch_wear, one launch
+// fewer)
+void launch_tool_fem_end_wear(sphfem_ctx *ctx, sphfem_tool *t) {
+    {
+        KTimer tm(ctx, "fem_rate");
+        k_fem_rate_fused<<<nblk(t->n_nodes, 256), 256, 0, ctx->stream>>>(ctx->c, *t);
+    }
+    KTimer tm(ctx, "fem_temp_wear");
+    const int n = std::max(std::max(t->n_nodes, t->n_layer), 1);
+    k_fem_temp_layer_mats<<<nblk(n, 256), 256, 0, ctx->stream>>>(ctx->c, *t, ctx->c.dt);
+    launch_wear_rest(ctx, t);
 }
 
 // ---------------------------------------------------------------------------------------------
